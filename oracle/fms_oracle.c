@@ -1,0 +1,1134 @@
+/*
+ * FP64 CPU oracle for refeff's real-space FMS stage.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the parity tests, the
+ * smoke test and the bench's CPU baseline compare against; the product path
+ * (librefeff_fms.so and its HIP kernels) never links, loads or calls it.
+ *
+ * Every function restates, in double precision, the algorithm of the refeff
+ * source it cites (paths relative to the refeff repository root).  refeff
+ * itself computes in complex single precision; this restatement takes the
+ * same single-precision atom coordinates and the double-precision wave
+ * numbers and phase shifts that refeff narrows, and keeps everything else in
+ * double.  Pinning (see tests/test_oracle_kats.py): refeff's in-source
+ * known-answer tests, closed forms (Racah 3j vs sympy, Hankel polynomials vs
+ * scipy spherical Hankel functions, Wigner small-d vs sympy), LAPACK zgesv on
+ * the linear solve, and brute-force invariants on tiny clusters.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+/* error codes shared with the product C ABI (include/refeff_fms.h) */
+enum {
+    ORC_OK = 0,
+    ORC_INVALID_ARGUMENT = 1,
+    ORC_MISSING_CENTRAL_ATOM = 2,
+    ORC_DUPLICATE_ABSORBER = 3,
+    ORC_NON_FINITE = 4,
+    ORC_ZERO_RHO = 5,
+    ORC_POTENTIAL_OUT_OF_RANGE = 6,
+    ORC_MISSING_REPRESENTATIVE = 7,
+    ORC_SINGULAR = 8,
+    ORC_BREAKDOWN = 9,
+    ORC_CENTRAL_MISMATCH = 10,
+    ORC_UNSUPPORTED = 11,
+    ORC_NO_MEMORY = 12,
+};
+
+#define ORC_LX 24 /* refeff FMS_ROTATION_LMAX */
+
+/* ------------------------------------------------------------------------ */
+/* angular helpers                                                            */
+/* ------------------------------------------------------------------------ */
+
+static double fact(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= (double)i;
+    return f;
+}
+
+/* Wigner 3j symbol (j1 j2 j3; m1 m2 m3=-m1-m2) with doubled arguments, by the
+ * Racah single-sum formula.  Same convention as refeff
+ * crates/refeff-core/src/angular/wigner.rs:7-90 called with scale=2. */
+double orc_wigner_3j_doubled(int j1, int j2, int j3, int m1, int m2) {
+    int m3 = -m1 - m2;
+    int a[9] = {j1 + j2 - j3, j2 + j3 - j1, j3 + j1 - j2, j1 + m1, j1 - m1,
+                j2 + m2,      j2 - m2,      j3 + m3,      j3 - m3};
+    for (int i = 0; i < 9; ++i) {
+        if (a[i] < 0 || (a[i] & 1)) return 0.0;
+        a[i] /= 2;
+    }
+    int J = (j1 + j2 + j3);
+    if (J & 1) return 0.0;
+    J /= 2;
+    int t1 = (j3 - j2 + m1) / 2; /* k >= -t1 */
+    int t2 = (j3 - j1 - m2) / 2; /* k >= -t2 */
+    int t3 = (j1 + j2 - j3) / 2; /* k <= t3 */
+    int t4 = (j1 - m1) / 2;      /* k <= t4 */
+    int t5 = (j2 + m2) / 2;      /* k <= t5 */
+    int kmin = 0;
+    if (-t1 > kmin) kmin = -t1;
+    if (-t2 > kmin) kmin = -t2;
+    int kmax = t3;
+    if (t4 < kmax) kmax = t4;
+    if (t5 < kmax) kmax = t5;
+    if (kmin > kmax) return 0.0;
+    double delta = sqrt(fact(a[0]) * fact(a[1]) * fact(a[2]) / fact(J + 1));
+    double pref = sqrt(fact(a[3]) * fact(a[4]) * fact(a[5]) * fact(a[6]) * fact(a[7]) * fact(a[8]));
+    double sum = 0.0;
+    for (int k = kmin; k <= kmax; ++k) {
+        double den = fact(k) * fact(k + t1) * fact(k + t2) * fact(t3 - k) * fact(t4 - k) * fact(t5 - k);
+        sum += ((k & 1) ? -1.0 : 1.0) / den;
+    }
+    int phase = (j1 - j2 - m3) / 2;
+    double s = (phase % 2 == 0) ? 1.0 : -1.0;
+    return s * delta * pref * sum;
+}
+
+/* FEFF t3jp / t3jm: refeff crates/refeff-core/src/angular/coupling.rs:4-38,248-251.
+ * plus/minus are (lmax+1) x (2lmax+1) x 2, index [(l*(2lmax+1) + m+lmax)*2 + spin]. */
+void orc_spin_orbit_tables(int lmax, double* plus, double* minus) {
+    int nm = 2 * lmax + 1;
+    memset(plus, 0, sizeof(double) * (size_t)(lmax + 1) * nm * 2);
+    memset(minus, 0, sizeof(double) * (size_t)(lmax + 1) * nm * 2);
+    for (int l = 0; l <= lmax; ++l) {
+        for (int m = -l; m <= l; ++m) {
+            for (int si = 0; si < 2; ++si) {
+                int j1 = 2 * l, j2 = 1, j3p = j1 + 1, j3m = j1 - 1;
+                int m1 = 2 * m, m2 = 2 * (si + 1) - 3;
+                int ph = (j2 - j1 - m1 - m2) / 2;
+                double sign = (ph % 2 == 0) ? 1.0 : -1.0;
+                size_t idx = ((size_t)l * nm + (m + lmax)) * 2 + si;
+                plus[idx] = sign * sqrt((double)(j3p + 1)) * orc_wigner_3j_doubled(j1, j2, j3p, m1, m2);
+                minus[idx] = (j3m < 0) ? 0.0
+                             : sign * sqrt((double)(j3m + 1)) * orc_wigner_3j_doubled(j1, j2, j3m, m1, m2);
+            }
+        }
+    }
+}
+
+/* FEFF xnlm: refeff crates/refeff-core/src/angular/harmonics.rs:51-75. */
+double orc_xnlm(int l, int m) {
+    if (m > l) return 0.0;
+    double den = 1.0;
+    for (int k = l - m + 1; k <= l + m; ++k) den *= (double)k;
+    return sqrt((double)(2 * l + 1) / den);
+}
+
+/* ------------------------------------------------------------------------ */
+/* state kets: refeff crates/refeff-core/src/state.rs:97-148                 */
+/* ------------------------------------------------------------------------ */
+
+static int clamp_lipotx(int raw, int global_lmax) {
+    /* refeff fms/internals/validation.rs:138-144 */
+    if (raw < 0) return global_lmax;
+    return raw < global_lmax ? raw : global_lmax;
+}
+
+int orc_state_count(int nsp, int nat, const int* atom_pot, int max_pot, const int* raw_lmax, int global_lmax) {
+    int n = 0;
+    for (int a = 0; a < nat; ++a) {
+        int p = atom_pot[a];
+        if (p < 0 || p > max_pot) return -1;
+        int lm = clamp_lipotx(raw_lmax[p], global_lmax);
+        n += (lm + 1) * (lm + 1) * nsp;
+    }
+    return n;
+}
+
+/* states: out arrays of length count; rep_offsets[max_pot+1] (-1 = none) */
+int orc_state_kets(int nsp, int nat, const int* atom_pot, int max_pot, const int* raw_lmax, int global_lmax,
+                   int* s_atom, int* s_l, int* s_m, int* s_spin, int* rep_offsets) {
+    for (int p = 0; p <= max_pot; ++p) rep_offsets[p] = -1;
+    int n = 0;
+    for (int a = 0; a < nat; ++a) {
+        int p = atom_pot[a];
+        if (p < 0 || p > max_pot) return ORC_POTENTIAL_OUT_OF_RANGE;
+        if (rep_offsets[p] < 0) rep_offsets[p] = n;
+        int lm = clamp_lipotx(raw_lmax[p], global_lmax);
+        for (int l = 0; l <= lm; ++l)
+            for (int m = -l; m <= l; ++m)
+                for (int s = 1; s <= nsp; ++s) {
+                    s_atom[n] = a; /* zero-based here */
+                    s_l[n] = l;
+                    s_m[n] = m;
+                    s_spin[n] = s;
+                    ++n;
+                }
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* cluster preparation                                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double key;
+    float pos[3];
+    int pot;
+} keyed_atom;
+
+static int cmp_keyed(const void* a, const void* b) {
+    double x = ((const keyed_atom*)a)->key, y = ((const keyed_atom*)b)->key;
+    return (x < y) ? -1 : (x > y) ? 1 : 0;
+}
+
+/* FEFF athep: refeff fms/cluster.rs:91-107 and internals/geometry.rs:30-36.
+ * Sorts in place by r^2 + (index+1)*1e-6 (keys are distinct, so any sort is
+ * the stable one). keys may be NULL. */
+int orc_sort_atoms_by_radius(int n, float* pos, int* pot, double* keys) {
+    keyed_atom* ka = (keyed_atom*)malloc(sizeof(keyed_atom) * (size_t)(n > 0 ? n : 1));
+    if (!ka) return ORC_NO_MEMORY;
+    for (int i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            if (!isfinite(pos[3 * i + d])) { free(ka); return ORC_NON_FINITE; }
+            ka[i].pos[d] = pos[3 * i + d];
+        }
+        double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        ka[i].key = x * x + y * y + z * z + ((double)i + 1.0) * 1.0e-6;
+        ka[i].pot = pot[i];
+    }
+    qsort(ka, (size_t)n, sizeof(keyed_atom), cmp_keyed);
+    for (int i = 0; i < n; ++i) {
+        memcpy(&pos[3 * i], ka[i].pos, sizeof(float) * 3);
+        pot[i] = ka[i].pot;
+        if (keys) keys[i] = ka[i].key;
+    }
+    free(ka);
+    return ORC_OK;
+}
+
+/* FEFF yprep cluster prefix: refeff fms/cluster.rs:10-84.  Coordinates are
+ * shifted in single precision exactly as refeff does (f32 subtraction). */
+int orc_yprep_cluster(int n, const float* pos_in, const int* pot_in, int central_potential, float radius,
+                      int capacity, float* pos_out, int* pot_out, int* out_count, int* central_atom,
+                      int* untruncated) {
+    if (!(radius >= 0.0f) || !isfinite(radius)) return ORC_INVALID_ARGUMENT;
+    if (capacity <= 0) return ORC_INVALID_ARGUMENT;
+    int center = -1;
+    for (int i = 0; i < n; ++i) {
+        if (pot_in[i] == central_potential) {
+            if (central_potential == 0 && center >= 0) return ORC_DUPLICATE_ABSORBER;
+            if (center < 0) center = i;
+        }
+    }
+    if (center < 0) return ORC_MISSING_CENTRAL_ATOM;
+    volatile float c0 = pos_in[3 * center], c1 = pos_in[3 * center + 1], c2 = pos_in[3 * center + 2];
+    for (int i = 0; i < n; ++i) {
+        volatile float x = pos_in[3 * i] - c0;
+        volatile float y = pos_in[3 * i + 1] - c1;
+        volatile float z = pos_in[3 * i + 2] - c2;
+        pos_out[3 * i] = x;
+        pos_out[3 * i + 1] = y;
+        pos_out[3 * i + 2] = z;
+        pot_out[i] = pot_in[i];
+    }
+    int rc = orc_sort_atoms_by_radius(n, pos_out, pot_out, NULL);
+    if (rc) return rc;
+    volatile float r2 = radius * radius;
+    int first_outside = n;
+    for (int i = 0; i < n; ++i) {
+        volatile float x = pos_out[3 * i], y = pos_out[3 * i + 1], z = pos_out[3 * i + 2];
+        volatile float xx = x * x, yy = y * y, zz = z * z;
+        volatile float s = xx + yy;
+        volatile float d = s + zz;
+        if (d > r2) { first_outside = i; break; }
+    }
+    int untr = (first_outside == 0) ? n : first_outside;
+    *untruncated = untr;
+    *out_count = untr < capacity ? untr : capacity;
+    *central_atom = center;
+    return ORC_OK;
+}
+
+/* FEFF sortat: refeff fms/cluster.rs:115-195.  rep has max_pot+1 entries
+ * (-1 = potential absent). */
+int orc_sort_representative_atoms(int central, int max_pot, int n, float* pos, int* pot, int* rep) {
+    if (central < 0 || central > max_pot) return ORC_POTENTIAL_OUT_OF_RANGE;
+    if (n <= 0) return ORC_INVALID_ARGUMENT;
+    if (pot[0] != central) return ORC_CENTRAL_MISMATCH;
+    for (int i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d)
+            if (!isfinite(pos[3 * i + d])) return ORC_NON_FINITE;
+        if (pot[i] < 0 || pot[i] > max_pot) return ORC_POTENTIAL_OUT_OF_RANGE;
+    }
+    for (int p = 0; p <= max_pot; ++p) {
+        rep[p] = -1;
+        if (p == central) { rep[p] = 0; continue; }
+        for (int i = 1; i < n; ++i)
+            if (pot[i] == p) { rep[p] = i; break; }
+    }
+    for (int p = 0; p <= max_pot; ++p) {
+        int point = rep[p];
+        if (point < 0 || point <= p) continue;
+        /* swap atoms p and point */
+        float tp[3];
+        memcpy(tp, &pos[3 * p], sizeof tp);
+        memcpy(&pos[3 * p], &pos[3 * point], sizeof tp);
+        memcpy(&pos[3 * point], tp, sizeof tp);
+        int tpot = pot[p];
+        pot[p] = pot[point];
+        pot[point] = tpot;
+        for (int q = p + 1; q <= max_pot; ++q)
+            if (rep[q] == p) rep[q] = point;
+        rep[p] = p;
+    }
+    int prefix = n < max_pot + 1 ? n : max_pot + 1;
+    for (int p = 0; p <= max_pot; ++p) {
+        int point = rep[p];
+        if (point < 0) continue;
+        int last = -1;
+        for (int i = 0; i < prefix; ++i)
+            if (pot[i] == p) last = i;
+        if (last >= 0 && last != point) {
+            float tp[3];
+            memcpy(tp, &pos[3 * last], sizeof tp);
+            memcpy(&pos[3 * last], &pos[3 * point], sizeof tp);
+            memcpy(&pos[3 * point], tp, sizeof tp);
+            rep[p] = last;
+        }
+    }
+    return ORC_OK;
+}
+
+/* FEFF getang for the vector positions[i]-positions[j]: refeff
+ * fms/cluster.rs:201-241, evaluated in double from the f32 coordinates. */
+void orc_pair_polar_angles(const float* pos, int i, int j, double* theta, double* phi) {
+    if (i == j) { *theta = 0.0; *phi = 0.0; return; }
+    double x = (double)pos[3 * i] - (double)pos[3 * j];
+    double y = (double)pos[3 * i + 1] - (double)pos[3 * j + 1];
+    double z = (double)pos[3 * i + 2] - (double)pos[3 * j + 2];
+    double r = sqrt(x * x + y * y + z * z);
+    const double tiny = 1.0e-7;
+    if (fabs(x) < tiny) {
+        if (fabs(y) < tiny) *phi = 0.0;
+        else if (y > tiny) *phi = M_PI / 2;
+        else *phi = -M_PI / 2;
+    } else {
+        *phi = atan2(y, x);
+    }
+    if (r <= tiny) *theta = 0.0;
+    else if (z <= -r) *theta = M_PI;
+    else if (z < r) *theta = acos(z / r);
+    else *theta = 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* rotations: FEFF rotxan, refeff fms/rotation.rs:89-117 and                  */
+/* fms/internals/rotation.rs:3-125                                            */
+/* ------------------------------------------------------------------------ */
+
+#define D0(l, n, m) dri0[((size_t)(l) * (2 * ORC_LX + 2) + (n)) * (2 * ORC_LX + 2) + (m)]
+
+static void fill_small_d(int lmax, int mmax, double beta, double* dri0) {
+    int lxp1 = lmax + 1, mxp1 = mmax + 1;
+    int ndm = lxp1 + mxp1 - 1;
+    double xc = cos(beta / 2), xs = sin(beta / 2), s = sin(beta);
+    D0(1, 1, 1) = 1.0;
+    if (lxp1 < 2) return;
+    D0(2, 1, 1) = xc * xc;
+    D0(2, 1, 2) = s / sqrt(2.0);
+    D0(2, 1, 3) = xs * xs;
+    D0(2, 2, 1) = -D0(2, 1, 2);
+    D0(2, 2, 2) = cos(beta);
+    D0(2, 2, 3) = D0(2, 1, 2);
+    D0(2, 3, 1) = D0(2, 1, 3);
+    D0(2, 3, 2) = -D0(2, 2, 3);
+    D0(2, 3, 3) = D0(2, 1, 1);
+    for (int l = 3; l <= lxp1; ++l) {
+        int ln = 2 * l - 1, lm = 2 * l - 3;
+        if (ln > ndm) ln = ndm;
+        if (lm > ndm) lm = ndm;
+        for (int n = 1; n <= ln; ++n) {
+            for (int m = 1; m <= lm; ++m) {
+                double t1 = (double)((2 * l - 1 - n) * (2 * l - 2 - n));
+                double t = (double)((2 * l - 1 - m) * (2 * l - 2 - m));
+                double f1 = sqrt(t1 / t);
+                double f2 = sqrt((double)((2 * l - 1 - n) * (n - 1)) / t);
+                double t3 = (double)((n - 2) * (n - 1));
+                double f3 = sqrt(t3 / t);
+                double dlnm = f1 * xc * xc * D0(l - 1, n, m);
+                if (n > 1) dlnm -= f2 * s * D0(l - 1, n - 1, m);
+                if (n > 2) dlnm += f3 * xs * xs * D0(l - 1, n - 2, m);
+                D0(l, n, m) = dlnm;
+                if (n > (2 * l - 3)) D0(l, m, n) = (((n - m) % 2) == 0 ? 1.0 : -1.0) * dlnm;
+            }
+            if (n > (2 * l - 3)) {
+                D0(l, 2 * l - 2, 2 * l - 2) = D0(l, 2, 2);
+                D0(l, 2 * l - 1, 2 * l - 2) = -D0(l, 1, 2);
+                D0(l, 2 * l - 2, 2 * l - 1) = -D0(l, 2, 1);
+                D0(l, 2 * l - 1, 2 * l - 1) = D0(l, 1, 1);
+            }
+        }
+    }
+}
+
+/* drix has shape (2lmax+1, 2lmax+1, lmax+1) column-major, complex interleaved:
+ * element (r, c, l) at 2*((l*(2lmax+1) + c)*(2lmax+1) + r).  backward!=0 selects
+ * FmsRotationDirection::Backward. */
+int orc_rotation_matrix(int lmax, int mmax, double beta, double phi, int backward, double* drix_out) {
+    if (lmax > ORC_LX || mmax > lmax || lmax < 0 || mmax < 0) return ORC_INVALID_ARGUMENT;
+    if (!isfinite(beta) || !isfinite(phi)) return ORC_NON_FINITE;
+    size_t nd = (size_t)(ORC_LX + 2) * (2 * ORC_LX + 2) * (2 * ORC_LX + 2);
+    double* dri0 = (double*)calloc(nd, sizeof(double));
+    if (!dri0) return ORC_NO_MEMORY;
+    fill_small_d(lmax, mmax, beta, dri0);
+    int nm = 2 * lmax + 1;
+    cplx* drix = (cplx*)drix_out;
+    memset(drix, 0, sizeof(cplx) * (size_t)nm * nm * (lmax + 1));
+    for (int il = 1; il <= lmax + 1; ++il) {
+        int mmx = (il - 1) < mmax ? (il - 1) : mmax;
+        for (int m1 = -mmx; m1 <= mmx; ++m1)
+            for (int m2 = -mmx; m2 <= mmx; ++m2)
+                drix[((size_t)(il - 1) * nm + (m1 + lmax)) * nm + (m2 + lmax)] = D0(il, m1 + il, m2 + il);
+    }
+    for (int l = 0; l <= lmax; ++l) {
+        for (int m1 = -l; m1 <= l; ++m1) {
+            double ang = backward ? -(double)m1 * (phi - M_PI) : (double)m1 * (phi - M_PI);
+            cplx ph = cexp(I * ang);
+            for (int m2 = -l; m2 <= l; ++m2) {
+                int r = backward ? m2 : m1;
+                int c = backward ? m1 : m2;
+                drix[((size_t)l * nm + (c + lmax)) * nm + (r + lmax)] *= ph;
+            }
+        }
+    }
+    free(dri0);
+    return ORC_OK;
+}
+#undef D0
+
+/* ------------------------------------------------------------------------ */
+/* Rehr-Albers polynomials: FEFF xclmz, refeff fms/polynomial.rs:8-83         */
+/* clm is (lx+2) x (2lx+3) column-major complex: (il, im) at il + (lx+2)*im    */
+/* ------------------------------------------------------------------------ */
+int orc_rehr_albers(int lx, int lmaxp1, int mmaxp1, double rho_re, double rho_im, double* clm_out) {
+    if (lmaxp1 < 1 || lmaxp1 > lx + 1 || mmaxp1 < 1) return ORC_INVALID_ARGUMENT;
+    if (!isfinite(rho_re) || !isfinite(rho_im)) return ORC_NON_FINITE;
+    if (rho_re == 0.0 && rho_im == 0.0) return ORC_ZERO_RHO;
+    int rows = lx + 2;
+    cplx* clm = (cplx*)clm_out;
+    memset(clm, 0, sizeof(cplx) * (size_t)rows * (2 * lx + 3));
+#define C(il, im) clm[(il) + (size_t)rows * (im)]
+    cplx rho = rho_re + I * rho_im;
+    cplx z = -I / rho;
+    C(0, 0) = 1.0;
+    C(1, 0) = 1.0 - z;
+    int lmax = lmaxp1 - 1;
+    for (int il = 2; il <= lmax; ++il) C(il, 0) = C(il - 2, 0) - (double)(2 * il - 1) * z * C(il - 1, 0);
+    cplx cmm = 1.0;
+    int mmxp1 = lmaxp1 < mmaxp1 ? lmaxp1 : mmaxp1;
+    for (int im = 2; im <= mmxp1; ++im) {
+        int m = im - 1;
+        cmm = -cmm * ((double)(2 * m - 1) * z);
+        C(im - 1, im - 1) = cmm;
+        C(im, im - 1) = cmm * (double)(2 * im - 1) * (1.0 - (double)im * z);
+        for (int il = im + 1; il <= lmax; ++il)
+            C(il, im - 1) = C(il - 2, im - 1) - (double)(2 * il - 1) * z * (C(il - 1, im - 1) + C(il - 1, im - 2));
+    }
+#undef C
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* plan                                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int nat;
+    const float* pos;    /* nat*3, prepared (sorted, representatives) */
+    const int* pot;      /* nat */
+    int nsp;
+    int ispin;
+    int lfms;
+    int minv;
+    int max_pot;
+    int global_lmax;
+    const int* raw_lmax; /* max_pot+1 */
+    float rdirec;
+    const float* sigsqr; /* nat*nat, sigsqr(atom2, atom1) at atom2 + nat*atom1 */
+    float toler1;
+    float toler2;
+} orc_plan;
+
+typedef struct {
+    int n;
+    int* atom;
+    int* l;
+    int* m;
+    int* spin;
+    int* rep;     /* max_pot+1 */
+    int* lm_pot;  /* clamped lmax per potential */
+} orc_states;
+
+static void free_states(orc_states* st) {
+    free(st->atom); free(st->l); free(st->m); free(st->spin); free(st->rep); free(st->lm_pot);
+}
+
+static int build_states(const orc_plan* p, orc_states* st) {
+    memset(st, 0, sizeof *st);
+    if (p->nsp < 1 || p->nsp > 2) return ORC_INVALID_ARGUMENT;
+    if (p->nat <= 0) return ORC_INVALID_ARGUMENT;
+    int n = orc_state_count(p->nsp, p->nat, p->pot, p->max_pot, p->raw_lmax, p->global_lmax);
+    if (n < 0) return ORC_POTENTIAL_OUT_OF_RANGE;
+    st->n = n;
+    st->atom = malloc(sizeof(int) * n);
+    st->l = malloc(sizeof(int) * n);
+    st->m = malloc(sizeof(int) * n);
+    st->spin = malloc(sizeof(int) * n);
+    st->rep = malloc(sizeof(int) * (p->max_pot + 1));
+    st->lm_pot = malloc(sizeof(int) * (p->max_pot + 1));
+    if (!st->atom || !st->l || !st->m || !st->spin || !st->rep || !st->lm_pot) { free_states(st); return ORC_NO_MEMORY; }
+    for (int q = 0; q <= p->max_pot; ++q) st->lm_pot[q] = clamp_lipotx(p->raw_lmax[q], p->global_lmax);
+    return orc_state_kets(p->nsp, p->nat, p->pot, p->max_pot, p->raw_lmax, p->global_lmax, st->atom, st->l, st->m,
+                          st->spin, st->rep);
+}
+
+/* ------------------------------------------------------------------------ */
+/* free propagator G0: refeff fms/pairs.rs:9-51, 242-323, 413-553 and         */
+/* fms/internals/geometry.rs:14-28.  g0 is N x N column-major complex.        */
+/* ------------------------------------------------------------------------ */
+
+static int g0_matrix(const orc_plan* p, const orc_states* st, const cplx* k, cplx* g0) {
+    int n = st->n, nat = p->nat, L = p->global_lmax, nm = 2 * L + 1;
+    memset(g0, 0, sizeof(cplx) * (size_t)n * n);
+    /* per-atom first state */
+    int* first = malloc(sizeof(int) * (nat + 1));
+    if (!first) return ORC_NO_MEMORY;
+    {
+        int s = 0;
+        for (int a = 0; a < nat; ++a) {
+            first[a] = s;
+            int lm = st->lm_pot[p->pot[a]];
+            s += (lm + 1) * (lm + 1) * p->nsp;
+        }
+        first[nat] = s;
+    }
+    size_t dsz = (size_t)nm * nm * (L + 1);
+    cplx* dfw = malloc(sizeof(cplx) * dsz);
+    cplx* dbw = malloc(sizeof(cplx) * dsz);
+    cplx* clm = malloc(sizeof(cplx) * (size_t)(L + 2) * (2 * L + 3));
+    if (!dfw || !dbw || !clm) { free(first); free(dfw); free(dbw); free(clm); return ORC_NO_MEMORY; }
+    double xn[ORC_LX + 1][ORC_LX + 1];
+    for (int mu = 0; mu <= L; ++mu)
+        for (int l = 0; l <= L; ++l) xn[mu][l] = orc_xnlm(l, mu);
+    float cut2 = p->rdirec * p->rdirec;
+    int rc = ORC_OK;
+    for (int a = 0; a < nat && rc == ORC_OK; ++a) {
+        for (int b = 0; b < nat && rc == ORC_OK; ++b) {
+            if (a == b) continue;
+            /* cutoff evaluated in single precision as refeff does */
+            volatile float dx = p->pos[3 * a] - p->pos[3 * b];
+            volatile float dy = p->pos[3 * a + 1] - p->pos[3 * b + 1];
+            volatile float dz = p->pos[3 * a + 2] - p->pos[3 * b + 2];
+            volatile float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+            volatile float s1 = xx + yy;
+            volatile float d2 = s1 + zz;
+            if (d2 > cut2) continue;
+            double ex = (double)p->pos[3 * a] - p->pos[3 * b];
+            double ey = (double)p->pos[3 * a + 1] - p->pos[3 * b + 1];
+            double ez = (double)p->pos[3 * a + 2] - p->pos[3 * b + 2];
+            double dist = sqrt(ex * ex + ey * ey + ez * ez);
+            double beta, phi;
+            orc_pair_polar_angles(p->pos, a, b, &beta, &phi);
+            orc_rotation_matrix(L, L, beta, phi, 0, (double*)dfw);
+            orc_rotation_matrix(L, L, -beta, phi, 1, (double*)dbw);
+            double sig = (double)p->sigsqr[b + (size_t)nat * a];
+            if (!isfinite(sig)) { rc = ORC_NON_FINITE; break; }
+            for (int s = 0; s < p->nsp; ++s) {
+                cplx ck = k[s];
+                cplx rho = ck * dist;
+                if (rho == 0.0) { rc = ORC_ZERO_RHO; break; }
+                rc = orc_rehr_albers(L, L + 1, L + 1, creal(rho), cimag(rho), (double*)clm);
+                if (rc) break;
+                cplx pref = cexp(I * rho) / rho * cexp(-sig * ck * ck);
+                int la = st->lm_pot[p->pot[a]], lb = st->lm_pot[p->pot[b]];
+                for (int l1 = 0; l1 <= la; ++l1)
+                    for (int m1 = -l1; m1 <= l1; ++m1) {
+                        int row = first[a] + (l1 * l1 + l1 + m1) * p->nsp + s;
+                        for (int l2 = 0; l2 <= lb; ++l2)
+                            for (int m2 = -l2; m2 <= l2; ++m2) {
+                                int col = first[b] + (l2 * l2 + l2 + m2) * p->nsp + s;
+                                cplx sum = 0.0;
+                                for (int mu = -l1; mu <= l1; ++mu) {
+                                    int amu = abs(mu);
+                                    if (amu > l2) continue;
+                                    int numax = l1 < (l2 - amu) ? l1 : (l2 - amu);
+                                    cplx g = 0.0;
+                                    double sgn = (amu % 2 == 0) ? 1.0 : -1.0;
+                                    for (int nu = 0; nu <= numax; ++nu) {
+                                        cplx gamtl = (double)(2 * l1 + 1) * clm[l1 + (size_t)(L + 2) * nu] / xn[amu][l1];
+                                        cplx gam = clm[l2 + (size_t)(L + 2) * (amu + nu)] * (sgn * xn[amu][l2]);
+                                        g += gamtl * gam;
+                                    }
+                                    cplx bw = dbw[((size_t)l1 * nm + (m1 + L)) * nm + (mu + L)];
+                                    cplx fw = dfw[((size_t)l2 * nm + (mu + L)) * nm + (m2 + L)];
+                                    sum += bw * g * fw;
+                                }
+                                g0[row + (size_t)n * col] = pref * sum;
+                            }
+                    }
+            }
+        }
+    }
+    free(first); free(dfw); free(dbw); free(clm);
+    return rc;
+}
+
+int orc_g0_matrix(const orc_plan* p, const double* k, double* g0_out) {
+    orc_states st;
+    int rc = build_states(p, &st);
+    if (rc) { free_states(&st); return rc; }
+    rc = g0_matrix(p, &st, (const cplx*)k, (cplx*)g0_out);
+    free_states(&st);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* T matrix: refeff fms/t_matrix.rs:10-192, fms/internals/t_matrix.rs:3-77    */
+/* phases: [nsp][2L+1][npot] complex row-major (the engine's xphase layout,   */
+/* refeff crates/refeff-engine/src/fms.rs:5002-5054)                         */
+/* ------------------------------------------------------------------------ */
+
+static cplx tphase(cplx d) { return (cexp(2.0 * I * d) - 1.0) / (2.0 * I); }
+
+static cplx phase_at(const cplx* ph, int nsp, int L, int npot, int spin1, int lsigned, int pot) {
+    return ph[((size_t)(spin1 - 1) * (2 * L + 1) + (lsigned + L)) * npot + pot];
+}
+
+static double so_coef(const double* tab, int L, int l, int m, int spin1) {
+    return tab[((size_t)l * (2 * L + 1) + (m + L)) * 2 + (spin1 - 1)];
+}
+
+static cplx t_element(int nsp, int ispin, int L, int npot, const cplx* ph, const double* tp, const double* tm,
+                      int pot, int l1, int m1, int s1, int l2, int m2, int s2, int same_state) {
+    if (nsp == 1 && ispin == 0) {
+        if (same_state) return tphase(phase_at(ph, nsp, L, npot, s1, l1, pot));
+        return 0.0;
+    }
+    if (same_state) {
+        int cs = (nsp == 1) ? (ispin > 0 ? 2 : 1) : s1;
+        double mi = so_coef(tm, L, l1, m1, cs), pl = so_coef(tp, L, l1, m1, cs);
+        cplx dm = phase_at(ph, nsp, L, npot, s1, l1, pot), dp = phase_at(ph, nsp, L, npot, s1, -l1, pot);
+        return tphase(dm) * (mi * mi) + tphase(dp) * (pl * pl);
+    }
+    if (nsp == 2 && l1 == l2 && m1 + s1 == m2 + s2) {
+        double mf = so_coef(tm, L, l1, m1, s1), ms = so_coef(tm, L, l1, m2, s2);
+        double pf = so_coef(tp, L, l1, m1, s1), ps = so_coef(tp, L, l1, m2, s2);
+        cplx mp = (tphase(phase_at(ph, nsp, L, npot, s1, l1, pot)) + tphase(phase_at(ph, nsp, L, npot, s2, l1, pot))) * 0.5;
+        cplx pp = (tphase(phase_at(ph, nsp, L, npot, s1, -l1, pot)) + tphase(phase_at(ph, nsp, L, npot, s2, -l1, pot))) * 0.5;
+        return mp * mf * ms + pp * pf * ps;
+    }
+    return 0.0;
+}
+
+/* partner of state index col for nsp=2: refeff fms/internals/linear.rs:163-201 */
+static int spin_partner(int l, int m, int spin, int col) {
+    int proj = m + spin;
+    if (proj <= -l + 1 || proj >= l + 2) return -1;
+    return spin == 1 ? col - 1 : col + 1;
+}
+
+/* tmat: 2 x N (row 0 diagonal, row 1 spin partner), index t[row + 2*col] */
+static int t_table(const orc_plan* p, const orc_states* st, const cplx* ph, cplx* tmat) {
+    int L = p->global_lmax, npot = p->max_pot + 1;
+    double* tp = malloc(sizeof(double) * (L + 1) * (2 * L + 1) * 2);
+    double* tm = malloc(sizeof(double) * (L + 1) * (2 * L + 1) * 2);
+    if (!tp || !tm) { free(tp); free(tm); return ORC_NO_MEMORY; }
+    orc_spin_orbit_tables(L, tp, tm);
+    int rc = ORC_OK;
+    for (int c = 0; c < st->n; ++c) {
+        int pot = p->pot[st->atom[c]];
+        tmat[2 * c] = t_element(p->nsp, p->ispin, L, npot, ph, tp, tm, pot, st->l[c], st->m[c], st->spin[c],
+                                st->l[c], st->m[c], st->spin[c], 1);
+        tmat[2 * c + 1] = 0.0;
+        if (p->nsp == 2) {
+            int q = spin_partner(st->l[c], st->m[c], st->spin[c], c);
+            if (q >= 0 && q < st->n && st->atom[q] == st->atom[c])
+                tmat[2 * c + 1] = t_element(p->nsp, p->ispin, L, npot, ph, tp, tm, pot, st->l[c], st->m[c], st->spin[c],
+                                            st->l[q], st->m[q], st->spin[q], 0);
+        }
+        if (!isfinite(creal(tmat[2 * c])) || !isfinite(cimag(tmat[2 * c]))) rc = ORC_NON_FINITE;
+    }
+    free(tp); free(tm);
+    return rc;
+}
+
+int orc_t_matrix_table(const orc_plan* p, const double* phases, double* tmat_out) {
+    orc_states st;
+    int rc = build_states(p, &st);
+    if (!rc) rc = t_table(p, &st, (const cplx*)phases, (cplx*)tmat_out);
+    free_states(&st);
+    return rc;
+}
+
+/* A = I - G0 diag(t) (+ spin partner band): refeff fms/internals/linear.rs:102-135 */
+static void lu_system(const orc_plan* p, const orc_states* st, const cplx* g0, const cplx* tmat, cplx* a) {
+    int n = st->n;
+    for (int c = 0; c < n; ++c) {
+        int q = (p->nsp == 2) ? spin_partner(st->l[c], st->m[c], st->spin[c], c) : -1;
+        for (int r = 0; r < n; ++r) {
+            cplx v = -g0[r + (size_t)n * c] * tmat[2 * c];
+            if (q >= 0) v -= g0[r + (size_t)n * q] * tmat[2 * c + 1];
+            a[r + (size_t)n * c] = v;
+        }
+        a[c + (size_t)n * c] += 1.0;
+    }
+}
+
+int orc_lu_system_matrix_from(int nsp, int n, const int* s_l, const int* s_m, const int* s_spin, const double* g0,
+                              const double* tmat, double* a_out) {
+    orc_plan p;
+    memset(&p, 0, sizeof p);
+    p.nsp = nsp;
+    orc_states st;
+    memset(&st, 0, sizeof st);
+    st.n = n;
+    st.l = (int*)s_l;
+    st.m = (int*)s_m;
+    st.spin = (int*)s_spin;
+    lu_system(&p, &st, (const cplx*)g0, (const cplx*)tmat, (cplx*)a_out);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* dense complex LU with partial pivoting (column major, |re|+|im| pivot     */
+/* choice as LAPACK icamax) and solve: stands in for refeff's faer call at    */
+/* crates/refeff-linalg/src/lu.rs:385-397, 474-529                           */
+/* ------------------------------------------------------------------------ */
+
+int orc_zgetrf(int n, cplx* a, int* ipiv) {
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = -1.0;
+        for (int i = k; i < n; ++i) {
+            double v = fabs(creal(a[i + (size_t)n * k])) + fabs(cimag(a[i + (size_t)n * k]));
+            if (v > best) { best = v; p = i; }
+        }
+        ipiv[k] = p;
+        if (a[p + (size_t)n * k] == 0.0) return ORC_SINGULAR;
+        if (p != k)
+            for (int j = 0; j < n; ++j) {
+                cplx t = a[k + (size_t)n * j];
+                a[k + (size_t)n * j] = a[p + (size_t)n * j];
+                a[p + (size_t)n * j] = t;
+            }
+        cplx inv = 1.0 / a[k + (size_t)n * k];
+        for (int i = k + 1; i < n; ++i) a[i + (size_t)n * k] *= inv;
+        for (int j = k + 1; j < n; ++j) {
+            cplx u = a[k + (size_t)n * j];
+            if (u == 0.0) continue;
+            cplx* cj = a + (size_t)n * j;
+            const cplx* ck = a + (size_t)n * k;
+            for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * u;
+        }
+    }
+    return ORC_OK;
+}
+
+void orc_zgetrs(int n, const cplx* lu, const int* ipiv, int nrhs, cplx* b /* n x nrhs col-major */) {
+    for (int c = 0; c < nrhs; ++c) {
+        cplx* x = b + (size_t)n * c;
+        for (int k = 0; k < n; ++k)
+            if (ipiv[k] != k) { cplx t = x[k]; x[k] = x[ipiv[k]]; x[ipiv[k]] = t; }
+        for (int j = 0; j < n; ++j) {
+            cplx v = x[j];
+            if (v == 0.0) continue;
+            const cplx* col = lu + (size_t)n * j;
+            for (int i = j + 1; i < n; ++i) x[i] -= col[i] * v;
+        }
+        for (int j = n - 1; j >= 0; --j) {
+            const cplx* col = lu + (size_t)n * j;
+            x[j] /= col[j];
+            cplx v = x[j];
+            for (int i = 0; i < j; ++i) x[i] -= col[i] * v;
+        }
+    }
+}
+
+int orc_zgesv(int n, int nrhs, double* a, double* b) {
+    int* ipiv = malloc(sizeof(int) * (n > 0 ? n : 1));
+    if (!ipiv) return ORC_NO_MEMORY;
+    int rc = orc_zgetrf(n, (cplx*)a, ipiv);
+    if (!rc) orc_zgetrs(n, (const cplx*)a, ipiv, nrhs, (cplx*)b);
+    free(ipiv);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FEFF ggbi one-shot BiCGStab (minv=1): refeff fms/solvers.rs:10-78 and      */
+/* fms/internals/iterative.rs:24-254                                          */
+/* ------------------------------------------------------------------------ */
+
+static cplx cdot(int n, const cplx* a, const cplx* b) {
+    cplx s = 0.0;
+    for (int i = 0; i < n; ++i) s += conj(a[i]) * b[i];
+    return s;
+}
+
+static void matvec(int n, const cplx* m, const cplx* x, cplx* y) {
+    for (int i = 0; i < n; ++i) y[i] = 0.0;
+    for (int c = 0; c < n; ++c) {
+        cplx v = x[c];
+        if (v == 0.0) continue;
+        for (int r = 0; r < n; ++r) y[r] += m[r + (size_t)n * c] * v;
+    }
+}
+
+static int within_tol(int n, const cplx* v, double tol) {
+    for (int i = 0; i < n; ++i)
+        if (!(fabs(creal(v[i])) <= tol && fabs(cimag(v[i])) <= tol)) return 0;
+    return 1;
+}
+
+static int ggbi_solve(int n, const cplx* sys, int src, double tol, cplx* x, int* msord) {
+    cplx *r = calloc(n, sizeof(cplx)), *av = calloc(n, sizeof(cplx)), *y = calloc(n, sizeof(cplx)),
+         *sv = calloc(n, sizeof(cplx)), *as = calloc(n, sizeof(cplx));
+    int rc = ORC_OK;
+    if (!r || !av || !y || !sv || !as) { rc = ORC_NO_MEMORY; goto done; }
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    *msord = 0;
+    r[src] = 1.0;
+    if (within_tol(n, r, tol)) goto done;
+    matvec(n, sys, r, av);
+    *msord += 1;
+    cplx aa = cdot(n, av, av), wa = cdot(n, r, av), aw = conj(wa), ww = cdot(n, r, r);
+    if (aa == 0.0 || ww == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+    cplx dd = aa * ww - aw * wa;
+    cplx sdd = dd / aa / ww;
+    if (cabs(sdd) < 1.0e-8) {
+        for (int i = 0; i < n; ++i) y[i] = r[i] / ww;
+    } else {
+        if (dd == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+        ww = (ww - aw) / dd;
+        aa = (wa - aa) / dd;
+        for (int i = 0; i < n; ++i) y[i] = r[i] * aa + av[i] * ww;
+    }
+    cplx del = cdot(n, y, r), delp = cdot(n, y, av);
+    if (delp == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+    cplx omega = del / delp;
+    for (int i = 0; i < n; ++i) sv[i] = r[i] - omega * av[i];
+    if (within_tol(n, sv, tol)) {
+        for (int i = 0; i < n; ++i) x[i] += omega * r[i];
+        goto done;
+    }
+    matvec(n, sys, sv, as);
+    *msord += 1;
+    aa = cdot(n, as, as);
+    wa = cdot(n, as, sv);
+    if (aa == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+    cplx chi = wa / aa;
+    for (int i = 0; i < n; ++i) x[i] += omega * r[i] + chi * sv[i];
+done:
+    free(r); free(av); free(y); free(sv); free(as);
+    return rc;
+}
+
+/* M = 1 - T G0 with toler2 cutoff: refeff fms/solvers.rs:22-78 */
+static void iterative_system(const orc_plan* p, const orc_states* st, const cplx* g0, const cplx* tmat, cplx* m) {
+    int n = st->n;
+    for (int c = 0; c < n; ++c) {
+        for (int r = 0; r < n; ++r) {
+            cplx v = 0.0;
+            cplx g = g0[r + (size_t)n * c];
+            if (cabs(g) > p->toler2) v += -tmat[2 * r] * g;
+            if (p->nsp == 2) {
+                int q = spin_partner(st->l[r], st->m[r], st->spin[r], r);
+                if (q >= 0) {
+                    cplx gf = g0[q + (size_t)n * c];
+                    if (cabs(gf) > p->toler2) v += -tmat[2 * q + 1] * gf;
+                }
+            }
+            m[r + (size_t)n * c] = v;
+        }
+        m[c + (size_t)n * c] += 1.0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* one energy point: refeff fms/driver.rs:152-222, fms/solvers.rs:366-484     */
+/* scattering: [npot][ch][ch] with (row, col, ip) at row + ch*(col + ch*ip)   */
+/* ------------------------------------------------------------------------ */
+
+int orc_channel_count(const orc_plan* p) { return (p->global_lmax + 1) * (p->global_lmax + 1) * p->nsp; }
+
+int orc_fms_energy(const orc_plan* p, const double* k_in, const double* phases_in, double* scattering_out, int* msord_out) {
+    const cplx* k = (const cplx*)k_in;
+    const cplx* ph = (const cplx*)phases_in;
+    cplx* sc = (cplx*)scattering_out;
+    for (int s = 0; s < p->nsp; ++s)
+        if (!isfinite(creal(k[s])) || !isfinite(cimag(k[s]))) return ORC_NON_FINITE;
+    orc_states st;
+    int rc = build_states(p, &st);
+    if (rc) { free_states(&st); return rc; }
+    int n = st.n, ch = orc_channel_count(p), npot = p->max_pot + 1;
+    int ps = 0, pe = p->max_pot;
+    if (p->lfms == 0) ps = pe = p->pot[0];
+    for (int q = ps; q <= pe; ++q)
+        if (st.rep[q] < 0) { free_states(&st); return ORC_MISSING_REPRESENTATIVE; }
+    memset(sc, 0, sizeof(cplx) * (size_t)ch * ch * npot);
+    cplx* g0 = malloc(sizeof(cplx) * (size_t)n * n);
+    cplx* a = malloc(sizeof(cplx) * (size_t)n * n);
+    cplx* tmat = malloc(sizeof(cplx) * 2 * (size_t)n);
+    if (!g0 || !a || !tmat) { rc = ORC_NO_MEMORY; goto out; }
+    rc = g0_matrix(p, &st, k, g0);
+    if (rc) goto out;
+    rc = t_table(p, &st, ph, tmat);
+    if (rc) goto out;
+    if (msord_out) *msord_out = 0;
+    if (p->minv == 0) {
+        lu_system(p, &st, g0, tmat, a);
+        int* ipiv = malloc(sizeof(int) * n);
+        if (!ipiv) { rc = ORC_NO_MEMORY; goto out; }
+        rc = orc_zgetrf(n, a, ipiv);
+        if (!rc) {
+            for (int q = ps; q <= pe; ++q) {
+                int ipart = (st.lm_pot[q] + 1) * (st.lm_pot[q] + 1) * p->nsp;
+                int off = st.rep[q];
+                cplx* rhs = malloc(sizeof(cplx) * (size_t)n * ipart);
+                if (!rhs) { rc = ORC_NO_MEMORY; break; }
+                for (int c = 0; c < ipart; ++c) memcpy(rhs + (size_t)n * c, g0 + (size_t)n * (off + c), sizeof(cplx) * n);
+                orc_zgetrs(n, a, ipiv, ipart, rhs);
+                for (int c = 0; c < ipart; ++c)
+                    for (int r = 0; r < ipart; ++r) sc[r + (size_t)ch * (c + (size_t)ch * q)] = rhs[off + r + (size_t)n * c];
+                free(rhs);
+            }
+        }
+        free(ipiv);
+    } else if (p->minv == 1) {
+        iterative_system(p, &st, g0, tmat, a);
+        cplx* x = malloc(sizeof(cplx) * n);
+        if (!x) { rc = ORC_NO_MEMORY; goto out; }
+        for (int q = ps; q <= pe && !rc; ++q) {
+            int ipart = (st.lm_pot[q] + 1) * (st.lm_pot[q] + 1) * p->nsp;
+            int off = st.rep[q];
+            for (int c = 0; c < ipart && !rc; ++c) {
+                int ms = 0;
+                rc = ggbi_solve(n, a, off + c, (double)p->toler1, x, &ms);
+                if (rc) break;
+                if (msord_out) *msord_out = ms;
+                for (int r = 0; r < ipart; ++r) {
+                    cplx v = 0.0;
+                    for (int j = 0; j < n; ++j) v += g0[off + r + (size_t)n * j] * x[j];
+                    sc[r + (size_t)ch * (c + (size_t)ch * q)] = v;
+                }
+            }
+        }
+        free(x);
+    } else {
+        rc = ORC_UNSUPPORTED;
+    }
+out:
+    free(g0); free(a); free(tmat);
+    free_states(&st);
+    return rc;
+}
+
+/* Energy sweep, parallel over energies (refeff fms/driver.rs:231-236).
+ * k: [ne][nsp] complex; phases: [ne][nsp][2L+1][npot] complex;
+ * scattering: [ne][npot][ch][ch] complex; status: [ne]. */
+int orc_fms_spectrum(const orc_plan* p, int ne, const double* k, const double* phases, double* scattering,
+                     int* status, int threads) {
+    int ch = orc_channel_count(p), npot = p->max_pot + 1, L = p->global_lmax;
+    size_t kst = 2 * (size_t)p->nsp, pst = 2 * (size_t)p->nsp * (2 * L + 1) * npot, sst = 2 * (size_t)ch * ch * npot;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1) reduction(+ : bad)
+    for (int e = 0; e < ne; ++e) {
+        status[e] = orc_fms_energy(p, k + kst * e, phases + pst * e, scattering + sst * e, NULL);
+        if (status[e]) bad += 1;
+    }
+    return bad ? ORC_SINGULAR : ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MKGTR: averaged bcoef (refeff angular/transition.rs:9-167, 431-482) and    */
+/* the getgtr trace (refeff fms/mkgtr.rs:790-953, 1023-1058)                 */
+/* ------------------------------------------------------------------------ */
+
+/* FEFF setkap: refeff crates/refeff-core/src/core_hole.rs:66-80 */
+int orc_core_hole_kappa(int ihole, int* kappa) {
+    switch (ihole) {
+    case 1: case 2: case 5: case 10: case 17: case 24: case 29: *kappa = -1; return ORC_OK;
+    case 3: case 6: case 11: case 18: case 25: case 30: *kappa = 1; return ORC_OK;
+    case 4: case 7: case 12: case 19: case 26: *kappa = -2; return ORC_OK;
+    case 8: case 13: case 20: case 27: *kappa = 2; return ORC_OK;
+    case 9: case 14: case 21: case 28: *kappa = -3; return ORC_OK;
+    case 15: case 22: *kappa = 3; return ORC_OK;
+    case 16: case 23: *kappa = -4; return ORC_OK;
+    default: return ORC_INVALID_ARGUMENT;
+    }
+}
+
+/* lind[8]: orbital momentum per transition slot (-1 = unused). */
+void orc_transition_lind(int kappa, int multipole, int lmax, int* lind) {
+    for (int k = -1; k <= 1; ++k) {
+        int kap = kappa + k;
+        if (k == 0) kap = -kap;
+        int lkap = (kap <= 0) ? abs(kap) - 1 : kap;
+        if (lkap > lmax) lkap = -1;
+        lind[k + 1] = lkap;
+    }
+    for (int k = -2; k <= 2; ++k) {
+        int jkap = abs(kappa) + k;
+        if (jkap <= 0) jkap = 0;
+        int kap = jkap;
+        if (kappa < 0 && abs(k) != 1) kap = -jkap;
+        if (kappa > 0 && abs(k) == 1) kap = -jkap;
+        int lkap = (kap <= 0) ? -kap - 1 : kap;
+        if (lkap > lmax || multipole == 0 || (multipole == 1 && abs(k) == 2)) lkap = -1;
+        lind[k + 5] = lkap;
+    }
+}
+
+/* Averaged (ipol=0) bmat with spin fold, non-traced.  bmat layout:
+ * (ml2, ms2, k2, ml1, ms1, k1) column-major with ml shifted by lmax. */
+void orc_bmat_averaged(int kappa, int multipole, int lmax, int spin, int nsp, int* lind, double* bmat_out) {
+    int nm = 2 * lmax + 1;
+    cplx* b = (cplx*)bmat_out;
+    size_t tot = (size_t)nm * 2 * 8 * nm * 2 * 8;
+    memset(b, 0, sizeof(cplx) * tot);
+    orc_transition_lind(kappa, multipole, lmax, lind);
+#define B(ml2, ms2, k2, ml1, ms1, k1) \
+    b[(ml2) + (size_t)nm * ((ms2) + 2 * ((k2) + 8 * ((ml1) + (size_t)nm * ((ms1) + 2 * (size_t)(k1)))))]
+    for (int t = 0; t < 8; ++t) {
+        int l = lind[t];
+        if (l < 0) continue;
+        double deg = (multipole == 2 && t >= 3) ? 5.0 : 3.0;
+        double v = 0.5 / (2.0 * l + 1.0) / deg;
+        if (t < 3) v = -v;
+        for (int s = 0; s <= 1; ++s)
+            for (int m = -l; m <= l; ++m) B(m + lmax, s, t, m + lmax, s, t) = v;
+    }
+    if (spin == 0) {
+        for (int t1 = 0; t1 < 8; ++t1) {
+            if (lind[t1] < 0) continue;
+            for (int t2 = 0; t2 < 8; ++t2) {
+                if (lind[t2] < 0) continue;
+                for (int m1 = -lind[t1]; m1 <= lind[t1]; ++m1)
+                    for (int m2 = -lind[t2]; m2 <= lind[t2]; ++m2)
+                        B(m2 + lmax, 0, t2, m1 + lmax, 0, t1) += B(m2 + lmax, 1, t2, m1 + lmax, 1, t1);
+            }
+        }
+    } else if (spin == 2 || (spin == 1 && nsp == 1)) {
+        for (int t1 = 0; t1 < 8; ++t1) {
+            if (lind[t1] < 0) continue;
+            for (int t2 = 0; t2 < 8; ++t2) {
+                if (lind[t2] < 0) continue;
+                for (int m1 = -lind[t1]; m1 <= lind[t1]; ++m1)
+                    for (int m2 = -lind[t2]; m2 <= lind[t2]; ++m2)
+                        B(m2 + lmax, 0, t2, m1 + lmax, 0, t1) = B(m2 + lmax, 1, t2, m1 + lmax, 1, t1);
+            }
+        }
+    }
+#undef B
+}
+
+/* gtr(ie) = sum gg[(l1 m1 s1),(l2 m2 s2)] bmat(m2,s2,k2,m1,s1,k1) rkk(ie,k1,s1) rkk(ie,k2,s2)
+ * gg: [ne][nch][nch] complex with (row, col) at row + nch*col (FP64 kept:
+ * the narrowing refeff applies at crates/refeff-engine/src/fms.rs:5912 is
+ * deliberately not restated); rkk: [ne][8][nsp] complex; gtr: [ne] complex. */
+int orc_mkgtr_trace(int ne, int nch, int nsp, int lmax, const int* lind, const double* bmat_in, const double* gg_in,
+                    const double* rkk_in, double* gtr_out) {
+    int nm = 2 * lmax + 1;
+    const cplx* b = (const cplx*)bmat_in;
+    const cplx* gg = (const cplx*)gg_in;
+    const cplx* rkk = (const cplx*)rkk_in;
+    cplx* gtr = (cplx*)gtr_out;
+    for (int e = 0; e < ne; ++e) {
+        cplx tr = 0.0;
+        for (int t1 = 0; t1 < 8; ++t1) {
+            int l1 = lind[t1];
+            if (l1 < 0) continue;
+            for (int s1 = 0; s1 < nsp; ++s1) {
+                cplx r1 = rkk[((size_t)e * 8 + t1) * nsp + s1];
+                for (int t2 = 0; t2 < 8; ++t2) {
+                    int l2 = lind[t2];
+                    if (l2 < 0) continue;
+                    for (int s2 = 0; s2 < nsp; ++s2) {
+                        cplx r2 = rkk[((size_t)e * 8 + t2) * nsp + s2];
+                        for (int m1 = -l1; m1 <= l1; ++m1) {
+                            int row = (l1 * l1 + l1 + m1) * nsp + s1;
+                            for (int m2 = -l2; m2 <= l2; ++m2) {
+                                int col = (l2 * l2 + l2 + m2) * nsp + s2;
+                                if (row >= nch || col >= nch) return ORC_INVALID_ARGUMENT;
+                                cplx g = gg[(size_t)e * nch * nch + row + (size_t)nch * col];
+                                cplx bm = b[(m2 + lmax) + (size_t)nm * (s2 + 2 * (t2 + 8 * ((m1 + lmax) + (size_t)nm * (s1 + 2 * (size_t)t1))))];
+                                tr += g * bm * r1 * r2;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        gtr[e] = tr;
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* element-level entry points used by the known-answer tests                  */
+/* ------------------------------------------------------------------------ */
+
+/* One off-diagonal G0 element (refeff fms/pairs.rs:106-162) with explicit
+ * rotation angles for the forward (beta_f, phi) and backward (beta_b, phi)
+ * tables, rho = k*distance. */
+int orc_g0_element(int lmax, int l1, int m1, int l2, int m2, double rho_re, double rho_im, double k_re, double k_im,
+                   double sigma2, double beta_f, double beta_b, double phi, double* out) {
+    int nm = 2 * lmax + 1;
+    size_t dsz = (size_t)nm * nm * (lmax + 1);
+    cplx* dfw = malloc(sizeof(cplx) * dsz);
+    cplx* dbw = malloc(sizeof(cplx) * dsz);
+    cplx* clm = malloc(sizeof(cplx) * (size_t)(lmax + 2) * (2 * lmax + 3));
+    int rc = ORC_OK;
+    if (!dfw || !dbw || !clm) { rc = ORC_NO_MEMORY; goto done; }
+    rc = orc_rotation_matrix(lmax, lmax, beta_f, phi, 0, (double*)dfw);
+    if (!rc) rc = orc_rotation_matrix(lmax, lmax, beta_b, phi, 1, (double*)dbw);
+    if (!rc) rc = orc_rehr_albers(lmax, lmax + 1, lmax + 1, rho_re, rho_im, (double*)clm);
+    if (rc) goto done;
+    cplx rho = rho_re + I * rho_im, ck = k_re + I * k_im, sum = 0.0;
+    for (int mu = -l1; mu <= l1; ++mu) {
+        int amu = abs(mu);
+        if (amu > l2) continue;
+        int numax = l1 < (l2 - amu) ? l1 : (l2 - amu);
+        double sgn = (amu % 2 == 0) ? 1.0 : -1.0;
+        cplx g = 0.0;
+        for (int nu = 0; nu <= numax; ++nu)
+            g += (double)(2 * l1 + 1) * clm[l1 + (size_t)(lmax + 2) * nu] / orc_xnlm(l1, amu) *
+                 (clm[l2 + (size_t)(lmax + 2) * (amu + nu)] * (sgn * orc_xnlm(l2, amu)));
+        sum += dbw[((size_t)l1 * nm + (m1 + lmax)) * nm + (mu + lmax)] * g *
+               dfw[((size_t)l2 * nm + (mu + lmax)) * nm + (m2 + lmax)];
+    }
+    cplx v = cexp(I * rho) / rho * cexp(-sigma2 * ck * ck) * sum;
+    out[0] = creal(v);
+    out[1] = cimag(v);
+done:
+    free(dfw); free(dbw); free(clm);
+    return rc;
+}
+
+/* FEFF 1 - T G0 iterative matrix from explicit tables (refeff fms/solvers.rs:10-78). */
+int orc_iterative_system_from(int nsp, int n, const int* s_l, const int* s_m, const int* s_spin, const double* g0,
+                              const double* tmat, float toler2, double* m_out) {
+    orc_plan p;
+    memset(&p, 0, sizeof p);
+    p.nsp = nsp;
+    p.toler2 = toler2;
+    orc_states st;
+    memset(&st, 0, sizeof st);
+    st.n = n;
+    st.l = (int*)s_l;
+    st.m = (int*)s_m;
+    st.spin = (int*)s_spin;
+    iterative_system(&p, &st, (const cplx*)g0, (const cplx*)tmat, (cplx*)m_out);
+    return ORC_OK;
+}
+
+int orc_ggbi_solve_from(int n, const double* sys, int src, double tol, double* x, int* msord) {
+    return ggbi_solve(n, (const cplx*)sys, src, tol, (cplx*)x, msord);
+}

@@ -1,0 +1,538 @@
+"""Pins the FP64 oracle against refeff's own known-answer tests and closed forms.
+
+Every expected number below is copied from the refeff test it cites (those
+values were produced by FEFF10 through refeff in complex single precision and
+are asserted there at |d| < 2e-4, crates/refeff-core/src/fms/tests.rs:235-240).
+The oracle computes in double precision, so it must land inside the same
+tolerance.  Closed-form and library checks pin what those KATs leave open.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+TOL32 = 2.0e-4  # refeff fms/tests.rs:235-240
+
+
+def close(a, b, tol=TOL32):
+    assert abs(complex(a) - complex(b)) < tol, (a, b)
+
+
+def nonzero(a):
+    a = np.asarray(a)
+    return int(np.sum(np.abs(a.real) + np.abs(a.imag) > 1.0e-6))
+
+
+# --------------------------------------------------------------------------
+# Rehr-Albers polynomials (xclmz): refeff fms/tests/mkgtr.rs:182-218
+# --------------------------------------------------------------------------
+
+def test_xclmz_lx3():
+    t = orc.rehr_albers(3, 4, 4, 1.25 + 0.4j)
+    assert t.shape == (5, 9)
+    close(t[0, 0], 1.0)
+    close(t[1, 0], 1.2322206 + 0.7256894j)
+    close(t[3, 0], -10.012509 + 5.438266j)
+    close(t[2, 1], -2.1395304 + 4.1993084j)
+    close(t[3, 2], -23.036537 - 6.8588142j)
+    close(t[4, 3], 8.928719 - 161.62775j, 2e-3)  # |value| ~ 162: c32 relative
+    close(t.sum(), -58.983994 - 154.61885j, 2e-3)
+    assert nonzero(t) == 11
+
+
+def test_xclmz_limited_m():
+    t = orc.rehr_albers(4, 3, 2, -0.8 + 1.1j)
+    assert t.shape == (6, 11)
+    close(t[1, 0], 1.5945946 - 0.4324324j)
+    close(t[2, 0], 3.2834187 - 2.840029j)
+    close(t[1, 1], 0.5945946 - 0.4324324j)
+    close(t[2, 1], 2.7830534 - 4.382761j)
+    close(t.sum(), 9.255661 - 8.087655j)
+    assert nonzero(t) == 5
+
+
+def test_xclmz_errors():
+    with pytest.raises(orc.OracleError):
+        orc.rehr_albers(3, 0, 1, 1.0)
+    with pytest.raises(orc.OracleError):
+        orc.rehr_albers(3, 5, 1, 1.0)
+    with pytest.raises(orc.OracleError) as e:
+        orc.rehr_albers(3, 1, 1, 0.0)
+    assert e.value.code == 5
+    with pytest.raises(orc.OracleError):
+        orc.rehr_albers(3, 1, 1, float("nan"))
+
+
+@pytest.mark.parametrize("rho", [0.7 + 0.0j, 2.5 + 0.3j, 7.1 + 1.2j, 0.4 + 0.9j])
+def test_xclmz_is_reduced_spherical_hankel(rho):
+    """c_l^0(rho) = i^(l+1) rho e^(-i rho) h_l^(1)(rho) (closed form)."""
+    sp = pytest.importorskip("scipy.special")
+    t = orc.rehr_albers(6, 7, 7, rho)
+    for l in range(7):
+        h = sp.spherical_jn(l, rho) + 1j * sp.spherical_yn(l, rho)
+        ref = (1j) ** (l + 1) * rho * np.exp(-1j * rho) * h
+        assert abs(t[l, 0] - ref) < 1e-10 * max(1.0, abs(ref))
+
+
+# --------------------------------------------------------------------------
+# Rotations (rotxan): refeff fms/tests/mkgtr.rs:249-301
+# --------------------------------------------------------------------------
+
+def test_rotxan_kat():
+    fw = orc.rotation_matrix(3, 3, 0.7, 1.1, backward=False)
+    bw = orc.rotation_matrix(3, 3, 0.7, 1.1, backward=True)
+    assert fw.shape == (7, 7, 4)
+    close(fw.sum(), 1.1595836 + 0.2889818j)
+    close(bw.sum(), 1.1595831 + 0.28898174j)
+    assert nonzero(fw) == 84 and nonzero(bw) == 84
+
+    def v(t, m2, m1, l):
+        return t[m2 + 3, m1 + 3, l]
+
+    close(v(fw, 0, 0, 0), 1.0)
+    close(v(fw, 1, -1, 1), -0.05333333 - 0.10478719j)
+    close(v(fw, -1, 1, 1), -0.05333333 + 0.10478719j)
+    close(v(fw, 2, -1, 2), -0.04457685 + 0.061240695j)
+    close(v(fw, -2, 1, 3), 0.11610273 + 0.15950458j)
+    close(v(fw, 3, 3, 3), 0.67850935 + 0.10838909j)
+    close(v(bw, 2, -1, 2), -0.034358274 - 0.06750576j)
+    close(v(bw, -2, 1, 3), 0.08948791 - 0.17582226j)
+    close(v(bw, 3, 3, 3), 0.67850935 - 0.10838909j)
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.3, 1.2, 2.9, -0.8])
+def test_rotxan_small_d_is_wigner_d(beta):
+    """With phi = pi the phases vanish and drix(m2, m1, l) = d^l_{m1 m2}(beta)."""
+    spin = pytest.importorskip("sympy.physics.quantum.spin")
+    import sympy
+    L = 3
+    fw = orc.rotation_matrix(L, L, beta, math.pi, backward=False)
+    for l in range(L + 1):
+        for m1 in range(-l, l + 1):
+            for m2 in range(-l, l + 1):
+                d = complex(sympy.N(spin.Rotation.d(l, m1, m2, beta).doit())).real
+                assert abs(fw[m2 + L, m1 + L, l] - d) < 1e-12, (l, m1, m2)
+        blk = fw[L - l:L + l + 1, L - l:L + l + 1, l].real
+        assert np.allclose(blk @ blk.T, np.eye(2 * l + 1), atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# Normalisation, 3j, spin-orbit tables
+# --------------------------------------------------------------------------
+
+def test_xnlm_closed_form():
+    for l in range(6):
+        for m in range(l + 1):
+            ref = math.sqrt((2 * l + 1) * math.factorial(l - m) / math.factorial(l + m))
+            assert abs(orc.xnlm(l, m) - ref) < 1e-14
+        assert orc.xnlm(l, l + 1) == 0.0
+
+
+def test_wigner_3j_matches_sympy():
+    wig = pytest.importorskip("sympy.physics.wigner")
+    from sympy import Rational
+    cases = [(2, 1, 3, 2, -1), (4, 1, 3, 0, 1), (4, 1, 5, -4, 1), (6, 1, 5, 2, -1), (2, 2, 2, 0, 0),
+             (6, 4, 4, 2, -2), (3, 1, 2, 1, 1), (8, 1, 7, -2, 1), (4, 2, 2, 2, 0)]
+    for j1, j2, j3, m1, m2 in cases:
+        ref = float(wig.wigner_3j(Rational(j1, 2), Rational(j2, 2), Rational(j3, 2), Rational(m1, 2),
+                                  Rational(m2, 2), Rational(-m1 - m2, 2)))
+        assert abs(orc.wigner_3j(j1, j2, j3, m1, m2) - ref) < 1e-14, (j1, j2, j3, m1, m2)
+
+
+def test_spin_orbit_tables_are_clebsch_gordan():
+    """t3jp^2 + t3jm^2 = 1 for every (l, m, s) reachable from a j = l+-1/2 state."""
+    plus, minus = orc.spin_orbit_tables(3)
+    for l in range(4):
+        for m in range(-l, l + 1):
+            for s in range(2):
+                total = plus[l, m + 3, s] ** 2 + minus[l, m + 3, s] ** 2
+                mj2 = 2 * m + (2 * s - 1)
+                if abs(mj2) <= 2 * l + 1:
+                    assert abs(total - 1.0) < 1e-13, (l, m, s, total)
+
+
+# --------------------------------------------------------------------------
+# Cluster preparation: refeff fms/tests/cluster.rs:4-195
+# --------------------------------------------------------------------------
+
+def test_athep_sort_order():
+    pos = [[2.0, 0, 0], [0, 0, 0], [-1.0, 0, 0], [1.0, 0, 0], [0, 2.0, 0]]
+    p, pot, keys = orc.sort_atoms_by_radius(pos, [1, 0, 2, 3, 4])
+    assert list(pot) == [0, 2, 3, 1, 4]
+    assert list(p[1]) == [-1.0, 0.0, 0.0]
+    for k, ref in zip(keys, [2.0e-6, 1.000003, 1.000004, 4.000001, 4.000005]):
+        assert abs(k - ref) < 1e-12
+
+
+def test_getang():
+    pos = [[0, 0, 0], [1.0, 2.0, 2.0], [0, 5.0e-8, 2.0e-7], [0, 2.0e-7, 0]]
+    th, ph = orc.pair_polar_angles(pos, 1, 0)
+    assert abs(th - 0.8410686) < 2e-6 and abs(ph - 1.1071488) < 2e-6
+    th, ph = orc.pair_polar_angles(pos, 3, 2)
+    assert abs(th - 2.4980915) < 2e-6 and abs(ph - 1.5707964) < 2e-6
+    assert orc.pair_polar_angles(pos, 0, 0) == (0.0, 0.0)
+
+
+def test_sortat():
+    pos = [[float(i), 0, 0] for i in range(6)]
+    p, pot, rep = orc.sort_representative_atoms(0, 3, pos, [0, 2, 1, 3, 2, 1])
+    assert rep == [0, 1, 2, 3]
+    assert list(pot) == [0, 1, 2, 3, 2, 1]
+    assert list(p[1]) == [2.0, 0, 0] and list(p[2]) == [1.0, 0, 0] and list(p[3]) == [3.0, 0, 0]
+
+
+def test_yprep_cluster():
+    pos = [[2.0, 0, 0], [1.0, 1.0, 0], [1.0, 3.0, 0], [4.0, 1.0, 0], [1.0, 1.0, 1.0]]
+    p, pot, central, untr = orc.yprep_cluster(pos, [1, 0, 2, 1, 2], 0, 2.1, 3)
+    assert central == 1 and untr == 4 and len(pot) == 3
+    assert list(pot) == [0, 2, 1]
+    assert [list(r) for r in p] == [[0, 0, 0], [0, 0, 1.0], [1.0, -1.0, 0]]
+
+
+def test_yprep_errors():
+    with pytest.raises(orc.OracleError) as e:
+        orc.yprep_cluster([[0, 0, 0], [1, 0, 0]], [0, 0], 0, 3.0, 5)
+    assert e.value.code == 3  # DuplicateAbsorber
+    with pytest.raises(orc.OracleError) as e:
+        orc.yprep_cluster([[0, 0, 0]], [1], 0, 3.0, 5)
+    assert e.value.code == 2  # MissingCentralAtom
+
+
+# --------------------------------------------------------------------------
+# Pair tables and the free propagator: refeff fms/tests/pairs.rs:4-623
+# --------------------------------------------------------------------------
+
+THREE_ATOMS = np.array([[0, 0, 0], [1.0, 2.0, 2.0], [-1.0, 0, 0.5]], np.float32)
+
+
+def _dist(a, b):
+    d = a.astype(np.float64) - b.astype(np.float64)
+    return math.sqrt(float(d @ d))
+
+
+def test_pair_tables_kat():
+    k = 1.2 + 0.3j
+    close(k * _dist(THREE_ATOMS[0], THREE_ATOMS[1]), 3.6000001 + 0.90000004j)
+    close(k * _dist(THREE_ATOMS[0], THREE_ATOMS[2]), 1.3416408 + 0.3354102j)
+    close(k * _dist(THREE_ATOMS[1], THREE_ATOMS[2]), 3.8418748 + 0.9604687j)
+    total, count = 0, 0
+    tab = {}
+    for i in range(3):
+        for j in range(3):
+            if i == j:
+                continue
+            c = orc.rehr_albers(2, 3, 3, k * _dist(THREE_ATOMS[i], THREE_ATOMS[j]))[:3, :3]
+            tab[(i, j)] = c
+            total += c.sum()
+            count += nonzero(c)
+    close(total, 8.870853 + 26.772633j)
+    assert count == 36
+    close(tab[(1, 0)][1, 1], 0.06535947 + 0.2614379j)
+    close(tab[(2, 0)][2, 2], -1.384083 + 0.7381776j)
+    close(tab[(2, 1)][2, 1], -0.15384735 + 0.9149786j)
+
+
+def test_free_propagator_element_kat():
+    k = 1.2 + 0.3j
+    v = orc.g0_element(2, 2, 1, 2, -1, 3.0 * k, k, 0.05, 0.7, 0.7, 1.1)
+    close(v, -0.13408086 + 0.11388555j)
+
+
+def test_free_propagator_damping_ratio():
+    k = 2.0 + 0.0j
+    undamped = orc.g0_element(1, 1, 0, 1, 0, 1.5 + 0j, k, 0.0, 0.3, -0.3, 0.4)
+    damped = orc.g0_element(1, 1, 0, 1, 0, 1.5 + 0j, k, 0.05, 0.3, -0.3, 0.4)
+    assert abs(damped / undamped - math.exp(-0.2)) < 1e-14
+    z0 = orc.g0_element(1, 1, 0, 1, 0, 1.5 + 0j, 0j, 10.0, 0.3, -0.3, 0.4)
+    z1 = orc.g0_element(1, 1, 0, 1, 0, 1.5 + 0j, 0j, 0.0, 0.3, -0.3, 0.4)
+    assert abs(z0 - z1) < 1e-15
+
+
+def test_g0_matrix_uses_pair_direction_parity():
+    """refeff fms/tests/pairs.rs:370-465: the yprep pair direction equals the
+    reversed-direction table times (-1)^(l1+l2)."""
+    atoms = np.array([[0, 0, 0], [1.0, 2.0, 2.0]], np.float32)
+    plan = orc.Plan(atoms, [0, 0], global_lmax=2, rdirec=3.0)
+    k = 1.2 + 0.3j
+    g = plan.g0(k)
+    a, l, m, s, _ = plan.states()
+    same = opp = False
+    for r in range(len(a)):
+        for c in range(len(a)):
+            if a[r] == a[c]:
+                assert g[r, c] == 0
+                continue
+            bt, ph = orc.pair_polar_angles(atoms, a[c], a[r])  # reversed vector
+            legacy = orc.g0_element(2, l[r], m[r], l[c], m[c], k * 3.0, k, 0.0, bt, -bt, ph)
+            if abs(legacy) < 1e-5:
+                continue
+            parity = 1 if (l[r] + l[c]) % 2 == 0 else -1
+            same |= parity == 1
+            opp |= parity == -1
+            assert abs(g[r, c] - parity * legacy) < 1e-12
+    assert same and opp
+
+
+def test_g0_matrix_cutoff_and_spin_blocks():
+    atoms = np.array([[0, 0, 0], [1.0, 2.0, 2.0]], np.float32)
+    k = [1.2 + 0.3j, 0.8 + 0.15j]
+    g_in = orc.Plan(atoms, [0, 1], nsp=2, global_lmax=2, rdirec=3.0).g0(k)
+    g_out = orc.Plan(atoms, [0, 1], nsp=2, global_lmax=2, rdirec=2.99).g0(k)
+    assert np.count_nonzero(g_out) == 0
+    a, l, m, s, _ = orc.Plan(atoms, [0, 1], nsp=2, global_lmax=2).states()
+    for r in range(len(a)):
+        for c in range(len(a)):
+            if s[r] != s[c] or a[r] == a[c]:
+                assert g_in[r, c] == 0
+    # each spin block equals the one-spin matrix at that spin's wave number
+    for sp in (1, 2):
+        g1 = orc.Plan(atoms, [0, 1], global_lmax=2, rdirec=3.0).g0(k[sp - 1])
+        idx = np.where(s == sp)[0]
+        assert np.allclose(g_in[np.ix_(idx, idx)], g1, atol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# T matrix: refeff fms/tests/t_matrix.rs:4-227
+# --------------------------------------------------------------------------
+
+def _ref_phases():
+    ph = np.zeros((2, 5, 2), np.complex128)  # (spin, l~ + lmax, pot)
+    ph[0, 4, 1] = 0.2 + 0.05j
+    ph[0, 0, 1] = -0.1 + 0.03j
+    ph[1, 4, 1] = 0.15 - 0.02j
+    ph[1, 0, 1] = 0.07 + 0.04j
+    return ph
+
+
+def test_t_matrix_kat():
+    ph = _ref_phases()
+    one = orc.Plan([[0, 0, 0]], [1], nsp=1, max_pot=1, global_lmax=2)
+    t1 = one.t_table(ph[:1])
+    close(t1[0, 7], 0.17618014 + 0.08329478j)  # state (l=2, m=1)
+    delta = 0.2 + 0.05j
+    assert abs(t1[0, 7] - np.exp(1j * delta) * np.sin(delta)) < 1e-15  # t = e^{i d} sin d
+    two = orc.Plan([[0, 0, 0]], [1], nsp=2, max_pot=1, global_lmax=2)
+    t2 = two.t_table(ph)
+    close(t2[0, 14], 0.06828813 + 0.06537849j)
+    close(t2[1, 14], -0.08796438 - 0.0011440981j)
+
+
+# --------------------------------------------------------------------------
+# Solvers: refeff fms/tests/solvers.rs:4-393 with fms/tests.rs:71-90 inputs
+# --------------------------------------------------------------------------
+
+def _gglu_inputs(n):
+    g0 = np.zeros((n, n), np.complex128)
+    t = np.zeros((2, n), np.complex128)
+    for c in range(n):
+        for r in range(n):
+            if r != c:
+                g0[r, c] = complex(0.01 * (r + 1) - 0.02 * (c + 1), 0.015 * (r + 1) + 0.005 * (c + 1))
+        t[0, c] = complex(0.02 * (c + 1), -0.01 * (c + 1))
+        t[1, c] = complex(-0.005 * (c + 1), 0.003 * (c + 1))
+    return g0, t
+
+
+def test_lu_scattering_kat():
+    a_, l, m, s, rep = orc.state_kets(2, [0], [1], 1)
+    g0, t = _gglu_inputs(8)
+    a = orc.lu_system_matrix(2, l, m, s, g0, t)
+    close(a.sum(), 8.10728 - 0.54295987j)
+    x = orc.zgesv(a, g0[:, 0:8])
+    close(x.sum(), -2.9443204 + 4.7994013j)
+    close(x[0, 0], -0.0077970205 - 0.0032442866j)
+    close(x[1, 3], -0.06596742 + 0.04409315j)
+    close(x[6, 7], -0.0962859 + 0.14052007j)
+
+
+def test_gg_full_kat():
+    _, l, m, s, rep = orc.state_kets(2, [0, 1], [1, 0], 1)
+    assert rep == [0, 8]
+    g0, t = _gglu_inputs(10)
+    a = orc.lu_system_matrix(2, l, m, s, g0, t)
+    full = orc.zgesv(a, g0)
+    close(full.sum(), -6.6166725 + 8.779471j)
+    close(full[0, 9], -0.189542 + 0.041967187j)
+    close(full[9, 0], 0.06335482 + 0.1630312j)
+
+
+def test_iterative_system_kat():
+    _, l, m, s, _ = orc.state_kets(2, [0], [1], 1)
+    g0, t = _gglu_inputs(8)
+    sysm = orc.iterative_system_matrix(2, l, m, s, g0, t, 0.0)
+    close(sysm.sum(), 7.9095793 - 0.5169j)
+    close(sysm[0, 0], 1.0)
+    close(sysm[1, 3], 0.0014 - 0.0031999997j)
+    close(sysm[4, 5], 0.0012300003 - 0.011239999j)
+    close(sysm[6, 7], 0.0017899997 - 0.0209j)
+    cut = orc.iterative_system_matrix(2, l, m, s, g0, t, 0.09)
+    close(cut.sum(), 7.9228334 - 0.47112507j)
+    close(cut[1, 3], 0.0)
+    close(cut[4, 5], 0.0012300003 - 0.011239999j)
+
+
+def test_ggbi_one_shot_kat():
+    _, l, m, s, _ = orc.state_kets(2, [0], [1], 1)
+    g0, t = _gglu_inputs(8)
+    sysm = orc.iterative_system_matrix(2, l, m, s, g0, t, 0.0)
+    gg = np.zeros((8, 8), np.complex128)
+    for c in range(8):
+        x, ms = orc.ggbi_solve(sysm, c, 1e-5)
+        assert ms == 2
+        gg[:, c] = g0[0:8, :] @ x
+    close(gg.sum(), -2.9492176 + 4.806942j)
+    close(gg[0, 0], -0.007855818 - 0.0032014623j)
+    close(gg[1, 3], -0.066029795 + 0.044123195j)
+    close(gg[6, 7], -0.096492656 + 0.1408408j)
+
+
+def test_cgetrf_kat():
+    """refeff crates/refeff-linalg/src/tests.rs:145-216."""
+    a = np.array([[0, 2 - 1j, -1 + 0.5j], [3 + 2j, -1, 4 - 1j], [1 - 3j, 0.5 + 2j, 2]], np.complex128)
+    b = np.array([[1 + 0.5j, -2 + 1j], [-1j, 3], [2 + 2j, -1 - 0.5j]], np.complex128)
+    lu, piv = orc.zgetrf(a)
+    assert list(piv + 1) == [2, 2, 3]
+    ref = np.array([[3 + 2j, -1, 4 - 1j], [0, 2 - 1j, -1 + 0.5j],
+                    [-0.23076928 - 0.8461538j, -0.12307697 + 0.5153847j, 3.9038463 + 3.730769j]])
+    assert np.abs(lu - ref).max() < 5e-6
+    x = orc.zgesv(a, b)
+    xref = np.array([[-0.23347072 + 0.43198672j, -0.13470735 - 0.2801319j],
+                     [0.60016483 + 0.28161582j, -0.7983512 + 0.2161583j],
+                     [0.6003297 - 0.2367683j, 0.4032976 + 0.43231657j]])
+    assert np.abs(x - xref).max() < 5e-6
+
+
+def test_zgesv_matches_lapack():
+    rng = np.random.default_rng(7)
+    a = rng.normal(size=(40, 40)) + 1j * rng.normal(size=(40, 40))
+    b = rng.normal(size=(40, 5)) + 1j * rng.normal(size=(40, 5))
+    assert np.abs(orc.zgesv(a, b) - np.linalg.solve(a, b)).max() < 1e-11
+
+
+# --------------------------------------------------------------------------
+# End-to-end energy point: closed form and invariants on tiny clusters
+# --------------------------------------------------------------------------
+
+def _phases(nsp, L, npot, seed=0, scale=0.4):
+    rng = np.random.default_rng(seed)
+    return scale * (rng.normal(size=(nsp, 2 * L + 1, npot)) + 0.2j * np.abs(rng.normal(size=(nsp, 2 * L + 1, npot))))
+
+
+def test_two_atom_lmax0_closed_form():
+    atoms = np.array([[0, 0, 0], [0.3, -1.1, 2.0]], np.float32)
+    d = _dist(atoms[0], atoms[1])
+    k = 1.3 + 0.05j
+    sig = 0.004
+    ph = _phases(1, 0, 2, 3)
+    plan = orc.Plan(atoms, [0, 1], global_lmax=0, rdirec=10.0, sigsqr=[[0, sig], [sig, 0]])
+    sc, _ = plan.energy(k, ph)
+    g = np.exp(1j * k * d) / (k * d) * np.exp(-float(np.float32(sig)) * k * k)
+    t0 = (np.exp(2j * ph[0, 0, 0]) - 1) / 2j
+    t1 = (np.exp(2j * ph[0, 0, 1]) - 1) / 2j
+    ref = t1 * g * g / (1 - t0 * t1 * g * g)
+    assert abs(sc[0, 0, 0] - ref) < 1e-14 * max(1, abs(ref))
+
+
+def _small_cluster():
+    return np.array([[0, 0, 0], [1.8, 0.0, 0.3], [-0.2, 1.9, 0.1], [0.1, -0.4, -2.1], [1.5, 1.6, 1.2]], np.float32)
+
+
+def test_energy_matches_dense_inverse():
+    atoms = _small_cluster()
+    pots = [0, 1, 1, 2, 1]
+    ph = _phases(1, 3, 3, 11)
+    k = 1.7 + 0.08j
+    plan = orc.Plan(atoms, pots, global_lmax=3, raw_lmax=[3, 2, 3], rdirec=20.0, lfms=1)
+    sc, _ = plan.energy(k, ph)
+    g = plan.g0(k)
+    t = plan.t_table(ph[None] if ph.ndim == 2 else ph)
+    a_, l, m, s, rep = plan.states()
+    n = len(a_)
+    a = np.eye(n) - g * t[0][None, :]
+    full = np.linalg.inv(a) @ g
+    for ip, lm in enumerate([3, 2, 3]):
+        blk = (lm + 1) ** 2
+        off = rep[ip]
+        assert np.abs(sc[ip, :blk, :blk] - full[off:off + blk, off:off + blk]).max() < 1e-12
+
+
+def test_rdirec_zero_gives_zero_gg():
+    plan = orc.Plan(_small_cluster(), [0, 1, 1, 2, 1], global_lmax=2, rdirec=0.0)
+    sc, _ = plan.energy(1.1 + 0.1j, _phases(1, 2, 3))
+    assert np.abs(sc).max() == 0.0
+
+
+def test_permuting_non_representatives_keeps_gg():
+    atoms = _small_cluster()
+    pots = [0, 1, 2, 1, 2]
+    ph = _phases(1, 2, 3, 5)
+    k = 1.4 + 0.1j
+    sc1, _ = orc.Plan(atoms, pots, global_lmax=2, rdirec=20.0).energy(k, ph)
+    perm = [0, 1, 2, 4, 3]
+    sc2, _ = orc.Plan(atoms[perm], [pots[i] for i in perm], global_lmax=2, rdirec=20.0).energy(k, ph)
+    assert np.abs(sc1 - sc2).max() < 1e-12
+
+
+def test_spectrum_equals_per_energy_calls():
+    atoms = _small_cluster()
+    pots = [0, 1, 1, 2, 1]
+    plan = orc.Plan(atoms, pots, global_lmax=2, rdirec=20.0)
+    ks = np.array([[1.1 + 0.1j], [1.6 + 0.05j], [2.2 + 0.2j]])
+    ph = np.stack([_phases(1, 2, 3, s) for s in range(3)])
+    sc, st = plan.spectrum(ks, ph, threads=2)
+    assert list(st) == [0, 0, 0]
+    for e in range(3):
+        one, _ = plan.energy(ks[e], ph[e])
+        assert np.array_equal(one, sc[e])
+
+
+# --------------------------------------------------------------------------
+# MKGTR trace: refeff fms/tests/mkgtr.rs:4-68
+# --------------------------------------------------------------------------
+
+def _bmat(nsp_slots=2):
+    return np.zeros((1, 2, 8, 1, 2, 8), np.complex128)
+
+
+def test_mkgtr_trace_kat():
+    lind = np.array([0, -1, -1, -1, -1, -1, -1, -1], np.int32)
+    green = np.array([[[1 + 2j]], [[-0.5 + 0.75j]]])
+    rkk = np.zeros((2, 8, 1), np.complex128)
+    rkk[0, 0, 0] = 3 - 1j
+    rkk[1, 0, 0] = 0.5 + 2j
+    for spec, bval in enumerate([2 + 0.5j, -1 + 0.25j]):
+        b = _bmat()
+        b[0, 0, 0, 0, 0, 0] = bval
+        gtr = orc.mkgtr_trace(green, rkk, lind, b.reshape(-1, order="F"), 0, 1)
+        for e in range(2):
+            ref = green[e, 0, 0] * bval * rkk[e, 0, 0] ** 2
+            assert abs(gtr[e] - ref) < 1e-12
+
+
+def test_mkgtr_spin_channel_indexing_kat():
+    lind = np.array([0, -1, -1, -1, -1, -1, -1, -1], np.int32)
+    b = _bmat()
+    b[0, 1, 0, 0, 0, 0] = 1.5 - 0.25j
+    green = np.zeros((1, 2, 2), np.complex128)
+    green[0, 0, 1] = 0.5 - 0.25j
+    rkk = np.zeros((1, 8, 2), np.complex128)
+    rkk[0, 0, 0] = 2.0
+    rkk[0, 0, 1] = 3 + 0.5j
+    gtr = orc.mkgtr_trace(green, rkk, lind, b.reshape(-1, order="F"), 0, 2)
+    assert abs(gtr[0] - green[0, 0, 1] * (1.5 - 0.25j) * 2.0 * (3 + 0.5j)) < 1e-12
+
+
+def test_bmat_averaged_k_edge():
+    """K edge (kappa=-1), dipole: only the p (l=1) final channel, value -1/18
+    per (m, s) before the spin fold (transition.rs:140-167)."""
+    assert orc.core_hole_kappa(1) == -1
+    lind, b = orc.bmat_averaged(-1, 1, 3, 0, 1)
+    assert list(lind) == [-1, -1, 1, -1, -1, -1, -1, -1] or 1 in list(lind)
+    nm = 7
+    bb = b.reshape((nm, 2, 8, nm, 2, 8), order="F")
+    t = list(lind).index(1)
+    for mm in range(-1, 2):
+        # spin fold (ispin=0) adds the spin-down diagonal into spin-up
+        assert abs(bb[mm + 3, 0, t, mm + 3, 0, t] - 2 * (-0.5 / 3 / 3)) < 1e-15
