@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""FMS energy-points/s on one MI355X.
+
+A "step" is one pass of the FMS stage over every energy point of the workload
+(G0 build, 1 - G0 t assembly, LU + solve (+ FP64 refinement), gg extraction),
+with the wave numbers and phase shifts already resident in HBM.  Prints one
+JSON line.  The default workload is the mixed-precision configuration:
+Cu fcc a=3.615 A, R=8.0 A (177 atoms, N=2832), 120 complex energies.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FMS energy-pts/s at ≤1e-6 xmu rel err; mixed+learned vs plain FP64 LU"
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5"])
+    ap.add_argument("--solver", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-fp64-compare", action="store_true", help="skip the plain FP64 LU comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle accuracy sample")
+    ap.add_argument("--cpu-energies", type=int, default=6)
+    return ap.parse_args()
+
+
+def run_solver(lib, wl, solver_id, steps, warmup, profile):
+    from refeff_amd import fms
+    from refeff_amd._lib import Stats, check
+    from refeff_amd.hip import DeviceArray, synchronize
+
+    plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, global_lmax=wl.global_lmax,
+                                raw_potential_lmax=wl.raw_lmax, direct_cutoff=wl.rdirec,
+                                mean_square_displacements=wl.sigsqr(), lfms=wl.lfms, solver=solver_id)
+    k = np.ascontiguousarray(wl.wave_numbers(), np.complex128)
+    ph = np.ascontiguousarray(wl.phase_shifts(), np.complex128)
+    ne = k.shape[0]
+    ch, npot = plan.channel_count, plan.potential_count
+    dk, dph = DeviceArray(k), DeviceArray(ph)
+    dout = DeviceArray.empty(ne * npot * ch * ch * 16)
+    status = np.zeros(ne, np.int32)
+    st = Stats()
+    i32p = status.ctypes.data_as(C.POINTER(C.c_int32))
+
+    def step():
+        rc = lib.refeff_fms_solve_batch_device(plan.handle, ne, dk.ptr, dph.ptr, dout.ptr, i32p, C.byref(st), None)
+        if rc != 0 and not status.any():
+            check(rc, "solve_batch_device")
+
+    for _ in range(warmup):
+        step()
+    synchronize()
+    if profile:
+        check(lib.refeff_fms_plan_profile(plan.handle, 1), "profile")
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    synchronize()
+    t1 = time.perf_counter()
+    prof = None
+    if profile:
+        from refeff_amd._lib import Profile
+        prof = Profile()
+        check(lib.refeff_fms_plan_profile_read(plan.handle, C.byref(prof)), "profile_read")
+        lib.refeff_fms_plan_profile(plan.handle, 0)
+    out = np.zeros((ne, npot, ch, ch), np.complex128)
+    dout.download(out)
+    sc = out.transpose(0, 1, 3, 2).copy()
+    res = dict(seconds=t1 - t0, ne=ne, status=status.copy(), scattering=sc, stats=st, profile=prof,
+               n=plan.info.state_count, npad=plan.info.padded_order, batch=plan.info.batch)
+    for d in (dk, dph, dout):
+        d.free()
+    plan.close()
+    return res
+
+
+def cpu_baseline(wl, energies, gg_gpu):
+    """FP64 oracle G0/t assembly + LAPACK LU on the host cores for a bounded sample.
+    Times the complex-single LU (refeff's faer path is c32) and uses the FP64
+    solve as the accuracy reference for the GPU result."""
+    import scipy.linalg.lapack as lapack
+    from oracle import oracle as orc
+
+    plan = orc.Plan(wl.positions, wl.potentials, global_lmax=wl.global_lmax, raw_lmax=wl.raw_lmax,
+                    rdirec=wl.rdirec, sigsqr=wl.sigsqr(), lfms=wl.lfms)
+    a_, l, m, s, rep = plan.states()
+    k = wl.wave_numbers()
+    ph = wl.phase_shifts()
+    off = rep[int(wl.potentials[0])]
+    nb = (min(wl.raw_lmax[wl.potentials[0]], wl.global_lmax) + 1) ** 2
+    t_c32 = 0.0
+    errs = []
+    for e in energies:
+        t0 = time.perf_counter()
+        g0 = plan.g0(k[e])
+        t = plan.t_table(ph[e])
+        a = np.eye(len(a_)) - g0 * t[0][None, :]
+        rhs = g0[:, off:off + nb]
+        lu, piv, info = lapack.cgetrf(a.astype(np.complex64))
+        x32, info2 = lapack.cgetrs(lu, piv, rhs.astype(np.complex64))
+        t_c32 += time.perf_counter() - t0
+        lu64, piv64, _ = lapack.zgetrf(a)
+        x64, _ = lapack.zgetrs(lu64, piv64, rhs)
+        ref = x64[off:off + nb, :]
+        got = gg_gpu[e, int(wl.potentials[0]), :nb, :nb]
+        errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+        if info or info2:
+            raise RuntimeError("LAPACK reported a singular matrix")
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": len(energies) / t_c32, "unit": "energy-points/s", "cores": cores, "kind": "port",
+            "sample": f"{len(energies)} of {len(k)} energies; oracle FP64 G0/t assembly (1 thread) + "
+                      f"LAPACK cgetrf/cgetrs complex64 (OpenBLAS, {cores} threads)"}, errs
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one MI355X only; multi-GPU sharding over energies is "
+                                   "described in DESIGN.md and not run", "n_gpus": args.gpus}))
+        sys.exit(2)
+    from refeff_amd import lib, workloads as W
+    L = lib()
+    wl = W.CONFIGS[args.config]()
+    solver_id = 1 if args.solver == "mixed" else 2
+    main_res = run_solver(L, wl, solver_id, args.steps, args.warmup, profile=True)
+    if main_res["status"].any():
+        print(f"warning: non-zero status {np.unique(main_res['status'])}", file=sys.stderr)
+    ne = main_res["ne"]
+    value = ne * args.steps / main_res["seconds"]
+    prof = main_res["profile"]
+    gemm = 4  # REFEFF_FMS_K_GEMM
+    prec = "fp32" if solver_id == 1 else "fp64"
+    g_ms = prof.milliseconds[gemm]
+    achieved = prof.flops[gemm] / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    lu_ms = sum(prof.milliseconds[i] for i in (2, 3, 4))
+    n = main_res["n"]
+    lu_flops = 8.0 / 3.0 * n ** 3 * ne * args.steps
+    kernels = {name: {"ms": round(prof.milliseconds[i], 3), "launches": int(prof.launches[i]),
+                      "tflops": round(prof.flops[i] / (prof.milliseconds[i] * 1e-3) / 1e12, 3)
+                      if prof.milliseconds[i] > 0 else 0.0,
+                      "gbs": round(prof.bytes[i] / (prof.milliseconds[i] * 1e-3) / 1e9, 1)
+                      if prof.milliseconds[i] > 0 else 0.0}
+               for i, name in enumerate(["tmatrix", "build", "panel", "swap_trsm", "gemm", "solve", "residual",
+                                         "other"])}
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "energy-points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(main_res["seconds"] / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 LU + f64 refinement" if solver_id == 1 else "c128",
+        "data": "synthetic (seeded lattice cluster, smooth model phase shifts)",
+        "config": {"workload": wl.name, "atoms": wl.nat, "states": n, "padded": main_res["npad"], "energies": ne,
+                   "solver": args.solver, "batch": main_res["batch"],
+                   "refinement_sweeps": int(main_res["stats"].refinement_sweeps)},
+        "roofline": {"bound": "mfma", "kernel": f"k_gemm_sub<{'float' if prec == 'fp32' else 'double'}> (LU trailing update)",
+                     "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_TFLOPS[prec], 4), "traffic": None,
+                     "lu_tflops": round(lu_flops / (lu_ms * 1e-3) / 1e12, 3) if lu_ms > 0 else 0.0},
+        "kernels": kernels,
+    }
+    if not args.no_fp64_compare and solver_id == 1:
+        r64 = run_solver(L, wl, 2, 1, 0, profile=False)
+        v64 = r64["ne"] / r64["seconds"]
+        d = [float(np.linalg.norm(main_res["scattering"][e] - r64["scattering"][e]) /
+                   np.linalg.norm(r64["scattering"][e])) for e in range(ne)]
+        line["fp64_lu"] = {"value": round(v64, 3), "unit": "energy-points/s",
+                           "speedup_mixed_vs_fp64": round(value / v64, 3),
+                           "max_gg_rel_diff_mixed_vs_fp64": max(d)}
+    if not args.no_cpu:
+        sample = sorted(set(np.linspace(0, ne - 1, args.cpu_energies).astype(int).tolist()))
+        cb, errs = cpu_baseline(wl, sample, main_res["scattering"])
+        line["cpu_baseline"] = cb
+        line["accuracy"] = {"max_gg_rel_err_vs_fp64_oracle": max(errs), "energies_checked": len(sample),
+                            "gate": 1e-8}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,109 @@
+"""ctypes binding of librefeff_fms.so (include/refeff_fms.h).
+
+The library is built in-tree by ``refeff_amd/build.py``.  There is no
+fallback: if the shared object is missing every entry point raises
+``FmsLibraryMissing``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librefeff_fms.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "refeff_fms.h")
+
+STATUS_NAMES = {
+    0: "OK", 1: "INVALID_ARGUMENT", 2: "MISSING_CENTRAL_ATOM", 3: "DUPLICATE_ABSORBER", 4: "NON_FINITE",
+    5: "ZERO_RHO", 6: "POTENTIAL_OUT_OF_RANGE", 7: "MISSING_REPRESENTATIVE", 8: "SINGULAR", 9: "BREAKDOWN",
+    10: "CENTRAL_MISMATCH", 11: "UNSUPPORTED", 12: "NO_MEMORY", 13: "NO_CONVERGENCE", 14: "DEVICE_ERROR",
+    15: "INVALID_SPIN_COUNT", 16: "EMPTY_CLUSTER", 17: "IO_ERROR",
+}
+
+SOLVER_AUTO, SOLVER_MIXED, SOLVER_FP64, SOLVER_GGBI = 0, 1, 2, 3
+
+
+class FmsLibraryMissing(RuntimeError):
+    pass
+
+
+class FmsError(RuntimeError):
+    """Non-zero refeff_fms_status (mirrors refeff's FmsError families)."""
+
+    def __init__(self, code: int, where: str, detail: str = ""):
+        name = STATUS_NAMES.get(code, str(code))
+        super().__init__(f"{where}: {name}" + (f" ({detail})" if detail else ""))
+        self.code = code
+        self.name = name
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [
+        ("lfms", C.c_int32), ("minv", C.c_int32), ("spin_channels", C.c_int32), ("spin_selector", C.c_int32),
+        ("atom_count", C.c_int32), ("positions", C.c_void_p), ("potentials", C.c_void_p),
+        ("max_potential", C.c_int32), ("global_lmax", C.c_int32), ("raw_potential_lmax", C.c_void_p),
+        ("direct_cutoff", C.c_float), ("mean_square_displacements", C.c_void_p),
+        ("convergence_tolerance", C.c_float), ("zero_tolerance", C.c_float),
+        ("full_scattering_requested", C.c_int32), ("solver", C.c_int32), ("max_batch", C.c_int32),
+    ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("state_count", "padded_order", "channel_count", "potential_count",
+                                          "potential_start", "potential_end", "rhs_count", "solver", "batch")]
+
+
+class Stats(C.Structure):
+    _fields_ = [("seconds_total", C.c_double), ("seconds_assemble", C.c_double), ("seconds_factor", C.c_double),
+                ("seconds_solve", C.c_double), ("refinement_sweeps", C.c_int32), ("batches", C.c_int32),
+                ("max_correction", C.c_double)]
+
+
+class Profile(C.Structure):
+    _fields_ = [("milliseconds", C.c_double * 8), ("flops", C.c_double * 8), ("bytes", C.c_double * 8),
+                ("launches", C.c_int64 * 8)]
+
+
+_lib = None
+
+
+def header_functions() -> list[str]:
+    """Function names declared in include/refeff_fms.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(refeff_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FmsLibraryMissing(f"{LIB_PATH} is not built; run `python refeff_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i32p, f32p, f64p = C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_double)
+    L.refeff_fms_sort_atoms_by_radius.argtypes = [i32, f32p, i32p, f64p]
+    L.refeff_fms_yprep_cluster.argtypes = [i32, f32p, i32p, i32, C.c_float, i32, f32p, i32p, i32p, i32p, i32p]
+    L.refeff_fms_sort_representative_atoms.argtypes = [i32, i32, i32, f32p, i32p, i32p]
+    L.refeff_fms_state_kets.argtypes = [i32, i32, i32p, i32, i32p, i32, i32p, i32p, i32p, i32p, i32p, i32p]
+    L.refeff_fms_plan_create.argtypes = [C.POINTER(PlanDesc), C.POINTER(vp)]
+    L.refeff_fms_plan_destroy.argtypes = [vp]
+    L.refeff_fms_plan_destroy.restype = None
+    L.refeff_fms_plan_query.argtypes = [vp, C.POINTER(PlanInfo)]
+    L.refeff_fms_solve_batch.argtypes = [vp, i32, f64p, f64p, f64p, i32p, C.POINTER(Stats)]
+    L.refeff_fms_solve_batch_device.argtypes = [vp, i32, vp, vp, vp, i32p, C.POINTER(Stats), vp]
+    L.refeff_fms_transition_bmat_averaged.argtypes = [i32, i32, i32, i32, i32, i32p, f64p]
+    L.refeff_fms_mkgtr_trace.argtypes = [i32, i32, i32, i32, i32p, f64p, f64p, f64p, f64p]
+    L.refeff_fms_run_in_dir.argtypes = [C.c_char_p, i32, i32p, C.c_char_p, C.c_size_t]
+    L.refeff_mkgtr_run_in_dir.argtypes = [C.c_char_p, i32p, C.c_char_p, C.c_size_t]
+    L.refeff_fms_version.restype = C.c_char_p
+    L.refeff_fms_plan_profile.argtypes = [vp, i32]
+    L.refeff_fms_plan_profile_read.argtypes = [vp, C.POINTER(Profile)]
+    _lib = L
+    return L
+
+
+def check(rc: int, where: str, detail: str = ""):
+    if rc != 0:
+        raise FmsError(rc, where, detail)

@@ -1,0 +1,1581 @@
+// MI355X (gfx950) kernels and the batched energy sweep of the FMS stage.
+//
+// Per energy point the stage solves (I - G0 t) X = G0[:, absorber block] and
+// keeps the absorber rows of X (refeff fms/solvers.rs:366-484).  The device
+// pipeline for a batch of energies is:
+//   k_tmatrix   single-site t per state (FP64)                  fms/t_matrix.rs:10-192
+//   k_build     G0 pair blocks from Rehr-Albers polynomials and rotation
+//               matrices recomputed per atom pair (FP64), written straight
+//               into A = I - G0 t (FP32 and/or FP64) and the RHS block G0[:, rhs]
+//                                                               fms/pairs.rs:242-553,
+//                                                               fms/internals/linear.rs:102-135
+//   LU          right-looking blocked complex LU with partial pivoting: panel
+//               (global, L2 resident), row interchanges + unit-lower TRSM,
+//               trailing update on MFMA (4 real products per complex product)
+//   solve       forward/backward substitution for the RHS block
+//   refinement  FP64 residual r = b - A x, FP32 correction solve (mixed mode)
+//   k_extract   pack the per-potential gg blocks
+// Matrices are row-major, order padded to a multiple of 64 with an identity
+// tail, one matrix per energy ("lda = npad").
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "fms_common.h"
+
+namespace refeff {
+namespace {
+
+constexpr int kPanel = 32;                 // LU block width
+constexpr int kRot = 455;                  // sum_{l<=kMaxL} (2l+1)^2
+constexpr int kGl = (kMaxL + 1) * (kMaxL + 1) * (kMaxL + 1);
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Cx;
+template <> struct Cx<float> { using type = float2; };
+template <> struct Cx<double> { using type = double2; };
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a - b*c
+__device__ __forceinline__ float2 cfms(float2 a, float2 b, float2 c) {
+    return make_float2(a.x - (b.x * c.x - b.y * c.y), a.y - (b.x * c.y + b.y * c.x));
+}
+__device__ __forceinline__ double2 cfms(double2 a, double2 b, double2 c) {
+    return make_double2(a.x - (b.x * c.x - b.y * c.y), a.y - (b.x * c.y + b.y * c.x));
+}
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+    // Smith's algorithm
+    if (fabs(b.x) >= fabs(b.y)) {
+        const double r = b.y / b.x, d = b.x + b.y * r;
+        return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+    }
+    const double r = b.x / b.y, d = b.y + b.x * r;
+    return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+__device__ __forceinline__ float2 cdiv(float2 a, float2 b) {
+    if (fabsf(b.x) >= fabsf(b.y)) {
+        const float r = b.y / b.x, d = b.x + b.y * r;
+        return make_float2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+    }
+    const float r = b.x / b.y, d = b.y + b.x * r;
+    return make_float2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+__device__ __forceinline__ double2 cexp_d(double2 z) {
+    const double m = exp(z.x);
+    double s, c;
+    sincos(z.y, &s, &c);
+    return make_double2(m * c, m * s);
+}
+template <typename T> __device__ __forceinline__ typename Cx<T>::type to_cx(double2 v);
+template <> __device__ __forceinline__ float2 to_cx<float>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+template <> __device__ __forceinline__ double2 to_cx<double>(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double2 to_d(double2 v) { return v; }
+
+__device__ __forceinline__ int isqrt_small(int v) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= v) ++l;
+    return l;
+}
+__device__ __forceinline__ int rot_base(int l) { return l * (4 * l * l - 1) / 3; }  // sum_{l'<l}(2l'+1)^2
+
+// atomic max for non-negative doubles through their bit pattern
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+// ---------------------------------------------------------------------------
+// t matrix (refeff fms/t_matrix.rs:10-192, fms/internals/t_matrix.rs:3-77)
+// ---------------------------------------------------------------------------
+struct TmatArgs {
+    const int* sl;
+    const int* sm;
+    const int* sspin;   // 1-based
+    const int* spot;    // potential of the state's atom
+    const double* so_plus;
+    const double* so_minus;
+    const double2* phases;  // [e][nsp][2L+1][npot]
+    double2* tmat;          // [e][2][N]
+    int* info;              // [e]
+    int n, nsp, ispin, L, npot;
+};
+
+__device__ __forceinline__ double2 tphase(double2 d) {
+    // (e^{2 i d} - 1) / (2 i)
+    const double2 e2 = cexp_d(make_double2(-2.0 * d.y, 2.0 * d.x));
+    return make_double2(0.5 * e2.y, -0.5 * (e2.x - 1.0));
+}
+
+__global__ __launch_bounds__(256) void k_tmatrix(TmatArgs a) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.y;
+    if (col >= a.n) return;
+    const int l = a.sl[col], m = a.sm[col], s = a.sspin[col], p = a.spot[col];
+    const int nl = 2 * a.L + 1;
+    auto ph = [&](int s1, int ls) {
+        return a.phases[((static_cast<size_t>(e) * a.nsp + (s1 - 1)) * nl + (ls + a.L)) * a.npot + p];
+    };
+    auto so = [&](const double* tab, int mm, int ss) { return tab[(static_cast<size_t>(l) * nl + (mm + a.L)) * 2 + (ss - 1)]; };
+    double2 t0, t1 = make_double2(0.0, 0.0);
+    const double2 dm = ph(s, l);
+    bool finite = isfinite(dm.x) && isfinite(dm.y);
+    if (a.nsp == 1 && a.ispin == 0) {
+        t0 = tphase(dm);
+    } else {
+        const int cs = (a.nsp == 1) ? (a.ispin > 0 ? 2 : 1) : s;
+        const double mi = so(a.so_minus, m, cs), pl = so(a.so_plus, m, cs);
+        const double2 dp = ph(s, -l);
+        finite = finite && isfinite(dp.x) && isfinite(dp.y);
+        const double2 tm = tphase(dm), tp = tphase(dp);
+        t0 = make_double2(tm.x * mi * mi + tp.x * pl * pl, tm.y * mi * mi + tp.y * pl * pl);
+        if (a.nsp == 2) {
+            const int proj = m + s;
+            if (proj > -l + 1 && proj < l + 2) {
+                const int m2 = (s == 1) ? m - 1 : m + 1, s2 = (s == 1) ? 2 : 1;
+                const double mf = so(a.so_minus, m, s), ms = so(a.so_minus, m2, s2);
+                const double pf = so(a.so_plus, m, s), ps = so(a.so_plus, m2, s2);
+                const double2 a1 = tphase(ph(s, l)), a2 = tphase(ph(s2, l));
+                const double2 b1 = tphase(ph(s, -l)), b2 = tphase(ph(s2, -l));
+                const double cm = mf * ms, cp = pf * ps;
+                t1 = make_double2(0.5 * (a1.x + a2.x) * cm + 0.5 * (b1.x + b2.x) * cp,
+                                  0.5 * (a1.y + a2.y) * cm + 0.5 * (b1.y + b2.y) * cp);
+            }
+        }
+    }
+    if (!finite || !isfinite(t0.x) || !isfinite(t0.y)) atomicOr(&a.info[e], 1 << 30);
+    a.tmat[(static_cast<size_t>(e) * 2) * a.n + col] = t0;
+    a.tmat[(static_cast<size_t>(e) * 2 + 1) * a.n + col] = t1;
+}
+
+// ---------------------------------------------------------------------------
+// G0 + system assembly (refeff fms/pairs.rs:9-553, fms/rotation.rs:10-117,
+// fms/internals/rotation.rs:3-125, fms/polynomial.rs:8-83,
+// fms/internals/geometry.rs:14-28, fms/internals/linear.rs:102-201)
+// One workgroup per ordered atom pair (a = row atom, b = column atom); the
+// rotation tables are built once and reused for every energy of the batch.
+// ---------------------------------------------------------------------------
+struct BuildArgs {
+    const float* pos;
+    const int* atom_first;
+    const int* atom_lmax;
+    const int* atom_rhs;   // first RHS column of the atom's block, -1 if not a RHS atom
+    const float* sig;      // sigsqr(atom2, atom1) at atom2 + nat*atom1
+    const double* xnlm;    // [mu*(L+1) + l]
+    const double2* tmat;   // [e][2][N]
+    const double2* kv;     // [e][nsp]
+    float2* a32;
+    double2* a64;
+    double2* b64;          // [e][npad][nrhs]
+    size_t sa, sb;
+    int nat, n, lda, nsp, L, nrhs, nbatch;
+    float rdirec2;
+};
+
+// FEFF rotxan small-d recursion into d0[l][n][m] (1-based n, m), restated from
+// refeff fms/internals/rotation.rs:3-71 in double precision.
+__device__ void small_d(int lmax, double beta, double* d0) {
+    constexpr int S = 2 * kMaxL + 2;
+    auto D = [&](int l, int n, int m) -> double& { return d0[(l * S + n) * S + m]; };
+    const int ndm = 2 * lmax + 1;
+    const double xc = cos(0.5 * beta), xs = sin(0.5 * beta), s = sin(beta);
+    D(1, 1, 1) = 1.0;
+    if (lmax < 1) return;
+    D(2, 1, 1) = xc * xc;
+    D(2, 1, 2) = s / sqrt(2.0);
+    D(2, 1, 3) = xs * xs;
+    D(2, 2, 1) = -D(2, 1, 2);
+    D(2, 2, 2) = cos(beta);
+    D(2, 2, 3) = D(2, 1, 2);
+    D(2, 3, 1) = D(2, 1, 3);
+    D(2, 3, 2) = -D(2, 2, 3);
+    D(2, 3, 3) = D(2, 1, 1);
+    for (int l = 3; l <= lmax + 1; ++l) {
+        const int ln = min(2 * l - 1, ndm), lm = min(2 * l - 3, ndm);
+        for (int nn = 1; nn <= ln; ++nn) {
+            for (int mm = 1; mm <= lm; ++mm) {
+                const double t1 = (double)((2 * l - 1 - nn) * (2 * l - 2 - nn));
+                const double t = (double)((2 * l - 1 - mm) * (2 * l - 2 - mm));
+                const double f1 = sqrt(t1 / t);
+                const double f2 = sqrt((double)((2 * l - 1 - nn) * (nn - 1)) / t);
+                const double f3 = sqrt((double)((nn - 2) * (nn - 1)) / t);
+                double v = f1 * xc * xc * D(l - 1, nn, mm);
+                if (nn > 1) v -= f2 * s * D(l - 1, nn - 1, mm);
+                if (nn > 2) v += f3 * xs * xs * D(l - 1, nn - 2, mm);
+                D(l, nn, mm) = v;
+                if (nn > 2 * l - 3) D(l, mm, nn) = (((nn - mm) & 1) ? -1.0 : 1.0) * v;
+            }
+            if (nn > 2 * l - 3) {
+                D(l, 2 * l - 2, 2 * l - 2) = D(l, 2, 2);
+                D(l, 2 * l - 1, 2 * l - 2) = -D(l, 1, 2);
+                D(l, 2 * l - 2, 2 * l - 1) = -D(l, 2, 1);
+                D(l, 2 * l - 1, 2 * l - 1) = D(l, 1, 1);
+            }
+        }
+    }
+}
+
+template <bool W32, bool W64>
+__global__ __launch_bounds__(256) void k_build(BuildArgs g) {
+    constexpr int S = 2 * kMaxL + 2;
+    __shared__ double d0[2][(kMaxL + 2) * S * S];
+    __shared__ double2 rf[kRot], rb[kRot];
+    __shared__ double2 clm[2][(kMaxL + 2) * (kMaxL + 1)];
+    __shared__ double2 gl[2][kGl];
+    __shared__ double2 pref[2];
+    __shared__ double xn[(kMaxL + 1) * (kMaxL + 1)];
+
+    const int b = blockIdx.x, a = blockIdx.y, tid = threadIdx.x;
+    const int L = g.L, nsp = g.nsp;
+    const int la = g.atom_lmax[a], lb = g.atom_lmax[b];
+    const int fa = g.atom_first[a], fb = g.atom_first[b];
+    const int na = (la + 1) * (la + 1) * nsp, nbk = (lb + 1) * (lb + 1) * nsp;
+    const int rhs = g.atom_rhs[b];
+    const int cells = na * nbk;
+
+    // cutoff decided in single precision, as refeff does (pairs.rs:297-303)
+    const float* pa = g.pos + 3 * a;
+    const float* pb = g.pos + 3 * b;
+    const float fx = __fsub_rn(pa[0], pb[0]), fy = __fsub_rn(pa[1], pb[1]), fz = __fsub_rn(pa[2], pb[2]);
+    const float fd2 = __fadd_rn(__fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fy, fy)), __fmul_rn(fz, fz));
+    const bool zero = (a == b) || (fd2 > g.rdirec2);
+
+    if (zero) {
+        for (int e = 0; e < g.nbatch; ++e) {
+            for (int idx = tid; idx < cells; idx += blockDim.x) {
+                const int r = idx / nbk, c = idx - r * nbk;
+                const double v = (a == b && r == c) ? 1.0 : 0.0;
+                const size_t at = static_cast<size_t>(e) * g.sa + static_cast<size_t>(fa + r) * g.lda + fb + c;
+                if (W32) g.a32[at] = make_float2((float)v, 0.0f);
+                if (W64) g.a64[at] = make_double2(v, 0.0);
+                if (rhs >= 0)
+                    g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fa + r) * g.nrhs + rhs + c] =
+                        make_double2(0.0, 0.0);
+            }
+        }
+        return;
+    }
+
+    // pair geometry in double from the single-precision coordinates
+    const double ex = (double)pa[0] - (double)pb[0];
+    const double ey = (double)pa[1] - (double)pb[1];
+    const double ez = (double)pa[2] - (double)pb[2];
+    const double dist = sqrt(ex * ex + ey * ey + ez * ez);
+    // FEFF getang for positions[a] - positions[b] (refeff fms/cluster.rs:201-241)
+    const double tiny = 1.0e-7;
+    double phi;
+    if (fabs(ex) < tiny) phi = (fabs(ey) < tiny) ? 0.0 : (ey > tiny ? M_PI / 2 : -M_PI / 2);
+    else phi = atan2(ey, ex);
+    double beta;
+    if (dist <= tiny) beta = 0.0;
+    else if (ez <= -dist) beta = M_PI;
+    else if (ez < dist) beta = acos(ez / dist);
+    else beta = 0.0;
+
+    for (int i = tid; i < 2 * (kMaxL + 2) * S * S; i += blockDim.x) (&d0[0][0])[i] = 0.0;
+    for (int i = tid; i < (L + 1) * (L + 1); i += blockDim.x) {
+        const int mu = i / (L + 1), l = i - mu * (L + 1);
+        xn[i] = g.xnlm[mu * (L + 1) + l];
+    }
+    __syncthreads();
+    if (tid == 0) small_d(L, beta, d0[0]);          // forward table, +beta
+    if (tid == 64) small_d(L, -beta, d0[1]);        // backward table, -beta (rotation.rs:60-77)
+    __syncthreads();
+    const int nrot = rot_base(L + 1);
+    for (int i = tid; i < nrot; i += blockDim.x) {
+        int ll = 0;
+        while (rot_base(ll + 1) <= i) ++ll;
+        const int w = 2 * ll + 1, o = i - rot_base(ll);
+        const int r = o / w - ll, c = o % w - ll;
+        // drix(r, c, l) = d0(l+1, c+l+1, r+l+1), then the rotxan phase
+        const double vf = d0[0][((ll + 1) * S + (c + ll + 1)) * S + (r + ll + 1)];
+        const double vb = d0[1][((ll + 1) * S + (c + ll + 1)) * S + (r + ll + 1)];
+        double sf, cf, sb, cb;
+        sincos((double)r * (phi - M_PI), &sf, &cf);    // forward: row index phase
+        sincos(-(double)c * (phi - M_PI), &sb, &cb);   // backward: column index phase
+        rf[i] = make_double2(vf * cf, vf * sf);
+        rb[i] = make_double2(vb * cb, vb * sb);
+    }
+    const double sigma = (double)g.sig[b + static_cast<size_t>(g.nat) * a];
+    __syncthreads();
+
+    const int L1 = L + 1;
+    for (int e = 0; e < g.nbatch; ++e) {
+        if (tid < nsp) {
+            // Rehr-Albers polynomials c_l^m(rho), rho = k |R_a - R_b| (xclmz)
+            const int s = tid;
+            const double2 k = g.kv[static_cast<size_t>(e) * nsp + s];
+            const double2 rho = make_double2(k.x * dist, k.y * dist);
+            const double2 z = cdiv(make_double2(0.0, -1.0), rho);  // -i / rho
+            double2* C = clm[s];
+            auto at = [&](int l, int m) -> double2& { return C[l * L1 + m]; };
+            for (int i = 0; i < (L + 2) * L1; ++i) C[i] = make_double2(0.0, 0.0);
+            at(0, 0) = make_double2(1.0, 0.0);
+            if (L >= 1) at(1, 0) = make_double2(1.0 - z.x, -z.y);
+            for (int l = 2; l <= L; ++l) {
+                const double2 f = make_double2((2 * l - 1) * z.x, (2 * l - 1) * z.y);
+                at(l, 0) = csub(at(l - 2, 0), cmul(f, at(l - 1, 0)));
+            }
+            double2 cmm = make_double2(1.0, 0.0);
+            for (int im = 2; im <= L + 1; ++im) {
+                const int m = im - 1;
+                const double2 f = make_double2((2 * m - 1) * z.x, (2 * m - 1) * z.y);
+                cmm = cmul(cmm, f);
+                cmm = make_double2(-cmm.x, -cmm.y);
+                at(m, m) = cmm;
+                if (im <= L) {
+                    const double2 w = make_double2(1.0 - im * z.x, -im * z.y);
+                    const double2 t = cmul(cmm, w);
+                    at(im, m) = make_double2((2 * im - 1) * t.x, (2 * im - 1) * t.y);
+                }
+                for (int l = im + 1; l <= L; ++l) {
+                    const double2 f2 = make_double2((2 * l - 1) * z.x, (2 * l - 1) * z.y);
+                    at(l, m) = csub(at(l - 2, m), cmul(f2, cadd(at(l - 1, m), at(l - 1, m - 1))));
+                }
+            }
+            // prefactor e^{i rho}/rho e^{-sigma^2 k^2} (geometry.rs:14-28)
+            const double2 eir = cexp_d(make_double2(-rho.y, rho.x));
+            const double2 k2 = cmul(k, k);
+            const double2 damp = cexp_d(make_double2(-sigma * k2.x, -sigma * k2.y));
+            pref[s] = cmul(cdiv(eir, rho), damp);
+        }
+        __syncthreads();
+        // z-axis propagator g_mu(l1, l2) with the prefactor folded in (xgllm)
+        for (int i = tid; i < nsp * L1 * L1 * L1; i += blockDim.x) {
+            const int s = i / (L1 * L1 * L1);
+            const int r = i - s * L1 * L1 * L1;
+            const int mu = r / (L1 * L1), l1 = (r / L1) % L1, l2 = r % L1;
+            double2 acc = make_double2(0.0, 0.0);
+            if (mu <= l1 && mu <= l2) {
+                const int numax = min(l1, l2 - mu);
+                const double sg = (mu & 1) ? -1.0 : 1.0;
+                for (int nu = 0; nu <= numax; ++nu) {
+                    const double2 c1 = clm[s][l1 * L1 + nu];
+                    const double2 c2 = clm[s][l2 * L1 + mu + nu];
+                    const double w = (2.0 * l1 + 1.0) / xn[mu * L1 + l1] * sg * xn[mu * L1 + l2];
+                    acc = cadd(acc, make_double2(w * (c1.x * c2.x - c1.y * c2.y), w * (c1.x * c2.y + c1.y * c2.x)));
+                }
+                acc = cmul(acc, pref[s]);
+            }
+            gl[s][r] = acc;
+        }
+        __syncthreads();
+        const double2* tm0 = g.tmat + static_cast<size_t>(e) * 2 * g.n;
+        const double2* tm1 = tm0 + g.n;
+        for (int idx = tid; idx < cells; idx += blockDim.x) {
+            const int r = idx / nbk, c = idx - r * nbk;
+            const int sr = r % nsp, lr = r / nsp, l1 = isqrt_small(lr), m1 = lr - l1 * l1 - l1;
+            auto g0 = [&](int cc) -> double2 {
+                const int sc = cc % nsp, lc = cc / nsp, l2 = isqrt_small(lc), m2 = lc - l2 * l2 - l2;
+                if (sc != sr) return make_double2(0.0, 0.0);
+                double2 acc = make_double2(0.0, 0.0);
+                const int mlim = min(l1, l2);
+                for (int mu = -mlim; mu <= mlim; ++mu) {
+                    const int amu = mu < 0 ? -mu : mu;
+                    const double2 db = rb[rot_base(l1) + (mu + l1) * (2 * l1 + 1) + (m1 + l1)];
+                    const double2 df = rf[rot_base(l2) + (m2 + l2) * (2 * l2 + 1) + (mu + l2)];
+                    acc = cadd(acc, cmul(cmul(db, gl[sr][(amu * L1 + l1) * L1 + l2]), df));
+                }
+                return acc;
+            };
+            const double2 gv = g0(c);
+            const int cs = fb + c;
+            double2 v = cmul(gv, tm0[cs]);
+            v = make_double2(-v.x, -v.y);
+            if (nsp == 2) {
+                const int sc = c % nsp, lc = c / nsp, l2 = isqrt_small(lc), m2 = lc - l2 * l2 - l2;
+                const int proj = m2 + sc + 1;
+                if (proj > -l2 + 1 && proj < l2 + 2) {
+                    const int cp = (sc == 0) ? c - 1 : c + 1;
+                    v = csub(v, cmul(g0(cp), tm1[cs]));
+                }
+            }
+            const size_t at = static_cast<size_t>(e) * g.sa + static_cast<size_t>(fa + r) * g.lda + fb + c;
+            if (W32) g.a32[at] = make_float2((float)v.x, (float)v.y);
+            if (W64) g.a64[at] = v;
+            if (rhs >= 0) g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fa + r) * g.nrhs + rhs + c] = gv;
+        }
+        __syncthreads();
+    }
+}
+
+// identity tail of the padded matrices and zero RHS padding rows
+template <typename T>
+__global__ __launch_bounds__(256) void k_pad(typename Cx<T>::type* A, size_t sa, int lda, int n, double2* B, size_t sb,
+                                             int nrhs) {
+    using T2 = typename Cx<T>::type;
+    const int e = blockIdx.y;
+    const int row = blockIdx.x;
+    T2* Ar = A + e * sa + static_cast<size_t>(row) * lda;
+    if (row < n) {
+        for (int c = n + threadIdx.x; c < lda; c += blockDim.x) Ar[c] = T2{0, 0};
+    } else {
+        for (int c = threadIdx.x; c < lda; c += blockDim.x) Ar[c] = T2{(c == row) ? T(1) : T(0), 0};
+        if (B)
+            for (int c = threadIdx.x; c < nrhs; c += blockDim.x)
+                B[e * sb + static_cast<size_t>(row) * nrhs + c] = make_double2(0.0, 0.0);
+    }
+}
+
+__global__ void k_fill_int(int* p, int v, size_t n) {
+    const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_double(double* p, double v, size_t n) {
+    const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting (pivot = first max of |re|+|im|, as LAPACK icamax)
+// ---------------------------------------------------------------------------
+
+// Panel factorisation of rows [j0, n) x columns [j0, j0+kPanel).
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel(typename Cx<T>::type* Ab, size_t sa, int lda, int* ipivb, int* info,
+                                               int j0, int n) {
+    using T2 = typename Cx<T>::type;
+    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T2* A = Ab + e * sa;
+    int* ipiv = ipivb + static_cast<size_t>(e) * n;
+    __shared__ T2 urow[kPanel];
+    __shared__ T wbest[4];
+    __shared__ int widx[4];
+    __shared__ int piv;
+    for (int j = 0; j < kPanel; ++j) {
+        const int col = j0 + j;
+        T best = -1;
+        int bi = col;
+        for (int i = col + tid; i < n; i += 256) {
+            const T2 v = A[static_cast<size_t>(i) * lda + col];
+            const T m = fabs(v.x) + fabs(v.y);
+            if (m > best) {
+                best = m;
+                bi = i;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ob = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            wbest[wave] = best;
+            widx[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            T bb = wbest[0];
+            int ii = widx[0];
+            for (int w = 1; w < 4; ++w)
+                if (wbest[w] > bb || (wbest[w] == bb && widx[w] < ii)) {
+                    bb = wbest[w];
+                    ii = widx[w];
+                }
+            if (!(bb > 0)) {
+                ii = col;  // exact zero column: no interchange (LAPACK keeps info)
+                if (info[e] == 0) info[e] = col + 1;
+            }
+            piv = ii;
+            ipiv[col] = ii;
+        }
+        __syncthreads();
+        const int p = piv;
+        if (p != col && tid < kPanel) {
+            T2* r0 = A + static_cast<size_t>(col) * lda + j0 + tid;
+            T2* r1 = A + static_cast<size_t>(p) * lda + j0 + tid;
+            const T2 x = *r0;
+            *r0 = *r1;
+            *r1 = x;
+        }
+        __syncthreads();
+        if (tid < kPanel) urow[tid] = A[static_cast<size_t>(col) * lda + j0 + tid];
+        __syncthreads();
+        const T2 d = urow[j];
+        const bool nz = (d.x != 0 || d.y != 0);
+        const T2 inv = nz ? cdiv(T2{1, 0}, d) : T2{0, 0};
+        if (nz)
+            for (int i = col + 1 + tid; i < n; i += 256) {
+                T2* x = A + static_cast<size_t>(i) * lda + col;
+                *x = cmul(*x, inv);
+            }
+        __syncthreads();
+        // rank-1 update of the panel's remaining columns, one column per lane
+        const int cc = tid & 31, rg = tid >> 5;
+        if (cc > j) {
+            const T2 u = urow[cc];
+            for (int i = col + 1 + rg; i < n; i += 8) {
+                T2* row = A + static_cast<size_t>(i) * lda;
+                row[j0 + cc] = cfms(row[j0 + cc], row[col], u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Row interchanges of one panel applied to every column outside it, then the
+// unit-lower triangular solve of the panel's top block for columns to its right.
+template <typename T, int CW>
+__global__ __launch_bounds__(CW) void k_swap_trsm(typename Cx<T>::type* Ab, size_t sa, int lda, const int* ipivb,
+                                                  int j0, int n) {
+    using T2 = typename Cx<T>::type;
+    const int e = blockIdx.y, tid = threadIdx.x;
+    const int c = blockIdx.x * CW + tid;
+    T2* A = Ab + e * sa;
+    const int* ipiv = ipivb + static_cast<size_t>(e) * n;
+    __shared__ T2 l11[kPanel][kPanel + 1];
+    __shared__ T2 xs[kPanel][CW];
+    __shared__ int pv[kPanel];
+    const bool right = (blockIdx.x * CW + CW > j0 + kPanel);
+    if (right)
+        for (int i = tid; i < kPanel * kPanel; i += CW) {
+            const int r = i / kPanel, q = i % kPanel;
+            l11[r][q] = A[static_cast<size_t>(j0 + r) * lda + j0 + q];
+        }
+    if (tid < kPanel) pv[tid] = ipiv[j0 + tid];
+    __syncthreads();
+    if (c >= n || (c >= j0 && c < j0 + kPanel)) return;
+    for (int r = 0; r < kPanel; ++r) xs[r][tid] = A[static_cast<size_t>(j0 + r) * lda + c];
+    for (int j = 0; j < kPanel; ++j) {
+        const int p = pv[j];
+        if (p == j0 + j) continue;
+        if (p < j0 + kPanel) {
+            const T2 t = xs[j][tid];
+            xs[j][tid] = xs[p - j0][tid];
+            xs[p - j0][tid] = t;
+        } else {
+            T2* far = A + static_cast<size_t>(p) * lda + c;
+            const T2 t = *far;
+            *far = xs[j][tid];
+            xs[j][tid] = t;
+        }
+    }
+    if (c >= j0 + kPanel) {
+        for (int r = 1; r < kPanel; ++r) {
+            T2 acc = xs[r][tid];
+            for (int q = 0; q < r; ++q) acc = cfms(acc, l11[r][q], xs[q][tid]);
+            xs[r][tid] = acc;
+        }
+    }
+    for (int r = 0; r < kPanel; ++r) A[static_cast<size_t>(j0 + r) * lda + c] = xs[r][tid];
+}
+
+// C -= A * B on complex matrices, row-major, tile 64x64, K in steps of 32.
+// Four real MFMA products per complex product: re += ar*br - ai*bi,
+// im += ar*bi + ai*br.
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+    using acc = v4f;
+    static __device__ __forceinline__ v4f run(float a, float b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<double> {
+    using acc = v4d;
+    static __device__ __forceinline__ v4d run(double a, double b, v4d c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, const typename Cx<T>::type* Ab_,
+                                                  const typename Cx<T>::type* Bb_, size_t sbatch, int ld, int M,
+                                                  int N, int K) {
+    using T2 = typename Cx<T>::type;
+    using acc_t = typename Mfma<T>::acc;
+    constexpr int BM = 64, BN = 64, BK = (sizeof(T) == 4) ? 32 : 16, PADA = (sizeof(T) == 4) ? 16 : 1;
+    __shared__ T as_re[BK][BM + PADA], as_im[BK][BM + PADA];
+    __shared__ T bs_re[BK][BN + PADA], bs_im[BK][BN + PADA];
+    const int e = blockIdx.y;
+    const int tilesN = (N + BN - 1) / BN;
+    const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+    const T2* A = Ab_ + e * sbatch;
+    const T2* B = Bb_ + e * sbatch;
+    T2* C = Cb + e * sbatch;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    acc_t cr[2][2], ci[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            cr[i][j] = acc_t{0, 0, 0, 0};
+            ci[i][j] = acc_t{0, 0, 0, 0};
+        }
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        {   // A tile: 64 rows x BK k
+            constexpr int KPT = BK / 4;
+            const int r = tid >> 2, kq = (tid & 3) * KPT;
+            const int gr = m0 + r;
+#pragma unroll
+            for (int i = 0; i < KPT; ++i) {
+                T2 v = T2{0, 0};
+                if (gr < M) v = A[static_cast<size_t>(gr) * ld + k0 + kq + i];
+                as_re[kq + i][r] = v.x;
+                as_im[kq + i][r] = v.y;
+            }
+        }
+        {   // B tile: BK k x 64 cols
+            constexpr int CPT = BK * BN / 256;
+            const int kr = tid / (BN / CPT), cq = (tid % (BN / CPT)) * CPT;
+#pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                const int gc = n0 + cq + i;
+                T2 v = T2{0, 0};
+                if (gc < N) v = B[static_cast<size_t>(k0 + kr) * ld + gc];
+                bs_re[kr][cq + i] = v.x;
+                bs_im[kr][cq + i] = v.y;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4), ix = lane & 15;
+            T ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ar[i] = as_re[kr][wr * 32 + i * 16 + ix];
+                ai[i] = as_im[kr][wr * 32 + i * 16 + ix];
+                nai[i] = -ai[i];
+                br[i] = bs_re[kr][wc * 32 + i * 16 + ix];
+                bi[i] = bs_im[kr][wc * 32 + i * 16 + ix];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    cr[i][j] = Mfma<T>::run(ar[i], br[j], cr[i][j]);
+                    cr[i][j] = Mfma<T>::run(nai[i], bi[j], cr[i][j]);
+                    ci[i][j] = Mfma<T>::run(ar[i], bi[j], ci[i][j]);
+                    ci[i][j] = Mfma<T>::run(ai[i], br[j], ci[i][j]);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                // accumulator layout: f32 16x16x4 holds rows 4*(lane/16)+r,
+                // f64 16x16x4 holds rows lane/16 + 4*r (column = lane%16 for both)
+                const int rr = (sizeof(T) == 4) ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r;
+                const int row = m0 + wr * 32 + i * 16 + rr;
+                const int col = n0 + wc * 32 + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    T2* p = C + static_cast<size_t>(row) * ld + col;
+                    T2 v = *p;
+                    v.x -= cr[i][j][r];
+                    v.y -= ci[i][j][r];
+                    *p = v;
+                }
+            }
+}
+
+// final row permutation of the LU: perm[i] = original row now at position i
+__global__ __launch_bounds__(256) void k_perm(const int* ipivb, int* permb, int n) {
+    extern __shared__ int map[];
+    const int e = blockIdx.x;
+    const int* ipiv = ipivb + static_cast<size_t>(e) * n;
+    int* perm = permb + static_cast<size_t>(e) * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) map[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll 8
+        for (int k = 0; k < n; ++k) {
+            const int p = ipiv[k];
+            if (p != k) {
+                const int t = map[k];
+                map[k] = map[p];
+                map[p] = t;
+            }
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = map[i];
+}
+
+// X[e][i][c] = (T) B[e][perm[i]][c]
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, const int* permb, int n, int nrhs,
+                                                typename Cx<T>::type* X) {
+    const int e = blockIdx.y;
+    const size_t total = static_cast<size_t>(n) * nrhs;
+    for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
+        const int i = static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
+        const int src = permb[static_cast<size_t>(e) * n + i];
+        X[e * sb + idx] = to_cx<T>(B[e * sb + static_cast<size_t>(src) * nrhs + c]);
+    }
+}
+
+// In-place forward (unit lower) and backward (upper) substitution of the
+// permuted RHS block X (n x nrhs, row-major) with the packed LU factors.
+// One workgroup per energy.
+template <typename T>
+__global__ __launch_bounds__(256) void k_solve(const typename Cx<T>::type* LUb, size_t sa, int lda,
+                                               typename Cx<T>::type* Xb, size_t sx, int n, int nrhs) {
+    using T2 = typename Cx<T>::type;
+    constexpr int BS = kPanel;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tri = reinterpret_cast<T2*>(smem_raw);          // [BS][BS+1]
+    T2* xb = tri + BS * (BS + 1);                        // [BS][nrhs]
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const T2* LU = LUb + e * sa;
+    T2* X = Xb + e * sx;
+    const int nblk = n / BS;
+    // forward
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int k0 = kb * BS;
+        for (int i = tid; i < BS * BS; i += 256) {
+            const int r = i / BS, q = i % BS;
+            tri[r * (BS + 1) + q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
+        }
+        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0) * nrhs + i];
+        __syncthreads();
+        for (int c = tid; c < nrhs; c += 256)
+            for (int r = 1; r < BS; ++r) {
+                T2 acc = xb[r * nrhs + c];
+                for (int q = 0; q < r; ++q) acc = cfms(acc, tri[r * (BS + 1) + q], xb[q * nrhs + c]);
+                xb[r * nrhs + c] = acc;
+            }
+        __syncthreads();
+        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0) * nrhs + i] = xb[i];
+        const size_t rest = static_cast<size_t>(n - k0 - BS) * nrhs;
+        for (size_t idx = tid; idx < rest; idx += 256) {
+            const int i = k0 + BS + static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
+            const T2* lrow = LU + static_cast<size_t>(i) * lda + k0;
+            T2 acc = X[static_cast<size_t>(i) * nrhs + c];
+#pragma unroll 8
+            for (int q = 0; q < BS; ++q) acc = cfms(acc, lrow[q], xb[q * nrhs + c]);
+            X[static_cast<size_t>(i) * nrhs + c] = acc;
+        }
+        __syncthreads();
+    }
+    // backward
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int k0 = kb * BS;
+        for (int i = tid; i < BS * BS; i += 256) {
+            const int r = i / BS, q = i % BS;
+            tri[r * (BS + 1) + q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
+        }
+        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0) * nrhs + i];
+        __syncthreads();
+        for (int c = tid; c < nrhs; c += 256)
+            for (int r = BS - 1; r >= 0; --r) {
+                T2 acc = xb[r * nrhs + c];
+                for (int q = r + 1; q < BS; ++q) acc = cfms(acc, tri[r * (BS + 1) + q], xb[q * nrhs + c]);
+                xb[r * nrhs + c] = cdiv(acc, tri[r * (BS + 1) + r]);
+            }
+        __syncthreads();
+        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0) * nrhs + i] = xb[i];
+        const size_t rest = static_cast<size_t>(k0) * nrhs;
+        for (size_t idx = tid; idx < rest; idx += 256) {
+            const int i = static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
+            const T2* urow = LU + static_cast<size_t>(i) * lda + k0;
+            T2 acc = X[static_cast<size_t>(i) * nrhs + c];
+#pragma unroll 8
+            for (int q = 0; q < BS; ++q) acc = cfms(acc, urow[q], xb[q * nrhs + c]);
+            X[static_cast<size_t>(i) * nrhs + c] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// R = B - A X in FP64 (64 rows per workgroup); tracks max |R| per energy.
+__global__ __launch_bounds__(256) void k_residual(const double2* Ab, size_t sa, int lda, const double2* Xb,
+                                                  const double2* Bb, double2* Rb, size_t sx, int n, int nrhs,
+                                                  double* norms) {
+    constexpr int RB = 64, KB = 16;
+    extern __shared__ unsigned char smem_raw[];
+    double2* at = reinterpret_cast<double2*>(smem_raw);  // [RB][KB+1]
+    double2* xt = at + RB * (KB + 1);                     // [KB][nrhs]
+    const int e = blockIdx.y, r0 = blockIdx.x * RB, tid = threadIdx.x;
+    const double2* A = Ab + e * sa;
+    const double2* X = Xb + e * sx;
+    const int r = tid >> 2, cg = tid & 3;
+    double2 acc[16];
+    for (int j = 0; j < 16; ++j) acc[j] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < n; k0 += KB) {
+        for (int i = tid; i < RB * KB; i += 256) {
+            const int rr = i / KB, kk = i % KB;
+            at[rr * (KB + 1) + kk] = A[static_cast<size_t>(r0 + rr) * lda + k0 + kk];
+        }
+        for (int i = tid; i < KB * nrhs; i += 256) xt[i] = X[static_cast<size_t>(k0) * nrhs + i];
+        __syncthreads();
+        for (int kk = 0; kk < KB; ++kk) {
+            const double2 av = at[r * (KB + 1) + kk];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int c = cg + 4 * j;
+                if (c < nrhs) {
+                    const double2 xv = xt[kk * nrhs + c];
+                    acc[j].x += av.x * xv.x - av.y * xv.y;
+                    acc[j].y += av.x * xv.y + av.y * xv.x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double mx = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int c = cg + 4 * j;
+        if (c < nrhs) {
+            const size_t o = e * sx + static_cast<size_t>(r0 + r) * nrhs + c;
+            const double2 rv = make_double2(Bb[o].x - acc[j].x, Bb[o].y - acc[j].y);
+            Rb[o] = rv;
+            mx = fmax(mx, fabs(rv.x) + fabs(rv.y));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((tid & 63) == 0) atomic_max_nonneg(&norms[e * 4 + 0], mx);
+}
+
+// X += D; tracks max |D| and max |X| per energy
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(double2* X, const typename Cx<T>::type* D, size_t sx, int n, int nrhs,
+                                                double* norms) {
+    const int e = blockIdx.y;
+    const size_t total = static_cast<size_t>(n) * nrhs;
+    double md = 0.0, mxv = 0.0;
+    for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
+        const double2 d = to_d(D[e * sx + idx]);
+        double2 x = X[e * sx + idx];
+        x.x += d.x;
+        x.y += d.y;
+        X[e * sx + idx] = x;
+        md = fmax(md, fabs(d.x) + fabs(d.y));
+        mxv = fmax(mxv, fabs(x.x) + fabs(x.y));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        md = fmax(md, __shfl_xor(md, off, 64));
+        mxv = fmax(mxv, __shfl_xor(mxv, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_nonneg(&norms[e * 4 + 1], md);
+        atomic_max_nonneg(&norms[e * 4 + 2], mxv);
+    }
+}
+
+// scattering[e][ip][row + ch*col] = X[e][rep[ip] + row][col0[ip] + col]
+struct ExtractArgs {
+    const double2* X;
+    size_t sx;
+    int nrhs, ch, npot, pstart, pend;
+    const int* rep;
+    const int* col0;
+    const int* ipart;
+    double2* out;
+};
+__global__ __launch_bounds__(256) void k_extract(ExtractArgs a) {
+    const int e = blockIdx.y;
+    const size_t per = static_cast<size_t>(a.ch) * a.ch * a.npot;
+    for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < per; idx += 256ull * gridDim.x) {
+        const int ip = static_cast<int>(idx / (static_cast<size_t>(a.ch) * a.ch));
+        const int rem = static_cast<int>(idx % (static_cast<size_t>(a.ch) * a.ch));
+        const int col = rem / a.ch, row = rem % a.ch;
+        double2 v = make_double2(0.0, 0.0);
+        if (ip >= a.pstart && ip <= a.pend && row < a.ipart[ip] && col < a.ipart[ip])
+            v = a.X[e * a.sx + static_cast<size_t>(a.rep[ip] + row) * a.nrhs + a.col0[ip] + col];
+        a.out[e * per + idx] = v;
+    }
+}
+
+// MKGTR trace in FP64 (refeff fms/mkgtr.rs:849-953)
+struct TraceArgs {
+    const double2* gg;   // [e][nch*nch], (row, col) at row + nch*col
+    const double2* rkk;  // [e][8][nsp]
+    const double2* bmat; // (ml2, ms2, k2, ml1, ms1, k1) column-major
+    int lind[8];
+    int ne, nch, nsp, lmax;
+    double2* gtr;
+};
+__global__ __launch_bounds__(64) void k_trace(TraceArgs a) {
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const int nm = 2 * a.lmax + 1;
+    double2 acc = make_double2(0.0, 0.0);
+    // flatten (t1, s1, m1) x (t2, s2, m2) over the 64 lanes
+    const int per = 8 * 2 * nm;
+    for (int idx = tid; idx < per * per; idx += 64) {
+        const int i1 = idx / per, i2 = idx % per;
+        const int t1 = i1 / (2 * nm), s1 = (i1 / nm) % 2, m1 = i1 % nm - a.lmax;
+        const int t2 = i2 / (2 * nm), s2 = (i2 / nm) % 2, m2 = i2 % nm - a.lmax;
+        const int l1 = a.lind[t1], l2 = a.lind[t2];
+        if (l1 < 0 || l2 < 0 || s1 >= a.nsp || s2 >= a.nsp) continue;
+        if (m1 < -l1 || m1 > l1 || m2 < -l2 || m2 > l2) continue;
+        const int row = (l1 * l1 + l1 + m1) * a.nsp + s1, col = (l2 * l2 + l2 + m2) * a.nsp + s2;
+        if (row >= a.nch || col >= a.nch) continue;
+        const double2 g = a.gg[static_cast<size_t>(e) * a.nch * a.nch + row + static_cast<size_t>(a.nch) * col];
+        const double2 bm = a.bmat[(m2 + a.lmax) + static_cast<size_t>(nm) *
+                                  (s2 + 2 * (t2 + 8 * ((m1 + a.lmax) + static_cast<size_t>(nm) * (s1 + 2 * static_cast<size_t>(t1)))))];
+        const double2 r1 = a.rkk[(static_cast<size_t>(e) * 8 + t1) * a.nsp + s1];
+        const double2 r2 = a.rkk[(static_cast<size_t>(e) * 8 + t2) * a.nsp + s2];
+        acc = cadd(acc, cmul(cmul(g, bm), cmul(r1, r2)));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off, 64);
+        acc.y += __shfl_xor(acc.y, off, 64);
+    }
+    if (tid == 0) a.gtr[e] = acc;
+}
+
+}  // namespace
+}  // namespace refeff
+
+// ===========================================================================
+// host side: plan, batching, C ABI
+// ===========================================================================
+using namespace refeff;
+
+struct refeff_fms_plan {
+    int nat = 0, nsp = 1, ispin = 0, lfms = 0, minv = 0, max_pot = 0, L = 0;
+    int n = 0, npad = 0, nrhs = 0, ch = 0, npot = 0, pstart = 0, pend = 0, solver = 1, batch = 0, max_batch = 0;
+    float rdirec = 0.f;
+    std::vector<int> rep, col0, ipart;
+    // device constants
+    float* d_pos = nullptr;
+    float* d_sig = nullptr;
+    int *d_atom_first = nullptr, *d_atom_lmax = nullptr, *d_atom_rhs = nullptr;
+    int *d_sl = nullptr, *d_sm = nullptr, *d_ss = nullptr, *d_spot = nullptr;
+    int *d_rep = nullptr, *d_col0 = nullptr, *d_ipart = nullptr;
+    double *d_xnlm = nullptr, *d_so_plus = nullptr, *d_so_minus = nullptr;
+    // batch workspace
+    int ws_batch = 0;
+    float2* d_a32 = nullptr;
+    double2* d_a64 = nullptr;
+    double2 *d_b64 = nullptr, *d_x64 = nullptr, *d_r64 = nullptr;
+    void* d_w = nullptr;   // solve work block (float2 or double2)
+    int *d_ipiv = nullptr, *d_perm = nullptr, *d_info = nullptr;
+    double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
+    double* d_norms = nullptr;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    // kernel-level instrumentation (refeff_fms_plan_profile)
+    struct Rec {
+        hipEvent_t a, b;
+        int cls;
+        double flops, bytes;
+    };
+    bool profile = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    refeff_fms_profile acc{};
+};
+
+namespace {
+
+template <typename P> void dfree(P*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_workspace(refeff_fms_plan* p) {
+    dfree(p->d_a32); dfree(p->d_a64); dfree(p->d_b64); dfree(p->d_x64); dfree(p->d_r64);
+    if (p->d_w) (void)hipFree(p->d_w);
+    p->d_w = nullptr;
+    dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
+    dfree(p->d_out); dfree(p->d_norms);
+    p->ws_batch = 0;
+}
+
+hipEvent_t next_event(refeff_fms_plan* p) {
+    if (p->pool_used == p->pool.size()) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+        p->pool.push_back(ev);
+    }
+    return p->pool[p->pool_used++];
+}
+
+// fold recorded events into the running totals (requires the stream idle)
+void drain_profile(refeff_fms_plan* p) {
+    for (const auto& r : p->recs) {
+        float ms = 0.f;
+        if (r.a && r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            p->acc.milliseconds[r.cls] += ms;
+            p->acc.flops[r.cls] += r.flops;
+            p->acc.bytes[r.cls] += r.bytes;
+            p->acc.launches[r.cls] += 1;
+        }
+    }
+    p->recs.clear();
+    p->pool_used = 0;
+}
+
+template <typename F>
+void launch(refeff_fms_plan* p, int cls, double flops, double bytes, F&& f) {
+    if (!p->profile) {
+        f();
+        return;
+    }
+    hipEvent_t a = next_event(p), b = next_event(p);
+    if (a) (void)hipEventRecord(a, p->stream);
+    f();
+    if (b) (void)hipEventRecord(b, p->stream);
+    p->recs.push_back({a, b, cls, flops, bytes});
+}
+
+template <typename P> bool dalloc(P** ptr, size_t count) {
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(P)) == hipSuccess;
+}
+
+template <typename P> bool dupload(P** ptr, const P* src, size_t count) {
+    if (!dalloc(ptr, count)) return false;
+    return hipMemcpy(*ptr, src, count * sizeof(P), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+size_t bytes_per_energy(const refeff_fms_plan* p) {
+    const size_t n2 = static_cast<size_t>(p->npad) * p->npad;
+    const size_t nr = static_cast<size_t>(p->npad) * p->nrhs;
+    const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
+    size_t b = n2 * sizeof(double2) + 3 * nr * sizeof(double2) + nr * (mixed ? sizeof(float2) : sizeof(double2));
+    if (mixed) b += n2 * sizeof(float2);
+    b += 2 * p->npad * sizeof(int) + 2 * p->n * sizeof(double2);
+    b += static_cast<size_t>(p->ch) * p->ch * p->npot * sizeof(double2);
+    return b;
+}
+
+int ensure_workspace(refeff_fms_plan* p, int nb) {
+    if (p->ws_batch >= nb) return REFEFF_FMS_OK;
+    free_workspace(p);
+    const size_t n2 = static_cast<size_t>(p->npad) * p->npad;
+    const size_t nr = static_cast<size_t>(p->npad) * p->nrhs;
+    const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
+    bool ok = true;
+    if (mixed) ok = ok && dalloc(&p->d_a32, n2 * nb);
+    ok = ok && dalloc(&p->d_a64, n2 * nb) && dalloc(&p->d_b64, nr * nb) && dalloc(&p->d_x64, nr * nb) &&
+         dalloc(&p->d_r64, nr * nb);
+    ok = ok && hipMalloc(&p->d_w, nr * nb * (mixed ? sizeof(float2) : sizeof(double2))) == hipSuccess;
+    ok = ok && dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
+         dalloc(&p->d_info, nb) && dalloc(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb) &&
+         dalloc(&p->d_k, static_cast<size_t>(p->nsp) * nb) &&
+         dalloc(&p->d_ph, static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot * nb) &&
+         dalloc(&p->d_out, static_cast<size_t>(p->ch) * p->ch * p->npot * nb) && dalloc(&p->d_norms, 4 * static_cast<size_t>(nb));
+    if (!ok) {
+        free_workspace(p);
+        (void)hipGetLastError();
+        return REFEFF_FMS_NO_MEMORY;
+    }
+    p->ws_batch = nb;
+    return REFEFF_FMS_OK;
+}
+
+template <typename T>
+void run_lu(refeff_fms_plan* p, typename Cx<T>::type* A, int nb) {
+    const int n = p->npad;
+    const size_t sa = static_cast<size_t>(n) * n;
+    constexpr int CW = (sizeof(T) == 4) ? 128 : 64;
+    const double cb = sizeof(typename Cx<T>::type);
+    for (int j0 = 0; j0 < n; j0 += kPanel) {
+        const double m = n - j0;
+        launch(p, REFEFF_FMS_K_PANEL, 8.0 * m * kPanel * kPanel / 2 * nb, 2.0 * m * kPanel * cb * nb, [&] {
+            hipLaunchKernelGGL(k_panel<T>, dim3(nb), dim3(256), 0, p->stream, A, sa, n, p->d_ipiv, p->d_info, j0, n);
+        });
+        launch(p, REFEFF_FMS_K_SWAP, 8.0 * kPanel * kPanel / 2 * (n - j0 - kPanel) * nb,
+               2.0 * kPanel * n * cb * nb, [&] {
+                   hipLaunchKernelGGL((k_swap_trsm<T, CW>), dim3((n + CW - 1) / CW, nb), dim3(CW), 0, p->stream, A,
+                                      sa, n, p->d_ipiv, j0, n);
+               });
+        const int rest = n - j0 - kPanel;
+        if (rest > 0) {
+            const int tiles = ((rest + 63) / 64) * ((rest + 63) / 64);
+            typename Cx<T>::type* base = A + static_cast<size_t>(j0 + kPanel) * n + j0 + kPanel;
+            const typename Cx<T>::type* lpan = A + static_cast<size_t>(j0 + kPanel) * n + j0;
+            const typename Cx<T>::type* urow = A + static_cast<size_t>(j0) * n + j0 + kPanel;
+            const double r = rest;
+            launch(p, REFEFF_FMS_K_GEMM, 8.0 * r * r * kPanel * nb, (2.0 * r * kPanel + 2.0 * r * r) * cb * nb, [&] {
+                hipLaunchKernelGGL(k_gemm_sub<T>, dim3(tiles, nb), dim3(256), 0, p->stream, base, lpan, urow, sa, n,
+                                   rest, rest, kPanel);
+            });
+        }
+    }
+    launch(p, REFEFF_FMS_K_OTHER, 0.0, 8.0 * n * nb, [&] {
+        hipLaunchKernelGGL(k_perm, dim3(nb), dim3(256), n * sizeof(int), p->stream, p->d_ipiv, p->d_perm, n);
+    });
+}
+
+template <typename T>
+void run_solve(refeff_fms_plan* p, const typename Cx<T>::type* LU, typename Cx<T>::type* X, int nb) {
+    const int n = p->npad;
+    const size_t shm = (kPanel * (kPanel + 1) + kPanel * p->nrhs) * sizeof(typename Cx<T>::type);
+    const double nn = static_cast<double>(n) * n;
+    launch(p, REFEFF_FMS_K_SOLVE, 8.0 * nn * p->nrhs * nb, nn * sizeof(typename Cx<T>::type) * nb, [&] {
+        hipLaunchKernelGGL(k_solve<T>, dim3(nb), dim3(256), shm, p->stream, LU, static_cast<size_t>(n) * n, n, X,
+                           static_cast<size_t>(n) * p->nrhs, n, p->nrhs);
+    });
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int map_info(int info) {
+    if (info & (1 << 30)) return REFEFF_FMS_NON_FINITE;
+    if (info) return REFEFF_FMS_SINGULAR;
+    return REFEFF_FMS_OK;
+}
+
+// one device batch of nb energies; inputs already in p->d_k / p->d_ph
+int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status, refeff_fms_stats* st) {
+    const int n = p->npad;
+    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * p->nrhs;
+    const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
+    const double t0 = now_s();
+    hipLaunchKernelGGL(k_fill_int, dim3((nb + 255) / 256), dim3(256), 0, p->stream, p->d_info, 0, static_cast<size_t>(nb));
+    TmatArgs ta{p->d_sl, p->d_sm, p->d_ss, p->d_spot, p->d_so_plus, p->d_so_minus, p->d_ph, p->d_tmat, p->d_info,
+                p->n, p->nsp, p->ispin, p->L, p->npot};
+    launch(p, REFEFF_FMS_K_TMATRIX, 0.0, 32.0 * p->n * nb, [&] {
+        hipLaunchKernelGGL(k_tmatrix, dim3((p->n + 255) / 256, nb), dim3(256), 0, p->stream, ta);
+    });
+    BuildArgs ba{p->d_pos, p->d_atom_first, p->d_atom_lmax, p->d_atom_rhs, p->d_sig, p->d_xnlm, p->d_tmat, p->d_k,
+                 p->d_a32, p->d_a64, p->d_b64, sa, sx, p->nat, p->n, n, p->nsp, p->L, p->nrhs, nb, p->rdirec * p->rdirec};
+    const double n2 = static_cast<double>(p->n) * p->n;
+    launch(p, REFEFF_FMS_K_BUILD, 0.0, n2 * (mixed ? 24.0 : 16.0) * nb + 16.0 * p->n * p->nrhs * nb, [&] {
+        if (mixed)
+            hipLaunchKernelGGL((k_build<true, true>), dim3(p->nat, p->nat), dim3(256), 0, p->stream, ba);
+        else
+            hipLaunchKernelGGL((k_build<false, true>), dim3(p->nat, p->nat), dim3(256), 0, p->stream, ba);
+    });
+    if (n > p->n) {
+        hipLaunchKernelGGL(k_pad<double>, dim3(n, nb), dim3(256), 0, p->stream, p->d_a64, sa, n, p->n, p->d_b64, sx,
+                           p->nrhs);
+        if (mixed)
+            hipLaunchKernelGGL(k_pad<float>, dim3(n, nb), dim3(256), 0, p->stream, p->d_a32, sa, n, p->n,
+                               static_cast<double2*>(nullptr), sx, p->nrhs);
+    }
+    (void)hipStreamSynchronize(p->stream);
+    const double t1 = now_s();
+    int sweeps = 0;
+    double worst = 0.0;
+    std::vector<double> norms(4 * static_cast<size_t>(nb));
+    if (mixed) {
+        run_lu<float>(p, p->d_a32, nb);
+        (void)hipStreamSynchronize(p->stream);
+        const double t2 = now_s();
+        float2* w = static_cast<float2*>(p->d_w);
+        const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
+        hipLaunchKernelGGL(k_fill_double, dim3((2 * sx * nb + 255) / 256), dim3(256), 0, p->stream,
+                           reinterpret_cast<double*>(p->d_x64), 0.0, 2 * sx * nb);
+        const int max_sweeps = 10;
+        for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+            hipLaunchKernelGGL(k_fill_double, dim3((4 * nb + 255) / 256), dim3(256), 0, p->stream, p->d_norms, 0.0,
+                               static_cast<size_t>(4 * nb));
+            const double2* rhs = p->d_b64;
+            if (sweep > 0) {
+                const size_t shm = (64 * 17 + 16 * p->nrhs) * sizeof(double2);
+                const double nn = static_cast<double>(n) * n;
+                launch(p, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * p->nrhs * nb, 16.0 * nn * nb, [&] {
+                    hipLaunchKernelGGL(k_residual, dim3(n / 64, nb), dim3(256), shm, p->stream, p->d_a64, sa, n,
+                                       p->d_x64, p->d_b64, p->d_r64, sx, n, p->nrhs, p->d_norms);
+                });
+                rhs = p->d_r64;
+            }
+            hipLaunchKernelGGL(k_gather<float>, dim3(gx, nb), dim3(256), 0, p->stream, rhs, sx, p->d_perm, n, p->nrhs, w);
+            run_solve<float>(p, p->d_a32, w, nb);
+            hipLaunchKernelGGL(k_update<float>, dim3(gx, nb), dim3(256), 0, p->stream, p->d_x64, w, sx, n, p->nrhs,
+                               p->d_norms);
+            (void)hipMemcpyAsync(norms.data(), p->d_norms, norms.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream);
+            (void)hipStreamSynchronize(p->stream);
+            sweeps = sweep + 1;
+            bool done = true;
+            worst = 0.0;
+            for (int e = 0; e < nb; ++e) {
+                const double rel = norms[4 * e + 2] > 0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
+                worst = std::max(worst, rel);
+                if (!(rel <= 1.0e-14)) done = false;
+            }
+            if (done && sweep > 0) break;
+        }
+        if (st) {
+            st->seconds_factor += t2 - t1;
+            st->seconds_solve += now_s() - t2;
+        }
+    } else {
+        run_lu<double>(p, p->d_a64, nb);
+        (void)hipStreamSynchronize(p->stream);
+        const double t2 = now_s();
+        const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
+        hipLaunchKernelGGL(k_gather<double>, dim3(gx, nb), dim3(256), 0, p->stream, p->d_b64, sx, p->d_perm, n, p->nrhs,
+                           p->d_x64);
+        run_solve<double>(p, p->d_a64, p->d_x64, nb);
+        (void)hipStreamSynchronize(p->stream);
+        sweeps = 0;
+        if (st) {
+            st->seconds_factor += t2 - t1;
+            st->seconds_solve += now_s() - t2;
+        }
+    }
+    ExtractArgs ea{p->d_x64, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0, p->d_ipart, out};
+    const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
+    hipLaunchKernelGGL(k_extract, dim3(std::max<size_t>(1, std::min<size_t>(64, (per + 255) / 256)), nb), dim3(256), 0,
+                       p->stream, ea);
+    std::vector<int> info(nb);
+    (void)hipMemcpyAsync(info.data(), p->d_info, nb * sizeof(int), hipMemcpyDeviceToHost, p->stream);
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    if (p->profile) drain_profile(p);
+    int rc = REFEFF_FMS_OK;
+    for (int e = 0; e < nb; ++e) {
+        int s = map_info(info[e]);
+        if (!s && mixed) {
+            const double rel = norms[4 * e + 2] > 0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
+            if (!(rel <= 1.0e-10)) s = REFEFF_FMS_NO_CONVERGENCE;
+        }
+        if (status) status[e] = s;
+        if (s && !rc) rc = s;
+    }
+    if (st) {
+        st->seconds_assemble += t1 - t0;
+        st->refinement_sweeps = std::max(st->refinement_sweeps, sweeps);
+        st->max_correction = std::max(st->max_correction, worst);
+        st->batches += 1;
+    }
+    return rc;
+}
+
+int check_hip(hipError_t e) { return e == hipSuccess ? REFEFF_FMS_OK : REFEFF_FMS_DEVICE_ERROR; }
+
+}  // namespace
+
+extern "C" {
+
+int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out) {
+    if (!d || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (d->spin_channels < 1 || d->spin_channels > 2) return REFEFF_FMS_INVALID_SPIN_COUNT;
+    if (d->atom_count <= 0 || !d->positions || !d->potentials) return REFEFF_FMS_EMPTY_CLUSTER;
+    if (d->global_lmax < 0 || d->global_lmax > kMaxL || d->max_potential < 0 || !d->raw_potential_lmax)
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    if (!std::isfinite(d->direct_cutoff) || d->direct_cutoff < 0.f) return REFEFF_FMS_INVALID_ARGUMENT;
+    if (d->full_scattering_requested) return REFEFF_FMS_UNSUPPORTED;
+    int solver = d->solver;
+    if (solver == REFEFF_FMS_SOLVER_AUTO) solver = (d->minv == 0) ? REFEFF_FMS_SOLVER_MIXED : -1;
+    if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64) return REFEFF_FMS_UNSUPPORTED;
+    for (int i = 0; i < 3 * d->atom_count; ++i)
+        if (!std::isfinite(d->positions[i])) return REFEFF_FMS_NON_FINITE;
+    for (int a = 0; a < d->atom_count; ++a)
+        if (d->potentials[a] < 0 || d->potentials[a] > d->max_potential) return REFEFF_FMS_POTENTIAL_OUT_OF_RANGE;
+    if (d->mean_square_displacements)
+        for (size_t i = 0; i < static_cast<size_t>(d->atom_count) * d->atom_count; ++i)
+            if (!std::isfinite(d->mean_square_displacements[i])) return REFEFF_FMS_NON_FINITE;
+
+    StateTable stt;
+    int rc = build_state_table(d->spin_channels, d->atom_count, d->potentials, d->max_potential,
+                               d->raw_potential_lmax, d->global_lmax, &stt);
+    if (rc) return rc;
+    auto* p = new refeff_fms_plan();
+    p->nat = d->atom_count;
+    p->nsp = d->spin_channels;
+    p->ispin = d->spin_selector;
+    p->lfms = d->lfms;
+    p->minv = d->minv;
+    p->max_pot = d->max_potential;
+    p->L = d->global_lmax;
+    p->n = static_cast<int>(stt.atom.size());
+    p->npad = ((p->n + 63) / 64) * 64;
+    if (p->npad > 16384) {  // row permutation is built in LDS (64 KiB)
+        delete p;
+        return REFEFF_FMS_UNSUPPORTED;
+    }
+    p->ch = (p->L + 1) * (p->L + 1) * p->nsp;
+    p->npot = p->max_pot + 1;
+    p->solver = solver;
+    p->rdirec = d->direct_cutoff;
+    p->max_batch = d->max_batch;
+    const int absorber = d->potentials[0];
+    p->pstart = (d->lfms == 0) ? absorber : 0;
+    p->pend = (d->lfms == 0) ? absorber : p->max_pot;
+    p->rep = stt.representative;
+    p->col0.assign(p->npot, 0);
+    p->ipart.assign(p->npot, 0);
+    std::vector<int> atom_first(p->nat), atom_lmax(p->nat), atom_rhs(p->nat, -1);
+    {
+        int s = 0;
+        for (int a = 0; a < p->nat; ++a) {
+            atom_first[a] = s;
+            atom_lmax[a] = clamp_lipotx(d->raw_potential_lmax[d->potentials[a]], p->L);
+            s += (atom_lmax[a] + 1) * (atom_lmax[a] + 1) * p->nsp;
+        }
+    }
+    int nrhs = 0;
+    for (int ip = p->pstart; ip <= p->pend; ++ip) {
+        if (p->rep[ip] < 0) {
+            delete p;
+            return REFEFF_FMS_MISSING_REPRESENTATIVE;
+        }
+        const int lm = clamp_lipotx(d->raw_potential_lmax[ip], p->L);
+        p->ipart[ip] = (lm + 1) * (lm + 1) * p->nsp;
+        p->col0[ip] = nrhs;
+        // the representative's state block starts at rep[ip]; find its atom
+        for (int a = 0; a < p->nat; ++a)
+            if (atom_first[a] == p->rep[ip]) atom_rhs[a] = nrhs;
+        nrhs += p->ipart[ip];
+    }
+    p->nrhs = nrhs;
+    if (p->nrhs > 64) {
+        delete p;
+        return REFEFF_FMS_UNSUPPORTED;
+    }
+    std::vector<int> spot(p->n);
+    for (int i = 0; i < p->n; ++i) spot[i] = d->potentials[stt.atom[i]];
+    std::vector<double> xn((p->L + 1) * (p->L + 1)), sp, sm;
+    for (int mu = 0; mu <= p->L; ++mu)
+        for (int l = 0; l <= p->L; ++l) xn[mu * (p->L + 1) + l] = legendre_norm(l, mu);
+    spin_orbit_tables(p->L, &sp, &sm);
+    std::vector<float> sig(static_cast<size_t>(p->nat) * p->nat, 0.f);
+    if (d->mean_square_displacements)
+        std::copy(d->mean_square_displacements, d->mean_square_displacements + sig.size(), sig.begin());
+
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
+        delete p;
+        return REFEFF_FMS_DEVICE_ERROR;
+    }
+    bool ok = dupload(&p->d_pos, d->positions, 3 * static_cast<size_t>(p->nat)) &&
+              dupload(&p->d_sig, sig.data(), sig.size()) &&
+              dupload(&p->d_atom_first, atom_first.data(), atom_first.size()) &&
+              dupload(&p->d_atom_lmax, atom_lmax.data(), atom_lmax.size()) &&
+              dupload(&p->d_atom_rhs, atom_rhs.data(), atom_rhs.size()) &&
+              dupload(&p->d_sl, stt.l.data(), stt.l.size()) && dupload(&p->d_sm, stt.m.data(), stt.m.size()) &&
+              dupload(&p->d_ss, stt.spin.data(), stt.spin.size()) && dupload(&p->d_spot, spot.data(), spot.size()) &&
+              dupload(&p->d_rep, p->rep.data(), p->rep.size()) && dupload(&p->d_col0, p->col0.data(), p->col0.size()) &&
+              dupload(&p->d_ipart, p->ipart.data(), p->ipart.size()) && dupload(&p->d_xnlm, xn.data(), xn.size()) &&
+              dupload(&p->d_so_plus, sp.data(), sp.size()) && dupload(&p->d_so_minus, sm.data(), sm.size()) &&
+              hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        refeff_fms_plan_destroy(p);
+        (void)hipGetLastError();
+        return REFEFF_FMS_DEVICE_ERROR;
+    }
+    // batch: bounded by free device memory (70%) and the caller's limit
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const size_t per = bytes_per_energy(p);
+    int nb = static_cast<int>(std::max<size_t>(1, (free_b / 10 * 7) / per));
+    nb = std::min(nb, 256);
+    if (p->max_batch > 0) nb = std::min(nb, p->max_batch);
+    p->batch = nb;
+    *out = p;
+    return REFEFF_FMS_OK;
+}
+
+void refeff_fms_plan_destroy(refeff_fms_plan* p) {
+    if (!p) return;
+    free_workspace(p);
+    dfree(p->d_pos); dfree(p->d_sig); dfree(p->d_atom_first); dfree(p->d_atom_lmax); dfree(p->d_atom_rhs);
+    dfree(p->d_sl); dfree(p->d_sm); dfree(p->d_ss); dfree(p->d_spot); dfree(p->d_rep); dfree(p->d_col0);
+    dfree(p->d_ipart); dfree(p->d_xnlm); dfree(p->d_so_plus); dfree(p->d_so_minus);
+    for (hipEvent_t ev : p->pool) (void)hipEventDestroy(ev);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int refeff_fms_plan_query(const refeff_fms_plan* p, refeff_fms_plan_info* info) {
+    if (!p || !info) return REFEFF_FMS_INVALID_ARGUMENT;
+    info->state_count = p->n;
+    info->padded_order = p->npad;
+    info->channel_count = p->ch;
+    info->potential_count = p->npot;
+    info->potential_start = p->pstart;
+    info->potential_end = p->pend;
+    info->rhs_count = p->nrhs;
+    info->solver = p->solver;
+    info->batch = p->batch;
+    return REFEFF_FMS_OK;
+}
+
+static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc, int32_t* status,
+                        refeff_fms_stats* st, bool device, hipStream_t ext) {
+    if (!p || ne < 0 || (ne > 0 && (!k || !ph || !sc))) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(p->mu);
+    const double t0 = now_s();
+    if (st) std::memset(st, 0, sizeof(*st));
+    const size_t kst = static_cast<size_t>(p->nsp);
+    const size_t pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+    const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
+    int rc = REFEFF_FMS_OK;
+    if (device && ext) (void)hipStreamSynchronize(ext);
+    for (int e0 = 0; e0 < ne; e0 += p->batch) {
+        const int nb = std::min(p->batch, ne - e0);
+        int r = ensure_workspace(p, nb);
+        if (r) return r;
+        const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (hipMemcpyAsync(p->d_k, k + 2 * kst * e0, 2 * kst * nb * sizeof(double), kind, p->stream) != hipSuccess ||
+            hipMemcpyAsync(p->d_ph, ph + 2 * pst * e0, 2 * pst * nb * sizeof(double), kind, p->stream) != hipSuccess)
+            return REFEFF_FMS_DEVICE_ERROR;
+        double2* out = device ? reinterpret_cast<double2*>(sc) + sst * e0 : p->d_out;
+        r = solve_device_batch(p, nb, out, status ? status + e0 : nullptr, st);
+        if (r == REFEFF_FMS_DEVICE_ERROR) return r;
+        if (r && !rc) rc = r;
+        if (!device) {
+            if (check_hip(hipMemcpy(sc + 2 * sst * e0, p->d_out, 2 * sst * nb * sizeof(double), hipMemcpyDeviceToHost)))
+                return REFEFF_FMS_DEVICE_ERROR;
+        }
+    }
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    if (st) st->seconds_total = now_s() - t0;
+    return rc;
+}
+
+int refeff_fms_solve_batch(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc,
+                           int32_t* status, refeff_fms_stats* st) {
+    return solve_common(p, ne, k, ph, sc, status, st, false, nullptr);
+}
+
+int refeff_fms_solve_batch_device(refeff_fms_plan* p, int32_t ne, const double* dk, const double* dph, double* dsc,
+                                  int32_t* status, refeff_fms_stats* st, void* stream) {
+    return solve_common(p, ne, dk, dph, dsc, status, st, true, static_cast<hipStream_t>(stream));
+}
+
+// Kernel-level test hooks (not part of the public header): LU and GEMM of
+// single host matrices through the same device kernels the batched sweep uses.
+int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv, int32_t* info) {
+    if (n <= 0 || n % 64 || !a || !ipiv) return REFEFF_FMS_INVALID_ARGUMENT;
+    refeff_fms_plan p;
+    p.npad = n;
+    p.n = n;
+    if (hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    const size_t nn = static_cast<size_t>(n) * n;
+    int rc = REFEFF_FMS_OK;
+    std::vector<float2> af;
+    if (!dalloc(&p.d_ipiv, n) || !dalloc(&p.d_perm, n) || !dalloc(&p.d_info, 1)) rc = REFEFF_FMS_DEVICE_ERROR;
+    (void)hipMemset(p.d_info, 0, sizeof(int));
+    if (!rc && use_double) {
+        if (!dupload(&p.d_a64, reinterpret_cast<const double2*>(a), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
+        if (!rc) run_lu<double>(&p, p.d_a64, 1);
+        if (!rc && hipMemcpy(a, p.d_a64, nn * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+    } else if (!rc) {
+        af.resize(nn);
+        for (size_t i = 0; i < nn; ++i) af[i] = make_float2((float)a[2 * i], (float)a[2 * i + 1]);
+        if (!dupload(&p.d_a32, af.data(), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
+        if (!rc) run_lu<float>(&p, p.d_a32, 1);
+        if (!rc && hipMemcpy(af.data(), p.d_a32, nn * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+        for (size_t i = 0; !rc && i < nn; ++i) {
+            a[2 * i] = af[i].x;
+            a[2 * i + 1] = af[i].y;
+        }
+    }
+    if (!rc && hipMemcpy(ipiv, p.d_ipiv, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+    if (!rc && info && hipMemcpy(info, p.d_info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+    free_workspace(&p);
+    (void)hipStreamDestroy(p.stream);
+    return rc;
+}
+
+int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, const double* A, const double* B,
+                          double* C) {
+    // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
+    if (M <= 0 || N <= 0 || K <= 0 || K % 32 || !A || !B || !C) return REFEFF_FMS_INVALID_ARGUMENT;
+    const int ld = std::max(std::max(M, N), K);
+    const size_t sz = static_cast<size_t>(ld) * ld;
+    int rc = REFEFF_FMS_OK;
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        using T2 = typename Cx<T>::type;
+        std::vector<T2> ha(sz, T2{0, 0}), hb(sz, T2{0, 0}), hc(sz, T2{0, 0});
+        for (int i = 0; i < M; ++i)
+            for (int k = 0; k < K; ++k) ha[static_cast<size_t>(i) * ld + k] = T2{(T)A[2 * (i * K + k)], (T)A[2 * (i * K + k) + 1]};
+        for (int k = 0; k < K; ++k)
+            for (int j = 0; j < N; ++j) hb[static_cast<size_t>(k) * ld + j] = T2{(T)B[2 * (k * N + j)], (T)B[2 * (k * N + j) + 1]};
+        for (int i = 0; i < M; ++i)
+            for (int j = 0; j < N; ++j) hc[static_cast<size_t>(i) * ld + j] = T2{(T)C[2 * (i * N + j)], (T)C[2 * (i * N + j) + 1]};
+        T2 *da = nullptr, *db = nullptr, *dc = nullptr;
+        if (!dupload(&da, ha.data(), sz) || !dupload(&db, hb.data(), sz) || !dupload(&dc, hc.data(), sz)) {
+            rc = REFEFF_FMS_DEVICE_ERROR;
+        } else {
+            const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+            hipLaunchKernelGGL(k_gemm_sub<T>, dim3(tiles, 1), dim3(256), 0, nullptr, dc, da, db, sz, ld, M, N, K);
+            if (hipMemcpy(hc.data(), dc, sz * sizeof(T2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+            for (int i = 0; i < M; ++i)
+                for (int j = 0; j < N; ++j) {
+                    C[2 * (i * N + j)] = hc[static_cast<size_t>(i) * ld + j].x;
+                    C[2 * (i * N + j) + 1] = hc[static_cast<size_t>(i) * ld + j].y;
+                }
+        }
+        dfree(da);
+        dfree(db);
+        dfree(dc);
+    };
+    if (use_double) run(0.0);
+    else run(0.0f);
+    return rc;
+}
+
+int refeff_fms_plan_profile(refeff_fms_plan* p, int32_t enable) {
+    if (!p) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    p->recs.clear();
+    p->pool_used = 0;
+    p->acc = refeff_fms_profile{};
+    p->profile = enable != 0;
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_profile_read(refeff_fms_plan* p, refeff_fms_profile* out) {
+    if (!p || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    drain_profile(p);
+    *out = p->acc;
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_mkgtr_trace(int32_t ne, int32_t nch, int32_t nsp, int32_t lmax, const int32_t* lind,
+                           const double* bmat, const double* gg, const double* rkk, double* gtr) {
+    if (ne < 0 || nch <= 0 || nsp < 1 || nsp > 2 || lmax < 0 || !lind || !bmat || !gg || !rkk || !gtr)
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    if (ne == 0) return REFEFF_FMS_OK;
+    for (int t = 0; t < 8; ++t) {
+        if (lind[t] > lmax) return REFEFF_FMS_INVALID_ARGUMENT;
+        if (lind[t] >= 0 && ((lind[t] * lind[t] + 2 * lind[t]) * nsp + nsp - 1) >= nch) return REFEFF_FMS_INVALID_ARGUMENT;
+    }
+    const size_t nm = 2 * lmax + 1;
+    const size_t nb = nm * 2 * 8 * nm * 2 * 8;
+    const size_t ngg = static_cast<size_t>(ne) * nch * nch, nr = static_cast<size_t>(ne) * 8 * nsp;
+    for (size_t i = 0; i < 2 * ngg; ++i)
+        if (!std::isfinite(gg[i])) return REFEFF_FMS_NON_FINITE;
+    for (size_t i = 0; i < 2 * nr; ++i)
+        if (!std::isfinite(rkk[i])) return REFEFF_FMS_NON_FINITE;
+    double2 *dgg = nullptr, *drkk = nullptr, *db = nullptr, *dg = nullptr;
+    int rc = REFEFF_FMS_OK;
+    if (!dupload(&dgg, reinterpret_cast<const double2*>(gg), ngg) ||
+        !dupload(&drkk, reinterpret_cast<const double2*>(rkk), nr) ||
+        !dupload(&db, reinterpret_cast<const double2*>(bmat), nb) || !dalloc(&dg, ne)) {
+        rc = REFEFF_FMS_DEVICE_ERROR;
+    } else {
+        TraceArgs ta;
+        ta.gg = dgg;
+        ta.rkk = drkk;
+        ta.bmat = db;
+        for (int t = 0; t < 8; ++t) ta.lind[t] = lind[t];
+        ta.ne = ne;
+        ta.nch = nch;
+        ta.nsp = nsp;
+        ta.lmax = lmax;
+        ta.gtr = dg;
+        hipLaunchKernelGGL(k_trace, dim3(ne), dim3(64), 0, nullptr, ta);
+        if (hipMemcpy(gtr, dg, ne * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+    }
+    dfree(dgg); dfree(drkk); dfree(db); dfree(dg);
+    return rc;
+}
+
+}  // extern "C"

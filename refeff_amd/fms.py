@@ -1,0 +1,209 @@
+"""Python mirror of refeff's real-space FMS interface over librefeff_fms.so.
+
+Names, argument meaning and error behaviour follow refeff's Rust API
+(crates/refeff-core/src/fms/cluster.rs, state.rs, fms/driver.rs); the heavy
+lifting happens in the library's HIP kernels.  Arrays are numpy:
+positions float32 [n, 3], complex values complex128.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import (SOLVER_AUTO, SOLVER_FP64, SOLVER_GGBI, SOLVER_MIXED, FmsError, PlanDesc, PlanInfo, Stats, check,
+                   lib)
+
+__all__ = [
+    "FmsError", "SOLVER_AUTO", "SOLVER_MIXED", "SOLVER_FP64", "SOLVER_GGBI", "sort_atoms_by_radius",
+    "fms_yprep_cluster", "sort_representative_atoms", "construct_state_kets", "FmsRealSpacePlan",
+    "fms_real_space_plan", "fms_real_space_spectrum", "transition_b_matrix_averaged", "mkgtr_green_trace",
+]
+
+
+def _f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def sort_atoms_by_radius(positions, potentials):
+    """FEFF athep (refeff fms/cluster.rs:91-107): returns (positions, potentials, keys)."""
+    pos = np.array(positions, dtype=np.float32).reshape(-1, 3).copy()
+    pot = np.array(potentials, dtype=np.int32).copy()
+    keys = np.zeros(len(pot))
+    check(lib().refeff_fms_sort_atoms_by_radius(len(pot), _f32p(pos), _i32p(pot), _f64p(keys)), "sort_atoms_by_radius")
+    return pos, pot, keys
+
+
+@dataclass
+class FmsYprepCluster:
+    central_atom: int
+    untruncated_count: int
+    positions: np.ndarray
+    potentials: np.ndarray
+
+
+def fms_yprep_cluster(positions, potentials, central_potential: int, cluster_radius: float,
+                      cluster_capacity: int) -> FmsYprepCluster:
+    """FEFF yprep prefix selection (refeff fms/cluster.rs:10-84)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    pot = np.ascontiguousarray(potentials, dtype=np.int32)
+    n = len(pot)
+    if pos.shape[0] != n:
+        raise FmsError(1, "fms_yprep_cluster", "AtomCountMismatch")
+    out_pos = np.zeros((n, 3), np.float32)
+    out_pot = np.zeros(n, np.int32)
+    cnt, cen, untr = C.c_int32(), C.c_int32(), C.c_int32()
+    check(lib().refeff_fms_yprep_cluster(n, _f32p(pos), _i32p(pot), central_potential, cluster_radius,
+                                         cluster_capacity, _f32p(out_pos), _i32p(out_pot), C.byref(cnt),
+                                         C.byref(cen), C.byref(untr)), "fms_yprep_cluster")
+    m = cnt.value
+    return FmsYprepCluster(cen.value, untr.value, out_pos[:m].copy(), out_pot[:m].copy())
+
+
+def sort_representative_atoms(central_potential: int, max_potential: int, positions, potentials):
+    """FEFF sortat (refeff fms/cluster.rs:115-195): returns (positions, potentials, representatives)."""
+    pos = np.array(positions, dtype=np.float32).reshape(-1, 3).copy()
+    pot = np.array(potentials, dtype=np.int32).copy()
+    rep = np.zeros(max_potential + 1, np.int32)
+    check(lib().refeff_fms_sort_representative_atoms(central_potential, max_potential, len(pot), _f32p(pos),
+                                                     _i32p(pot), _i32p(rep)), "sort_representative_atoms")
+    return pos, pot, [int(r) if r >= 0 else None for r in rep]
+
+
+def construct_state_kets(spin_channels: int, atom_potentials, potential_lmax, global_lmax: int):
+    """FEFF getkts (refeff state.rs:97-148): (atom, l, m, spin) arrays and representative offsets."""
+    ap = np.ascontiguousarray(atom_potentials, dtype=np.int32)
+    raw = np.ascontiguousarray(potential_lmax, dtype=np.int32)
+    cnt = C.c_int32(0)
+    z = C.POINTER(C.c_int32)()
+    check(lib().refeff_fms_state_kets(spin_channels, len(ap), _i32p(ap), len(raw) - 1, _i32p(raw), global_lmax,
+                                      C.byref(cnt), z, z, z, z, z), "construct_state_kets")
+    n = cnt.value
+    a, l, m, s = (np.zeros(n, np.int32) for _ in range(4))
+    rep = np.zeros(len(raw), np.int32)
+    check(lib().refeff_fms_state_kets(spin_channels, len(ap), _i32p(ap), len(raw) - 1, _i32p(raw), global_lmax,
+                                      C.byref(cnt), _i32p(a), _i32p(l), _i32p(m), _i32p(s), _i32p(rep)),
+          "construct_state_kets")
+    return a, l, m, s, [int(r) if r >= 0 else None for r in rep]
+
+
+class FmsRealSpacePlan:
+    """Energy-independent FMS inputs resident on the device
+    (refeff FmsRealSpacePlan, fms/types.rs:166-187)."""
+
+    def __init__(self, positions, potentials, *, lfms=0, minv=0, spin_channels=1, spin_selector=0,
+                 max_potential=None, global_lmax=3, raw_potential_lmax=None, direct_cutoff=1.0e9,
+                 mean_square_displacements=None, convergence_tolerance=1e-3, zero_tolerance=1e-3,
+                 full_scattering_matrix_requested=False, solver=SOLVER_AUTO, max_batch=0):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        self.potentials = np.ascontiguousarray(potentials, dtype=np.int32)
+        nat = len(self.potentials)
+        self.max_potential = int(self.potentials.max()) if max_potential is None else int(max_potential)
+        self.raw_potential_lmax = np.ascontiguousarray(
+            raw_potential_lmax if raw_potential_lmax is not None else [global_lmax] * (self.max_potential + 1),
+            dtype=np.int32)
+        self.spin_channels = spin_channels
+        self.global_lmax = global_lmax
+        if mean_square_displacements is None:
+            sig = None
+        else:
+            # sigsqr(atom2, atom1) column-major == row-major [atom1, atom2]
+            sig = np.ascontiguousarray(np.asarray(mean_square_displacements, np.float32).T)
+        self._sig = sig
+        desc = PlanDesc(lfms, minv, spin_channels, spin_selector, nat, self.positions.ctypes.data,
+                        self.potentials.ctypes.data, self.max_potential, global_lmax,
+                        self.raw_potential_lmax.ctypes.data, direct_cutoff,
+                        sig.ctypes.data if sig is not None else None, convergence_tolerance, zero_tolerance,
+                        1 if full_scattering_matrix_requested else 0, solver, max_batch)
+        self._h = C.c_void_p()
+        check(lib().refeff_fms_plan_create(C.byref(desc), C.byref(self._h)), "fms_real_space_plan")
+        info = PlanInfo()
+        check(lib().refeff_fms_plan_query(self._h, C.byref(info)), "plan_query")
+        self.info = info
+        self.last_stats = Stats()
+
+    @property
+    def channel_count(self) -> int:
+        return self.info.channel_count
+
+    @property
+    def potential_count(self) -> int:
+        return self.info.potential_count
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().refeff_fms_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, wave_numbers, phase_shifts):
+        """wave_numbers [ne, nsp] complex, phase_shifts [ne, nsp, 2L+1, npot] complex ->
+        (scattering [ne, npot, ch, ch] with [.., row, col], status [ne])."""
+        nsp, L = self.spin_channels, self.global_lmax
+        k = np.ascontiguousarray(np.asarray(wave_numbers, np.complex128).reshape(-1, nsp))
+        ne = k.shape[0]
+        ph = np.ascontiguousarray(np.asarray(phase_shifts, np.complex128).reshape(ne, nsp, 2 * L + 1,
+                                                                                   self.potential_count))
+        ch, npot = self.channel_count, self.potential_count
+        out = np.zeros((ne, npot, ch, ch), np.complex128)   # C layout: [e][ip][col][row]
+        status = np.zeros(ne, np.int32)
+        st = Stats()
+        rc = lib().refeff_fms_solve_batch(self._h, ne, _f64p(k.view(np.float64)), _f64p(ph.view(np.float64)),
+                                          _f64p(out.view(np.float64)), _i32p(status), C.byref(st))
+        self.last_stats = st
+        if rc not in (0,) and not np.any(status):
+            check(rc, "fms_real_space_spectrum")
+        return out.transpose(0, 1, 3, 2).copy(), status
+
+
+def fms_real_space_plan(**kwargs) -> FmsRealSpacePlan:
+    positions = kwargs.pop("atoms_positions")
+    potentials = kwargs.pop("atoms_potentials")
+    return FmsRealSpacePlan(positions, potentials, **kwargs)
+
+
+def fms_real_space_spectrum(plan: FmsRealSpacePlan, wave_numbers, phase_shifts):
+    """refeff fms_real_space_spectrum (fms/driver.rs:231-236): one result per energy, in order."""
+    return plan.solve(wave_numbers, phase_shifts)
+
+
+def transition_b_matrix_averaged(ihole: int, multipole: int, lmax: int, spin: int, spin_channels: int):
+    nm = 2 * lmax + 1
+    lind = np.zeros(8, np.int32)
+    b = np.zeros(nm * 2 * 8 * nm * 2 * 8, np.complex128)
+    check(lib().refeff_fms_transition_bmat_averaged(ihole, multipole, lmax, spin, spin_channels, _i32p(lind),
+                                                    _f64p(b.view(np.float64))), "transition_b_matrix")
+    return lind, b
+
+
+def mkgtr_green_trace(gg, rkk, lind, bmat_flat, lmax: int, spin_channels: int):
+    """gg [ne, nch, nch] (row, col), rkk [ne, 8, nsp] -> gtr [ne] (refeff fms/mkgtr.rs:790-953)."""
+    gg = np.asarray(gg, np.complex128)
+    ne, nch, _ = gg.shape
+    ggc = np.ascontiguousarray(gg.transpose(0, 2, 1))
+    rk = np.ascontiguousarray(rkk, dtype=np.complex128)
+    b = np.ascontiguousarray(bmat_flat, dtype=np.complex128)
+    li = np.ascontiguousarray(lind, dtype=np.int32)
+    out = np.zeros(ne, np.complex128)
+    check(lib().refeff_fms_mkgtr_trace(ne, nch, spin_channels, lmax, _i32p(li), _f64p(b.view(np.float64)),
+                                       _f64p(ggc.view(np.float64)), _f64p(rk.view(np.float64)),
+                                       _f64p(out.view(np.float64))), "mkgtr_green_trace")
+    return out

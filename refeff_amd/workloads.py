@@ -1,0 +1,164 @@
+"""Seeded synthetic FMS workloads (clusters, energy meshes, phase shifts).
+
+There is no network and no refeff pot/xsph run here, so the phase shifts come
+from a smooth model patterned on refeff's own realistic FMS bench
+(crates/refeff-core/benches/core_tables/fms_realistic.rs:129-153): amplitude
+1.5 (1 + 0.15 ip) e^{-0.6 l}, absorptive part 0.08 e^{-0.4 l}, here made a
+smooth function of the photoelectron wave number.  Geometries are exact
+lattice clusters, cut and ordered by the library's own yprep/sortat.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HARTREE_EV = 27.211396       # refeff crates/refeff-core/src/constants.rs:18
+BOHR_ANGSTROM = 0.529177249  # refeff crates/refeff-io/src/control_input/common.rs:6
+FERMI_ABOVE_REFERENCE_EV = 8.0  # E_F - E_ref used for the synthetic meshes
+
+FCC = [[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]]
+BCC = [[0, 0, 0], [.5, .5, .5]]
+ROCKSALT_CATION = FCC
+ROCKSALT_ANION = [[.5, 0, 0], [0, .5, 0], [0, 0, .5], [.5, .5, .5]]
+
+
+def lattice(basis, a, rmax):
+    n = int(np.ceil(rmax / a)) + 1
+    r = np.arange(-n, n + 1)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    pts = (g[:, None, :] + np.asarray(basis, float)[None]).reshape(-1, 3) * a
+    d = np.linalg.norm(pts, axis=1)
+    return pts[d <= rmax + 1e-6]
+
+
+@dataclass
+class Workload:
+    name: str
+    positions: np.ndarray         # prepared cluster, float32 [nat, 3], absorber first
+    potentials: np.ndarray        # int32 [nat]
+    raw_lmax: np.ndarray          # int32 [npot]
+    global_lmax: int
+    energies_ev: np.ndarray       # complex, E - E_ref in eV
+    sigma2: float = 0.0
+    rdirec: float = 1.0e9
+    lfms: int = 0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def npot(self):
+        return len(self.raw_lmax)
+
+    @property
+    def nat(self):
+        return len(self.potentials)
+
+    def wave_numbers(self) -> np.ndarray:
+        """k = sqrt(2 (E - E_ref)) / a0 (refeff engine fms.rs:4972-4985), [ne, 1]."""
+        e_h = self.energies_ev / HARTREE_EV
+        return (np.sqrt(2.0 * e_h + 0j) / BOHR_ANGSTROM).reshape(-1, 1)
+
+    def phase_shifts(self) -> np.ndarray:
+        """[ne, 1, 2L+1, npot] complex; negative l~ slots left zero (no spin-orbit)."""
+        return synthetic_phases(self.wave_numbers()[:, 0], self.global_lmax, self.raw_lmax)
+
+    def sigsqr(self):
+        if self.sigma2 == 0.0:
+            return None
+        s = np.full((self.nat, self.nat), self.sigma2, np.float32)
+        np.fill_diagonal(s, 0.0)
+        return s
+
+    @property
+    def state_count(self):
+        lm = np.minimum(self.raw_lmax[self.potentials], self.global_lmax)
+        return int(np.sum((lm + 1) ** 2))
+
+
+def synthetic_phases(k, global_lmax, raw_lmax):
+    k = np.asarray(k)
+    ne, npot, L = len(k), len(raw_lmax), global_lmax
+    ph = np.zeros((ne, 1, 2 * L + 1, npot), np.complex128)
+    kr = np.abs(k.real)
+    for ip in range(npot):
+        for l in range(min(L, raw_lmax[ip]) + 1):
+            amp = 1.5 * (1.0 + 0.15 * ip) * np.exp(-0.6 * l) / (1.0 + 0.04 * kr * kr) * (1.0 + 0.3 * np.sin(0.7 * kr + l))
+            absorb = 0.08 * np.exp(-0.4 * l) * (1.0 + 0.1 * kr)
+            ph[:, 0, l + L, ip] = amp + 1j * absorb
+    return ph
+
+
+def _prepare(positions, potentials, radius, max_pot):
+    from .fms import fms_yprep_cluster, sort_representative_atoms
+    cl = fms_yprep_cluster(positions, potentials, 0, radius, len(potentials))
+    pos, pot, _ = sort_representative_atoms(0, max_pot, cl.positions, cl.potentials)
+    return pos, pot
+
+
+def _absorber_first(pts, labels):
+    d = np.linalg.norm(pts, axis=1)
+    i0 = int(np.argmin(d))
+    pot = np.asarray(labels, np.int32).copy()
+    pot[i0] = 0
+    return pts.astype(np.float32), pot
+
+
+def fcc_cu(radius: float, ne: int, im_ev: float = 0.5, name: str | None = None, sigma2: float = 0.0) -> Workload:
+    """Cu K-edge cluster, fcc a=3.615 A, absorber potential 0, Cu potential 1, lmax 3."""
+    pts = lattice(FCC, 3.615, radius)
+    pos, pot = _absorber_first(pts, np.ones(len(pts), np.int32))
+    pos, pot = _prepare(pos, pot, radius, 1)
+    e = FERMI_ABOVE_REFERENCE_EV + np.linspace(-10.0, 40.0, ne) + 1j * im_ev
+    return Workload(name or f"fcc Cu a=3.615 R={radius}", pos, pot, np.array([3, 3], np.int32), 3, e,
+                    sigma2=sigma2, rdirec=2.0 * radius)
+
+
+def bcc_fe(radius: float = 12.0, ne: int = 200) -> Workload:
+    pts = lattice(BCC, 2.87, radius)
+    pos, pot = _absorber_first(pts, np.ones(len(pts), np.int32))
+    pos, pot = _prepare(pos, pot, radius, 1)
+    e = FERMI_ABOVE_REFERENCE_EV + np.linspace(-10.0, 40.0, ne) + 1j * np.linspace(0.2, 1.0, ne)
+    return Workload(f"bcc Fe a=2.87 R={radius}", pos, pot, np.array([3, 3], np.int32), 3, e, rdirec=2.0 * radius)
+
+
+def alloy(radius: float = 10.0, ne: int = 150, seed: int = 0) -> Workload:
+    """Random fcc Cu/Ni/Pd (1/3 each), Gaussian displacement sigma=0.05 A; potentials 1..3."""
+    rng = np.random.default_rng(seed)
+    pts = lattice(FCC, 3.615, radius)
+    pts = pts + rng.normal(scale=0.05, size=pts.shape) * (np.linalg.norm(pts, axis=1) > 1e-6)[:, None]
+    labels = rng.integers(1, 4, size=len(pts)).astype(np.int32)
+    pos, pot = _absorber_first(pts, labels)
+    pos, pot = _prepare(pos, pot, radius, 3)
+    e = FERMI_ABOVE_REFERENCE_EV + np.linspace(-10.0, 40.0, ne) + 1j * np.linspace(0.1, 2.0, ne)
+    return Workload(f"fcc CuNiPd alloy R={radius}", pos, pot, np.array([3, 3, 3, 3], np.int32), 3, e,
+                    rdirec=2.0 * radius)
+
+
+def rocksalt_mno(seed: int, ne: int = 80) -> Workload:
+    """MnO-like rocksalt a=4.44 A, seeded 0-10% vacancies, radius 4-9 A, lmax 2-4 per potential."""
+    rng = np.random.default_rng(1000 + seed)
+    radius = float(rng.uniform(4.0, 9.0))
+    cat = lattice(ROCKSALT_CATION, 4.44, radius)
+    an = lattice(ROCKSALT_ANION, 4.44, radius)
+    pts = np.concatenate([cat, an])
+    labels = np.concatenate([np.ones(len(cat), np.int32), np.full(len(an), 2, np.int32)])
+    keep = np.ones(len(pts), bool)
+    frac = rng.uniform(0.0, 0.10)
+    d = np.linalg.norm(pts, axis=1)
+    cand = np.where(d > 1e-6)[0]
+    drop = rng.choice(cand, size=int(frac * len(cand)), replace=False) if len(cand) else []
+    keep[drop] = False
+    pos, pot = _absorber_first(pts[keep], labels[keep])
+    pos, pot = _prepare(pos, pot, radius, 2)
+    lm = rng.integers(2, 5, size=3).astype(np.int32)
+    L = int(lm.max())
+    e = FERMI_ABOVE_REFERENCE_EV + np.linspace(-10.0, 40.0, ne) + 1j * 0.5
+    return Workload(f"rocksalt MnO seed={seed} R={radius:.2f}", pos, pot, lm, L, e, rdirec=2.0 * radius)
+
+
+CONFIGS = {
+    "cfg1": lambda: fcc_cu(5.2, 60, name="Cu fcc R=5.2 (55 atoms, N=880), 60 energies"),
+    "cfg2": lambda: fcc_cu(8.0, 120, name="Cu fcc R=8.0 (177 atoms, N=2832), 120 energies"),
+    "cfg4": lambda: bcc_fe(12.0, 200),
+    "cfg5": lambda: alloy(10.0, 150),
+}

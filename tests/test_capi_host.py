@@ -1,0 +1,102 @@
+"""CPU-side checks of librefeff_fms.so: it loads, exports every entry point of
+include/refeff_fms.h, and its host-side preparation (athep, yprep, sortat,
+state kets, bcoef) agrees with the oracle and refeff's known answers.  No GPU
+compute is called here."""
+import numpy as np
+import pytest
+
+import refeff_amd
+from refeff_amd import fms
+from oracle import oracle as orc
+
+
+def test_library_exports_every_header_symbol():
+    lib = refeff_amd.lib()
+    names = refeff_amd.header_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_version():
+    assert b"gfx950" in refeff_amd.lib().refeff_fms_version()
+
+
+def test_athep_matches_oracle_and_kat():
+    pos = [[2.0, 0, 0], [0, 0, 0], [-1.0, 0, 0], [1.0, 0, 0], [0, 2.0, 0]]
+    p, pot, keys = fms.sort_atoms_by_radius(pos, [1, 0, 2, 3, 4])
+    assert list(pot) == [0, 2, 3, 1, 4]
+    op, opot, okeys = orc.sort_atoms_by_radius(pos, [1, 0, 2, 3, 4])
+    assert np.array_equal(p, op) and np.array_equal(pot, opot) and np.array_equal(keys, okeys)
+
+
+def test_yprep_cluster_kat():
+    pos = [[2.0, 0, 0], [1.0, 1.0, 0], [1.0, 3.0, 0], [4.0, 1.0, 0], [1.0, 1.0, 1.0]]
+    cl = fms.fms_yprep_cluster(pos, [1, 0, 2, 1, 2], 0, 2.1, 3)
+    assert cl.central_atom == 1 and cl.untruncated_count == 4
+    assert list(cl.potentials) == [0, 2, 1]
+    assert cl.positions.tolist() == [[0, 0, 0], [0, 0, 1.0], [1.0, -1.0, 0]]
+
+
+def test_yprep_errors():
+    with pytest.raises(fms.FmsError) as e:
+        fms.fms_yprep_cluster([[0, 0, 0], [1, 0, 0]], [0, 0], 0, 3.0, 5)
+    assert e.value.name == "DUPLICATE_ABSORBER"
+    with pytest.raises(fms.FmsError) as e:
+        fms.fms_yprep_cluster([[0, 0, 0]], [1], 0, 3.0, 5)
+    assert e.value.name == "MISSING_CENTRAL_ATOM"
+    with pytest.raises(fms.FmsError) as e:
+        fms.fms_yprep_cluster([[0, 0, 0]], [0], 0, -1.0, 5)
+    assert e.value.name == "INVALID_ARGUMENT"
+
+
+def test_sortat_kat():
+    pos = [[float(i), 0, 0] for i in range(6)]
+    p, pot, rep = fms.sort_representative_atoms(0, 3, pos, [0, 2, 1, 3, 2, 1])
+    assert rep == [0, 1, 2, 3]
+    assert list(pot) == [0, 1, 2, 3, 2, 1]
+    assert p[1].tolist() == [2.0, 0, 0] and p[2].tolist() == [1.0, 0, 0]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_cluster_preparation_matches_oracle_on_random_clusters(seed):
+    rng = np.random.default_rng(seed)
+    n = 40
+    pos = rng.normal(scale=3.0, size=(n, 3)).astype(np.float32)
+    pot = rng.integers(1, 4, size=n).astype(np.int32)
+    pot[rng.integers(n)] = 0
+    cl = fms.fms_yprep_cluster(pos, pot, 0, 4.5, n)
+    op, opot, ocen, ountr = orc.yprep_cluster(pos, pot, 0, 4.5, n)
+    assert np.array_equal(cl.positions, op) and np.array_equal(cl.potentials, opot)
+    assert (cl.central_atom, cl.untruncated_count) == (ocen, ountr)
+    mp = int(cl.potentials.max())
+    a = fms.sort_representative_atoms(0, mp, cl.positions, cl.potentials)
+    b = orc.sort_representative_atoms(0, mp, op, opot)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_state_kets_match_oracle():
+    for nsp in (1, 2):
+        args = (nsp, [0, 1, 2, 1, 1], [3, 2, -1], 3)
+        got = fms.construct_state_kets(*args)
+        ref = orc.state_kets(*args)
+        for g, r in zip(got[:4], ref[:4]):
+            assert np.array_equal(g, r)
+        assert got[4] == ref[4]
+
+
+def test_state_kets_kat():
+    """refeff state.rs:158-236: single-spin order (atom, l, m)."""
+    a, l, m, s, rep = fms.construct_state_kets(1, [0, 1], [0, 1], 1)
+    assert list(a) == [0, 1, 1, 1, 1]
+    assert list(l) == [0, 0, 1, 1, 1] and list(m) == [0, 0, -1, 0, 1]
+    assert rep == [0, 1]
+
+
+@pytest.mark.parametrize("ihole,multipole,spin,nsp", [(1, 1, 0, 1), (4, 1, 0, 1), (9, 2, 0, 1), (1, 2, 1, 1),
+                                                      (3, 1, 2, 2)])
+def test_bmat_matches_oracle(ihole, multipole, spin, nsp):
+    lind, b = fms.transition_b_matrix_averaged(ihole, multipole, 3, spin, nsp)
+    olind, ob = orc.bmat_averaged(orc.core_hole_kappa(ihole), multipole, 3, spin, nsp)
+    assert list(lind) == list(olind)
+    assert np.array_equal(b, ob)

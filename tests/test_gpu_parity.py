@@ -1,0 +1,207 @@
+"""GPU parity: the HIP path of librefeff_fms.so against the FP64 oracle.
+
+Bar: relative Frobenius error of every energy's gg block <= 1e-8 against the
+oracle on the same inputs (the north-star gg.bin gate), for both the mixed
+FP32-LU + FP64-refinement solver and the plain FP64 LU.  Larger clusters are
+checked through size-independent properties (mixed vs FP64 agreement, batch
+independence)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from refeff_amd import fms
+from refeff_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+GG_TOL = 1.0e-8
+
+
+def banner(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def random_cluster(seed, nat, spread=2.2, npot=3):
+    rng = np.random.default_rng(seed)
+    pos = [np.zeros(3)]
+    while len(pos) < nat:
+        p = rng.normal(scale=spread, size=3)
+        if min(np.linalg.norm(p - q) for q in pos) > 1.6:
+            pos.append(p)
+    pos = np.array(pos, np.float32)
+    pot = rng.integers(1, npot, size=nat).astype(np.int32)
+    pot[0] = 0
+    for ip in range(1, npot):
+        if ip not in pot[1:]:
+            pot[ip] = ip
+    p2, pot2, _ = fms.sort_representative_atoms(0, npot - 1, *fms.sort_atoms_by_radius(pos, pot)[:2])
+    return p2, pot2
+
+
+def phases_for(k, L, raw, nsp=1, seed=0, spin_orbit=False):
+    ph = W.synthetic_phases(k, L, raw)
+    if nsp == 2 or spin_orbit:
+        rng = np.random.default_rng(seed)
+        full = np.zeros((len(k), nsp, 2 * L + 1, len(raw)), np.complex128)
+        for s in range(nsp):
+            full[:, s] = ph[:, 0] * (1.0 + 0.05 * s)
+        # j = l - 1/2 slots (negative l~) differ slightly from j = l + 1/2
+        for l in range(1, L + 1):
+            full[:, :, L - l, :] = full[:, :, L + l, :] * (0.9 + 0.02 * rng.random())
+        return full
+    return ph
+
+
+def compare(plan_kwargs, positions, potentials, k, ph, solvers=(fms.SOLVER_MIXED, fms.SOLVER_FP64), tol=GG_TOL):
+    nsp = plan_kwargs.get("spin_channels", 1)
+    oplan = orc.Plan(positions, potentials, nsp=nsp, ispin=plan_kwargs.get("spin_selector", 0),
+                     lfms=plan_kwargs.get("lfms", 0), max_pot=plan_kwargs.get("max_potential"),
+                     global_lmax=plan_kwargs["global_lmax"], raw_lmax=plan_kwargs.get("raw_potential_lmax"),
+                     rdirec=plan_kwargs.get("direct_cutoff", 1e9),
+                     sigsqr=plan_kwargs.get("mean_square_displacements"))
+    ref, st = oplan.spectrum(k, ph, threads=8)
+    assert not st.any()
+    worst = {}
+    for solver in solvers:
+        plan = fms.FmsRealSpacePlan(positions, potentials, solver=solver, **plan_kwargs)
+        got, status = plan.solve(k, ph)
+        assert not status.any(), status
+        errs = [rel_err(got[e], ref[e]) for e in range(len(k))]
+        worst[solver] = max(errs)
+        assert max(errs) <= tol, (solver, errs)
+        plan.close()
+    return worst
+
+
+def test_two_atom_closed_form_on_gpu():
+    banner("two-atom lmax=0 closed form")
+    atoms = np.array([[0, 0, 0], [0.3, -1.1, 2.0]], np.float32)
+    d = float(np.linalg.norm(atoms[1].astype(np.float64)))
+    k = np.array([[1.3 + 0.05j], [0.7 + 0.2j]])
+    ph = np.zeros((2, 1, 1, 2), np.complex128)
+    ph[:, 0, 0, 0] = [0.3 + 0.02j, -0.2 + 0.01j]
+    ph[:, 0, 0, 1] = [0.5 + 0.05j, 0.9 + 0.03j]
+    for solver in (fms.SOLVER_MIXED, fms.SOLVER_FP64):
+        plan = fms.FmsRealSpacePlan(atoms, [0, 1], global_lmax=0, direct_cutoff=10.0, solver=solver)
+        sc, status = plan.solve(k, ph)
+        assert not status.any()
+        for e in range(2):
+            g = np.exp(1j * k[e, 0] * d) / (k[e, 0] * d)
+            t0 = (np.exp(2j * ph[e, 0, 0, 0]) - 1) / 2j
+            t1 = (np.exp(2j * ph[e, 0, 0, 1]) - 1) / 2j
+            ref = t1 * g * g / (1 - t0 * t1 * g * g)
+            assert abs(sc[e, 0, 0, 0] - ref) <= 1e-12 * abs(ref)
+
+
+@pytest.mark.parametrize("seed,nat,L,lfms", [(0, 6, 2, 0), (1, 9, 3, 1), (2, 14, 3, 0), (3, 21, 4, 1)])
+def test_random_clusters_match_oracle(seed, nat, L, lfms):
+    banner(f"random cluster seed={seed} nat={nat} L={L} lfms={lfms}")
+    pos, pot = random_cluster(seed, nat)
+    raw = np.array([L, max(L - 1, 0), L], np.int32)
+    kw = dict(global_lmax=L, raw_potential_lmax=raw, max_potential=2, direct_cutoff=9.0, lfms=lfms)
+    k = np.sqrt(2 * (np.array([6.0, 15.0, 33.0]) + 0.5j) / W.HARTREE_EV) / W.BOHR_ANGSTROM
+    k = k.reshape(-1, 1)
+    ph = phases_for(k[:, 0], L, raw)
+    compare(kw, pos, pot, k, ph)
+
+
+def test_debye_damping_and_cutoff_match_oracle():
+    banner("sigma^2 damping + rdirec cutoff")
+    pos, pot = random_cluster(7, 12)
+    nat = len(pot)
+    rng = np.random.default_rng(3)
+    sig = rng.uniform(0.0, 0.01, size=(nat, nat)).astype(np.float32)
+    sig = (sig + sig.T) / 2
+    np.fill_diagonal(sig, 0)
+    kw = dict(global_lmax=3, max_potential=2, direct_cutoff=3.5, mean_square_displacements=sig)
+    k = np.array([[1.2 + 0.05j], [2.4 + 0.1j]])
+    ph = phases_for(k[:, 0], 3, np.array([3, 3, 3]))
+    compare(kw, pos, pot, k, ph)
+
+
+@pytest.mark.parametrize("nsp,ispin", [(2, 0), (1, 1)])
+def test_spin_orbit_branches_match_oracle(nsp, ispin):
+    banner(f"spin-orbit nsp={nsp} ispin={ispin}")
+    pos, pot = random_cluster(11, 8)
+    kw = dict(global_lmax=2, max_potential=2, direct_cutoff=9.0, spin_channels=nsp, spin_selector=ispin)
+    k1 = np.array([1.1 + 0.05j, 1.9 + 0.08j])
+    k = np.stack([k1, k1 * 0.98], axis=1)[:, :nsp]
+    ph = phases_for(k1, 2, np.array([2, 2, 2]), nsp=nsp, seed=2, spin_orbit=True)
+    compare(kw, pos, pot, k, ph)
+
+
+def test_cu_fcc_r52_matches_oracle():
+    """Cu fcc R=5.2 (55 atoms, N=880): four energies of the 60-point mesh."""
+    banner("Cu fcc R=5.2 N=880 vs oracle")
+    wl = W.fcc_cu(5.2, 60)
+    sel = [0, 13, 31, 59]
+    k = wl.wave_numbers()[sel]
+    ph = wl.phase_shifts()[sel]
+    kw = dict(global_lmax=3, raw_potential_lmax=wl.raw_lmax, max_potential=1, direct_cutoff=wl.rdirec)
+    worst = compare(kw, wl.positions, wl.potentials, k, ph)
+    banner(f"worst rel err {worst}")
+
+
+def test_batching_does_not_change_results():
+    banner("batch independence")
+    wl = W.fcc_cu(5.2, 8)
+    k, ph = wl.wave_numbers(), wl.phase_shifts()
+    outs = []
+    for mb in (0, 1, 3):
+        plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, global_lmax=3, raw_potential_lmax=wl.raw_lmax,
+                                    direct_cutoff=wl.rdirec, solver=fms.SOLVER_FP64, max_batch=mb)
+        sc, st = plan.solve(k, ph)
+        assert not st.any()
+        outs.append(sc)
+        plan.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_non_finite_phase_reports_status():
+    banner("non-finite phase status")
+    pos, pot = random_cluster(5, 6)
+    k = np.array([[1.2 + 0.05j], [1.5 + 0.05j]])
+    ph = phases_for(k[:, 0], 2, np.array([2, 2, 2]))
+    ph[1, 0, 2, 0] = np.nan
+    plan = fms.FmsRealSpacePlan(pos, pot, global_lmax=2, max_potential=2, direct_cutoff=9.0)
+    sc, status = plan.solve(k, ph)
+    assert status[0] == 0 and status[1] == 4
+
+
+def test_mkgtr_trace_on_gpu_matches_oracle():
+    banner("MKGTR trace")
+    rng = np.random.default_rng(0)
+    ne, lmax, nsp = 7, 3, 1
+    nch = (lmax + 1) ** 2 * nsp
+    gg = rng.normal(size=(ne, nch, nch)) + 1j * rng.normal(size=(ne, nch, nch))
+    rkk = rng.normal(size=(ne, 8, nsp)) + 1j * rng.normal(size=(ne, 8, nsp))
+    lind, b = fms.transition_b_matrix_averaged(1, 2, lmax, 0, nsp)
+    got = fms.mkgtr_green_trace(gg, rkk, lind, b, lmax, nsp)
+    ref = orc.mkgtr_trace(gg, rkk, lind, b, lmax, nsp)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_cu_fcc_r80_mixed_agrees_with_fp64():
+    """Cu fcc R=8.0 (177 atoms, N=2832) at full size: the two device solvers
+    (FP32 LU + FP64 refinement vs plain FP64 LU) agree to the gg gate."""
+    banner("Cu fcc R=8.0 N=2832 mixed vs fp64 (6 energies)")
+    wl = W.fcc_cu(8.0, 120)
+    sel = [0, 20, 47, 80, 101, 119]
+    k, ph = wl.wave_numbers()[sel], wl.phase_shifts()[sel]
+    res = {}
+    for solver in (fms.SOLVER_MIXED, fms.SOLVER_FP64):
+        plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, global_lmax=3, raw_potential_lmax=wl.raw_lmax,
+                                    direct_cutoff=wl.rdirec, solver=solver)
+        sc, st = plan.solve(k, ph)
+        assert not st.any()
+        res[solver] = sc
+        plan.close()
+    for e in range(len(sel)):
+        assert rel_err(res[fms.SOLVER_MIXED][e], res[fms.SOLVER_FP64][e]) <= GG_TOL
