@@ -32,6 +32,7 @@ namespace refeff {
 namespace {
 
 constexpr int kPanel = 32;                 // LU block width
+constexpr int kRhsChunk = 64;              // right-hand sides per solve/residual workgroup
 constexpr int kRot = 455;                  // sum_{l<=kMaxL} (2l+1)^2
 constexpr int kGl = (kMaxL + 1) * (kMaxL + 1) * (kMaxL + 1);
 
@@ -729,15 +730,17 @@ __global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, con
 // One workgroup per energy.
 template <typename T>
 __global__ __launch_bounds__(256) void k_solve(const typename Cx<T>::type* LUb, size_t sa, int lda,
-                                               typename Cx<T>::type* Xb, size_t sx, int n, int nrhs) {
+                                               typename Cx<T>::type* Xb, size_t sx, int n, int ldx) {
     using T2 = typename Cx<T>::type;
     constexpr int BS = kPanel;
     extern __shared__ unsigned char smem_raw[];
     T2* tri = reinterpret_cast<T2*>(smem_raw);          // [BS][BS+1]
-    T2* xb = tri + BS * (BS + 1);                        // [BS][nrhs]
+    T2* xb = tri + BS * (BS + 1);                        // [BS][nrhs] (this block's column chunk)
     const int e = blockIdx.x, tid = threadIdx.x;
+    const int c0 = blockIdx.y * kRhsChunk;
+    const int nrhs = min(kRhsChunk, ldx - c0);
     const T2* LU = LUb + e * sa;
-    T2* X = Xb + e * sx;
+    T2* X = Xb + e * sx + c0;
     const int nblk = n / BS;
     // forward
     for (int kb = 0; kb < nblk; ++kb) {
@@ -746,7 +749,7 @@ __global__ __launch_bounds__(256) void k_solve(const typename Cx<T>::type* LUb, 
             const int r = i / BS, q = i % BS;
             tri[r * (BS + 1) + q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
         }
-        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0) * nrhs + i];
+        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs];
         __syncthreads();
         for (int c = tid; c < nrhs; c += 256)
             for (int r = 1; r < BS; ++r) {
@@ -755,15 +758,15 @@ __global__ __launch_bounds__(256) void k_solve(const typename Cx<T>::type* LUb, 
                 xb[r * nrhs + c] = acc;
             }
         __syncthreads();
-        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0) * nrhs + i] = xb[i];
+        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs] = xb[i];
         const size_t rest = static_cast<size_t>(n - k0 - BS) * nrhs;
         for (size_t idx = tid; idx < rest; idx += 256) {
             const int i = k0 + BS + static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
             const T2* lrow = LU + static_cast<size_t>(i) * lda + k0;
-            T2 acc = X[static_cast<size_t>(i) * nrhs + c];
+            T2 acc = X[static_cast<size_t>(i) * ldx + c];
 #pragma unroll 8
             for (int q = 0; q < BS; ++q) acc = cfms(acc, lrow[q], xb[q * nrhs + c]);
-            X[static_cast<size_t>(i) * nrhs + c] = acc;
+            X[static_cast<size_t>(i) * ldx + c] = acc;
         }
         __syncthreads();
     }
@@ -774,7 +777,7 @@ __global__ __launch_bounds__(256) void k_solve(const typename Cx<T>::type* LUb, 
             const int r = i / BS, q = i % BS;
             tri[r * (BS + 1) + q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
         }
-        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0) * nrhs + i];
+        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs];
         __syncthreads();
         for (int c = tid; c < nrhs; c += 256)
             for (int r = BS - 1; r >= 0; --r) {
@@ -783,15 +786,15 @@ __global__ __launch_bounds__(256) void k_solve(const typename Cx<T>::type* LUb, 
                 xb[r * nrhs + c] = cdiv(acc, tri[r * (BS + 1) + r]);
             }
         __syncthreads();
-        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0) * nrhs + i] = xb[i];
+        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs] = xb[i];
         const size_t rest = static_cast<size_t>(k0) * nrhs;
         for (size_t idx = tid; idx < rest; idx += 256) {
             const int i = static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
             const T2* urow = LU + static_cast<size_t>(i) * lda + k0;
-            T2 acc = X[static_cast<size_t>(i) * nrhs + c];
+            T2 acc = X[static_cast<size_t>(i) * ldx + c];
 #pragma unroll 8
             for (int q = 0; q < BS; ++q) acc = cfms(acc, urow[q], xb[q * nrhs + c]);
-            X[static_cast<size_t>(i) * nrhs + c] = acc;
+            X[static_cast<size_t>(i) * ldx + c] = acc;
         }
         __syncthreads();
     }
@@ -806,8 +809,11 @@ __global__ __launch_bounds__(256) void k_residual(const double2* Ab, size_t sa, 
     double2* at = reinterpret_cast<double2*>(smem_raw);  // [RB][KB+1]
     double2* xt = at + RB * (KB + 1);                     // [KB][nrhs]
     const int e = blockIdx.y, r0 = blockIdx.x * RB, tid = threadIdx.x;
+    const int c0 = blockIdx.z * kRhsChunk;
+    const int ldx = nrhs;
+    nrhs = min(kRhsChunk, ldx - c0);
     const double2* A = Ab + e * sa;
-    const double2* X = Xb + e * sx;
+    const double2* X = Xb + e * sx + c0;
     const int r = tid >> 2, cg = tid & 3;
     double2 acc[16];
     for (int j = 0; j < 16; ++j) acc[j] = make_double2(0.0, 0.0);
@@ -816,7 +822,7 @@ __global__ __launch_bounds__(256) void k_residual(const double2* Ab, size_t sa, 
             const int rr = i / KB, kk = i % KB;
             at[rr * (KB + 1) + kk] = A[static_cast<size_t>(r0 + rr) * lda + k0 + kk];
         }
-        for (int i = tid; i < KB * nrhs; i += 256) xt[i] = X[static_cast<size_t>(k0) * nrhs + i];
+        for (int i = tid; i < KB * nrhs; i += 256) xt[i] = X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs];
         __syncthreads();
         for (int kk = 0; kk < KB; ++kk) {
             const double2 av = at[r * (KB + 1) + kk];
@@ -837,7 +843,7 @@ __global__ __launch_bounds__(256) void k_residual(const double2* Ab, size_t sa, 
     for (int j = 0; j < 16; ++j) {
         const int c = cg + 4 * j;
         if (c < nrhs) {
-            const size_t o = e * sx + static_cast<size_t>(r0 + r) * nrhs + c;
+            const size_t o = e * sx + static_cast<size_t>(r0 + r) * ldx + c0 + c;
             const double2 rv = make_double2(Bb[o].x - acc[j].x, Bb[o].y - acc[j].y);
             Rb[o] = rv;
             mx = fmax(mx, fabs(rv.x) + fabs(rv.y));
@@ -1115,11 +1121,11 @@ void run_lu(refeff_fms_plan* p, typename Cx<T>::type* A, int nb) {
 template <typename T>
 void run_solve(refeff_fms_plan* p, const typename Cx<T>::type* LU, typename Cx<T>::type* X, int nb) {
     const int n = p->npad;
-    const size_t shm = (kPanel * (kPanel + 1) + kPanel * p->nrhs) * sizeof(typename Cx<T>::type);
+    const size_t shm = (kPanel * (kPanel + 1) + kPanel * std::min(p->nrhs, kRhsChunk)) * sizeof(typename Cx<T>::type);
     const double nn = static_cast<double>(n) * n;
     launch(p, REFEFF_FMS_K_SOLVE, 8.0 * nn * p->nrhs * nb, nn * sizeof(typename Cx<T>::type) * nb, [&] {
-        hipLaunchKernelGGL(k_solve<T>, dim3(nb), dim3(256), shm, p->stream, LU, static_cast<size_t>(n) * n, n, X,
-                           static_cast<size_t>(n) * p->nrhs, n, p->nrhs);
+        hipLaunchKernelGGL(k_solve<T>, dim3(nb, (p->nrhs + kRhsChunk - 1) / kRhsChunk), dim3(256), shm, p->stream,
+                           LU, static_cast<size_t>(n) * n, n, X, static_cast<size_t>(n) * p->nrhs, n, p->nrhs);
     });
 }
 
@@ -1180,10 +1186,11 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                                static_cast<size_t>(4 * nb));
             const double2* rhs = p->d_b64;
             if (sweep > 0) {
-                const size_t shm = (64 * 17 + 16 * p->nrhs) * sizeof(double2);
+                const size_t shm = (64 * 17 + 16 * std::min(p->nrhs, kRhsChunk)) * sizeof(double2);
                 const double nn = static_cast<double>(n) * n;
                 launch(p, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * p->nrhs * nb, 16.0 * nn * nb, [&] {
-                    hipLaunchKernelGGL(k_residual, dim3(n / 64, nb), dim3(256), shm, p->stream, p->d_a64, sa, n,
+                    hipLaunchKernelGGL(k_residual, dim3(n / 64, nb, (p->nrhs + kRhsChunk - 1) / kRhsChunk), dim3(256),
+                                       shm, p->stream, p->d_a64, sa, n,
                                        p->d_x64, p->d_b64, p->d_r64, sx, n, p->nrhs, p->d_norms);
                 });
                 rhs = p->d_r64;
@@ -1329,10 +1336,6 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         nrhs += p->ipart[ip];
     }
     p->nrhs = nrhs;
-    if (p->nrhs > 64) {
-        delete p;
-        return REFEFF_FMS_UNSUPPORTED;
-    }
     std::vector<int> spot(p->n);
     for (int i = 0; i < p->n; ++i) spot[i] = d->potentials[stt.atom[i]];
     std::vector<double> xn((p->L + 1) * (p->L + 1)), sp, sm;
@@ -1456,16 +1459,19 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
     int rc = REFEFF_FMS_OK;
     std::vector<float2> af;
     if (!dalloc(&p.d_ipiv, n) || !dalloc(&p.d_perm, n) || !dalloc(&p.d_info, 1)) rc = REFEFF_FMS_DEVICE_ERROR;
-    (void)hipMemset(p.d_info, 0, sizeof(int));
+    if (!rc && hipMemset(p.d_info, 0, sizeof(int)) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
     if (!rc && use_double) {
         if (!dupload(&p.d_a64, reinterpret_cast<const double2*>(a), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc) run_lu<double>(&p, p.d_a64, 1);
+        if (!rc && hipStreamSynchronize(p.stream) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc && hipMemcpy(a, p.d_a64, nn * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
     } else if (!rc) {
         af.resize(nn);
         for (size_t i = 0; i < nn; ++i) af[i] = make_float2((float)a[2 * i], (float)a[2 * i + 1]);
         if (!dupload(&p.d_a32, af.data(), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc) run_lu<float>(&p, p.d_a32, 1);
+        if (!rc && hipStreamSynchronize(p.stream) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc && hipMemcpy(af.data(), p.d_a32, nn * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         for (size_t i = 0; !rc && i < nn; ++i) {
             a[2 * i] = af[i].x;
