@@ -61,8 +61,8 @@ class Stats(C.Structure):
 
 
 class Profile(C.Structure):
-    _fields_ = [("milliseconds", C.c_double * 8), ("flops", C.c_double * 8), ("bytes", C.c_double * 8),
-                ("launches", C.c_int64 * 8)]
+    _fields_ = [("milliseconds", C.c_double * 9), ("flops", C.c_double * 9), ("bytes", C.c_double * 9),
+                ("launches", C.c_int64 * 9)]
 
 
 _lib = None

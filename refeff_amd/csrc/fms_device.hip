@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -446,60 +447,78 @@ __global__ void k_fill_double(double* p, double v, size_t n) {
 // LU with partial pivoting (pivot = first max of |re|+|im|, as LAPACK icamax)
 // ---------------------------------------------------------------------------
 
-// Panel factorisation of rows [j0, n) x columns [j0, j0+kPanel).
+// Block-wide first-maximum reduction of (value, row) pairs; returns the winning
+// row and value to every thread.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel(typename Cx<T>::type* Ab, size_t sa, int lda, int* ipivb, int* info,
-                                               int j0, int n) {
+__device__ __forceinline__ void block_argmax(T best, int bi, T* wbest, int* widx, int* out_idx, T* out_best) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) {
+        const T ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+        }
+    }
+    if (lane == 0) {
+        wbest[wave] = best;
+        widx[wave] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T bb = wbest[0];
+        int ii = widx[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            if (wbest[w] > bb || (wbest[w] == bb && widx[w] < ii)) {
+                bb = wbest[w];
+                ii = widx[w];
+            }
+        *out_idx = ii;
+        *out_best = bb;
+    }
+    __syncthreads();
+}
+
+// Panel factorisation of rows [j0, n) x columns [j0, j0+kPanel) with partial
+// pivoting.  1024 threads: lane (tid & 31) owns one panel column, the 32 row
+// groups stride over the rows.  Per column one pass scales the multipliers,
+// applies the rank-1 update and searches the next column's pivot; every lane
+// keeps four rows in flight.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_panel(typename Cx<T>::type* Ab, size_t sa, int lda, int* ipivb, int* info,
+                                                int j0, int n) {
     using T2 = typename Cx<T>::type;
-    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int RG = 1024 / kPanel;
+    const int e = blockIdx.x, tid = threadIdx.x, cc = tid & (kPanel - 1), rg = tid / kPanel;
     T2* A = Ab + e * sa;
     int* ipiv = ipivb + static_cast<size_t>(e) * n;
     __shared__ T2 urow[kPanel];
-    __shared__ T wbest[4];
-    __shared__ int widx[4];
+    __shared__ T wbest[16];
+    __shared__ int widx[16];
     __shared__ int piv;
-    for (int j = 0; j < kPanel; ++j) {
-        const int col = j0 + j;
+    __shared__ T pbest;
+    {
         T best = -1;
-        int bi = col;
-        for (int i = col + tid; i < n; i += 256) {
-            const T2 v = A[static_cast<size_t>(i) * lda + col];
+        int bi = j0;
+        for (int i = j0 + tid; i < n; i += 1024) {
+            const T2 v = A[static_cast<size_t>(i) * lda + j0];
             const T m = fabs(v.x) + fabs(v.y);
             if (m > best) {
                 best = m;
                 bi = i;
             }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ob = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ob > best || (ob == best && oi < bi)) {
-                best = ob;
-                bi = oi;
-            }
-        }
-        if (lane == 0) {
-            wbest[wave] = best;
-            widx[wave] = bi;
-        }
-        __syncthreads();
+        block_argmax(best, bi, wbest, widx, &piv, &pbest);
+    }
+    for (int j = 0; j < kPanel; ++j) {
+        const int col = j0 + j;
+        int p = piv;
+        const bool singular = !(pbest > 0);
+        if (singular) p = col;  // exact zero column: no interchange, LAPACK-style info
         if (tid == 0) {
-            T bb = wbest[0];
-            int ii = widx[0];
-            for (int w = 1; w < 4; ++w)
-                if (wbest[w] > bb || (wbest[w] == bb && widx[w] < ii)) {
-                    bb = wbest[w];
-                    ii = widx[w];
-                }
-            if (!(bb > 0)) {
-                ii = col;  // exact zero column: no interchange (LAPACK keeps info)
-                if (info[e] == 0) info[e] = col + 1;
-            }
-            piv = ii;
-            ipiv[col] = ii;
+            ipiv[col] = p;
+            if (singular && info[e] == 0) info[e] = col + 1;
         }
-        __syncthreads();
-        const int p = piv;
         if (p != col && tid < kPanel) {
             T2* r0 = A + static_cast<size_t>(col) * lda + j0 + tid;
             T2* r1 = A + static_cast<size_t>(p) * lda + j0 + tid;
@@ -513,63 +532,261 @@ __global__ __launch_bounds__(256) void k_panel(typename Cx<T>::type* Ab, size_t 
         const T2 d = urow[j];
         const bool nz = (d.x != 0 || d.y != 0);
         const T2 inv = nz ? cdiv(T2{1, 0}, d) : T2{0, 0};
-        if (nz)
-            for (int i = col + 1 + tid; i < n; i += 256) {
-                T2* x = A + static_cast<size_t>(i) * lda + col;
-                *x = cmul(*x, inv);
-            }
-        __syncthreads();
-        // rank-1 update of the panel's remaining columns, one column per lane
-        const int cc = tid & 31, rg = tid >> 5;
-        if (cc > j) {
+        T best = -1;
+        int bi = col + 1;
+        if (cc >= j) {
             const T2 u = urow[cc];
-            for (int i = col + 1 + rg; i < n; i += 8) {
-                T2* row = A + static_cast<size_t>(i) * lda;
-                row[j0 + cc] = cfms(row[j0 + cc], row[col], u);
+            for (int i0 = col + 1 + rg; i0 < n; i0 += 4 * RG) {
+                T2 a[4], x[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + q * RG;
+                    if (i < n) {
+                        const T2* row = A + static_cast<size_t>(i) * lda + j0;
+                        a[q] = row[j];
+                        x[q] = row[cc];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + q * RG;
+                    if (i < n) {
+                        T2* row = A + static_cast<size_t>(i) * lda + j0;
+                        const T2 l = nz ? cmul(a[q], inv) : a[q];
+                        if (cc == j) {
+                            row[j] = l;
+                        } else {
+                            const T2 v = cfms(x[q], l, u);
+                            row[cc] = v;
+                            if (cc == j + 1) {
+                                const T m = fabs(v.x) + fabs(v.y);
+                                if (m > best) {
+                                    best = m;
+                                    bi = i;
+                                }
+                            }
+                        }
+                    }
+                }
             }
         }
-        __syncthreads();
+        if (j + 1 < kPanel) block_argmax(best, bi, wbest, widx, &piv, &pbest);
     }
 }
 
-// Row interchanges of one panel applied to every column outside it, then the
-// unit-lower triangular solve of the panel's top block for columns to its right.
+// Register-blocked panel factorisation of rows [j0, n) x columns [j0, j0+kPanel)
+// with partial pivoting (same pivots and info as LAPACK's unblocked getf2
+// order: first maximum of |re|+|im|).  Thread t owns rows j0 + t + 1024 q,
+// q < Q (host guarantees n - j0 <= 1024 Q).  The panel is split into
+// sub-panels of W columns held in registers: a column step is a block argmax
+// plus one exchange of the pivot row through LDS, with no memory traffic.
+// After each sub-panel its interchanges and unit-lower solve are applied to
+// the panel columns right of it and one rank-W pass updates the rows below.
+template <typename T, int W, int Q>
+__global__ __launch_bounds__(1024) void k_panel_rb(typename Cx<T>::type* Ab, size_t sa, int lda, int* ipivb,
+                                                   int* info, int j0, int n) {
+    using T2 = typename Cx<T>::type;
+    static_assert(kPanel % W == 0, "sub-panel width must divide the panel");
+    const int e = blockIdx.x, tid = threadIdx.x;
+    T2* A = Ab + e * sa;
+    int* ipiv = ipivb + static_cast<size_t>(e) * n;
+    __shared__ T2 prow[W], crow[W];
+    __shared__ T2 u12[W][kPanel];
+    __shared__ T wbest[16];
+    __shared__ int widx[16];
+    __shared__ int pv[W];
+    __shared__ int piv;
+    __shared__ T pbest;
+    const int jend = j0 + kPanel;
+    T2 P[Q][W];
+    for (int c0 = j0; c0 < jend; c0 += W) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = j0 + tid + 1024 * q;
+            if (r < n) {
+                const T2* src = A + static_cast<size_t>(r) * lda + c0;
+#pragma unroll
+                for (int c = 0; c < W; ++c) P[q][c] = src[c];
+            }
+        }
+        {
+            T best = -1;
+            int bi = c0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int r = j0 + tid + 1024 * q;
+                if (r >= c0 && r < n) {
+                    const T m = fabs(P[q][0].x) + fabs(P[q][0].y);
+                    if (m > best) {
+                        best = m;
+                        bi = r;
+                    }
+                }
+            }
+            block_argmax(best, bi, wbest, widx, &piv, &pbest);
+        }
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const int col = c0 + j;
+            int p = piv;
+            const bool singular = !(pbest > 0);
+            if (singular) p = col;  // exact zero column: no interchange, LAPACK-style info
+            if (tid == 0) {
+                ipiv[col] = p;
+                pv[j] = p;
+                if (singular && info[e] == 0) info[e] = col + 1;
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int r = j0 + tid + 1024 * q;
+                if (r == p)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) prow[c] = P[q][c];
+                if (r == col)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) crow[c] = P[q][c];
+            }
+            __syncthreads();
+            if (p != col) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int r = j0 + tid + 1024 * q;
+                    if (r == p)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) P[q][c] = crow[c];
+                    if (r == col)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) P[q][c] = prow[c];
+                }
+            }
+            const T2 d = prow[j];
+            const bool nz = (d.x != 0 || d.y != 0);
+            const T2 inv = nz ? cdiv(T2{1, 0}, d) : T2{0, 0};
+            T best = -1;
+            int bi = col + 1;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int r = j0 + tid + 1024 * q;
+                if (r > col && r < n) {
+                    const T2 l = nz ? cmul(P[q][j], inv) : P[q][j];
+                    P[q][j] = l;
+#pragma unroll
+                    for (int c = j + 1; c < W; ++c) P[q][c] = cfms(P[q][c], l, prow[c]);
+                    if (j + 1 < W) {
+                        const T m = fabs(P[q][j + 1].x) + fabs(P[q][j + 1].y);
+                        if (m > best) {
+                            best = m;
+                            bi = r;
+                        }
+                    }
+                }
+            }
+            if (j + 1 < W) block_argmax(best, bi, wbest, widx, &piv, &pbest);
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = j0 + tid + 1024 * q;
+            if (r >= c0 && r < n) {
+                T2* dst = A + static_cast<size_t>(r) * lda + c0;
+#pragma unroll
+                for (int c = 0; c < W; ++c) dst[c] = P[q][c];
+            }
+        }
+        __syncthreads();
+        // this sub-panel's interchanges on the panel columns left and right of
+        // it; right of it also the unit-lower solve with its diagonal block
+        const int R = jend - (c0 + W);
+        if (tid < kPanel && (j0 + tid < c0 || j0 + tid >= c0 + W)) {
+            const int c = j0 + tid;
+            for (int jj = 0; jj < W; ++jj) {
+                const int pr = pv[jj];
+                if (pr != c0 + jj) {
+                    T2* a0 = A + static_cast<size_t>(c0 + jj) * lda + c;
+                    T2* a1 = A + static_cast<size_t>(pr) * lda + c;
+                    const T2 x = *a0;
+                    *a0 = *a1;
+                    *a1 = x;
+                }
+            }
+            if (c >= c0 + W) {
+                T2 u[W];
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    const T2* lrow = A + static_cast<size_t>(c0 + i) * lda;
+                    u[i] = lrow[c];
+#pragma unroll
+                    for (int k = 0; k < i; ++k) u[i] = cfms(u[i], lrow[c0 + k], u[k]);
+                }
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    A[static_cast<size_t>(c0 + i) * lda + c] = u[i];
+                    u12[i][c - (c0 + W)] = u[i];
+                }
+            }
+        }
+        if (R == 0) break;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = j0 + tid + 1024 * q;
+            if (r >= c0 + W && r < n) {
+                T2* row = A + static_cast<size_t>(r) * lda + c0 + W;
+                for (int cb = 0; cb < R; cb += W) {
+                    T2 v[W];
+#pragma unroll
+                    for (int c = 0; c < W; ++c) v[c] = row[cb + c];
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) v[c] = cfms(v[c], P[q][i], u12[i][cb + c]);
+#pragma unroll
+                    for (int c = 0; c < W; ++c) row[cb + c] = v[c];
+                }
+            }
+        }
+    }
+}
+
+// Row interchanges of panel j0 and/or the unit-lower triangular solve with the
+// panel's top 32x32 block, for columns [c_lo, c_hi) outside the panel.
+// do_swap applies the interchanges; columns >= trsm_from also get the solve.
 template <typename T, int CW>
 __global__ __launch_bounds__(CW) void k_swap_trsm(typename Cx<T>::type* Ab, size_t sa, int lda, const int* ipivb,
-                                                  int j0, int n) {
+                                                  int j0, int n, int c_lo, int c_hi, int do_swap, int trsm_from) {
     using T2 = typename Cx<T>::type;
     const int e = blockIdx.y, tid = threadIdx.x;
-    const int c = blockIdx.x * CW + tid;
+    const int c = c_lo + blockIdx.x * CW + tid;
     T2* A = Ab + e * sa;
     const int* ipiv = ipivb + static_cast<size_t>(e) * n;
     __shared__ T2 l11[kPanel][kPanel + 1];
     __shared__ T2 xs[kPanel][CW];
     __shared__ int pv[kPanel];
-    const bool right = (blockIdx.x * CW + CW > j0 + kPanel);
-    if (right)
+    const bool any_trsm = (c_lo + static_cast<int>(blockIdx.x) * CW + CW > trsm_from);
+    if (any_trsm)
         for (int i = tid; i < kPanel * kPanel; i += CW) {
             const int r = i / kPanel, q = i % kPanel;
             l11[r][q] = A[static_cast<size_t>(j0 + r) * lda + j0 + q];
         }
     if (tid < kPanel) pv[tid] = ipiv[j0 + tid];
     __syncthreads();
-    if (c >= n || (c >= j0 && c < j0 + kPanel)) return;
+    if (c >= c_hi || (c >= j0 && c < j0 + kPanel)) return;
     for (int r = 0; r < kPanel; ++r) xs[r][tid] = A[static_cast<size_t>(j0 + r) * lda + c];
-    for (int j = 0; j < kPanel; ++j) {
-        const int p = pv[j];
-        if (p == j0 + j) continue;
-        if (p < j0 + kPanel) {
-            const T2 t = xs[j][tid];
-            xs[j][tid] = xs[p - j0][tid];
-            xs[p - j0][tid] = t;
-        } else {
-            T2* far = A + static_cast<size_t>(p) * lda + c;
-            const T2 t = *far;
-            *far = xs[j][tid];
-            xs[j][tid] = t;
+    if (do_swap)
+        for (int j = 0; j < kPanel; ++j) {
+            const int p = pv[j];
+            if (p == j0 + j) continue;
+            if (p < j0 + kPanel) {
+                const T2 t = xs[j][tid];
+                xs[j][tid] = xs[p - j0][tid];
+                xs[p - j0][tid] = t;
+            } else {
+                T2* far = A + static_cast<size_t>(p) * lda + c;
+                const T2 t = *far;
+                *far = xs[j][tid];
+                xs[j][tid] = t;
+            }
         }
-    }
-    if (c >= j0 + kPanel) {
+    if (c >= trsm_from) {
         for (int r = 1; r < kPanel; ++r) {
             T2 acc = xs[r][tid];
             for (int q = 0; q < r; ++q) acc = cfms(acc, l11[r][q], xs[q][tid]);
@@ -597,9 +814,10 @@ template <> struct Mfma<double> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, const typename Cx<T>::type* Ab_,
-                                                  const typename Cx<T>::type* Bb_, size_t sbatch, int ld, int M,
-                                                  int N, int K) {
+__global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, int ldc, size_t sC,
+                                                  const typename Cx<T>::type* Ab_, int lda, size_t sA,
+                                                  const typename Cx<T>::type* Bb_, int ldb, size_t sB, int M, int N,
+                                                  int K) {
     using T2 = typename Cx<T>::type;
     using acc_t = typename Mfma<T>::acc;
     constexpr int BM = 64, BN = 64, BK = (sizeof(T) == 4) ? 32 : 16, PADA = (sizeof(T) == 4) ? 16 : 1;
@@ -608,9 +826,9 @@ __global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, cons
     const int e = blockIdx.y;
     const int tilesN = (N + BN - 1) / BN;
     const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
-    const T2* A = Ab_ + e * sbatch;
-    const T2* B = Bb_ + e * sbatch;
-    T2* C = Cb + e * sbatch;
+    const T2* A = Ab_ + e * sA;
+    const T2* B = Bb_ + e * sB;
+    T2* C = Cb + e * sC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     acc_t cr[2][2], ci[2][2];
@@ -627,7 +845,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, cons
 #pragma unroll
             for (int i = 0; i < KPT; ++i) {
                 T2 v = T2{0, 0};
-                if (gr < M) v = A[static_cast<size_t>(gr) * ld + k0 + kq + i];
+                if (gr < M) v = A[static_cast<size_t>(gr) * lda + k0 + kq + i];
                 as_re[kq + i][r] = v.x;
                 as_im[kq + i][r] = v.y;
             }
@@ -639,7 +857,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, cons
             for (int i = 0; i < CPT; ++i) {
                 const int gc = n0 + cq + i;
                 T2 v = T2{0, 0};
-                if (gc < N) v = B[static_cast<size_t>(k0 + kr) * ld + gc];
+                if (gc < N) v = B[static_cast<size_t>(k0 + kr) * ldb + gc];
                 bs_re[kr][cq + i] = v.x;
                 bs_im[kr][cq + i] = v.y;
             }
@@ -662,8 +880,13 @@ __global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, cons
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     cr[i][j] = Mfma<T>::run(ar[i], br[j], cr[i][j]);
-                    cr[i][j] = Mfma<T>::run(nai[i], bi[j], cr[i][j]);
                     ci[i][j] = Mfma<T>::run(ar[i], bi[j], ci[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    cr[i][j] = Mfma<T>::run(nai[i], bi[j], cr[i][j]);
                     ci[i][j] = Mfma<T>::run(ai[i], br[j], ci[i][j]);
                 }
         }
@@ -681,11 +904,225 @@ __global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, cons
                 const int row = m0 + wr * 32 + i * 16 + rr;
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
                 if (row < M && col < N) {
-                    T2* p = C + static_cast<size_t>(row) * ld + col;
+                    T2* p = C + static_cast<size_t>(row) * ldc + col;
                     T2 v = *p;
                     v.x -= cr[i][j][r];
                     v.y -= ci[i][j][r];
                     *p = v;
+                }
+            }
+}
+
+// C -= A * B for large FP32 complex updates: 128x128 tile, 4 waves of 64x64,
+// K in steps of 16 with two LDS stages (global loads of step k+1 overlap the
+// MFMAs of step k).  M, N, K multiples of 128, 128, 16 are not required: edge
+// tiles are guarded.
+__global__ __launch_bounds__(256) void k_gemm_big(float2* Cb, int ldc, size_t sC, const float2* Ab_, int lda,
+                                                  size_t sA, const float2* Bb_, int ldb, size_t sB, int M, int N,
+                                                  int K) {
+    constexpr int BM = 128, BN = 128, BK = 16, LDA = BM + 16, LDB = BN + 16;
+    __shared__ float as_re[2][BK][LDA], as_im[2][BK][LDA];
+    __shared__ float bs_re[2][BK][LDB], bs_im[2][BK][LDB];
+    const int e = blockIdx.y;
+    const int tilesN = (N + BN - 1) / BN;
+    const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+    const float2* A = Ab_ + e * sA;
+    const float2* B = Bb_ + e * sB;
+    float2* C = Cb + e * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    // global load mapping: A rows (tid/2), 8 k each; B k-row (tid/16), 8 cols each
+    const int ar = tid >> 1, ak = (tid & 1) * 8;
+    const int bk = tid >> 4, bc = (tid & 15) * 8;
+    float2 ra[8], rb[8];
+    auto gload = [&](int k0) {
+        const int gr = m0 + ar;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ra[i] = (gr < M) ? A[static_cast<size_t>(gr) * lda + k0 + ak + i] : float2{0, 0};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int gc = n0 + bc + i;
+            rb[i] = (gc < N) ? B[static_cast<size_t>(k0 + bk) * ldb + gc] : float2{0, 0};
+        }
+    };
+    auto sstore = [&](int st) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            as_re[st][ak + i][ar] = ra[i].x;
+            as_im[st][ak + i][ar] = ra[i].y;
+            bs_re[st][bk][bc + i] = rb[i].x;
+            bs_im[st][bk][bc + i] = rb[i].y;
+        }
+    };
+    v4f cr[4][4], ci[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cr[i][j] = v4f{0, 0, 0, 0};
+            ci[i][j] = v4f{0, 0, 0, 0};
+        }
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int st = 0;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) gload(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4), ix = lane & 15;
+            float ar_[4], ai_[4], nai[4], br_[4], bi_[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ar_[i] = as_re[st][kr][wr * 64 + i * 16 + ix];
+                ai_[i] = as_im[st][kr][wr * 64 + i * 16 + ix];
+                nai[i] = -ai_[i];
+                br_[i] = bs_re[st][kr][wc * 64 + i * 16 + ix];
+                bi_[i] = bs_im[st][kr][wc * 64 + i * 16 + ix];
+            }
+            // two passes so that back-to-back MFMAs never share an accumulator
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar_[i], br_[j], cr[i][j], 0, 0, 0);
+                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar_[i], bi_[j], ci[i][j], 0, 0, 0);
+                }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], bi_[j], cr[i][j], 0, 0, 0);
+                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai_[i], br_[j], ci[i][j], 0, 0, 0);
+                }
+        }
+        if (more) sstore(st ^ 1);
+        __syncthreads();
+        st ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    float2* q = C + static_cast<size_t>(row) * ldc + col;
+                    float2 v = *q;
+                    v.x -= cr[i][j][r];
+                    v.y -= ci[i][j][r];
+                    *q = v;
+                }
+            }
+}
+
+// C -= A * B, FP32 complex, 64x64 tile per 256-thread workgroup (4 waves of
+// 32x32), K in steps of 32 with register prefetch of the next step and two
+// LDS stages; 16-byte global loads (two complex values) where the operand
+// offsets are even.
+__global__ __launch_bounds__(256, 2) void k_gemm_v2(float2* Cb, int ldc, size_t sC, const float2* Ab_, int lda,
+                                                     size_t sA, const float2* Bb_, int ldb, size_t sB, int M, int N,
+                                                     int K) {
+    constexpr int BM = 64, BN = 64, BK = 32, LD = 64 + 16;
+    __shared__ float as_re[2][BK][LD], as_im[2][BK][LD];
+    __shared__ float bs_re[2][BK][LD], bs_im[2][BK][LD];
+    const int e = blockIdx.y;
+    const int tilesN = (N + BN - 1) / BN;
+    const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+    const float2* A = Ab_ + e * sA;
+    const float2* B = Bb_ + e * sB;
+    float2* C = Cb + e * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int ar = tid >> 2, ak = (tid & 3) * 8;   // A: row, 8 consecutive k
+    const int bk = tid >> 3, bc = (tid & 7) * 8;   // B: k row, 8 consecutive columns
+    float4 ra[4], rb[4];
+    auto gload = [&](int k0) {
+        const int gr = m0 + ar;
+        const float4* pa = reinterpret_cast<const float4*>(A + static_cast<size_t>(gr) * lda + k0 + ak);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ra[i] = (gr < M) ? pa[i] : float4{0, 0, 0, 0};
+        const int gc = n0 + bc;
+        const float4* pb = reinterpret_cast<const float4*>(B + static_cast<size_t>(k0 + bk) * ldb + gc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rb[i] = (gc + 2 * i < N) ? pb[i] : float4{0, 0, 0, 0};
+    };
+    auto sstore = [&](int st) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            as_re[st][ak + 2 * i][ar] = ra[i].x;
+            as_im[st][ak + 2 * i][ar] = ra[i].y;
+            as_re[st][ak + 2 * i + 1][ar] = ra[i].z;
+            as_im[st][ak + 2 * i + 1][ar] = ra[i].w;
+            bs_re[st][bk][bc + 2 * i] = rb[i].x;
+            bs_im[st][bk][bc + 2 * i] = rb[i].y;
+            bs_re[st][bk][bc + 2 * i + 1] = rb[i].z;
+            bs_im[st][bk][bc + 2 * i + 1] = rb[i].w;
+        }
+    };
+    v4f cr[2][2], ci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            cr[i][j] = v4f{0, 0, 0, 0};
+            ci[i][j] = v4f{0, 0, 0, 0};
+        }
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int st = 0;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) gload(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4), ix = lane & 15;
+            float xr[2], xi[2], nxi[2], yr[2], yi[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                xr[i] = as_re[st][kr][wr * 32 + i * 16 + ix];
+                xi[i] = as_im[st][kr][wr * 32 + i * 16 + ix];
+                nxi[i] = -xi[i];
+                yr[i] = bs_re[st][kr][wc * 32 + i * 16 + ix];
+                yi[i] = bs_im[st][kr][wc * 32 + i * 16 + ix];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yr[j], cr[i][j], 0, 0, 0);
+                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yi[j], ci[i][j], 0, 0, 0);
+                }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nxi[i], yi[j], cr[i][j], 0, 0, 0);
+                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[i], yr[j], ci[i][j], 0, 0, 0);
+                }
+        }
+        if (more) sstore(st ^ 1);
+        __syncthreads();
+        st ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int col = n0 + wc * 32 + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    float2* q = C + static_cast<size_t>(row) * ldc + col;
+                    float2 v = *q;
+                    v.x -= cr[i][j][r];
+                    v.y -= ci[i][j][r];
+                    *q = v;
                 }
             }
 }
@@ -725,79 +1162,45 @@ __global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, con
     }
 }
 
-// In-place forward (unit lower) and backward (upper) substitution of the
-// permuted RHS block X (n x nrhs, row-major) with the packed LU factors.
-// One workgroup per energy.
-template <typename T>
-__global__ __launch_bounds__(256) void k_solve(const typename Cx<T>::type* LUb, size_t sa, int lda,
-                                               typename Cx<T>::type* Xb, size_t sx, int n, int ldx) {
+// One 32-row block of the forward (unit lower, LOWER=true) or backward (upper)
+// substitution, in place on X rows [k0, k0+32); the coupling to the other
+// blocks is applied by k_gemm_sub.  Grid: (energy, RHS chunk).
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_block(const typename Cx<T>::type* LUb, size_t sa, int lda,
+                                                    typename Cx<T>::type* Xb, size_t sx, int ldx, int k0) {
     using T2 = typename Cx<T>::type;
     constexpr int BS = kPanel;
-    extern __shared__ unsigned char smem_raw[];
-    T2* tri = reinterpret_cast<T2*>(smem_raw);          // [BS][BS+1]
-    T2* xb = tri + BS * (BS + 1);                        // [BS][nrhs] (this block's column chunk)
+    __shared__ T2 tri[BS][BS + 1];
+    __shared__ T2 xb[BS][kRhsChunk];
     const int e = blockIdx.x, tid = threadIdx.x;
     const int c0 = blockIdx.y * kRhsChunk;
-    const int nrhs = min(kRhsChunk, ldx - c0);
+    const int nr = min(kRhsChunk, ldx - c0);
     const T2* LU = LUb + e * sa;
     T2* X = Xb + e * sx + c0;
-    const int nblk = n / BS;
-    // forward
-    for (int kb = 0; kb < nblk; ++kb) {
-        const int k0 = kb * BS;
-        for (int i = tid; i < BS * BS; i += 256) {
-            const int r = i / BS, q = i % BS;
-            tri[r * (BS + 1) + q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
-        }
-        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs];
-        __syncthreads();
-        for (int c = tid; c < nrhs; c += 256)
+    for (int i = tid; i < BS * BS; i += 256) {
+        const int r = i / BS, q = i % BS;
+        tri[r][q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
+    }
+    for (int i = tid; i < BS * nr; i += 256) xb[i / nr][i % nr] = X[static_cast<size_t>(k0 + i / nr) * ldx + i % nr];
+    __syncthreads();
+    if (tid < nr) {
+        const int c = tid;
+        if (LOWER) {
             for (int r = 1; r < BS; ++r) {
-                T2 acc = xb[r * nrhs + c];
-                for (int q = 0; q < r; ++q) acc = cfms(acc, tri[r * (BS + 1) + q], xb[q * nrhs + c]);
-                xb[r * nrhs + c] = acc;
+                T2 acc = xb[r][c];
+                for (int q = 0; q < r; ++q) acc = cfms(acc, tri[r][q], xb[q][c]);
+                xb[r][c] = acc;
             }
-        __syncthreads();
-        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs] = xb[i];
-        const size_t rest = static_cast<size_t>(n - k0 - BS) * nrhs;
-        for (size_t idx = tid; idx < rest; idx += 256) {
-            const int i = k0 + BS + static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
-            const T2* lrow = LU + static_cast<size_t>(i) * lda + k0;
-            T2 acc = X[static_cast<size_t>(i) * ldx + c];
-#pragma unroll 8
-            for (int q = 0; q < BS; ++q) acc = cfms(acc, lrow[q], xb[q * nrhs + c]);
-            X[static_cast<size_t>(i) * ldx + c] = acc;
-        }
-        __syncthreads();
-    }
-    // backward
-    for (int kb = nblk - 1; kb >= 0; --kb) {
-        const int k0 = kb * BS;
-        for (int i = tid; i < BS * BS; i += 256) {
-            const int r = i / BS, q = i % BS;
-            tri[r * (BS + 1) + q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
-        }
-        for (int i = tid; i < BS * nrhs; i += 256) xb[i] = X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs];
-        __syncthreads();
-        for (int c = tid; c < nrhs; c += 256)
+        } else {
             for (int r = BS - 1; r >= 0; --r) {
-                T2 acc = xb[r * nrhs + c];
-                for (int q = r + 1; q < BS; ++q) acc = cfms(acc, tri[r * (BS + 1) + q], xb[q * nrhs + c]);
-                xb[r * nrhs + c] = cdiv(acc, tri[r * (BS + 1) + r]);
+                T2 acc = xb[r][c];
+                for (int q = r + 1; q < BS; ++q) acc = cfms(acc, tri[r][q], xb[q][c]);
+                xb[r][c] = cdiv(acc, tri[r][r]);
             }
-        __syncthreads();
-        for (int i = tid; i < BS * nrhs; i += 256) X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs] = xb[i];
-        const size_t rest = static_cast<size_t>(k0) * nrhs;
-        for (size_t idx = tid; idx < rest; idx += 256) {
-            const int i = static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
-            const T2* urow = LU + static_cast<size_t>(i) * lda + k0;
-            T2 acc = X[static_cast<size_t>(i) * ldx + c];
-#pragma unroll 8
-            for (int q = 0; q < BS; ++q) acc = cfms(acc, urow[q], xb[q * nrhs + c]);
-            X[static_cast<size_t>(i) * ldx + c] = acc;
         }
-        __syncthreads();
     }
+    __syncthreads();
+    for (int i = tid; i < BS * nr; i += 256) X[static_cast<size_t>(k0 + i / nr) * ldx + i % nr] = xb[i / nr][i % nr];
 }
 
 // R = B - A X in FP64 (64 rows per workgroup); tracks max |R| per energy.
@@ -952,6 +1355,7 @@ using namespace refeff;
 struct refeff_fms_plan {
     int nat = 0, nsp = 1, ispin = 0, lfms = 0, minv = 0, max_pot = 0, L = 0;
     int n = 0, npad = 0, nrhs = 0, ch = 0, npot = 0, pstart = 0, pend = 0, solver = 1, batch = 0, max_batch = 0;
+    int outer_block = 128;  // LU outer block width = K of the trailing MFMA update
     float rdirec = 0.f;
     std::vector<int> rep, col0, ipart;
     // device constants
@@ -970,7 +1374,9 @@ struct refeff_fms_plan {
     int *d_ipiv = nullptr, *d_perm = nullptr, *d_info = nullptr;
     double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_norms = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // main stream (copies, single-group work)
+    int nstreams = 1;                   // energy sub-batches issued on separate streams
+    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex mu;
     // kernel-level instrumentation (refeff_fms_plan_profile)
     struct Rec {
@@ -1025,16 +1431,22 @@ void drain_profile(refeff_fms_plan* p) {
     p->pool_used = 0;
 }
 
+// One sub-batch of energies [e0, e0+nb) issued on its own stream.
+struct Group {
+    hipStream_t s;
+    int e0, nb;
+};
+
 template <typename F>
-void launch(refeff_fms_plan* p, int cls, double flops, double bytes, F&& f) {
+void launch(refeff_fms_plan* p, hipStream_t s, int cls, double flops, double bytes, F&& f) {
     if (!p->profile) {
         f();
         return;
     }
     hipEvent_t a = next_event(p), b = next_event(p);
-    if (a) (void)hipEventRecord(a, p->stream);
+    if (a) (void)hipEventRecord(a, s);
     f();
-    if (b) (void)hipEventRecord(b, p->stream);
+    if (b) (void)hipEventRecord(b, s);
     p->recs.push_back({a, b, cls, flops, bytes});
 }
 
@@ -1084,49 +1496,135 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     return REFEFF_FMS_OK;
 }
 
+// Register-blocked panel when the panel's rows fit the register tiles without
+// spilling (sub-panel width shrinking as the row count grows); the streaming
+// kernel otherwise.
 template <typename T>
-void run_lu(refeff_fms_plan* p, typename Cx<T>::type* A, int nb) {
-    const int n = p->npad;
-    const size_t sa = static_cast<size_t>(n) * n;
-    constexpr int CW = (sizeof(T) == 4) ? 128 : 64;
-    const double cb = sizeof(typename Cx<T>::type);
-    for (int j0 = 0; j0 < n; j0 += kPanel) {
-        const double m = n - j0;
-        launch(p, REFEFF_FMS_K_PANEL, 8.0 * m * kPanel * kPanel / 2 * nb, 2.0 * m * kPanel * cb * nb, [&] {
-            hipLaunchKernelGGL(k_panel<T>, dim3(nb), dim3(256), 0, p->stream, A, sa, n, p->d_ipiv, p->d_info, j0, n);
-        });
-        launch(p, REFEFF_FMS_K_SWAP, 8.0 * kPanel * kPanel / 2 * (n - j0 - kPanel) * nb,
-               2.0 * kPanel * n * cb * nb, [&] {
-                   hipLaunchKernelGGL((k_swap_trsm<T, CW>), dim3((n + CW - 1) / CW, nb), dim3(CW), 0, p->stream, A,
-                                      sa, n, p->d_ipiv, j0, n);
-               });
-        const int rest = n - j0 - kPanel;
-        if (rest > 0) {
-            const int tiles = ((rest + 63) / 64) * ((rest + 63) / 64);
-            typename Cx<T>::type* base = A + static_cast<size_t>(j0 + kPanel) * n + j0 + kPanel;
-            const typename Cx<T>::type* lpan = A + static_cast<size_t>(j0 + kPanel) * n + j0;
-            const typename Cx<T>::type* urow = A + static_cast<size_t>(j0) * n + j0 + kPanel;
-            const double r = rest;
-            launch(p, REFEFF_FMS_K_GEMM, 8.0 * r * r * kPanel * nb, (2.0 * r * kPanel + 2.0 * r * r) * cb * nb, [&] {
-                hipLaunchKernelGGL(k_gemm_sub<T>, dim3(tiles, nb), dim3(256), 0, p->stream, base, lpan, urow, sa, n,
-                                   rest, rest, kPanel);
-            });
-        }
+void launch_panel(hipStream_t s, int nb, typename Cx<T>::type* A, size_t sa, int lda, int* ipiv, int* info, int j0,
+                  int n) {
+    const int m = n - j0;
+    const dim3 grid(nb), block(1024);
+    if constexpr (sizeof(T) == 4) {
+        if (m <= 3072)
+            hipLaunchKernelGGL((k_panel_rb<T, 8, 3>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else if (m <= 4096)
+            hipLaunchKernelGGL((k_panel_rb<T, 4, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else if (m <= 8192)
+            hipLaunchKernelGGL((k_panel_rb<T, 2, 8>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else
+            hipLaunchKernelGGL(k_panel<T>, grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+    } else {
+        if (m <= 2048)
+            hipLaunchKernelGGL((k_panel_rb<T, 4, 2>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else if (m <= 4096)
+            hipLaunchKernelGGL((k_panel_rb<T, 2, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else
+            hipLaunchKernelGGL(k_panel<T>, grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
     }
-    launch(p, REFEFF_FMS_K_OTHER, 0.0, 8.0 * n * nb, [&] {
-        hipLaunchKernelGGL(k_perm, dim3(nb), dim3(256), n * sizeof(int), p->stream, p->d_ipiv, p->d_perm, n);
-    });
 }
 
 template <typename T>
-void run_solve(refeff_fms_plan* p, const typename Cx<T>::type* LU, typename Cx<T>::type* X, int nb) {
+void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int ldc, size_t sC,
+              const typename Cx<T>::type* A, int lda, size_t sA, const typename Cx<T>::type* B, int ldb, size_t sB,
+              int M, int N, int K) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    const int nb = g.nb;
+    const double cb = sizeof(typename Cx<T>::type);
+    // the double-buffered FP32 kernel reads two complex values per 16-byte load:
+    // every operand row must start 16-byte aligned and N must be even
+    const bool v2 = sizeof(T) == 4 && (N % 2 == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
+                    ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % 16 == 0);
+    launch(p, g.s, K > kPanel ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL, 8.0 * M * N * K * nb,
+           (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
+               const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+               if (v2)
+                   hipLaunchKernelGGL(k_gemm_v2, dim3(tiles, nb), dim3(256), 0, g.s, reinterpret_cast<float2*>(C), ldc,
+                                      sC, reinterpret_cast<const float2*>(A), lda, sA,
+                                      reinterpret_cast<const float2*>(B), ldb, sB, M, N, K);
+               else
+                   hipLaunchKernelGGL(k_gemm_sub<T>, dim3(tiles, nb), dim3(256), 0, g.s, C, ldc, sC, A, lda, sA, B,
+                                      ldb, sB, M, N, K);
+           });
+}
+
+template <typename T>
+void swap_trsm(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, const int* ipiv, int j0, int c_lo,
+               int c_hi, int do_swap, int trsm_from) {
     const int n = p->npad;
-    const size_t shm = (kPanel * (kPanel + 1) + kPanel * std::min(p->nrhs, kRhsChunk)) * sizeof(typename Cx<T>::type);
-    const double nn = static_cast<double>(n) * n;
-    launch(p, REFEFF_FMS_K_SOLVE, 8.0 * nn * p->nrhs * nb, nn * sizeof(typename Cx<T>::type) * nb, [&] {
-        hipLaunchKernelGGL(k_solve<T>, dim3(nb, (p->nrhs + kRhsChunk - 1) / kRhsChunk), dim3(256), shm, p->stream,
-                           LU, static_cast<size_t>(n) * n, n, X, static_cast<size_t>(n) * p->nrhs, n, p->nrhs);
+    if (c_hi <= c_lo) return;
+    constexpr int CW = (sizeof(T) == 4) ? 128 : 64;
+    const double cb = sizeof(typename Cx<T>::type);
+    const double cols = c_hi - c_lo, tcols = std::max(0, c_hi - std::max(c_lo, trsm_from));
+    launch(p, g.s, REFEFF_FMS_K_SWAP, 8.0 * kPanel * kPanel / 2 * tcols * g.nb, 2.0 * kPanel * cols * cb * g.nb, [&] {
+        hipLaunchKernelGGL((k_swap_trsm<T, CW>), dim3((c_hi - c_lo + CW - 1) / CW, g.nb), dim3(CW), 0, g.s, A,
+                           static_cast<size_t>(n) * n, n, ipiv, j0, n, c_lo, c_hi, do_swap, trsm_from);
     });
+}
+
+// Blocked right-looking LU: outer blocks of outer_block columns are factored as
+// a tall panel (inner 32-column steps), their interchanges are then applied to
+// the rest of the matrix, U12 is solved in 32-row steps, and the trailing
+// matrix gets one K = outer_block MFMA update.  A, ipiv, perm, info already
+// point at the group's first energy.
+template <typename T>
+void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ipiv, int* perm, int* info) {
+    const int kOuter = p->outer_block;
+    const int n = p->npad, nb = g.nb;
+    const size_t sa = static_cast<size_t>(n) * n;
+    const double cb = sizeof(typename Cx<T>::type);
+    auto at = [&](int r, int c) { return A + static_cast<size_t>(r) * n + c; };
+    for (int k0 = 0; k0 < n; k0 += kOuter) {
+        const int kend = std::min(k0 + kOuter, n);
+        for (int j0 = k0; j0 < kend; j0 += kPanel) {
+            const double m = n - j0;
+            launch(p, g.s, REFEFF_FMS_K_PANEL, 8.0 * m * kPanel * kPanel / 2 * nb, 2.0 * m * kPanel * cb * nb, [&] {
+                launch_panel<T>(g.s, nb, A, sa, n, ipiv, info, j0, n);
+            });
+            swap_trsm<T>(p, g, A, ipiv, j0, k0, kend, 1, j0 + kPanel);
+            gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, j0 + kPanel), n,
+                        sa, n - j0 - kPanel, kend - j0 - kPanel, kPanel);
+        }
+        // all interchanges of the outer block, left and right of it, before any far update
+        for (int j0 = k0; j0 < kend; j0 += kPanel) {
+            swap_trsm<T>(p, g, A, ipiv, j0, 0, k0, 1, n);
+            swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 1, n);
+        }
+        if (kend < n) {
+            for (int j0 = k0; j0 < kend; j0 += kPanel) {
+                swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 0, kend);
+                gemm_sub<T>(p, g, at(j0 + kPanel, kend), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, kend), n, sa,
+                            kend - j0 - kPanel, n - kend, kPanel);
+            }
+            gemm_sub<T>(p, g, at(kend, kend), n, sa, at(kend, k0), n, sa, at(k0, kend), n, sa, n - kend, n - kend,
+                        kend - k0);
+        }
+    }
+    launch(p, g.s, REFEFF_FMS_K_OTHER, 0.0, 8.0 * n * nb, [&] {
+        hipLaunchKernelGGL(k_perm, dim3(nb), dim3(256), n * sizeof(int), g.s, ipiv, perm, n);
+    });
+}
+
+// X <- U^-1 L^-1 X (X already row-permuted), 32-row blocks: block solve, then
+// an MFMA update of the rows still to be solved.
+template <typename T>
+void run_solve(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, typename Cx<T>::type* X) {
+    const int n = p->npad, nr = p->nrhs, nb = g.nb;
+    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
+    const int chunks = (nr + kRhsChunk - 1) / kRhsChunk;
+    const double cb = sizeof(typename Cx<T>::type);
+    for (int k0 = 0; k0 < n; k0 += kPanel) {
+        launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * kPanel * kPanel / 2 * nr * nb, kPanel * kPanel * cb * nb, [&] {
+            hipLaunchKernelGGL((k_trsv_block<T, true>), dim3(nb, chunks), dim3(256), 0, g.s, LU, sa, n, X, sx, nr, k0);
+        });
+        gemm_sub<T>(p, g, X + static_cast<size_t>(k0 + kPanel) * nr, nr, sx, LU + static_cast<size_t>(k0 + kPanel) * n + k0,
+                    n, sa, X + static_cast<size_t>(k0) * nr, nr, sx, n - k0 - kPanel, nr, kPanel);
+    }
+    for (int k0 = n - kPanel; k0 >= 0; k0 -= kPanel) {
+        launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * kPanel * kPanel / 2 * nr * nb, kPanel * kPanel * cb * nb, [&] {
+            hipLaunchKernelGGL((k_trsv_block<T, false>), dim3(nb, chunks), dim3(256), 0, g.s, LU, sa, n, X, sx, nr, k0);
+        });
+        gemm_sub<T>(p, g, X, nr, sx, LU + k0, n, sa, X + static_cast<size_t>(k0) * nr, nr, sx, k0, nr, kPanel);
+    }
 }
 
 double now_s() {
@@ -1139,117 +1637,151 @@ int map_info(int info) {
     return REFEFF_FMS_OK;
 }
 
-// one device batch of nb energies; inputs already in p->d_k / p->d_ph
+bool sync_groups(const std::vector<Group>& gs) {
+    bool ok = true;
+    for (const auto& g : gs) ok = (hipStreamSynchronize(g.s) == hipSuccess) && ok;
+    return ok;
+}
+
+// one device batch of nb energies; inputs already in p->d_k / p->d_ph.  The
+// batch is split into sub-batches on separate streams so that latency-bound
+// kernels of one sub-batch overlap MFMA work of another.
 int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status, refeff_fms_stats* st) {
     const int n = p->npad;
     const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * p->nrhs;
+    const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+    const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
     const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
     const double t0 = now_s();
-    hipLaunchKernelGGL(k_fill_int, dim3((nb + 255) / 256), dim3(256), 0, p->stream, p->d_info, 0, static_cast<size_t>(nb));
-    TmatArgs ta{p->d_sl, p->d_sm, p->d_ss, p->d_spot, p->d_so_plus, p->d_so_minus, p->d_ph, p->d_tmat, p->d_info,
-                p->n, p->nsp, p->ispin, p->L, p->npot};
-    launch(p, REFEFF_FMS_K_TMATRIX, 0.0, 32.0 * p->n * nb, [&] {
-        hipLaunchKernelGGL(k_tmatrix, dim3((p->n + 255) / 256, nb), dim3(256), 0, p->stream, ta);
-    });
-    BuildArgs ba{p->d_pos, p->d_atom_first, p->d_atom_lmax, p->d_atom_rhs, p->d_sig, p->d_xnlm, p->d_tmat, p->d_k,
-                 p->d_a32, p->d_a64, p->d_b64, sa, sx, p->nat, p->n, n, p->nsp, p->L, p->nrhs, nb, p->rdirec * p->rdirec};
-    const double n2 = static_cast<double>(p->n) * p->n;
-    launch(p, REFEFF_FMS_K_BUILD, 0.0, n2 * (mixed ? 24.0 : 16.0) * nb + 16.0 * p->n * p->nrhs * nb, [&] {
-        if (mixed)
-            hipLaunchKernelGGL((k_build<true, true>), dim3(p->nat, p->nat), dim3(256), 0, p->stream, ba);
-        else
-            hipLaunchKernelGGL((k_build<false, true>), dim3(p->nat, p->nat), dim3(256), 0, p->stream, ba);
-    });
-    if (n > p->n) {
-        hipLaunchKernelGGL(k_pad<double>, dim3(n, nb), dim3(256), 0, p->stream, p->d_a64, sa, n, p->n, p->d_b64, sx,
-                           p->nrhs);
-        if (mixed)
-            hipLaunchKernelGGL(k_pad<float>, dim3(n, nb), dim3(256), 0, p->stream, p->d_a32, sa, n, p->n,
-                               static_cast<double2*>(nullptr), sx, p->nrhs);
+    std::vector<Group> gs;
+    {
+        const int G = std::max(1, std::min(p->nstreams, nb));
+        int e0 = 0;
+        for (int i = 0; i < G; ++i) {
+            const int cnt = nb / G + (i < nb % G ? 1 : 0);
+            gs.push_back(Group{p->streams[i] ? p->streams[i] : p->stream, e0, cnt});
+            e0 += cnt;
+        }
     }
-    (void)hipStreamSynchronize(p->stream);
-    const double t1 = now_s();
+    const double n2 = static_cast<double>(p->n) * p->n;
+    for (const auto& g : gs) {
+        hipLaunchKernelGGL(k_fill_int, dim3((g.nb + 255) / 256), dim3(256), 0, g.s, p->d_info + g.e0, 0,
+                           static_cast<size_t>(g.nb));
+        TmatArgs ta{p->d_sl, p->d_sm, p->d_ss, p->d_spot, p->d_so_plus, p->d_so_minus, p->d_ph + pst * g.e0,
+                    p->d_tmat + 2 * static_cast<size_t>(p->n) * g.e0, p->d_info + g.e0, p->n, p->nsp, p->ispin, p->L,
+                    p->npot};
+        launch(p, g.s, REFEFF_FMS_K_TMATRIX, 0.0, 32.0 * p->n * g.nb, [&] {
+            hipLaunchKernelGGL(k_tmatrix, dim3((p->n + 255) / 256, g.nb), dim3(256), 0, g.s, ta);
+        });
+        BuildArgs ba{p->d_pos, p->d_atom_first, p->d_atom_lmax, p->d_atom_rhs, p->d_sig, p->d_xnlm,
+                     p->d_tmat + 2 * static_cast<size_t>(p->n) * g.e0, p->d_k + kst * g.e0,
+                     mixed ? p->d_a32 + sa * g.e0 : nullptr, p->d_a64 + sa * g.e0, p->d_b64 + sx * g.e0, sa, sx,
+                     p->nat, p->n, n, p->nsp, p->L, p->nrhs, g.nb, p->rdirec * p->rdirec};
+        launch(p, g.s, REFEFF_FMS_K_BUILD, 0.0, n2 * (mixed ? 24.0 : 16.0) * g.nb + 16.0 * p->n * p->nrhs * g.nb, [&] {
+            if (mixed)
+                hipLaunchKernelGGL((k_build<true, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
+            else
+                hipLaunchKernelGGL((k_build<false, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
+        });
+        if (n > p->n) {
+            hipLaunchKernelGGL(k_pad<double>, dim3(n, g.nb), dim3(256), 0, g.s, p->d_a64 + sa * g.e0, sa, n, p->n,
+                               p->d_b64 + sx * g.e0, sx, p->nrhs);
+            if (mixed)
+                hipLaunchKernelGGL(k_pad<float>, dim3(n, g.nb), dim3(256), 0, g.s, p->d_a32 + sa * g.e0, sa, n, p->n,
+                                   static_cast<double2*>(nullptr), sx, p->nrhs);
+        }
+        int* ipiv = p->d_ipiv + static_cast<size_t>(n) * g.e0;
+        int* perm = p->d_perm + static_cast<size_t>(n) * g.e0;
+        if (mixed)
+            run_lu<float>(p, g, p->d_a32 + sa * g.e0, ipiv, perm, p->d_info + g.e0);
+        else
+            run_lu<double>(p, g, p->d_a64 + sa * g.e0, ipiv, perm, p->d_info + g.e0);
+    }
     int sweeps = 0;
     double worst = 0.0;
+    double t_first = 0.0;
     std::vector<double> norms(4 * static_cast<size_t>(nb));
+    const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
     if (mixed) {
-        run_lu<float>(p, p->d_a32, nb);
-        (void)hipStreamSynchronize(p->stream);
-        const double t2 = now_s();
         float2* w = static_cast<float2*>(p->d_w);
-        const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
-        hipLaunchKernelGGL(k_fill_double, dim3((2 * sx * nb + 255) / 256), dim3(256), 0, p->stream,
-                           reinterpret_cast<double*>(p->d_x64), 0.0, 2 * sx * nb);
-        const int max_sweeps = 10;
+        for (const auto& g : gs)
+            hipLaunchKernelGGL(k_fill_double, dim3((2 * sx * g.nb + 255) / 256), dim3(256), 0, g.s,
+                               reinterpret_cast<double*>(p->d_x64 + sx * g.e0), 0.0, 2 * sx * g.nb);
+        const int max_sweeps = 8;
+        std::vector<double> prev_rel(nb, 1.0e300);
         for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-            hipLaunchKernelGGL(k_fill_double, dim3((4 * nb + 255) / 256), dim3(256), 0, p->stream, p->d_norms, 0.0,
-                               static_cast<size_t>(4 * nb));
-            const double2* rhs = p->d_b64;
-            if (sweep > 0) {
-                const size_t shm = (64 * 17 + 16 * std::min(p->nrhs, kRhsChunk)) * sizeof(double2);
-                const double nn = static_cast<double>(n) * n;
-                launch(p, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * p->nrhs * nb, 16.0 * nn * nb, [&] {
-                    hipLaunchKernelGGL(k_residual, dim3(n / 64, nb, (p->nrhs + kRhsChunk - 1) / kRhsChunk), dim3(256),
-                                       shm, p->stream, p->d_a64, sa, n,
-                                       p->d_x64, p->d_b64, p->d_r64, sx, n, p->nrhs, p->d_norms);
-                });
-                rhs = p->d_r64;
+            for (const auto& g : gs) {
+                double* gn = p->d_norms + 4 * static_cast<size_t>(g.e0);
+                hipLaunchKernelGGL(k_fill_double, dim3((4 * g.nb + 255) / 256), dim3(256), 0, g.s, gn, 0.0,
+                                   static_cast<size_t>(4 * g.nb));
+                const double2* rhs = p->d_b64 + sx * g.e0;
+                if (sweep > 0) {
+                    const size_t shm = (64 * 17 + 16 * std::min(p->nrhs, kRhsChunk)) * sizeof(double2);
+                    const double nn = static_cast<double>(n) * n;
+                    launch(p, g.s, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * p->nrhs * g.nb, 16.0 * nn * g.nb, [&] {
+                        hipLaunchKernelGGL(k_residual, dim3(n / 64, g.nb, (p->nrhs + kRhsChunk - 1) / kRhsChunk),
+                                           dim3(256), shm, g.s, p->d_a64 + sa * g.e0, sa, n, p->d_x64 + sx * g.e0,
+                                           p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, n, p->nrhs, gn);
+                    });
+                    rhs = p->d_r64 + sx * g.e0;
+                }
+                hipLaunchKernelGGL(k_gather<float>, dim3(gx, g.nb), dim3(256), 0, g.s, rhs, sx,
+                                   p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0);
+                run_solve<float>(p, g, p->d_a32 + sa * g.e0, w + sx * g.e0);
+                hipLaunchKernelGGL(k_update<float>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_x64 + sx * g.e0,
+                                   w + sx * g.e0, sx, n, p->nrhs, gn);
+                (void)hipMemcpyAsync(norms.data() + 4 * static_cast<size_t>(g.e0), gn, 4 * g.nb * sizeof(double),
+                                     hipMemcpyDeviceToHost, g.s);
             }
-            hipLaunchKernelGGL(k_gather<float>, dim3(gx, nb), dim3(256), 0, p->stream, rhs, sx, p->d_perm, n, p->nrhs, w);
-            run_solve<float>(p, p->d_a32, w, nb);
-            hipLaunchKernelGGL(k_update<float>, dim3(gx, nb), dim3(256), 0, p->stream, p->d_x64, w, sx, n, p->nrhs,
-                               p->d_norms);
-            (void)hipMemcpyAsync(norms.data(), p->d_norms, norms.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream);
-            (void)hipStreamSynchronize(p->stream);
+            if (!sync_groups(gs)) return REFEFF_FMS_DEVICE_ERROR;
+            if (sweep == 0) t_first = now_s();
             sweeps = sweep + 1;
-            bool done = true;
+            // stop when every energy's last correction is below 1e-10 of |x|,
+            // or has stopped shrinking at the FP64 noise floor
+            bool done = sweep > 0;
             worst = 0.0;
             for (int e = 0; e < nb; ++e) {
                 const double rel = norms[4 * e + 2] > 0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
                 worst = std::max(worst, rel);
-                if (!(rel <= 1.0e-14)) done = false;
+                const bool small = rel <= 1.0e-10;
+                const bool floor = rel <= 1.0e-9 && rel >= 0.5 * prev_rel[e];
+                if (!(small || floor)) done = false;
+                prev_rel[e] = rel;
             }
-            if (done && sweep > 0) break;
-        }
-        if (st) {
-            st->seconds_factor += t2 - t1;
-            st->seconds_solve += now_s() - t2;
+            if (done) break;
         }
     } else {
-        run_lu<double>(p, p->d_a64, nb);
-        (void)hipStreamSynchronize(p->stream);
-        const double t2 = now_s();
-        const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
-        hipLaunchKernelGGL(k_gather<double>, dim3(gx, nb), dim3(256), 0, p->stream, p->d_b64, sx, p->d_perm, n, p->nrhs,
-                           p->d_x64);
-        run_solve<double>(p, p->d_a64, p->d_x64, nb);
-        (void)hipStreamSynchronize(p->stream);
-        sweeps = 0;
-        if (st) {
-            st->seconds_factor += t2 - t1;
-            st->seconds_solve += now_s() - t2;
+        for (const auto& g : gs) {
+            hipLaunchKernelGGL(k_gather<double>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, sx,
+                               p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, p->d_x64 + sx * g.e0);
+            run_solve<double>(p, g, p->d_a64 + sa * g.e0, p->d_x64 + sx * g.e0);
         }
+        t_first = now_s();
     }
-    ExtractArgs ea{p->d_x64, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0, p->d_ipart, out};
-    const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
-    hipLaunchKernelGGL(k_extract, dim3(std::max<size_t>(1, std::min<size_t>(64, (per + 255) / 256)), nb), dim3(256), 0,
-                       p->stream, ea);
     std::vector<int> info(nb);
-    (void)hipMemcpyAsync(info.data(), p->d_info, nb * sizeof(int), hipMemcpyDeviceToHost, p->stream);
-    if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
+    for (const auto& g : gs) {
+        ExtractArgs ea{p->d_x64 + sx * g.e0, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0,
+                       p->d_ipart, out + sst * g.e0};
+        hipLaunchKernelGGL(k_extract, dim3(std::max<size_t>(1, std::min<size_t>(64, (per + 255) / 256)), g.nb),
+                           dim3(256), 0, g.s, ea);
+        (void)hipMemcpyAsync(info.data() + g.e0, p->d_info + g.e0, g.nb * sizeof(int), hipMemcpyDeviceToHost, g.s);
+    }
+    if (!sync_groups(gs)) return REFEFF_FMS_DEVICE_ERROR;
     if (p->profile) drain_profile(p);
     int rc = REFEFF_FMS_OK;
     for (int e = 0; e < nb; ++e) {
         int s = map_info(info[e]);
         if (!s && mixed) {
             const double rel = norms[4 * e + 2] > 0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
-            if (!(rel <= 1.0e-10)) s = REFEFF_FMS_NO_CONVERGENCE;
+            if (!(rel <= 1.0e-9)) s = REFEFF_FMS_NO_CONVERGENCE;
         }
         if (status) status[e] = s;
         if (s && !rc) rc = s;
     }
     if (st) {
-        st->seconds_assemble += t1 - t0;
+        st->seconds_factor += t_first - t0;
+        st->seconds_solve += now_s() - t_first;
         st->refinement_sweeps = std::max(st->refinement_sweeps, sweeps);
         st->max_correction = std::max(st->max_correction, worst);
         st->batches += 1;
@@ -1306,6 +1838,10 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->solver = solver;
     p->rdirec = d->direct_cutoff;
     p->max_batch = d->max_batch;
+    if (const char* ob = std::getenv("REFEFF_FMS_LU_BLOCK")) {
+        const int v = std::atoi(ob);
+        if (v >= kPanel && v % kPanel == 0 && v <= 1024) p->outer_block = v;
+    }
     const int absorber = d->potentials[0];
     p->pstart = (d->lfms == 0) ? absorber : 0;
     p->pend = (d->lfms == 0) ? absorber : p->max_pot;
@@ -1362,6 +1898,9 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
               dupload(&p->d_ipart, p->ipart.data(), p->ipart.size()) && dupload(&p->d_xnlm, xn.data(), xn.size()) &&
               dupload(&p->d_so_plus, sp.data(), sp.size()) && dupload(&p->d_so_minus, sm.data(), sm.size()) &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
+    if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
+    for (int i = 0; ok && i < p->nstreams; ++i)
+        ok = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
         refeff_fms_plan_destroy(p);
         (void)hipGetLastError();
@@ -1387,6 +1926,8 @@ void refeff_fms_plan_destroy(refeff_fms_plan* p) {
     dfree(p->d_ipart); dfree(p->d_xnlm); dfree(p->d_so_plus); dfree(p->d_so_minus);
     for (hipEvent_t ev : p->pool) (void)hipEventDestroy(ev);
     if (p->stream) (void)hipStreamDestroy(p->stream);
+    for (hipStream_t s : p->streams)
+        if (s) (void)hipStreamDestroy(s);
     delete p;
 }
 
@@ -1423,6 +1964,8 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
         if (hipMemcpyAsync(p->d_k, k + 2 * kst * e0, 2 * kst * nb * sizeof(double), kind, p->stream) != hipSuccess ||
             hipMemcpyAsync(p->d_ph, ph + 2 * pst * e0, 2 * pst * nb * sizeof(double), kind, p->stream) != hipSuccess)
             return REFEFF_FMS_DEVICE_ERROR;
+        // the sub-batch streams read the inputs
+        if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
         double2* out = device ? reinterpret_cast<double2*>(sc) + sst * e0 : p->d_out;
         r = solve_device_batch(p, nb, out, status ? status + e0 : nullptr, st);
         if (r == REFEFF_FMS_DEVICE_ERROR) return r;
@@ -1463,14 +2006,14 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
     if (!rc && hipDeviceSynchronize() != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
     if (!rc && use_double) {
         if (!dupload(&p.d_a64, reinterpret_cast<const double2*>(a), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
-        if (!rc) run_lu<double>(&p, p.d_a64, 1);
+        if (!rc) run_lu<double>(&p, Group{p.stream, 0, 1}, p.d_a64, p.d_ipiv, p.d_perm, p.d_info);
         if (!rc && hipStreamSynchronize(p.stream) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc && hipMemcpy(a, p.d_a64, nn * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
     } else if (!rc) {
         af.resize(nn);
         for (size_t i = 0; i < nn; ++i) af[i] = make_float2((float)a[2 * i], (float)a[2 * i + 1]);
         if (!dupload(&p.d_a32, af.data(), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
-        if (!rc) run_lu<float>(&p, p.d_a32, 1);
+        if (!rc) run_lu<float>(&p, Group{p.stream, 0, 1}, p.d_a32, p.d_ipiv, p.d_perm, p.d_info);
         if (!rc && hipStreamSynchronize(p.stream) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc && hipMemcpy(af.data(), p.d_a32, nn * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         for (size_t i = 0; !rc && i < nn; ++i) {
@@ -1488,7 +2031,7 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
 int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, const double* A, const double* B,
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
-    if (M <= 0 || N <= 0 || K <= 0 || K % 32 || !A || !B || !C) return REFEFF_FMS_INVALID_ARGUMENT;
+    if (M <= 0 || N <= 0 || K <= 0 || K % ((use_double == 0 || use_double == 3) ? 32 : 16) || !A || !B || !C) return REFEFF_FMS_INVALID_ARGUMENT;
     const int ld = std::max(std::max(M, N), K);
     const size_t sz = static_cast<size_t>(ld) * ld;
     int rc = REFEFF_FMS_OK;
@@ -1506,8 +2049,21 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
         if (!dupload(&da, ha.data(), sz) || !dupload(&db, hb.data(), sz) || !dupload(&dc, hc.data(), sz)) {
             rc = REFEFF_FMS_DEVICE_ERROR;
         } else {
-            const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-            hipLaunchKernelGGL(k_gemm_sub<T>, dim3(tiles, 1), dim3(256), 0, nullptr, dc, da, db, sz, ld, M, N, K);
+            if (use_double == 3 && sizeof(T) == 4) {
+                const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+                hipLaunchKernelGGL(k_gemm_v2, dim3(tiles, 1), dim3(256), 0, nullptr, reinterpret_cast<float2*>(dc), ld,
+                                   sz, reinterpret_cast<const float2*>(da), ld, sz, reinterpret_cast<const float2*>(db),
+                                   ld, sz, M, N, K);
+            } else if (use_double == 2 && sizeof(T) == 4) {
+                const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
+                hipLaunchKernelGGL(k_gemm_big, dim3(tiles, 1), dim3(256), 0, nullptr, reinterpret_cast<float2*>(dc), ld,
+                                   sz, reinterpret_cast<const float2*>(da), ld, sz, reinterpret_cast<const float2*>(db),
+                                   ld, sz, M, N, K);
+            } else {
+                const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+                hipLaunchKernelGGL(k_gemm_sub<T>, dim3(tiles, 1), dim3(256), 0, nullptr, dc, ld, sz, da, ld, sz, db, ld,
+                                   sz, M, N, K);
+            }
             if (hipMemcpy(hc.data(), dc, sz * sizeof(T2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
             for (int i = 0; i < M; ++i)
                 for (int j = 0; j < N; ++j) {
@@ -1519,8 +2075,67 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
         dfree(db);
         dfree(dc);
     };
-    if (use_double) run(0.0);
+    if (use_double == 1) run(0.0);
     else run(0.0f);
+    return rc;
+}
+
+// Times `iters` launches of one GEMM kernel on a batch of zero-filled
+// matrices (kernel 0: 64x64 FP32, 1: 64x64 FP64, 2: 128x128 FP32).
+int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K, int32_t batch, int32_t iters,
+                                double* ms) {
+    if (M <= 0 || N <= 0 || K <= 0 || K % 32 || batch <= 0 || iters <= 0 || !ms) return REFEFF_FMS_INVALID_ARGUMENT;
+    const int ld = std::max(std::max(M, N), K);
+    const size_t per = static_cast<size_t>(ld) * ld;
+    const size_t esz = (kernel == 1) ? sizeof(double2) : sizeof(float2);
+    void *da = nullptr, *db = nullptr, *dc = nullptr;
+    if (hipMalloc(&da, per * batch * esz) != hipSuccess || hipMalloc(&db, per * batch * esz) != hipSuccess ||
+        hipMalloc(&dc, per * batch * esz) != hipSuccess) {
+        if (da) (void)hipFree(da);
+        if (db) (void)hipFree(db);
+        if (dc) (void)hipFree(dc);
+        return REFEFF_FMS_NO_MEMORY;
+    }
+    (void)hipMemset(da, 0, per * batch * esz);
+    (void)hipMemset(db, 0, per * batch * esz);
+    (void)hipMemset(dc, 0, per * batch * esz);
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    auto one = [&]() {
+        if (kernel == 3) {
+            const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+            hipLaunchKernelGGL(k_gemm_v2, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc), ld, per,
+                               static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K);
+        } else if (kernel == 2) {
+            const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
+            hipLaunchKernelGGL(k_gemm_big, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc), ld, per,
+                               static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K);
+        } else if (kernel == 1) {
+            const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+            hipLaunchKernelGGL(k_gemm_sub<double>, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<double2*>(dc),
+                               ld, per, static_cast<const double2*>(da), ld, per, static_cast<const double2*>(db), ld, per,
+                               M, N, K);
+        } else {
+            const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+            hipLaunchKernelGGL(k_gemm_sub<float>, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc),
+                               ld, per, static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per,
+                               M, N, K);
+        }
+    };
+    one();
+    (void)hipEventRecord(a, nullptr);
+    for (int i = 0; i < iters; ++i) one();
+    (void)hipEventRecord(b, nullptr);
+    int rc = (hipEventSynchronize(b) == hipSuccess) ? REFEFF_FMS_OK : REFEFF_FMS_DEVICE_ERROR;
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, a, b);
+    *ms = t / iters;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dc);
     return rc;
 }
 
