@@ -1203,6 +1203,194 @@ __global__ __launch_bounds__(256) void k_trsv_block(const typename Cx<T>::type* 
     for (int i = tid; i < BS * nr; i += 256) X[static_cast<size_t>(k0 + i / nr) * ldx + i % nr] = xb[i / nr][i % nr];
 }
 
+// Inverses of the 32 x 32 diagonal blocks of L (unit lower) and U, one
+// workgroup per (block, energy): lanes 0-31 build column c of inv(L_bb),
+// lanes 32-63 column c of inv(U_bb), by substitution in LDS.  Stored as
+// dinv[e][0 = L, 1 = U][block][32][32] (row-major).
+template <typename T>
+__global__ __launch_bounds__(64) void k_trtri_diag(const typename Cx<T>::type* LUb, size_t sa, int lda,
+                                                   typename Cx<T>::type* dinvb, int n) {
+    using T2 = typename Cx<T>::type;
+    constexpr int BS = kPanel;
+    __shared__ T2 d[BS][BS + 1];
+    __shared__ T2 y[2][BS][BS + 1];
+    const int blk = blockIdx.x, e = blockIdx.y, tid = threadIdx.x;
+    const int R0 = blk * BS, nblk = n / BS;
+    const T2* LU = LUb + e * sa;
+    for (int i = tid; i < BS * BS; i += 64) d[i / BS][i % BS] = LU[static_cast<size_t>(R0 + i / BS) * lda + R0 + i % BS];
+    __syncthreads();
+    const int c = tid & (BS - 1);
+    if (tid < BS) {
+        for (int r = 0; r < BS; ++r) {
+            T2 v = T2{r == c ? T(1) : T(0), 0};
+            for (int k = c; k < r; ++k) v = cfms(v, d[r][k], y[0][k][c]);
+            y[0][r][c] = r < c ? T2{0, 0} : v;
+        }
+    } else {
+        for (int r = BS - 1; r >= 0; --r) {
+            T2 v = T2{r == c ? T(1) : T(0), 0};
+            for (int k = r + 1; k <= c; ++k) v = cfms(v, d[r][k], y[1][k][c]);
+            y[1][r][c] = r > c ? T2{0, 0} : cdiv(v, d[r][r]);
+        }
+    }
+    __syncthreads();
+    const size_t blk_sz = static_cast<size_t>(BS) * BS;
+    T2* outL = dinvb + (static_cast<size_t>(e) * 2 * nblk + blk) * blk_sz;
+    T2* outU = outL + nblk * blk_sz;
+    for (int i = tid; i < BS * BS; i += 64) {
+        outL[i] = y[0][i / BS][i % BS];
+        outU[i] = y[1][i / BS][i % BS];
+    }
+}
+
+// Whole triangular solve with one workgroup per (energy, right-hand-side
+// chunk of up to 16*NT columns): X <- L^-1 X (unit lower, forward) or
+// X <- U^-1 X (backward).  Per 32-row block the eight waves split the
+// off-diagonal K range of the block row and accumulate 32 x NC complex partial
+// products with MFMA straight from global memory (the next 16-wide K chunk is
+// prefetched while the current one multiplies); the partials are reduced in a
+// fixed order through LDS and the block is finished as X_b = inv(D_b) (X_b - S)
+// with the precomputed inverse diagonal block.  L/U is read once per solve.
+template <typename T, bool LOWER, int NT>
+__global__ __launch_bounds__(512) void k_trsm_wg(const typename Cx<T>::type* LUb, size_t sa, int lda,
+                                                 const typename Cx<T>::type* dinvb, typename Cx<T>::type* Xb,
+                                                 size_t sx, int ldx, int n) {
+    using T2 = typename Cx<T>::type;
+    using acc_t = typename Mfma<T>::acc;
+    constexpr int BS = kPanel, NW = 8, NC = NT * 16;
+    __shared__ T2 red[4][BS][NC];
+    __shared__ T2 dia[BS][BS + 1];
+    const int e = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 15, q = lane >> 4;
+    const int c0 = blockIdx.y * NC;
+    const int ncol = min(NC, ldx - c0);
+    const T2* LU = LUb + e * sa;
+    T2* X = Xb + e * sx;
+    const int nblk = n / BS;
+    const size_t blk_sz = static_cast<size_t>(BS) * BS;
+    const T2* dinv = dinvb + (static_cast<size_t>(e) * 2 + (LOWER ? 0 : 1)) * nblk * blk_sz;
+    for (int step = 0; step < nblk; ++step) {
+        const int blk = LOWER ? step : nblk - 1 - step;
+        const int R0 = blk * BS;
+        for (int i = tid; i < BS * BS; i += 512) dia[i / BS][i % BS] = dinv[blk * blk_sz + i];
+        const int kb = LOWER ? 0 : R0 + BS;
+        const int nch = ((LOWER ? R0 : n) - kb) / 16;
+        acc_t cr[2][NT], ci[2][NT];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+                cr[a][b] = acc_t{0, 0, 0, 0};
+                ci[a][b] = acc_t{0, 0, 0, 0};
+            }
+        T2 av[2][4], xv[NT][4];
+        auto load = [&](int t, T2 (&aa)[2][4], T2 (&xx)[NT][4]) {
+            const int k0 = kb + 16 * t + 4 * q;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const T2* pa = LU + static_cast<size_t>(R0 + 16 * a + m) * lda + k0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) aa[a][u] = pa[u];
+            }
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+                const int col = c0 + 16 * b + m;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    xx[b][u] = (16 * b + m < ncol) ? X[static_cast<size_t>(k0 + u) * ldx + col] : T2{0, 0};
+            }
+        };
+        int t = wave;
+        if (t < nch) load(t, av, xv);
+        for (; t < nch; t += NW) {
+            T2 an[2][4], xn[NT][4];
+            const bool more = t + NW < nch;
+            if (more) load(t + NW, an, xn);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) {
+                        cr[a][b] = Mfma<T>::run(av[a][u].x, xv[b][u].x, cr[a][b]);
+                        ci[a][b] = Mfma<T>::run(av[a][u].x, xv[b][u].y, ci[a][b]);
+                        cr[a][b] = Mfma<T>::run(-av[a][u].y, xv[b][u].y, cr[a][b]);
+                        ci[a][b] = Mfma<T>::run(av[a][u].y, xv[b][u].x, ci[a][b]);
+                    }
+            if (more) {
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) av[a][u] = an[a][u];
+#pragma unroll
+                for (int b = 0; b < NT; ++b)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) xv[b][u] = xn[b][u];
+            }
+        }
+        // fixed-order tree over the waves: (w, w+4), then (w, w+2), then (0, 1)
+        auto put = [&](int slot) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * a + (sizeof(T) == 4 ? 4 * q + r : q + 4 * r);
+                        red[slot][row][16 * b + m] = T2{cr[a][b][r], ci[a][b][r]};
+                    }
+        };
+        auto add = [&](int slot) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * a + (sizeof(T) == 4 ? 4 * q + r : q + 4 * r);
+                        const T2 v = red[slot][row][16 * b + m];
+                        cr[a][b][r] += v.x;
+                        ci[a][b][r] += v.y;
+                    }
+        };
+        if (wave >= 4) put(wave - 4);
+        __syncthreads();
+        if (wave < 4) add(wave);
+        __syncthreads();
+        if (wave == 2 || wave == 3) put(wave - 2);
+        __syncthreads();
+        if (wave < 2) add(wave);
+        __syncthreads();
+        if (wave == 1) put(0);
+        __syncthreads();
+        if (wave == 0) {
+            add(0);
+            put(1);
+        }
+        __syncthreads();
+        // v = X_b - S into red[2]
+        for (int i = tid; i < BS * NC; i += 512) {
+            const int r = i / NC, c = i % NC;
+            red[2][r][c] = c < ncol ? csub(X[static_cast<size_t>(R0 + r) * ldx + c0 + c], red[1][r][c]) : T2{0, 0};
+        }
+        __syncthreads();
+        for (int i = tid; i < BS * NC; i += 512) {
+            const int r = i / NC, c = i % NC;
+            if (c < ncol) {
+                T2 acc{0, 0};
+                const int klo = LOWER ? 0 : r, khi = LOWER ? r : BS - 1;
+                for (int k = klo; k <= khi; ++k) {
+                    const T2 dv = dia[r][k], vv = red[2][k][c];
+                    acc.x += dv.x * vv.x - dv.y * vv.y;
+                    acc.y += dv.x * vv.y + dv.y * vv.x;
+                }
+                X[static_cast<size_t>(R0 + r) * ldx + c0 + c] = acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // R = B - A X in FP64 (64 rows per workgroup); tracks max |R| per energy.
 __global__ __launch_bounds__(256) void k_residual(const double2* Ab, size_t sa, int lda, const double2* Xb,
                                                   const double2* Bb, double2* Rb, size_t sx, int n, int nrhs,
@@ -1371,6 +1559,7 @@ struct refeff_fms_plan {
     double2* d_a64 = nullptr;
     double2 *d_b64 = nullptr, *d_x64 = nullptr, *d_r64 = nullptr;
     void* d_w = nullptr;   // solve work block (float2 or double2)
+    void* d_dinv = nullptr;  // inverse 32x32 diagonal blocks of L and U (float2 or double2)
     int *d_ipiv = nullptr, *d_perm = nullptr, *d_info = nullptr;
     double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_norms = nullptr;
@@ -1402,6 +1591,8 @@ void free_workspace(refeff_fms_plan* p) {
     dfree(p->d_a32); dfree(p->d_a64); dfree(p->d_b64); dfree(p->d_x64); dfree(p->d_r64);
     if (p->d_w) (void)hipFree(p->d_w);
     p->d_w = nullptr;
+    if (p->d_dinv) (void)hipFree(p->d_dinv);
+    p->d_dinv = nullptr;
     dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
     dfree(p->d_out); dfree(p->d_norms);
     p->ws_batch = 0;
@@ -1464,7 +1655,9 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
     const size_t n2 = static_cast<size_t>(p->npad) * p->npad;
     const size_t nr = static_cast<size_t>(p->npad) * p->nrhs;
     const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
-    size_t b = n2 * sizeof(double2) + 3 * nr * sizeof(double2) + nr * (mixed ? sizeof(float2) : sizeof(double2));
+    const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
+    size_t b = n2 * sizeof(double2) + 3 * nr * sizeof(double2) + nr * cs;
+    b += 2 * static_cast<size_t>(p->npad) * kPanel * cs;
     if (mixed) b += n2 * sizeof(float2);
     b += 2 * p->npad * sizeof(int) + 2 * p->n * sizeof(double2);
     b += static_cast<size_t>(p->ch) * p->ch * p->npot * sizeof(double2);
@@ -1481,7 +1674,9 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     if (mixed) ok = ok && dalloc(&p->d_a32, n2 * nb);
     ok = ok && dalloc(&p->d_a64, n2 * nb) && dalloc(&p->d_b64, nr * nb) && dalloc(&p->d_x64, nr * nb) &&
          dalloc(&p->d_r64, nr * nb);
-    ok = ok && hipMalloc(&p->d_w, nr * nb * (mixed ? sizeof(float2) : sizeof(double2))) == hipSuccess;
+    const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
+    ok = ok && hipMalloc(&p->d_w, nr * nb * cs) == hipSuccess;
+    ok = ok && hipMalloc(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs) == hipSuccess;
     ok = ok && dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
          dalloc(&p->d_info, nb) && dalloc(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb) &&
          dalloc(&p->d_k, static_cast<size_t>(p->nsp) * nb) &&
@@ -1604,27 +1799,49 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
     });
 }
 
-// X <- U^-1 L^-1 X (X already row-permuted), 32-row blocks: block solve, then
-// an MFMA update of the rows still to be solved.
+// Inverse diagonal blocks of a fresh LU factorisation (used by every solve).
 template <typename T>
-void run_solve(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, typename Cx<T>::type* X) {
+void trtri_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, typename Cx<T>::type* dinv) {
+    const int n = p->npad;
+    const size_t sa = static_cast<size_t>(n) * n;
+    const double cb = sizeof(typename Cx<T>::type);
+    launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * n * kPanel * kPanel / 3 * g.nb, 3.0 * n * kPanel * cb * g.nb, [&] {
+        hipLaunchKernelGGL(k_trtri_diag<T>, dim3(n / kPanel, g.nb), dim3(64), 0, g.s, LU, sa, n, dinv, n);
+    });
+}
+
+// RHS chunk width per solve workgroup: 64 columns in FP32, 32 in FP64.
+template <typename T> constexpr int solve_chunk() { return sizeof(T) == 4 ? 64 : 32; }
+
+// X <- U^-1 L^-1 X (X already row-permuted): one workgroup per energy and
+// right-hand-side chunk walks each triangle once.
+template <typename T, bool LOWER>
+void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
+                 typename Cx<T>::type* X) {
     const int n = p->npad, nr = p->nrhs, nb = g.nb;
     const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
-    const int chunks = (nr + kRhsChunk - 1) / kRhsChunk;
+    constexpr int CW = solve_chunk<T>();
+    const int nt = (std::min(nr, CW) + 15) / 16;
+    const int ntile = nt == 1 ? 1 : nt == 2 ? 2 : 4;
+    const int chunks = (nr + 16 * ntile - 1) / (16 * ntile);
     const double cb = sizeof(typename Cx<T>::type);
-    for (int k0 = 0; k0 < n; k0 += kPanel) {
-        launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * kPanel * kPanel / 2 * nr * nb, kPanel * kPanel * cb * nb, [&] {
-            hipLaunchKernelGGL((k_trsv_block<T, true>), dim3(nb, chunks), dim3(256), 0, g.s, LU, sa, n, X, sx, nr, k0);
-        });
-        gemm_sub<T>(p, g, X + static_cast<size_t>(k0 + kPanel) * nr, nr, sx, LU + static_cast<size_t>(k0 + kPanel) * n + k0,
-                    n, sa, X + static_cast<size_t>(k0) * nr, nr, sx, n - k0 - kPanel, nr, kPanel);
-    }
-    for (int k0 = n - kPanel; k0 >= 0; k0 -= kPanel) {
-        launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * kPanel * kPanel / 2 * nr * nb, kPanel * kPanel * cb * nb, [&] {
-            hipLaunchKernelGGL((k_trsv_block<T, false>), dim3(nb, chunks), dim3(256), 0, g.s, LU, sa, n, X, sx, nr, k0);
-        });
-        gemm_sub<T>(p, g, X, nr, sx, LU + k0, n, sa, X + static_cast<size_t>(k0) * nr, nr, sx, k0, nr, kPanel);
-    }
+    const double nn = static_cast<double>(n) * n;
+    launch(p, g.s, REFEFF_FMS_K_SOLVE, 4.0 * nn * nr * nb, (nn / 2 * chunks + 2.0 * n * nr) * cb * nb, [&] {
+        const dim3 grid(nb, chunks), block(512);
+        if (ntile == 1)
+            hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1>), grid, block, 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+        else if (ntile == 2)
+            hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2>), grid, block, 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+        else if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4>), grid, block, 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+    });
+}
+
+template <typename T>
+void run_solve(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
+               typename Cx<T>::type* X) {
+    trsm_launch<T, true>(p, g, LU, dinv, X);
+    trsm_launch<T, false>(p, g, LU, dinv, X);
 }
 
 double now_s() {
@@ -1692,10 +1909,14 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         }
         int* ipiv = p->d_ipiv + static_cast<size_t>(n) * g.e0;
         int* perm = p->d_perm + static_cast<size_t>(n) * g.e0;
-        if (mixed)
+        const size_t sd = 2 * static_cast<size_t>(n) * kPanel * g.e0;
+        if (mixed) {
             run_lu<float>(p, g, p->d_a32 + sa * g.e0, ipiv, perm, p->d_info + g.e0);
-        else
+            trtri_launch<float>(p, g, p->d_a32 + sa * g.e0, static_cast<float2*>(p->d_dinv) + sd);
+        } else {
             run_lu<double>(p, g, p->d_a64 + sa * g.e0, ipiv, perm, p->d_info + g.e0);
+            trtri_launch<double>(p, g, p->d_a64 + sa * g.e0, static_cast<double2*>(p->d_dinv) + sd);
+        }
     }
     int sweeps = 0;
     double worst = 0.0;
@@ -1727,7 +1948,9 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                 }
                 hipLaunchKernelGGL(k_gather<float>, dim3(gx, g.nb), dim3(256), 0, g.s, rhs, sx,
                                    p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0);
-                run_solve<float>(p, g, p->d_a32 + sa * g.e0, w + sx * g.e0);
+                run_solve<float>(p, g, p->d_a32 + sa * g.e0,
+                                 static_cast<const float2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
+                                 w + sx * g.e0);
                 hipLaunchKernelGGL(k_update<float>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_x64 + sx * g.e0,
                                    w + sx * g.e0, sx, n, p->nrhs, gn);
                 (void)hipMemcpyAsync(norms.data() + 4 * static_cast<size_t>(g.e0), gn, 4 * g.nb * sizeof(double),
@@ -1754,7 +1977,9 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         for (const auto& g : gs) {
             hipLaunchKernelGGL(k_gather<double>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, sx,
                                p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, p->d_x64 + sx * g.e0);
-            run_solve<double>(p, g, p->d_a64 + sa * g.e0, p->d_x64 + sx * g.e0);
+            run_solve<double>(p, g, p->d_a64 + sa * g.e0,
+                              static_cast<const double2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
+                              p->d_x64 + sx * g.e0);
         }
         t_first = now_s();
     }
