@@ -144,7 +144,9 @@ typedef struct refeff_fms_stats {
     double seconds_solve;        /* triangular solves + refinement */
     int32_t refinement_sweeps;   /* largest number of FP64 refinement sweeps over the energies */
     int32_t batches;
-    double max_correction;       /* largest final |dx|/|x| over energies (mixed solver) */
+    double max_correction;       /* largest estimated relative error left in x after refinement:
+                                    d_s * min(1, d_s / d_{s-1}) from the last two corrections
+                                    |dx|/|x| (mixed solver) */
 } refeff_fms_stats;
 
 /* Build a plan (uploads geometry to the current HIP device).  Mirrors

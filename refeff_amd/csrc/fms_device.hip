@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -1162,46 +1163,6 @@ __global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, con
     }
 }
 
-// One 32-row block of the forward (unit lower, LOWER=true) or backward (upper)
-// substitution, in place on X rows [k0, k0+32); the coupling to the other
-// blocks is applied by k_gemm_sub.  Grid: (energy, RHS chunk).
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_block(const typename Cx<T>::type* LUb, size_t sa, int lda,
-                                                    typename Cx<T>::type* Xb, size_t sx, int ldx, int k0) {
-    using T2 = typename Cx<T>::type;
-    constexpr int BS = kPanel;
-    __shared__ T2 tri[BS][BS + 1];
-    __shared__ T2 xb[BS][kRhsChunk];
-    const int e = blockIdx.x, tid = threadIdx.x;
-    const int c0 = blockIdx.y * kRhsChunk;
-    const int nr = min(kRhsChunk, ldx - c0);
-    const T2* LU = LUb + e * sa;
-    T2* X = Xb + e * sx + c0;
-    for (int i = tid; i < BS * BS; i += 256) {
-        const int r = i / BS, q = i % BS;
-        tri[r][q] = LU[static_cast<size_t>(k0 + r) * lda + k0 + q];
-    }
-    for (int i = tid; i < BS * nr; i += 256) xb[i / nr][i % nr] = X[static_cast<size_t>(k0 + i / nr) * ldx + i % nr];
-    __syncthreads();
-    if (tid < nr) {
-        const int c = tid;
-        if (LOWER) {
-            for (int r = 1; r < BS; ++r) {
-                T2 acc = xb[r][c];
-                for (int q = 0; q < r; ++q) acc = cfms(acc, tri[r][q], xb[q][c]);
-                xb[r][c] = acc;
-            }
-        } else {
-            for (int r = BS - 1; r >= 0; --r) {
-                T2 acc = xb[r][c];
-                for (int q = r + 1; q < BS; ++q) acc = cfms(acc, tri[r][q], xb[q][c]);
-                xb[r][c] = cdiv(acc, tri[r][r]);
-            }
-        }
-    }
-    __syncthreads();
-    for (int i = tid; i < BS * nr; i += 256) X[static_cast<size_t>(k0 + i / nr) * ldx + i % nr] = xb[i / nr][i % nr];
-}
 
 // Inverses of the 32 x 32 diagonal blocks of L (unit lower) and U, one
 // workgroup per (block, energy): lanes 0-31 build column c of inv(L_bb),
@@ -1243,25 +1204,134 @@ __global__ __launch_bounds__(64) void k_trtri_diag(const typename Cx<T>::type* L
     }
 }
 
+// S = A[R0:R0+32, kb:kb+16*nch] X[kb:kb+16*nch, c0:c0+16*NT] for one 32-row
+// block, the eight waves of the workgroup taking 16-wide K chunks in turn:
+// complex MFMA straight from global memory, the next chunk prefetched while
+// the current one multiplies.  Each wave leaves its partial sum in cr/ci.
+template <typename T, int NT>
+__device__ __forceinline__ void row_block_product(const typename Cx<T>::type* A, int lda, int R0, int kb, int nch,
+                                                  const typename Cx<T>::type* X, int ldx, int c0, int ncol,
+                                                  typename Mfma<T>::acc (&cr)[2][NT],
+                                                  typename Mfma<T>::acc (&ci)[2][NT]) {
+    using T2 = typename Cx<T>::type;
+    using acc_t = typename Mfma<T>::acc;
+    constexpr int NW = 8;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            cr[a][b] = acc_t{0, 0, 0, 0};
+            ci[a][b] = acc_t{0, 0, 0, 0};
+        }
+    T2 av[2][4], xv[NT][4];
+    auto load = [&](int t, T2 (&aa)[2][4], T2 (&xx)[NT][4]) {
+        const int k0 = kb + 16 * t + 4 * q;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const T2* pa = A + static_cast<size_t>(R0 + 16 * a + m) * lda + k0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) aa[a][u] = pa[u];
+        }
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            const int col = c0 + 16 * b + m;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                xx[b][u] = (16 * b + m < ncol) ? X[static_cast<size_t>(k0 + u) * ldx + col] : T2{0, 0};
+        }
+    };
+    int t = wave;
+    if (t < nch) load(t, av, xv);
+    for (; t < nch; t += NW) {
+        T2 an[2][4], xn[NT][4];
+        const bool more = t + NW < nch;
+        if (more) load(t + NW, an, xn);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b) {
+                    cr[a][b] = Mfma<T>::run(av[a][u].x, xv[b][u].x, cr[a][b]);
+                    ci[a][b] = Mfma<T>::run(av[a][u].x, xv[b][u].y, ci[a][b]);
+                    cr[a][b] = Mfma<T>::run(-av[a][u].y, xv[b][u].y, cr[a][b]);
+                    ci[a][b] = Mfma<T>::run(av[a][u].y, xv[b][u].x, ci[a][b]);
+                }
+        if (more) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) av[a][u] = an[a][u];
+#pragma unroll
+            for (int b = 0; b < NT; ++b)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xv[b][u] = xn[b][u];
+        }
+    }
+}
+
+// Sums the eight waves' partial products in a fixed order ((w, w+4), then
+// (w, w+2), then (0, 1)) through LDS; the 32 x 16*NT total lands in red[1].
+template <typename T, int NT>
+__device__ __forceinline__ void reduce_waves(typename Cx<T>::type (&red)[4][kPanel][NT * 16],
+                                             typename Mfma<T>::acc (&cr)[2][NT], typename Mfma<T>::acc (&ci)[2][NT]) {
+    using T2 = typename Cx<T>::type;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
+    auto row_of = [&](int a, int r) { return 16 * a + (sizeof(T) == 4 ? 4 * q + r : q + 4 * r); };
+    auto put = [&](int slot) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) red[slot][row_of(a, r)][16 * b + m] = T2{cr[a][b][r], ci[a][b][r]};
+    };
+    auto add = [&](int slot) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const T2 v = red[slot][row_of(a, r)][16 * b + m];
+                    cr[a][b][r] += v.x;
+                    ci[a][b][r] += v.y;
+                }
+    };
+    if (wave >= 4) put(wave - 4);
+    __syncthreads();
+    if (wave < 4) add(wave);
+    __syncthreads();
+    if (wave == 2 || wave == 3) put(wave - 2);
+    __syncthreads();
+    if (wave < 2) add(wave);
+    __syncthreads();
+    if (wave == 1) put(0);
+    __syncthreads();
+    if (wave == 0) {
+        add(0);
+        put(1);
+    }
+    __syncthreads();
+}
+
 // Whole triangular solve with one workgroup per (energy, right-hand-side
 // chunk of up to 16*NT columns): X <- L^-1 X (unit lower, forward) or
-// X <- U^-1 X (backward).  Per 32-row block the eight waves split the
-// off-diagonal K range of the block row and accumulate 32 x NC complex partial
-// products with MFMA straight from global memory (the next 16-wide K chunk is
-// prefetched while the current one multiplies); the partials are reduced in a
-// fixed order through LDS and the block is finished as X_b = inv(D_b) (X_b - S)
-// with the precomputed inverse diagonal block.  L/U is read once per solve.
+// X <- U^-1 X (backward).  Per 32-row block the off-diagonal part of the block
+// row times the already solved rows is formed with row_block_product, and the
+// block is finished as X_b = inv(D_b) (X_b - S) with the precomputed inverse
+// diagonal block.  L/U is read once per solve.
 template <typename T, bool LOWER, int NT>
 __global__ __launch_bounds__(512) void k_trsm_wg(const typename Cx<T>::type* LUb, size_t sa, int lda,
                                                  const typename Cx<T>::type* dinvb, typename Cx<T>::type* Xb,
                                                  size_t sx, int ldx, int n) {
     using T2 = typename Cx<T>::type;
     using acc_t = typename Mfma<T>::acc;
-    constexpr int BS = kPanel, NW = 8, NC = NT * 16;
+    constexpr int BS = kPanel, NC = NT * 16;
     __shared__ T2 red[4][BS][NC];
     __shared__ T2 dia[BS][BS + 1];
-    const int e = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int m = lane & 15, q = lane >> 4;
+    const int e = blockIdx.x, tid = threadIdx.x;
     const int c0 = blockIdx.y * NC;
     const int ncol = min(NC, ldx - c0);
     const T2* LU = LUb + e * sa;
@@ -1276,98 +1346,8 @@ __global__ __launch_bounds__(512) void k_trsm_wg(const typename Cx<T>::type* LUb
         const int kb = LOWER ? 0 : R0 + BS;
         const int nch = ((LOWER ? R0 : n) - kb) / 16;
         acc_t cr[2][NT], ci[2][NT];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < NT; ++b) {
-                cr[a][b] = acc_t{0, 0, 0, 0};
-                ci[a][b] = acc_t{0, 0, 0, 0};
-            }
-        T2 av[2][4], xv[NT][4];
-        auto load = [&](int t, T2 (&aa)[2][4], T2 (&xx)[NT][4]) {
-            const int k0 = kb + 16 * t + 4 * q;
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const T2* pa = LU + static_cast<size_t>(R0 + 16 * a + m) * lda + k0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) aa[a][u] = pa[u];
-            }
-#pragma unroll
-            for (int b = 0; b < NT; ++b) {
-                const int col = c0 + 16 * b + m;
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    xx[b][u] = (16 * b + m < ncol) ? X[static_cast<size_t>(k0 + u) * ldx + col] : T2{0, 0};
-            }
-        };
-        int t = wave;
-        if (t < nch) load(t, av, xv);
-        for (; t < nch; t += NW) {
-            T2 an[2][4], xn[NT][4];
-            const bool more = t + NW < nch;
-            if (more) load(t + NW, an, xn);
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int b = 0; b < NT; ++b) {
-                        cr[a][b] = Mfma<T>::run(av[a][u].x, xv[b][u].x, cr[a][b]);
-                        ci[a][b] = Mfma<T>::run(av[a][u].x, xv[b][u].y, ci[a][b]);
-                        cr[a][b] = Mfma<T>::run(-av[a][u].y, xv[b][u].y, cr[a][b]);
-                        ci[a][b] = Mfma<T>::run(av[a][u].y, xv[b][u].x, ci[a][b]);
-                    }
-            if (more) {
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) av[a][u] = an[a][u];
-#pragma unroll
-                for (int b = 0; b < NT; ++b)
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) xv[b][u] = xn[b][u];
-            }
-        }
-        // fixed-order tree over the waves: (w, w+4), then (w, w+2), then (0, 1)
-        auto put = [&](int slot) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < NT; ++b)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * a + (sizeof(T) == 4 ? 4 * q + r : q + 4 * r);
-                        red[slot][row][16 * b + m] = T2{cr[a][b][r], ci[a][b][r]};
-                    }
-        };
-        auto add = [&](int slot) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < NT; ++b)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * a + (sizeof(T) == 4 ? 4 * q + r : q + 4 * r);
-                        const T2 v = red[slot][row][16 * b + m];
-                        cr[a][b][r] += v.x;
-                        ci[a][b][r] += v.y;
-                    }
-        };
-        if (wave >= 4) put(wave - 4);
-        __syncthreads();
-        if (wave < 4) add(wave);
-        __syncthreads();
-        if (wave == 2 || wave == 3) put(wave - 2);
-        __syncthreads();
-        if (wave < 2) add(wave);
-        __syncthreads();
-        if (wave == 1) put(0);
-        __syncthreads();
-        if (wave == 0) {
-            add(0);
-            put(1);
-        }
-        __syncthreads();
+        row_block_product<T, NT>(LU, lda, R0, kb, nch, X, ldx, c0, ncol, cr, ci);
+        reduce_waves<T, NT>(red, cr, ci);
         // v = X_b - S into red[2]
         for (int i = tid; i < BS * NC; i += 512) {
             const int r = i / NC, c = i % NC;
@@ -1391,51 +1371,26 @@ __global__ __launch_bounds__(512) void k_trsm_wg(const typename Cx<T>::type* LUb
     }
 }
 
-// R = B - A X in FP64 (64 rows per workgroup); tracks max |R| per energy.
-__global__ __launch_bounds__(256) void k_residual(const double2* Ab, size_t sa, int lda, const double2* Xb,
-                                                  const double2* Bb, double2* Rb, size_t sx, int n, int nrhs,
-                                                  double* norms) {
-    constexpr int RB = 64, KB = 16;
-    extern __shared__ unsigned char smem_raw[];
-    double2* at = reinterpret_cast<double2*>(smem_raw);  // [RB][KB+1]
-    double2* xt = at + RB * (KB + 1);                     // [KB][nrhs]
-    const int e = blockIdx.y, r0 = blockIdx.x * RB, tid = threadIdx.x;
-    const int c0 = blockIdx.z * kRhsChunk;
-    const int ldx = nrhs;
-    nrhs = min(kRhsChunk, ldx - c0);
-    const double2* A = Ab + e * sa;
-    const double2* X = Xb + e * sx + c0;
-    const int r = tid >> 2, cg = tid & 3;
-    double2 acc[16];
-    for (int j = 0; j < 16; ++j) acc[j] = make_double2(0.0, 0.0);
-    for (int k0 = 0; k0 < n; k0 += KB) {
-        for (int i = tid; i < RB * KB; i += 256) {
-            const int rr = i / KB, kk = i % KB;
-            at[rr * (KB + 1) + kk] = A[static_cast<size_t>(r0 + rr) * lda + k0 + kk];
-        }
-        for (int i = tid; i < KB * nrhs; i += 256) xt[i] = X[static_cast<size_t>(k0 + i / nrhs) * ldx + i % nrhs];
-        __syncthreads();
-        for (int kk = 0; kk < KB; ++kk) {
-            const double2 av = at[r * (KB + 1) + kk];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int c = cg + 4 * j;
-                if (c < nrhs) {
-                    const double2 xv = xt[kk * nrhs + c];
-                    acc[j].x += av.x * xv.x - av.y * xv.y;
-                    acc[j].y += av.x * xv.y + av.y * xv.x;
-                }
-            }
-        }
-        __syncthreads();
-    }
+// R = B - A X in FP64 with FP64 MFMA, one workgroup per (32-row block, energy,
+// right-hand-side chunk); tracks max |R| per energy in norms[4 e].
+template <int NT>
+__global__ __launch_bounds__(512) void k_residual_wg(const double2* Ab, size_t sa, int lda, const double2* Xb,
+                                                     const double2* Bb, double2* Rb, size_t sx, int ldx, int n,
+                                                     double* norms) {
+    constexpr int BS = kPanel, NC = NT * 16;
+    __shared__ double2 red[4][BS][NC];
+    const int R0 = blockIdx.x * BS, e = blockIdx.y, tid = threadIdx.x;
+    const int c0 = blockIdx.z * NC;
+    const int ncol = min(NC, ldx - c0);
+    typename Mfma<double>::acc cr[2][NT], ci[2][NT];
+    row_block_product<double, NT>(Ab + e * sa, lda, R0, 0, n / 16, Xb + e * sx, ldx, c0, ncol, cr, ci);
+    reduce_waves<double, NT>(red, cr, ci);
     double mx = 0.0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int c = cg + 4 * j;
-        if (c < nrhs) {
-            const size_t o = e * sx + static_cast<size_t>(r0 + r) * ldx + c0 + c;
-            const double2 rv = make_double2(Bb[o].x - acc[j].x, Bb[o].y - acc[j].y);
+    for (int i = tid; i < BS * NC; i += 512) {
+        const int r = i / NC, c = i % NC;
+        if (c < ncol) {
+            const size_t o = e * sx + static_cast<size_t>(R0 + r) * ldx + c0 + c;
+            const double2 rv = csub(Bb[o], red[1][r][c]);
             Rb[o] = rv;
             mx = fmax(mx, fabs(rv.x) + fabs(rv.y));
         }
@@ -1565,6 +1520,7 @@ struct refeff_fms_plan {
     double* d_norms = nullptr;
     hipStream_t stream = nullptr;       // main stream (copies, single-group work)
     int nstreams = 1;                   // energy sub-batches issued on separate streams
+    bool trace = false;                 // REFEFF_FMS_TRACE: print refinement progress to stderr
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex mu;
     // kernel-level instrumentation (refeff_fms_plan_profile)
@@ -1922,6 +1878,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     double worst = 0.0;
     double t_first = 0.0;
     std::vector<double> norms(4 * static_cast<size_t>(nb));
+    std::vector<double> est_err(nb, 0.0);  // estimated relative error left after refinement
     const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
     if (mixed) {
         float2* w = static_cast<float2*>(p->d_w);
@@ -1937,12 +1894,19 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                                    static_cast<size_t>(4 * g.nb));
                 const double2* rhs = p->d_b64 + sx * g.e0;
                 if (sweep > 0) {
-                    const size_t shm = (64 * 17 + 16 * std::min(p->nrhs, kRhsChunk)) * sizeof(double2);
                     const double nn = static_cast<double>(n) * n;
-                    launch(p, g.s, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * p->nrhs * g.nb, 16.0 * nn * g.nb, [&] {
-                        hipLaunchKernelGGL(k_residual, dim3(n / 64, g.nb, (p->nrhs + kRhsChunk - 1) / kRhsChunk),
-                                           dim3(256), shm, g.s, p->d_a64 + sa * g.e0, sa, n, p->d_x64 + sx * g.e0,
-                                           p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, n, p->nrhs, gn);
+                    const int ntile = p->nrhs <= 16 ? 1 : 2;
+                    const int chunks = (p->nrhs + 16 * ntile - 1) / (16 * ntile);
+                    launch(p, g.s, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * p->nrhs * g.nb, 16.0 * nn * chunks * g.nb, [&] {
+                        const dim3 grid(n / kPanel, g.nb, chunks), block(512);
+                        if (ntile == 1)
+                            hipLaunchKernelGGL(k_residual_wg<1>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
+                                               p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
+                                               p->nrhs, n, gn);
+                        else
+                            hipLaunchKernelGGL(k_residual_wg<2>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
+                                               p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
+                                               p->nrhs, n, gn);
                     });
                     rhs = p->d_r64 + sx * g.e0;
                 }
@@ -1959,18 +1923,24 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
             if (!sync_groups(gs)) return REFEFF_FMS_DEVICE_ERROR;
             if (sweep == 0) t_first = now_s();
             sweeps = sweep + 1;
-            // stop when every energy's last correction is below 1e-10 of |x|,
-            // or has stopped shrinking at the FP64 noise floor
+            // Refinement converges linearly, so after a correction of relative
+            // size d_s (previous d_{s-1}) the error left in x is about
+            // d_s * (d_s / d_{s-1}).  Stop when that estimate is below 1e-10
+            // for every energy, or when a correction below 1e-9 has stopped
+            // shrinking (the FP64 residual noise floor).
             bool done = sweep > 0;
             worst = 0.0;
             for (int e = 0; e < nb; ++e) {
                 const double rel = norms[4 * e + 2] > 0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
-                worst = std::max(worst, rel);
-                const bool small = rel <= 1.0e-10;
+                const double est = sweep == 0 ? rel : rel * std::min(1.0, rel / prev_rel[e]);
+                worst = std::max(worst, est);
+                const bool small = est <= 1.0e-10;
                 const bool floor = rel <= 1.0e-9 && rel >= 0.5 * prev_rel[e];
                 if (!(small || floor)) done = false;
+                est_err[e] = floor ? rel : est;
                 prev_rel[e] = rel;
             }
+            if (p->trace) std::fprintf(stderr, "refeff_fms: sweep %d estimated max relative error %.3e\n", sweep, worst);
             if (done) break;
         }
     } else {
@@ -1998,8 +1968,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     for (int e = 0; e < nb; ++e) {
         int s = map_info(info[e]);
         if (!s && mixed) {
-            const double rel = norms[4 * e + 2] > 0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
-            if (!(rel <= 1.0e-9)) s = REFEFF_FMS_NO_CONVERGENCE;
+            if (!(est_err[e] <= 1.0e-9)) s = REFEFF_FMS_NO_CONVERGENCE;
         }
         if (status) status[e] = s;
         if (s && !rc) rc = s;
@@ -2123,6 +2092,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
               dupload(&p->d_ipart, p->ipart.data(), p->ipart.size()) && dupload(&p->d_xnlm, xn.data(), xn.size()) &&
               dupload(&p->d_so_plus, sp.data(), sp.size()) && dupload(&p->d_so_minus, sm.data(), sm.size()) &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
+    p->trace = std::getenv("REFEFF_FMS_TRACE") != nullptr;
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
     for (int i = 0; ok && i < p->nstreams; ++i)
         ok = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking) == hipSuccess;
