@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 METRIC = "FMS energy-pts/s at ≤1e-6 xmu rel err; mixed+learned vs plain FP64 LU"
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the LU trailing-update kernel, averaged over one factorisation's
+# launches, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script scaled by the
+# same counters read for a 1 GiB device copy (profiles/fms_cu_r8_gemm_v2_traffic.md).
+TRAFFIC_BYTES_PER_LAUNCH = {("cfg2", "mixed"): 9191833259}
 
 
 def parse():
@@ -36,6 +40,9 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip the plain FP64 LU comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle accuracy sample")
     ap.add_argument("--cpu-energies", type=int, default=6)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (the reference kernel for "
+                         "FETCH_SIZE / WRITE_SIZE counter scaling under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -128,6 +135,18 @@ def cpu_baseline(wl, energies, gg_gpu):
                       f"LAPACK cgetrf/cgetrs complex64 (OpenBLAS, {cores} threads)"}, errs
 
 
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of a known size: rocprofv3 --pmc reads FETCH_SIZE / WRITE_SIZE
+    for it, and the ratio to nbytes scales the counters of the FMS kernels."""
+    from refeff_amd.hip import DeviceArray, hip, synchronize
+    src, dst = DeviceArray.empty(nbytes), DeviceArray.empty(nbytes)
+    if hip().hipMemcpy(dst.ptr, src.ptr, nbytes, 3) != 0:
+        raise RuntimeError("hipMemcpy D2D failed")
+    synchronize()
+    src.free()
+    dst.free()
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -137,6 +156,8 @@ def main():
         sys.exit(2)
     from refeff_amd import lib, workloads as W
     L = lib()
+    if args.copy_calibration:
+        copy_calibration()
     wl = W.CONFIGS[args.config]()
     solver_id = 1 if args.solver == "mixed" else 2
     main_res = run_solver(L, wl, solver_id, args.steps, args.warmup, profile=True)
@@ -168,9 +189,11 @@ def main():
         "config": {"workload": wl.name, "atoms": wl.nat, "states": n, "padded": main_res["npad"], "energies": ne,
                    "solver": args.solver, "batch": main_res["batch"],
                    "refinement_sweeps": int(main_res["stats"].refinement_sweeps)},
-        "roofline": {"bound": "mfma", "kernel": f"{'k_gemm_big' if prec == 'fp32' else 'k_gemm_sub<double>'} (LU trailing update, K=128)",
+        "roofline": {"bound": "mfma",
+                     "kernel": f"{'k_gemm_v2' if prec == 'fp32' else 'k_gemm_sub<double>'} (LU trailing update, K=128)",
                      "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_TFLOPS[prec], 4), "traffic": None,
+                     "frac": round(achieved / PEAK_TFLOPS[prec], 4),
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((args.config, args.solver)),
                      "lu_tflops": round(lu_flops / (lu_ms * 1e-3) / 1e12, 3) if lu_ms > 0 else 0.0},
         "kernels": kernels,
     }
