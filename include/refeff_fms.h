@@ -189,10 +189,10 @@ enum refeff_fms_kernel_class {
     REFEFF_FMS_K_PANEL = 2,     /* LU panel factorisation */
     REFEFF_FMS_K_SWAP = 3,      /* row interchanges + U12 triangular solve */
     REFEFF_FMS_K_GEMM = 4,      /* LU trailing update, K = outer block width (MFMA) */
-    REFEFF_FMS_K_SOLVE = 5,     /* forward/backward substitution */
+    REFEFF_FMS_K_SOLVE = 5,     /* forward/backward substitution + diagonal-block inverses */
     REFEFF_FMS_K_RESIDUAL = 6,  /* FP64 residual */
     REFEFF_FMS_K_OTHER = 7,
-    REFEFF_FMS_K_GEMM_SMALL = 8, /* panel-width updates (in-panel, U12, substitution) */
+    REFEFF_FMS_K_GEMM_SMALL = 8, /* panel-width (K = 32) LU updates inside an outer block */
     REFEFF_FMS_K_COUNT = 9
 };
 

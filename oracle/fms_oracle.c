@@ -14,6 +14,11 @@
  * known-answer tests, closed forms (Racah 3j vs sympy, Hankel polynomials vs
  * scipy spherical Hankel functions, Wigner small-d vs sympy), LAPACK zgesv on
  * the linear solve, and brute-force invariants on tiny clusters.
+ *
+ * Parity unpinned: the state order of clusters larger than refeff's <=5-atom
+ * cluster KATs (athep ties and sortat swaps at hundreds of atoms,
+ * crates/refeff-core/src/fms/cluster.rs:91-195).  It is restated from source
+ * here and in the product, and no reference fixture holds a large cluster.
  */
 #include <complex.h>
 #include <math.h>
