@@ -232,14 +232,142 @@ int refeff_fms_mkgtr_trace(int32_t energy_count, int32_t channel_count, int32_t 
 
 /* Reads fms.inp, phase.bin, geom.dat, global.inp (optionally .dimensions.dat)
  * and writes gg.bin, gg.dat and log3.dat.  Replaces run_fms_in_dir,
- * crates/refeff-engine/src/fms.rs:462-539.  *files_written receives the file
- * count; message receives a context string on failure. */
+ * crates/refeff-engine/src/fms.rs:462-539, on its ordinary source path
+ * (build_fms_source_outputs, :3549-3695) with the global SIG2 damping
+ * (idwopt < 0); other Debye-Waller models, save_gg_slice, reciprocal-space
+ * and Hubbard sources return REFEFF_FMS_UNSUPPORTED.  Without source files a
+ * cached gg.bin / gg.dat is re-emitted canonically.  solver: a
+ * refeff_fms_solver.  *files_written receives the file count; message
+ * receives a context string on failure. */
 int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_written, char* message,
                           size_t message_len);
 
-/* Reads gg.bin, fms.inp, global.inp, phase.bin and writes fms.bin and gtr.dat.
- * Replaces run_mkgtr_in_dir, crates/refeff-engine/src/fms.rs:559-631. */
+/* Reads gg.bin, fms.inp, global.inp, phase.bin and writes fms.bin and gtr.dat
+ * (ordinary orientation-averaged trace, ipol = 0) and appends the MKGTR lines
+ * to log3.dat.  Replaces run_mkgtr_in_dir, crates/refeff-engine/src/fms.rs:559-631
+ * with build_mkgtr_ordinary_outputs (:5528-5608); EELS, NRIXS and polarised
+ * traces return REFEFF_FMS_UNSUPPORTED. */
 int refeff_mkgtr_run_in_dir(const char* work_dir, int32_t* files_written, char* message, size_t message_len);
+
+/* ---------------------------------------------------------------------- */
+/* text codecs of the FEFF handoff files (crates/refeff-io)                 */
+/* ---------------------------------------------------------------------- */
+
+/* Text outputs: out receives a NUL-terminated string when out_len allows;
+ * *needed (when given) receives strlen + 1; out = NULL only queries the size.
+ * Codec failures return REFEFF_FMS_IO_ERROR (malformed text) or
+ * REFEFF_FMS_INVALID_ARGUMENT (values that cannot be written);
+ * refeff_io_last_error() then describes the failure on the calling thread. */
+const char* refeff_io_last_error(void);
+
+#define REFEFF_IO_MAX_POTENTIALS 32
+
+/* PAD field of npack characters.  Replaces encode_f64 / decode_f64,
+ * crates/refeff-io/src/pad.rs:33-120 (FEFF padlib, byte-identical output).
+ * out must hold npack + 1 bytes. */
+int refeff_io_pad_encode(double value, int32_t npack, char* out, size_t out_len);
+int refeff_io_pad_decode(const char* field, int32_t npack, double* value);
+
+/* PAD lines ('!' reals, '$' complex pairs).  Replaces encode_reals /
+ * encode_complex / decode_reals / decode_complex, crates/refeff-io/src/pad.rs:122-220.
+ * values: count reals, or count complex (re, im) pairs. */
+int refeff_io_pad_encode_array(const double* values, size_t count, int32_t complex_values, int32_t npack,
+                               char* out, size_t out_len, size_t* needed);
+int refeff_io_pad_decode_array(const char* text, int32_t complex_values, int32_t npack, size_t expected,
+                               double* values, size_t* decoded);
+
+/* Fortran Ew.d with a [0.1, 1) mantissa (gg.dat's E20.10).  Replaces
+ * write_fortran_zero_scaled_exp, crates/refeff-io/src/format.rs:164-209. */
+int refeff_io_fortran_exp(double value, int32_t width, int32_t precision, char* out, size_t out_len);
+
+/* fms.inp.  Mirrors FmsInput (crates/refeff-io/src/fms_input.rs:19-60);
+ * parse replaces FmsInput::parse_str (:112, :274-410), string replaces
+ * fms_input_string (:164-189). */
+typedef struct refeff_io_fms_inp {
+    int32_t mfms, idwopt, minv;
+    double rfms2, rdirec, toler1, toler2;
+    double tk, thetad, sig2g;
+    int32_t lmaxph_count;
+    int32_t lmaxph[REFEFF_IO_MAX_POTENTIALS];
+    int32_t decomposition_channels;
+    int32_t save_gg_slice;
+    int32_t do_fms;
+} refeff_io_fms_inp;
+int refeff_io_fms_inp_parse(const char* text, refeff_io_fms_inp* out);
+int refeff_io_fms_inp_string(const refeff_io_fms_inp* in, char* out, size_t out_len, size_t* needed);
+
+/* phase.bin.  Mirrors PhaseBinData (crates/refeff-io/src/phase_bin/types.rs:85-127);
+ * parse replaces parse_phase_bin (parse.rs:18-123), string replaces
+ * phase_bin_string (render.rs:16-95).  Arrays (complex, C order):
+ *   energy_grid [ne], reference_energy [ne][nsp],
+ *   phase_shifts: per potential in order, [ne][2 lmax + 1][nsp], concatenated,
+ *   transition_moments [ne][nq][ntr][nsp].
+ * Parse with NULL arrays to read only the header. */
+typedef struct refeff_io_phase_bin_info {
+    int32_t spin_count, energy_count, main_energy_count, auxiliary_energy_count;
+    int32_t potential_count, ihole, fermi_index, pad_width;
+    int32_t final_state_count, transition_count, q_count;
+    int32_t header_fields;  /* 8, 10 or 11 integers on the header line (parse only) */
+    double scalars[3];      /* rnrmav, xmu, edge */
+    int32_t lmax[REFEFF_IO_MAX_POTENTIALS];
+    int32_t atomic_number[REFEFF_IO_MAX_POTENTIALS];
+    char label[REFEFF_IO_MAX_POTENTIALS][8];
+} refeff_io_phase_bin_info;
+int refeff_io_phase_bin_parse(const char* text, refeff_io_phase_bin_info* info, double* energy_grid,
+                              double* reference_energy, double* phase_shifts, double* transition_moments);
+int refeff_io_phase_bin_string(const refeff_io_phase_bin_info* info, const double* energy_grid,
+                               const double* reference_energy, const double* phase_shifts,
+                               const double* transition_moments, char* out, size_t out_len, size_t* needed);
+
+/* geom.dat.  Replaces GeomDat::parse_str / geom_dat_string,
+ * crates/refeff-io/src/structure_output.rs:204-222, 470-530.  Parse with NULL
+ * arrays to read nat / nph; model_atoms holds nph + 1 entries, xyz 3 nat. */
+int refeff_io_geom_dat_parse(const char* text, int32_t* nat, int32_t* nph, int32_t* model_atoms, int32_t* index,
+                             double* xyz, int32_t* iph, int32_t* boundary);
+int refeff_io_geom_dat_string(int32_t nat, int32_t nph, const int32_t* model_atoms, const int32_t* index,
+                              const double* xyz, const int32_t* iph, const int32_t* boundary, char* out,
+                              size_t out_len, size_t* needed);
+
+/* global.inp (the controls the FMS and MKGTR stages read).  Replaces
+ * GlobalInput::parse_str, crates/refeff-io/src/global_input.rs:376-470. */
+typedef struct refeff_io_global_inp {
+    int32_t nabs, iphabs;
+    double rclabs;
+    int32_t ipol, ispin, le2;
+    double elpty, angks;
+    int32_t l2lp, do_nrixs, ldecmx, lj;
+    double evec[3], xivec[3], spvec[3];
+    double polarization_tensor[3][6];
+    double evnorm, xivnorm, spvnorm;
+    int32_t nq, imdff, qaverage, mixdff;
+} refeff_io_global_inp;
+int refeff_io_global_inp_parse(const char* text, refeff_io_global_inp* out);
+
+/* gg.dat / gg.bin sections (E20.10 complex matrices).  Replaces parse_gg_dat /
+ * gg_dat_string, crates/refeff-io/src/gg_dat.rs:89-170, 410-640.  values:
+ * each section's (row, col) matrix row-major, sections concatenated.  Parse
+ * with NULL arrays to read the counts. */
+int refeff_io_gg_dat_parse(const char* text, int32_t* section_count, size_t* value_count, int32_t* section_numbers,
+                           int32_t* rows, int32_t* cols, double* values);
+int refeff_io_gg_dat_string(int32_t section_count, const int32_t* section_numbers, const int32_t* rows,
+                            const int32_t* cols, const double* values, char* out, size_t out_len, size_t* needed);
+
+/* fms.bin.  Replaces fms_bin_string / parse_fms_bin,
+ * crates/refeff-io/src/fms_bin.rs:77-196.  declared_spectrum_count = -1
+ * writes the five-integer legacy header; spectra [spectrum][energy] complex.
+ * counts: ne, ne1, ne3, nph, npadx. */
+int refeff_io_fms_bin_string(double cluster_radius, int32_t energy_count, int32_t main_energy_count,
+                             int32_t auxiliary_energy_count, int32_t highest_potential_index, int32_t pad_width,
+                             int32_t declared_spectrum_count, int32_t spectrum_count, const double* spectra,
+                             char* out, size_t out_len, size_t* needed);
+int refeff_io_fms_bin_parse(const char* text, double* cluster_radius, int32_t* counts,
+                            int32_t* declared_spectrum_count, int32_t* spectrum_count, double* spectra);
+
+/* gtr.dat (energy, trace) rows.  Replaces gtr_dat_string / parse_gtr_dat,
+ * crates/refeff-io/src/gtr_dat.rs. */
+int refeff_io_gtr_dat_string(int32_t count, const double* energy, const double* trace, char* out, size_t out_len,
+                             size_t* needed);
+int refeff_io_gtr_dat_parse(const char* text, int32_t* count, double* energy, double* trace);
 
 /* Library version string (static storage). */
 const char* refeff_fms_version(void);

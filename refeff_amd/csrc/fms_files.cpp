@@ -1,34 +1,791 @@
-// FEFF file handoffs of the FMS and MKGTR module stages (work-directory seam).
+// FEFF file handoffs of the FMS and MKGTR module stages (work-directory seam)
+// and the C ABI of the text codecs they use.
+#include <cmath>
+#include <complex>
+#include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "fms_common.h"
+#include "fms_io.h"
 
 namespace {
+
+using refeff::io::cplx;
+using refeff::io::Error;
+namespace rio = refeff::io;
+
+thread_local std::string g_last_error;
+
 void put_message(char* dst, size_t len, const std::string& msg) {
     if (!dst || len == 0) return;
     const size_t n = std::min(len - 1, msg.size());
     std::memcpy(dst, msg.data(), n);
     dst[n] = '\0';
 }
+
+// Runs f, turning codec / stage failures into a status and the last-error text.
+template <typename F>
+int guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return REFEFF_FMS_NO_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return REFEFF_FMS_IO_ERROR;
+    }
+}
+
+// Text out-parameter convention: *needed = length + 1; out may be NULL to query.
+int emit(const std::string& text, char* out, size_t out_len, size_t* needed) {
+    if (needed) *needed = text.size() + 1;
+    if (!out) return REFEFF_FMS_OK;
+    if (out_len < text.size() + 1) {
+        g_last_error = "output buffer too small";
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    }
+    std::memcpy(out, text.data(), text.size());
+    out[text.size()] = '\0';
+    return REFEFF_FMS_OK;
+}
+
+std::vector<cplx> cvec(const double* p, size_t n) {
+    std::vector<cplx> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = cplx(p[2 * i], p[2 * i + 1]);
+    return v;
+}
+void cput(const std::vector<cplx>& v, double* p) {
+    for (size_t i = 0; i < v.size(); ++i) {
+        p[2 * i] = v[i].real();
+        p[2 * i + 1] = v[i].imag();
+    }
+}
+
+[[noreturn]] void stage_fail(int code, const std::string& msg) { throw Error(code, msg); }
+void check_status(int rc, const std::string& where) {
+    if (rc != REFEFF_FMS_OK) stage_fail(rc, where);
+}
+
+std::string join(const std::string& dir, const char* name) {
+    if (dir.empty()) return name;
+    return dir.back() == '/' ? dir + name : dir + "/" + name;
+}
+
+float narrow_nonnegative(double v, const char* what) {
+    const float f = static_cast<float>(v);
+    if (!std::isfinite(v) || v < 0.0 || !std::isfinite(f)) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, std::string(what) + " must be a finite nonnegative f32");
+    return f;
+}
+
+// Controls the module stage supports: refeff fms.rs:971-1000 (finite radii,
+// sig2g >= 0, toler1 > 0, toler2 >= 0) plus what this library provides.
+void validate_controls(const rio::FmsInp& in) {
+    if (!std::isfinite(in.rfms2) || !std::isfinite(in.rdirec) ||
+        (in.do_fms != 0 && (in.rfms2 < 0.0 || in.rdirec < 0.0)))
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative cluster radii for full FMS");
+    if (!std::isfinite(in.sig2g) || in.sig2g < 0.0)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative global SIG2");
+    if (!std::isfinite(in.toler1) || in.toler1 <= 0.0)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires positive finite toler1");
+    if (!std::isfinite(in.toler2) || in.toler2 < 0.0)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative toler2");
+    if (in.idwopt >= 0)
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "Debye-Waller model idwopt=" + std::to_string(in.idwopt) +
+                                               " is not provided; only the global SIG2 (idwopt < 0)");
+    if (in.save_gg_slice) stage_fail(REFEFF_FMS_UNSUPPORTED, "save_gg_slice (gg_full = A^-1 G0) is not provided");
+}
+
+struct FmsStageResult {
+    int energy_count = 0;
+    int cluster_atoms = 0;
+};
+
+// gg sections from phase.bin / geom.dat / global.inp: refeff
+// build_fms_source_outputs, crates/refeff-engine/src/fms.rs:3549-3695, with
+// fms_atoms_from_geom (:4286-4322), the idwopt < 0 branch of
+// fms_mean_square_displacements (:4345-4433), fms_wave_numbers (:4972-4985, kept
+// in FP64), fms_phase_shifts_for_energy (:5002-5054, kept in FP64) and
+// fms_gg_section_values (:2004-2047).
+std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsInp& in, int solver,
+                                         FmsStageResult& res) {
+    const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
+    const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
+    const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
+    if (rio::file_exists(join(dir, "reciprocal.inp")))
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "reciprocal-space FMS (reciprocal.inp) is not provided");
+    if (rio::file_exists(join(dir, "hubbard.inp")) && rio::file_exists(join(dir, "v_hubbard.bin")))
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "active Hubbard FMS sources are not provided");
+    const int npot = static_cast<int>(phase.potentials.size());
+    const int max_pot = npot - 1;
+    if (geom.nph != max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "geom.dat nph " + std::to_string(geom.nph) +
+                                                    " does not match phase.bin maximum potential " + std::to_string(max_pot));
+    if (static_cast<int>(in.lmaxph.size()) <= max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp has too few lmaxph values for FMS generation");
+    int global_lmax = 0;
+    for (int ip = 0; ip <= max_pot; ++ip) {
+        if (in.lmaxph[ip] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp lmaxph must be nonnegative");
+        if (in.lmaxph[ip] > phase.potentials[ip].lmax)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp lmaxph exceeds the phase.bin lmax of potential " + std::to_string(ip));
+        global_lmax = std::max(global_lmax, in.lmaxph[ip]);
+    }
+    const float radius = in.rfms2 >= 0.0 ? narrow_nonnegative(in.rfms2, "FMS cluster radius") : 0.0f;
+    const float rdirec = in.rdirec >= 0.0 ? narrow_nonnegative(in.rdirec, "FMS direct cutoff") : 0.0f;
+    if (geom.atoms.empty()) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
+
+    // cluster: f32 coordinates, yprep (absorber = first atom of potential 0), sortat
+    const int nat_all = static_cast<int>(geom.atoms.size());
+    std::vector<float> pos(3 * static_cast<size_t>(nat_all));
+    std::vector<int32_t> pot(nat_all);
+    for (int i = 0; i < nat_all; ++i) {
+        pos[3 * i] = static_cast<float>(geom.atoms[i].x);
+        pos[3 * i + 1] = static_cast<float>(geom.atoms[i].y);
+        pos[3 * i + 2] = static_cast<float>(geom.atoms[i].z);
+        pot[i] = geom.atoms[i].iph;
+    }
+    std::vector<float> cpos(pos.size());
+    std::vector<int32_t> cpot(nat_all);
+    int32_t nat = 0, central = 0, untruncated = 0;
+    check_status(refeff_fms_yprep_cluster(nat_all, pos.data(), pot.data(), 0, radius, nat_all, cpos.data(), cpot.data(),
+                                          &nat, &central, &untruncated),
+                 "failed to select FMS cluster from geom.dat");
+    if (in.do_fms != 0 && nat <= max_pot)
+        stage_fail(REFEFF_FMS_MISSING_REPRESENTATIVE, "FMS cluster has " + std::to_string(nat) +
+                                                          " atom(s), but full-potential generation needs representatives through potential " +
+                                                          std::to_string(max_pot));
+    std::vector<int32_t> reps(npot);
+    check_status(refeff_fms_sort_representative_atoms(0, max_pot, nat, cpos.data(), cpot.data(), reps.data()),
+                 "failed to prepare FMS representative atoms from geom.dat");
+    if (cpot[0] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS absorber potential must be nonnegative");
+    const int absorber = cpot[0];
+
+    int nspu = phase.spin_count;
+    if (rio::file_exists(join(dir, ".dimensions.dat"))) {
+        nspu = rio::parse_dimensions_nspu(rio::read_file(join(dir, ".dimensions.dat")));
+        if (nspu <= 0 || nspu < phase.spin_count)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, ".dimensions.dat nspu must be positive and at least the phase.bin spin count");
+    }
+    // sigsqr(atom2, atom1) = sig2g for atom1 != atom2 (idwopt < 0)
+    const float sig2g = narrow_nonnegative(in.sig2g, "FMS SIG2");
+    const float pair = narrow_nonnegative(static_cast<double>(sig2g) + 0.0, "FMS pair sigsqr");
+    std::vector<float> sig(static_cast<size_t>(nat) * nat, 0.0f);
+    for (int a2 = 0; a2 < nat; ++a2)
+        for (int a1 = 0; a1 < nat; ++a1)
+            if (a1 != a2) sig[a2 + static_cast<size_t>(nat) * a1] = pair;
+
+    refeff_fms_plan_desc d{};
+    d.lfms = in.do_fms;
+    d.minv = in.minv;
+    d.spin_channels = phase.spin_count;
+    d.spin_selector = global.ispin;
+    d.atom_count = nat;
+    d.positions = cpos.data();
+    d.potentials = cpot.data();
+    d.max_potential = max_pot;
+    d.global_lmax = global_lmax;
+    std::vector<int32_t> raw(in.lmaxph.begin(), in.lmaxph.begin() + npot);
+    d.raw_potential_lmax = raw.data();
+    d.direct_cutoff = rdirec;
+    d.mean_square_displacements = sig.data();
+    d.convergence_tolerance = static_cast<float>(in.toler1);
+    d.zero_tolerance = static_cast<float>(in.toler2);
+    d.full_scattering_requested = 0;
+    d.solver = solver;
+    d.max_batch = 0;
+    refeff_fms_plan* plan = nullptr;
+    check_status(refeff_fms_plan_create(&d, &plan), "failed to prepare FMS real-space plan");
+    refeff_fms_plan_info info{};
+    int rc = refeff_fms_plan_query(plan, &info);
+
+    const int ne = phase.energy_count, nsp = phase.spin_count, L = global_lmax;
+    const int nl = 2 * L + 1;
+    std::vector<double> k(2 * static_cast<size_t>(ne) * nsp), ph(2 * static_cast<size_t>(ne) * nsp * nl * npot, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int s = 0; s < nsp; ++s) {
+            const cplx w = std::sqrt(cplx(2.0, 0.0) * (phase.energy_grid[e] - phase.reference_energy[e * nsp + s])) / 0.529177249;
+            k[2 * (e * nsp + s)] = w.real();
+            k[2 * (e * nsp + s) + 1] = w.imag();
+        }
+    for (int ip = 0; ip < npot; ++ip) {
+        const rio::PhasePotential& pp = phase.potentials[ip];
+        const int eff = std::min(in.lmaxph[ip], L), pnl = 2 * pp.lmax + 1;
+        for (int e = 0; e < ne; ++e)
+            for (int s = 0; s < nsp; ++s)
+                for (int sl = -eff; sl <= eff; ++sl) {
+                    const cplx v = pp.phase_shifts[(static_cast<size_t>(e) * pnl + (sl + pp.lmax)) * nsp + s];
+                    const size_t o = ((static_cast<size_t>(e) * nsp + s) * nl + (sl + L)) * npot + ip;
+                    ph[2 * o] = v.real();
+                    ph[2 * o + 1] = v.imag();
+                }
+    }
+    const int ch = info.channel_count;
+    std::vector<double> sc(2 * static_cast<size_t>(ne) * npot * ch * ch, 0.0);
+    std::vector<int32_t> status(ne, 0);
+    if (rc == REFEFF_FMS_OK) rc = refeff_fms_solve_batch(plan, ne, k.data(), ph.data(), sc.data(), status.data(), nullptr);
+    refeff_fms_plan_destroy(plan);
+    for (int e = 0; e < ne; ++e)
+        if (status[e]) stage_fail(status[e], "failed to solve FMS energy section " + std::to_string(e + 1));
+    check_status(rc, "failed to solve the FMS energy sweep");
+
+    const int alm = std::min(in.lmaxph[absorber], L);
+    const int ang = (alm + 1) * (alm + 1), active = ang * nsp, outc = ang * nspu;
+    if (active > ch) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS absorber block exceeds the scattering shape");
+    std::vector<rio::GgSection> secs(ne);
+    for (int e = 0; e < ne; ++e) {
+        rio::GgSection& g = secs[e];
+        g.section_number = e + 1;
+        g.rows = g.cols = outc;
+        g.values.assign(static_cast<size_t>(outc) * outc, cplx(0, 0));
+        const double* blk = sc.data() + 2 * (static_cast<size_t>(e) * npot + absorber) * ch * ch;
+        for (int c = 0; c < active; ++c)
+            for (int r = 0; r < active; ++r) {
+                const size_t o = r + static_cast<size_t>(ch) * c;
+                g.values[static_cast<size_t>(r) * outc + c] = cplx(blk[2 * o], blk[2 * o + 1]);
+            }
+    }
+    res.energy_count = ne;
+    res.cluster_atoms = in.do_fms != 0 ? nat : 0;
+    return secs;
+}
+
+// log3.dat of a generated FMS stage: refeff write_generated_fms_module_log,
+// crates/refeff-engine/src/fms.rs:5995-6050.
+std::string fms_module_log(const rio::FmsInp& in, const FmsStageResult& r) {
+    std::vector<std::string> lines = {"FMS calculation of full Green's function ...", "FEFF-serial using 1 thread."};
+    char buf[128];
+    std::snprintf(buf, sizeof(buf), "Using %5d energy points.", r.energy_count);
+    lines.push_back(buf);
+    static const char* debye[] = {"Applying Debye-Waller factors using a Correlated Debye model.",
+                                  "Applying Debye-Waller factors using the Equation-of-Motion method.",
+                                  "Applying Debye-Waller factors using the Recursion method.",
+                                  "Applying Debye-Waller factors using the Classical Debye model.",
+                                  "Applying Debye-Waller factors using the sig.dat file.",
+                                  "Applying Debye-Waller factors using the ab-initio Dynamical Matrix model."};
+    if (in.idwopt >= 0 && in.idwopt <= 5) lines.push_back(debye[in.idwopt]);
+    lines.push_back("xprep done");
+    if (r.cluster_atoms > 1) {
+        std::snprintf(buf, sizeof(buf), "FMS for a cluster of %4d atoms", r.cluster_atoms);
+        lines.push_back(buf);
+        int next = 1;
+        for (int e = 1; e <= r.energy_count; ++e) {
+            if (e < next) continue;
+            std::snprintf(buf, sizeof(buf), "Energy point %4d/%4d", e, r.energy_count);
+            lines.push_back(buf);
+            next = next == 1 ? 10 : next + 10;
+        }
+    }
+    lines.push_back("Done with module: FMS.");
+    std::string out;
+    for (const std::string& l : lines) out += l + "\n";
+    return out;
+}
+
+int run_fms(const std::string& dir, int solver, int32_t* files_written) {
+    if (files_written) *files_written = 0;
+    const rio::FmsInp in = rio::parse_fms_inp(rio::read_file(join(dir, "fms.inp")));
+    if (in.mfms == 0) return REFEFF_FMS_OK;
+    const bool sources = rio::file_exists(join(dir, "phase.bin")) && rio::file_exists(join(dir, "geom.dat")) &&
+                         rio::file_exists(join(dir, "global.inp"));
+    int written = 0;
+    if (sources) {
+        validate_controls(in);
+        FmsStageResult r;
+        const std::vector<rio::GgSection> secs = fms_sections(dir, in, solver, r);
+        const std::string gg = rio::gg_dat_string(secs);
+        rio::write_file(join(dir, "gg.bin"), gg);
+        rio::write_file(join(dir, "gg.dat"), gg);
+        // a refreshed gg invalidates every MKGTR projection derived from it
+        for (const char* name : {"fms.bin", "gtr.dat", "fmsl.bin", "gtrl.dat"}) std::remove(join(dir, name).c_str());
+        rio::write_file(join(dir, "log3.dat"), fms_module_log(in, r));
+        written = 3;
+    } else {
+        // cached gg only: re-emit it canonically
+        const std::string bin = join(dir, "gg.bin"), dat = join(dir, "gg.dat");
+        const bool have_bin = rio::file_exists(bin), have_dat = rio::file_exists(dat);
+        if (!have_bin && !have_dat)
+            stage_fail(REFEFF_FMS_IO_ERROR,
+                       "FMS Green's-function generation requires cached FMS output or phase.bin/geom.dat/global.inp source handoffs");
+        if (have_bin) rio::write_file(bin, rio::gg_dat_string(rio::parse_gg_dat(rio::read_file(bin))));
+        if (have_dat) rio::write_file(dat, rio::gg_dat_string(rio::parse_gg_dat(rio::read_file(dat))));
+        written = (have_bin ? 1 : 0) + (have_dat ? 1 : 0);
+        if (!rio::file_exists(join(dir, "log3.dat"))) {
+            rio::write_file(join(dir, "log3.dat"),
+                            "FMS calculation of full Green's function ...\nFEFF-serial using 1 thread.\nDone with module: FMS.\n");
+        }
+        written += 1;
+    }
+    if (files_written) *files_written = written;
+    return REFEFF_FMS_OK;
+}
+
+// MKGTR over a cached gg: refeff run_mkgtr_in_dir / build_mkgtr_ordinary_outputs,
+// crates/refeff-engine/src/fms.rs:559-631, 5255-5330, 5528-5608, 5849-5918,
+// 5941-5963.  The gg sections stay in FP64 (refeff narrows them to c32).
+int run_mkgtr(const std::string& dir, int32_t* files_written) {
+    if (files_written) *files_written = 0;
+    const rio::FmsInp in = rio::parse_fms_inp(rio::read_file(join(dir, "fms.inp")));
+    if (in.mfms == 0) return REFEFF_FMS_OK;
+    const std::string bin = join(dir, "gg.bin"), dat = join(dir, "gg.dat");
+    if (!rio::file_exists(bin) && !rio::file_exists(dat))
+        stage_fail(REFEFF_FMS_IO_ERROR, "MKGTR requires gg.bin or gg.dat from the FMS stage");
+    int written = 0;
+    const std::string fms_path = join(dir, "fms.bin"), gtr_path = join(dir, "gtr.dat");
+    const bool have_sources = rio::file_exists(join(dir, "phase.bin")) && rio::file_exists(join(dir, "global.inp"));
+    if (have_sources && (!rio::file_exists(fms_path) || !rio::file_exists(gtr_path))) {
+        if (rio::file_exists(join(dir, "eels.inp")))
+            stage_fail(REFEFF_FMS_UNSUPPORTED, "EELS MKGTR projections (eels.inp) are not provided");
+        const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
+        if (global.do_nrixs == 1) stage_fail(REFEFF_FMS_UNSUPPORTED, "NRIXS/JAS MKGTR traces are not provided");
+        if (global.ipol != 0)
+            stage_fail(REFEFF_FMS_UNSUPPORTED, "polarised MKGTR (ipol != 0) is not provided; orientation average only");
+        const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
+        const std::vector<rio::GgSection> gg = rio::parse_gg_dat(rio::read_file(rio::file_exists(bin) ? bin : dat));
+        if (in.lmaxph.empty() || in.lmaxph[0] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS lmaxph(0) must be nonnegative for MKGTR");
+        const int lmax = in.lmaxph[0];
+        const int ne = phase.energy_count;
+        if (static_cast<int>(gg.size()) != ne)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "gg cache section count does not match phase.bin energy count");
+        const int nch = gg[0].rows;
+        for (const rio::GgSection& s : gg)
+            if (s.rows != nch || s.cols != nch) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "gg cache sections must be square and equal");
+        if (phase.transition_count != 8) stage_fail(REFEFF_FMS_UNSUPPORTED, "MKGTR needs 8 transition moments per energy");
+        const int active = std::abs(global.ispin) == 1 ? phase.spin_count : 1;
+        const int nsp = phase.spin_count;
+        std::vector<int32_t> lind(8);
+        const int nm = 2 * lmax + 1;
+        std::vector<double> bmat(2 * static_cast<size_t>(nm) * 2 * 8 * nm * 2 * 8);
+        check_status(refeff_fms_transition_bmat_averaged(phase.ihole, global.le2, lmax, global.ispin, nsp, lind.data(), bmat.data()),
+                     "failed to build MKGTR transition B matrix");
+        std::vector<double> g(2 * static_cast<size_t>(ne) * nch * nch), rkk(2 * static_cast<size_t>(ne) * 8 * active, 0.0);
+        for (int e = 0; e < ne; ++e)
+            for (int r = 0; r < nch; ++r)
+                for (int c = 0; c < nch; ++c) {
+                    const cplx v = gg[e].values[static_cast<size_t>(r) * nch + c];
+                    const size_t o = static_cast<size_t>(e) * nch * nch + r + static_cast<size_t>(nch) * c;
+                    g[2 * o] = v.real();
+                    g[2 * o + 1] = v.imag();
+                }
+        const size_t nq = phase.q_count;
+        for (int e = 0; e < ne; ++e)
+            for (int t = 0; t < 8; ++t)
+                for (int s = 0; s < active; ++s) {
+                    const cplx v = phase.transition_moments[((static_cast<size_t>(e) * nq + 0) * 8 + t) * nsp + s];
+                    const size_t o = (static_cast<size_t>(e) * 8 + t) * active + s;
+                    rkk[2 * o] = v.real();
+                    rkk[2 * o + 1] = v.imag();
+                }
+        std::vector<double> gtr(2 * static_cast<size_t>(ne));
+        check_status(refeff_fms_mkgtr_trace(ne, nch, active, lmax, lind.data(), bmat.data(), g.data(), rkk.data(), gtr.data()),
+                     "failed to fold cached gg matrices into MKGTR trace");
+        rio::FmsBin f;
+        f.cluster_radius = in.rfms2;
+        f.energy_count = ne;
+        f.main_energy_count = phase.main_energy_count;
+        f.auxiliary_energy_count = phase.auxiliary_energy_count;
+        f.highest_potential_index = static_cast<int>(phase.potentials.size()) - 1;
+        f.pad_width = phase.pad_width;
+        f.declared_spectrum_count = 0;
+        f.spectrum_count = 1;
+        f.spectra = cvec(gtr.data(), ne);
+        if (!rio::file_exists(fms_path)) {
+            rio::write_file(fms_path, rio::fms_bin_string(f));
+            ++written;
+        }
+        if (!rio::file_exists(gtr_path)) {
+            rio::write_file(gtr_path, rio::gtr_dat_string(phase.energy_grid, f.spectra));
+            ++written;
+        }
+    } else {
+        // cached traces: re-emit canonically
+        if (rio::file_exists(fms_path)) {
+            rio::write_file(fms_path, rio::fms_bin_string(rio::parse_fms_bin(rio::read_file(fms_path))));
+            ++written;
+        }
+        if (rio::file_exists(gtr_path)) {
+            std::vector<cplx> en, tr;
+            rio::parse_gtr_dat(rio::read_file(gtr_path), en, tr);
+            rio::write_file(gtr_path, rio::gtr_dat_string(en, tr));
+            ++written;
+        }
+    }
+    const std::string log_path = join(dir, "log3.dat");
+    std::string log = rio::file_exists(log_path)
+                          ? rio::read_file(log_path)
+                          : std::string("FMS calculation of full Green's function ...\nFEFF-serial using 1 thread.\nDone with module: FMS.\n");
+    if (log.find("Done with module: MKGTR.") == std::string::npos) {
+        if (!log.empty() && log.back() != '\n') log += '\n';
+        log += "\nMKGTR: Tracing over Green's function ...\nDone with module: MKGTR.\n";
+    }
+    rio::write_file(log_path, log);
+    if (files_written) *files_written = written + 1;
+    return REFEFF_FMS_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
+const char* refeff_io_last_error(void) { return g_last_error.c_str(); }
+
 int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_written, char* message,
                           size_t message_len) {
-    (void)work_dir;
-    (void)solver;
-    if (files_written) *files_written = 0;
-    put_message(message, message_len, "module stage not available in this build");
-    return REFEFF_FMS_UNSUPPORTED;
+    if (!work_dir) return REFEFF_FMS_INVALID_ARGUMENT;
+    const int rc = guarded([&] { return run_fms(work_dir, solver, files_written); });
+    put_message(message, message_len, rc ? g_last_error : std::string());
+    return rc;
 }
 
 int refeff_mkgtr_run_in_dir(const char* work_dir, int32_t* files_written, char* message, size_t message_len) {
-    (void)work_dir;
-    if (files_written) *files_written = 0;
-    put_message(message, message_len, "module stage not available in this build");
-    return REFEFF_FMS_UNSUPPORTED;
+    if (!work_dir) return REFEFF_FMS_INVALID_ARGUMENT;
+    const int rc = guarded([&] { return run_mkgtr(work_dir, files_written); });
+    put_message(message, message_len, rc ? g_last_error : std::string());
+    return rc;
+}
+
+int refeff_io_pad_encode(double value, int32_t npack, char* out, size_t out_len) {
+    return guarded([&] { return emit(rio::pad_encode(value, npack), out, out_len, nullptr); });
+}
+
+int refeff_io_pad_decode(const char* field, int32_t npack, double* value) {
+    if (!value) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        *value = rio::pad_decode(field, npack);
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_pad_encode_array(const double* values, size_t count, int32_t complex_values, int32_t npack, char* out,
+                               size_t out_len, size_t* needed) {
+    if (count && !values) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        if (complex_values) return emit(rio::pad_encode_complex(cvec(values, count), npack), out, out_len, needed);
+        return emit(rio::pad_encode_reals(std::vector<double>(values, values + count), npack), out, out_len, needed);
+    });
+}
+
+int refeff_io_pad_decode_array(const char* text, int32_t complex_values, int32_t npack, size_t expected,
+                               double* values, size_t* decoded) {
+    if (!text || !values || !decoded) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        if (complex_values) {
+            const std::vector<cplx> v = rio::pad_decode_complex(text, npack, expected);
+            cput(v, values);
+            *decoded = v.size();
+        } else {
+            const std::vector<double> v = rio::pad_decode_reals(text, npack, expected);
+            std::copy(v.begin(), v.end(), values);
+            *decoded = v.size();
+        }
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_fortran_exp(double value, int32_t width, int32_t precision, char* out, size_t out_len) {
+    if (width < 0 || precision < 0 || precision > 30) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] { return emit(rio::fortran_zero_scaled_exp(value, width, precision), out, out_len, nullptr); });
+}
+
+int refeff_io_fms_inp_parse(const char* text, refeff_io_fms_inp* out) {
+    if (!text || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::FmsInp in = rio::parse_fms_inp(text);
+        if (in.lmaxph.size() > REFEFF_IO_MAX_POTENTIALS) throw Error(REFEFF_FMS_IO_ERROR, "fms.inp: too many lmaxph values");
+        *out = refeff_io_fms_inp{};
+        out->mfms = in.mfms;
+        out->idwopt = in.idwopt;
+        out->minv = in.minv;
+        out->rfms2 = in.rfms2;
+        out->rdirec = in.rdirec;
+        out->toler1 = in.toler1;
+        out->toler2 = in.toler2;
+        out->tk = in.tk;
+        out->thetad = in.thetad;
+        out->sig2g = in.sig2g;
+        out->lmaxph_count = static_cast<int32_t>(in.lmaxph.size());
+        for (size_t i = 0; i < in.lmaxph.size(); ++i) out->lmaxph[i] = in.lmaxph[i];
+        out->decomposition_channels = in.decomposition_channels;
+        out->save_gg_slice = in.save_gg_slice ? 1 : 0;
+        out->do_fms = in.do_fms;
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_fms_inp_string(const refeff_io_fms_inp* in, char* out, size_t out_len, size_t* needed) {
+    if (!in || in->lmaxph_count < 0 || in->lmaxph_count > REFEFF_IO_MAX_POTENTIALS) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::FmsInp f;
+        f.mfms = in->mfms;
+        f.idwopt = in->idwopt;
+        f.minv = in->minv;
+        f.rfms2 = in->rfms2;
+        f.rdirec = in->rdirec;
+        f.toler1 = in->toler1;
+        f.toler2 = in->toler2;
+        f.tk = in->tk;
+        f.thetad = in->thetad;
+        f.sig2g = in->sig2g;
+        f.lmaxph.assign(in->lmaxph, in->lmaxph + in->lmaxph_count);
+        f.decomposition_channels = in->decomposition_channels;
+        f.save_gg_slice = in->save_gg_slice != 0;
+        f.do_fms = in->do_fms;
+        return emit(rio::fms_inp_string(f), out, out_len, needed);
+    });
+}
+
+int refeff_io_phase_bin_parse(const char* text, refeff_io_phase_bin_info* info, double* energy_grid,
+                              double* reference_energy, double* phase_shifts, double* transition_moments) {
+    if (!text || !info) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::PhaseBin p = rio::parse_phase_bin(text);
+        if (p.potentials.size() > REFEFF_IO_MAX_POTENTIALS) throw Error(REFEFF_FMS_IO_ERROR, "phase.bin: too many potentials");
+        *info = refeff_io_phase_bin_info{};
+        info->spin_count = p.spin_count;
+        info->energy_count = p.energy_count;
+        info->main_energy_count = p.main_energy_count;
+        info->auxiliary_energy_count = p.auxiliary_energy_count;
+        info->potential_count = static_cast<int32_t>(p.potentials.size());
+        info->ihole = p.ihole;
+        info->fermi_index = p.fermi_index;
+        info->pad_width = p.pad_width;
+        info->final_state_count = p.final_state_count;
+        info->transition_count = p.transition_count;
+        info->q_count = p.q_count;
+        info->header_fields = p.header_fields;
+        for (int i = 0; i < 3; ++i) info->scalars[i] = p.scalars[i];
+        for (size_t i = 0; i < p.potentials.size(); ++i) {
+            info->lmax[i] = p.potentials[i].lmax;
+            info->atomic_number[i] = p.potentials[i].atomic_number;
+            std::snprintf(info->label[i], sizeof(info->label[i]), "%s", p.potentials[i].label.c_str());
+        }
+        if (energy_grid) cput(p.energy_grid, energy_grid);
+        if (reference_energy) cput(p.reference_energy, reference_energy);
+        if (phase_shifts) {
+            size_t off = 0;
+            for (const rio::PhasePotential& pot : p.potentials) {
+                cput(pot.phase_shifts, phase_shifts + 2 * off);
+                off += pot.phase_shifts.size();
+            }
+        }
+        if (transition_moments) cput(p.transition_moments, transition_moments);
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_phase_bin_string(const refeff_io_phase_bin_info* info, const double* energy_grid,
+                               const double* reference_energy, const double* phase_shifts,
+                               const double* transition_moments, char* out, size_t out_len, size_t* needed) {
+    if (!info || !energy_grid || !reference_energy || !phase_shifts || !transition_moments ||
+        info->potential_count < 1 || info->potential_count > REFEFF_IO_MAX_POTENTIALS || info->energy_count < 0 ||
+        info->spin_count < 1 || info->q_count < 0 || info->transition_count < 0)
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::PhaseBin p;
+        p.spin_count = info->spin_count;
+        p.energy_count = info->energy_count;
+        p.main_energy_count = info->main_energy_count;
+        p.auxiliary_energy_count = info->auxiliary_energy_count;
+        p.ihole = info->ihole;
+        p.fermi_index = info->fermi_index;
+        p.pad_width = info->pad_width;
+        p.final_state_count = info->final_state_count;
+        p.transition_count = info->transition_count;
+        p.q_count = info->q_count;
+        for (int i = 0; i < 3; ++i) p.scalars[i] = info->scalars[i];
+        const size_t ne = info->energy_count, nsp = info->spin_count;
+        p.energy_grid = cvec(energy_grid, ne);
+        p.reference_energy = cvec(reference_energy, ne * nsp);
+        size_t off = 0;
+        for (int i = 0; i < info->potential_count; ++i) {
+            rio::PhasePotential pot;
+            if (info->lmax[i] < 0 || info->atomic_number[i] < 0) throw Error(REFEFF_FMS_INVALID_ARGUMENT, "phase.bin: negative lmax or iz");
+            pot.lmax = info->lmax[i];
+            pot.atomic_number = info->atomic_number[i];
+            pot.label = std::string(info->label[i], strnlen(info->label[i], sizeof(info->label[i])));
+            const size_t n = ne * (2 * static_cast<size_t>(pot.lmax) + 1) * nsp;
+            pot.phase_shifts = cvec(phase_shifts + 2 * off, n);
+            off += n;
+            p.potentials.push_back(std::move(pot));
+        }
+        p.transition_moments = cvec(transition_moments, ne * info->q_count * info->transition_count * nsp);
+        return emit(rio::phase_bin_string(p), out, out_len, needed);
+    });
+}
+
+int refeff_io_geom_dat_parse(const char* text, int32_t* nat, int32_t* nph, int32_t* model_atoms, int32_t* index,
+                             double* xyz, int32_t* iph, int32_t* boundary) {
+    if (!text || !nat || !nph) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::GeomDat g = rio::parse_geom_dat(text);
+        *nat = g.nat;
+        *nph = g.nph;
+        if (model_atoms)
+            for (size_t i = 0; i < g.model_atoms.size(); ++i) model_atoms[i] = g.model_atoms[i];
+        for (size_t i = 0; i < g.atoms.size(); ++i) {
+            if (index) index[i] = g.atoms[i].index;
+            if (xyz) {
+                xyz[3 * i] = g.atoms[i].x;
+                xyz[3 * i + 1] = g.atoms[i].y;
+                xyz[3 * i + 2] = g.atoms[i].z;
+            }
+            if (iph) iph[i] = g.atoms[i].iph;
+            if (boundary) boundary[i] = g.atoms[i].boundary;
+        }
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_geom_dat_string(int32_t nat, int32_t nph, const int32_t* model_atoms, const int32_t* index,
+                              const double* xyz, const int32_t* iph, const int32_t* boundary, char* out,
+                              size_t out_len, size_t* needed) {
+    if (nat < 0 || nph < 0 || !model_atoms || (nat > 0 && (!index || !xyz || !iph || !boundary)))
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::GeomDat g;
+        g.nat = nat;
+        g.nph = nph;
+        g.model_atoms.assign(model_atoms, model_atoms + nph + 1);
+        for (int i = 0; i < nat; ++i)
+            g.atoms.push_back(rio::GeomRow{index[i], xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], iph[i], boundary[i]});
+        return emit(rio::geom_dat_string(g), out, out_len, needed);
+    });
+}
+
+int refeff_io_global_inp_parse(const char* text, refeff_io_global_inp* out) {
+    if (!text || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::GlobalInp g = rio::parse_global_inp(text);
+        *out = refeff_io_global_inp{};
+        out->nabs = g.nabs;
+        out->iphabs = g.iphabs;
+        out->rclabs = g.rclabs;
+        out->ipol = g.ipol;
+        out->ispin = g.ispin;
+        out->le2 = g.le2;
+        out->elpty = g.elpty;
+        out->angks = g.angks;
+        out->l2lp = g.l2lp;
+        out->do_nrixs = g.do_nrixs;
+        out->ldecmx = g.ldecmx;
+        out->lj = g.lj;
+        for (int i = 0; i < 3; ++i) {
+            out->evec[i] = g.evec[i];
+            out->xivec[i] = g.xivec[i];
+            out->spvec[i] = g.spvec[i];
+            for (int j = 0; j < 6; ++j) out->polarization_tensor[i][j] = g.polarization_tensor[i][j];
+        }
+        out->evnorm = g.evnorm;
+        out->xivnorm = g.xivnorm;
+        out->spvnorm = g.spvnorm;
+        out->nq = g.nq;
+        out->imdff = g.imdff;
+        out->qaverage = g.qaverage ? 1 : 0;
+        out->mixdff = g.mixdff ? 1 : 0;
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_gg_dat_parse(const char* text, int32_t* section_count, size_t* value_count, int32_t* section_numbers,
+                           int32_t* rows, int32_t* cols, double* values) {
+    if (!text || !section_count || !value_count) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const std::vector<rio::GgSection> s = rio::parse_gg_dat(text);
+        *section_count = static_cast<int32_t>(s.size());
+        size_t total = 0, off = 0;
+        for (const rio::GgSection& x : s) total += x.values.size();
+        *value_count = total;
+        for (size_t i = 0; i < s.size(); ++i) {
+            if (section_numbers) section_numbers[i] = s[i].section_number;
+            if (rows) rows[i] = s[i].rows;
+            if (cols) cols[i] = s[i].cols;
+            if (values) cput(s[i].values, values + 2 * off);
+            off += s[i].values.size();
+        }
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_gg_dat_string(int32_t section_count, const int32_t* section_numbers, const int32_t* rows,
+                            const int32_t* cols, const double* values, char* out, size_t out_len, size_t* needed) {
+    if (section_count < 0 || (section_count > 0 && (!section_numbers || !rows || !cols || !values)))
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        std::vector<rio::GgSection> s(section_count);
+        size_t off = 0;
+        for (int i = 0; i < section_count; ++i) {
+            if (rows[i] <= 0 || cols[i] <= 0) throw Error(REFEFF_FMS_INVALID_ARGUMENT, "gg.dat: section dimensions must be positive");
+            s[i].section_number = section_numbers[i];
+            s[i].rows = rows[i];
+            s[i].cols = cols[i];
+            const size_t n = static_cast<size_t>(rows[i]) * cols[i];
+            s[i].values = cvec(values + 2 * off, n);
+            off += n;
+        }
+        return emit(rio::gg_dat_string(s), out, out_len, needed);
+    });
+}
+
+int refeff_io_fms_bin_string(double cluster_radius, int32_t energy_count, int32_t main_energy_count,
+                             int32_t auxiliary_energy_count, int32_t highest_potential_index, int32_t pad_width,
+                             int32_t declared_spectrum_count, int32_t spectrum_count, const double* spectra,
+                             char* out, size_t out_len, size_t* needed) {
+    if (spectrum_count < 0 || energy_count < 0 || (spectrum_count * static_cast<size_t>(energy_count) > 0 && !spectra))
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::FmsBin f;
+        f.cluster_radius = cluster_radius;
+        f.energy_count = energy_count;
+        f.main_energy_count = main_energy_count;
+        f.auxiliary_energy_count = auxiliary_energy_count;
+        f.highest_potential_index = highest_potential_index;
+        f.pad_width = pad_width;
+        f.declared_spectrum_count = declared_spectrum_count;
+        f.spectrum_count = spectrum_count;
+        f.spectra = cvec(spectra, static_cast<size_t>(spectrum_count) * energy_count);
+        return emit(rio::fms_bin_string(f), out, out_len, needed);
+    });
+}
+
+int refeff_io_fms_bin_parse(const char* text, double* cluster_radius, int32_t* counts, int32_t* declared_spectrum_count,
+                            int32_t* spectrum_count, double* spectra) {
+    if (!text || !cluster_radius || !counts || !declared_spectrum_count || !spectrum_count) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::FmsBin f = rio::parse_fms_bin(text);
+        *cluster_radius = f.cluster_radius;
+        counts[0] = f.energy_count;
+        counts[1] = f.main_energy_count;
+        counts[2] = f.auxiliary_energy_count;
+        counts[3] = f.highest_potential_index;
+        counts[4] = f.pad_width;
+        *declared_spectrum_count = f.declared_spectrum_count;
+        *spectrum_count = f.spectrum_count;
+        if (spectra) cput(f.spectra, spectra);
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_gtr_dat_string(int32_t count, const double* energy, const double* trace, char* out, size_t out_len,
+                             size_t* needed) {
+    if (count < 0 || (count > 0 && (!energy || !trace))) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] { return emit(rio::gtr_dat_string(cvec(energy, count), cvec(trace, count)), out, out_len, needed); });
+}
+
+int refeff_io_gtr_dat_parse(const char* text, int32_t* count, double* energy, double* trace) {
+    if (!text || !count) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        std::vector<cplx> en, tr;
+        rio::parse_gtr_dat(text, en, tr);
+        *count = static_cast<int32_t>(en.size());
+        if (energy) cput(en, energy);
+        if (trace) cput(tr, trace);
+        return REFEFF_FMS_OK;
+    });
 }
 
 }  // extern "C"

@@ -1,0 +1,131 @@
+"""GPU parity of the module stages: ``refeff module fms`` / ``mkgtr`` over a
+work directory (refeff engine fms.rs:462-631), through librefeff_fms.so.
+
+The work directory is written with the library's own FEFF codecs (pinned by
+tests/test_io_codecs.py against refeff-io's known answers).  The expected
+values come from the FP64 oracle on the same inputs: the oracle's own
+yprep/sortat cluster from the same f32 geometry, k = sqrt(2 (E - E_ref)) / a0
+and the phase shifts from the arrays written into phase.bin (PAD at npack 8
+keeps ~1e-12 relative), σ² = sig2g between distinct atoms.  gg.bin's E20.10
+fields carry 10 significant digits, so the gate is the north-star 1e-8."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from refeff_amd import io
+from refeff_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+A0 = 0.529177249
+HARTREE = 27.211396
+
+
+def banner(msg):
+    print(f"[gpu-module] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+GLOBAL_INP = """\
+ nabs, iphabs - CFAVERAGE data
+       1       0 100000.00000
+ ipol, ispin, le2, elpty, angks, l2lp, do_nrixs, ldecmx, lj
+    0    0    0      0.0000      0.0000    0    0   -1   -1
+evec\t\t  xivec \t   spvec
+      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000
+ polarization tensor
+      0.33333      0.00000      0.00000      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.33333      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000      0.00000      0.33333      0.00000
+evnorm, xivnorm, spvnorm - only used for nrixs
+      0.00000      0.00000      0.00000
+nq,    imdff,   qaverage,   mixdff
+           0           0 T F
+ q-vectors : qx, qy, qz, q(norm), weight, qcosth, qsinth, qcosfi, qsinfi
+"""
+
+
+def make_workdir(d, radius=4.6, ne=7, sig2g=0.003):
+    pts = W.lattice(W.FCC, 3.615, radius + 1.0)  # geom.dat holds more atoms than the FMS sphere
+    order = np.argsort(np.linalg.norm(pts, axis=1), kind="stable")
+    pts = pts[order]
+    iph = np.ones(len(pts), np.int32)
+    iph[0] = 0
+    g = io.GeomDat(len(pts), 1, [1, 2], np.arange(1, len(pts) + 1), pts, iph, np.ones(len(pts), np.int32))
+    (d / "geom.dat").write_text(io.geom_dat_string(g))
+    e_ev = np.linspace(-8.0, 30.0, ne) + 8.0 + 0.5j        # E - E_ref in eV, above the reference
+    em = e_ev / HARTREE + 0.1                                 # Hartree, E_ref = 0.1 Ha
+    eref = np.full((ne, 1), 0.1 + 0.0j)
+    k = np.sqrt(2.0 * (em - eref[:, 0])) / A0
+    ph_model = W.synthetic_phases(k, 3, np.array([3, 3]))     # [ne, 1, 7, 2], l~ = l + 3
+    pots = [io.PhaseBinPotential(3, 29, "Cu", ph_model[:, 0, :, ip][:, :, None].copy()) for ip in range(2)]
+    rng = np.random.default_rng(11)
+    rkk = (rng.normal(size=(ne, 1, 8, 1)) + 1j * rng.normal(size=(ne, 1, 8, 1))) * 0.3
+    pb = io.PhaseBinData(1, ne, ne, 0, 1, 1, 8, 8, 8, 1, (2.6, -0.14, -0.14), em, eref, pots, rkk)
+    (d / "phase.bin").write_text(io.phase_bin_string(pb))
+    fin = io.FmsInput(mfms=1, idwopt=-1, minv=0, rfms2=radius, rdirec=10.0, toler1=0.001, toler2=0.001, tk=0.0,
+                      thetad=0.0, sig2g=sig2g, lmaxph=[3, 3], decomposition_channels=-1, save_gg_slice=False,
+                      do_fms=1)
+    (d / "fms.inp").write_text(io.fms_input_string(fin))
+    (d / "global.inp").write_text(GLOBAL_INP)
+    return pts, iph, k, ph_model, rkk, em, radius, sig2g
+
+
+def oracle_gg(pts, iph, k, ph, radius, sig2g):
+    pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(radius), len(iph))
+    pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
+    nat = len(pot)
+    sig = np.full((nat, nat), np.float32(sig2g), np.float32)
+    np.fill_diagonal(sig, 0.0)
+    plan = orc.Plan(pos, pot, nsp=1, lfms=1, max_pot=1, global_lmax=3, raw_lmax=[3, 3], rdirec=10.0, sigsqr=sig)
+    sc, st = plan.spectrum(k.reshape(-1, 1), ph, threads=8)
+    assert not st.any()
+    return sc[:, 0], nat  # absorber potential 0 block [ne, 16, 16]
+
+
+@pytest.mark.parametrize("solver", [0, 2])
+def test_module_fms_writes_gg_matching_oracle(tmp_path, solver):
+    banner(f"module fms, solver {solver}")
+    pts, iph, k, ph, rkk, em, radius, sig2g = make_workdir(tmp_path)
+    assert io.run_fms_in_dir(tmp_path, solver) == 3
+    sections = io.parse_gg_bin((tmp_path / "gg.bin").read_text())
+    assert (tmp_path / "gg.dat").read_text() == (tmp_path / "gg.bin").read_text()
+    ref, nat = oracle_gg(pts, iph, k, ph, radius, sig2g)
+    assert len(sections) == len(k)
+    for e, s in enumerate(sections):
+        assert s.section_number == e + 1 and s.values.shape == (16, 16)
+        err = np.linalg.norm(s.values - ref[e]) / np.linalg.norm(ref[e])
+        assert err <= 1e-8, (e, err)
+    log = (tmp_path / "log3.dat").read_text().splitlines()
+    assert log[:4] == ["FMS calculation of full Green's function ...", "FEFF-serial using 1 thread.",
+                       f"Using {len(k):5d} energy points.", "xprep done"]
+    assert log[4] == f"FMS for a cluster of {nat:4d} atoms"
+    assert log[-1] == "Done with module: FMS."
+    banner(f"gg.bin matches oracle over {len(k)} energies, cluster {nat} atoms")
+
+
+def test_module_mkgtr_writes_trace_matching_oracle(tmp_path):
+    banner("module mkgtr")
+    pts, iph, k, ph, rkk, em, radius, sig2g = make_workdir(tmp_path)
+    io.run_fms_in_dir(tmp_path, 0)
+    (tmp_path / "fms.bin").write_text("stale")  # a refreshed gg must invalidate traces
+    io.run_fms_in_dir(tmp_path, 0)
+    assert not (tmp_path / "fms.bin").exists()
+    assert io.run_mkgtr_in_dir(tmp_path) == 3
+    fb = io.parse_fms_bin((tmp_path / "fms.bin").read_text())
+    e_gtr, t_gtr = io.parse_gtr_dat((tmp_path / "gtr.dat").read_text())
+    # expected: oracle trace over the gg the stage wrote (FP64 throughout)
+    gg = np.stack([s.values for s in io.parse_gg_bin((tmp_path / "gg.bin").read_text())])
+    lind, b = orc.bmat_averaged(orc.core_hole_kappa(1), 0, 3, 0, 1)
+    ref = orc.mkgtr_trace(gg, rkk[:, 0], lind, b, 3, 1)
+    assert fb.energy_count == len(k) and fb.declared_spectrum_count == 0 and fb.spectrum_count() == 1
+    assert fb.cluster_radius_angstrom == radius and fb.pad_width == 8
+    assert np.all(np.abs(fb.spectra[0] - ref) <= 1e-10 * np.abs(ref).max())  # PAD(8) precision
+    assert np.all(np.abs(t_gtr - ref) <= 1e-6 * np.maximum(np.abs(ref), 1))  # F13.6
+    assert np.allclose(e_gtr, em, atol=1e-6)
+    assert (tmp_path / "log3.dat").read_text().endswith("MKGTR: Tracing over Green's function ...\n"
+                                                         "Done with module: MKGTR.\n")
+    banner("fms.bin / gtr.dat match oracle trace")

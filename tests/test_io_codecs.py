@@ -1,0 +1,512 @@
+"""FEFF handoff codecs of librefeff_fms.so against refeff-io's own known-answer
+tests (CPU only).
+
+Golden strings and values are the ones refeff's tests hold, cited per test
+(paths relative to the refeff repository root); round trips and precision
+bounds are the properties refeff's proptests state."""
+import numpy as np
+import pytest
+
+from refeff_amd import io
+from refeff_amd.io import CodecError
+
+
+# --- PAD: crates/refeff-io/src/pad.rs tests ----------------------------------
+
+def test_known_zero_encoding_matches_padlib_shape():
+    # pad.rs:250-255
+    assert io.encode_f64(0.0, 8) == "Q%%%%%%%"
+    assert io.decode_f64("Q%%%%%%%", 8) == 0.0
+
+
+def test_roundtrips_real_pad_values():
+    # pad.rs:226-235
+    values = [0.0, 1.0, -1.25, 12345.678, 1.0e-20]
+    got = io.decode_reals(io.encode_reals(values, 8), 8, len(values))
+    for a, e in zip(got, values):
+        assert abs(a - e) <= max(abs(e), 1.0) * 1e-6
+
+
+def test_roundtrips_complex_pad_values():
+    # pad.rs:237-248
+    values = np.array([1.0 - 2.0j, 0.25 + 1000.0j])
+    text = io.encode_complex(values, 8)
+    assert text.startswith("$")
+    got = io.decode_complex(text, 8, 2)
+    for a, e in zip(got, values):
+        assert abs(a.real - e.real) <= max(abs(e.real), 1) * 1e-6
+        assert abs(a.imag - e.imag) <= max(abs(e.imag), 1) * 1e-6
+
+
+def test_known_bug_npack3_rounding_carry_flips_sign():
+    # pad.rs:293-304: the npack=3 rounding carry lands in the sign byte
+    enc = io.encode_f64(1.1099547846496504e-18, 3)
+    assert np.signbit(io.decode_f64(enc, 3))
+
+
+def test_known_bug_rounding_drift_errors_for_wide_npack():
+    # pad.rs:306-311
+    io.encode_f64(-0.0009999999999980088, 8)
+    with pytest.raises(CodecError):
+        io.encode_f64(-0.0009999999999980088, 9)
+
+
+@pytest.mark.parametrize("npack", [4, 8, 11])
+def test_known_bug_huge_boundary_values_encode_as_zero(npack):
+    # pad.rs:313-329
+    for v in (1.0e38, -1.0e38, 1.0e39, -5.0e40):
+        assert io.decode_f64(io.encode_f64(v, npack), npack) == 0.0
+
+
+@pytest.mark.parametrize("npack", [4, 8, 12])
+def test_true_subnormal_magnitudes_collapse_to_zero(npack):
+    # pad.rs:338-360
+    tiny = np.finfo(float).tiny
+    for m in (5e-324, tiny / 2, tiny):
+        for s in (1.0, -1.0):
+            assert io.decode_f64(io.encode_f64(s * m, npack), npack) == 0.0
+
+
+def _pad_eps(npack):
+    # pad.rs:390-393: 5 * 90^-(npack-2), floored at 1e-13
+    return max(5.0 * 90.0 ** (-(npack - 2)), 1e-13)
+
+
+def test_single_values_roundtrip_within_precision():
+    # pad.rs proptest single_value_roundtrips_within_precision, seeded sample
+    rng = np.random.default_rng(7)
+    for _ in range(3000):
+        npack = int(rng.integers(4, 16))
+        v = rng.uniform(1, 10) * 10.0 ** int(rng.integers(-36, 37)) * (1 if rng.random() < 0.5 else -1)
+        try:
+            enc = io.encode_f64(v, npack)
+        except CodecError:
+            continue  # the documented drift band (pad.rs:306-311) is skipped there too
+        assert len(enc) == npack
+        assert abs(io.decode_f64(enc, npack) - v) / abs(v) <= _pad_eps(npack)
+
+
+def test_sign_is_preserved_just_above_tiny():
+    # pad.rs proptest sign_is_preserved_just_above_the_tiny_boundary
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        npack = int(rng.integers(4, 16))
+        v = rng.uniform(1.1e-38, 9.9e-38) * (1 if rng.random() < 0.5 else -1)
+        d = io.decode_f64(io.encode_f64(v, npack), npack)
+        assert np.signbit(d) == (v < 0)
+        assert abs(d - v) / abs(v) <= _pad_eps(npack)
+
+
+def test_pad_lines_break_at_the_padlib_width():
+    # pad.rs:122-160: a line is closed once its payload reaches 82 - npack + 1 characters
+    text = io.encode_reals(np.arange(1, 25, dtype=float), 8)
+    lines = text.splitlines()
+    assert all(l[0] == "!" for l in lines)
+    assert [len(l) - 1 for l in lines] == [80, 80, 32]
+    ctext = io.encode_complex(np.arange(1, 12) * (1 + 1j), 8)
+    assert [len(l) - 1 for l in ctext.splitlines()] == [80, 80, 16]  # closes at 82 - 2*8 + 1 = 67
+
+
+def test_pad_decoding_rejects_bad_markers_and_payloads():
+    with pytest.raises(CodecError):
+        io.decode_reals("$Q%%%%%%%\n", 8, 1)
+    with pytest.raises(CodecError):
+        io.decode_reals("!Q%%%%%%\n", 8, 1)
+    with pytest.raises(CodecError):
+        io.encode_f64(1.0, 2)
+
+
+# --- Fortran E20.10: crates/refeff-io/src/format.rs, gg_dat.rs tests ---------
+
+def test_fortran_zero_scaled_exp_matches_feff_fields():
+    # gg_dat.rs:685-694 byte fixture:  "    0.1000000000E+01    -0.2500000000E+01"
+    assert io.fortran_zero_scaled_exp(1.0, 20, 10) == "    0.1000000000E+01"
+    assert io.fortran_zero_scaled_exp(-2.5, 20, 10) == "   -0.2500000000E+01"
+    # format.rs:179-187: mantissa in [0.1, 1), e.g. 0.3809984030E-01
+    assert io.fortran_zero_scaled_exp(0.0380998403, 16, 10) == "0.3809984030E-01"
+    # format.rs:188-191: IEEE negative zero keeps its sign
+    assert io.fortran_zero_scaled_exp(-0.0, 20, 10) == "   -0.0000000000E+00"
+    assert io.fortran_zero_scaled_exp(0.0, 20, 10) == "    0.0000000000E+00"
+    # no renormalisation after rounding: 0.99999999999 prints a 1.0 mantissa
+    assert io.fortran_zero_scaled_exp(0.99999999999, 20, 10) == "    1.0000000000E+00"
+    # three-digit exponents widen the field
+    assert io.fortran_zero_scaled_exp(1.5e-120, 20, 10) == "   0.1500000000E-119"
+
+
+def test_fortran_exp_round_trips_ten_significant_digits():
+    rng = np.random.default_rng(1)
+    for v in rng.normal(size=500) * 10.0 ** rng.integers(-30, 30, size=500):
+        s = io.fortran_zero_scaled_exp(v, 20, 10)
+        assert len(s) == 20
+        e = int(s[-3:])  # mantissa in [0.1, 1): ten decimals leave at most half a unit of 10^(e-10)
+        assert 0.1 <= abs(float(s[:-4])) <= 1.0
+        assert abs(float(s) - v) <= 0.5e-10 * 10.0 ** e * (1 + 1e-6)
+
+
+# --- gg.dat / gg.bin: crates/refeff-io/src/gg_dat.rs tests -------------------
+
+GG_DAT = """#SN#   Section:    1
+#DF# This section written in txt.
+#H#
+#DT# 2D complex array with sizes    2   2
+#H# File is organized as follows:  Array(1,i)     Array(1,i+1)    Array(1,i+2)  . . .
+#H#                                Array(2,i)
+    1.0000000000E+00    -5.0000000000E-01    2.0000000000E+00     2.5000000000E+00
+    3.0000000000E+00     0.0000000000E+00   -4.0000000000E+00     7.5000000000E-01
+#SN#   Section:    2
+#DT# 2D complex array with sizes    1   2
+    5.0000000000E+00    -5.5000000000E+00    6.0000000000E+00    -6.5000000000E+00
+"""
+
+
+def test_parses_gg_dat_sections():
+    # gg_dat.rs:652-670
+    s = io.parse_gg_dat(GG_DAT)
+    assert len(s) == 2 and s[0].section_number == 1
+    assert s[0].values.shape == (2, 2)
+    assert s[0].values[0, 0] == 1.0 - 0.5j
+    assert s[0].values[1, 1] == -4.0 + 0.75j
+    assert s[1].values.shape == (1, 2)
+    assert s[1].values[0, 1] == 6.0 - 6.5j
+    again = io.parse_gg_dat(io.gg_dat_string(s))
+    for a, b in zip(again, s):
+        assert a.section_number == b.section_number and np.array_equal(a.values, b.values)
+
+
+def test_accepts_fortran_d_exponents_and_header_data_prefix():
+    # gg_dat.rs:672-683
+    s = io.parse_gg_dat("#SN#   Section:    1\n#DT# 2D double complex array with sizes    1   1\n"
+                        "#DT# 2D complex array with sizes    1   1\n#HD# 1.0D+00 -2.5D+00\n")
+    assert len(s) == 1 and s[0].values[0, 0] == 1.0 - 2.5j
+
+
+def test_canonical_gg_dat_text_matches_feff_layout():
+    # gg_dat.rs:685-694 (canonical descriptor) and gg_dat.rs:213-233 prefix
+    text = ("#SN#   Section:    1\n#DF# This section written in txt.\n#H#\n"
+            "#DT# 2D complex array with sizes    1   1\n"
+            "#H# File is organized as follows:  Array(1,i)     Array(1,i+1)    Array(1,i+2)  . . .\n"
+            "#H#                                Array(2,i)\n#H#                                     .\n"
+            "#H#                                     .\n#H#                                     .\n"
+            "    0.1000000000E+01    -0.2500000000E+01\n")
+    s = io.parse_gg_dat(text)
+    assert s[0].values[0, 0] == 1.0 - 2.5j
+    assert io.gg_dat_string(s) == text
+
+
+def test_descriptor_payload_with_embedded_line_feed_is_skipped():
+    # gg_dat.rs:696-706: a #DF# payload may carry a line feed before #H#
+    text = ("#SN#   Section:    1\n#DF# This section written in v\n.\n#H#\n"
+            "#DT# 2D complex array with sizes    1   1\n"
+            "    0.1000000000E+01    -0.2500000000E+01\n")
+    assert io.parse_gg_dat(text)[0].values[0, 0] == 1.0 - 2.5j
+
+
+@pytest.mark.parametrize("bad", [
+    "",
+    "#SN# Section: 0\n#DT# 2D complex array with sizes 1 1\n1 2\n",
+    "#SN# Section: 1\n1 2\n",
+    "#SN# Section: 1\n#DT# 2D real array with sizes 1 1\n1 2\n",
+    "#SN# Section: 1\n#DT# 2D complex array with sizes 1 1\n1\n",
+    "#SN# Section: 1\n#DT# 2D complex array with sizes 1 1\nNaN 1\n",
+])
+def test_rejects_bad_gg_dat_inputs(bad):
+    # gg_dat.rs:752-765
+    with pytest.raises(CodecError):
+        io.parse_gg_dat(bad)
+
+
+# --- fms.bin: crates/refeff-io/src/fms_bin.rs tests --------------------------
+
+def sample_fms_bin():
+    # fms_bin.rs:511-530
+    e = np.arange(3)[None, :] + 1.0
+    s = np.arange(2)[:, None] * 1.0
+    spectra = (0.25 * e + s) + 1j * (-0.05 * e - s)
+    return io.FmsBinData(5.5, 3, 2, 1, 1, 8, 2, spectra)
+
+
+def test_writes_fms_header_like_feff():
+    # fms_bin.rs:394-405
+    lines = io.fms_bin_string(sample_fms_bin()).splitlines()
+    assert lines[0] == "FMS rfms= 5.5000"
+    assert lines[1] == "       3       2       1       1       8       2"
+    assert lines[2].startswith("$")
+
+
+def test_roundtrips_fms_bin_text_with_pad_tolerance():
+    # fms_bin.rs:407-427
+    d = sample_fms_bin()
+    p = io.parse_fms_bin(io.fms_bin_string(d))
+    assert (p.cluster_radius_angstrom, p.energy_count, p.main_energy_count, p.auxiliary_energy_count,
+            p.highest_potential_index, p.pad_width) == (5.5, 3, 2, 1, 1, 8)
+    assert p.spectra.shape == d.spectra.shape
+    assert np.all(np.abs(p.spectra.real - d.spectra.real) <= np.maximum(np.abs(d.spectra.real), 1) * 1e-6)
+    assert np.all(np.abs(p.spectra.imag - d.spectra.imag) <= np.maximum(np.abs(d.spectra.imag), 1) * 1e-6)
+
+
+def test_parses_legacy_five_integer_header_as_single_spectrum():
+    # fms_bin.rs:436-455
+    e = np.arange(3)
+    sp = (0.1 * e - 0.01j * e)[None, :]
+    text = "FMS rfms= 5.5000\n   3   2   1   1   8\n" + io.encode_complex(sp.ravel(), 8)
+    p = io.parse_fms_bin(text)
+    assert p.spectrum_count() == 1 and p.declared_spectrum_count is None and p.energy_count == 3
+    assert io.fms_bin_string(p) == text
+
+
+def test_infers_zero_declared_spectrum_count_from_feff_payload():
+    # fms_bin.rs:457-475
+    e = np.arange(3)
+    payload = io.encode_complex(0.1 * e - 0.01j * e, 8)
+    text = "FMS rfms=-1.0000\n       3       2       0       1       8       0\n" + payload
+    p = io.parse_fms_bin(text)
+    assert p.spectrum_count() == 1 and p.declared_spectrum_count == 0 and p.spectra.shape == (1, 3)
+    assert io.fms_bin_string(p) == text
+
+
+def test_parses_empty_zero_spectrum_fms_bin():
+    # fms_bin.rs:477-483
+    p = io.parse_fms_bin("FMS rfms=-1.0000\n   3   2   0   1   8   0\n")
+    assert p.spectrum_count() == 0 and p.declared_spectrum_count == 0 and p.spectra.shape == (0, 3)
+
+
+def test_rejects_bad_fms_bin_shapes_and_tokens():
+    # fms_bin.rs:485-509
+    bad = sample_fms_bin()
+    bad.energy_count = 4
+    with pytest.raises(CodecError):
+        io.fms_bin_string(bad)
+    with pytest.raises(CodecError):
+        io.parse_fms_bin("FMS rfms= nope\n       3       2       1       1       8       1\n")
+    with pytest.raises(CodecError):
+        io.parse_fms_bin("FMS rfms=-1.0000\n       3       2\n")
+
+
+# --- fms.inp: crates/refeff-io/src/fms_input.rs tests ------------------------
+
+LEGACY_FMS_INP = ("mfms, idwopt, minv\n"
+                  "   1  -1   0\n"
+                  "rfms2, rdirec, toler1, toler2\n"
+                  "      6.00000     12.00000      0.00100      0.00100\n"
+                  "tk, thetad, sig2g\n"
+                  "      0.00000      0.00000      0.00000\n"
+                  " lmaxph(0:nph)\n"
+                  "   2   2   2\n"
+                  " the number of decomposi\n"
+                  "   -1\n")
+
+
+def test_parses_legacy_fms_input_without_save_slice_tail():
+    # fms_input.rs:460-483
+    f = io.FmsInput.parse_str(LEGACY_FMS_INP)
+    assert (f.mfms, f.idwopt, f.minv) == (1, -1, 0)
+    assert f.lmaxph == [2, 2, 2] and f.decomposition_channels == -1
+    assert not f.save_gg_slice and f.do_fms == 1
+    assert (f.rfms2, f.rdirec, f.toler1, f.toler2) == (6.0, 12.0, 0.001, 0.001)
+
+
+def test_generated_copper_fms_input_round_trips():
+    # fms_input.rs:431-458 / 485-506: FMS 7.0 1 2 0.002 0.003 10.0, DEBYE 300 400
+    f = io.FmsInput(mfms=1, idwopt=0, minv=2, rfms2=7.0, rdirec=10.0, toler1=0.002, toler2=0.003, tk=300.0,
+                    thetad=400.0, sig2g=0.0, lmaxph=[3, 3], decomposition_channels=-1, save_gg_slice=False, do_fms=1)
+    text = io.fms_input_string(f)
+    assert text.splitlines()[1] == "   1   0   2"
+    assert text.splitlines()[3] == "      7.00000     10.00000      0.00200      0.00300"
+    assert text.splitlines()[5] == "    300.00000    400.00000      0.00000"
+    assert text.splitlines()[-8:] == [" lmaxph(0:nph)", "   3   3", " the number of decomposi", "   -1",
+                                      " save_gg_slice", "F", "do_fms", "   1"]
+    assert text.endswith("do_fms\n   1\n")
+    g = io.FmsInput.parse_str(text)
+    assert g == f
+    assert io.fms_input_string(g) == text
+
+
+def test_rejects_invalid_fms_rendering_and_headers():
+    # fms_input.rs:559-585
+    f = io.FmsInput(rfms2=float("nan"), rdirec=10.0, lmaxph=[3])
+    with pytest.raises(CodecError):
+        io.fms_input_string(f)
+    f = io.FmsInput(rfms2=7.0, rdirec=10.0, lmaxph=[])
+    with pytest.raises(CodecError):
+        io.fms_input_string(f)
+    with pytest.raises(CodecError):
+        io.FmsInput.parse_str(LEGACY_FMS_INP.replace("mfms, idwopt, minv", "mfms idwopt minv"))
+    with pytest.raises(CodecError):
+        io.FmsInput.parse_str(LEGACY_FMS_INP + "extra\n")
+
+
+# --- phase.bin: crates/refeff-io/src/phase_bin/tests.rs ----------------------
+
+def sample_potential(lmax, iz, label, ne, nsp, scale):
+    # phase_bin/tests.rs:1261-1284
+    e = np.arange(ne)[:, None, None]
+    l = np.arange(2 * lmax + 1)[None, :, None]
+    s = np.arange(nsp)[None, None, :]
+    return io.PhaseBinPotential(lmax, iz, label, (scale + 0.01 * e + 0.1 * l + 0 * s) + 1j * (0.001 * s + 0 * e * l))
+
+
+def sample_phase_bin():
+    # phase_bin/tests.rs:1053-1095
+    nsp, ne, nq, ntr = 2, 3, 2, 3
+    e = np.arange(ne)
+    em = (0.5 + e) + 1j * (0.1 * e)
+    eref = (-1.0 + e[:, None] * 0.2) + 1j * (0.05 * np.arange(nsp)[None, :])
+    E, Q, T, S = np.meshgrid(np.arange(ne), np.arange(nq), np.arange(ntr), np.arange(nsp), indexing="ij")
+    rkk = (0.01 * (E + 1) + 0.1 * Q + T) + 1j * (-0.02 * S)
+    pots = [sample_potential(1, 29, "Cu", ne, nsp, 0.1), sample_potential(2, 8, "O", ne, nsp, 0.2)]
+    return io.PhaseBinData(nsp, ne, 2, 1, 4, 2, 8, 4, ntr, nq, (1.2, -0.35, 9.8), em, eref, pots, rkk)
+
+
+def test_writes_phase_bin_header_like_feff():
+    # phase_bin/tests.rs:20-31
+    text = io.phase_bin_string(sample_phase_bin())
+    assert text.splitlines()[0] == "    2    3    2    1    1    4    2    8    4    3    2"
+    assert "   1  29 Cu    " in text.splitlines()
+    assert "   2   8 O     " in text.splitlines()
+
+
+def test_roundtrips_phase_bin_text_with_pad_tolerance():
+    # phase_bin/tests.rs:33-62
+    d = sample_phase_bin()
+    p = io.parse_phase_bin(io.phase_bin_string(d))
+    for attr in ("spin_count", "energy_count", "main_energy_count", "auxiliary_energy_count", "ihole", "fermi_index",
+                 "pad_width", "final_state_count", "transition_count", "q_count"):
+        assert getattr(p, attr) == getattr(d, attr), attr
+
+    def close(a, b):
+        a, b = np.asarray(a), np.asarray(b)
+        assert a.shape == b.shape
+        assert np.all(np.abs(a - b) <= 1e-6 * np.maximum(np.abs(b), 1))
+
+    close(p.scalars, d.scalars)
+    close(p.energy_grid, d.energy_grid)
+    close(p.reference_energy, d.reference_energy)
+    close(p.transition_moments, d.transition_moments)
+    for a, b in zip(p.potentials, d.potentials):
+        assert (a.lmax, a.atomic_number, a.label) == (b.lmax, b.atomic_number, b.label)
+        close(a.phase_shifts, b.phase_shifts)
+
+
+def test_phase_bin_rejects_bad_headers():
+    text = io.phase_bin_string(sample_phase_bin())
+    with pytest.raises(CodecError):
+        io.parse_phase_bin("    2    3    2\n" + text.split("\n", 1)[1])
+    with pytest.raises(CodecError):
+        io.parse_phase_bin(text + "trailing\n")
+
+
+# --- geom.dat / global.inp / gtr.dat -----------------------------------------
+
+def test_geom_dat_round_trip_and_layout():
+    # structure_output.rs:204-222 writer layout; rows of the rhorrp sample (fms_input.rs:617-661)
+    a0 = 0.529177249
+    g = io.GeomDat(4, 2, [1, 2, 4], np.array([1, 2, 3, 4]),
+                   np.array([[0, 0, 0], [a0, 0, 0], [0.4 * a0, 0, 0], [0, 2 * a0, 0]]),
+                   np.array([0, 1, 0, 2]), np.array([1, 1, 1, 1]))
+    text = io.geom_dat_string(g)
+    lines = text.splitlines()
+    assert lines[0] == "nat, nph =     4    2"
+    assert lines[1] == "    1    2    4"
+    assert lines[2] == " iat     x       y        z       iph  "
+    assert lines[3] == " " + "-" * 71
+    assert lines[5] == "   2      0.52918      0.00000      0.00000   1   1"
+    p = io.GeomDat.parse_str(text)
+    assert (p.nat, p.nph, p.model_atoms) == (4, 2, [1, 2, 4])
+    assert np.allclose(p.xyz, np.round(g.xyz, 5))
+    assert list(p.iph) == [0, 1, 0, 2]
+
+
+LEGACY_GLOBAL = """\
+ nabs, iphabs - CFAVERAGE data
+       1       0 100000.00000
+ ipol, ispin, le2, elpty, angks, l2lp, do_nrixs, ldecmx, lj
+    1    0   10      1.0000      0.0000    0    1    2   10
+evec\t\t  xivec \t   spvec
+      0.70711      0.00000      0.00000
+      0.70711      0.00000      0.00000
+      0.00000      1.00000      0.00000
+ polarization tensor
+      0.50000      0.00000      0.00000      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000      0.00000      0.50000      0.00000
+evnorm, xivnorm, spvnorm - only used for nrixs
+      0.00000     24.00000      0.00000
+nq,    imdff,   qaverage,   mixdff,   qqmdff,   cos<q,q'>
+           1           0 T F  -1.00000000000000        1.00000000000000
+ q-vectors : qx, qy, qz, q(norm), weight, qcosth, qsinth, qcosfi, qsinfi
+      0.00000      0.00000     24.00000     24.00000      1.00000      0.00000     -1.00000      0.00000      1.00000      0.00000
+"""
+
+
+def test_parses_legacy_nrixs_global_input():
+    # global_input.rs:609-640
+    g = io.parse_global_input(LEGACY_GLOBAL)
+    assert g["do_nrixs"] == 1 and g["nq"] == 1 and g["qaverage"] and not g["mixdff"]
+    assert (g["ipol"], g["ispin"], g["le2"], g["ldecmx"], g["lj"]) == (1, 0, 10, 2, 10)
+    assert g["evec"] == [0.70711, 0.70711, 0.0] and g["xivec"] == [0.0, 0.0, 1.0]
+    assert g["polarization_tensor"][2][4] == 0.5 and g["xivnorm"] == 24.0
+
+
+GTR_DAT = ("    -0.138801     0.031773     0.000000     0.000000\n"
+           "    -0.137401     0.031773     0.000000     0.000000\n"
+           "    55.866911     0.031773     1.624106     1.081113\n")
+
+
+def test_parses_and_renders_gtr_dat():
+    # gtr_dat.rs:149-163
+    e, t = io.parse_gtr_dat(GTR_DAT)
+    assert len(e) == 3 and e[0] == -0.138801 + 0.031773j and t[0] == 0
+    assert e[2] == 55.866911 + 0.031773j and t[2] == 1.624106 + 1.081113j
+    assert io.gtr_dat_string(e, t) == GTR_DAT
+
+
+def test_gtr_dat_fortran_d_exponents_and_rejections():
+    # gtr_dat.rs:165-181
+    e, t = io.parse_gtr_dat(" -1.38801D-01 3.1773D-02 0.0D+00 1.0D+00\n")
+    assert e[0] == -0.138801 + 0.031773j and t[0] == 1j
+    for bad in ("", "1 2 3\n", "1 2 3 4 5\n", "1 2 NaN 4\n", "1 bad 3 4\n"):
+        with pytest.raises(CodecError):
+            io.parse_gtr_dat(bad)
+
+
+# --- module stages that stop before any device work --------------------------
+
+def _write(d, name, text):
+    (d / name).write_text(text)
+
+
+def test_module_stage_is_a_no_op_when_fms_is_disabled(tmp_path):
+    # engine fms.rs:462-466: mfms = 0 writes nothing
+    _write(tmp_path, "fms.inp", LEGACY_FMS_INP.replace("   1  -1   0", "   0  -1   0"))
+    assert io.run_fms_in_dir(tmp_path) == 0
+    assert io.run_mkgtr_in_dir(tmp_path) == 0
+
+
+def test_module_stage_requires_sources_or_cache(tmp_path):
+    # engine fms.rs:469-477
+    _write(tmp_path, "fms.inp", LEGACY_FMS_INP)
+    with pytest.raises(CodecError) as e:
+        io.run_fms_in_dir(tmp_path)
+    assert "requires cached FMS output" in str(e.value)
+    with pytest.raises(CodecError):
+        io.run_mkgtr_in_dir(tmp_path)
+
+
+def test_module_stage_re_emits_a_cached_gg_canonically(tmp_path):
+    # engine fms.rs:484-499: a cached gg.bin is read and written back
+    _write(tmp_path, "fms.inp", LEGACY_FMS_INP)
+    _write(tmp_path, "gg.bin", GG_DAT)
+    assert io.run_fms_in_dir(tmp_path) == 2  # gg.bin + log3.dat
+    s = io.parse_gg_dat((tmp_path / "gg.bin").read_text())
+    assert s[0].values[1, 1] == -4.0 + 0.75j
+    assert (tmp_path / "gg.bin").read_text().startswith("#SN#   Section:    1\n#DF# This section written in txt.\n")
+    assert "Done with module: FMS." in (tmp_path / "log3.dat").read_text()
+
+
+def test_module_stage_rejects_unprovided_debye_models_before_solving(tmp_path):
+    _write(tmp_path, "fms.inp", LEGACY_FMS_INP.replace("   1  -1   0", "   1   0   0"))
+    for name in ("phase.bin", "geom.dat", "global.inp"):
+        _write(tmp_path, name, "")
+    with pytest.raises(CodecError) as e:
+        io.run_fms_in_dir(tmp_path)
+    assert e.value.name == "UNSUPPORTED" and "idwopt=0" in str(e.value)
