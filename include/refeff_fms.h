@@ -60,7 +60,7 @@ typedef enum refeff_fms_solver {
     REFEFF_FMS_SOLVER_AUTO = 0,    /* minv=0 -> MIXED, minv=1 -> GGBI, other minv -> UNSUPPORTED */
     REFEFF_FMS_SOLVER_MIXED = 1,   /* complex FP32 MFMA LU + FP64 iterative refinement */
     REFEFF_FMS_SOLVER_FP64 = 2,    /* plain complex FP64 MFMA LU */
-    REFEFF_FMS_SOLVER_GGBI = 3     /* refeff minv=1 one-shot BiCGStab (FP32, as refeff) */
+    REFEFF_FMS_SOLVER_GGBI = 3     /* refeff minv=1 one-shot BiCGStab, in FP64 (refeff runs it in c32) */
 } refeff_fms_solver;
 
 /* ---------------------------------------------------------------------- */

@@ -188,6 +188,7 @@ struct BuildArgs {
     size_t sa, sb;
     int nat, n, lda, nsp, L, nrhs, nbatch;
     float rdirec2;
+    double toler2;   // G0 magnitude cutoff of the iterative system (GG mode)
 };
 
 // FEFF rotxan small-d recursion into d0[l][n][m] (1-based n, m), restated from
@@ -233,7 +234,10 @@ __device__ void small_d(int lmax, double beta, double* d0) {
     }
 }
 
-template <bool W32, bool W64>
+// GG = false: A = 1 - G0 t (FP32 and/or FP64) and the RHS columns G0[:, rep].
+// GG = true (minv = 1): A64 holds G0 with entries |G0| <= toler2 dropped, and
+// B64 the rows of the representative blocks, B[j][rhs + r] = G0[rep + r, j].
+template <bool W32, bool W64, bool GG = false>
 __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
     constexpr int S = 2 * kMaxL + 2;
     __shared__ double d0[2][(kMaxL + 2) * S * S];
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
     const int la = g.atom_lmax[a], lb = g.atom_lmax[b];
     const int fa = g.atom_first[a], fb = g.atom_first[b];
     const int na = (la + 1) * (la + 1) * nsp, nbk = (lb + 1) * (lb + 1) * nsp;
-    const int rhs = g.atom_rhs[b];
+    const int rhs = GG ? g.atom_rhs[a] : g.atom_rhs[b];
     const int cells = na * nbk;
 
     // cutoff decided in single precision, as refeff does (pairs.rs:297-303)
@@ -262,13 +266,15 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
         for (int e = 0; e < g.nbatch; ++e) {
             for (int idx = tid; idx < cells; idx += blockDim.x) {
                 const int r = idx / nbk, c = idx - r * nbk;
-                const double v = (a == b && r == c) ? 1.0 : 0.0;
+                const double v = (!GG && a == b && r == c) ? 1.0 : 0.0;
                 const size_t at = static_cast<size_t>(e) * g.sa + static_cast<size_t>(fa + r) * g.lda + fb + c;
                 if (W32) g.a32[at] = make_float2((float)v, 0.0f);
                 if (W64) g.a64[at] = make_double2(v, 0.0);
-                if (rhs >= 0)
-                    g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fa + r) * g.nrhs + rhs + c] =
-                        make_double2(0.0, 0.0);
+                if (rhs >= 0) {
+                    const size_t bi = GG ? static_cast<size_t>(fb + c) * g.nrhs + rhs + r
+                                         : static_cast<size_t>(fa + r) * g.nrhs + rhs + c;
+                    g.b64[static_cast<size_t>(e) * g.sb + bi] = make_double2(0.0, 0.0);
+                }
             }
         }
         return;
@@ -397,6 +403,12 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
                 return acc;
             };
             const double2 gv = g0(c);
+            if (GG) {
+                const size_t at = static_cast<size_t>(e) * g.sa + static_cast<size_t>(fa + r) * g.lda + fb + c;
+                g.a64[at] = (sqrt(gv.x * gv.x + gv.y * gv.y) > g.toler2) ? gv : make_double2(0.0, 0.0);
+                if (rhs >= 0) g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fb + c) * g.nrhs + rhs + r] = gv;
+                continue;
+            }
             const int cs = fb + c;
             double2 v = cmul(gv, tm0[cs]);
             v = make_double2(-v.x, -v.y);
@@ -436,12 +448,14 @@ __global__ __launch_bounds__(256) void k_pad(typename Cx<T>::type* A, size_t sa,
 }
 
 __global__ void k_fill_int(int* p, int v, size_t n) {
-    const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-    if (i < n) p[i] = v;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        p[i] = v;
 }
 __global__ void k_fill_double(double* p, double v, size_t n) {
-    const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-    if (i < n) p[i] = v;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1426,6 +1440,206 @@ __global__ __launch_bounds__(256) void k_update(double2* X, const typename Cx<T>
 }
 
 // scattering[e][ip][row + ch*col] = X[e][rep[ip] + row][col0[ip] + col]
+// ---------------------------------------------------------------------------
+// FEFF ggbi (minv = 1): one-shot BiCGStab on M x = e_src with M = 1 - T G0
+// (refeff fms/internals/iterative.rs:24-254, fms/solvers.rs:22-78), then
+// gg = G0[rep rows] x.  M v = v - t0 (G0c v) - t1[q] (G0c v)[q] with G0c the
+// toler2-cut G0 and q the spin partner of the row (linear.rs:163-201).  With
+// r = e_src the shadow vector y and its products reduce to aa = <M r, M r> and
+// wa = (M r)[src], so the first half needs no matvec; the one true matvec per
+// column is the batched FP64 MFMA product of G0c with S.
+// ---------------------------------------------------------------------------
+
+struct GgbiArgs {
+    const double2* g0c;   // [e] npad x npad, A64 layout
+    const double2* tmat;  // [e][2][n]
+    const int* sl;
+    const int* sm;
+    const int* sspin;     // 1-based
+    const int* rhs_src;   // state index of each RHS column
+    double2* S;           // [e][npad][nrhs]: sv, then the solution x
+    const double2* R;     // [e][npad][nrhs]: -G0c S (second kernel)
+    double2* omega;       // [e][nrhs]
+    int* within;          // [e][nrhs]
+    int* info;            // [e]: nonzero -> breakdown
+    size_t sa, sx;
+    int n, npad, lda, nrhs, nsp;
+    double tol;
+};
+
+__device__ __forceinline__ int spin_partner_row(const GgbiArgs& a, int i) {
+    const int l = a.sl[i], m = a.sm[i], s = a.sspin[i];
+    const int proj = m + s;
+    if (proj <= -l + 1 || proj >= l + 2) return -1;
+    return s == 1 ? i - 1 : i + 1;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < NT / 64; ++i) s += red[i];
+    return s;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < NT / 64; ++i) s = fmax(s, red[i]);
+    return s;
+}
+
+// av = M e_src (column src of M), omega, sv = e_src - omega av; grid (nrhs, nb)
+__global__ __launch_bounds__(256) void k_ggbi_first(GgbiArgs a) {
+    __shared__ double red[4];
+    __shared__ double2 wa_s;
+    const int c = blockIdx.x, e = blockIdx.y, tid = threadIdx.x;
+    const int src = a.rhs_src[c];
+    const double2* G = a.g0c + e * a.sa;
+    const double2* t0 = a.tmat + static_cast<size_t>(e) * 2 * a.n;
+    const double2* t1 = t0 + a.n;
+    auto av_at = [&](int i) {
+        double2 v = make_double2(i == src ? 1.0 : 0.0, 0.0);
+        v = csub(v, cmul(t0[i], G[static_cast<size_t>(i) * a.lda + src]));
+        if (a.nsp == 2) {
+            const int q = spin_partner_row(a, i);
+            if (q >= 0) v = csub(v, cmul(t1[q], G[static_cast<size_t>(q) * a.lda + src]));
+        }
+        return v;
+    };
+    if (tid == 0) wa_s = av_at(src);
+    double aa = 0.0;
+    for (int i = tid; i < a.n; i += 256) {
+        const double2 v = av_at(i);
+        aa += v.x * v.x + v.y * v.y;
+    }
+    aa = block_sum<256>(aa, red);  // synchronises, so wa_s is visible
+    const double2 wa = wa_s;
+    // r = e_src is within tolerance only when tol >= 1: x = 0
+    const bool r_within = 1.0 <= a.tol;
+    double2 omega = make_double2(0.0, 0.0);
+    bool bad = false;
+    if (!r_within) {
+        // dd = aa ww - conj(wa) wa with ww = <r, r> = 1; y = r / ww or r aa' + av ww'
+        const double2 aac = make_double2(aa, 0.0);
+        const double2 dd = csub(aac, cmul(make_double2(wa.x, -wa.y), wa));
+        const double2 sdd = cdiv(cdiv(dd, aac), make_double2(1.0, 0.0));
+        double2 del, delp;
+        if (sqrt(sdd.x * sdd.x + sdd.y * sdd.y) < 1.0e-8) {
+            del = make_double2(1.0, 0.0);
+            delp = wa;
+        } else {
+            const double2 ww2 = cdiv(csub(make_double2(1.0, 0.0), make_double2(wa.x, -wa.y)), dd);
+            const double2 aa2 = cdiv(csub(wa, aac), dd);
+            const double2 ysrc = cadd(aa2, cmul(wa, ww2));
+            del = make_double2(ysrc.x, -ysrc.y);
+            delp = cadd(cmul(make_double2(aa2.x, -aa2.y), wa), cmul(make_double2(ww2.x, -ww2.y), aac));
+        }
+        bad = (aa == 0.0) || (delp.x == 0.0 && delp.y == 0.0);
+        if (!bad) omega = cdiv(del, delp);
+    }
+    double2* S = a.S + e * a.sx;
+    double mx = 0.0;
+    for (int i = tid; i < a.npad; i += 256) {
+        double2 sv = make_double2(0.0, 0.0);
+        if (i < a.n && !r_within) {
+            const double2 v = av_at(i);
+            sv = csub(make_double2(i == src ? 1.0 : 0.0, 0.0), cmul(omega, v));
+            mx = fmax(mx, fmax(fabs(sv.x), fabs(sv.y)));
+        }
+        S[static_cast<size_t>(i) * a.nrhs + c] = sv;
+    }
+    mx = block_max<256>(mx, red);
+    if (tid == 0) {
+        a.omega[static_cast<size_t>(e) * a.nrhs + c] = omega;
+        a.within[static_cast<size_t>(e) * a.nrhs + c] = r_within ? 2 : (mx <= a.tol ? 1 : 0);
+        if (bad) atomicOr(&a.info[e], 1 << 29);
+    }
+}
+
+// AS = M sv from R = -G0c S; chi = <AS, sv> / <AS, AS>; x = omega e_src + chi sv
+__global__ __launch_bounds__(256) void k_ggbi_second(GgbiArgs a) {
+    __shared__ double red[4];
+    const int c = blockIdx.x, e = blockIdx.y, tid = threadIdx.x;
+    const int src = a.rhs_src[c];
+    const double2* t0 = a.tmat + static_cast<size_t>(e) * 2 * a.n;
+    const double2* t1 = t0 + a.n;
+    double2* S = a.S + e * a.sx;
+    const double2* R = a.R + e * a.sx;
+    const int flag = a.within[static_cast<size_t>(e) * a.nrhs + c];
+    const double2 omega = a.omega[static_cast<size_t>(e) * a.nrhs + c];
+    double aa = 0.0, war = 0.0, wai = 0.0;
+    if (flag == 0) {
+        for (int i = tid; i < a.n; i += 256) {
+            const size_t o = static_cast<size_t>(i) * a.nrhs + c;
+            double2 as = cadd(S[o], cmul(t0[i], R[o]));
+            if (a.nsp == 2) {
+                const int q = spin_partner_row(a, i);
+                if (q >= 0) as = cadd(as, cmul(t1[q], R[static_cast<size_t>(q) * a.nrhs + c]));
+            }
+            aa += as.x * as.x + as.y * as.y;
+            const double2 sv = S[o];
+            war += as.x * sv.x + as.y * sv.y;  // conj(as) sv
+            wai += as.x * sv.y - as.y * sv.x;
+        }
+    }
+    aa = block_sum<256>(aa, red);
+    war = block_sum<256>(war, red);
+    wai = block_sum<256>(wai, red);
+    double2 chi = make_double2(0.0, 0.0);
+    if (flag == 0) {
+        if (aa == 0.0) {
+            if (tid == 0) atomicOr(&a.info[e], 1 << 29);
+        } else {
+            chi = cdiv(make_double2(war, wai), make_double2(aa, 0.0));
+        }
+    }
+    for (int i = tid; i < a.npad; i += 256) {
+        const size_t o = static_cast<size_t>(i) * a.nrhs + c;
+        double2 x = make_double2(0.0, 0.0);
+        if (flag != 2 && i < a.n) {
+            x = (i == src) ? omega : make_double2(0.0, 0.0);
+            if (flag == 0) x = cadd(x, cmul(chi, S[o]));
+        }
+        S[o] = x;
+    }
+}
+
+// gg block of each RHS column: out[(rep_q + r)][c] = sum_j B[j][col0_q + r] x_c[j];
+// one wave per output row r, grid (nrhs, nb)
+__global__ __launch_bounds__(256) void k_ggbi_gg(const double2* Bb, const double2* Xb, double2* Ob, size_t sx,
+                                                 int n, int nrhs, const int* rhs_pot, const int* rep,
+                                                 const int* col0, const int* ipart) {
+    const int c = blockIdx.x, e = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = rhs_pot[c];
+    const double2* B = Bb + e * sx;
+    const double2* X = Xb + e * sx;
+    double2* O = Ob + e * sx;
+    for (int r = wave; r < ipart[q]; r += 4) {
+        double sr = 0.0, si = 0.0;
+        for (int j = lane; j < n; j += 64) {
+            const double2 b = B[static_cast<size_t>(j) * nrhs + col0[q] + r];
+            const double2 x = X[static_cast<size_t>(j) * nrhs + c];
+            sr += b.x * x.x - b.y * x.y;
+            si += b.x * x.y + b.y * x.x;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, 64);
+            si += __shfl_xor(si, off, 64);
+        }
+        if (lane == 0) O[static_cast<size_t>(rep[q] + r) * nrhs + c] = make_double2(sr, si);
+    }
+}
+
 struct ExtractArgs {
     const double2* X;
     size_t sx;
@@ -1500,6 +1714,7 @@ struct refeff_fms_plan {
     int n = 0, npad = 0, nrhs = 0, ch = 0, npot = 0, pstart = 0, pend = 0, solver = 1, batch = 0, max_batch = 0;
     int outer_block = 128;  // LU outer block width = K of the trailing MFMA update
     float rdirec = 0.f;
+    float toler1 = 0.f, toler2 = 0.f;  // one-shot iterative branch (minv = 1)
     std::vector<int> rep, col0, ipart;
     // device constants
     float* d_pos = nullptr;
@@ -1507,6 +1722,7 @@ struct refeff_fms_plan {
     int *d_atom_first = nullptr, *d_atom_lmax = nullptr, *d_atom_rhs = nullptr;
     int *d_sl = nullptr, *d_sm = nullptr, *d_ss = nullptr, *d_spot = nullptr;
     int *d_rep = nullptr, *d_col0 = nullptr, *d_ipart = nullptr;
+    int *d_rhs_src = nullptr, *d_rhs_pot = nullptr;  // state index / potential of each RHS column
     double *d_xnlm = nullptr, *d_so_plus = nullptr, *d_so_minus = nullptr;
     // batch workspace
     int ws_batch = 0;
@@ -1800,12 +2016,50 @@ void run_solve(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* L
     trsm_launch<T, false>(p, g, LU, dinv, X);
 }
 
+// minv = 1 for one sub-batch: A64 holds the toler2-cut G0, B64 the rows of the
+// representative blocks.  sv per column, one batched FP64 MFMA product
+// R = -G0c S, the second half of the one-shot update, then gg into R64's
+// representative rows (read by k_extract).  Work buffers: omega in d_w,
+// the tolerance flags in d_perm (both [e][nrhs]).
+void run_ggbi(refeff_fms_plan* p, const Group& g) {
+    const int n = p->npad, nr = p->nrhs;
+    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
+    GgbiArgs ga{p->d_a64 + sa * g.e0, p->d_tmat + 2 * static_cast<size_t>(p->n) * g.e0, p->d_sl, p->d_sm, p->d_ss,
+                p->d_rhs_src, p->d_x64 + sx * g.e0, p->d_r64 + sx * g.e0,
+                static_cast<double2*>(p->d_w) + static_cast<size_t>(nr) * g.e0, p->d_perm + static_cast<size_t>(nr) * g.e0,
+                p->d_info + g.e0, sa, sx, p->n, n, n, nr, p->nsp, static_cast<double>(p->toler1)};
+    const double nn = static_cast<double>(p->n) * p->n;
+    launch(p, g.s, REFEFF_FMS_K_SOLVE, 0.0, 32.0 * p->n * nr * g.nb, [&] {
+        hipLaunchKernelGGL(k_ggbi_first, dim3(nr, g.nb), dim3(256), 0, g.s, ga);
+    });
+    hipLaunchKernelGGL(k_fill_double, dim3(std::max<size_t>(1, std::min<size_t>(65535, (2 * sx * g.nb + 255) / 256))),
+                       dim3(256), 0, g.s, reinterpret_cast<double*>(p->d_r64 + sx * g.e0), 0.0, 2 * sx * g.nb);
+    const int ntile = nr <= 16 ? 1 : 2;
+    const int chunks = (nr + 16 * ntile - 1) / (16 * ntile);
+    double* gn = p->d_norms + 4 * static_cast<size_t>(g.e0);
+    launch(p, g.s, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * nr * g.nb, 16.0 * nn * chunks * g.nb, [&] {
+        const dim3 grid(n / kPanel, g.nb, chunks), block(512);
+        if (ntile == 1)
+            hipLaunchKernelGGL(k_residual_wg<1>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n, p->d_x64 + sx * g.e0,
+                               p->d_r64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, nr, n, gn);
+        else
+            hipLaunchKernelGGL(k_residual_wg<2>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n, p->d_x64 + sx * g.e0,
+                               p->d_r64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, nr, n, gn);
+    });
+    launch(p, g.s, REFEFF_FMS_K_SOLVE, 0.0, 48.0 * p->n * nr * g.nb, [&] {
+        hipLaunchKernelGGL(k_ggbi_second, dim3(nr, g.nb), dim3(256), 0, g.s, ga);
+        hipLaunchKernelGGL(k_ggbi_gg, dim3(nr, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, p->d_x64 + sx * g.e0,
+                           p->d_r64 + sx * g.e0, sx, p->n, nr, p->d_rhs_pot, p->d_rep, p->d_col0, p->d_ipart);
+    });
+}
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 int map_info(int info) {
     if (info & (1 << 30)) return REFEFF_FMS_NON_FINITE;
+    if (info & (1 << 29)) return REFEFF_FMS_BREAKDOWN;
     if (info) return REFEFF_FMS_SINGULAR;
     return REFEFF_FMS_OK;
 }
@@ -1825,6 +2079,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
     const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
     const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
+    const bool ggbi = p->solver == REFEFF_FMS_SOLVER_GGBI;
     const double t0 = now_s();
     std::vector<Group> gs;
     {
@@ -1849,10 +2104,13 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         BuildArgs ba{p->d_pos, p->d_atom_first, p->d_atom_lmax, p->d_atom_rhs, p->d_sig, p->d_xnlm,
                      p->d_tmat + 2 * static_cast<size_t>(p->n) * g.e0, p->d_k + kst * g.e0,
                      mixed ? p->d_a32 + sa * g.e0 : nullptr, p->d_a64 + sa * g.e0, p->d_b64 + sx * g.e0, sa, sx,
-                     p->nat, p->n, n, p->nsp, p->L, p->nrhs, g.nb, p->rdirec * p->rdirec};
+                     p->nat, p->n, n, p->nsp, p->L, p->nrhs, g.nb, p->rdirec * p->rdirec,
+                     static_cast<double>(p->toler2)};
         launch(p, g.s, REFEFF_FMS_K_BUILD, 0.0, n2 * (mixed ? 24.0 : 16.0) * g.nb + 16.0 * p->n * p->nrhs * g.nb, [&] {
             if (mixed)
                 hipLaunchKernelGGL((k_build<true, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
+            else if (ggbi)
+                hipLaunchKernelGGL((k_build<false, true, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
             else
                 hipLaunchKernelGGL((k_build<false, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
         });
@@ -1862,6 +2120,10 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
             if (mixed)
                 hipLaunchKernelGGL(k_pad<float>, dim3(n, g.nb), dim3(256), 0, g.s, p->d_a32 + sa * g.e0, sa, n, p->n,
                                    static_cast<double2*>(nullptr), sx, p->nrhs);
+        }
+        if (ggbi) {
+            run_ggbi(p, g);
+            continue;
         }
         int* ipiv = p->d_ipiv + static_cast<size_t>(n) * g.e0;
         int* perm = p->d_perm + static_cast<size_t>(n) * g.e0;
@@ -1943,6 +2205,8 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
             if (p->trace) std::fprintf(stderr, "refeff_fms: sweep %d estimated max relative error %.3e\n", sweep, worst);
             if (done) break;
         }
+    } else if (ggbi) {
+        t_first = now_s();
     } else {
         for (const auto& g : gs) {
             hipLaunchKernelGGL(k_gather<double>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, sx,
@@ -1956,7 +2220,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     std::vector<int> info(nb);
     const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
     for (const auto& g : gs) {
-        ExtractArgs ea{p->d_x64 + sx * g.e0, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0,
+        ExtractArgs ea{(ggbi ? p->d_r64 : p->d_x64) + sx * g.e0, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0,
                        p->d_ipart, out + sst * g.e0};
         hipLaunchKernelGGL(k_extract, dim3(std::max<size_t>(1, std::min<size_t>(64, (per + 255) / 256)), g.nb),
                            dim3(256), 0, g.s, ea);
@@ -1999,8 +2263,15 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (!std::isfinite(d->direct_cutoff) || d->direct_cutoff < 0.f) return REFEFF_FMS_INVALID_ARGUMENT;
     if (d->full_scattering_requested) return REFEFF_FMS_UNSUPPORTED;
     int solver = d->solver;
-    if (solver == REFEFF_FMS_SOLVER_AUTO) solver = (d->minv == 0) ? REFEFF_FMS_SOLVER_MIXED : -1;
-    if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64) return REFEFF_FMS_UNSUPPORTED;
+    // refeff fms_scattering_method_selection (fms/driver.rs:72-95): minv 0 -> LU,
+    // 1 -> ggbi; the other FEFF iterative branches are not provided
+    if (solver == REFEFF_FMS_SOLVER_AUTO)
+        solver = (d->minv == 0) ? REFEFF_FMS_SOLVER_MIXED : (d->minv == 1 ? REFEFF_FMS_SOLVER_GGBI : -1);
+    if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64 && solver != REFEFF_FMS_SOLVER_GGBI)
+        return REFEFF_FMS_UNSUPPORTED;
+    if (solver == REFEFF_FMS_SOLVER_GGBI &&
+        (!std::isfinite(d->zero_tolerance) || d->zero_tolerance < 0.f || !std::isfinite(d->convergence_tolerance)))
+        return REFEFF_FMS_INVALID_ARGUMENT;
     for (int i = 0; i < 3 * d->atom_count; ++i)
         if (!std::isfinite(d->positions[i])) return REFEFF_FMS_NON_FINITE;
     for (int a = 0; a < d->atom_count; ++a)
@@ -2031,6 +2302,8 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->npot = p->max_pot + 1;
     p->solver = solver;
     p->rdirec = d->direct_cutoff;
+    p->toler1 = d->convergence_tolerance;
+    p->toler2 = d->zero_tolerance;
     p->max_batch = d->max_batch;
     if (const char* ob = std::getenv("REFEFF_FMS_LU_BLOCK")) {
         const int v = std::atoi(ob);
@@ -2066,6 +2339,12 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         nrhs += p->ipart[ip];
     }
     p->nrhs = nrhs;
+    std::vector<int> rhs_src(std::max(1, nrhs)), rhs_pot(std::max(1, nrhs));
+    for (int ip = p->pstart; ip <= p->pend; ++ip)
+        for (int r = 0; r < p->ipart[ip]; ++r) {
+            rhs_src[p->col0[ip] + r] = p->rep[ip] + r;
+            rhs_pot[p->col0[ip] + r] = ip;
+        }
     std::vector<int> spot(p->n);
     for (int i = 0; i < p->n; ++i) spot[i] = d->potentials[stt.atom[i]];
     std::vector<double> xn((p->L + 1) * (p->L + 1)), sp, sm;
@@ -2089,6 +2368,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
               dupload(&p->d_sl, stt.l.data(), stt.l.size()) && dupload(&p->d_sm, stt.m.data(), stt.m.size()) &&
               dupload(&p->d_ss, stt.spin.data(), stt.spin.size()) && dupload(&p->d_spot, spot.data(), spot.size()) &&
               dupload(&p->d_rep, p->rep.data(), p->rep.size()) && dupload(&p->d_col0, p->col0.data(), p->col0.size()) &&
+              dupload(&p->d_rhs_src, rhs_src.data(), rhs_src.size()) && dupload(&p->d_rhs_pot, rhs_pot.data(), rhs_pot.size()) &&
               dupload(&p->d_ipart, p->ipart.data(), p->ipart.size()) && dupload(&p->d_xnlm, xn.data(), xn.size()) &&
               dupload(&p->d_so_plus, sp.data(), sp.size()) && dupload(&p->d_so_minus, sm.data(), sm.size()) &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
@@ -2118,7 +2398,7 @@ void refeff_fms_plan_destroy(refeff_fms_plan* p) {
     free_workspace(p);
     dfree(p->d_pos); dfree(p->d_sig); dfree(p->d_atom_first); dfree(p->d_atom_lmax); dfree(p->d_atom_rhs);
     dfree(p->d_sl); dfree(p->d_sm); dfree(p->d_ss); dfree(p->d_spot); dfree(p->d_rep); dfree(p->d_col0);
-    dfree(p->d_ipart); dfree(p->d_xnlm); dfree(p->d_so_plus); dfree(p->d_so_minus);
+    dfree(p->d_ipart); dfree(p->d_rhs_src); dfree(p->d_rhs_pot); dfree(p->d_xnlm); dfree(p->d_so_plus); dfree(p->d_so_minus);
     for (hipEvent_t ev : p->pool) (void)hipEventDestroy(ev);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     for (hipStream_t s : p->streams)

@@ -205,3 +205,55 @@ def test_cu_fcc_r80_mixed_agrees_with_fp64():
         plan.close()
     for e in range(len(sel)):
         assert rel_err(res[fms.SOLVER_MIXED][e], res[fms.SOLVER_FP64][e]) <= GG_TOL
+
+
+def ggbi_compare(positions, potentials, k, ph, *, nsp=1, ispin=0, lfms=0, L=3, raw=None, max_pot=None,
+                 toler1=1e-3, toler2=1e-3, sig=None, rdirec=1e9):
+    oplan = orc.Plan(positions, potentials, nsp=nsp, ispin=ispin, lfms=lfms, minv=1, max_pot=max_pot,
+                     global_lmax=L, raw_lmax=raw, rdirec=rdirec, sigsqr=sig, toler1=toler1, toler2=toler2)
+    ref, st = oplan.spectrum(k, ph, threads=8)
+    assert not st.any(), st
+    plan = fms.FmsRealSpacePlan(positions, potentials, lfms=lfms, minv=1, spin_channels=nsp, spin_selector=ispin,
+                                max_potential=max_pot, global_lmax=L, raw_potential_lmax=raw, direct_cutoff=rdirec,
+                                mean_square_displacements=sig, convergence_tolerance=toler1, zero_tolerance=toler2,
+                                solver=fms.SOLVER_AUTO)
+    assert plan.info.solver == fms.SOLVER_GGBI
+    got, status = plan.solve(k, ph)
+    plan.close()
+    assert not status.any(), status
+    errs = [rel_err(got[e], ref[e]) for e in range(len(k))]
+    assert max(errs) <= GG_TOL, errs
+    return max(errs)
+
+
+@pytest.mark.parametrize("seed,nat,lfms", [(4, 8, 0), (5, 13, 1)])
+def test_ggbi_random_clusters_match_oracle(seed, nat, lfms):
+    """minv = 1 one-shot BiCGStab (refeff iterative.rs:181-254) on the GPU vs the oracle."""
+    banner(f"ggbi random cluster seed={seed} nat={nat} lfms={lfms}")
+    pos, pot = random_cluster(seed, nat)
+    k = np.array([[1.3 + 0.05j], [2.1 + 0.1j], [3.4 + 0.02j]])
+    raw = np.array([3, 3, 2], np.int32)
+    ph = phases_for(k[:, 0], 3, raw)
+    worst = ggbi_compare(pos, pot, k, ph, lfms=lfms, raw=raw, max_pot=2)
+    banner(f"worst rel err {worst:.2e}")
+
+
+def test_ggbi_spin_orbit_matches_oracle():
+    banner("ggbi nsp=2")
+    pos, pot = random_cluster(6, 9)
+    k1 = np.array([1.7 + 0.05j, 2.6 + 0.08j])
+    k = np.stack([k1, k1 * 1.01], axis=1)
+    raw = np.array([2, 2, 2], np.int32)
+    ph = phases_for(k1, 2, raw, nsp=2, seed=3)
+    worst = ggbi_compare(pos, pot, k, ph, nsp=2, L=2, raw=raw, max_pot=2)
+    banner(f"worst rel err {worst:.2e}")
+
+
+def test_ggbi_cu_fcc_r52_with_cutoff_matches_oracle():
+    """Cu fcc R=5.2 (N=880), toler2 = 0.02 so that far G0 entries are dropped."""
+    banner("ggbi Cu fcc R=5.2")
+    wl = W.fcc_cu(5.2, 60)
+    sel = [0, 31, 59]
+    worst = ggbi_compare(wl.positions, wl.potentials, wl.wave_numbers()[sel], wl.phase_shifts()[sel],
+                         raw=wl.raw_lmax, max_pot=1, toler2=0.02)
+    banner(f"worst rel err {worst:.2e}")
