@@ -190,7 +190,7 @@ def main():
                    "solver": args.solver, "batch": main_res["batch"],
                    "refinement_sweeps": int(main_res["stats"].refinement_sweeps)},
         "roofline": {"bound": "mfma",
-                     "kernel": f"{'k_gemm_v2' if prec == 'fp32' else 'k_gemm_sub<double>'} (LU trailing update, K=128)",
+                     "kernel": f"{'k_gemm_v2<16>' if prec == 'fp32' else 'k_gemm_sub<double>'} (LU trailing update, K=128)",
                      "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_TFLOPS[prec], 4),
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((args.config, args.solver)),

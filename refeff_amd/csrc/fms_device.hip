@@ -1034,44 +1034,73 @@ __global__ __launch_bounds__(256) void k_gemm_big(float2* Cb, int ldc, size_t sC
             }
 }
 
+// Workgroup -> (energy, tile) with the 8 XCDs in mind: the dispatcher deals
+// linear workgroup ids round-robin to the XCDs, so id L runs on XCD L % 8.
+// Remapping gives each XCD a contiguous run of the energy-major (energy, tile)
+// order (the first 8 floor(total / 8) ids; the rest keep their place), so one
+// XCD sweeps the tiles of an energy together and its L2 keeps that energy's
+// A row panels and B column panels instead of every XCD fetching all of them.
+__device__ __forceinline__ void xcd_tile(int tiles, int& tile, int& e) {
+    const unsigned total = static_cast<unsigned>(tiles) * gridDim.y;
+    const unsigned L = blockIdx.y * static_cast<unsigned>(tiles) + blockIdx.x;
+    const unsigned q = total / 8;
+    const unsigned m = (L < 8 * q) ? (L % 8) * q + L / 8 : L;
+    e = static_cast<int>(m / tiles);
+    tile = static_cast<int>(m % tiles);
+}
+
 // C -= A * B, FP32 complex, 64x64 tile per 256-thread workgroup (4 waves of
-// 32x32), K in steps of 32 with register prefetch of the next step and two
+// 32x32), K in steps of BK with register prefetch of the next step and two
 // LDS stages; 16-byte global loads (two complex values) where the operand
-// offsets are even.
-__global__ __launch_bounds__(256, 2) void k_gemm_v2(float2* Cb, int ldc, size_t sC, const float2* Ab_, int lda,
-                                                     size_t sA, const float2* Bb_, int ldb, size_t sB, int M, int N,
-                                                     int K) {
-    constexpr int BM = 64, BN = 64, BK = 32, LD = 64 + 16;
+// offsets are even.  BK = 16 halves the LDS footprint (40 KiB) so four
+// workgroups share a CU.
+template <int BK, bool XCD>
+__global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb, int ldc, size_t sC, const float2* Ab_,
+                                                                   int lda, size_t sA, const float2* Bb_, int ldb,
+                                                                   size_t sB, int M, int N, int K) {
+    constexpr int BM = 64, BN = 64, LD = 64 + 16;
+    constexpr int AV = BK / 4;        // complex values of A per thread per step (4 threads per row)
+    constexpr int TPB = 256 / BK;     // threads per B row
+    constexpr int BV = 64 / TPB;      // complex values of B per thread per step
     __shared__ float as_re[2][BK][LD], as_im[2][BK][LD];
     __shared__ float bs_re[2][BK][LD], bs_im[2][BK][LD];
-    const int e = blockIdx.y;
     const int tilesN = (N + BN - 1) / BN;
-    const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
+    int tile, e;
+    if (XCD) {
+        xcd_tile(gridDim.x, tile, e);
+    } else {
+        tile = blockIdx.x;
+        e = blockIdx.y;
+    }
+    const int m0 = (tile / tilesN) * BM, n0 = (tile % tilesN) * BN;
     const float2* A = Ab_ + e * sA;
     const float2* B = Bb_ + e * sB;
     float2* C = Cb + e * sC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const int ar = tid >> 2, ak = (tid & 3) * 8;   // A: row, 8 consecutive k
-    const int bk = tid >> 3, bc = (tid & 7) * 8;   // B: k row, 8 consecutive columns
-    float4 ra[4], rb[4];
+    const int ar = tid >> 2, ak = (tid & 3) * AV;   // A: row, AV consecutive k
+    const int bk = tid / TPB, bc = (tid % TPB) * BV; // B: k row, BV consecutive columns
+    float4 ra[AV / 2], rb[BV / 2];
     auto gload = [&](int k0) {
         const int gr = m0 + ar;
         const float4* pa = reinterpret_cast<const float4*>(A + static_cast<size_t>(gr) * lda + k0 + ak);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ra[i] = (gr < M) ? pa[i] : float4{0, 0, 0, 0};
+        for (int i = 0; i < AV / 2; ++i) ra[i] = (gr < M) ? pa[i] : float4{0, 0, 0, 0};
         const int gc = n0 + bc;
         const float4* pb = reinterpret_cast<const float4*>(B + static_cast<size_t>(k0 + bk) * ldb + gc);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) rb[i] = (gc + 2 * i < N) ? pb[i] : float4{0, 0, 0, 0};
+        for (int i = 0; i < BV / 2; ++i) rb[i] = (gc + 2 * i < N) ? pb[i] : float4{0, 0, 0, 0};
     };
     auto sstore = [&](int st) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < AV / 2; ++i) {
             as_re[st][ak + 2 * i][ar] = ra[i].x;
             as_im[st][ak + 2 * i][ar] = ra[i].y;
             as_re[st][ak + 2 * i + 1][ar] = ra[i].z;
             as_im[st][ak + 2 * i + 1][ar] = ra[i].w;
+        }
+#pragma unroll
+        for (int i = 0; i < BV / 2; ++i) {
             bs_re[st][bk][bc + 2 * i] = rb[i].x;
             bs_im[st][bk][bc + 2 * i] = rb[i].y;
             bs_re[st][bk][bc + 2 * i + 1] = rb[i].z;
@@ -1897,15 +1926,16 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
     if (M <= 0 || N <= 0 || K <= 0) return;
     const int nb = g.nb;
     const double cb = sizeof(typename Cx<T>::type);
-    // the double-buffered FP32 kernel reads two complex values per 16-byte load:
-    // every operand row must start 16-byte aligned and N must be even
-    const bool v2 = sizeof(T) == 4 && (N % 2 == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
+    // the double-buffered FP32 kernel (BK = 16: four workgroups per CU) reads two
+    // complex values per 16-byte load: every operand row must start 16-byte
+    // aligned, N must be even and K a multiple of 16
+    const bool v2 = sizeof(T) == 4 && (N % 2 == 0) && (K % 16 == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
                     ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % 16 == 0);
     launch(p, g.s, K > kPanel ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL, 8.0 * M * N * K * nb,
            (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
                const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
                if (v2)
-                   hipLaunchKernelGGL(k_gemm_v2, dim3(tiles, nb), dim3(256), 0, g.s, reinterpret_cast<float2*>(C), ldc,
+                   hipLaunchKernelGGL((k_gemm_v2<16, false>), dim3(tiles, nb), dim3(256), 0, g.s, reinterpret_cast<float2*>(C), ldc,
                                       sC, reinterpret_cast<const float2*>(A), lda, sA,
                                       reinterpret_cast<const float2*>(B), ldb, sB, M, N, K);
                else
@@ -2506,7 +2536,9 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
 int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, const double* A, const double* B,
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
-    if (M <= 0 || N <= 0 || K <= 0 || K % ((use_double == 0 || use_double == 3) ? 32 : 16) || !A || !B || !C) return REFEFF_FMS_INVALID_ARGUMENT;
+    const int kmul = (use_double == 0 || use_double == 3 || use_double == 5) ? 32 : 16;
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 6)
+        return REFEFF_FMS_INVALID_ARGUMENT;
     const int ld = std::max(std::max(M, N), K);
     const size_t sz = static_cast<size_t>(ld) * ld;
     int rc = REFEFF_FMS_OK;
@@ -2524,11 +2556,19 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
         if (!dupload(&da, ha.data(), sz) || !dupload(&db, hb.data(), sz) || !dupload(&dc, hc.data(), sz)) {
             rc = REFEFF_FMS_DEVICE_ERROR;
         } else {
-            if (use_double == 3 && sizeof(T) == 4) {
+            if (use_double >= 3 && sizeof(T) == 4) {
                 const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-                hipLaunchKernelGGL(k_gemm_v2, dim3(tiles, 1), dim3(256), 0, nullptr, reinterpret_cast<float2*>(dc), ld,
-                                   sz, reinterpret_cast<const float2*>(da), ld, sz, reinterpret_cast<const float2*>(db),
-                                   ld, sz, M, N, K);
+                auto* c2 = reinterpret_cast<float2*>(dc);
+                auto* a2 = reinterpret_cast<const float2*>(da);
+                auto* b2 = reinterpret_cast<const float2*>(db);
+                if (use_double == 3)
+                    hipLaunchKernelGGL((k_gemm_v2<32, false>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else if (use_double == 4)
+                    hipLaunchKernelGGL((k_gemm_v2<16, false>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else if (use_double == 5)
+                    hipLaunchKernelGGL((k_gemm_v2<32, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else
+                    hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
             } else if (use_double == 2 && sizeof(T) == 4) {
                 const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
                 hipLaunchKernelGGL(k_gemm_big, dim3(tiles, 1), dim3(256), 0, nullptr, reinterpret_cast<float2*>(dc), ld,
@@ -2578,10 +2618,19 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     auto one = [&]() {
-        if (kernel == 3) {
+        if (kernel >= 3 && kernel <= 6) {
             const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-            hipLaunchKernelGGL(k_gemm_v2, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc), ld, per,
-                               static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K);
+            auto* c2 = static_cast<float2*>(dc);
+            auto* a2 = static_cast<const float2*>(da);
+            auto* b2 = static_cast<const float2*>(db);
+            if (kernel == 3)
+                hipLaunchKernelGGL((k_gemm_v2<32, false>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else if (kernel == 4)
+                hipLaunchKernelGGL((k_gemm_v2<16, false>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else if (kernel == 5)
+                hipLaunchKernelGGL((k_gemm_v2<32, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else
+                hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
         } else if (kernel == 2) {
             const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
             hipLaunchKernelGGL(k_gemm_big, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc), ld, per,

@@ -40,10 +40,12 @@ def test_gemm_big_tile(m, n, k):
     _gemm_case(2, m, n, k)
 
 
+@pytest.mark.parametrize("kind", [3, 4, 5, 6])
 @pytest.mark.parametrize("m,n,k", [(256, 256, 128), (320, 448, 64), (130, 260, 32), (96, 16, 32), (2880, 16, 32)])
-def test_gemm_v2(m, n, k):
-    """64x64-tile double-buffered FP32 kernel used for the LU and solve updates."""
-    _gemm_case(3, m, n, k)
+def test_gemm_v2(kind, m, n, k):
+    """64x64-tile double-buffered FP32 kernel of the LU updates: BK 32 / 16, with and
+    without the XCD-aware tile order (kinds 3, 4, 5, 6)."""
+    _gemm_case(kind, m, n, k)
 
 
 def _gemm_case(use_double, m, n, k):
@@ -56,6 +58,7 @@ def _gemm_case(use_double, m, n, k):
                                       _p(out))
     assert rc == 0
     tol = 1e-12 if use_double == 1 else 2e-5
+    assert out.shape == ref.shape
     assert np.abs(out - ref).max() <= tol * np.abs(ref).max()
 
 
