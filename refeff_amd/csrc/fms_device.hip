@@ -35,8 +35,6 @@ namespace {
 
 constexpr int kPanel = 32;                 // LU block width
 constexpr int kRhsChunk = 64;              // right-hand sides per solve/residual workgroup
-constexpr int kRot = 455;                  // sum_{l<=kMaxL} (2l+1)^2
-constexpr int kGl = (kMaxL + 1) * (kMaxL + 1) * (kMaxL + 1);
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -193,8 +191,8 @@ struct BuildArgs {
 
 // FEFF rotxan small-d recursion into d0[l][n][m] (1-based n, m), restated from
 // refeff fms/internals/rotation.rs:3-71 in double precision.
+template <int S>
 __device__ void small_d(int lmax, double beta, double* d0) {
-    constexpr int S = 2 * kMaxL + 2;
     auto D = [&](int l, int n, int m) -> double& { return d0[(l * S + n) * S + m]; };
     const int ndm = 2 * lmax + 1;
     const double xc = cos(0.5 * beta), xs = sin(0.5 * beta), s = sin(beta);
@@ -237,15 +235,20 @@ __device__ void small_d(int lmax, double beta, double* d0) {
 // GG = false: A = 1 - G0 t (FP32 and/or FP64) and the RHS columns G0[:, rep].
 // GG = true (minv = 1): A64 holds G0 with entries |G0| <= toler2 dropped, and
 // B64 the rows of the representative blocks, B[j][rhs + r] = G0[rep + r, j].
-template <bool W32, bool W64, bool GG = false>
+// LM: largest l this instance serves (LDS tables are sized by it); E: energies
+// whose Rehr-Albers polynomials, z-axis propagators and G0 cells are formed
+// together in one pass (one thread per energy and spin for the recursion).
+template <bool W32, bool W64, bool GG, int LM, int E>
 __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
-    constexpr int S = 2 * kMaxL + 2;
-    __shared__ double d0[2][(kMaxL + 2) * S * S];
-    __shared__ double2 rf[kRot], rb[kRot];
-    __shared__ double2 clm[2][(kMaxL + 2) * (kMaxL + 1)];
-    __shared__ double2 gl[2][kGl];
-    __shared__ double2 pref[2];
-    __shared__ double xn[(kMaxL + 1) * (kMaxL + 1)];
+    constexpr int S = 2 * LM + 2;
+    constexpr int LP = LM + 1;
+    constexpr int NROT = LP * (4 * LP * LP - 1) / 3;  // rot_base(LM + 1)
+    __shared__ double d0[2][(LM + 2) * S * S];
+    __shared__ double2 rf[NROT], rb[NROT];
+    __shared__ double2 clm[E][2][(LM + 2) * LP];
+    __shared__ double2 gl[E][2][LP * LP * LP];
+    __shared__ double2 pref[E][2];
+    __shared__ double xn[LP * LP];
 
     const int b = blockIdx.x, a = blockIdx.y, tid = threadIdx.x;
     const int L = g.L, nsp = g.nsp;
@@ -296,14 +299,14 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
     else if (ez < dist) beta = acos(ez / dist);
     else beta = 0.0;
 
-    for (int i = tid; i < 2 * (kMaxL + 2) * S * S; i += blockDim.x) (&d0[0][0])[i] = 0.0;
+    for (int i = tid; i < 2 * (LM + 2) * S * S; i += blockDim.x) (&d0[0][0])[i] = 0.0;
     for (int i = tid; i < (L + 1) * (L + 1); i += blockDim.x) {
         const int mu = i / (L + 1), l = i - mu * (L + 1);
         xn[i] = g.xnlm[mu * (L + 1) + l];
     }
     __syncthreads();
-    if (tid == 0) small_d(L, beta, d0[0]);          // forward table, +beta
-    if (tid == 64) small_d(L, -beta, d0[1]);        // backward table, -beta (rotation.rs:60-77)
+    if (tid == 0) small_d<S>(L, beta, d0[0]);          // forward table, +beta
+    if (tid == 64) small_d<S>(L, -beta, d0[1]);        // backward table, -beta (rotation.rs:60-77)
     __syncthreads();
     const int nrot = rot_base(L + 1);
     for (int i = tid; i < nrot; i += blockDim.x) {
@@ -324,14 +327,15 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
     __syncthreads();
 
     const int L1 = L + 1;
-    for (int e = 0; e < g.nbatch; ++e) {
-        if (tid < nsp) {
+    for (int e0 = 0; e0 < g.nbatch; e0 += E) {
+        const int ne = min(E, g.nbatch - e0);
+        if (tid < nsp * ne) {
             // Rehr-Albers polynomials c_l^m(rho), rho = k |R_a - R_b| (xclmz)
-            const int s = tid;
-            const double2 k = g.kv[static_cast<size_t>(e) * nsp + s];
+            const int s = tid % nsp, ec = tid / nsp;
+            const double2 k = g.kv[static_cast<size_t>(e0 + ec) * nsp + s];
             const double2 rho = make_double2(k.x * dist, k.y * dist);
             const double2 z = cdiv(make_double2(0.0, -1.0), rho);  // -i / rho
-            double2* C = clm[s];
+            double2* C = clm[ec][s];
             auto at = [&](int l, int m) -> double2& { return C[l * L1 + m]; };
             for (int i = 0; i < (L + 2) * L1; ++i) C[i] = make_double2(0.0, 0.0);
             at(0, 0) = make_double2(1.0, 0.0);
@@ -361,33 +365,37 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
             const double2 eir = cexp_d(make_double2(-rho.y, rho.x));
             const double2 k2 = cmul(k, k);
             const double2 damp = cexp_d(make_double2(-sigma * k2.x, -sigma * k2.y));
-            pref[s] = cmul(cdiv(eir, rho), damp);
+            pref[ec][s] = cmul(cdiv(eir, rho), damp);
         }
         __syncthreads();
         // z-axis propagator g_mu(l1, l2) with the prefactor folded in (xgllm)
-        for (int i = tid; i < nsp * L1 * L1 * L1; i += blockDim.x) {
-            const int s = i / (L1 * L1 * L1);
-            const int r = i - s * L1 * L1 * L1;
+        const int per = nsp * L1 * L1 * L1;
+        for (int i = tid; i < ne * per; i += blockDim.x) {
+            const int ec = i / per, q = i - ec * per;
+            const int s = q / (L1 * L1 * L1);
+            const int r = q - s * L1 * L1 * L1;
             const int mu = r / (L1 * L1), l1 = (r / L1) % L1, l2 = r % L1;
             double2 acc = make_double2(0.0, 0.0);
             if (mu <= l1 && mu <= l2) {
                 const int numax = min(l1, l2 - mu);
                 const double sg = (mu & 1) ? -1.0 : 1.0;
                 for (int nu = 0; nu <= numax; ++nu) {
-                    const double2 c1 = clm[s][l1 * L1 + nu];
-                    const double2 c2 = clm[s][l2 * L1 + mu + nu];
+                    const double2 c1 = clm[ec][s][l1 * L1 + nu];
+                    const double2 c2 = clm[ec][s][l2 * L1 + mu + nu];
                     const double w = (2.0 * l1 + 1.0) / xn[mu * L1 + l1] * sg * xn[mu * L1 + l2];
                     acc = cadd(acc, make_double2(w * (c1.x * c2.x - c1.y * c2.y), w * (c1.x * c2.y + c1.y * c2.x)));
                 }
-                acc = cmul(acc, pref[s]);
+                acc = cmul(acc, pref[ec][s]);
             }
-            gl[s][r] = acc;
+            gl[ec][s][r] = acc;
         }
         __syncthreads();
-        const double2* tm0 = g.tmat + static_cast<size_t>(e) * 2 * g.n;
-        const double2* tm1 = tm0 + g.n;
-        for (int idx = tid; idx < cells; idx += blockDim.x) {
-            const int r = idx / nbk, c = idx - r * nbk;
+        for (int idx = tid; idx < ne * cells; idx += blockDim.x) {
+            const int ec = idx / cells, cell = idx - ec * cells;
+            const int e = e0 + ec;
+            const double2* tm0 = g.tmat + static_cast<size_t>(e) * 2 * g.n;
+            const double2* tm1 = tm0 + g.n;
+            const int r = cell / nbk, c = cell - r * nbk;
             const int sr = r % nsp, lr = r / nsp, l1 = isqrt_small(lr), m1 = lr - l1 * l1 - l1;
             auto g0 = [&](int cc) -> double2 {
                 const int sc = cc % nsp, lc = cc / nsp, l2 = isqrt_small(lc), m2 = lc - l2 * l2 - l2;
@@ -398,7 +406,7 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
                     const int amu = mu < 0 ? -mu : mu;
                     const double2 db = rb[rot_base(l1) + (mu + l1) * (2 * l1 + 1) + (m1 + l1)];
                     const double2 df = rf[rot_base(l2) + (m2 + l2) * (2 * l2 + 1) + (mu + l2)];
-                    acc = cadd(acc, cmul(cmul(db, gl[sr][(amu * L1 + l1) * L1 + l2]), df));
+                    acc = cadd(acc, cmul(cmul(db, gl[ec][sr][(amu * L1 + l1) * L1 + l2]), df));
                 }
                 return acc;
             };
@@ -2001,6 +2009,18 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
     });
 }
 
+// G0 build launch: instance by the largest l (LDS tables) and energies per pass.
+template <bool W32, bool W64, bool GG>
+void launch_build(hipStream_t s, const BuildArgs& ba, int nat, int L) {
+    const dim3 grid(nat, nat), block(256);
+    if (L <= 3)
+        hipLaunchKernelGGL((k_build<W32, W64, GG, 3, 8>), grid, block, 0, s, ba);
+    else if (L <= 4)
+        hipLaunchKernelGGL((k_build<W32, W64, GG, 4, 4>), grid, block, 0, s, ba);
+    else
+        hipLaunchKernelGGL((k_build<W32, W64, GG, kMaxL, 2>), grid, block, 0, s, ba);
+}
+
 // Inverse diagonal blocks of a fresh LU factorisation (used by every solve).
 template <typename T>
 void trtri_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, typename Cx<T>::type* dinv) {
@@ -2138,11 +2158,11 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                      static_cast<double>(p->toler2)};
         launch(p, g.s, REFEFF_FMS_K_BUILD, 0.0, n2 * (mixed ? 24.0 : 16.0) * g.nb + 16.0 * p->n * p->nrhs * g.nb, [&] {
             if (mixed)
-                hipLaunchKernelGGL((k_build<true, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
+                launch_build<true, true, false>(g.s, ba, p->nat, p->L);
             else if (ggbi)
-                hipLaunchKernelGGL((k_build<false, true, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
+                launch_build<false, true, true>(g.s, ba, p->nat, p->L);
             else
-                hipLaunchKernelGGL((k_build<false, true>), dim3(p->nat, p->nat), dim3(256), 0, g.s, ba);
+                launch_build<false, true, false>(g.s, ba, p->nat, p->L);
         });
         if (n > p->n) {
             hipLaunchKernelGGL(k_pad<double>, dim3(n, g.nb), dim3(256), 0, g.s, p->d_a64 + sa * g.e0, sa, n, p->n,
