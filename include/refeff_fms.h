@@ -203,7 +203,9 @@ typedef struct refeff_fms_profile {
     int64_t launches[REFEFF_FMS_K_COUNT];
 } refeff_fms_profile;
 
-/* enable != 0 starts (and clears) the per-launch event record. */
+/* enable != 0 starts (and clears) the per-launch event record: 1 brackets
+ * every launch, any other value is a bitmask (1 << class) of the kernel
+ * classes to bracket (the others run without events); 0 stops recording. */
 int refeff_fms_plan_profile(refeff_fms_plan* plan, int32_t enable);
 int refeff_fms_plan_profile_read(refeff_fms_plan* plan, refeff_fms_profile* out);
 

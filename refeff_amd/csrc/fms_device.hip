@@ -1259,14 +1259,13 @@ __global__ __launch_bounds__(64) void k_trtri_diag(const typename Cx<T>::type* L
 // block, the eight waves of the workgroup taking 16-wide K chunks in turn:
 // complex MFMA straight from global memory, the next chunk prefetched while
 // the current one multiplies.  Each wave leaves its partial sum in cr/ci.
-template <typename T, int NT>
+template <typename T, int NT, int NW = 8>
 __device__ __forceinline__ void row_block_product(const typename Cx<T>::type* A, int lda, int R0, int kb, int nch,
                                                   const typename Cx<T>::type* X, int ldx, int c0, int ncol,
                                                   typename Mfma<T>::acc (&cr)[2][NT],
                                                   typename Mfma<T>::acc (&ci)[2][NT]) {
     using T2 = typename Cx<T>::type;
     using acc_t = typename Mfma<T>::acc;
-    constexpr int NW = 8;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -1322,12 +1321,14 @@ __device__ __forceinline__ void row_block_product(const typename Cx<T>::type* A,
     }
 }
 
-// Sums the eight waves' partial products in a fixed order ((w, w+4), then
-// (w, w+2), then (0, 1)) through LDS; the 32 x 16*NT total lands in red[1].
-template <typename T, int NT>
-__device__ __forceinline__ void reduce_waves(typename Cx<T>::type (&red)[4][kPanel][NT * 16],
+// Sums the NW waves' partial products in a fixed order (pairs (w, w + NW/2),
+// then (w, w + NW/4), ... down to (0, 1)) through LDS; the 32 x 16*NT total
+// lands in red[1].
+template <typename T, int NT, int NW = 8>
+__device__ __forceinline__ void reduce_waves(typename Cx<T>::type (&red)[NW / 2][kPanel][NT * 16],
                                              typename Mfma<T>::acc (&cr)[2][NT], typename Mfma<T>::acc (&ci)[2][NT]) {
     using T2 = typename Cx<T>::type;
+    static_assert(NW >= 8 && (NW & (NW - 1)) == 0, "power-of-two wave count, at least 8");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
     auto row_of = [&](int a, int r) { return 16 * a + (sizeof(T) == 4 ? 4 * q + r : q + 4 * r); };
     auto put = [&](int slot) {
@@ -1350,20 +1351,13 @@ __device__ __forceinline__ void reduce_waves(typename Cx<T>::type (&red)[4][kPan
                     ci[a][b][r] += v.y;
                 }
     };
-    if (wave >= 4) put(wave - 4);
-    __syncthreads();
-    if (wave < 4) add(wave);
-    __syncthreads();
-    if (wave == 2 || wave == 3) put(wave - 2);
-    __syncthreads();
-    if (wave < 2) add(wave);
-    __syncthreads();
-    if (wave == 1) put(0);
-    __syncthreads();
-    if (wave == 0) {
-        add(0);
-        put(1);
+    for (int half = NW / 2; half >= 1; half /= 2) {
+        if (wave >= half && wave < 2 * half) put(wave - half);
+        __syncthreads();
+        if (wave < half) add(wave);
+        __syncthreads();
     }
+    if (wave == 0) put(1);
     __syncthreads();
 }
 
@@ -1373,14 +1367,14 @@ __device__ __forceinline__ void reduce_waves(typename Cx<T>::type (&red)[4][kPan
 // row times the already solved rows is formed with row_block_product, and the
 // block is finished as X_b = inv(D_b) (X_b - S) with the precomputed inverse
 // diagonal block.  L/U is read once per solve.
-template <typename T, bool LOWER, int NT>
-__global__ __launch_bounds__(512) void k_trsm_wg(const typename Cx<T>::type* LUb, size_t sa, int lda,
-                                                 const typename Cx<T>::type* dinvb, typename Cx<T>::type* Xb,
-                                                 size_t sx, int ldx, int n) {
+template <typename T, bool LOWER, int NT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_trsm_wg(const typename Cx<T>::type* LUb, size_t sa, int lda,
+                                                     const typename Cx<T>::type* dinvb, typename Cx<T>::type* Xb,
+                                                     size_t sx, int ldx, int n) {
     using T2 = typename Cx<T>::type;
     using acc_t = typename Mfma<T>::acc;
-    constexpr int BS = kPanel, NC = NT * 16;
-    __shared__ T2 red[4][BS][NC];
+    constexpr int BS = kPanel, NC = NT * 16, NTH = NW * 64;
+    __shared__ T2 red[NW / 2][BS][NC];
     __shared__ T2 dia[BS][BS + 1];
     const int e = blockIdx.x, tid = threadIdx.x;
     const int c0 = blockIdx.y * NC;
@@ -1393,19 +1387,19 @@ __global__ __launch_bounds__(512) void k_trsm_wg(const typename Cx<T>::type* LUb
     for (int step = 0; step < nblk; ++step) {
         const int blk = LOWER ? step : nblk - 1 - step;
         const int R0 = blk * BS;
-        for (int i = tid; i < BS * BS; i += 512) dia[i / BS][i % BS] = dinv[blk * blk_sz + i];
+        for (int i = tid; i < BS * BS; i += NTH) dia[i / BS][i % BS] = dinv[blk * blk_sz + i];
         const int kb = LOWER ? 0 : R0 + BS;
         const int nch = ((LOWER ? R0 : n) - kb) / 16;
         acc_t cr[2][NT], ci[2][NT];
-        row_block_product<T, NT>(LU, lda, R0, kb, nch, X, ldx, c0, ncol, cr, ci);
-        reduce_waves<T, NT>(red, cr, ci);
+        row_block_product<T, NT, NW>(LU, lda, R0, kb, nch, X, ldx, c0, ncol, cr, ci);
+        reduce_waves<T, NT, NW>(red, cr, ci);
         // v = X_b - S into red[2]
-        for (int i = tid; i < BS * NC; i += 512) {
+        for (int i = tid; i < BS * NC; i += NTH) {
             const int r = i / NC, c = i % NC;
             red[2][r][c] = c < ncol ? csub(X[static_cast<size_t>(R0 + r) * ldx + c0 + c], red[1][r][c]) : T2{0, 0};
         }
         __syncthreads();
-        for (int i = tid; i < BS * NC; i += 512) {
+        for (int i = tid; i < BS * NC; i += NTH) {
             const int r = i / NC, c = i % NC;
             if (c < ncol) {
                 T2 acc{0, 0};
@@ -1783,6 +1777,7 @@ struct refeff_fms_plan {
         double flops, bytes;
     };
     bool profile = false;
+    unsigned profile_mask = ~0u;  // kernel classes bracketed by events
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -1839,7 +1834,7 @@ struct Group {
 
 template <typename F>
 void launch(refeff_fms_plan* p, hipStream_t s, int cls, double flops, double bytes, F&& f) {
-    if (!p->profile) {
+    if (!p->profile || !(p->profile_mask & (1u << cls))) {
         f();
         return;
     }
@@ -2049,13 +2044,22 @@ void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type*
     const double cb = sizeof(typename Cx<T>::type);
     const double nn = static_cast<double>(n) * n;
     launch(p, g.s, REFEFF_FMS_K_SOLVE, 4.0 * nn * nr * nb, (nn / 2 * chunks + 2.0 * n * nr) * cb * nb, [&] {
-        const dim3 grid(nb, chunks), block(512);
-        if (ntile == 1)
-            hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1>), grid, block, 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
-        else if (ntile == 2)
-            hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2>), grid, block, 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
-        else if constexpr (sizeof(T) == 4)
-            hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4>), grid, block, 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+        const dim3 grid(nb, chunks);
+        // one workgroup walks a whole triangle: 16 waves for the common <= 16
+        // right-hand sides in FP32 (more loads in flight), 8 otherwise
+        if constexpr (sizeof(T) == 4) {
+            if (ntile == 1)
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 16>), grid, dim3(1024), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+            else if (ntile == 2)
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+            else
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+        } else {
+            if (ntile == 1)
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+            else
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+        }
     });
 }
 
@@ -2691,6 +2695,7 @@ int refeff_fms_plan_profile(refeff_fms_plan* p, int32_t enable) {
     p->pool_used = 0;
     p->acc = refeff_fms_profile{};
     p->profile = enable != 0;
+    p->profile_mask = (enable == 1) ? ~0u : static_cast<unsigned>(enable);
     return REFEFF_FMS_OK;
 }
 
