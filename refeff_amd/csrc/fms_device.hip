@@ -1062,7 +1062,13 @@ __device__ __forceinline__ void xcd_tile(int tiles, int& tile, int& e) {
 // LDS stages; 16-byte global loads (two complex values) where the operand
 // offsets are even.  BK = 16 halves the LDS footprint (40 KiB) so four
 // workgroups share a CU.
-template <int BK, bool XCD>
+//
+// M3 (the 3M method, Higham, Accuracy and Stability of Numerical Algorithms,
+// sec. 23.2.4): per 16x16 tile three real products P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi) instead of four; Re = P1 - P2, Im = P3 - P1 - P2.
+// Its rounding error is of the same order as the four-product form's in
+// norm; in the LU of the mixed solver the FP64 residual refinement absorbs it.
+template <int BK, bool XCD, bool M3 = false>
 __global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb, int ldc, size_t sC, const float2* Ab_,
                                                                    int lda, size_t sA, const float2* Bb_, int ldb,
                                                                    size_t sB, int M, int N, int K) {
@@ -1115,13 +1121,14 @@ __global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb,
             bs_im[st][bk][bc + 2 * i + 1] = rb[i].w;
         }
     };
-    v4f cr[2][2], ci[2][2];
+    v4f cr[2][2], ci[2][2], cs[M3 ? 2 : 1][M3 ? 2 : 1];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             cr[i][j] = v4f{0, 0, 0, 0};
             ci[i][j] = v4f{0, 0, 0, 0};
+            if (M3) cs[M3 ? i : 0][M3 ? j : 0] = v4f{0, 0, 0, 0};
         }
     gload(0);
     sstore(0);
@@ -1142,24 +1149,47 @@ __global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb,
                 yr[i] = bs_re[st][kr][wc * 32 + i * 16 + ix];
                 yi[i] = bs_im[st][kr][wc * 32 + i * 16 + ix];
             }
+            if constexpr (M3) {
+                // cr: Ar Br, ci: Ai Bi, cs: (Ar + Ai)(Br + Bi)
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yr[j], cr[i][j], 0, 0, 0);
-                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yi[j], ci[i][j], 0, 0, 0);
-                }
+                    for (int j = 0; j < 2; ++j) {
+                        cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yr[j], cr[i][j], 0, 0, 0);
+                        ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[i], yi[j], ci[i][j], 0, 0, 0);
+                        cs[M3 ? i : 0][M3 ? j : 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                            xr[i] + xi[i], yr[j] + yi[j], cs[M3 ? i : 0][M3 ? j : 0], 0, 0, 0);
+                    }
+            } else {
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nxi[i], yi[j], cr[i][j], 0, 0, 0);
-                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[i], yr[j], ci[i][j], 0, 0, 0);
-                }
+                    for (int j = 0; j < 2; ++j) {
+                        cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yr[j], cr[i][j], 0, 0, 0);
+                        ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yi[j], ci[i][j], 0, 0, 0);
+                    }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nxi[i], yi[j], cr[i][j], 0, 0, 0);
+                        ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[i], yr[j], ci[i][j], 0, 0, 0);
+                    }
+            }
         }
         if (more) sstore(st ^ 1);
         __syncthreads();
         st ^= 1;
+    }
+    if constexpr (M3) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const v4f p1 = cr[i][j], p2 = ci[i][j];
+                cr[i][j] = p1 - p2;
+                ci[i][j] = cs[M3 ? i : 0][M3 ? j : 0] - p1 - p2;
+            }
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -1204,8 +1234,8 @@ __global__ __launch_bounds__(256) void k_perm(const int* ipivb, int* permb, int 
 // X[e][i][c] = (T) B[e][perm[i]][c]
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, const int* permb, int n, int nrhs,
-                                                typename Cx<T>::type* X) {
-    const int e = blockIdx.y;
+                                                typename Cx<T>::type* X, const int* emap) {
+    const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y);
     const size_t total = static_cast<size_t>(n) * nrhs;
     for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
         const int i = static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
@@ -1370,13 +1400,13 @@ __device__ __forceinline__ void reduce_waves(typename Cx<T>::type (&red)[NW / 2]
 template <typename T, bool LOWER, int NT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_trsm_wg(const typename Cx<T>::type* LUb, size_t sa, int lda,
                                                      const typename Cx<T>::type* dinvb, typename Cx<T>::type* Xb,
-                                                     size_t sx, int ldx, int n) {
+                                                     size_t sx, int ldx, int n, const int* emap) {
     using T2 = typename Cx<T>::type;
     using acc_t = typename Mfma<T>::acc;
     constexpr int BS = kPanel, NC = NT * 16, NTH = NW * 64;
     __shared__ T2 red[NW / 2][BS][NC];
     __shared__ T2 dia[BS][BS + 1];
-    const int e = blockIdx.x, tid = threadIdx.x;
+    const int e = emap ? emap[blockIdx.x] : static_cast<int>(blockIdx.x), tid = threadIdx.x;
     const int c0 = blockIdx.y * NC;
     const int ncol = min(NC, ldx - c0);
     const T2* LU = LUb + e * sa;
@@ -1421,10 +1451,10 @@ __global__ __launch_bounds__(NW * 64) void k_trsm_wg(const typename Cx<T>::type*
 template <int NT>
 __global__ __launch_bounds__(512) void k_residual_wg(const double2* Ab, size_t sa, int lda, const double2* Xb,
                                                      const double2* Bb, double2* Rb, size_t sx, int ldx, int n,
-                                                     double* norms) {
+                                                     double* norms, const int* emap) {
     constexpr int BS = kPanel, NC = NT * 16;
     __shared__ double2 red[4][BS][NC];
-    const int R0 = blockIdx.x * BS, e = blockIdx.y, tid = threadIdx.x;
+    const int R0 = blockIdx.x * BS, e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y), tid = threadIdx.x;
     const int c0 = blockIdx.z * NC;
     const int ncol = min(NC, ldx - c0);
     typename Mfma<double>::acc cr[2][NT], ci[2][NT];
@@ -1447,8 +1477,8 @@ __global__ __launch_bounds__(512) void k_residual_wg(const double2* Ab, size_t s
 // X += D; tracks max |D| and max |X| per energy
 template <typename T>
 __global__ __launch_bounds__(256) void k_update(double2* X, const typename Cx<T>::type* D, size_t sx, int n, int nrhs,
-                                                double* norms) {
-    const int e = blockIdx.y;
+                                                double* norms, const int* emap) {
+    const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y);
     const size_t total = static_cast<size_t>(n) * nrhs;
     double md = 0.0, mxv = 0.0;
     for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
@@ -1763,11 +1793,13 @@ struct refeff_fms_plan {
     void* d_w = nullptr;   // solve work block (float2 or double2)
     void* d_dinv = nullptr;  // inverse 32x32 diagonal blocks of L and U (float2 or double2)
     int *d_ipiv = nullptr, *d_perm = nullptr, *d_info = nullptr;
+    int* d_emap = nullptr;              // energies still refining, per sub-batch
     double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_norms = nullptr;
     hipStream_t stream = nullptr;       // main stream (copies, single-group work)
     int nstreams = 1;                   // energy sub-batches issued on separate streams
     bool trace = false;                 // REFEFF_FMS_TRACE: print refinement progress to stderr
+    bool gemm_3m = false;               // REFEFF_FMS_GEMM_3M=1: FP32 trailing updates with three real products
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex mu;
     // kernel-level instrumentation (refeff_fms_plan_profile)
@@ -1797,7 +1829,7 @@ void free_workspace(refeff_fms_plan* p) {
     p->d_w = nullptr;
     if (p->d_dinv) (void)hipFree(p->d_dinv);
     p->d_dinv = nullptr;
-    dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
+    dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
     dfree(p->d_out); dfree(p->d_norms);
     p->ws_batch = 0;
 }
@@ -1881,7 +1913,8 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
     ok = ok && hipMalloc(&p->d_w, nr * nb * cs) == hipSuccess;
     ok = ok && hipMalloc(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs) == hipSuccess;
-    ok = ok && dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
+    ok = ok && dalloc(&p->d_emap, static_cast<size_t>(nb)) &&
+         dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
          dalloc(&p->d_info, nb) && dalloc(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb) &&
          dalloc(&p->d_k, static_cast<size_t>(p->nsp) * nb) &&
          dalloc(&p->d_ph, static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot * nb) &&
@@ -1910,6 +1943,10 @@ void launch_panel(hipStream_t s, int nb, typename Cx<T>::type* A, size_t sa, int
             hipLaunchKernelGGL((k_panel_rb<T, 4, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 8192)
             hipLaunchKernelGGL((k_panel_rb<T, 2, 8>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else if (m <= 10240)
+            hipLaunchKernelGGL((k_panel_rb<T, 2, 10>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else if (m <= 12288)
+            hipLaunchKernelGGL((k_panel_rb<T, 2, 12>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else
             hipLaunchKernelGGL(k_panel<T>, grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
     } else {
@@ -1917,6 +1954,10 @@ void launch_panel(hipStream_t s, int nb, typename Cx<T>::type* A, size_t sa, int
             hipLaunchKernelGGL((k_panel_rb<T, 4, 2>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 4096)
             hipLaunchKernelGGL((k_panel_rb<T, 2, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else if (m <= 8192)
+            hipLaunchKernelGGL((k_panel_rb<T, 1, 8>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+        else if (m <= 10240)
+            hipLaunchKernelGGL((k_panel_rb<T, 1, 10>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else
             hipLaunchKernelGGL(k_panel<T>, grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
     }
@@ -1937,7 +1978,11 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
     launch(p, g.s, K > kPanel ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL, 8.0 * M * N * K * nb,
            (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
                const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-               if (v2)
+               if (v2 && p->gemm_3m)
+                   hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, nb), dim3(256), 0, g.s,
+                                      reinterpret_cast<float2*>(C), ldc, sC, reinterpret_cast<const float2*>(A), lda,
+                                      sA, reinterpret_cast<const float2*>(B), ldb, sB, M, N, K);
+               else if (v2)
                    hipLaunchKernelGGL((k_gemm_v2<16, false>), dim3(tiles, nb), dim3(256), 0, g.s, reinterpret_cast<float2*>(C), ldc,
                                       sC, reinterpret_cast<const float2*>(A), lda, sA,
                                       reinterpret_cast<const float2*>(B), ldb, sB, M, N, K);
@@ -2034,7 +2079,7 @@ template <typename T> constexpr int solve_chunk() { return sizeof(T) == 4 ? 64 :
 // right-hand-side chunk walks each triangle once.
 template <typename T, bool LOWER>
 void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
-                 typename Cx<T>::type* X) {
+                 typename Cx<T>::type* X, const int* emap) {
     const int n = p->npad, nr = p->nrhs, nb = g.nb;
     const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
     constexpr int CW = solve_chunk<T>();
@@ -2049,25 +2094,27 @@ void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type*
         // right-hand sides in FP32 (more loads in flight), 8 otherwise
         if constexpr (sizeof(T) == 4) {
             if (ntile == 1)
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 16>), grid, dim3(1024), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 16>), grid, dim3(1024), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
             else if (ntile == 2)
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
             else
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
         } else {
             if (ntile == 1)
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
             else
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n);
+                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
         }
     });
 }
 
 template <typename T>
+// g.nb energies of the group: all of them when emap is null, else the
+// group-relative energies emap[0 .. g.nb)
 void run_solve(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
-               typename Cx<T>::type* X) {
-    trsm_launch<T, true>(p, g, LU, dinv, X);
-    trsm_launch<T, false>(p, g, LU, dinv, X);
+               typename Cx<T>::type* X, const int* emap = nullptr) {
+    trsm_launch<T, true>(p, g, LU, dinv, X, emap);
+    trsm_launch<T, false>(p, g, LU, dinv, X, emap);
 }
 
 // minv = 1 for one sub-batch: A64 holds the toler2-cut G0, B64 the rows of the
@@ -2095,10 +2142,10 @@ void run_ggbi(refeff_fms_plan* p, const Group& g) {
         const dim3 grid(n / kPanel, g.nb, chunks), block(512);
         if (ntile == 1)
             hipLaunchKernelGGL(k_residual_wg<1>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n, p->d_x64 + sx * g.e0,
-                               p->d_r64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, nr, n, gn);
+                               p->d_r64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, nr, n, gn, nullptr);
         else
             hipLaunchKernelGGL(k_residual_wg<2>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n, p->d_x64 + sx * g.e0,
-                               p->d_r64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, nr, n, gn);
+                               p->d_r64 + sx * g.e0, p->d_r64 + sx * g.e0, sx, nr, n, gn, nullptr);
     });
     launch(p, g.s, REFEFF_FMS_K_SOLVE, 0.0, 48.0 * p->n * nr * g.nb, [&] {
         hipLaunchKernelGGL(k_ggbi_second, dim3(nr, g.nb), dim3(256), 0, g.s, ga);
@@ -2203,11 +2250,34 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                                reinterpret_cast<double*>(p->d_x64 + sx * g.e0), 0.0, 2 * sx * g.nb);
         const int max_sweeps = 8;
         std::vector<double> prev_rel(nb, 1.0e300);
+        // Sweeps after the first run over the energies still refining only:
+        // emap (device, [e0 + i] = group-relative energy) lists them per group.
+        std::vector<char> live(nb, 1);
+        std::vector<int> emap_h(nb);
+        std::vector<int> nlive(gs.size());
         for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-            for (const auto& g : gs) {
+            for (size_t gi = 0; gi < gs.size(); ++gi) {
+                const Group& gall = gs[gi];
+                const int* emap = nullptr;
+                Group g = gall;
+                if (sweep > 0) {
+                    int m = 0;
+                    for (int i = 0; i < gall.nb; ++i)
+                        if (live[gall.e0 + i]) emap_h[gall.e0 + m++] = i;
+                    nlive[gi] = m;
+                    if (m == 0) continue;
+                    if (m < gall.nb) {
+                        (void)hipMemcpyAsync(p->d_emap + gall.e0, emap_h.data() + gall.e0, m * sizeof(int),
+                                             hipMemcpyHostToDevice, g.s);
+                        emap = p->d_emap + gall.e0;
+                    }
+                    g.nb = m;
+                } else {
+                    nlive[gi] = gall.nb;
+                }
                 double* gn = p->d_norms + 4 * static_cast<size_t>(g.e0);
-                hipLaunchKernelGGL(k_fill_double, dim3((4 * g.nb + 255) / 256), dim3(256), 0, g.s, gn, 0.0,
-                                   static_cast<size_t>(4 * g.nb));
+                hipLaunchKernelGGL(k_fill_double, dim3((4 * gall.nb + 255) / 256), dim3(256), 0, g.s, gn, 0.0,
+                                   static_cast<size_t>(4 * gall.nb));
                 const double2* rhs = p->d_b64 + sx * g.e0;
                 if (sweep > 0) {
                     const double nn = static_cast<double>(n) * n;
@@ -2218,22 +2288,22 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                         if (ntile == 1)
                             hipLaunchKernelGGL(k_residual_wg<1>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
                                                p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
-                                               p->nrhs, n, gn);
+                                               p->nrhs, n, gn, emap);
                         else
                             hipLaunchKernelGGL(k_residual_wg<2>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
                                                p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
-                                               p->nrhs, n, gn);
+                                               p->nrhs, n, gn, emap);
                     });
                     rhs = p->d_r64 + sx * g.e0;
                 }
                 hipLaunchKernelGGL(k_gather<float>, dim3(gx, g.nb), dim3(256), 0, g.s, rhs, sx,
-                                   p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0);
+                                   p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0, emap);
                 run_solve<float>(p, g, p->d_a32 + sa * g.e0,
                                  static_cast<const float2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
-                                 w + sx * g.e0);
+                                 w + sx * g.e0, emap);
                 hipLaunchKernelGGL(k_update<float>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_x64 + sx * g.e0,
-                                   w + sx * g.e0, sx, n, p->nrhs, gn);
-                (void)hipMemcpyAsync(norms.data() + 4 * static_cast<size_t>(g.e0), gn, 4 * g.nb * sizeof(double),
+                                   w + sx * g.e0, sx, n, p->nrhs, gn, emap);
+                (void)hipMemcpyAsync(norms.data() + 4 * static_cast<size_t>(g.e0), gn, 4 * gall.nb * sizeof(double),
                                      hipMemcpyDeviceToHost, g.s);
             }
             if (!sync_groups(gs)) return REFEFF_FMS_DEVICE_ERROR;
@@ -2241,22 +2311,31 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
             sweeps = sweep + 1;
             // Refinement converges linearly, so after a correction of relative
             // size d_s (previous d_{s-1}) the error left in x is about
-            // d_s * (d_s / d_{s-1}).  Stop when that estimate is below 1e-10
-            // for every energy, or when a correction below 1e-9 has stopped
-            // shrinking (the FP64 residual noise floor).
-            bool done = sweep > 0;
+            // d_s * (d_s / d_{s-1}).  An energy stops refining when that
+            // estimate is below 1e-10, or when a correction below 1e-9 has
+            // stopped shrinking (the FP64 residual noise floor).
+            bool done = true;
             worst = 0.0;
             for (int e = 0; e < nb; ++e) {
+                if (!live[e]) continue;
                 const double rel = norms[4 * e + 2] > 0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
                 const double est = sweep == 0 ? rel : rel * std::min(1.0, rel / prev_rel[e]);
                 worst = std::max(worst, est);
                 const bool small = est <= 1.0e-10;
                 const bool floor = rel <= 1.0e-9 && rel >= 0.5 * prev_rel[e];
-                if (!(small || floor)) done = false;
                 est_err[e] = floor ? rel : est;
                 prev_rel[e] = rel;
+                if (sweep > 0 && (small || floor))
+                    live[e] = 0;
+                else
+                    done = false;
             }
-            if (p->trace) std::fprintf(stderr, "refeff_fms: sweep %d estimated max relative error %.3e\n", sweep, worst);
+            if (p->trace) {
+                int m = 0;
+                for (size_t gi = 0; gi < gs.size(); ++gi) m += nlive[gi];
+                std::fprintf(stderr, "refeff_fms: sweep %d over %d energies, estimated max relative error %.3e\n",
+                             sweep, m, worst);
+            }
             if (done) break;
         }
     } else if (ggbi) {
@@ -2264,7 +2343,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     } else {
         for (const auto& g : gs) {
             hipLaunchKernelGGL(k_gather<double>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, sx,
-                               p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, p->d_x64 + sx * g.e0);
+                               p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, p->d_x64 + sx * g.e0, nullptr);
             run_solve<double>(p, g, p->d_a64 + sa * g.e0,
                               static_cast<const double2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
                               p->d_x64 + sx * g.e0);
@@ -2427,6 +2506,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
               dupload(&p->d_so_plus, sp.data(), sp.size()) && dupload(&p->d_so_minus, sm.data(), sm.size()) &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
     p->trace = std::getenv("REFEFF_FMS_TRACE") != nullptr;
+    if (const char* e = std::getenv("REFEFF_FMS_GEMM_3M")) p->gemm_3m = std::atoi(e) != 0;
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
     for (int i = 0; ok && i < p->nstreams; ++i)
         ok = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking) == hipSuccess;
@@ -2561,7 +2641,7 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
     const int kmul = (use_double == 0 || use_double == 3 || use_double == 5) ? 32 : 16;
-    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 6)
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 7)
         return REFEFF_FMS_INVALID_ARGUMENT;
     const int ld = std::max(std::max(M, N), K);
     const size_t sz = static_cast<size_t>(ld) * ld;
@@ -2591,8 +2671,10 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                     hipLaunchKernelGGL((k_gemm_v2<16, false>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
                 else if (use_double == 5)
                     hipLaunchKernelGGL((k_gemm_v2<32, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
-                else
+                else if (use_double == 6)
                     hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else
+                    hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
             } else if (use_double == 2 && sizeof(T) == 4) {
                 const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
                 hipLaunchKernelGGL(k_gemm_big, dim3(tiles, 1), dim3(256), 0, nullptr, reinterpret_cast<float2*>(dc), ld,
@@ -2642,7 +2724,7 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     auto one = [&]() {
-        if (kernel >= 3 && kernel <= 6) {
+        if (kernel >= 3 && kernel <= 7) {
             const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
             auto* c2 = static_cast<float2*>(dc);
             auto* a2 = static_cast<const float2*>(da);
@@ -2653,8 +2735,10 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
                 hipLaunchKernelGGL((k_gemm_v2<16, false>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
             else if (kernel == 5)
                 hipLaunchKernelGGL((k_gemm_v2<32, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
-            else
+            else if (kernel == 6)
                 hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else
+                hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
         } else if (kernel == 2) {
             const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
             hipLaunchKernelGGL(k_gemm_big, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc), ld, per,

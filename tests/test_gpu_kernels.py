@@ -40,11 +40,11 @@ def test_gemm_big_tile(m, n, k):
     _gemm_case(2, m, n, k)
 
 
-@pytest.mark.parametrize("kind", [3, 4, 5, 6])
+@pytest.mark.parametrize("kind", [3, 4, 5, 6, 7])
 @pytest.mark.parametrize("m,n,k", [(256, 256, 128), (320, 448, 64), (130, 260, 32), (96, 16, 32), (2880, 16, 32)])
 def test_gemm_v2(kind, m, n, k):
     """64x64-tile double-buffered FP32 kernel of the LU updates: BK 32 / 16, with and
-    without the XCD-aware tile order (kinds 3, 4, 5, 6)."""
+    without the XCD-aware tile order (kinds 3, 4, 5, 6), and the 3M form (kind 7)."""
     _gemm_case(kind, m, n, k)
 
 
@@ -111,6 +111,27 @@ def test_lu_float_reconstructs_matrix(n):
     for i, p in enumerate(piv):
         pa[[i, p]] = pa[[p, i]]
     assert np.abs(lo @ up - pa).max() <= 1e-4 * np.sqrt(n / 64) * np.abs(a).max()
+
+
+@pytest.mark.parametrize("n,use_double", [(9792, 0), (8256, 1)])
+def test_lu_tall_panels_solve_with_small_backward_error(n, use_double):
+    """Panels of 8192-10240 rows (register tiles of 2 columns in FP32, 1 in FP64):
+    the factors solve A x = b with a backward error at the working precision."""
+    import scipy.linalg as sla
+    print(f"[gpu-kernels] lu tall panels n={n} double={use_double}", flush=True)
+    rng = np.random.default_rng(n)
+    a = crand(rng, n, n) / np.sqrt(n) + 2.0 * np.eye(n)
+    work = np.ascontiguousarray(a.copy())
+    piv = np.zeros(n, np.int32)
+    info = C.c_int32(-1)
+    assert _lib().refeff_fms_debug_lu(n, use_double, _p(work), piv.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      C.byref(info)) == 0
+    assert info.value == 0
+    assert np.all((piv >= np.arange(n)) & (piv < n))
+    b = crand(rng, n, 2)
+    x = sla.lu_solve((work, piv), b, check_finite=False)
+    berr = np.linalg.norm(a @ x - b) / (np.linalg.norm(a, 2 if n < 0 else "fro") * np.linalg.norm(x) + np.linalg.norm(b))
+    assert berr <= (1e-13 if use_double else 5e-6), berr
 
 
 def test_lu_reports_exact_singularity():
