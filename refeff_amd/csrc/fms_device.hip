@@ -770,6 +770,263 @@ __global__ __launch_bounds__(1024) void k_panel_rb(typename Cx<T>::type* Ab, siz
     }
 }
 
+// ---- Tournament-pivoting panel (CALU), FP32 LU of the mixed solver ----
+//
+// Panel j0 (rows [j0, n) x kPanel columns) in three steps, spread over many
+// workgroups per energy instead of one:
+//   1. k_calu_select: every 512-row chunk runs partial pivoting on its own
+//      rows and proposes the kPanel rows it would pivot on; further levels
+//      reduce up to 16 proposals at a time the same way;
+//   2. k_calu_final: partial pivoting over the last <= 512 candidates fixes
+//      the kPanel pivot rows, their L11\U11 block and LAPACK-style
+//      interchanges (ipiv);
+//   3. k_calu_apply: L21 = A21 U11^-1 for every row below.
+// Communication-avoiding LU (Grigori, Demmel, Xiang, SIAM J. Matrix Anal.
+// Appl. 32 (2011) 1317): its pivots differ from row-by-row partial pivoting
+// but it is stable in practice; the FP64 refinement around this
+// factorisation is what sets the final accuracy.
+constexpr int kSelRows = 512;  // rows per selection workgroup, one per thread
+
+// Partial pivoting over the rows held one per thread (v, valid), without
+// moving them: each column's pivot is the first largest |re|+|im| among the
+// rows not yet chosen.  win[c] = thread chosen for column c (-1 when no row is
+// left); fact[c] = that row's factored values (multipliers left of c, U right
+// of it) when fact is given; zero_col = first column whose pivot is zero.
+template <typename T>
+__device__ __forceinline__ void calu_gepp(typename Cx<T>::type (&v)[kPanel], bool valid, int* win,
+                                          typename Cx<T>::type (*fact)[kPanel + 1], typename Cx<T>::type* prow,
+                                          T* wbest, int* widx, int* zero_col) {
+    using T2 = typename Cx<T>::type;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = kSelRows / 64;
+    bool used = false;
+    if (tid == 0) *zero_col = -1;
+#pragma unroll
+    for (int c = 0; c < kPanel; ++c) {
+        // first largest |re|+|im| (lowest thread on ties): per wave by
+        // shuffles, then every thread reduces the per-wave results itself
+        T best = (valid && !used) ? fabs(v[c].x) + fabs(v[c].y) : T(-1);
+        int bi = tid;
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ob = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            wbest[wave] = best;
+            widx[wave] = bi;
+        }
+        __syncthreads();
+        T pb = wbest[0];
+        int pt = widx[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w)
+            if (wbest[w] > pb) {
+                pb = wbest[w];
+                pt = widx[w];
+            }
+        if (!(pb >= 0)) {  // no row left (or a NaN pivot: treated as no pivot)
+            if (tid == 0) {
+                win[c] = -1;
+                if (*zero_col < 0) *zero_col = c;
+            }
+            __syncthreads();  // every thread has read wbest before it is rewritten
+            continue;
+        }
+        if (tid == pt) {
+            used = true;
+#pragma unroll
+            for (int k = 0; k < kPanel; ++k) prow[k] = v[k];
+            if (fact)
+#pragma unroll
+                for (int k = 0; k < kPanel; ++k) fact[c][k] = v[k];
+            win[c] = pt;
+            if (!(pb > 0) && *zero_col < 0) *zero_col = c;
+        }
+        __syncthreads();
+        if (valid && !used) {
+            const T2 d = prow[c];
+            const bool nz = pb > 0;
+            const T2 l = nz ? cdiv(v[c], d) : v[c];
+            v[c] = l;
+#pragma unroll
+            for (int k = c + 1; k < kPanel; ++k) v[k] = cfms(v[k], l, prow[k]);
+        }
+    }
+}
+
+// Row source of a selection workgroup: with rows == nullptr the contiguous
+// rows j0 + g*512 + t (< n), else rows[g*512 + t] (< count, -1 = none).
+template <typename T>
+__device__ __forceinline__ int calu_load(const typename Cx<T>::type* A, int lda, int j0, int n, const int* rows,
+                                         int count, int g, typename Cx<T>::type (&v)[kPanel]) {
+    const int t = g * kSelRows + static_cast<int>(threadIdx.x);
+    int r = -1;
+    if (!rows) {
+        if (j0 + t < n) r = j0 + t;
+    } else if (t < count) {
+        r = rows[t];
+    }
+    if (r >= 0) {
+        const typename Cx<T>::type* src = A + static_cast<size_t>(r) * lda + j0;
+#pragma unroll
+        for (int k = 0; k < kPanel; ++k) v[k] = src[k];
+    }
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSelRows) void k_calu_select(const typename Cx<T>::type* Ab, size_t sa, int lda, int j0,
+                                                          int n, const int* in_rows, int in_stride, int in_count,
+                                                          int* out_rows, int out_stride) {
+    using T2 = typename Cx<T>::type;
+    __shared__ T2 prow[kPanel];
+    __shared__ T wbest[kSelRows / 64];
+    __shared__ int widx[kSelRows / 64];
+    __shared__ int win[kPanel];
+    __shared__ int zero_col;
+    const int g = blockIdx.x, e = blockIdx.y;
+    const T2* A = Ab + e * sa;
+    T2 v[kPanel];
+    const int r = calu_load<T>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
+                               in_count, g, v);
+    calu_gepp<T>(v, r >= 0, win, nullptr, prow, wbest, widx, &zero_col);
+    __syncthreads();
+    // winners by column, as global rows (the thread that won column c holds r)
+    __shared__ int rowof[kSelRows];
+    rowof[threadIdx.x] = r;
+    __syncthreads();
+    if (threadIdx.x < kPanel) {
+        const int w = win[threadIdx.x];
+        out_rows[static_cast<size_t>(e) * out_stride + g * kPanel + threadIdx.x] = w >= 0 ? rowof[w] : -1;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSelRows) void k_calu_final(typename Cx<T>::type* Ab, size_t sa, int lda, int j0, int n,
+                                                         const int* in_rows, int in_stride, int in_count, int* ipivb,
+                                                         int* info) {
+    using T2 = typename Cx<T>::type;
+    __shared__ T2 prow[kPanel];
+    __shared__ T wbest[kSelRows / 64];
+    __shared__ int widx[kSelRows / 64];
+    __shared__ int win[kPanel];
+    __shared__ int zero_col;
+    __shared__ T2 fact[kPanel][kPanel + 1];
+    __shared__ T2 orig[kPanel][kPanel + 1];
+    __shared__ int rowof[kSelRows];
+    __shared__ int mv_pos[kPanel], mv_src[kPanel];
+    __shared__ int nmv;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    T2* A = Ab + e * sa;
+    int* ipiv = ipivb + static_cast<size_t>(e) * n;
+    T2 v[kPanel];
+    const int r = calu_load<T>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
+                               in_count, 0, v);
+    calu_gepp<T>(v, r >= 0, win, fact, prow, wbest, widx, &zero_col);
+    rowof[tid] = r;
+    for (int i = tid; i < kPanel * kPanel; i += kSelRows)
+        orig[i / kPanel][i % kPanel] = A[static_cast<size_t>(j0 + i / kPanel) * lda + j0 + i % kPanel];
+    __syncthreads();
+    if (tid < 64) {
+        // LAPACK-style interchanges bringing the winners to rows j0 .. j0+31 in
+        // pivot order, tracked by wave 0: lane l < 32 is row j0 + l, lane
+        // 32 + c the row winner c came from (when below the top block); dat =
+        // the row whose original data the lane's row holds now.
+        if (tid == 0 && zero_col >= 0 && info[e] == 0) info[e] = j0 + zero_col + 1;
+        const int lane = tid;
+        int pos = -1;
+        if (lane < kPanel) {
+            pos = j0 + lane;
+        } else {
+            const int w = win[lane - kPanel];
+            const int r = w >= 0 ? rowof[w] : -1;
+            if (r >= j0 + kPanel) pos = r;
+        }
+        int dat = pos;
+        for (int c = 0; c < kPanel; ++c) {
+            const int w = win[c] >= 0 ? rowof[win[c]] : -1;  // a missing winner keeps its row
+            if (w < 0) {
+                if (lane == 0) ipiv[j0 + c] = j0 + c;
+                continue;
+            }
+            const unsigned long long hit = __ballot(pos >= 0 && dat == w);
+            if (hit == 0) {  // cannot happen: every winner's data is tracked
+                if (lane == 0) ipiv[j0 + c] = j0 + c;
+                continue;
+            }
+            const int L = static_cast<int>(__ffsll(static_cast<unsigned long long>(hit))) - 1;
+            const int q = __shfl(pos, L, 64);
+            const int dc = __shfl(dat, c, 64);
+            if (lane == 0) ipiv[j0 + c] = q;
+            if (lane == L) dat = dc;
+            if (lane == c) dat = w;
+        }
+        // displaced top rows: their new rows below the top block
+        const bool mv = lane >= kPanel && pos >= 0 && dat != pos && dat >= j0 && dat < j0 + kPanel;
+        const unsigned long long mask = __ballot(mv);
+        if (mv) {
+            const int slot = __popcll(mask & ((1ull << lane) - 1));
+            mv_pos[slot] = pos;
+            mv_src[slot] = dat - j0;
+        }
+        if (lane == 0) nmv = __popcll(mask);
+    }
+    __syncthreads();
+    // rows j0 .. j0+31: L11\U11 (a missing winner keeps its row, unfactored);
+    // displaced top rows move to the winners' old rows
+    for (int i = tid; i < kPanel * kPanel; i += kSelRows) {
+        const int rr = i / kPanel, k = i % kPanel;
+        A[static_cast<size_t>(j0 + rr) * lda + j0 + k] = win[rr] >= 0 ? fact[rr][k] : orig[rr][k];
+    }
+    for (int i = tid; i < nmv * kPanel; i += kSelRows) {
+        const int j = i / kPanel, k = i % kPanel;
+        A[static_cast<size_t>(mv_pos[j]) * lda + j0 + k] = orig[mv_src[j]][k];
+    }
+}
+
+// L21 = A21 U11^-1 for rows [j0 + 32, n) of panel j0 by substitution along
+// each row (x_c = (a_c - sum_{k<c} x_k U_kc) / U_cc, a zero pivot leaving
+// its column unscaled as in partial pivoting): one row per thread, 256 rows
+// per workgroup staged through LDS for coalesced loads and stores.
+template <typename T>
+__global__ __launch_bounds__(256) void k_calu_apply(typename Cx<T>::type* Ab, size_t sa, int lda, int j0, int n) {
+    using T2 = typename Cx<T>::type;
+    constexpr int RW = 256;
+    __shared__ T2 u[kPanel][kPanel + 1];
+    __shared__ T2 rcp[kPanel];
+    __shared__ T2 tile[RW][kPanel + 1];
+    const int e = blockIdx.y, tid = threadIdx.x;
+    T2* A = Ab + e * sa;
+    const int r0 = j0 + kPanel + blockIdx.x * RW;
+    const int nr = min(RW, n - r0);
+    for (int i = tid; i < kPanel * kPanel; i += RW) u[i / kPanel][i % kPanel] = A[static_cast<size_t>(j0 + i / kPanel) * lda + j0 + i % kPanel];
+    for (int i = tid; i < nr * kPanel; i += RW) tile[i / kPanel][i % kPanel] = A[static_cast<size_t>(r0 + i / kPanel) * lda + j0 + i % kPanel];
+    __syncthreads();
+    if (tid < kPanel) {
+        const T2 d = u[tid][tid];
+        rcp[tid] = (d.x != 0 || d.y != 0) ? cdiv(T2{1, 0}, d) : T2{1, 0};
+    }
+    __syncthreads();
+    if (tid < nr) {
+        T2 x[kPanel];
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) {
+            T2 acc = tile[tid][c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) acc = cfms(acc, x[k], u[k][c]);
+            x[c] = cmul(acc, rcp[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) tile[tid][c] = x[c];
+    }
+    __syncthreads();
+    for (int i = tid; i < nr * kPanel; i += RW) A[static_cast<size_t>(r0 + i / kPanel) * lda + j0 + i % kPanel] = tile[i / kPanel][i % kPanel];
+}
+
 // Row interchanges of panel j0 and/or the unit-lower triangular solve with the
 // panel's top 32x32 block, for columns [c_lo, c_hi) outside the panel.
 // do_swap applies the interchanges; columns >= trsm_from also get the solve.
@@ -1794,12 +2051,16 @@ struct refeff_fms_plan {
     void* d_dinv = nullptr;  // inverse 32x32 diagonal blocks of L and U (float2 or double2)
     int *d_ipiv = nullptr, *d_perm = nullptr, *d_info = nullptr;
     int* d_emap = nullptr;              // energies still refining, per sub-batch
+    int* d_cand = nullptr;              // tournament-pivoting candidates: two [e][calu_cap] lists
+    int calu_cap = 0;                   // candidate list length per energy
+    bool panel_calu = true;             // REFEFF_FMS_PANEL=gepp: FP32 panels by partial pivoting
+    int calu_min_rows = 3072;           // tournament pivoting for panels of at least this many rows
     double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_norms = nullptr;
     hipStream_t stream = nullptr;       // main stream (copies, single-group work)
     int nstreams = 1;                   // energy sub-batches issued on separate streams
     bool trace = false;                 // REFEFF_FMS_TRACE: print refinement progress to stderr
-    bool gemm_3m = false;               // REFEFF_FMS_GEMM_3M=1: FP32 trailing updates with three real products
+    bool gemm_3m = true;                // REFEFF_FMS_GEMM_3M=0: FP32 trailing updates with four real products
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex mu;
     // kernel-level instrumentation (refeff_fms_plan_profile)
@@ -1829,7 +2090,7 @@ void free_workspace(refeff_fms_plan* p) {
     p->d_w = nullptr;
     if (p->d_dinv) (void)hipFree(p->d_dinv);
     p->d_dinv = nullptr;
-    dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
+    dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_cand); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
     dfree(p->d_out); dfree(p->d_norms);
     p->ws_batch = 0;
 }
@@ -1882,6 +2143,12 @@ template <typename P> bool dalloc(P** ptr, size_t count) {
     return hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(P)) == hipSuccess;
 }
 
+// Tournament-pivoting buffers of the FP32 LU for nb energies.
+bool alloc_calu(refeff_fms_plan* p, int nb) {
+    p->calu_cap = ((p->npad + kSelRows - 1) / kSelRows) * kPanel;
+    return dalloc(&p->d_cand, 2 * static_cast<size_t>(p->calu_cap) * nb);
+}
+
 template <typename P> bool dupload(P** ptr, const P* src, size_t count) {
     if (!dalloc(ptr, count)) return false;
     return hipMemcpy(*ptr, src, count * sizeof(P), hipMemcpyHostToDevice) == hipSuccess;
@@ -1913,6 +2180,7 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
     ok = ok && hipMalloc(&p->d_w, nr * nb * cs) == hipSuccess;
     ok = ok && hipMalloc(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs) == hipSuccess;
+    if (mixed) ok = ok && alloc_calu(p, nb);
     ok = ok && dalloc(&p->d_emap, static_cast<size_t>(nb)) &&
          dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
          dalloc(&p->d_info, nb) && dalloc(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb) &&
@@ -1932,12 +2200,31 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
 // spilling (sub-panel width shrinking as the row count grows); the streaming
 // kernel otherwise.
 template <typename T>
-void launch_panel(hipStream_t s, int nb, typename Cx<T>::type* A, size_t sa, int lda, int* ipiv, int* info, int j0,
-                  int n) {
+void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, typename Cx<T>::type* A, size_t sa, int lda, int* ipiv,
+                  int* info, int j0, int n) {
     const int m = n - j0;
     const dim3 grid(nb), block(1024);
     if constexpr (sizeof(T) == 4) {
-        if (m <= 3072)
+        if (p->panel_calu && p->d_cand && m >= p->calu_min_rows) {
+            // selection levels until <= 512 candidates remain, then the final pivoting
+            const int cap = p->calu_cap;
+            int* buf[2] = {p->d_cand, p->d_cand + static_cast<size_t>(cap) * nb};
+            const int* in = nullptr;
+            int count = m, cur = 0;
+            while (count > kSelRows) {
+                const int G = (count + kSelRows - 1) / kSelRows;
+                hipLaunchKernelGGL(k_calu_select<float>, dim3(G, nb), dim3(kSelRows), 0, s, A, sa, lda, j0, n, in, cap,
+                                   count, buf[cur], cap);
+                in = buf[cur];
+                cur ^= 1;
+                count = G * kPanel;
+            }
+            hipLaunchKernelGGL(k_calu_final<float>, dim3(nb), dim3(kSelRows), 0, s, A, sa, lda, j0, n, in, cap, count,
+                               ipiv, info);
+            if (m > kPanel)
+                hipLaunchKernelGGL(k_calu_apply<float>, dim3((m - kPanel + 255) / 256, nb), dim3(256), 0, s, A, sa, lda,
+                                   j0, n);
+        } else if (m <= 3072)
             hipLaunchKernelGGL((k_panel_rb<T, 8, 3>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 4096)
             hipLaunchKernelGGL((k_panel_rb<T, 4, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
@@ -1975,7 +2262,9 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
     // aligned, N must be even and K a multiple of 16
     const bool v2 = sizeof(T) == 4 && (N % 2 == 0) && (K % 16 == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
                     ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % 16 == 0);
-    launch(p, g.s, K > kPanel ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL, 8.0 * M * N * K * nb,
+    // flops as executed: 6 M N K real for the 3M form, 8 M N K otherwise
+    launch(p, g.s, K > kPanel ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL,
+           (v2 && p->gemm_3m ? 6.0 : 8.0) * M * N * K * nb,
            (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
                const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
                if (v2 && p->gemm_3m)
@@ -2023,7 +2312,7 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
         for (int j0 = k0; j0 < kend; j0 += kPanel) {
             const double m = n - j0;
             launch(p, g.s, REFEFF_FMS_K_PANEL, 8.0 * m * kPanel * kPanel / 2 * nb, 2.0 * m * kPanel * cb * nb, [&] {
-                launch_panel<T>(g.s, nb, A, sa, n, ipiv, info, j0, n);
+                launch_panel<T>(p, g.s, nb, A, sa, n, ipiv, info, j0, n);
             });
             swap_trsm<T>(p, g, A, ipiv, j0, k0, kend, 1, j0 + kPanel);
             gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, j0 + kPanel), n,
@@ -2507,6 +2796,8 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
     p->trace = std::getenv("REFEFF_FMS_TRACE") != nullptr;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_3M")) p->gemm_3m = std::atoi(e) != 0;
+    if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p->panel_calu = std::strcmp(e, "gepp") != 0;
+    if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
     for (int i = 0; ok && i < p->nstreams; ++i)
         ok = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking) == hipSuccess;
@@ -2622,6 +2913,8 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
         af.resize(nn);
         for (size_t i = 0; i < nn; ++i) af[i] = make_float2((float)a[2 * i], (float)a[2 * i + 1]);
         if (!dupload(&p.d_a32, af.data(), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
+        if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p.panel_calu = std::strcmp(e, "gepp") != 0;
+        if (!rc && p.panel_calu && !alloc_calu(&p, 1)) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc) run_lu<float>(&p, Group{p.stream, 0, 1}, p.d_a32, p.d_ipiv, p.d_perm, p.d_info);
         if (!rc && hipStreamSynchronize(p.stream) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc && hipMemcpy(af.data(), p.d_a32, nn * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;

@@ -95,9 +95,13 @@ def test_lu_double_large_matches_lapack(n):
     assert np.abs(work - ref_lu).max() <= 1e-10 * np.abs(ref_lu).max()
 
 
-@pytest.mark.parametrize("n", [64, 256, 4160])
-def test_lu_float_reconstructs_matrix(n):
-    print(f"[gpu-kernels] lu float n={n}", flush=True)
+@pytest.mark.parametrize("panel", ["calu", "gepp"])
+@pytest.mark.parametrize("n", [64, 256, 1152, 4160])
+def test_lu_float_reconstructs_matrix(n, panel, monkeypatch):
+    """FP32 LU with tournament-pivoting panels (default; 1152 and 4160 rows take a
+    selection level) and with the row-by-row partial-pivoting panels."""
+    monkeypatch.setenv("REFEFF_FMS_PANEL", panel)
+    print(f"[gpu-kernels] lu float n={n} panel={panel}", flush=True)
     rng = np.random.default_rng(100 + n)
     a = crand(rng, n, n)
     work = np.ascontiguousarray(a.copy())
@@ -113,12 +117,14 @@ def test_lu_float_reconstructs_matrix(n):
     assert np.abs(lo @ up - pa).max() <= 1e-4 * np.sqrt(n / 64) * np.abs(a).max()
 
 
-@pytest.mark.parametrize("n,use_double", [(9792, 0), (8256, 1)])
-def test_lu_tall_panels_solve_with_small_backward_error(n, use_double):
-    """Panels of 8192-10240 rows (register tiles of 2 columns in FP32, 1 in FP64):
-    the factors solve A x = b with a backward error at the working precision."""
+@pytest.mark.parametrize("n,use_double,panel", [(9792, 0, "calu"), (9792, 0, "gepp"), (8256, 1, "gepp")])
+def test_lu_tall_panels_solve_with_small_backward_error(n, use_double, panel, monkeypatch):
+    """Panels of 8192-10240 rows (FP32: two tournament selection levels, or register
+    tiles of 2 columns; FP64: register tiles of 1 column): the factors solve
+    A x = b with a backward error at the working precision."""
     import scipy.linalg as sla
-    print(f"[gpu-kernels] lu tall panels n={n} double={use_double}", flush=True)
+    monkeypatch.setenv("REFEFF_FMS_PANEL", panel)
+    print(f"[gpu-kernels] lu tall panels n={n} double={use_double} panel={panel}", flush=True)
     rng = np.random.default_rng(n)
     a = crand(rng, n, n) / np.sqrt(n) + 2.0 * np.eye(n)
     work = np.ascontiguousarray(a.copy())
@@ -145,3 +151,36 @@ def test_lu_reports_exact_singularity():
     info = C.c_int32(-1)
     assert _lib().refeff_fms_debug_lu(n, 1, _p(work), piv.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(info)) == 0
     assert info.value == 8  # first zero pivot, one-based, as LAPACK info
+
+
+def test_lu_float_tournament_reports_singularity_and_top_row_pivots(monkeypatch):
+    """Tournament-pivoting panels: exactly zero columns give LAPACK's info (first
+    zero pivot, one-based) and pivots drawn from the panel's own top rows and from
+    far below both land as valid interchanges."""
+    monkeypatch.setenv("REFEFF_FMS_PANEL", "calu")
+    print("[gpu-kernels] lu float tournament singular / top-row pivots", flush=True)
+    n = 1088
+    rng = np.random.default_rng(9)
+    a = crand(rng, n, n)
+    a[:16, :32] *= 50.0       # half the first panel's pivots come from its own top rows
+    a[900:916, :32] *= 1e-3    # (never pivots of the first panel, so they stay in place)
+    a[900:916, 32:64] *= 50.0  # the second panel's from the third selection chunk
+    work = np.ascontiguousarray(a.copy())
+    piv = np.zeros(n, np.int32)
+    info = C.c_int32(-1)
+    assert _lib().refeff_fms_debug_lu(n, 0, _p(work), piv.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(info)) == 0
+    assert info.value == 0
+    assert np.all((piv >= np.arange(n)) & (piv < n))
+    assert set(range(900, 916)) <= set(piv[32:64].tolist())
+    lo = np.tril(work, -1) + np.eye(n)
+    up = np.triu(work)
+    pa = a.copy()
+    for i, p in enumerate(piv):
+        pa[[i, p]] = pa[[p, i]]
+    assert np.abs(lo @ up - pa).max() <= 1e-4 * np.sqrt(n / 64) * np.abs(a).max()
+    s = a.copy()
+    s[:, 7] = 0.0
+    s[:, 600] = 0.0
+    work = np.ascontiguousarray(s)
+    assert _lib().refeff_fms_debug_lu(n, 0, _p(work), piv.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(info)) == 0
+    assert info.value == 8
