@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the LU trailing-update kernel, averaged over one factorisation's
 # launches, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script scaled by the
 # same counters read for a 1 GiB device copy (profiles/fms_cu_r8_gemm_v2_traffic.md).
-TRAFFIC_BYTES_PER_LAUNCH = {("cfg2", "mixed"): 9191833259}
+TRAFFIC_BYTES_PER_LAUNCH = {("cfg2", "mixed"): 9234017224}
 
 
 def parse():

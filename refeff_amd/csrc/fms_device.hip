@@ -1648,16 +1648,18 @@ __device__ __forceinline__ void reduce_waves(typename Cx<T>::type (&red)[NW / 2]
     __syncthreads();
 }
 
-// Whole triangular solve with one workgroup per (energy, right-hand-side
-// chunk of up to 16*NT columns): X <- L^-1 X (unit lower, forward) or
-// X <- U^-1 X (backward).  Per 32-row block the off-diagonal part of the block
-// row times the already solved rows is formed with row_block_product, and the
-// block is finished as X_b = inv(D_b) (X_b - S) with the precomputed inverse
-// diagonal block.  L/U is read once per solve.
+// Triangular solve of the diagonal block rows [r_lo, r_hi) with one workgroup
+// per (energy, right-hand-side chunk of up to 16*NT columns): X <- L^-1 X
+// (unit lower, forward) or X <- U^-1 X (backward), the coupling to rows
+// outside [r_lo, r_hi) already subtracted (k_solve_gemv).  Per 32-row block
+// the off-diagonal part of the block row times the already solved rows is
+// formed with row_block_product, and the block is finished as
+// X_b = inv(D_b) (X_b - S) with the precomputed inverse diagonal block.
 template <typename T, bool LOWER, int NT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_trsm_wg(const typename Cx<T>::type* LUb, size_t sa, int lda,
                                                      const typename Cx<T>::type* dinvb, typename Cx<T>::type* Xb,
-                                                     size_t sx, int ldx, int n, const int* emap) {
+                                                     size_t sx, int ldx, int n, int r_lo, int r_hi,
+                                                     const int* emap) {
     using T2 = typename Cx<T>::type;
     using acc_t = typename Mfma<T>::acc;
     constexpr int BS = kPanel, NC = NT * 16, NTH = NW * 64;
@@ -1671,12 +1673,13 @@ __global__ __launch_bounds__(NW * 64) void k_trsm_wg(const typename Cx<T>::type*
     const int nblk = n / BS;
     const size_t blk_sz = static_cast<size_t>(BS) * BS;
     const T2* dinv = dinvb + (static_cast<size_t>(e) * 2 + (LOWER ? 0 : 1)) * nblk * blk_sz;
-    for (int step = 0; step < nblk; ++step) {
-        const int blk = LOWER ? step : nblk - 1 - step;
+    const int b_lo = r_lo / BS, b_hi = r_hi / BS;
+    for (int step = 0; step < b_hi - b_lo; ++step) {
+        const int blk = LOWER ? b_lo + step : b_hi - 1 - step;
         const int R0 = blk * BS;
         for (int i = tid; i < BS * BS; i += NTH) dia[i / BS][i % BS] = dinv[blk * blk_sz + i];
-        const int kb = LOWER ? 0 : R0 + BS;
-        const int nch = ((LOWER ? R0 : n) - kb) / 16;
+        const int kb = LOWER ? r_lo : R0 + BS;
+        const int nch = ((LOWER ? R0 : r_hi) - kb) / 16;
         acc_t cr[2][NT], ci[2][NT];
         row_block_product<T, NT, NW>(LU, lda, R0, kb, nch, X, ldx, c0, ncol, cr, ci);
         reduce_waves<T, NT, NW>(red, cr, ci);
@@ -1700,6 +1703,33 @@ __global__ __launch_bounds__(NW * 64) void k_trsm_wg(const typename Cx<T>::type*
             }
         }
         __syncthreads();
+    }
+}
+
+// X[rows] -= A[rows, k_lo:k_hi) X[k_lo:k_hi) for the 32-row blocks from r_lo,
+// one workgroup per (row block, energy, right-hand-side chunk): the coupling
+// of a diagonal block row of a blocked triangular solve to the rows already
+// solved, spread over the whole GPU.
+template <typename T, int NT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_solve_gemv(const typename Cx<T>::type* Ab, size_t sa, int lda,
+                                                        typename Cx<T>::type* Xb, size_t sx, int ldx, int r_lo,
+                                                        int k_lo, int k_hi, const int* emap) {
+    using T2 = typename Cx<T>::type;
+    constexpr int BS = kPanel, NC = NT * 16;
+    __shared__ T2 red[NW / 2][BS][NC];
+    const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y), tid = threadIdx.x;
+    const int R0 = r_lo + blockIdx.x * BS, c0 = blockIdx.z * NC;
+    const int ncol = min(NC, ldx - c0);
+    T2* X = Xb + e * sx;
+    typename Mfma<T>::acc cr[2][NT], ci[2][NT];
+    row_block_product<T, NT, NW>(Ab + e * sa, lda, R0, k_lo, (k_hi - k_lo) / 16, X, ldx, c0, ncol, cr, ci);
+    reduce_waves<T, NT, NW>(red, cr, ci);
+    for (int i = tid; i < BS * NC; i += NW * 64) {
+        const int r = i / NC, c = i % NC;
+        if (c < ncol) {
+            T2* q = X + static_cast<size_t>(R0 + r) * ldx + c0 + c;
+            *q = csub(*q, red[1][r][c]);
+        }
     }
 }
 
@@ -2377,24 +2407,58 @@ void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type*
     const int chunks = (nr + 16 * ntile - 1) / (16 * ntile);
     const double cb = sizeof(typename Cx<T>::type);
     const double nn = static_cast<double>(n) * n;
-    launch(p, g.s, REFEFF_FMS_K_SOLVE, 4.0 * nn * nr * nb, (nn / 2 * chunks + 2.0 * n * nr) * cb * nb, [&] {
-        const dim3 grid(nb, chunks);
-        // one workgroup walks a whole triangle: 16 waves for the common <= 16
-        // right-hand sides in FP32 (more loads in flight), 8 otherwise
-        if constexpr (sizeof(T) == 4) {
-            if (ntile == 1)
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 16>), grid, dim3(1024), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
-            else if (ntile == 2)
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
-            else
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
-        } else {
-            if (ntile == 1)
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
-            else
-                hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n, emap);
+    (void)nn;
+    // Blocked by row blocks of bs rows (about 16 per triangle): the coupling
+    // of each block row to the rows already solved is one launch over every
+    // (32-row block, energy) pair; only the diagonal blocks are walked by one
+    // workgroup per energy.
+    const int bs = std::max(256, ((n / 16 + kPanel - 1) / kPanel) * kPanel);
+    const int nbig = (n + bs - 1) / bs;
+    for (int step = 0; step < nbig; ++step) {
+        const int I = LOWER ? step : nbig - 1 - step;
+        const int r_lo = I * bs, r_hi = std::min(n, r_lo + bs);
+        const int k_lo = LOWER ? 0 : r_hi, k_hi = LOWER ? r_lo : n;
+        const double rows = r_hi - r_lo;
+        if (k_hi > k_lo) {
+            const double kl = k_hi - k_lo;
+            launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * rows * kl * nr * nb, (rows * kl * chunks + 2.0 * kl * nr) * cb * nb,
+                   [&] {
+                       const dim3 grid((r_hi - r_lo) / kPanel, nb, chunks);
+                       if (ntile == 1)
+                           hipLaunchKernelGGL((k_solve_gemv<T, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo,
+                                              k_lo, k_hi, emap);
+                       else if (ntile == 2 || sizeof(T) == 8)  // FP64 chunks hold at most 32 columns
+                           hipLaunchKernelGGL((k_solve_gemv<T, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo,
+                                              k_lo, k_hi, emap);
+                       else if constexpr (sizeof(T) == 4)
+                           hipLaunchKernelGGL((k_solve_gemv<T, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo,
+                                              k_lo, k_hi, emap);
+                   });
         }
-    });
+        launch(p, g.s, REFEFF_FMS_K_SOLVE, 4.0 * rows * rows * nr * nb, (rows * rows / 2 * chunks + 2.0 * rows * nr) * cb * nb, [&] {
+            const dim3 grid(nb, chunks);
+            // 16 waves for the common <= 16 right-hand sides in FP32 (more
+            // loads in flight), 8 otherwise
+            if constexpr (sizeof(T) == 4) {
+                if (ntile == 1)
+                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 16>), grid, dim3(1024), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
+                                       r_lo, r_hi, emap);
+                else if (ntile == 2)
+                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
+                                       r_lo, r_hi, emap);
+                else
+                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
+                                       r_lo, r_hi, emap);
+            } else {
+                if (ntile == 1)
+                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
+                                       r_lo, r_hi, emap);
+                else
+                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
+                                       r_lo, r_hi, emap);
+            }
+        });
+    }
 }
 
 template <typename T>
