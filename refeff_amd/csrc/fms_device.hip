@@ -2092,6 +2092,9 @@ struct refeff_fms_plan {
     bool trace = false;                 // REFEFF_FMS_TRACE: print refinement progress to stderr
     bool gemm_3m = true;                // REFEFF_FMS_GEMM_3M=0: FP32 trailing updates with four real products
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool lookahead = false;             // REFEFF_FMS_LOOKAHEAD=1: far trailing update on a second stream
+    hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t aux_ev[4][2] = {};
     std::mutex mu;
     // kernel-level instrumentation (refeff_fms_plan_profile)
     struct Rec {
@@ -2153,6 +2156,10 @@ void drain_profile(refeff_fms_plan* p) {
 struct Group {
     hipStream_t s;
     int e0, nb;
+    // LU lookahead: second stream for the far trailing update and the two
+    // events ordering it (null: no lookahead)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_u12 = nullptr, ev_far = nullptr;
 };
 
 template <typename F>
@@ -2337,6 +2344,15 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
     const size_t sa = static_cast<size_t>(n) * n;
     const double cb = sizeof(typename Cx<T>::type);
     auto at = [&](int r, int c) { return A + static_cast<size_t>(r) * n + c; };
+    // Lookahead: the trailing update of block k is split into the next
+    // block's columns (main stream, first) and the far columns (second
+    // stream), so the next block's panels overlap the far update.  The next
+    // block's interchanges touch block k's L21 columns and the far columns,
+    // so they wait for the far update.
+    const bool la = g.s2 != nullptr;
+    Group g2 = g;
+    g2.s = g.s2;
+    bool far_pending = false;
     for (int k0 = 0; k0 < n; k0 += kOuter) {
         const int kend = std::min(k0 + kOuter, n);
         for (int j0 = k0; j0 < kend; j0 += kPanel) {
@@ -2347,6 +2363,10 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
             swap_trsm<T>(p, g, A, ipiv, j0, k0, kend, 1, j0 + kPanel);
             gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, j0 + kPanel), n,
                         sa, n - j0 - kPanel, kend - j0 - kPanel, kPanel);
+        }
+        if (far_pending) {
+            (void)hipStreamWaitEvent(g.s, g.ev_far, 0);
+            far_pending = false;
         }
         // all interchanges of the outer block, left and right of it, before any far update
         for (int j0 = k0; j0 < kend; j0 += kPanel) {
@@ -2359,10 +2379,23 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
                 gemm_sub<T>(p, g, at(j0 + kPanel, kend), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, kend), n, sa,
                             kend - j0 - kPanel, n - kend, kPanel);
             }
-            gemm_sub<T>(p, g, at(kend, kend), n, sa, at(kend, k0), n, sa, at(k0, kend), n, sa, n - kend, n - kend,
-                        kend - k0);
+            const int knext = std::min(kend + kOuter, n);
+            if (la && knext < n) {
+                gemm_sub<T>(p, g, at(kend, kend), n, sa, at(kend, k0), n, sa, at(k0, kend), n, sa, n - kend,
+                            knext - kend, kend - k0);
+                (void)hipEventRecord(g.ev_u12, g.s);
+                (void)hipStreamWaitEvent(g.s2, g.ev_u12, 0);
+                gemm_sub<T>(p, g2, at(kend, knext), n, sa, at(kend, k0), n, sa, at(k0, knext), n, sa, n - kend,
+                            n - knext, kend - k0);
+                (void)hipEventRecord(g.ev_far, g.s2);
+                far_pending = true;
+            } else {
+                gemm_sub<T>(p, g, at(kend, kend), n, sa, at(kend, k0), n, sa, at(k0, kend), n, sa, n - kend,
+                            n - kend, kend - k0);
+            }
         }
     }
+    if (far_pending) (void)hipStreamWaitEvent(g.s, g.ev_far, 0);
     launch(p, g.s, REFEFF_FMS_K_OTHER, 0.0, 8.0 * n * nb, [&] {
         hipLaunchKernelGGL(k_perm, dim3(nb), dim3(256), n * sizeof(int), g.s, ipiv, perm, n);
     });
@@ -2541,7 +2574,13 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         int e0 = 0;
         for (int i = 0; i < G; ++i) {
             const int cnt = nb / G + (i < nb % G ? 1 : 0);
-            gs.push_back(Group{p->streams[i] ? p->streams[i] : p->stream, e0, cnt});
+            Group g{p->streams[i] ? p->streams[i] : p->stream, e0, cnt};
+            if (p->lookahead && p->aux[i]) {
+                g.s2 = p->aux[i];
+                g.ev_u12 = p->aux_ev[i][0];
+                g.ev_far = p->aux_ev[i][1];
+            }
+            gs.push_back(g);
             e0 += cnt;
         }
     }
@@ -2863,8 +2902,13 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p->panel_calu = std::strcmp(e, "gepp") != 0;
     if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
+    if (const char* e = std::getenv("REFEFF_FMS_LOOKAHEAD")) p->lookahead = std::atoi(e) != 0;
     for (int i = 0; ok && i < p->nstreams; ++i)
         ok = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking) == hipSuccess;
+    for (int i = 0; ok && p->lookahead && i < p->nstreams; ++i)
+        ok = hipStreamCreateWithFlags(&p->aux[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&p->aux_ev[i][0], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&p->aux_ev[i][1], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         refeff_fms_plan_destroy(p);
         (void)hipGetLastError();
@@ -2892,6 +2936,11 @@ void refeff_fms_plan_destroy(refeff_fms_plan* p) {
     if (p->stream) (void)hipStreamDestroy(p->stream);
     for (hipStream_t s : p->streams)
         if (s) (void)hipStreamDestroy(s);
+    for (int i = 0; i < 4; ++i) {
+        if (p->aux[i]) (void)hipStreamDestroy(p->aux[i]);
+        for (hipEvent_t ev : p->aux_ev[i])
+            if (ev) (void)hipEventDestroy(ev);
+    }
     delete p;
 }
 
