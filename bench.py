@@ -101,7 +101,8 @@ def run_solver(lib, wl, solver_id, steps, warmup, profile):
     dout.download(out)
     sc = out.transpose(0, 1, 3, 2).copy()
     res = dict(seconds=t1 - t0, ne=ne, status=status.copy(), scattering=sc, stats=st, profile=prof,
-               breakdown=breakdown, n=plan.info.state_count, npad=plan.info.padded_order, batch=plan.info.batch)
+               breakdown=breakdown, n=plan.info.state_count, npad=plan.info.padded_order, batch=plan.info.batch,
+               lu_block=plan.info.lu_block)
     for d in (dk, dph, dout):
         d.free()
     plan.close()
@@ -186,6 +187,7 @@ def main():
     lead = re.match(r"\s*[+-]?\d+", flag or "")
     gemm_3m = prec == "fp32" and (flag is None or (lead is not None and int(lead.group()) != 0))
     bd = main_res["breakdown"]
+    kb = main_res["lu_block"]
     n = main_res["n"]
     kernels = {name: {"ms": round(bd.milliseconds[i], 3), "launches": int(bd.launches[i]),
                       "tflops": round(bd.flops[i] / (bd.milliseconds[i] * 1e-3) / 1e12, 3)
@@ -201,13 +203,13 @@ def main():
         "dtype": "c64 LU + f64 refinement" if solver_id == 1 else "c128",
         "data": "synthetic (seeded lattice cluster, smooth model phase shifts)",
         "config": {"workload": wl.name, "atoms": wl.nat, "states": n, "padded": main_res["npad"], "energies": ne,
-                   "solver": args.solver, "batch": main_res["batch"],
+                   "solver": args.solver, "batch": main_res["batch"], "lu_block": kb,
                    "refinement_sweeps": int(main_res["stats"].refinement_sweeps)},
         "roofline": {"bound": "mfma",
-                     "kernel": (("k_gemm_v2<16, 3M> (LU trailing update, K=128; 3 real MFMA products per complex "
-                                 "product, achieved counts the 6MNK flops executed)" if gemm_3m else
-                                 "k_gemm_v2<16> (LU trailing update, K=128)") if prec == "fp32" else
-                                "k_gemm_sub<double> (LU trailing update, K=128)"),
+                     "kernel": ((f"k_gemm_v2<16, 3M> (LU trailing update, K={kb}; 3 real MFMA products per "
+                                 "complex product, achieved counts the 6MNK flops executed)" if gemm_3m else
+                                 f"k_gemm_v2<16> (LU trailing update, K={kb})") if prec == "fp32" else
+                                f"k_gemm_sub<double> (LU trailing update, K={kb})"),
                      "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_TFLOPS[prec], 4),
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((args.config, args.solver)),

@@ -135,6 +135,7 @@ typedef struct refeff_fms_plan_info {
     int32_t rhs_count;        /* right-hand sides per energy */
     int32_t solver;           /* effective refeff_fms_solver */
     int32_t batch;            /* energies per device batch */
+    int32_t lu_block;         /* LU outer block = K of the trailing update (LU solvers) */
 } refeff_fms_plan_info;
 
 typedef struct refeff_fms_stats {

@@ -51,7 +51,8 @@ class PlanDesc(C.Structure):
 
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("state_count", "padded_order", "channel_count", "potential_count",
-                                          "potential_start", "potential_end", "rhs_count", "solver", "batch")]
+                                          "potential_start", "potential_end", "rhs_count", "solver", "batch",
+                                          "lu_block")]
 
 
 class Stats(C.Structure):

@@ -460,6 +460,17 @@ __global__ void k_fill_int(int* p, int v, size_t n) {
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
         p[i] = v;
 }
+// x[i] = a hash of (i, seed) mapped to [-1, 1): benchmark operands only.
+__global__ __launch_bounds__(256) void k_fill_hash(float* x, size_t n, unsigned seed) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+        unsigned h = static_cast<unsigned>(i) * 2654435761u ^ (seed * 0x9E3779B9u) ^ static_cast<unsigned>(i >> 32);
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        x[i] = static_cast<float>(h & 0xFFFFFF) * (2.0f / 16777216.0f) - 1.0f;
+    }
+}
+
 __global__ void k_fill_double(double* p, double v, size_t n) {
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
@@ -1325,7 +1336,10 @@ __device__ __forceinline__ void xcd_tile(int tiles, int& tile, int& e) {
 // P3 = (Ar + Ai)(Br + Bi) instead of four; Re = P1 - P2, Im = P3 - P1 - P2.
 // Its rounding error is of the same order as the four-product form's in
 // norm; in the LU of the mixed solver the FP64 residual refinement absorbs it.
-template <int BK, bool XCD, bool M3 = false>
+//
+// CPF: the workgroup's C tile is loaded into registers before the K loop, so
+// its latency overlaps the products instead of stalling the epilogue.
+template <int BK, bool XCD, bool M3 = false, bool CPF = false>
 __global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb, int ldc, size_t sC, const float2* Ab_,
                                                                    int lda, size_t sA, const float2* Bb_, int ldb,
                                                                    size_t sB, int M, int N, int K) {
@@ -1388,6 +1402,19 @@ __global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb,
             if (M3) cs[M3 ? i : 0][M3 ? j : 0] = v4f{0, 0, 0, 0};
         }
     gload(0);
+    float2 cpre[CPF ? 2 : 1][CPF ? 2 : 1][CPF ? 4 : 1];
+    if constexpr (CPF) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+                    const int col = n0 + wc * 32 + j * 16 + (lane & 15);
+                    cpre[i][j][r] = (row < M && col < N) ? C[static_cast<size_t>(row) * ldc + col] : float2{0, 0};
+                }
+    }
     sstore(0);
     __syncthreads();
     int st = 0;
@@ -1458,10 +1485,138 @@ __global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb,
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
                 if (row < M && col < N) {
                     float2* q = C + static_cast<size_t>(row) * ldc + col;
-                    float2 v = *q;
+                    float2 v;
+                    if constexpr (CPF)
+                        v = cpre[CPF ? i : 0][CPF ? j : 0][CPF ? r : 0];
+                    else
+                        v = *q;
                     v.x -= cr[i][j][r];
                     v.y -= ci[i][j][r];
                     *q = v;
+                }
+            }
+}
+
+// C -= A * B, FP32 complex in the 3M form, with a larger register tile than
+// k_gemm_v2: each of the 4 waves computes (16 TI) x (16 TJ) of C, the
+// workgroup (32 TI) x (32 TJ).  K in steps of BK with register prefetch of the
+// next step and two LDS stages; re and im are kept in separate LDS planes,
+// A transposed to [k][m], B as [k][n], both padded to a row length = 16
+// (mod 64) so the MFMA operand reads are free of bank conflicts.
+// Per K step of 4 a wave reads 2 TI + 2 TJ values and issues 3 TI TJ MFMAs.
+// Requirements (host-checked): N even, K % BK == 0, 16-byte aligned rows.
+template <int TI, int TJ, int BK>
+__global__ __launch_bounds__(256, 2) void k_gemm_v3(float2* Cb, int ldc, size_t sC, const float2* Ab_, int lda,
+                                                    size_t sA, const float2* Bb_, int ldb, size_t sB, int M, int N,
+                                                    int K) {
+    constexpr int BM = 32 * TI, BN = 32 * TJ;
+    constexpr int LDA = BM + 16, LDB = BN + 16;
+    constexpr int APT = BM * BK / 256;      // complex values of A per thread per step
+    constexpr int TPR = BK / APT;           // threads per A row
+    constexpr int BPT = BN * BK / 256;      // complex values of B per thread per step
+    constexpr int TPB = BN / BPT;           // threads per B row
+    static_assert(APT % 2 == 0 && BPT % 2 == 0 && TPR >= 1 && TPB >= 1, "16-byte loads");
+    __shared__ float as_re[2][BK][LDA], as_im[2][BK][LDA];
+    __shared__ float bs_re[2][BK][LDB], bs_im[2][BK][LDB];
+    const int tilesN = (N + BN - 1) / BN;
+    const int tile = blockIdx.x, e = blockIdx.y;
+    const int m0 = (tile / tilesN) * BM, n0 = (tile % tilesN) * BN;
+    const float2* A = Ab_ + e * sA;
+    const float2* B = Bb_ + e * sB;
+    float2* C = Cb + e * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int ar = tid / TPR, ak = (tid % TPR) * APT;
+    const int bk = tid / TPB, bc = (tid % TPB) * BPT;
+    float4 ra[APT / 2], rb[BPT / 2];
+    auto gload = [&](int k0) {
+        const int gr = m0 + ar;
+        const float4* pa = reinterpret_cast<const float4*>(A + static_cast<size_t>(gr) * lda + k0 + ak);
+#pragma unroll
+        for (int i = 0; i < APT / 2; ++i) ra[i] = (gr < M) ? pa[i] : float4{0, 0, 0, 0};
+        const int gc = n0 + bc;
+        const float4* pb = reinterpret_cast<const float4*>(B + static_cast<size_t>(k0 + bk) * ldb + gc);
+#pragma unroll
+        for (int i = 0; i < BPT / 2; ++i) rb[i] = (gc + 2 * i < N) ? pb[i] : float4{0, 0, 0, 0};
+    };
+    auto sstore = [&](int st) {
+#pragma unroll
+        for (int i = 0; i < APT / 2; ++i) {
+            as_re[st][ak + 2 * i][ar] = ra[i].x;
+            as_im[st][ak + 2 * i][ar] = ra[i].y;
+            as_re[st][ak + 2 * i + 1][ar] = ra[i].z;
+            as_im[st][ak + 2 * i + 1][ar] = ra[i].w;
+        }
+#pragma unroll
+        for (int i = 0; i < BPT / 2; ++i) {
+            bs_re[st][bk][bc + 2 * i] = rb[i].x;
+            bs_im[st][bk][bc + 2 * i] = rb[i].y;
+            bs_re[st][bk][bc + 2 * i + 1] = rb[i].z;
+            bs_im[st][bk][bc + 2 * i + 1] = rb[i].w;
+        }
+    };
+    v4f p1[TI][TJ], p2[TI][TJ], p3[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+            p1[i][j] = v4f{0, 0, 0, 0};
+            p2[i][j] = v4f{0, 0, 0, 0};
+            p3[i][j] = v4f{0, 0, 0, 0};
+        }
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int st = 0;
+    const int ix = lane & 15, q = lane >> 4;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) gload(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + q;
+            float xr[TI], xi[TI], xs[TI], yr[TJ], yi[TJ], ys[TJ];
+#pragma unroll
+            for (int i = 0; i < TI; ++i) {
+                xr[i] = as_re[st][kr][wr * 16 * TI + i * 16 + ix];
+                xi[i] = as_im[st][kr][wr * 16 * TI + i * 16 + ix];
+                xs[i] = xr[i] + xi[i];
+            }
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) {
+                yr[j] = bs_re[st][kr][wc * 16 * TJ + j * 16 + ix];
+                yi[j] = bs_im[st][kr][wc * 16 * TJ + j * 16 + ix];
+                ys[j] = yr[j] + yi[j];
+            }
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yr[j], p1[i][j], 0, 0, 0);
+                    p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[i], yi[j], p2[i][j], 0, 0, 0);
+                    p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xs[i], ys[j], p3[i][j], 0, 0, 0);
+                }
+        }
+        if (more) sstore(st ^ 1);
+        __syncthreads();
+        st ^= 1;
+    }
+    // C -= (P1 - P2) + i (P3 - P1 - P2)
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 16 * TI + i * 16 + 4 * q + r;
+                const int col = n0 + wc * 16 * TJ + j * 16 + ix;
+                if (row < M && col < N) {
+                    float2* c = C + static_cast<size_t>(row) * ldc + col;
+                    float2 v = *c;
+                    const float a = p1[i][j][r], b = p2[i][j][r];
+                    v.x -= a - b;
+                    v.y -= p3[i][j][r] - a - b;
+                    *c = v;
                 }
             }
 }
@@ -2060,7 +2215,7 @@ using namespace refeff;
 struct refeff_fms_plan {
     int nat = 0, nsp = 1, ispin = 0, lfms = 0, minv = 0, max_pot = 0, L = 0;
     int n = 0, npad = 0, nrhs = 0, ch = 0, npot = 0, pstart = 0, pend = 0, solver = 1, batch = 0, max_batch = 0;
-    int outer_block = 128;  // LU outer block width = K of the trailing MFMA update
+    int outer_block = 128;  // LU outer block width = K of the trailing MFMA update (set by size)
     float rdirec = 0.f;
     float toler1 = 0.f, toler2 = 0.f;  // one-shot iterative branch (minv = 1)
     std::vector<int> rep, col0, ipart;
@@ -2091,6 +2246,7 @@ struct refeff_fms_plan {
     int nstreams = 1;                   // energy sub-batches issued on separate streams
     bool trace = false;                 // REFEFF_FMS_TRACE: print refinement progress to stderr
     bool gemm_3m = true;                // REFEFF_FMS_GEMM_3M=0: FP32 trailing updates with four real products
+    bool gemm_cpf = true;               // REFEFF_FMS_GEMM_CPF=0: C tile read in the epilogue
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     bool lookahead = false;             // REFEFF_FMS_LOOKAHEAD=1: far trailing update on a second stream
     hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -2304,7 +2460,11 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
            (v2 && p->gemm_3m ? 6.0 : 8.0) * M * N * K * nb,
            (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
                const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-               if (v2 && p->gemm_3m)
+               if (v2 && p->gemm_3m && p->gemm_cpf)
+                   hipLaunchKernelGGL((k_gemm_v2<16, false, true, true>), dim3(tiles, nb), dim3(256), 0, g.s,
+                                      reinterpret_cast<float2*>(C), ldc, sC, reinterpret_cast<const float2*>(A), lda,
+                                      sA, reinterpret_cast<const float2*>(B), ldb, sB, M, N, K);
+               else if (v2 && p->gemm_3m)
                    hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, nb), dim3(256), 0, g.s,
                                       reinterpret_cast<float2*>(C), ldc, sC, reinterpret_cast<const float2*>(A), lda,
                                       sA, reinterpret_cast<const float2*>(B), ldb, sB, M, N, K);
@@ -2830,6 +2990,10 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->toler1 = d->convergence_tolerance;
     p->toler2 = d->zero_tolerance;
     p->max_batch = d->max_batch;
+    // outer block (K of the trailing update) by size, as measured: a wider
+    // block halves the passes over the trailing matrix but grows the in-block
+    // K = 32 work (N = 2832: 192 best; N = 9744: 256 best)
+    p->outer_block = p->npad >= 4096 ? 256 : p->npad >= 2048 ? 192 : 128;
     if (const char* ob = std::getenv("REFEFF_FMS_LU_BLOCK")) {
         const int v = std::atoi(ob);
         if (v >= kPanel && v % kPanel == 0 && v <= 1024) p->outer_block = v;
@@ -2899,6 +3063,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
     p->trace = std::getenv("REFEFF_FMS_TRACE") != nullptr;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_3M")) p->gemm_3m = std::atoi(e) != 0;
+    if (const char* e = std::getenv("REFEFF_FMS_GEMM_CPF")) p->gemm_cpf = std::atoi(e) != 0;
     if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p->panel_calu = std::strcmp(e, "gepp") != 0;
     if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
@@ -2955,6 +3120,7 @@ int refeff_fms_plan_query(const refeff_fms_plan* p, refeff_fms_plan_info* info) 
     info->rhs_count = p->nrhs;
     info->solver = p->solver;
     info->batch = p->batch;
+    info->lu_block = p->outer_block;
     return REFEFF_FMS_OK;
 }
 
@@ -3047,7 +3213,7 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
     const int kmul = (use_double == 0 || use_double == 3 || use_double == 5) ? 32 : 16;
-    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 7)
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 11)
         return REFEFF_FMS_INVALID_ARGUMENT;
     const int ld = std::max(std::max(M, N), K);
     const size_t sz = static_cast<size_t>(ld) * ld;
@@ -3079,8 +3245,16 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                     hipLaunchKernelGGL((k_gemm_v2<32, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
                 else if (use_double == 6)
                     hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
-                else
+                else if (use_double == 7)
                     hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else if (use_double == 8)
+                    hipLaunchKernelGGL((k_gemm_v2<16, false, true, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else if (use_double == 9)
+                    hipLaunchKernelGGL((k_gemm_v3<4, 2, 16>), dim3(((M + 127) / 128) * ((N + 63) / 64), 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else if (use_double == 10)
+                    hipLaunchKernelGGL((k_gemm_v3<4, 4, 8>), dim3(((M + 127) / 128) * ((N + 127) / 128), 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
+                else
+                    hipLaunchKernelGGL((k_gemm_v3<2, 4, 16>), dim3(((M + 63) / 64) * ((N + 127) / 128), 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
             } else if (use_double == 2 && sizeof(T) == 4) {
                 const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
                 hipLaunchKernelGGL(k_gemm_big, dim3(tiles, 1), dim3(256), 0, nullptr, reinterpret_cast<float2*>(dc), ld,
@@ -3123,14 +3297,23 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
         if (dc) (void)hipFree(dc);
         return REFEFF_FMS_NO_MEMORY;
     }
-    (void)hipMemset(da, 0, per * batch * esz);
-    (void)hipMemset(db, 0, per * batch * esz);
-    (void)hipMemset(dc, 0, per * batch * esz);
+    // pseudo-random operands in [-1, 1): zeros would understate the power draw
+    const size_t nfl = per * batch * esz / sizeof(float);
+    const dim3 fg(static_cast<unsigned>(std::min<size_t>(65535, (nfl + 255) / 256)));
+    if (kernel == 1) {
+        (void)hipMemset(da, 0, per * batch * esz);
+        (void)hipMemset(db, 0, per * batch * esz);
+        (void)hipMemset(dc, 0, per * batch * esz);
+    } else {
+        hipLaunchKernelGGL(k_fill_hash, fg, dim3(256), 0, nullptr, static_cast<float*>(da), nfl, 1u);
+        hipLaunchKernelGGL(k_fill_hash, fg, dim3(256), 0, nullptr, static_cast<float*>(db), nfl, 2u);
+        hipLaunchKernelGGL(k_fill_hash, fg, dim3(256), 0, nullptr, static_cast<float*>(dc), nfl, 3u);
+    }
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     auto one = [&]() {
-        if (kernel >= 3 && kernel <= 7) {
+        if (kernel >= 3 && kernel <= 11) {
             const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
             auto* c2 = static_cast<float2*>(dc);
             auto* a2 = static_cast<const float2*>(da);
@@ -3143,8 +3326,16 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
                 hipLaunchKernelGGL((k_gemm_v2<32, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
             else if (kernel == 6)
                 hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
-            else
+            else if (kernel == 7)
                 hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else if (kernel == 8)
+                hipLaunchKernelGGL((k_gemm_v2<16, false, true, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else if (kernel == 9)
+                hipLaunchKernelGGL((k_gemm_v3<4, 2, 16>), dim3(((M + 127) / 128) * ((N + 63) / 64), batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else if (kernel == 10)
+                hipLaunchKernelGGL((k_gemm_v3<4, 4, 8>), dim3(((M + 127) / 128) * ((N + 127) / 128), batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
+            else
+                hipLaunchKernelGGL((k_gemm_v3<2, 4, 16>), dim3(((M + 63) / 64) * ((N + 127) / 128), batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
         } else if (kernel == 2) {
             const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
             hipLaunchKernelGGL(k_gemm_big, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc), ld, per,

@@ -40,11 +40,13 @@ def test_gemm_big_tile(m, n, k):
     _gemm_case(2, m, n, k)
 
 
-@pytest.mark.parametrize("kind", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("kind", [3, 4, 5, 6, 7, 8, 9, 10, 11])
 @pytest.mark.parametrize("m,n,k", [(256, 256, 128), (320, 448, 64), (130, 260, 32), (96, 16, 32), (2880, 16, 32)])
 def test_gemm_v2(kind, m, n, k):
     """64x64-tile double-buffered FP32 kernel of the LU updates: BK 32 / 16, with and
-    without the XCD-aware tile order (kinds 3, 4, 5, 6), and the 3M form (kind 7)."""
+    without the XCD-aware tile order (kinds 3, 4, 5, 6), and the 3M form (kind 7; kind 8 with
+    the C tile prefetched); the larger register tiles of k_gemm_v3 (kinds 9, 10, 11: 64x32,
+    64x64 and 32x64 per wave)."""
     _gemm_case(kind, m, n, k)
 
 
