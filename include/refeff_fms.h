@@ -120,7 +120,7 @@ typedef struct refeff_fms_plan_desc {
     const float* mean_square_displacements; /* [atom_count*atom_count], sigsqr(atom2,atom1) at atom2+atom_count*atom1; NULL = 0 */
     float convergence_tolerance;        /* toler1 */
     float zero_tolerance;               /* toler2 */
-    int32_t full_scattering_requested;  /* save_gg_slice: not provided (REFEFF_FMS_UNSUPPORTED) */
+    int32_t full_scattering_requested;  /* save_gg_slice: also solve A^-1 G0 for every G0 column (forces LU) */
     int32_t solver;                     /* refeff_fms_solver */
     int32_t max_batch;                  /* energies per device batch; 0 = sized from free device memory */
 } refeff_fms_plan_desc;
@@ -176,6 +176,16 @@ int refeff_fms_solve_batch(refeff_fms_plan* plan, int32_t energy_count, const do
 int refeff_fms_solve_batch_device(refeff_fms_plan* plan, int32_t energy_count, const double* d_wave_numbers,
                                   const double* d_phase_shifts, double* d_scattering, int32_t* status,
                                   refeff_fms_stats* stats, void* stream);
+
+/* Same as refeff_fms_solve_batch, and also returns full_scattering = A^-1 G0
+ * (fms_lu_scattering with calculate_full_scattering, solvers.rs:473-477, as
+ * engine fms.rs:3659-3665 collects it for save_gg_slice).  The plan must have
+ * been created with full_scattering_requested.  Host pointers:
+ *   full_scattering: [ne][N][N] complex, column-major per energy (row + N*col),
+ *                    N = state_count. */
+int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, const double* wave_numbers,
+                                const double* phase_shifts, double* scattering, double* full_scattering,
+                                int32_t* status, refeff_fms_stats* stats);
 
 /* ---------------------------------------------------------------------- */
 /* measurement hooks                                                        */
@@ -234,14 +244,15 @@ int refeff_fms_mkgtr_trace(int32_t energy_count, int32_t channel_count, int32_t 
 /* ---------------------------------------------------------------------- */
 
 /* Reads fms.inp, phase.bin, geom.dat, global.inp (optionally .dimensions.dat)
- * and writes gg.bin, gg.dat and log3.dat.  Replaces run_fms_in_dir,
+ * and writes gg.bin, gg.dat and log3.dat, plus gg_slice.bin and gg_diag.bin
+ * when fms.inp sets save_gg_slice.  Replaces run_fms_in_dir,
  * crates/refeff-engine/src/fms.rs:462-539, on its ordinary source path
  * (build_fms_source_outputs, :3549-3695) with the global SIG2 damping
- * (idwopt < 0); other Debye-Waller models, save_gg_slice, reciprocal-space
- * and Hubbard sources return REFEFF_FMS_UNSUPPORTED.  Without source files a
- * cached gg.bin / gg.dat is re-emitted canonically.  solver: a
- * refeff_fms_solver.  *files_written receives the file count; message
- * receives a context string on failure. */
+ * (idwopt < 0); other Debye-Waller models, reciprocal-space and Hubbard
+ * sources return REFEFF_FMS_UNSUPPORTED.  Without source files a cached
+ * gg.bin / gg.dat (and gg_slice.bin / gg_diag.bin) is re-emitted
+ * canonically.  solver: a refeff_fms_solver.  *files_written receives the
+ * file count; message receives a context string on failure. */
 int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_written, char* message,
                           size_t message_len);
 
@@ -371,6 +382,20 @@ int refeff_io_fms_bin_parse(const char* text, double* cluster_radius, int32_t* c
 int refeff_io_gtr_dat_string(int32_t count, const double* energy, const double* trace, char* out, size_t out_len,
                              size_t* needed);
 int refeff_io_gtr_dat_parse(const char* text, int32_t* count, double* energy, double* trace);
+
+/* gg_slice.bin / gg_diag.bin (binary: *needed = byte count, no terminator).
+ * Replace rhorrp_gg_slice_bin_bytes / parse_rhorrp_gg_slice_bin and
+ * rhorrp_gg_diag_bin_bytes / parse_rhorrp_gg_diag_bin,
+ * crates/refeff-io/src/rhorrp_gg_bin.rs:98-260.  Values are complex float:
+ * slice [ne][row][col] (dims = ne, ldim, istate); diag [ne][atom][row][col]
+ * (dims = ne, inclus, ldim).  Parsing accepts either byte order; pass
+ * values = NULL to read dims only. */
+int refeff_io_gg_slice_bin_bytes(int32_t energy_count, int32_t row_count, int32_t column_count, const float* values,
+                                 char* out, size_t out_len, size_t* needed);
+int refeff_io_gg_slice_bin_parse(const char* bytes, size_t len, int32_t* dims, float* values);
+int refeff_io_gg_diag_bin_bytes(int32_t energy_count, int32_t atom_count, int32_t dim, const float* values, char* out,
+                                size_t out_len, size_t* needed);
+int refeff_io_gg_diag_bin_parse(const char* bytes, size_t len, int32_t* dims, float* values);
 
 /* Library version string (static storage). */
 const char* refeff_fms_version(void);

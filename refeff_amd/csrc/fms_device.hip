@@ -2236,6 +2236,7 @@ struct refeff_fms_plan {
     void* d_dinv = nullptr;  // inverse 32x32 diagonal blocks of L and U (float2 or double2)
     int *d_ipiv = nullptr, *d_perm = nullptr, *d_info = nullptr;
     int* d_emap = nullptr;              // energies still refining, per sub-batch
+    bool full = false;                  // every G0 column is a right-hand side (A^-1 G0)
     int* d_cand = nullptr;              // tournament-pivoting candidates: two [e][calu_cap] lists
     int calu_cap = 0;                   // candidate list length per energy
     bool panel_calu = true;             // REFEFF_FMS_PANEL=gepp: FP32 panels by partial pivoting
@@ -2946,12 +2947,15 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (d->global_lmax < 0 || d->global_lmax > kMaxL || d->max_potential < 0 || !d->raw_potential_lmax)
         return REFEFF_FMS_INVALID_ARGUMENT;
     if (!std::isfinite(d->direct_cutoff) || d->direct_cutoff < 0.f) return REFEFF_FMS_INVALID_ARGUMENT;
-    if (d->full_scattering_requested) return REFEFF_FMS_UNSUPPORTED;
+    const bool full = d->full_scattering_requested != 0;
     int solver = d->solver;
     // refeff fms_scattering_method_selection (fms/driver.rs:72-95): minv 0 -> LU,
-    // 1 -> ggbi; the other FEFF iterative branches are not provided
+    // 1 -> ggbi; the other FEFF iterative branches are not provided.  A full
+    // scattering matrix forces LU whatever minv says.
     if (solver == REFEFF_FMS_SOLVER_AUTO)
-        solver = (d->minv == 0) ? REFEFF_FMS_SOLVER_MIXED : (d->minv == 1 ? REFEFF_FMS_SOLVER_GGBI : -1);
+        solver = (d->minv == 0 || full) ? REFEFF_FMS_SOLVER_MIXED
+                                        : (d->minv == 1 ? REFEFF_FMS_SOLVER_GGBI : -1);
+    if (full && solver == REFEFF_FMS_SOLVER_GGBI) return REFEFF_FMS_UNSUPPORTED;
     if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64 && solver != REFEFF_FMS_SOLVER_GGBI)
         return REFEFF_FMS_UNSUPPORTED;
     if (solver == REFEFF_FMS_SOLVER_GGBI &&
@@ -3002,6 +3006,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->pstart = (d->lfms == 0) ? absorber : 0;
     p->pend = (d->lfms == 0) ? absorber : p->max_pot;
     p->rep = stt.representative;
+    p->full = full;
     p->col0.assign(p->npot, 0);
     p->ipart.assign(p->npot, 0);
     std::vector<int> atom_first(p->nat), atom_lmax(p->nat), atom_rhs(p->nat, -1);
@@ -3026,6 +3031,13 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         for (int a = 0; a < p->nat; ++a)
             if (atom_first[a] == p->rep[ip]) atom_rhs[a] = nrhs;
         nrhs += p->ipart[ip];
+    }
+    if (full) {
+        // X = A^-1 G0 over every column of G0 (solvers.rs:473-477): each
+        // atom's block is a right-hand side at its own state offset
+        for (int a = 0; a < p->nat; ++a) atom_rhs[a] = atom_first[a];
+        for (int ip = p->pstart; ip <= p->pend; ++ip) p->col0[ip] = p->rep[ip];
+        nrhs = p->n;
     }
     p->nrhs = nrhs;
     std::vector<int> rhs_src(std::max(1, nrhs)), rhs_pot(std::max(1, nrhs));
@@ -3125,7 +3137,7 @@ int refeff_fms_plan_query(const refeff_fms_plan* p, refeff_fms_plan_info* info) 
 }
 
 static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc, int32_t* status,
-                        refeff_fms_stats* st, bool device, hipStream_t ext) {
+                        refeff_fms_stats* st, bool device, hipStream_t ext, double* full_out = nullptr) {
     if (!p || ne < 0 || (ne > 0 && (!k || !ph || !sc))) return REFEFF_FMS_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(p->mu);
     const double t0 = now_s();
@@ -3153,6 +3165,21 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
             if (check_hip(hipMemcpy(sc + 2 * sst * e0, p->d_out, 2 * sst * nb * sizeof(double), hipMemcpyDeviceToHost)))
                 return REFEFF_FMS_DEVICE_ERROR;
         }
+        if (full_out) {
+            // X = A^-1 G0 (row-major, ldx = N) -> column-major N x N per energy
+            const size_t n = static_cast<size_t>(p->n), sx = static_cast<size_t>(p->npad) * p->nrhs;
+            std::vector<double2> xh(n * n);
+            for (int e = 0; e < nb; ++e) {
+                if (check_hip(hipMemcpy(xh.data(), p->d_x64 + sx * e, n * n * sizeof(double2), hipMemcpyDeviceToHost)))
+                    return REFEFF_FMS_DEVICE_ERROR;
+                double* dst = full_out + 2 * n * n * static_cast<size_t>(e0 + e);
+                for (size_t r = 0; r < n; ++r)
+                    for (size_t c = 0; c < n; ++c) {
+                        dst[2 * (r + n * c)] = xh[r * n + c].x;
+                        dst[2 * (r + n * c) + 1] = xh[r * n + c].y;
+                    }
+            }
+        }
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
     if (st) st->seconds_total = now_s() - t0;
@@ -3167,6 +3194,12 @@ int refeff_fms_solve_batch(refeff_fms_plan* p, int32_t ne, const double* k, cons
 int refeff_fms_solve_batch_device(refeff_fms_plan* p, int32_t ne, const double* dk, const double* dph, double* dsc,
                                   int32_t* status, refeff_fms_stats* st, void* stream) {
     return solve_common(p, ne, dk, dph, dsc, status, st, true, static_cast<hipStream_t>(stream));
+}
+
+int refeff_fms_solve_batch_full(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc,
+                                double* full_scattering, int32_t* status, refeff_fms_stats* st) {
+    if (!p || !p->full || (ne > 0 && !full_scattering)) return REFEFF_FMS_INVALID_ARGUMENT;
+    return solve_common(p, ne, k, ph, sc, status, st, false, nullptr, full_scattering);
 }
 
 // Kernel-level test hooks (not part of the public header): LU and GEMM of

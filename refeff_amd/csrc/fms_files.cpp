@@ -56,6 +56,18 @@ int emit(const std::string& text, char* out, size_t out_len, size_t* needed) {
     return REFEFF_FMS_OK;
 }
 
+// binary outputs: *needed = byte count; copied when out_len allows
+int emit_bytes(const std::string& bytes, char* out, size_t out_len, size_t* needed) {
+    if (needed) *needed = bytes.size();
+    if (!out) return REFEFF_FMS_OK;
+    if (out_len < bytes.size()) {
+        g_last_error = "output buffer too small";
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    }
+    std::memcpy(out, bytes.data(), bytes.size());
+    return REFEFF_FMS_OK;
+}
+
 std::vector<cplx> cvec(const double* p, size_t n) {
     std::vector<cplx> v(n);
     for (size_t i = 0; i < n; ++i) v[i] = cplx(p[2 * i], p[2 * i + 1]);
@@ -99,12 +111,14 @@ void validate_controls(const rio::FmsInp& in) {
     if (in.idwopt >= 0)
         stage_fail(REFEFF_FMS_UNSUPPORTED, "Debye-Waller model idwopt=" + std::to_string(in.idwopt) +
                                                " is not provided; only the global SIG2 (idwopt < 0)");
-    if (in.save_gg_slice) stage_fail(REFEFF_FMS_UNSUPPORTED, "save_gg_slice (gg_full = A^-1 G0) is not provided");
 }
 
 struct FmsStageResult {
     int energy_count = 0;
     int cluster_atoms = 0;
+    bool saved_scattering = false;  // save_gg_slice: slice and diag below hold the outputs
+    rio::GgSliceBin slice;
+    rio::GgDiagBin diag;
 };
 
 // gg sections from phase.bin / geom.dat / global.inp: refeff
@@ -196,7 +210,7 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
     d.mean_square_displacements = sig.data();
     d.convergence_tolerance = static_cast<float>(in.toler1);
     d.zero_tolerance = static_cast<float>(in.toler2);
-    d.full_scattering_requested = 0;
+    d.full_scattering_requested = in.save_gg_slice ? 1 : 0;
     d.solver = solver;
     d.max_batch = 0;
     refeff_fms_plan* plan = nullptr;
@@ -228,11 +242,57 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
     const int ch = info.channel_count;
     std::vector<double> sc(2 * static_cast<size_t>(ne) * npot * ch * ch, 0.0);
     std::vector<int32_t> status(ne, 0);
-    if (rc == REFEFF_FMS_OK) rc = refeff_fms_solve_batch(plan, ne, k.data(), ph.data(), sc.data(), status.data(), nullptr);
+    std::vector<double> full;
+    const size_t nst = static_cast<size_t>(info.state_count);
+    // save_gg_slice: fixed-width atom blocks of nsp (global_lmax+1)^2 states
+    // (engine fms.rs:4144-4173)
+    const int bd = nsp * (L + 1) * (L + 1);
+    if (in.save_gg_slice && rc == REFEFF_FMS_OK) {
+        if (nst != static_cast<size_t>(nat) * bd) {
+            refeff_fms_plan_destroy(plan);
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS save_gg_slice requires fixed-width atom blocks: got " +
+                                                        std::to_string(nst) + " state(s), expected " +
+                                                        std::to_string(static_cast<size_t>(nat) * bd) + " from " +
+                                                        std::to_string(nat) + " atom(s) and block dimension " +
+                                                        std::to_string(bd));
+        }
+        full.assign(2 * static_cast<size_t>(ne) * nst * nst, 0.0);
+        rc = refeff_fms_solve_batch_full(plan, ne, k.data(), ph.data(), sc.data(), full.data(), status.data(), nullptr);
+    } else if (rc == REFEFF_FMS_OK) {
+        rc = refeff_fms_solve_batch(plan, ne, k.data(), ph.data(), sc.data(), status.data(), nullptr);
+    }
     refeff_fms_plan_destroy(plan);
     for (int e = 0; e < ne; ++e)
         if (status[e]) stage_fail(status[e], "failed to solve FMS energy section " + std::to_string(e + 1));
     check_status(rc, "failed to solve the FMS energy sweep");
+    if (in.save_gg_slice) {
+        // build_saved_fms_scattering_outputs (engine fms.rs:4154-4216): the
+        // absorber rows of A^-1 G0 and every atom's diagonal block, as c32
+        res.saved_scattering = true;
+        res.slice.energy_count = ne;
+        res.slice.row_count = bd;
+        res.slice.column_count = static_cast<int>(nst);
+        res.slice.values.resize(static_cast<size_t>(ne) * bd * nst);
+        res.diag.energy_count = ne;
+        res.diag.atom_count = nat;
+        res.diag.dim = bd;
+        res.diag.values.resize(static_cast<size_t>(ne) * nat * bd * bd);
+        auto at = [&](int e, size_t r, size_t c) {  // full is column-major per energy
+            const size_t o = static_cast<size_t>(e) * nst * nst + r + nst * c;
+            return rio::cplxf(static_cast<float>(full[2 * o]), static_cast<float>(full[2 * o + 1]));
+        };
+        for (int e = 0; e < ne; ++e) {
+            for (int r = 0; r < bd; ++r)
+                for (size_t c = 0; c < nst; ++c)
+                    res.slice.values[(static_cast<size_t>(e) * bd + r) * nst + c] = at(e, r, c);
+            for (int a = 0; a < nat; ++a) {
+                const size_t s0 = static_cast<size_t>(a) * bd;
+                for (int r = 0; r < bd; ++r)
+                    for (int c = 0; c < bd; ++c)
+                        res.diag.values[((static_cast<size_t>(e) * nat + a) * bd + r) * bd + c] = at(e, s0 + r, s0 + c);
+            }
+        }
+    }
 
     const int alm = std::min(in.lmaxph[absorber], L);
     const int ang = (alm + 1) * (alm + 1), active = ang * nsp, outc = ang * nspu;
@@ -287,6 +347,22 @@ std::string fms_module_log(const rio::FmsInp& in, const FmsStageResult& r) {
     return out;
 }
 
+// gg_slice.bin / gg_diag.bin already in the directory are solver outputs too:
+// re-emitted canonically and counted (engine fms.rs:489-525).
+int reemit_saved_scattering(const std::string& dir) {
+    int n = 0;
+    const std::string sl = join(dir, "gg_slice.bin"), dg = join(dir, "gg_diag.bin");
+    if (rio::file_exists(sl)) {
+        rio::write_file(sl, rio::gg_slice_bin_bytes(rio::parse_gg_slice_bin(rio::read_file(sl))));
+        ++n;
+    }
+    if (rio::file_exists(dg)) {
+        rio::write_file(dg, rio::gg_diag_bin_bytes(rio::parse_gg_diag_bin(rio::read_file(dg))));
+        ++n;
+    }
+    return n;
+}
+
 int run_fms(const std::string& dir, int solver, int32_t* files_written) {
     if (files_written) *files_written = 0;
     const rio::FmsInp in = rio::parse_fms_inp(rio::read_file(join(dir, "fms.inp")));
@@ -305,6 +381,13 @@ int run_fms(const std::string& dir, int solver, int32_t* files_written) {
         for (const char* name : {"fms.bin", "gtr.dat", "fmsl.bin", "gtrl.dat"}) std::remove(join(dir, name).c_str());
         rio::write_file(join(dir, "log3.dat"), fms_module_log(in, r));
         written = 3;
+        if (r.saved_scattering) {
+            rio::write_file(join(dir, "gg_slice.bin"), rio::gg_slice_bin_bytes(r.slice));
+            rio::write_file(join(dir, "gg_diag.bin"), rio::gg_diag_bin_bytes(r.diag));
+            written += 2;
+        } else {
+            written += reemit_saved_scattering(dir);
+        }
     } else {
         // cached gg only: re-emit it canonically
         const std::string bin = join(dir, "gg.bin"), dat = join(dir, "gg.dat");
@@ -314,7 +397,7 @@ int run_fms(const std::string& dir, int solver, int32_t* files_written) {
                        "FMS Green's-function generation requires cached FMS output or phase.bin/geom.dat/global.inp source handoffs");
         if (have_bin) rio::write_file(bin, rio::gg_dat_string(rio::parse_gg_dat(rio::read_file(bin))));
         if (have_dat) rio::write_file(dat, rio::gg_dat_string(rio::parse_gg_dat(rio::read_file(dat))));
-        written = (have_bin ? 1 : 0) + (have_dat ? 1 : 0);
+        written = (have_bin ? 1 : 0) + (have_dat ? 1 : 0) + reemit_saved_scattering(dir);
         if (!rio::file_exists(join(dir, "log3.dat"))) {
             rio::write_file(join(dir, "log3.dat"),
                             "FMS calculation of full Green's function ...\nFEFF-serial using 1 thread.\nDone with module: FMS.\n");
@@ -774,6 +857,68 @@ int refeff_io_gtr_dat_string(int32_t count, const double* energy, const double* 
                              size_t* needed) {
     if (count < 0 || (count > 0 && (!energy || !trace))) return REFEFF_FMS_INVALID_ARGUMENT;
     return guarded([&] { return emit(rio::gtr_dat_string(cvec(energy, count), cvec(trace, count)), out, out_len, needed); });
+}
+
+int refeff_io_gg_slice_bin_bytes(int32_t energy_count, int32_t row_count, int32_t column_count, const float* values,
+                                 char* out, size_t out_len, size_t* needed) {
+    if (energy_count <= 0 || row_count <= 0 || column_count <= 0 || !values) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::GgSliceBin d;
+        d.energy_count = energy_count;
+        d.row_count = row_count;
+        d.column_count = column_count;
+        const size_t n = static_cast<size_t>(energy_count) * row_count * column_count;
+        d.values.resize(n);
+        for (size_t i = 0; i < n; ++i) d.values[i] = rio::cplxf(values[2 * i], values[2 * i + 1]);
+        return emit_bytes(rio::gg_slice_bin_bytes(d), out, out_len, needed);
+    });
+}
+
+int refeff_io_gg_slice_bin_parse(const char* bytes, size_t len, int32_t* dims, float* values) {
+    if (!bytes || !dims) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::GgSliceBin d = rio::parse_gg_slice_bin(std::string(bytes, len));
+        dims[0] = d.energy_count;
+        dims[1] = d.row_count;
+        dims[2] = d.column_count;
+        if (values)
+            for (size_t i = 0; i < d.values.size(); ++i) {
+                values[2 * i] = d.values[i].real();
+                values[2 * i + 1] = d.values[i].imag();
+            }
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_gg_diag_bin_bytes(int32_t energy_count, int32_t atom_count, int32_t dim, const float* values, char* out,
+                                size_t out_len, size_t* needed) {
+    if (energy_count <= 0 || atom_count <= 0 || dim <= 0 || !values) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::GgDiagBin d;
+        d.energy_count = energy_count;
+        d.atom_count = atom_count;
+        d.dim = dim;
+        const size_t n = static_cast<size_t>(energy_count) * atom_count * dim * dim;
+        d.values.resize(n);
+        for (size_t i = 0; i < n; ++i) d.values[i] = rio::cplxf(values[2 * i], values[2 * i + 1]);
+        return emit_bytes(rio::gg_diag_bin_bytes(d), out, out_len, needed);
+    });
+}
+
+int refeff_io_gg_diag_bin_parse(const char* bytes, size_t len, int32_t* dims, float* values) {
+    if (!bytes || !dims) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::GgDiagBin d = rio::parse_gg_diag_bin(std::string(bytes, len));
+        dims[0] = d.energy_count;
+        dims[1] = d.atom_count;
+        dims[2] = d.dim;
+        if (values)
+            for (size_t i = 0; i < d.values.size(); ++i) {
+                values[2 * i] = d.values[i].real();
+                values[2 * i + 1] = d.values[i].imag();
+            }
+        return REFEFF_FMS_OK;
+    });
 }
 
 int refeff_io_gtr_dat_parse(const char* text, int32_t* count, double* energy, double* trace) {

@@ -5,6 +5,8 @@
 
 #include <cerrno>
 #include <cmath>
+#include <climits>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -990,6 +992,175 @@ void parse_gtr_dat(const std::string& text, std::vector<cplx>& energy, std::vect
         trace.emplace_back(v[2], v[3]);
     }
     if (energy.empty()) fail("gtr.dat: at least one gtr row is required");
+}
+
+// ---- gg_slice.bin / gg_diag.bin (rhorrp_gg_bin.rs:98-260, 494-716) ----
+
+namespace {
+
+void put_u32(std::string& b, uint32_t v) {
+    for (int i = 0; i < 4; ++i) b.push_back(static_cast<char>((v >> (8 * i)) & 0xFF));
+}
+void put_f32(std::string& b, float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    put_u32(b, u);
+}
+void put_record(std::string& out, const std::string& payload) {
+    if (payload.size() > 0xFFFFFFFFull) invalid("record length does not fit in u32");
+    put_u32(out, static_cast<uint32_t>(payload.size()));
+    out += payload;
+    put_u32(out, static_cast<uint32_t>(payload.size()));
+}
+uint32_t get_u32(const std::string& b, size_t at, bool big) {
+    if (at + 4 > b.size()) invalid("record is truncated");
+    uint32_t v = 0;
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t byte = static_cast<unsigned char>(b[at + (big ? 3 - i : i)]);
+        v |= byte << (8 * i);
+    }
+    return v;
+}
+// first marker = header record length, little- or big-endian
+bool detect_big_endian(const std::string& b, uint32_t header_bytes) {
+    if (b.size() < 4) invalid("first record marker is missing");
+    if (get_u32(b, 0, false) == header_bytes) return false;
+    if (get_u32(b, 0, true) == header_bytes) return true;
+    invalid("first record marker does not match the expected header length");
+}
+std::string get_record(const std::string& b, size_t& pos, bool big, const char* label) {
+    const uint32_t len = get_u32(b, pos, big);
+    pos += 4;
+    if (pos + len > b.size()) invalid(std::string(label) + " record is truncated by its payload length");
+    std::string payload = b.substr(pos, len);
+    pos += len;
+    if (get_u32(b, pos, big) != len) invalid(std::string(label) + " record trailing marker does not match");
+    pos += 4;
+    return payload;
+}
+int positive_i32(uint32_t raw, const char* field) {
+    const int32_t v = static_cast<int32_t>(raw);
+    if (v <= 0) invalid(std::string(field) + " must be positive");
+    return v;
+}
+std::vector<cplxf> complex_payload(const std::string& p, bool big, size_t expected, const char* field) {
+    if (expected > p.size() / 8 || p.size() != expected * 8)
+        invalid(std::string(field) + " payload has the wrong byte count");
+    std::vector<cplxf> v(expected);
+    for (size_t i = 0; i < expected; ++i) {
+        const uint32_t re = get_u32(p, 8 * i, big), im = get_u32(p, 8 * i + 4, big);
+        float fr, fi;
+        std::memcpy(&fr, &re, 4);
+        std::memcpy(&fi, &im, 4);
+        v[i] = cplxf(fr, fi);
+    }
+    return v;
+}
+void check_finite(const std::vector<cplxf>& v, const char* field) {
+    for (const cplxf& c : v)
+        if (!std::isfinite(c.real()) || !std::isfinite(c.imag()))
+            invalid(std::string(field) + " values must be finite");
+}
+size_t checked_mul(size_t a, size_t b) {
+    if (a != 0 && b > SIZE_MAX / a) invalid("dimension product overflows");
+    return a * b;
+}
+
+}  // namespace
+
+std::string gg_slice_bin_bytes(const GgSliceBin& d) {
+    if (d.energy_count <= 0 || d.row_count <= 0 || d.column_count <= 0)
+        invalid("gg_slice dimensions must be positive");
+    const size_t n = checked_mul(checked_mul(d.energy_count, d.row_count), d.column_count);
+    if (d.values.size() != n) invalid("gg_slice value count does not match its dimensions");
+    check_finite(d.values, "gg_slice");
+    std::string out, head, pay;
+    put_u32(head, static_cast<uint32_t>(d.row_count));
+    put_u32(head, static_cast<uint32_t>(d.column_count));
+    put_u32(head, static_cast<uint32_t>(d.energy_count));
+    put_record(out, head);
+    pay.reserve(8 * n);
+    for (int e = 0; e < d.energy_count; ++e)
+        for (int c = 0; c < d.column_count; ++c)
+            for (int r = 0; r < d.row_count; ++r) {
+                const cplxf v = d.values[(static_cast<size_t>(e) * d.row_count + r) * d.column_count + c];
+                put_f32(pay, v.real());
+                put_f32(pay, v.imag());
+            }
+    put_record(out, pay);
+    return out;
+}
+
+GgSliceBin parse_gg_slice_bin(const std::string& b) {
+    const bool big = detect_big_endian(b, 12);
+    size_t pos = 0;
+    const std::string head = get_record(b, pos, big, "gg_slice header");
+    if (head.size() != 12) invalid("gg_slice header record must hold 12 bytes");
+    GgSliceBin d;
+    d.row_count = positive_i32(get_u32(head, 0, big), "ldim");
+    d.column_count = positive_i32(get_u32(head, 4, big), "istate");
+    d.energy_count = positive_i32(get_u32(head, 8, big), "ne");
+    const size_t n = checked_mul(checked_mul(d.row_count, d.column_count), d.energy_count);
+    const std::vector<cplxf> raw = complex_payload(get_record(b, pos, big, "gg_slice payload"), big, n, "gg_slice");
+    if (pos != b.size()) invalid("gg_slice.bin has trailing bytes");
+    d.values.resize(n);
+    size_t i = 0;
+    for (int e = 0; e < d.energy_count; ++e)
+        for (int c = 0; c < d.column_count; ++c)
+            for (int r = 0; r < d.row_count; ++r)
+                d.values[(static_cast<size_t>(e) * d.row_count + r) * d.column_count + c] = raw[i++];
+    check_finite(d.values, "gg_slice");
+    return d;
+}
+
+std::string gg_diag_bin_bytes(const GgDiagBin& d) {
+    if (d.energy_count <= 0 || d.atom_count <= 0 || d.dim <= 0) invalid("gg_diag dimensions must be positive");
+    const size_t n = checked_mul(checked_mul(checked_mul(d.energy_count, d.atom_count), d.dim), d.dim);
+    if (d.values.size() != n) invalid("gg_diag value count does not match its dimensions");
+    check_finite(d.values, "gg_diag");
+    std::string out, head, pay;
+    put_u32(head, static_cast<uint32_t>(d.dim));
+    put_u32(head, static_cast<uint32_t>(d.dim));
+    put_u32(head, static_cast<uint32_t>(d.atom_count));
+    put_u32(head, static_cast<uint32_t>(d.energy_count));
+    put_record(out, head);
+    pay.reserve(8 * n);
+    for (int e = 0; e < d.energy_count; ++e)
+        for (int a = 0; a < d.atom_count; ++a)
+            for (int c = 0; c < d.dim; ++c)
+                for (int r = 0; r < d.dim; ++r) {
+                    const cplxf v = d.values[((static_cast<size_t>(e) * d.atom_count + a) * d.dim + r) * d.dim + c];
+                    put_f32(pay, v.real());
+                    put_f32(pay, v.imag());
+                }
+    put_record(out, pay);
+    return out;
+}
+
+GgDiagBin parse_gg_diag_bin(const std::string& b) {
+    const bool big = detect_big_endian(b, 16);
+    size_t pos = 0;
+    const std::string head = get_record(b, pos, big, "gg_diag header");
+    if (head.size() != 16) invalid("gg_diag header record must hold 16 bytes");
+    GgDiagBin d;
+    const int rows = positive_i32(get_u32(head, 0, big), "ldim");
+    const int cols = positive_i32(get_u32(head, 4, big), "ldim");
+    d.atom_count = positive_i32(get_u32(head, 8, big), "inclus");
+    d.energy_count = positive_i32(get_u32(head, 12, big), "ne");
+    if (rows != cols) invalid("gg_diag ldim values differ");
+    d.dim = rows;
+    const size_t n = checked_mul(checked_mul(checked_mul(rows, cols), d.atom_count), d.energy_count);
+    const std::vector<cplxf> raw = complex_payload(get_record(b, pos, big, "gg_diag payload"), big, n, "gg_diag");
+    if (pos != b.size()) invalid("gg_diag.bin has trailing bytes");
+    d.values.resize(n);
+    size_t i = 0;
+    for (int e = 0; e < d.energy_count; ++e)
+        for (int a = 0; a < d.atom_count; ++a)
+            for (int c = 0; c < d.dim; ++c)
+                for (int r = 0; r < d.dim; ++r)
+                    d.values[((static_cast<size_t>(e) * d.atom_count + a) * d.dim + r) * d.dim + c] = raw[i++];
+    check_finite(d.values, "gg_diag");
+    return d;
 }
 
 std::string read_file(const std::string& path) {

@@ -125,6 +125,25 @@ FmsBin parse_fms_bin(const std::string& text);
 std::string gtr_dat_string(const std::vector<cplx>& energy, const std::vector<cplx>& trace);
 void parse_gtr_dat(const std::string& text, std::vector<cplx>& energy, std::vector<cplx>& trace);
 
+// gg_slice.bin / gg_diag.bin, crates/refeff-io/src/rhorrp_gg_bin.rs: two
+// sequential Fortran-unformatted records (32-bit length markers around each)
+// -- a header of 32-bit integer dimensions, then complex64 values in
+// column-major order per energy.  Written little-endian; parsed in either byte
+// order (the first marker tells which).
+using cplxf = std::complex<float>;
+struct GgSliceBin {
+    int energy_count = 0, row_count = 0, column_count = 0;  // ne, ldim, istate
+    std::vector<cplxf> values;                               // [e][row][col]
+};
+struct GgDiagBin {
+    int energy_count = 0, atom_count = 0, dim = 0;  // ne, inclus, ldim
+    std::vector<cplxf> values;                      // [e][atom][row][col]
+};
+std::string gg_slice_bin_bytes(const GgSliceBin& d);
+GgSliceBin parse_gg_slice_bin(const std::string& bytes);
+std::string gg_diag_bin_bytes(const GgDiagBin& d);
+GgDiagBin parse_gg_diag_bin(const std::string& bytes);
+
 // str::lines(): split on '\n', drop one trailing '\r', no final empty line.
 std::vector<std::string> split_lines(const std::string& text);
 std::string read_file(const std::string& path);

@@ -173,6 +173,28 @@ class FmsRealSpacePlan:
             check(rc, "fms_real_space_spectrum")
         return out.transpose(0, 1, 3, 2).copy(), status
 
+    def solve_full(self, wave_numbers, phase_shifts):
+        """As solve, plus full_scattering = A^-1 G0 [ne, N, N] (row, col) per energy
+        (FmsLuResult.full_scattering, solvers.rs:473-477).  Needs a plan created with
+        full_scattering_matrix_requested=True."""
+        nsp, L = self.spin_channels, self.global_lmax
+        k = np.ascontiguousarray(np.asarray(wave_numbers, np.complex128).reshape(-1, nsp))
+        ne = k.shape[0]
+        ph = np.ascontiguousarray(np.asarray(phase_shifts, np.complex128).reshape(ne, nsp, 2 * L + 1,
+                                                                                   self.potential_count))
+        ch, npot, n = self.channel_count, self.potential_count, self.info.state_count
+        out = np.zeros((ne, npot, ch, ch), np.complex128)
+        full = np.zeros((ne, n, n), np.complex128)          # C layout: [e][col][row]
+        status = np.zeros(ne, np.int32)
+        st = Stats()
+        rc = lib().refeff_fms_solve_batch_full(self._h, ne, _f64p(k.view(np.float64)), _f64p(ph.view(np.float64)),
+                                               _f64p(out.view(np.float64)), _f64p(full.view(np.float64)),
+                                               _i32p(status), C.byref(st))
+        self.last_stats = st
+        if rc not in (0,) and not np.any(status):
+            check(rc, "fms_real_space_spectrum (full scattering)")
+        return out.transpose(0, 1, 3, 2).copy(), status, full.transpose(0, 2, 1).copy()
+
 
 def fms_real_space_plan(**kwargs) -> FmsRealSpacePlan:
     positions = kwargs.pop("atoms_positions")

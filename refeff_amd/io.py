@@ -79,6 +79,10 @@ def _L():
     L.refeff_io_fms_bin_parse.argtypes = [cp, f64p, i32p, i32p, i32p, vp]
     L.refeff_io_gtr_dat_string.argtypes = [i32, vp, vp, C.c_char_p, sz, szp]
     L.refeff_io_gtr_dat_parse.argtypes = [cp, i32p, vp, vp]
+    L.refeff_io_gg_slice_bin_bytes.argtypes = [i32, i32, i32, vp, C.c_char_p, sz, szp]
+    L.refeff_io_gg_slice_bin_parse.argtypes = [C.c_char_p, sz, i32p, vp]
+    L.refeff_io_gg_diag_bin_bytes.argtypes = [i32, i32, i32, vp, C.c_char_p, sz, szp]
+    L.refeff_io_gg_diag_bin_parse.argtypes = [C.c_char_p, sz, i32p, vp]
     _bound = True
     return L
 
@@ -404,6 +408,55 @@ def parse_gtr_dat(text: str):
     e, tr = np.zeros(n.value, np.complex128), np.zeros(n.value, np.complex128)
     _check(L.refeff_io_gtr_dat_parse(t, C.byref(n), _ptr(e), _ptr(tr)), "gtr.dat")
     return e, tr
+
+
+# --- gg_slice.bin / gg_diag.bin (rhorrp_gg_bin.rs) --------------------------
+
+def _bytes(call, where) -> bytes:
+    """Two-call binary output: size query, then fill."""
+    need = C.c_size_t(0)
+    _check(call(None, 0, C.byref(need)), where)
+    buf = C.create_string_buffer(max(1, need.value))
+    _check(call(buf, need.value, C.byref(need)), where)
+    return buf.raw[:need.value]
+
+
+def gg_slice_bin_bytes(values) -> bytes:
+    """rhorrp_gg_slice_bin_bytes: values [ne, ldim, istate] complex64 (energy, row, column)."""
+    v = np.ascontiguousarray(values, np.complex64)
+    if v.ndim != 3:
+        raise ValueError("gg_slice values must be [ne, ldim, istate]")
+    ne, r, c = v.shape
+    return _bytes(lambda b, n, need: _L().refeff_io_gg_slice_bin_bytes(ne, r, c, _ptr(v), b, n, need),
+                  "gg_slice_bin_bytes")
+
+
+def parse_gg_slice_bin(data: bytes) -> np.ndarray:
+    """parse_rhorrp_gg_slice_bin: -> [ne, ldim, istate] complex64."""
+    L, dims = _L(), (C.c_int32 * 3)()
+    _check(L.refeff_io_gg_slice_bin_parse(data, len(data), dims, None), "gg_slice.bin")
+    out = np.zeros((dims[0], dims[1], dims[2]), np.complex64)
+    _check(L.refeff_io_gg_slice_bin_parse(data, len(data), dims, _ptr(out)), "gg_slice.bin")
+    return out
+
+
+def gg_diag_bin_bytes(values) -> bytes:
+    """rhorrp_gg_diag_bin_bytes: values [ne, inclus, ldim, ldim] complex64."""
+    v = np.ascontiguousarray(values, np.complex64)
+    if v.ndim != 4 or v.shape[2] != v.shape[3]:
+        raise ValueError("gg_diag values must be [ne, inclus, ldim, ldim]")
+    ne, na, d, _ = v.shape
+    return _bytes(lambda b, n, need: _L().refeff_io_gg_diag_bin_bytes(ne, na, d, _ptr(v), b, n, need),
+                  "gg_diag_bin_bytes")
+
+
+def parse_gg_diag_bin(data: bytes) -> np.ndarray:
+    """parse_rhorrp_gg_diag_bin: -> [ne, inclus, ldim, ldim] complex64."""
+    L, dims = _L(), (C.c_int32 * 3)()
+    _check(L.refeff_io_gg_diag_bin_parse(data, len(data), dims, None), "gg_diag.bin")
+    out = np.zeros((dims[0], dims[1], dims[2], dims[2]), np.complex64)
+    _check(L.refeff_io_gg_diag_bin_parse(data, len(data), dims, _ptr(out)), "gg_diag.bin")
+    return out
 
 
 # --- module stages ---------------------------------------------------------

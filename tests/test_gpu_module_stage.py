@@ -48,7 +48,7 @@ nq,    imdff,   qaverage,   mixdff
 """
 
 
-def make_workdir(d, radius=4.6, ne=7, sig2g=0.003):
+def make_workdir(d, radius=4.6, ne=7, sig2g=0.003, save_gg_slice=False, lmaxph=(3, 3)):
     pts = W.lattice(W.FCC, 3.615, radius + 1.0)  # geom.dat holds more atoms than the FMS sphere
     order = np.argsort(np.linalg.norm(pts, axis=1), kind="stable")
     pts = pts[order]
@@ -67,7 +67,8 @@ def make_workdir(d, radius=4.6, ne=7, sig2g=0.003):
     pb = io.PhaseBinData(1, ne, ne, 0, 1, 1, 8, 8, 8, 1, (2.6, -0.14, -0.14), em, eref, pots, rkk)
     (d / "phase.bin").write_text(io.phase_bin_string(pb))
     fin = io.FmsInput(mfms=1, idwopt=-1, minv=0, rfms2=radius, rdirec=10.0, toler1=0.001, toler2=0.001, tk=0.0,
-                      thetad=0.0, sig2g=sig2g, lmaxph=[3, 3], decomposition_channels=-1, save_gg_slice=False,
+                      thetad=0.0, sig2g=sig2g, lmaxph=list(lmaxph), decomposition_channels=-1,
+                      save_gg_slice=save_gg_slice,
                       do_fms=1)
     (d / "fms.inp").write_text(io.fms_input_string(fin))
     (d / "global.inp").write_text(GLOBAL_INP)
@@ -129,3 +130,46 @@ def test_module_mkgtr_writes_trace_matching_oracle(tmp_path):
     assert (tmp_path / "log3.dat").read_text().endswith("MKGTR: Tracing over Green's function ...\n"
                                                          "Done with module: MKGTR.\n")
     banner("fms.bin / gtr.dat match oracle trace")
+
+
+@pytest.mark.parametrize("solver", [0, 2])
+def test_module_fms_save_gg_slice_writes_slice_and_diag(tmp_path, solver):
+    """save_gg_slice: gg_slice.bin holds the absorber rows of A^-1 G0 and gg_diag.bin
+    every atom's diagonal block (engine fms.rs:3659-3683, 4154-4216), as complex64."""
+    banner(f"module fms save_gg_slice, solver {solver}")
+    pts, iph, k, ph, rkk, em, radius, sig2g = make_workdir(tmp_path, save_gg_slice=True)
+    assert io.run_fms_in_dir(tmp_path, solver) == 5  # gg.bin, gg.dat, gg_slice.bin, gg_diag.bin, log3.dat
+    ref, nat = oracle_gg(pts, iph, k, ph, radius, sig2g)
+    sections = io.parse_gg_bin((tmp_path / "gg.bin").read_text())
+    for e, s in enumerate(sections):
+        assert np.linalg.norm(s.values - ref[e]) / np.linalg.norm(ref[e]) <= 1e-8
+    sl = io.parse_gg_slice_bin((tmp_path / "gg_slice.bin").read_bytes())
+    dg = io.parse_gg_diag_bin((tmp_path / "gg_diag.bin").read_bytes())
+    n = 16 * nat
+    assert sl.shape == (len(k), 16, n) and dg.shape == (len(k), nat, 16, 16)
+    # the oracle's full A^-1 G0 on the same prepared cluster
+    pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(radius), len(iph))
+    pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
+    sig = np.full((nat, nat), np.float32(sig2g), np.float32)
+    np.fill_diagonal(sig, 0.0)
+    oplan = orc.Plan(pos, pot, nsp=1, lfms=1, max_pot=1, global_lmax=3, raw_lmax=[3, 3], rdirec=10.0, sigsqr=sig)
+    _, l, m, s_, _ = oplan.states()
+    for e in range(len(k)):
+        g = oplan.g0(k[e:e + 1])
+        t = oplan.t_table(ph[e])
+        full = orc.zgesv(orc.lu_system_matrix(1, l, m, s_, g, t), g)
+        scale = np.abs(full).max()
+        assert np.abs(sl[e] - full[:16, :]).max() <= 1e-6 * scale, e  # complex64 on disk
+        for a in range(nat):
+            blk = full[16 * a:16 * a + 16, 16 * a:16 * a + 16]
+            assert np.abs(dg[e, a] - blk).max() <= 1e-6 * scale, (e, a)
+    banner(f"gg_slice.bin / gg_diag.bin match the oracle over {len(k)} energies, {nat} atoms")
+
+
+def test_module_fms_save_gg_slice_needs_fixed_width_atom_blocks(tmp_path):
+    """engine fms.rs:4166-4173: uneven lmax per potential cannot be saved."""
+    banner("module fms save_gg_slice with uneven blocks")
+    make_workdir(tmp_path, save_gg_slice=True, lmaxph=(3, 2))
+    with pytest.raises(io.CodecError) as e:
+        io.run_fms_in_dir(tmp_path, 0)
+    assert "fixed-width atom blocks" in str(e.value)

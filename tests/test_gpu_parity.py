@@ -257,3 +257,63 @@ def test_ggbi_cu_fcc_r52_with_cutoff_matches_oracle():
     worst = ggbi_compare(wl.positions, wl.potentials, wl.wave_numbers()[sel], wl.phase_shifts()[sel],
                          raw=wl.raw_lmax, max_pot=1, toler2=0.02)
     banner(f"worst rel err {worst:.2e}")
+
+
+def oracle_full_scattering(oplan, k_e, ph_e):
+    """A^-1 G0 of one energy from the oracle (fms_lu_scattering's full_scattering,
+    solvers.rs:473-477; pinned by test_oracle_kats.test_gg_full_kat)."""
+    g = oplan.g0(k_e)
+    t = oplan.t_table(ph_e)
+    _, l, m, s, _ = oplan.states()
+    a = orc.lu_system_matrix(oplan.nsp, l, m, s, g, t)
+    return orc.zgesv(a, g)
+
+
+@pytest.mark.parametrize("nsp,solver", [(1, fms.SOLVER_MIXED), (1, fms.SOLVER_FP64), (2, fms.SOLVER_MIXED)])
+def test_full_scattering_matches_oracle(nsp, solver):
+    """save_gg_slice's A^-1 G0 for every G0 column (N right-hand sides), against the
+    oracle's dense solve; the gg blocks are those of the plain sweep."""
+    banner(f"full scattering nsp={nsp} solver={solver}")
+    pos, pot = random_cluster(5, 10)
+    L = 3 if nsp == 1 else 2
+    kw = dict(global_lmax=L, max_potential=2, direct_cutoff=9.0, spin_channels=nsp, lfms=1)
+    k1 = np.array([1.1 + 0.05j, 1.9 + 0.08j, 2.6 + 0.03j])
+    k = np.stack([k1, k1 * 0.98], axis=1)[:, :nsp]
+    ph = phases_for(k1, L, np.array([L, L, L]), nsp=nsp, seed=4, spin_orbit=nsp == 2)
+    plan = fms.FmsRealSpacePlan(pos, pot, solver=solver, full_scattering_matrix_requested=True, **kw)
+    sc, st, full = plan.solve_full(k, ph)
+    assert not st.any(), st
+    n = plan.info.state_count
+    assert full.shape == (len(k1), n, n) and plan.info.rhs_count == n
+    plan.close()
+    oplan = orc.Plan(pos, pot, nsp=nsp, lfms=1, max_pot=2, global_lmax=L, rdirec=9.0)
+    for e in range(len(k1)):
+        ref = oracle_full_scattering(oplan, k[e], ph[e])
+        err = rel_err(full[e], ref)
+        assert err <= GG_TOL, (e, err)
+    plain = fms.FmsRealSpacePlan(pos, pot, solver=solver, **kw)
+    sc2, st2 = plain.solve(k, ph)
+    plain.close()
+    assert not st2.any()
+    for e in range(len(k1)):
+        assert rel_err(sc[e], sc2[e]) <= 1e-10
+    banner(f"A^-1 G0 over N={n} columns matches the oracle")
+
+
+def test_full_scattering_requires_a_full_plan_and_forces_lu():
+    """minv=1 with a full scattering request runs LU (driver.rs:72-95); a plan
+    without the request refuses the full entry point."""
+    banner("full scattering plan checks")
+    pos, pot = random_cluster(6, 7)
+    kw = dict(global_lmax=2, max_potential=2, direct_cutoff=9.0)
+    plan = fms.FmsRealSpacePlan(pos, pot, minv=1, full_scattering_matrix_requested=True, **kw)
+    assert plan.info.solver == fms.SOLVER_MIXED
+    plan.close()
+    plain = fms.FmsRealSpacePlan(pos, pot, **kw)
+    k = np.array([[1.3 + 0.05j]])
+    ph = phases_for(k[:, 0], 2, np.array([2, 2, 2]))
+    with pytest.raises(fms.FmsError):
+        plain.solve_full(k, ph)
+    plain.close()
+    with pytest.raises(fms.FmsError):
+        fms.FmsRealSpacePlan(pos, pot, minv=1, solver=fms.SOLVER_GGBI, full_scattering_matrix_requested=True, **kw)

@@ -510,3 +510,102 @@ def test_module_stage_rejects_unprovided_debye_models_before_solving(tmp_path):
     with pytest.raises(CodecError) as e:
         io.run_fms_in_dir(tmp_path)
     assert e.value.name == "UNSUPPORTED" and "idwopt=0" in str(e.value)
+
+
+# --- gg_slice.bin / gg_diag.bin: crates/refeff-io/src/rhorrp_gg_bin.rs tests --
+
+# FEFF reference bytes of rhorrp_gg_bin.rs:785-809 (FEFF_GG_SLICE_BYTES, FEFF_GG_DIAG_BYTES)
+FEFF_GG_SLICE = bytes([
+    0x0c, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00, 0x00, 0x03, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00,
+    0x00, 0x0c, 0x00, 0x00, 0x00, 0x60, 0x00, 0x00, 0x00, 0x9a, 0x99, 0x31, 0x41, 0x9a, 0x99,
+    0xa9, 0xc1, 0x9a, 0x99, 0x41, 0x41, 0x9a, 0x99, 0xb1, 0xc1, 0x33, 0x33, 0x33, 0x41, 0x33,
+    0x33, 0xab, 0xc1, 0x33, 0x33, 0x43, 0x41, 0x33, 0x33, 0xb3, 0xc1, 0xcd, 0xcc, 0x34, 0x41,
+    0xcd, 0xcc, 0xac, 0xc1, 0xcd, 0xcc, 0x44, 0x41, 0xcd, 0xcc, 0xb4, 0xc1, 0xcd, 0xcc, 0xa8,
+    0x41, 0xcd, 0xcc, 0x24, 0xc2, 0xcd, 0xcc, 0xb0, 0x41, 0xcd, 0xcc, 0x28, 0xc2, 0x9a, 0x99,
+    0xa9, 0x41, 0x9a, 0x99, 0x25, 0xc2, 0x9a, 0x99, 0xb1, 0x41, 0x9a, 0x99, 0x29, 0xc2, 0x66,
+    0x66, 0xaa, 0x41, 0x66, 0x66, 0x26, 0xc2, 0x66, 0x66, 0xb2, 0x41, 0x66, 0x66, 0x2a, 0xc2,
+    0x60, 0x00, 0x00, 0x00,
+])
+FEFF_GG_DIAG = bytes([
+    0x10, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00,
+    0x00, 0x02, 0x00, 0x00, 0x00, 0x10, 0x00, 0x00, 0x00, 0x80, 0x00, 0x00, 0x00, 0x33, 0x33,
+    0xde, 0x42, 0x33, 0x33, 0x53, 0xc3, 0x33, 0x33, 0xe0, 0x42, 0x33, 0x33, 0x54, 0xc3, 0x66,
+    0x66, 0xde, 0x42, 0x66, 0x66, 0x53, 0xc3, 0x66, 0x66, 0xe0, 0x42, 0x66, 0x66, 0x54, 0xc3,
+    0x33, 0x33, 0xf2, 0x42, 0x33, 0x33, 0x5d, 0xc3, 0x33, 0x33, 0xf4, 0x42, 0x33, 0x33, 0x5e,
+    0xc3, 0x66, 0x66, 0xf2, 0x42, 0x66, 0x66, 0x5d, 0xc3, 0x66, 0x66, 0xf4, 0x42, 0x66, 0x66,
+    0x5e, 0xc3, 0x9a, 0x19, 0x53, 0x43, 0x9a, 0x99, 0xcd, 0xc3, 0x9a, 0x19, 0x54, 0x43, 0x9a,
+    0x19, 0xce, 0xc3, 0x33, 0x33, 0x53, 0x43, 0x33, 0xb3, 0xcd, 0xc3, 0x33, 0x33, 0x54, 0x43,
+    0x33, 0x33, 0xce, 0xc3, 0x9a, 0x19, 0x5d, 0x43, 0x9a, 0x99, 0xd2, 0xc3, 0x9a, 0x19, 0x5e,
+    0x43, 0x9a, 0x19, 0xd3, 0xc3, 0x33, 0x33, 0x5d, 0x43, 0x33, 0xb3, 0xd2, 0xc3, 0x33, 0x33,
+    0x5e, 0x43, 0x33, 0x33, 0xd3, 0xc3, 0x80, 0x00, 0x00, 0x00,
+])
+
+
+def test_parses_feff_gg_slice_reference_bytes():
+    # rhorrp_gg_bin.rs:811-823
+    v = io.parse_gg_slice_bin(FEFF_GG_SLICE)
+    assert v.shape == (2, 2, 3)
+    assert v[0, 0, 0] == np.complex64(11.1 - 21.2j)
+    assert v[0, 1, 2] == np.complex64(12.3 - 22.6j)
+    assert v[1, 0, 0] == np.complex64(21.1 - 41.2j)
+    assert v[1, 1, 2] == np.complex64(22.3 - 42.6j)
+    assert io.gg_slice_bin_bytes(v) == FEFF_GG_SLICE
+
+
+def test_parses_feff_gg_diag_reference_bytes():
+    # rhorrp_gg_bin.rs:825-838
+    v = io.parse_gg_diag_bin(FEFF_GG_DIAG)
+    assert v.shape == (2, 2, 2, 2)
+    assert v[0, 0, 0, 0] == np.complex64(111.1 - 211.2j)
+    assert v[0, 1, 1, 1] == np.complex64(122.2 - 222.4j)
+    assert v[1, 0, 0, 1] == np.complex64(211.2 - 411.4j)
+    assert v[1, 1, 1, 1] == np.complex64(222.2 - 422.4j)
+    assert io.gg_diag_bin_bytes(v) == FEFF_GG_DIAG
+
+
+def test_roundtrips_gg_slice_and_diag_bytes():
+    # rhorrp_gg_bin.rs:840-861: the payload is column-major per energy (and atom)
+    e, r, c = np.meshgrid(np.arange(2), np.arange(2), np.arange(3), indexing="ij")
+    val = (0.1 * e + 0.01 * r + c).astype(np.float32)
+    slice_ = (val - 1j * val).astype(np.complex64)
+    assert np.array_equal(io.parse_gg_slice_bin(io.gg_slice_bin_bytes(slice_)), slice_)
+    e, a, r, c = np.meshgrid(np.arange(2), np.arange(2), np.arange(2), np.arange(2), indexing="ij")
+    val = (e + 0.1 * a + 0.01 * r + 0.001 * c).astype(np.float32)
+    diag = (val - 1j * val).astype(np.complex64)
+    b = io.gg_diag_bin_bytes(diag)
+    assert np.array_equal(io.parse_gg_diag_bin(b), diag)
+    # second value of the payload is row 1 of column 0 (Fortran order)
+    assert np.frombuffer(b[28:44], "<f4")[2] == diag[0, 0, 1, 0].real  # after 4+16+4+4 header bytes
+
+
+def test_gg_slice_and_diag_accept_big_endian_and_reject_bad_data():
+    # rhorrp_gg_bin.rs:562-575 (either byte order) and :1012-1043 (rejections)
+    words = np.frombuffer(FEFF_GG_SLICE, "<u4")
+    assert np.array_equal(io.parse_gg_slice_bin(words.astype(">u4").tobytes()), io.parse_gg_slice_bin(FEFF_GG_SLICE))
+    with pytest.raises(CodecError):
+        io.parse_gg_slice_bin(b"")
+    bad = bytearray(FEFF_GG_SLICE)
+    bad[0] = 0
+    with pytest.raises(CodecError):
+        io.parse_gg_slice_bin(bytes(bad))
+    with pytest.raises(CodecError):
+        io.parse_gg_slice_bin(FEFF_GG_SLICE + b"\x00")
+    with pytest.raises(CodecError):
+        io.gg_slice_bin_bytes(np.full((1, 1, 1), np.nan, np.complex64))
+    with pytest.raises(ValueError):
+        io.gg_diag_bin_bytes(np.zeros((1, 1, 1, 2), np.complex64))
+    bad = bytearray(FEFF_GG_DIAG)
+    bad[8] = 3  # header ldim values differ
+    with pytest.raises(CodecError):
+        io.parse_gg_diag_bin(bytes(bad))
+
+
+def test_module_stage_re_emits_cached_slice_and_diag(tmp_path):
+    # engine fms.rs:504-523: cached gg_slice.bin / gg_diag.bin are solver outputs too
+    _write(tmp_path, "fms.inp", LEGACY_FMS_INP)
+    _write(tmp_path, "gg.bin", GG_DAT)
+    (tmp_path / "gg_slice.bin").write_bytes(FEFF_GG_SLICE)
+    (tmp_path / "gg_diag.bin").write_bytes(FEFF_GG_DIAG)
+    assert io.run_fms_in_dir(tmp_path) == 4  # gg.bin, gg_slice.bin, gg_diag.bin, log3.dat
+    assert (tmp_path / "gg_slice.bin").read_bytes() == FEFF_GG_SLICE
+    assert (tmp_path / "gg_diag.bin").read_bytes() == FEFF_GG_DIAG
