@@ -57,10 +57,15 @@ typedef enum refeff_fms_status {
  * REFEFF_FMS_SOLVER_GGBI, which reproduces refeff's one-shot minv=1 branch
  * (crates/refeff-core/src/fms/internals/iterative.rs:181-254). */
 typedef enum refeff_fms_solver {
-    REFEFF_FMS_SOLVER_AUTO = 0,    /* minv=0 -> MIXED, minv=1 -> GGBI, other minv -> UNSUPPORTED */
+    REFEFF_FMS_SOLVER_AUTO = 0,    /* fms_scattering_method_selection (fms/driver.rs:72-95): minv 0 -> MIXED,
+                                      1 -> GGBI, 2 -> GGRM, 3 -> GGGM, any other -> GGTF; a full scattering
+                                      request -> MIXED */
     REFEFF_FMS_SOLVER_MIXED = 1,   /* complex FP32 MFMA LU + FP64 iterative refinement */
     REFEFF_FMS_SOLVER_FP64 = 2,    /* plain complex FP64 MFMA LU */
-    REFEFF_FMS_SOLVER_GGBI = 3     /* refeff minv=1 one-shot BiCGStab, in FP64 (refeff runs it in c32) */
+    REFEFF_FMS_SOLVER_GGBI = 3,    /* refeff minv=1 one-shot BiCGStab, in FP64 (refeff runs it in c32) */
+    REFEFF_FMS_SOLVER_GGRM = 4,    /* minv=2 recursion (fms_recursion_scattering), in FP64 */
+    REFEFF_FMS_SOLVER_GGGM = 5,    /* minv=3 Graves-Morris/Salam (fms_graves_morris_scattering), in FP64 */
+    REFEFF_FMS_SOLVER_GGTF = 6     /* other minv: TFQMR (fms_tfqmr_scattering), in FP64 */
 } refeff_fms_solver;
 
 /* ---------------------------------------------------------------------- */

@@ -43,6 +43,7 @@ enum {
     ORC_CENTRAL_MISMATCH = 10,
     ORC_UNSUPPORTED = 11,
     ORC_NO_MEMORY = 12,
+    ORC_NO_CONVERGENCE = 13,
 };
 
 #define ORC_LX 24 /* refeff FMS_ROTATION_LMAX */
@@ -835,25 +836,294 @@ done:
     return rc;
 }
 
-/* M = 1 - T G0 with toler2 cutoff: refeff fms/solvers.rs:22-78 */
-static void iterative_system(const orc_plan* p, const orc_states* st, const cplx* g0, const cplx* tmat, cplx* m) {
+/* factor * T G0 + diagonal * 1 with the toler2 cutoff per G0 element:
+ * refeff fms_compact_tg_work_matrix, fms/solvers.rs:22-78.  factor -1,
+ * diagonal 1 is the 1 - T G0 of ggbi/ggrm/ggtf; factor 1, diagonal 0 the
+ * T G0 work matrix of gggm (solvers.rs:16-20). */
+static void compact_tg(const orc_plan* p, const orc_states* st, const cplx* g0, const cplx* tmat, cplx factor,
+                       cplx diagonal, cplx* m) {
     int n = st->n;
     for (int c = 0; c < n; ++c) {
         for (int r = 0; r < n; ++r) {
             cplx v = 0.0;
             cplx g = g0[r + (size_t)n * c];
-            if (cabs(g) > p->toler2) v += -tmat[2 * r] * g;
+            if (cabs(g) > p->toler2) v += factor * tmat[2 * r] * g;
             if (p->nsp == 2) {
                 int q = spin_partner(st->l[r], st->m[r], st->spin[r], r);
                 if (q >= 0) {
                     cplx gf = g0[q + (size_t)n * c];
-                    if (cabs(gf) > p->toler2) v += -tmat[2 * q + 1] * gf;
+                    if (cabs(gf) > p->toler2) v += factor * tmat[2 * q + 1] * gf;
                 }
             }
             m[r + (size_t)n * c] = v;
         }
-        m[c + (size_t)n * c] += 1.0;
+        m[c + (size_t)n * c] += diagonal;
     }
+}
+
+static void iterative_system(const orc_plan* p, const orc_states* st, const cplx* g0, const cplx* tmat, cplx* m) {
+    compact_tg(p, st, g0, tmat, -1.0, 1.0, m);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FEFF ggrm (recursion), gggm (Graves-Morris/Salam) and ggtf (TFQMR):       */
+/* refeff fms/internals/iterative.rs:256-612, restated in FP64 with the same  */
+/* restart / iteration limits, convergence tests and breakdown checks.        */
+/* ------------------------------------------------------------------------ */
+
+static void adjoint_matvec(int n, const cplx* m, const cplx* x, cplx* y) {
+    for (int c = 0; c < n; ++c) {
+        cplx s = 0.0;
+        for (int r = 0; r < n; ++r) s += conj(m[r + (size_t)n * c]) * x[r];
+        y[c] = s;
+    }
+}
+
+/* iterative.rs:256-376 */
+static int ggrm_solve(int n, const cplx* m, int src, double tol, cplx* x, int* msord) {
+    enum { NV = 10 };
+    cplx* buf = calloc((size_t)NV * n, sizeof(cplx));
+    if (!buf) return ORC_NO_MEMORY;
+    cplx *r = buf, *xket = buf + n, *xbra = buf + 2 * n, *tket = buf + 3 * n, *tbra = buf + 4 * n,
+         *xketp = buf + 5 * n, *xbrap = buf + 6 * n, *z = buf + 7 * n, *sv = buf + 8 * n, *tmp = buf + 9 * n;
+    int rc = ORC_NO_CONVERGENCE;
+    *msord = 0;
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    for (int restart = 0; restart < 128; ++restart) {
+        if (restart > 0) matvec(n, m, x, r);
+        else for (int i = 0; i < n; ++i) r[i] = 0.0;
+        r[src] -= 1.0;
+        for (int i = 0; i < n; ++i) xket[i] = -r[i];
+        cplx rn = cdot(n, xket, xket);
+        if (rn == 0.0) { rc = ORC_OK; goto done; }
+        if (!(isfinite(creal(rn)) && creal(rn) > 0.0)) { rc = ORC_BREAKDOWN; goto done; }
+        double xfnorm = 1.0 / creal(rn);
+        for (int i = 0; i < n; ++i) xbra[i] = xket[i] * xfnorm;
+        matvec(n, m, xket, tket);
+        *msord += 1;
+        cplx aa = cdot(n, xbra, tket), aac = conj(aa), bb = 0.0, bbc = 0.0, betac = aa;
+        if (betac == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+        cplx yy = 1.0;
+        for (int i = 0; i < n; ++i) {
+            xketp[i] = 0.0;
+            xbrap[i] = 0.0;
+            z[i] = xket[i];
+            x[i] += z[i] / betac;
+            sv[i] = tket[i];
+            r[i] += sv[i] / betac;
+        }
+        for (int it = 0; it < 100; ++it) {
+            for (int i = 0; i < n; ++i) tket[i] -= aa * xket[i] + bb * xketp[i];
+            adjoint_matvec(n, m, xbra, tbra);
+            for (int i = 0; i < n; ++i) tbra[i] -= aac * xbra[i] + bbc * xbrap[i];
+            cplx rn2 = cdot(n, tbra, tket);
+            if (rn2 == 0.0) { rc = ORC_OK; goto done; }
+            bb = csqrt(rn2);
+            bbc = conj(bb);
+            if (bb == 0.0 || bbc == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            for (int i = 0; i < n; ++i) {
+                xketp[i] = xket[i];
+                xbrap[i] = xbra[i];
+                xket[i] = tket[i] / bb;
+                xbra[i] = tbra[i] / bbc;
+            }
+            matvec(n, m, xket, tket);
+            *msord += 1;
+            aa = cdot(n, xbra, tket);
+            aac = conj(aa);
+            if (betac == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            cplx alphac = bb / betac;
+            for (int i = 0; i < n; ++i) {
+                z[i] = xket[i] - alphac * z[i];
+                sv[i] = tket[i] - alphac * sv[i];
+            }
+            betac = aa - alphac * bb;
+            if (betac == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            yy = -alphac * yy;
+            cplx gamma = yy / betac;
+            for (int i = 0; i < n; ++i) {
+                x[i] += gamma * z[i];
+                r[i] += gamma * sv[i];
+            }
+            if (within_tol(n, r, tol)) { rc = ORC_OK; goto done; }
+        }
+    }
+done:
+    (void)tmp;
+    free(buf);
+    return rc;
+}
+
+/* iterative.rs:378-515, on the T G0 work matrix w */
+static int gggm_solve(int n, const cplx* w, int src, double tol, cplx* x, int* msord) {
+    enum { NV = 9 };
+    cplx* buf = calloc((size_t)NV * n, sizeof(cplx));
+    if (!buf) return ORC_NO_MEMORY;
+    cplx *b = buf, *x0 = buf + n, *r0 = buf + 2 * n, *x1 = buf + 3 * n, *r1 = buf + 4 * n, *wv = buf + 5 * n,
+         *t0 = buf + 6 * n, *t1 = buf + 7 * n, *av = buf + 8 * n;
+    int rc = ORC_NO_CONVERGENCE;
+    *msord = 0;
+    cplx q0 = 1.0;
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    b[src] = 1.0;
+    for (int restart = 0; restart < 128; ++restart) {
+        if (restart > 0) {
+            if (q0 == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            for (int i = 0; i < n; ++i) x[i] += x0[i] / q0;
+            matvec(n, w, x, av);
+            for (int i = 0; i < n; ++i) b[i] = av[i] - x[i];
+            b[src] += 1.0;
+        }
+        for (int i = 0; i < n; ++i) {
+            r0[i] = b[i];
+            x0[i] = 0.0;
+            x1[i] = b[i];
+        }
+        matvec(n, w, b, r1);
+        *msord += 1;
+        cplx ww = cdot(n, r0, r0), aa = cdot(n, r1, r1), wa = cdot(n, r0, r1), aw = conj(wa);
+        if (aa == 0.0 || ww == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+        cplx dd = aa * ww - aw * wa;
+        cplx sdd = dd / aa / ww;
+        if (cabs(sdd) < 1.0e-8) {
+            for (int i = 0; i < n; ++i) wv[i] = r0[i] / ww;
+        } else {
+            if (dd == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            ww = (ww - aw) / dd;
+            aa = (wa - aa) / dd;
+            for (int i = 0; i < n; ++i) wv[i] = r0[i] * aa + r1[i] * ww;
+        }
+        cplx e0 = cdot(n, wv, r0), e1 = cdot(n, wv, r1);
+        q0 = 1.0;
+        cplx q1 = 1.0;
+        for (int it = 0; it < 10; ++it) {
+            double tl = tol * (cabs(q1) / 10.0);
+            if (!(isfinite(tl) && tl >= 0.0)) { rc = ORC_BREAKDOWN; goto done; }
+            if (within_tol(n, r1, tl)) {
+                if (q1 == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+                for (int i = 0; i < n; ++i) x[i] += x1[i] / q1;
+                rc = ORC_OK;
+                goto done;
+            }
+            if (e0 == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            cplx alpha = e1 / e0;
+            for (int i = 0; i < n; ++i) t0[i] = r1[i] - alpha * r0[i];
+            matvec(n, w, t0, t1);
+            *msord += 1;
+            cplx wa2 = cdot(n, t0, t1), ww2 = cdot(n, t0, t0), aa2 = cdot(n, t1, t1), aw2 = conj(wa2);
+            if (ww2 - aw2 == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            cplx theta = (wa2 - aa2) / (ww2 - aw2);
+            for (int i = 0; i < n; ++i) r0[i] = t1[i] - theta * t0[i];
+            cplx d1 = 1.0 - theta;
+            for (int i = 0; i < n; ++i) x0[i] = t0[i] + d1 * (x1[i] - alpha * x0[i]);
+            q0 = d1 * (q1 - alpha * q0);
+            double tl0 = tol * cabs(q0);
+            if (!(isfinite(tl0) && tl0 >= 0.0)) { rc = ORC_BREAKDOWN; goto done; }
+            if (within_tol(n, r0, tl0)) {
+                if (q0 == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+                for (int i = 0; i < n; ++i) x[i] += x0[i] / q0;
+                rc = ORC_OK;
+                goto done;
+            }
+            e0 = cdot(n, wv, r0);
+            if (e1 == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            cplx beta = e0 / e1;
+            for (int i = 0; i < n; ++i) t0[i] = r0[i] - beta * r1[i];
+            matvec(n, w, t0, av);
+            *msord += 1;
+            cplx d2 = beta * theta;
+            for (int i = 0; i < n; ++i) r1[i] = av[i] + d2 * r1[i];
+            e1 = cdot(n, wv, r1);
+            cplx d3 = beta * (1.0 - theta);
+            for (int i = 0; i < n; ++i) x1[i] = x0[i] - d3 * x1[i] + t0[i];
+            q1 = q0 - (1.0 - theta) * beta * q1;
+        }
+    }
+done:
+    free(buf);
+    return rc;
+}
+
+/* iterative.rs:517-612 */
+static int ggtf_solve(int n, const cplx* m, int src, double tol, cplx* x, int* msord) {
+    enum { NV = 7 };
+    cplx* buf = calloc((size_t)NV * n, sizeof(cplx));
+    if (!buf) return ORC_NO_MEMORY;
+    cplx *a = buf, *u = buf + n, *wv = buf + 2 * n, *v = buf + 3 * n, *d = buf + 4 * n, *r = buf + 5 * n,
+         *tmp = buf + 6 * n;
+    int rc = ORC_NO_CONVERGENCE;
+    *msord = 0;
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    for (int restart = 0; restart < 128; ++restart) {
+        if (restart > 0) matvec(n, m, x, a);
+        for (int i = 0; i < n; ++i) u[i] = -a[i];
+        u[src] += 1.0;
+        matvec(n, m, u, a);
+        *msord += 1;
+        for (int i = 0; i < n; ++i) {
+            wv[i] = u[i];
+            v[i] = a[i];
+            d[i] = 0.0;
+        }
+        cplx aa = cdot(n, u, u);
+        if (aa == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+        if (!(isfinite(creal(aa)) && creal(aa) > 0.0)) { rc = ORC_BREAKDOWN; goto done; }
+        double tau = sqrt(creal(aa)), nu = 0.0;
+        cplx eta = 0.0, rho = 1.0, alpha = 0.0;
+        for (int i = 0; i < n; ++i) r[i] = u[i] / aa;
+        for (int nit = 0; nit <= 20; ++nit) {
+            if (nit % 2 == 0) {
+                cplx rv = cdot(n, r, v);
+                if (rv == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+                alpha = rho / rv;
+            } else {
+                matvec(n, m, u, a);
+                *msord += 1;
+            }
+            for (int i = 0; i < n; ++i) wv[i] -= alpha * a[i];
+            if (alpha == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+            cplx f = ((nu * nu) * eta) / alpha;
+            for (int i = 0; i < n; ++i) d[i] = u[i] + f * d[i];
+            cplx w2 = cdot(n, wv, wv);
+            if (!(isfinite(creal(w2)) && creal(w2) >= 0.0)) { rc = ORC_BREAKDOWN; goto done; }
+            double norm = sqrt(creal(w2));
+            nu = norm / tau;
+            double cm = 1.0 / sqrt(1.0 + nu * nu);
+            tau *= nu * cm;
+            eta = (cm * cm) * alpha;
+            for (int i = 0; i < n; ++i) x[i] += eta * d[i];
+            double err = tau * sqrt((1.0 + nit) / (double)n) * 10.0;
+            if (fabs(err) < tol) { rc = ORC_OK; goto done; }
+            if (nit % 2 != 0) {
+                cplx rho_prev = rho;
+                rho = cdot(n, r, wv);
+                if (rho_prev == 0.0) { rc = ORC_BREAKDOWN; goto done; }
+                cplx beta = rho / rho_prev;
+                for (int i = 0; i < n; ++i) {
+                    u[i] = wv[i] + beta * u[i];
+                    v[i] = beta * (a[i] + beta * v[i]);
+                }
+                matvec(n, m, u, a);
+                *msord += 1;
+                for (int i = 0; i < n; ++i) v[i] += a[i];
+            } else {
+                for (int i = 0; i < n; ++i) u[i] -= alpha * v[i];
+            }
+        }
+    }
+done:
+    (void)tmp;
+    free(buf);
+    return rc;
+}
+
+/* minv 2 -> ggrm, 3 -> gggm, any other non-LU, non-ggbi value -> ggtf
+ * (refeff fms_scattering_method_selection, fms/driver.rs:72-95) */
+static int iterative_solve(int minv, int n, const cplx* sys, int src, double tol, cplx* x, int* msord) {
+    if (minv == 1) return ggbi_solve(n, sys, src, tol, x, msord);
+    if (minv == 2) return ggrm_solve(n, sys, src, tol, x, msord);
+    if (minv == 3) return gggm_solve(n, sys, src, tol, x, msord);
+    return ggtf_solve(n, sys, src, tol, x, msord);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -906,8 +1176,9 @@ int orc_fms_energy(const orc_plan* p, const double* k_in, const double* phases_i
             }
         }
         free(ipiv);
-    } else if (p->minv == 1) {
-        iterative_system(p, &st, g0, tmat, a);
+    } else {
+        if (p->minv == 3) compact_tg(p, &st, g0, tmat, 1.0, 0.0, a);
+        else iterative_system(p, &st, g0, tmat, a);
         cplx* x = malloc(sizeof(cplx) * n);
         if (!x) { rc = ORC_NO_MEMORY; goto out; }
         for (int q = ps; q <= pe && !rc; ++q) {
@@ -915,7 +1186,7 @@ int orc_fms_energy(const orc_plan* p, const double* k_in, const double* phases_i
             int off = st.rep[q];
             for (int c = 0; c < ipart && !rc; ++c) {
                 int ms = 0;
-                rc = ggbi_solve(n, a, off + c, (double)p->toler1, x, &ms);
+                rc = iterative_solve(p->minv, n, a, off + c, (double)p->toler1, x, &ms);
                 if (rc) break;
                 if (msord_out) *msord_out = ms;
                 for (int r = 0; r < ipart; ++r) {
@@ -926,8 +1197,6 @@ int orc_fms_energy(const orc_plan* p, const double* k_in, const double* phases_i
             }
         }
         free(x);
-    } else {
-        rc = ORC_UNSUPPORTED;
     }
 out:
     free(g0); free(a); free(tmat);
@@ -1136,4 +1405,27 @@ int orc_iterative_system_from(int nsp, int n, const int* s_l, const int* s_m, co
 
 int orc_ggbi_solve_from(int n, const double* sys, int src, double tol, double* x, int* msord) {
     return ggbi_solve(n, (const cplx*)sys, src, tol, (cplx*)x, msord);
+}
+
+/* FEFF ggrm / gggm / ggtf (minv = 2 / 3 / other) on an explicit matrix:
+ * 1 - T G0 for ggrm and ggtf, the T G0 work matrix for gggm. */
+int orc_iterative_solve_from(int minv, int n, const double* sys, int src, double tol, double* x, int* msord) {
+    return iterative_solve(minv, n, (const cplx*)sys, src, tol, (cplx*)x, msord);
+}
+
+/* gggm's T G0 work matrix (refeff fms/solvers.rs:16-20) from explicit tables. */
+int orc_graves_morris_system_from(int nsp, int n, const int* s_l, const int* s_m, const int* s_spin, const double* g0,
+                                  const double* tmat, float toler2, double* m_out) {
+    orc_plan p;
+    memset(&p, 0, sizeof(p));
+    p.nsp = nsp;
+    p.toler2 = toler2;
+    orc_states st;
+    memset(&st, 0, sizeof st);
+    st.n = n;
+    st.l = (int*)s_l;
+    st.m = (int*)s_m;
+    st.spin = (int*)s_spin;
+    compact_tg(&p, &st, (const cplx*)g0, (const cplx*)tmat, 1.0, 0.0, (cplx*)m_out);
+    return ORC_OK;
 }

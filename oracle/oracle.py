@@ -306,6 +306,9 @@ _lib.orc_g0_element.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c
                                 C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _f64p]
 _lib.orc_iterative_system_from.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, C.c_float, _f64p]
 _lib.orc_ggbi_solve_from.argtypes = [C.c_int, _f64p, C.c_int, C.c_double, _f64p, C.POINTER(C.c_int)]
+_lib.orc_iterative_solve_from.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, C.c_double, _f64p, C.POINTER(C.c_int)]
+_lib.orc_graves_morris_system_from.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, C.c_float,
+                                               _f64p]
 _lib.orc_zgetrf.argtypes = [C.c_int, _f64p, _i32p]
 
 
@@ -333,6 +336,30 @@ def ggbi_solve(system, src, tol):
     x = np.zeros(2 * n)
     ms = C.c_int()
     _check(_lib.orc_ggbi_solve_from(n, sc.view(np.float64).reshape(-1), src, tol, x, C.byref(ms)), "ggbi")
+    return x.view(np.complex128).copy(), ms.value
+
+
+def graves_morris_system_matrix(nsp, s_l, s_m, s_spin, g0, tmat, toler2):
+    """T G0 work matrix of FEFF gggm (refeff fms/solvers.rs:16-20), (row, col)."""
+    n = len(s_l)
+    g0c = np.ascontiguousarray(np.asarray(g0, np.complex128).T)
+    tc = np.ascontiguousarray(np.asarray(tmat, np.complex128).T)
+    out = np.zeros(2 * n * n)
+    _lib.orc_graves_morris_system_from(nsp, n, np.ascontiguousarray(s_l, np.int32),
+                                       np.ascontiguousarray(s_m, np.int32), np.ascontiguousarray(s_spin, np.int32),
+                                       g0c.view(np.float64).reshape(-1), tc.view(np.float64).reshape(-1), toler2, out)
+    return out.view(np.complex128).reshape(n, n).T.copy()
+
+
+def iterative_solve(minv, system, src, tol):
+    """FEFF ggbi / ggrm / gggm / ggtf (minv 1 / 2 / 3 / other) for source state src:
+    -> (x, multiple scattering order)."""
+    n = system.shape[0]
+    sc = np.ascontiguousarray(np.asarray(system, np.complex128).T)
+    x = np.zeros(2 * n)
+    ms = C.c_int()
+    _check(_lib.orc_iterative_solve_from(minv, n, sc.view(np.float64).reshape(-1), src, tol, x, C.byref(ms)),
+           f"iterative minv={minv}")
     return x.view(np.complex128).copy(), ms.value
 
 

@@ -21,7 +21,7 @@ STATUS_NAMES = {
     15: "INVALID_SPIN_COUNT", 16: "EMPTY_CLUSTER", 17: "IO_ERROR",
 }
 
-SOLVER_AUTO, SOLVER_MIXED, SOLVER_FP64, SOLVER_GGBI = 0, 1, 2, 3
+SOLVER_AUTO, SOLVER_MIXED, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGRM, SOLVER_GGGM, SOLVER_GGTF = 0, 1, 2, 3, 4, 5, 6
 
 
 class FmsLibraryMissing(RuntimeError):

@@ -1977,6 +1977,371 @@ __device__ __forceinline__ int spin_partner_row(const GgbiArgs& a, int i) {
     return s == 1 ? i - 1 : i + 1;
 }
 
+// ---- FEFF ggrm / gggm / ggtf (minv = 2 / 3 / other) -----------------------
+//
+// refeff fms/internals/iterative.rs:256-612 in FP64 (refeff runs them in c32):
+// one 256-thread workgroup per (source column, energy) runs the whole
+// restarted iteration.  Operators on the toler2-cut G0 (A64), applied on the
+// fly: M = 1 - T G0c (ggrm, ggtf; fms/solvers.rs:10-14), W = T G0c (gggm;
+// :16-20) and M^H (ggrm's bra recursion), with T acting on rows and the
+// nsp = 2 partner row adding t1.  Scalars come from fixed-order block
+// reductions, so every thread takes the same branch; the solution goes to
+// X[e][i][c] for k_ggbi_gg.  info bits: 1 << 29 breakdown, 1 << 28 no
+// convergence.
+constexpr int kIterVecs = 11;
+
+struct IterArgs {
+    const double2* g0c;   // [e] npad x npad, A64 layout
+    const double2* tmat;  // [e][2][n]
+    const int* sl;
+    const int* sm;
+    const int* sspin;     // 1-based
+    const int* rhs_src;   // state index of each RHS column
+    double2* work;        // [e][nrhs][kIterVecs][npad]
+    double2* X;           // [e][npad][nrhs]
+    int* info;            // [e]
+    size_t sa, sx;
+    int n, npad, lda, nrhs, nsp;
+    double tol;
+};
+
+__device__ __forceinline__ int iter_partner(const IterArgs& a, int i) {
+    const int l = a.sl[i], m = a.sm[i], s = a.sspin[i];
+    const int proj = m + s;
+    if (proj <= -l + 1 || proj >= l + 2) return -1;
+    return s == 1 ? i - 1 : i + 1;
+}
+
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ bool cnz(double2 a) { return a.x != 0.0 || a.y != 0.0; }
+
+// principal square root (C99 csqrt branch)
+__device__ __forceinline__ double2 csqrt_d(double2 z) {
+    if (!cnz(z)) return make_double2(0.0, z.y);
+    const double r = hypot(z.x, z.y);
+    const double t = sqrt(0.5 * (r + fabs(z.x)));
+    if (z.x >= 0.0) return make_double2(t, z.y / (2.0 * t));
+    return make_double2(fabs(z.y) / (2.0 * t), copysign(t, z.y));
+}
+
+// <a, b> = sum conj(a_i) b_i in a fixed order; every thread gets the value
+__device__ double2 iter_cdot(const double2* a, const double2* b, int n, double2* red) {
+    double sr = 0.0, si = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double2 u = a[i], v = b[i];
+        sr += u.x * v.x + u.y * v.y;
+        si += u.x * v.y - u.y * v.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_double2(sr, si);
+    __syncthreads();
+    double2 t = red[0];
+    for (int w = 1; w < 4; ++w) t = cadd(t, red[w]);
+    return t;
+}
+
+// every |re|, |im| <= tol (fms_vector_within_tolerance); NaN is outside
+__device__ bool iter_within(const double2* v, int n, double tol) {
+    int bad = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double2 x = v[i];
+        if (!(fabs(x.x) <= tol && fabs(x.y) <= tol)) bad = 1;
+    }
+    return !__syncthreads_or(bad);
+}
+
+// y = M v (factor -1, diagonal 1) or W v (factor 1, diagonal 0); g: scratch
+template <bool GM>
+__device__ void iter_op(const IterArgs& a, const double2* G, const double2* t0, const double2* t1, const double2* v,
+                        double2* y, double2* g) {
+    __syncthreads();
+    for (int r = threadIdx.x; r < a.n; r += 256) {
+        const double2* row = G + static_cast<size_t>(r) * a.lda;
+        double ar = 0.0, ai = 0.0;
+        for (int c = 0; c < a.n; ++c) {
+            const double2 m = row[c], x = v[c];
+            ar += m.x * x.x - m.y * x.y;
+            ai += m.x * x.y + m.y * x.x;
+        }
+        g[r] = make_double2(ar, ai);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < a.n; r += 256) {
+        double2 tg = cmul(t0[r], g[r]);
+        if (a.nsp == 2) {
+            const int q = iter_partner(a, r);
+            if (q >= 0) tg = cadd(tg, cmul(t1[q], g[q]));
+        }
+        y[r] = GM ? tg : csub(v[r], tg);
+    }
+    __syncthreads();
+}
+
+// y = M^H u = u - G0c^H (conj(t0) u + conj(t1) u[partner])
+__device__ void iter_adjoint(const IterArgs& a, const double2* G, const double2* t0, const double2* t1,
+                             const double2* u, double2* y, double2* g) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < a.n; j += 256) {
+        double2 w = cmul(cconj(t0[j]), u[j]);
+        if (a.nsp == 2) {
+            const int q = iter_partner(a, j);
+            if (q >= 0) w = cadd(w, cmul(cconj(t1[j]), u[q]));
+        }
+        g[j] = w;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < a.n; c += 256) {
+        double ar = 0.0, ai = 0.0;
+        for (int j = 0; j < a.n; ++j) {
+            const double2 m = G[static_cast<size_t>(j) * a.lda + c], w = g[j];
+            ar += m.x * w.x + m.y * w.y;  // conj(m) w
+            ai += m.x * w.y - m.y * w.x;
+        }
+        y[c] = csub(u[c], make_double2(ar, ai));
+    }
+    __syncthreads();
+}
+
+// status: 0 converged, 1 breakdown, 2 no convergence
+template <int METHOD>
+__global__ __launch_bounds__(256) void k_iter_solve(IterArgs a) {
+    __shared__ double2 red[4];
+    const int c = blockIdx.x, e = blockIdx.y, tid = threadIdx.x, n = a.n;
+    const int src = a.rhs_src[c];
+    const double2* G = a.g0c + e * a.sa;
+    const double2* t0 = a.tmat + static_cast<size_t>(e) * 2 * n;
+    const double2* t1 = t0 + n;
+    double2* V = a.work + (static_cast<size_t>(e) * a.nrhs + c) * kIterVecs * a.npad;
+    auto vec = [&](int k) { return V + static_cast<size_t>(k) * a.npad; };
+    const double2 zero = make_double2(0.0, 0.0), one = make_double2(1.0, 0.0);
+    const double tol = a.tol;
+    int status = 2;
+    double2* x = vec(0);
+    for (int i = tid; i < n; i += 256) x[i] = zero;
+    if constexpr (METHOD == 2) {
+        // ggrm, iterative.rs:256-376
+        double2 *r = vec(1), *xket = vec(2), *xbra = vec(3), *tket = vec(4), *tbra = vec(5), *xketp = vec(6),
+                *xbrap = vec(7), *z = vec(8), *sv = vec(9), *g = vec(10);
+        for (int restart = 0; restart < 128 && status == 2; ++restart) {
+            if (restart > 0) {
+                iter_op<false>(a, G, t0, t1, x, r, g);
+            } else {
+                for (int i = tid; i < n; i += 256) r[i] = zero;
+            }
+            __syncthreads();
+            if (tid == 0) r[src] = csub(r[src], one);
+            __syncthreads();
+            for (int i = tid; i < n; i += 256) xket[i] = make_double2(-r[i].x, -r[i].y);
+            const double2 rn = iter_cdot(xket, xket, n, red);
+            if (!cnz(rn)) { status = 0; break; }
+            if (!(isfinite(rn.x) && rn.x > 0.0)) { status = 1; break; }
+            const double xfnorm = 1.0 / rn.x;
+            for (int i = tid; i < n; i += 256) xbra[i] = cscale(xfnorm, xket[i]);
+            iter_op<false>(a, G, t0, t1, xket, tket, g);
+            double2 aa = iter_cdot(xbra, tket, n, red), aac = cconj(aa), bb = zero, bbc = zero, betac = aa;
+            if (!cnz(betac)) { status = 1; break; }
+            double2 yy = one;
+            for (int i = tid; i < n; i += 256) {
+                xketp[i] = zero;
+                xbrap[i] = zero;
+                z[i] = xket[i];
+                x[i] = cadd(x[i], cdiv(z[i], betac));
+                sv[i] = tket[i];
+                r[i] = cadd(r[i], cdiv(sv[i], betac));
+            }
+            for (int it = 0; it < 100; ++it) {
+                for (int i = tid; i < n; i += 256)
+                    tket[i] = csub(tket[i], cadd(cmul(aa, xket[i]), cmul(bb, xketp[i])));
+                iter_adjoint(a, G, t0, t1, xbra, tbra, g);
+                for (int i = tid; i < n; i += 256)
+                    tbra[i] = csub(tbra[i], cadd(cmul(aac, xbra[i]), cmul(bbc, xbrap[i])));
+                const double2 rn2 = iter_cdot(tbra, tket, n, red);
+                if (!cnz(rn2)) { status = 0; break; }
+                bb = csqrt_d(rn2);
+                bbc = cconj(bb);
+                if (!cnz(bb) || !cnz(bbc)) { status = 1; break; }
+                for (int i = tid; i < n; i += 256) {
+                    xketp[i] = xket[i];
+                    xbrap[i] = xbra[i];
+                    xket[i] = cdiv(tket[i], bb);
+                    xbra[i] = cdiv(tbra[i], bbc);
+                }
+                iter_op<false>(a, G, t0, t1, xket, tket, g);
+                aa = iter_cdot(xbra, tket, n, red);
+                aac = cconj(aa);
+                const double2 alphac = cdiv(bb, betac);
+                for (int i = tid; i < n; i += 256) {
+                    z[i] = csub(xket[i], cmul(alphac, z[i]));
+                    sv[i] = csub(tket[i], cmul(alphac, sv[i]));
+                }
+                betac = csub(aa, cmul(alphac, bb));
+                if (!cnz(betac)) { status = 1; break; }
+                yy = cmul(make_double2(-alphac.x, -alphac.y), yy);
+                const double2 gamma = cdiv(yy, betac);
+                for (int i = tid; i < n; i += 256) {
+                    x[i] = cadd(x[i], cmul(gamma, z[i]));
+                    r[i] = cadd(r[i], cmul(gamma, sv[i]));
+                }
+                if (iter_within(r, n, tol)) { status = 0; break; }
+            }
+        }
+    } else if constexpr (METHOD == 3) {
+        // gggm, iterative.rs:378-515, on W = T G0c
+        double2 *b = vec(1), *x0 = vec(2), *r0 = vec(3), *x1 = vec(4), *r1 = vec(5), *wv = vec(6), *tv0 = vec(7),
+                *tv1 = vec(8), *av = vec(9), *g = vec(10);
+        double2 q0 = one;
+        for (int i = tid; i < n; i += 256) b[i] = zero;
+        __syncthreads();
+        if (tid == 0) b[src] = one;
+        for (int restart = 0; restart < 128 && status == 2; ++restart) {
+            if (restart > 0) {
+                if (!cnz(q0)) { status = 1; break; }
+                for (int i = tid; i < n; i += 256) x[i] = cadd(x[i], cdiv(x0[i], q0));
+                iter_op<true>(a, G, t0, t1, x, av, g);
+                for (int i = tid; i < n; i += 256) b[i] = csub(av[i], x[i]);
+                __syncthreads();
+                if (tid == 0) b[src] = cadd(b[src], one);
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += 256) {
+                r0[i] = b[i];
+                x0[i] = zero;
+                x1[i] = b[i];
+            }
+            iter_op<true>(a, G, t0, t1, b, r1, g);
+            double2 ww = iter_cdot(r0, r0, n, red), aa = iter_cdot(r1, r1, n, red), wa = iter_cdot(r0, r1, n, red);
+            const double2 aw = cconj(wa);
+            if (!cnz(aa) || !cnz(ww)) { status = 1; break; }
+            const double2 dd = csub(cmul(aa, ww), cmul(aw, wa));
+            const double2 sdd = cdiv(cdiv(dd, aa), ww);
+            if (hypot(sdd.x, sdd.y) < 1.0e-8) {
+                for (int i = tid; i < n; i += 256) wv[i] = cdiv(r0[i], ww);
+            } else {
+                if (!cnz(dd)) { status = 1; break; }
+                ww = cdiv(csub(ww, aw), dd);
+                aa = cdiv(csub(wa, aa), dd);
+                for (int i = tid; i < n; i += 256) wv[i] = cadd(cmul(r0[i], aa), cmul(r1[i], ww));
+            }
+            double2 e0 = iter_cdot(wv, r0, n, red), e1 = iter_cdot(wv, r1, n, red);
+            q0 = one;
+            double2 q1 = one;
+            for (int it = 0; it < 10; ++it) {
+                const double tl = tol * (hypot(q1.x, q1.y) / 10.0);
+                if (!(isfinite(tl) && tl >= 0.0)) { status = 1; break; }
+                if (iter_within(r1, n, tl)) {
+                    if (!cnz(q1)) { status = 1; break; }
+                    for (int i = tid; i < n; i += 256) x[i] = cadd(x[i], cdiv(x1[i], q1));
+                    status = 0;
+                    break;
+                }
+                if (!cnz(e0)) { status = 1; break; }
+                const double2 alpha = cdiv(e1, e0);
+                for (int i = tid; i < n; i += 256) tv0[i] = csub(r1[i], cmul(alpha, r0[i]));
+                iter_op<true>(a, G, t0, t1, tv0, tv1, g);
+                const double2 wa2 = iter_cdot(tv0, tv1, n, red), ww2 = iter_cdot(tv0, tv0, n, red),
+                              aa2 = iter_cdot(tv1, tv1, n, red);
+                const double2 den = csub(ww2, cconj(wa2));
+                if (!cnz(den)) { status = 1; break; }
+                const double2 theta = cdiv(csub(wa2, aa2), den);
+                const double2 d1 = csub(one, theta);
+                for (int i = tid; i < n; i += 256) {
+                    r0[i] = csub(tv1[i], cmul(theta, tv0[i]));
+                    x0[i] = cadd(tv0[i], cmul(d1, csub(x1[i], cmul(alpha, x0[i]))));
+                }
+                q0 = cmul(d1, csub(q1, cmul(alpha, q0)));
+                const double tl0 = tol * hypot(q0.x, q0.y);
+                if (!(isfinite(tl0) && tl0 >= 0.0)) { status = 1; break; }
+                if (iter_within(r0, n, tl0)) {
+                    if (!cnz(q0)) { status = 1; break; }
+                    for (int i = tid; i < n; i += 256) x[i] = cadd(x[i], cdiv(x0[i], q0));
+                    status = 0;
+                    break;
+                }
+                e0 = iter_cdot(wv, r0, n, red);
+                if (!cnz(e1)) { status = 1; break; }
+                const double2 beta = cdiv(e0, e1);
+                for (int i = tid; i < n; i += 256) tv0[i] = csub(r0[i], cmul(beta, r1[i]));
+                iter_op<true>(a, G, t0, t1, tv0, av, g);
+                const double2 d2 = cmul(beta, theta);
+                for (int i = tid; i < n; i += 256) r1[i] = cadd(av[i], cmul(d2, r1[i]));
+                e1 = iter_cdot(wv, r1, n, red);
+                const double2 d3 = cmul(beta, d1);
+                for (int i = tid; i < n; i += 256) x1[i] = cadd(csub(x0[i], cmul(d3, x1[i])), tv0[i]);
+                q1 = csub(q0, cmul(cmul(d1, beta), q1));
+            }
+        }
+    } else {
+        // ggtf, iterative.rs:517-612
+        double2 *av = vec(1), *u = vec(2), *wv = vec(3), *v = vec(4), *d = vec(5), *r = vec(6), *g = vec(7);
+        for (int i = tid; i < n; i += 256) av[i] = zero;
+        for (int restart = 0; restart < 128 && status == 2; ++restart) {
+            if (restart > 0) iter_op<false>(a, G, t0, t1, x, av, g);
+            for (int i = tid; i < n; i += 256) u[i] = make_double2(-av[i].x, -av[i].y);
+            __syncthreads();
+            if (tid == 0) u[src] = cadd(u[src], one);
+            iter_op<false>(a, G, t0, t1, u, av, g);
+            for (int i = tid; i < n; i += 256) {
+                wv[i] = u[i];
+                v[i] = av[i];
+                d[i] = zero;
+            }
+            const double2 aa = iter_cdot(u, u, n, red);
+            if (!cnz(aa)) { status = 1; break; }
+            if (!(isfinite(aa.x) && aa.x > 0.0)) { status = 1; break; }
+            double tau = sqrt(aa.x), nu = 0.0;
+            double2 eta = zero, rho = one, alpha = zero;
+            for (int i = tid; i < n; i += 256) r[i] = cdiv(u[i], aa);
+            for (int nit = 0; nit <= 20; ++nit) {
+                if (nit % 2 == 0) {
+                    const double2 rv = iter_cdot(r, v, n, red);
+                    if (!cnz(rv)) { status = 1; break; }
+                    alpha = cdiv(rho, rv);
+                } else {
+                    iter_op<false>(a, G, t0, t1, u, av, g);
+                }
+                for (int i = tid; i < n; i += 256) wv[i] = csub(wv[i], cmul(alpha, av[i]));
+                if (!cnz(alpha)) { status = 1; break; }
+                const double2 f = cdiv(cscale(nu * nu, eta), alpha);
+                for (int i = tid; i < n; i += 256) d[i] = cadd(u[i], cmul(f, d[i]));
+                const double2 w2 = iter_cdot(wv, wv, n, red);
+                if (!(isfinite(w2.x) && w2.x >= 0.0)) { status = 1; break; }
+                nu = sqrt(w2.x) / tau;
+                const double cm = 1.0 / sqrt(1.0 + nu * nu);
+                tau *= nu * cm;
+                eta = cscale(cm * cm, alpha);
+                for (int i = tid; i < n; i += 256) x[i] = cadd(x[i], cmul(eta, d[i]));
+                const double err = tau * sqrt((1.0 + nit) / static_cast<double>(n)) * 10.0;
+                if (fabs(err) < tol) { status = 0; break; }
+                if (nit % 2 != 0) {
+                    const double2 rho_prev = rho;
+                    rho = iter_cdot(r, wv, n, red);
+                    if (!cnz(rho_prev)) { status = 1; break; }
+                    const double2 beta = cdiv(rho, rho_prev);
+                    for (int i = tid; i < n; i += 256) {
+                        u[i] = cadd(wv[i], cmul(beta, u[i]));
+                        v[i] = cmul(beta, cadd(av[i], cmul(beta, v[i])));
+                    }
+                    iter_op<false>(a, G, t0, t1, u, av, g);
+                    for (int i = tid; i < n; i += 256) v[i] = cadd(v[i], av[i]);
+                } else {
+                    for (int i = tid; i < n; i += 256) u[i] = csub(u[i], cmul(alpha, v[i]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    double2* X = a.X + e * a.sx;
+    for (int i = tid; i < n; i += 256) X[static_cast<size_t>(i) * a.nrhs + c] = x[i];
+    if (tid == 0 && status == 1) atomicOr(&a.info[e], 1 << 29);
+    if (tid == 0 && status == 2) atomicOr(&a.info[e], 1 << 28);
+}
+
+
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -2236,6 +2601,7 @@ struct refeff_fms_plan {
     void* d_dinv = nullptr;  // inverse 32x32 diagonal blocks of L and U (float2 or double2)
     int *d_ipiv = nullptr, *d_perm = nullptr, *d_info = nullptr;
     int* d_emap = nullptr;              // energies still refining, per sub-batch
+    double2* d_iter = nullptr;          // ggrm / gggm / ggtf vectors, [e][nrhs][kIterVecs][npad]
     bool full = false;                  // every G0 column is a right-hand side (A^-1 G0)
     int* d_cand = nullptr;              // tournament-pivoting candidates: two [e][calu_cap] lists
     int calu_cap = 0;                   // candidate list length per energy
@@ -2280,7 +2646,7 @@ void free_workspace(refeff_fms_plan* p) {
     p->d_w = nullptr;
     if (p->d_dinv) (void)hipFree(p->d_dinv);
     p->d_dinv = nullptr;
-    dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_cand); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
+    dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_cand); dfree(p->d_iter); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
     dfree(p->d_out); dfree(p->d_norms);
     p->ws_batch = 0;
 }
@@ -2348,6 +2714,11 @@ template <typename P> bool dupload(P** ptr, const P* src, size_t count) {
     return hipMemcpy(*ptr, src, count * sizeof(P), hipMemcpyHostToDevice) == hipSuccess;
 }
 
+// FEFF ggrm / gggm / ggtf: the per-column Krylov workspace
+bool krylov_solver(int s) {
+    return s == REFEFF_FMS_SOLVER_GGRM || s == REFEFF_FMS_SOLVER_GGGM || s == REFEFF_FMS_SOLVER_GGTF;
+}
+
 size_t bytes_per_energy(const refeff_fms_plan* p) {
     const size_t n2 = static_cast<size_t>(p->npad) * p->npad;
     const size_t nr = static_cast<size_t>(p->npad) * p->nrhs;
@@ -2358,6 +2729,7 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
     if (mixed) b += n2 * sizeof(float2);
     b += 2 * p->npad * sizeof(int) + 2 * p->n * sizeof(double2);
     b += static_cast<size_t>(p->ch) * p->ch * p->npot * sizeof(double2);
+    if (krylov_solver(p->solver)) b += static_cast<size_t>(kIterVecs) * nr * sizeof(double2);
     return b;
 }
 
@@ -2375,6 +2747,8 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     ok = ok && hipMalloc(&p->d_w, nr * nb * cs) == hipSuccess;
     ok = ok && hipMalloc(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs) == hipSuccess;
     if (mixed) ok = ok && alloc_calu(p, nb);
+    if (krylov_solver(p->solver))
+        ok = ok && dalloc(&p->d_iter, static_cast<size_t>(kIterVecs) * p->nrhs * p->npad * nb);
     ok = ok && dalloc(&p->d_emap, static_cast<size_t>(nb)) &&
          dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
          dalloc(&p->d_info, nb) && dalloc(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb) &&
@@ -2701,6 +3075,29 @@ void run_ggbi(refeff_fms_plan* p, const Group& g) {
     });
 }
 
+// minv = 2 / 3 / other for one sub-batch: A64 holds the toler2-cut G0, B64 the
+// rows of the representative blocks.  k_iter_solve leaves x in X64; gg =
+// G0[rep rows] x goes into R64's representative rows (read by k_extract).
+void run_iterative(refeff_fms_plan* p, const Group& g) {
+    const int n = p->npad, nr = p->nrhs;
+    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
+    IterArgs ia{p->d_a64 + sa * g.e0, p->d_tmat + 2 * static_cast<size_t>(p->n) * g.e0, p->d_sl, p->d_sm, p->d_ss,
+                p->d_rhs_src, p->d_iter + static_cast<size_t>(kIterVecs) * sx * g.e0, p->d_x64 + sx * g.e0,
+                p->d_info + g.e0, sa, sx, p->n, n, n, nr, p->nsp, static_cast<double>(p->toler1)};
+    launch(p, g.s, REFEFF_FMS_K_SOLVE, 0.0, 16.0 * static_cast<double>(p->n) * p->n * nr * g.nb, [&] {
+        if (p->solver == REFEFF_FMS_SOLVER_GGRM)
+            hipLaunchKernelGGL(k_iter_solve<2>, dim3(nr, g.nb), dim3(256), 0, g.s, ia);
+        else if (p->solver == REFEFF_FMS_SOLVER_GGGM)
+            hipLaunchKernelGGL(k_iter_solve<3>, dim3(nr, g.nb), dim3(256), 0, g.s, ia);
+        else
+            hipLaunchKernelGGL(k_iter_solve<4>, dim3(nr, g.nb), dim3(256), 0, g.s, ia);
+    });
+    hipLaunchKernelGGL(k_fill_double, dim3(std::max<size_t>(1, std::min<size_t>(65535, (2 * sx * g.nb + 255) / 256))),
+                       dim3(256), 0, g.s, reinterpret_cast<double*>(p->d_r64 + sx * g.e0), 0.0, 2 * sx * g.nb);
+    hipLaunchKernelGGL(k_ggbi_gg, dim3(nr, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, p->d_x64 + sx * g.e0,
+                       p->d_r64 + sx * g.e0, sx, p->n, nr, p->d_rhs_pot, p->d_rep, p->d_col0, p->d_ipart);
+}
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -2708,6 +3105,7 @@ double now_s() {
 int map_info(int info) {
     if (info & (1 << 30)) return REFEFF_FMS_NON_FINITE;
     if (info & (1 << 29)) return REFEFF_FMS_BREAKDOWN;
+    if (info & (1 << 28)) return REFEFF_FMS_NO_CONVERGENCE;
     if (info) return REFEFF_FMS_SINGULAR;
     return REFEFF_FMS_OK;
 }
@@ -2728,6 +3126,8 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
     const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
     const bool ggbi = p->solver == REFEFF_FMS_SOLVER_GGBI;
+    const bool krylov = krylov_solver(p->solver);
+    const bool cut_g0 = ggbi || krylov;  // the FEFF iterative branches work on the toler2-cut G0
     const double t0 = now_s();
     std::vector<Group> gs;
     {
@@ -2763,7 +3163,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         launch(p, g.s, REFEFF_FMS_K_BUILD, 0.0, n2 * (mixed ? 24.0 : 16.0) * g.nb + 16.0 * p->n * p->nrhs * g.nb, [&] {
             if (mixed)
                 launch_build<true, true, false>(g.s, ba, p->nat, p->L);
-            else if (ggbi)
+            else if (cut_g0)
                 launch_build<false, true, true>(g.s, ba, p->nat, p->L);
             else
                 launch_build<false, true, false>(g.s, ba, p->nat, p->L);
@@ -2777,6 +3177,10 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         }
         if (ggbi) {
             run_ggbi(p, g);
+            continue;
+        }
+        if (krylov) {
+            run_iterative(p, g);
             continue;
         }
         int* ipiv = p->d_ipiv + static_cast<size_t>(n) * g.e0;
@@ -2891,7 +3295,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
             }
             if (done) break;
         }
-    } else if (ggbi) {
+    } else if (cut_g0) {
         t_first = now_s();
     } else {
         for (const auto& g : gs) {
@@ -2906,7 +3310,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     std::vector<int> info(nb);
     const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
     for (const auto& g : gs) {
-        ExtractArgs ea{(ggbi ? p->d_r64 : p->d_x64) + sx * g.e0, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0,
+        ExtractArgs ea{(cut_g0 ? p->d_r64 : p->d_x64) + sx * g.e0, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0,
                        p->d_ipart, out + sst * g.e0};
         hipLaunchKernelGGL(k_extract, dim3(std::max<size_t>(1, std::min<size_t>(64, (per + 255) / 256)), g.nb),
                            dim3(256), 0, g.s, ea);
@@ -2954,12 +3358,17 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     // scattering matrix forces LU whatever minv says.
     if (solver == REFEFF_FMS_SOLVER_AUTO)
         solver = (d->minv == 0 || full) ? REFEFF_FMS_SOLVER_MIXED
-                                        : (d->minv == 1 ? REFEFF_FMS_SOLVER_GGBI : -1);
-    if (full && solver == REFEFF_FMS_SOLVER_GGBI) return REFEFF_FMS_UNSUPPORTED;
-    if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64 && solver != REFEFF_FMS_SOLVER_GGBI)
+                 : d->minv == 1         ? REFEFF_FMS_SOLVER_GGBI
+                 : d->minv == 2         ? REFEFF_FMS_SOLVER_GGRM
+                 : d->minv == 3         ? REFEFF_FMS_SOLVER_GGGM
+                                        : REFEFF_FMS_SOLVER_GGTF;
+    const bool iterative = solver == REFEFF_FMS_SOLVER_GGBI || krylov_solver(solver);
+    if (full && iterative) return REFEFF_FMS_UNSUPPORTED;  // FullScatteringRequiresLu
+    if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64 && !iterative)
         return REFEFF_FMS_UNSUPPORTED;
-    if (solver == REFEFF_FMS_SOLVER_GGBI &&
-        (!std::isfinite(d->zero_tolerance) || d->zero_tolerance < 0.f || !std::isfinite(d->convergence_tolerance)))
+    // InvalidTolerance (fms/solvers.rs:35-40, internals/iterative.rs:70-75)
+    if (iterative && (!std::isfinite(d->zero_tolerance) || d->zero_tolerance < 0.f ||
+                      !std::isfinite(d->convergence_tolerance) || d->convergence_tolerance < 0.f))
         return REFEFF_FMS_INVALID_ARGUMENT;
     for (int i = 0; i < 3 * d->atom_count; ++i)
         if (!std::isfinite(d->positions[i])) return REFEFF_FMS_NON_FINITE;

@@ -388,6 +388,50 @@ def test_ggbi_one_shot_kat():
     close(gg[6, 7], -0.096492656 + 0.1408408j)
 
 
+@pytest.mark.parametrize("minv,msord,total,e00,e13,e67", [
+    # refeff fms/tests/solvers.rs:150-196 (ggrm), :198-244 (gggm), :246-292 (ggtf)
+    (2, 3, -2.944324 + 4.799402j, -0.007797021 - 0.0032442873j, -0.06596752 + 0.044093154j,
+     -0.09628572 + 0.14052017j),
+    (3, 4, -2.9443216 + 4.799405j, -0.0077970494 - 0.003244209j, -0.06596747 + 0.044093188j,
+     -0.096285895 + 0.14052008j),
+    (4, 4, -2.9443207 + 4.7994027j, -0.0077970214 - 0.0032442873j, -0.06596743 + 0.044093173j,
+     -0.09628591 + 0.1405201j),
+])
+def test_iterative_branch_kats(minv, msord, total, e00, e13, e67):
+    _, l, m, s, _ = orc.state_kets(2, [0], [1], 1)
+    g0, t = _gglu_inputs(8)
+    if minv == 3:
+        sysm = orc.graves_morris_system_matrix(2, l, m, s, g0, t, 0.0)
+        close(sysm.sum(), 0.09041999 + 0.5169j)
+    else:
+        sysm = orc.iterative_system_matrix(2, l, m, s, g0, t, 0.0)
+    gg = np.zeros((8, 8), np.complex128)
+    for c in range(8):
+        x, ms = orc.iterative_solve(minv, sysm, c, 1e-5)
+        gg[:, c] = g0[0:8, :] @ x
+    assert ms == msord  # the last column's count, as refeff reports it
+    close(gg.sum(), total)
+    close(gg[0, 0], e00)
+    close(gg[1, 3], e13)
+    close(gg[6, 7], e67)
+
+
+def test_iterative_branches_converge_to_the_lu_answer():
+    """All four FEFF iterative branches solve the LU system to their tolerance
+    (no column of these small systems hits a breakdown)."""
+    _, l, m, s, _ = orc.state_kets(2, [0], [1], 1)
+    g0, t = _gglu_inputs(8)
+    ref = orc.zgesv(orc.lu_system_matrix(2, l, m, s, g0, t), g0[:, 0:8])
+    it = orc.iterative_system_matrix(2, l, m, s, g0, t, 0.0)
+    gm = orc.graves_morris_system_matrix(2, l, m, s, g0, t, 0.0)
+    for minv in (2, 3, 4):
+        gg = np.zeros((8, 8), np.complex128)
+        for c in range(8):
+            x, _ = orc.iterative_solve(minv, gm if minv == 3 else it, c, 1e-12)
+            gg[:, c] = g0[0:8, :] @ x
+        assert np.abs(gg - ref[0:8]).max() < 1e-9, minv
+
+
 def test_cgetrf_kat():
     """refeff crates/refeff-linalg/src/tests.rs:145-216."""
     a = np.array([[0, 2 - 1j, -1 + 0.5j], [3 + 2j, -1, 4 - 1j], [1 - 3j, 0.5 + 2j, 2]], np.complex128)
