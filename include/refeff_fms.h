@@ -101,6 +101,27 @@ int refeff_fms_state_kets(int32_t spin_channels, int32_t atom_count, const int32
                           int32_t* state_count, int32_t* state_atom, int32_t* state_l, int32_t* state_m,
                           int32_t* state_spin, int32_t* representative_offsets);
 
+/* FMS pair mean-square displacements of the Debye models, computed on the
+ * current HIP device (one FP64 thread per ordered atom pair).  model is the
+ * FEFF idwopt: 0 = correlated (quantum) Debye, FEFF sigms/corrfn; 3 = classical
+ * Debye, FEFF sigcl/corrfn2; any other value returns REFEFF_FMS_UNSUPPORTED.
+ * Replaces the idwopt 0 / 3 branches of fms_mean_square_displacements_with_metadata,
+ * crates/refeff-engine/src/fms.rs:4345-4433, with fms_debye_pair_sigma
+ * (:4849-4877) and quantum/classical_debye_waller_factor,
+ * crates/refeff-core/src/debye/models.rs:138-436.
+ *   temperature, debye_temperature: tk, thetad (K); average_norman_radius: rnrmav (bohr)
+ *   positions: [atom_count][3] f32 cluster coordinates (Angstrom)
+ *   atomic_numbers: [atom_count], 1..139
+ *   sigsqr: [atom_count*atom_count] out; sigsqr(atom2, atom1) at atom2 + atom_count*atom1
+ *     = f32(sig2g + sigma^2 of the path atom1 -> atom2 -> atom1), zero diagonal
+ *   first_failure: [2] or NULL; on a pair failure, (atom1, atom2) of the first failing
+ *     pair in refeff's order (atom2 outer, atom1 inner), else (-1, -1).
+ * A Romberg integral that does not converge returns REFEFF_FMS_NO_CONVERGENCE;
+ * coincident atoms (a zero-length leg) REFEFF_FMS_INVALID_ARGUMENT. */
+int refeff_fms_debye_sigsqr(int32_t model, double temperature, double debye_temperature,
+                            double average_norman_radius, int32_t atom_count, const float* positions,
+                            const int32_t* atomic_numbers, float sig2g, float* sigsqr, int32_t* first_failure);
+
 /* ---------------------------------------------------------------------- */
 /* plan + batched energy sweep (the hot path)                               */
 /* ---------------------------------------------------------------------- */
@@ -401,6 +422,12 @@ int refeff_io_gg_slice_bin_parse(const char* bytes, size_t len, int32_t* dims, f
 int refeff_io_gg_diag_bin_bytes(int32_t energy_count, int32_t atom_count, int32_t dim, const float* values, char* out,
                                 size_t out_len, size_t* needed);
 int refeff_io_gg_diag_bin_parse(const char* bytes, size_t len, int32_t* dims, float* values);
+
+/* sig2.dat pair damping values (idwopt = 4): the third field of the first
+ * pair_count data rows (pairs atom1 < atom2, atom1 outer), each finite and >= 0.
+ * Replaces fms_sig2_dat_pair_values, crates/refeff-engine/src/fms.rs:4778-4847.
+ * values: [pair_count]. */
+int refeff_io_sig2_dat_parse(const char* text, size_t pair_count, double* values);
 
 /* Library version string (static storage). */
 const char* refeff_fms_version(void);

@@ -1429,3 +1429,159 @@ int orc_graves_morris_system_from(int nsp, int n, const int* s_l, const int* s_m
     compact_tg(&p, &st, (const cplx*)g0, (const cplx*)tmat, 1.0, 0.0, (cplx*)m_out);
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Debye-Waller pair damping (FMS sigsqr for idwopt 0 / 3 / 4)                */
+/* ------------------------------------------------------------------------ */
+
+/* FEFF atomic weights (amu), Z = 1..139, as single-precision literals:
+ * crates/refeff-core/src/atomic/tables.rs:49-61, read through atomic_weight
+ * (:10-18) and widened to f64. */
+static const float ORC_ATOMIC_WEIGHTS[139] = {
+    1.0079f, 4.0026f, 6.941f, 9.0122f, 10.81f, 12.01f, 14.007f, 15.999f, 18.998f, 20.18f, 22.9898f, 24.305f,
+    26.982f, 28.086f, 30.974f, 32.064f, 35.453f, 39.948f, 39.09f, 40.08f, 44.956f, 47.90f, 50.942f, 52.00f,
+    54.938f, 55.85f, 58.93f, 58.71f, 63.55f, 65.38f, 69.72f, 72.59f, 74.922f, 78.96f, 79.91f, 83.80f, 85.47f,
+    87.62f, 88.91f, 91.22f, 92.91f, 95.94f, 98.91f, 101.07f, 102.90f, 106.40f, 107.87f, 112.40f, 114.82f,
+    118.69f, 121.75f, 127.60f, 126.90f, 131.30f, 132.91f, 137.34f, 138.91f, 140.12f, 140.91f, 144.24f, 145.0f,
+    150.35f, 151.96f, 157.25f, 158.92f, 162.50f, 164.93f, 167.26f, 168.93f, 173.04f, 174.97f, 178.49f, 180.95f,
+    183.85f, 186.2f, 190.20f, 192.22f, 195.09f, 196.97f, 200.59f, 204.37f, 207.19f, 208.98f, 210.0f, 210.0f,
+    222.0f, 223.0f, 226.0f, 227.0f, 232.04f, 231.0f, 238.03f, 237.05f, 244.0f, 243.0f, 247.0f, 247.0f, 251.0f,
+    252.0f, 257.0f, 258.0f, 259.0f, 266.0f, 267.0f, 268.0f, 269.0f, 270.0f, 269.0f, 278.0f, 281.0f, 282.0f,
+    285.0f, 286.0f, 289.0f, 289.0f, 293.0f, 294.0f, 294.0f, 315.0f, 320.0f, 330.0f, 334.0f, 337.0f, 340.0f,
+    344.0f, 347.0f, 350.0f, 354.0f, 357.0f, 361.0f, 364.0f, 367.0f, 371.0f, 374.0f, 378.0f, 381.0f, 385.0f,
+    388.0f, 392.0f,
+};
+
+double orc_atomic_weight(int z) { return (z >= 1 && z <= 139) ? (double)ORC_ATOMIC_WEIGHTS[z - 1] : -1.0; }
+
+/* quantum (corrfn) and classical (corrfn2) integrands:
+ * crates/refeff-core/src/debye/models.rs:384-399 */
+static double debye_integrand(int classical, double w, double x, double y_inv) {
+    if (w < 1.0e-20) return 2.0 * y_inv;
+    const double f = x > 0.0 ? sin(w * x) / x : w;
+    if (classical) return f * 2.0 * y_inv / w;
+    const double e = exp(-w / y_inv);
+    return f * (1.0 + e) / (1.0 - e);
+}
+
+/* Romberg with Richardson's first step on [0, 1], up to 10 halvings and a
+ * relative tolerance of 1e-5: integrate_debye_romberg, models.rs:402-436. */
+static int debye_romberg(int classical, double x, double y_inv, double* out) {
+    long intervals = 1;
+    double delta = 1.0;
+    double prev_t = (debye_integrand(classical, 0.0, x, y_inv) + debye_integrand(classical, 1.0, x, y_inv)) / 2.0;
+    double prev_x = prev_t;
+    for (int it = 1; it <= 10; ++it) {
+        delta /= 2.0;
+        double mid = 0.0;
+        for (long i = 1; i <= intervals; ++i) mid += debye_integrand(classical, (double)(2 * i - 1) * delta, x, y_inv);
+        const double t = prev_t / 2.0 + delta * mid;
+        const double ex = (4.0 * t - prev_t) / 3.0;
+        double err;
+        if (ex == 0.0) err = prev_x == 0.0 ? 0.0 : INFINITY;
+        else err = fabs((ex - prev_x) / ex);
+        if (err < 1.0e-5) {
+            *out = ex;
+            return ORC_OK;
+        }
+        prev_t = t;
+        prev_x = ex;
+        intervals *= 2;
+    }
+    return ORC_NO_CONVERGENCE;
+}
+
+/* Debye displacement correlation of two atoms a distance rij (Angstrom) apart:
+ * debye_correlation, crates/refeff-core/src/debye/models.rs:337-382
+ * (corrfn for classical = 0, corrfn2 for classical = 1). */
+int orc_debye_correlation(int classical, double rij, double thetad, double tk, int z1, int z2, double rsavg,
+                          double* value) {
+    if (!isfinite(rij) || rij < 0.0 || !isfinite(thetad) || thetad <= 0.0 || !isfinite(tk) || tk < 0.0 ||
+        !isfinite(rsavg) || rsavg <= 0.0)
+        return ORC_INVALID_ARGUMENT;
+    const double m1 = orc_atomic_weight(z1), m2 = orc_atomic_weight(z2);
+    if (m1 < 0.0 || m2 < 0.0) return ORC_INVALID_ARGUMENT;
+    const double y_inv = tk / thetad;
+    const double kd = pow(9.0 * M_PI / 2.0, 1.0 / 3.0) / (rsavg * 0.529177249);
+    const double x = kd * rij;
+    /* (3/2 as f32) * DEBYE_CORRELATION_FACTOR (48.50846 as f32), crates/refeff-core/src/debye.rs:24 */
+    const double factor = (double)1.5f * (double)48.50846f / (thetad * sqrt(m1 * m2));
+    double integral;
+    const int rc = debye_romberg(classical, x, y_inv, &integral);
+    if (rc) return rc;
+    *value = factor * integral;
+    return isfinite(*value) ? ORC_OK : ORC_NON_FINITE;
+}
+
+static double vnorm3(const double* v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* Debye-Waller factor sigma^2 of a path of rows+1 positions (Angstrom, [rows][3])
+ * and atomic numbers: debye_waller_factor, models.rs:176-265 (sigms / sigcl). */
+int orc_debye_waller_factor(int classical, double tk, double thetad, double rsavg, int rows, const double* pos,
+                            const int* z, double* value) {
+    if (rows < 2) return ORC_INVALID_ARGUMENT;
+    for (int i = 0; i < 3 * rows; ++i)
+        if (!isfinite(pos[i])) return ORC_NON_FINITE;
+    for (int i = 0; i < rows; ++i)
+        if (orc_atomic_weight(z[i]) < 0.0) return ORC_INVALID_ARGUMENT;
+    if (!isfinite(tk) || tk < 0.0 || !isfinite(thetad) || thetad <= 0.0 || !isfinite(rsavg) || rsavg <= 0.0)
+        return ORC_INVALID_ARGUMENT;
+    const int nleg = rows - 1;
+    double total = 0.0;
+    for (int il = 1; il <= nleg; ++il)
+        for (int jl = il; jl <= nleg; ++jl) {
+            const int a[4] = {il, il - 1, il, il - 1}, b[4] = {jl, jl - 1, jl - 1, jl};
+            double c[4];
+            for (int q = 0; q < 4; ++q) {
+                const double d[3] = {pos[3 * a[q]] - pos[3 * b[q]], pos[3 * a[q] + 1] - pos[3 * b[q] + 1],
+                                     pos[3 * a[q] + 2] - pos[3 * b[q] + 2]};
+                const int rc = orc_debye_correlation(classical, vnorm3(d), thetad, tk, z[a[q]], z[b[q]], rsavg, &c[q]);
+                if (rc) return rc;
+            }
+            double l1[3], l2[3];
+            for (int t = 0; t < 3; ++t) {
+                l1[t] = pos[3 * il + t] - pos[3 * (il - 1) + t];
+                l2[t] = pos[3 * jl + t] - pos[3 * (jl - 1) + t];
+            }
+            const double n1 = vnorm3(l1), n2 = vnorm3(l2);
+            if (n1 == 0.0 || n2 == 0.0) return ORC_INVALID_ARGUMENT; /* ZeroLengthPathLeg */
+            const double proj = (l1[0] * l2[0] + l1[1] * l2[1] + l1[2] * l2[2]) / (n1 * n2);
+            double contrib = c[0] + c[1] - c[2] - c[3];
+            if (jl != il) contrib *= 2.0;
+            total += contrib * proj;
+        }
+    *value = total / 4.0;
+    return isfinite(*value) ? ORC_OK : ORC_NON_FINITE;
+}
+
+/* FMS pair mean-square displacements of the correlated (model 0) or classical
+ * (model 3) Debye model: the idwopt 0 / 3 branches of
+ * fms_mean_square_displacements_with_metadata, crates/refeff-engine/src/fms.rs:4345-4433,
+ * with fms_debye_pair_sigma (:4849-4877).  sigsqr(atom2, atom1) at
+ * atom2 + n*atom1 = f32(f64(sig2g) + sigma^2 of the path atom1 -> atom2 -> atom1),
+ * zero diagonal.  pos: [n][3] f32 cluster coordinates; z: atomic number per atom. */
+int orc_debye_sigsqr(int model, double tk, double thetad, double rsavg, int n, const float* pos, const int* z,
+                     float sig2g, float* out) {
+    if (model != 0 && model != 3) return ORC_UNSUPPORTED;
+    if (!isfinite(rsavg) || rsavg <= 0.0) return ORC_INVALID_ARGUMENT;
+    for (size_t i = 0; i < (size_t)n * n; ++i) out[i] = 0.0f;
+    for (int a2 = 0; a2 < n; ++a2)
+        for (int a1 = 0; a1 < n; ++a1) {
+            if (a1 == a2) continue;
+            const int order[3] = {a1, a2, a1};
+            double p[9];
+            int zz[3];
+            for (int r = 0; r < 3; ++r) {
+                for (int t = 0; t < 3; ++t) p[3 * r + t] = (double)pos[3 * order[r] + t];
+                zz[r] = z[order[r]];
+            }
+            double s2;
+            const int rc = orc_debye_waller_factor(model == 3, tk, thetad, rsavg, 3, p, zz, &s2);
+            if (rc) return rc;
+            const double tot = (double)sig2g + s2;
+            const float f = (float)tot;
+            if (!isfinite(tot) || tot < 0.0 || !isfinite(f)) return ORC_INVALID_ARGUMENT;
+            out[a2 + (size_t)n * a1] = f;
+        }
+    return ORC_OK;
+}

@@ -371,3 +371,47 @@ def zgetrf(a):
     ipiv = np.zeros(n, np.int32)
     _check(_lib.orc_zgetrf(n, ac.view(np.float64).reshape(-1), ipiv), "zgetrf")
     return ac.T.copy(), ipiv
+
+
+_lib.orc_atomic_weight.restype = C.c_double
+_lib.orc_atomic_weight.argtypes = [C.c_int]
+_lib.orc_debye_correlation.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
+                                       C.POINTER(C.c_double)]
+_lib.orc_debye_waller_factor.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _f64p, _i32p,
+                                         C.POINTER(C.c_double)]
+_lib.orc_debye_sigsqr.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _f32p, _i32p, C.c_float,
+                                  _f32p]
+
+
+def atomic_weight(z):
+    """FEFF atomic weight (amu) of element z, or -1 outside 1..139."""
+    return _lib.orc_atomic_weight(int(z))
+
+
+def debye_correlation(classical, rij, thetad, tk, z1, z2, rsavg):
+    """FEFF corrfn (classical=False) / corrfn2 (classical=True)."""
+    v = C.c_double()
+    _check(_lib.orc_debye_correlation(int(bool(classical)), rij, thetad, tk, z1, z2, rsavg, C.byref(v)),
+           "debye correlation")
+    return v.value
+
+
+def debye_waller_factor(classical, tk, thetad, rsavg, positions, atomic_numbers):
+    """FEFF sigms (classical=False) / sigcl (classical=True) of a path [rows][3]."""
+    p = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+    z = np.ascontiguousarray(atomic_numbers, np.int32)
+    v = C.c_double()
+    _check(_lib.orc_debye_waller_factor(int(bool(classical)), tk, thetad, rsavg, p.shape[0], p.reshape(-1), z,
+                                        C.byref(v)), "debye waller factor")
+    return v.value
+
+
+def debye_sigsqr(model, tk, thetad, rsavg, positions, atomic_numbers, sig2g):
+    """FMS pair sigsqr [atom2, atom1] of the correlated (model 0) or classical
+    (model 3) Debye model over f32 cluster positions [n][3]."""
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    n = p.shape[0]
+    out = np.zeros(n * n, np.float32)
+    _check(_lib.orc_debye_sigsqr(model, tk, thetad, rsavg, n, p.reshape(-1),
+                                 np.ascontiguousarray(atomic_numbers, np.int32), sig2g, out), f"debye sigsqr model={model}")
+    return out.reshape(n, n).T.copy()  # [atom2, atom1]

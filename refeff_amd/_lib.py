@@ -87,6 +87,8 @@ def lib():
     L.refeff_fms_sort_atoms_by_radius.argtypes = [i32, f32p, i32p, f64p]
     L.refeff_fms_yprep_cluster.argtypes = [i32, f32p, i32p, i32, C.c_float, i32, f32p, i32p, i32p, i32p, i32p]
     L.refeff_fms_sort_representative_atoms.argtypes = [i32, i32, i32, f32p, i32p, i32p]
+    L.refeff_fms_debye_sigsqr.argtypes = [i32, C.c_double, C.c_double, C.c_double, i32, f32p, i32p, C.c_float, f32p,
+                                          i32p]
     L.refeff_fms_state_kets.argtypes = [i32, i32, i32p, i32, i32p, i32, i32p, i32p, i32p, i32p, i32p, i32p]
     L.refeff_fms_plan_create.argtypes = [C.POINTER(PlanDesc), C.POINTER(vp)]
     L.refeff_fms_plan_destroy.argtypes = [vp]

@@ -108,9 +108,84 @@ void validate_controls(const rio::FmsInp& in) {
         stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires positive finite toler1");
     if (!std::isfinite(in.toler2) || in.toler2 < 0.0)
         stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative toler2");
-    if (in.idwopt >= 0)
+    if (in.idwopt == 1 || in.idwopt == 2 || in.idwopt == 5)
         stage_fail(REFEFF_FMS_UNSUPPORTED, "Debye-Waller model idwopt=" + std::to_string(in.idwopt) +
-                                               " is not provided; only the global SIG2 (idwopt < 0)");
+                                               " is not provided; only the global SIG2 (idwopt < 0), the correlated "
+                                               "(0) and classical (3) Debye models and sig2.dat (4)");
+}
+
+// Pair sigsqr(atom2, atom1) at atom2 + nat*atom1 for the cluster atoms:
+// fms_mean_square_displacements_with_metadata, crates/refeff-engine/src/fms.rs:4345-4433.
+//   idwopt < 0: sig2g for every pair;
+//   idwopt 4:   sig2g + the sig2.dat row of pair (atom1 < atom2, atom1 outer), symmetric
+//               (fms_sig2_dat_mean_square_displacements, :4751-4776);
+//   idwopt 0/3: sig2g + the correlated / classical Debye-Waller factor of the path
+//               atom1 -> atom2 -> atom1, on the GPU (refeff_fms_debye_sigsqr), after
+//               validate_fms_debye_damping_metadata (:4879-4905) and
+//               fms_atom_atomic_number (:4913-4945).
+std::vector<float> pair_sigsqr(const std::string& dir, const rio::FmsInp& in, const rio::PhaseBin& phase, int nat,
+                               const std::vector<float>& cpos, const std::vector<int32_t>& cpot) {
+    const float sig2g = narrow_nonnegative(in.sig2g, "FMS SIG2");
+    std::vector<float> sig(static_cast<size_t>(nat) * nat, 0.0f);
+    if (in.idwopt < 0) {
+        const float pair = narrow_nonnegative(static_cast<double>(sig2g) + 0.0, "FMS pair sigsqr");
+        for (int a2 = 0; a2 < nat; ++a2)
+            for (int a1 = 0; a1 < nat; ++a1)
+                if (a1 != a2) sig[a2 + static_cast<size_t>(nat) * a1] = pair;
+        return sig;
+    }
+    if (in.idwopt == 4) {
+        const size_t pairs = static_cast<size_t>(nat) * (nat > 0 ? nat - 1 : 0) / 2;
+        if (pairs == 0) return sig;
+        const std::string path = join(dir, "sig2.dat");
+        const std::vector<double> v = rio::parse_sig2_dat(rio::read_file(path), pairs, path);
+        size_t idx = 0;
+        for (int a1 = 0; a1 + 1 < nat; ++a1)
+            for (int a2 = a1 + 1; a2 < nat; ++a2) {
+                const float t = narrow_nonnegative(static_cast<double>(sig2g) + v[idx++], "FMS sig2.dat pair sigsqr");
+                sig[a2 + static_cast<size_t>(nat) * a1] = t;
+                sig[a1 + static_cast<size_t>(nat) * a2] = t;
+            }
+        return sig;
+    }
+    if (in.idwopt != 0 && in.idwopt != 3) {  // refeff fails on the first pair it damps
+        if (nat < 2) return sig;
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation received unexpected idwopt=" +
+                                                    std::to_string(in.idwopt) + " Debye-Waller damping");
+    }
+    const double rnrmav = phase.scalars[0];
+    if (!std::isfinite(rnrmav) || rnrmav <= 0.0)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT,
+                   "FMS Debye damping requires a positive finite average Norman radius, got " + std::to_string(rnrmav));
+    for (size_t ip = 0; ip < phase.potentials.size(); ++ip)
+        if (phase.potentials[ip].atomic_number == 0)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT,
+                       "FMS Debye damping requires a positive atomic number for potential " + std::to_string(ip));
+    std::vector<int32_t> z(nat);
+    for (int i = 0; i < nat; ++i) {
+        if (cpot[i] < 0)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS atom potential must be nonnegative for Debye damping, got " +
+                                                        std::to_string(cpot[i]));
+        if (cpot[i] >= static_cast<int>(phase.potentials.size()))
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS atom potential " + std::to_string(cpot[i]) +
+                                                        " is outside phase.bin potential table");
+        z[i] = phase.potentials[cpot[i]].atomic_number;
+    }
+    const char* what = in.idwopt == 0 ? "FMS correlated Debye-Waller pair damping"
+                                      : "FMS classical Debye-Waller pair damping";
+    int32_t bad[2] = {-1, -1};
+    const int rc = refeff_fms_debye_sigsqr(in.idwopt, in.tk, in.thetad, rnrmav, nat, cpos.data(), z.data(), sig2g,
+                                           sig.data(), bad);
+    if (rc) {
+        std::string msg = what;
+        if (bad[0] >= 0)
+            msg += " failed for atoms " + std::to_string(bad[0]) + " and " + std::to_string(bad[1]);
+        else
+            msg += " rejected its inputs (tk, thetad, coordinates or atomic numbers)";
+        if (rc == REFEFF_FMS_NO_CONVERGENCE) msg += ": Romberg integration did not converge";
+        stage_fail(rc, msg);
+    }
+    return sig;
 }
 
 struct FmsStageResult {
@@ -123,8 +198,8 @@ struct FmsStageResult {
 
 // gg sections from phase.bin / geom.dat / global.inp: refeff
 // build_fms_source_outputs, crates/refeff-engine/src/fms.rs:3549-3695, with
-// fms_atoms_from_geom (:4286-4322), the idwopt < 0 branch of
-// fms_mean_square_displacements (:4345-4433), fms_wave_numbers (:4972-4985, kept
+// fms_atoms_from_geom (:4286-4322), fms_mean_square_displacements (:4345-4433,
+// idwopt < 0, 0, 3 and 4), fms_wave_numbers (:4972-4985, kept
 // in FP64), fms_phase_shifts_for_energy (:5002-5054, kept in FP64) and
 // fms_gg_section_values (:2004-2047).
 std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsInp& in, int solver,
@@ -186,13 +261,7 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
         if (nspu <= 0 || nspu < phase.spin_count)
             stage_fail(REFEFF_FMS_INVALID_ARGUMENT, ".dimensions.dat nspu must be positive and at least the phase.bin spin count");
     }
-    // sigsqr(atom2, atom1) = sig2g for atom1 != atom2 (idwopt < 0)
-    const float sig2g = narrow_nonnegative(in.sig2g, "FMS SIG2");
-    const float pair = narrow_nonnegative(static_cast<double>(sig2g) + 0.0, "FMS pair sigsqr");
-    std::vector<float> sig(static_cast<size_t>(nat) * nat, 0.0f);
-    for (int a2 = 0; a2 < nat; ++a2)
-        for (int a1 = 0; a1 < nat; ++a1)
-            if (a1 != a2) sig[a2 + static_cast<size_t>(nat) * a1] = pair;
+    const std::vector<float> sig = pair_sigsqr(dir, in, phase, nat, cpos, cpot);
 
     refeff_fms_plan_desc d{};
     d.lfms = in.do_fms;
@@ -929,6 +998,15 @@ int refeff_io_gtr_dat_parse(const char* text, int32_t* count, double* energy, do
         *count = static_cast<int32_t>(en.size());
         if (energy) cput(en, energy);
         if (trace) cput(tr, trace);
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_sig2_dat_parse(const char* text, size_t pair_count, double* values) {
+    if (!text || (pair_count && !values)) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const std::vector<double> v = rio::parse_sig2_dat(text, pair_count, "sig2.dat");
+        std::copy(v.begin(), v.end(), values);
         return REFEFF_FMS_OK;
     });
 }

@@ -704,6 +704,37 @@ int parse_dimensions_nspu(const std::string& text) {
     return to_int(f[3], "nspu");
 }
 
+std::vector<double> parse_sig2_dat(const std::string& text, size_t pair_count, const std::string& label) {
+    if (!valid_utf8(text)) fail("failed to read " + label + ": stream did not contain valid UTF-8");
+    std::vector<double> values;
+    values.reserve(pair_count);
+    const std::vector<std::string> lines = split_lines(text);
+    for (size_t li = 0; li < lines.size(); ++li) {
+        const std::string line = trim(lines[li]);
+        if (line.empty() || line[0] == '#' || line[0] == '!') continue;
+        if (values.size() == pair_count) break;
+        const std::string where = label + " line " + std::to_string(li + 1);
+        const std::vector<std::string> f = fields(line);
+        if (f.size() < 4)
+            fail(where + " has " + std::to_string(f.size()) + " field(s), expected at least 4 for sig2.dat pair data");
+        long long idx;
+        if (!parse_long(f[0], idx) || idx < std::numeric_limits<int32_t>::min() || idx > std::numeric_limits<int32_t>::max())
+            fail("failed to parse first sig2.dat atom index in " + where);
+        if (!parse_long(f[1], idx) || idx < std::numeric_limits<int32_t>::min() || idx > std::numeric_limits<int32_t>::max())
+            fail("failed to parse second sig2.dat atom index in " + where);
+        double pair, aux;
+        if (!parse_double(f[2], pair)) fail("failed to parse sig2.dat pair damping in " + where);
+        if (!parse_double(f[3], aux)) fail("failed to parse sig2.dat auxiliary value in " + where);
+        if (!std::isfinite(pair) || pair < 0.0)
+            fail(where + " has non-finite or negative sig2.dat pair damping " + f[2]);
+        values.push_back(pair);
+    }
+    if (values.size() != pair_count)
+        fail(label + " contains " + std::to_string(values.size()) + " sig2.dat pair row(s), expected " +
+             std::to_string(pair_count));
+    return values;
+}
+
 // ---------------------------------------------------------------------------
 // gg.dat / gg.bin, crates/refeff-io/src/gg_dat.rs:155-233, 410-640
 // ---------------------------------------------------------------------------

@@ -101,6 +101,12 @@ GlobalInp parse_global_inp(const std::string& text);
 // .dimensions.dat, crates/refeff-io/src/structure_output.rs:20-30, 428-441.
 int parse_dimensions_nspu(const std::string& text);
 
+// sig2.dat pair damping rows (idwopt = 4): the third field of the first
+// pair_count data lines ("#" / "!" comments and blank lines skipped), each a
+// finite value >= 0.  refeff fms_sig2_dat_pair_values,
+// crates/refeff-engine/src/fms.rs:4778-4847.  label names the file in messages.
+std::vector<double> parse_sig2_dat(const std::string& text, size_t pair_count, const std::string& label);
+
 // gg.dat / gg.bin sections, crates/refeff-io/src/gg_dat.rs:155-233, 410-640.
 struct GgSection {
     int section_number = 0, rows = 0, cols = 0;

@@ -20,6 +20,7 @@ __all__ = [
     "SOLVER_GGTF", "sort_atoms_by_radius",
     "fms_yprep_cluster", "sort_representative_atoms", "construct_state_kets", "FmsRealSpacePlan",
     "fms_real_space_plan", "fms_real_space_spectrum", "transition_b_matrix_averaged", "mkgtr_green_trace",
+    "debye_mean_square_displacements",
 ]
 
 
@@ -95,6 +96,27 @@ def construct_state_kets(spin_channels: int, atom_potentials, potential_lmax, gl
                                       C.byref(cnt), _i32p(a), _i32p(l), _i32p(m), _i32p(s), _i32p(rep)),
           "construct_state_kets")
     return a, l, m, s, [int(r) if r >= 0 else None for r in rep]
+
+
+def debye_mean_square_displacements(model: int, temperature: float, debye_temperature: float,
+                                    average_norman_radius: float, positions, atomic_numbers, sig2g: float):
+    """Pair sigsqr of the correlated (model 0, FEFF sigms) or classical (model 3,
+    FEFF sigcl) Debye model, computed on the GPU: returns float32 [n, n] with
+    [atom2, atom1] = f32(sig2g + sigma^2 of the path atom1 -> atom2 -> atom1)
+    (refeff engine fms.rs:4345-4433, 4849-4877; core debye/models.rs:138-436).
+    Raises FmsError with the first failing (atom1, atom2) pair in the detail."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    z = np.ascontiguousarray(atomic_numbers, dtype=np.int32)
+    n = pos.shape[0]
+    if z.shape != (n,):
+        raise FmsError(1, "debye_mean_square_displacements", "one atomic number per atom")
+    out = np.zeros(max(1, n * n), np.float32)
+    bad = np.full(2, -1, np.int32)
+    rc = lib().refeff_fms_debye_sigsqr(model, float(temperature), float(debye_temperature),
+                                       float(average_norman_radius), n, _f32p(pos), _i32p(z), float(sig2g),
+                                       _f32p(out), _i32p(bad))
+    check(rc, "debye_mean_square_displacements", f"first failing pair (atom1, atom2) = ({bad[0]}, {bad[1]})")
+    return out[:n * n].reshape(n, n).T.copy()
 
 
 class FmsRealSpacePlan:

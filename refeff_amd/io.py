@@ -83,6 +83,7 @@ def _L():
     L.refeff_io_gg_slice_bin_parse.argtypes = [C.c_char_p, sz, i32p, vp]
     L.refeff_io_gg_diag_bin_bytes.argtypes = [i32, i32, i32, vp, C.c_char_p, sz, szp]
     L.refeff_io_gg_diag_bin_parse.argtypes = [C.c_char_p, sz, i32p, vp]
+    L.refeff_io_sig2_dat_parse.argtypes = [cp, sz, vp]
     _bound = True
     return L
 
@@ -477,3 +478,13 @@ def run_mkgtr_in_dir(work_dir: str) -> int:
     if rc != 0:
         raise CodecError(rc, "run_mkgtr_in_dir", msg.value.decode(errors="replace"))
     return n.value
+
+
+# --- sig2.dat (engine fms.rs:4778-4847) ---------------------------------------
+
+def parse_sig2_dat(text: str, pair_count: int) -> np.ndarray:
+    """fms_sig2_dat_pair_values: the pair damping column of the first pair_count
+    data rows of sig2.dat, as float64."""
+    out = np.zeros(max(1, pair_count), np.float64)
+    _check(_L().refeff_io_sig2_dat_parse(_enc(text), pair_count, _ptr(out)), "sig2.dat")
+    return out[:pair_count]

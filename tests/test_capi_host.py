@@ -100,3 +100,34 @@ def test_bmat_matches_oracle(ihole, multipole, spin, nsp):
     olind, ob = orc.bmat_averaged(orc.core_hole_kappa(ihole), multipole, 3, spin, nsp)
     assert list(lind) == list(olind)
     assert np.array_equal(b, ob)
+
+
+def test_debye_sigsqr_validates_on_the_host():
+    """refeff_fms_debye_sigsqr rejects what refeff's Debye path rejects before any
+    device work (engine fms.rs:4345-4433, 4879-4905; core debye/models.rs:198-202,
+    272-298), and a cluster without pairs needs no path at all."""
+    pos = np.array([[0, 0, 0], [2.55, 0, 0]], np.float32)
+    z = [29, 29]
+    cases = [
+        (dict(model=1), "UNSUPPORTED"),                   # spring models are not this entry point's
+        (dict(model=5), "UNSUPPORTED"),
+        (dict(average_norman_radius=0.0), "INVALID_ARGUMENT"),
+        (dict(average_norman_radius=float("nan")), "INVALID_ARGUMENT"),
+        (dict(temperature=-1.0), "INVALID_ARGUMENT"),
+        (dict(debye_temperature=0.0), "INVALID_ARGUMENT"),
+        (dict(atomic_numbers=[29, 0]), "INVALID_ARGUMENT"),
+        (dict(atomic_numbers=[29, 140]), "INVALID_ARGUMENT"),
+        (dict(positions=np.array([[0, 0, 0], [np.inf, 0, 0]], np.float32)), "NON_FINITE"),
+        (dict(sig2g=-1e-3), "INVALID_ARGUMENT"),
+    ]
+    base = dict(model=0, temperature=300.0, debye_temperature=400.0, average_norman_radius=2.7, positions=pos,
+                atomic_numbers=z, sig2g=0.0)
+    for change, name in cases:
+        args = {**base, **change}
+        with pytest.raises(fms.FmsError) as e:
+            fms.debye_mean_square_displacements(**args)
+        assert e.value.name == name, (change, e.value.name)
+    one = fms.debye_mean_square_displacements(**{**base, "positions": pos[:1], "atomic_numbers": [29],
+                                                 "temperature": -5.0})
+    assert one.shape == (1, 1) and one[0, 0] == 0.0
+    assert fms.debye_mean_square_displacements(**{**base, "positions": pos[:0], "atomic_numbers": []}).shape == (0, 0)

@@ -6,7 +6,8 @@ tests/test_io_codecs.py against refeff-io's known answers).  The expected
 values come from the FP64 oracle on the same inputs: the oracle's own
 yprep/sortat cluster from the same f32 geometry, k = sqrt(2 (E - E_ref)) / a0
 and the phase shifts from the arrays written into phase.bin (PAD at npack 8
-keeps ~1e-12 relative), σ² = sig2g between distinct atoms.  gg.bin's E20.10
+keeps ~1e-12 relative), σ² = sig2g between distinct atoms (or the oracle's
+Debye / sig2.dat σ² for idwopt 0, 3, 4).  gg.bin's E20.10
 fields carry 10 significant digits, so the gate is the north-star 1e-8."""
 import sys
 
@@ -48,7 +49,8 @@ nq,    imdff,   qaverage,   mixdff
 """
 
 
-def make_workdir(d, radius=4.6, ne=7, sig2g=0.003, save_gg_slice=False, lmaxph=(3, 3)):
+def make_workdir(d, radius=4.6, ne=7, sig2g=0.003, save_gg_slice=False, lmaxph=(3, 3), idwopt=-1, tk=0.0,
+                 thetad=0.0, z=(29, 29)):
     pts = W.lattice(W.FCC, 3.615, radius + 1.0)  # geom.dat holds more atoms than the FMS sphere
     order = np.argsort(np.linalg.norm(pts, axis=1), kind="stable")
     pts = pts[order]
@@ -61,13 +63,14 @@ def make_workdir(d, radius=4.6, ne=7, sig2g=0.003, save_gg_slice=False, lmaxph=(
     eref = np.full((ne, 1), 0.1 + 0.0j)
     k = np.sqrt(2.0 * (em - eref[:, 0])) / A0
     ph_model = W.synthetic_phases(k, 3, np.array([3, 3]))     # [ne, 1, 7, 2], l~ = l + 3
-    pots = [io.PhaseBinPotential(3, 29, "Cu", ph_model[:, 0, :, ip][:, :, None].copy()) for ip in range(2)]
+    pots = [io.PhaseBinPotential(3, z[ip], "Cu" if z[ip] == 29 else "X", ph_model[:, 0, :, ip][:, :, None].copy())
+            for ip in range(2)]
     rng = np.random.default_rng(11)
     rkk = (rng.normal(size=(ne, 1, 8, 1)) + 1j * rng.normal(size=(ne, 1, 8, 1))) * 0.3
     pb = io.PhaseBinData(1, ne, ne, 0, 1, 1, 8, 8, 8, 1, (2.6, -0.14, -0.14), em, eref, pots, rkk)
     (d / "phase.bin").write_text(io.phase_bin_string(pb))
-    fin = io.FmsInput(mfms=1, idwopt=-1, minv=0, rfms2=radius, rdirec=10.0, toler1=0.001, toler2=0.001, tk=0.0,
-                      thetad=0.0, sig2g=sig2g, lmaxph=list(lmaxph), decomposition_channels=-1,
+    fin = io.FmsInput(mfms=1, idwopt=idwopt, minv=0, rfms2=radius, rdirec=10.0, toler1=0.001, toler2=0.001, tk=tk,
+                      thetad=thetad, sig2g=sig2g, lmaxph=list(lmaxph), decomposition_channels=-1,
                       save_gg_slice=save_gg_slice,
                       do_fms=1)
     (d / "fms.inp").write_text(io.fms_input_string(fin))
@@ -75,12 +78,19 @@ def make_workdir(d, radius=4.6, ne=7, sig2g=0.003, save_gg_slice=False, lmaxph=(
     return pts, iph, k, ph_model, rkk, em, radius, sig2g
 
 
-def oracle_gg(pts, iph, k, ph, radius, sig2g):
+def oracle_cluster(pts, iph, radius):
     pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(radius), len(iph))
     pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
+    return pos, pot
+
+
+def oracle_gg(pts, iph, k, ph, radius, sig2g, sig=None):
+    """sig: pair sigsqr [atom2, atom1] of the prepared cluster; None = sig2g between distinct atoms."""
+    pos, pot = oracle_cluster(pts, iph, radius)
     nat = len(pot)
-    sig = np.full((nat, nat), np.float32(sig2g), np.float32)
-    np.fill_diagonal(sig, 0.0)
+    if sig is None:
+        sig = np.full((nat, nat), np.float32(sig2g), np.float32)
+        np.fill_diagonal(sig, 0.0)
     plan = orc.Plan(pos, pot, nsp=1, lfms=1, max_pot=1, global_lmax=3, raw_lmax=[3, 3], rdirec=10.0, sigsqr=sig)
     sc, st = plan.spectrum(k.reshape(-1, 1), ph, threads=8)
     assert not st.any()
@@ -173,3 +183,54 @@ def test_module_fms_save_gg_slice_needs_fixed_width_atom_blocks(tmp_path):
     with pytest.raises(io.CodecError) as e:
         io.run_fms_in_dir(tmp_path, 0)
     assert "fixed-width atom blocks" in str(e.value)
+
+
+DEBYE_LOG = {0: "Applying Debye-Waller factors using a Correlated Debye model.",
+             3: "Applying Debye-Waller factors using the Classical Debye model.",
+             4: "Applying Debye-Waller factors using the sig.dat file."}
+
+
+@pytest.mark.parametrize("idwopt", [0, 3, 4])
+def test_module_fms_debye_models_match_oracle(tmp_path, idwopt):
+    """Pair damping beyond the global SIG2 (engine fms.rs:4345-4433): correlated (0)
+    and classical (3) Debye sigsqr computed on the GPU from phase.bin's average
+    Norman radius and atomic numbers, and sig2.dat rows (4); the gg written must
+    match the oracle's solve with the oracle's sigsqr (refeff's module tests,
+    engine fms/tests.rs:1382-1450, check only that damping changes gg)."""
+    banner(f"module fms, idwopt {idwopt}")
+    z = (29, 8)
+    pts, iph, k, ph, rkk, em, radius, sig2g = make_workdir(tmp_path, idwopt=idwopt, tk=300.0, thetad=315.0, z=z)
+    pos, pot = oracle_cluster(pts, iph, radius)
+    nat = len(pot)
+    if idwopt == 4:
+        rng = np.random.default_rng(17)
+        vals = rng.uniform(0.0, 0.01, nat * (nat - 1) // 2)
+        rows, sig, i = ["# atom1 atom2 sig2 amplitude"], np.zeros((nat, nat), np.float32), 0
+        for a1 in range(nat - 1):
+            for a2 in range(a1 + 1, nat):
+                rows.append(f"{a1 + 1} {a2 + 1} {float(vals[i])!r} 1.0")
+                sig[a2, a1] = sig[a1, a2] = np.float32(float(np.float32(sig2g)) + vals[i])
+                i += 1
+        (tmp_path / "sig2.dat").write_text("\n".join(rows) + "\n")
+    else:
+        rnrmav = io.parse_phase_bin((tmp_path / "phase.bin").read_text()).scalars[0]  # 2.6 through PAD(8)
+        sig = orc.debye_sigsqr(idwopt, 300.0, 315.0, rnrmav, pos, np.array(z, np.int32)[pot], np.float32(sig2g))
+    assert io.run_fms_in_dir(tmp_path, 0) == 3
+    sections = io.parse_gg_bin((tmp_path / "gg.bin").read_text())
+    ref, _ = oracle_gg(pts, iph, k, ph, radius, sig2g, sig=sig)
+    flat, _ = oracle_gg(pts, iph, k, ph, radius, sig2g)
+    for e, s in enumerate(sections):
+        err = np.linalg.norm(s.values - ref[e]) / np.linalg.norm(ref[e])
+        assert err <= 1e-8, (e, err)
+        assert np.linalg.norm(s.values - flat[e]) > 1e-6 * np.linalg.norm(flat[e])  # the damping acts
+    assert DEBYE_LOG[idwopt] in (tmp_path / "log3.dat").read_text().splitlines()
+    banner(f"gg.bin with idwopt {idwopt} matches oracle, cluster {nat} atoms")
+
+
+def test_module_fms_sig2_dat_must_cover_every_pair(tmp_path):
+    banner("module fms, short sig2.dat")
+    make_workdir(tmp_path, idwopt=4, tk=300.0, thetad=315.0)
+    (tmp_path / "sig2.dat").write_text("1 2 0.025 1.4\n")  # one row, the cluster has many pairs
+    with pytest.raises(io.CodecError) as e:
+        io.run_fms_in_dir(tmp_path, 0)
+    assert "sig2.dat pair row(s), expected" in str(e.value)

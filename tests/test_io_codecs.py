@@ -503,13 +503,16 @@ def test_module_stage_re_emits_a_cached_gg_canonically(tmp_path):
     assert "Done with module: FMS." in (tmp_path / "log3.dat").read_text()
 
 
-def test_module_stage_rejects_unprovided_debye_models_before_solving(tmp_path):
-    _write(tmp_path, "fms.inp", LEGACY_FMS_INP.replace("   1  -1   0", "   1   0   0"))
+@pytest.mark.parametrize("idwopt", [1, 2, 5])
+def test_module_stage_rejects_unprovided_debye_models_before_solving(tmp_path, idwopt):
+    """Spring equation of motion (1), recursion (2) and DMDW (5) are not provided;
+    the correlated / classical Debye models (0, 3) and sig2.dat (4) are."""
+    _write(tmp_path, "fms.inp", LEGACY_FMS_INP.replace("   1  -1   0", f"   1   {idwopt}   0"))
     for name in ("phase.bin", "geom.dat", "global.inp"):
         _write(tmp_path, name, "")
     with pytest.raises(CodecError) as e:
         io.run_fms_in_dir(tmp_path)
-    assert e.value.name == "UNSUPPORTED" and "idwopt=0" in str(e.value)
+    assert e.value.name == "UNSUPPORTED" and f"idwopt={idwopt}" in str(e.value)
 
 
 # --- gg_slice.bin / gg_diag.bin: crates/refeff-io/src/rhorrp_gg_bin.rs tests --
@@ -609,3 +612,43 @@ def test_module_stage_re_emits_cached_slice_and_diag(tmp_path):
     assert io.run_fms_in_dir(tmp_path) == 4  # gg.bin, gg_slice.bin, gg_diag.bin, log3.dat
     assert (tmp_path / "gg_slice.bin").read_bytes() == FEFF_GG_SLICE
     assert (tmp_path / "gg_diag.bin").read_bytes() == FEFF_GG_DIAG
+
+
+# --------------------------------------------------------------------------
+# sig2.dat pair rows (idwopt = 4), refeff engine fms.rs:4778-4847; the module
+# test writes "1 2 0.025 1.4" for a two-atom pair (engine fms/tests.rs:1429-1450)
+# --------------------------------------------------------------------------
+
+def test_sig2_dat_reads_the_third_field_of_each_pair_row():
+    text = "# comment\n! also a comment\n\n  1 2 0.025 1.4\r\n1 3 0.0  0.0 extra\n2 3 3e-3 -1\n"
+    assert io.parse_sig2_dat("1 2 0.025 1.4\n", 1).tolist() == [0.025]
+    assert io.parse_sig2_dat(text, 3).tolist() == [0.025, 0.0, 3e-3]
+
+
+def test_sig2_dat_stops_after_the_pairs_it_needs():
+    # rows past pair_count are not parsed at all, even malformed ones
+    assert io.parse_sig2_dat("1 2 0.5 1\n1 3 0.25 1\nnot a row\n", 2).tolist() == [0.5, 0.25]
+
+
+@pytest.mark.parametrize("text,count,needle", [
+    ("1 2 0.5\n", 1, "has 3 field(s), expected at least 4"),
+    ("a 2 0.5 1\n", 1, "first sig2.dat atom index"),
+    ("1 2.0 0.5 1\n", 1, "second sig2.dat atom index"),
+    ("1 2 x 1\n", 1, "sig2.dat pair damping"),
+    ("1 2 0.5 1d0\n", 1, "auxiliary value"),
+    ("1 2 -0.5 1\n", 1, "non-finite or negative"),
+    ("1 2 nan 1\n", 1, "non-finite or negative"),
+    ("1 2 inf 1\n", 1, "non-finite or negative"),
+    ("1 2 0.5 1\n", 3, "contains 1 sig2.dat pair row(s), expected 3"),
+    ("# only comments\n", 1, "contains 0 sig2.dat pair row(s), expected 1"),
+])
+def test_sig2_dat_rejects_malformed_rows(text, count, needle):
+    with pytest.raises(CodecError) as e:
+        io.parse_sig2_dat(text, count)
+    assert needle in str(e.value), str(e.value)
+
+
+def test_sig2_dat_names_the_line():
+    with pytest.raises(CodecError) as e:
+        io.parse_sig2_dat("# header\n1 2 0.5 1\n1 3 -1 1\n", 2)
+    assert "sig2.dat line 3" in str(e.value)

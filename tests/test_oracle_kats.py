@@ -580,3 +580,112 @@ def test_bmat_averaged_k_edge():
     for mm in range(-1, 2):
         # spin fold (ispin=0) adds the spin-down diagonal into spin-up
         assert abs(bb[mm + 3, 0, t, mm + 3, 0, t] - 2 * (-0.5 / 3 / 3)) < 1e-15
+
+
+# --------------------------------------------------------------------------
+# Debye-Waller pair damping: refeff debye/tests/models.rs:82-188 (asserted
+# there at |d| < 1e-18, support.rs:3-9); FEFF atomic weights, atomic/tables.rs:49-61
+# --------------------------------------------------------------------------
+
+DEBYE_CORRELATION_KATS = [  # (classical, rij, thetad, tk, z1, z2, rsavg, value), models.rs:83-96
+    (False, 0.0, 400.0, 300.0, 29, 29, 2.7, 4.501_999_849_393_054e-3),
+    (False, 2.55, 400.0, 300.0, 29, 29, 2.7, 1.691_640_883_386_128e-3),
+    (False, 1.91, 650.0, 120.0, 29, 8, 2.3, 7.447_746_368_694_431e-4),
+    (True, 0.0, 400.0, 300.0, 29, 29, 2.7, 4.293_628_582_101_32e-3),
+    (True, 2.55, 400.0, 300.0, 29, 29, 2.7, 1.685_437_153_407_153e-3),
+    (True, 1.91, 650.0, 120.0, 29, 8, 2.3, 6.129_399_740_209_465e-4),
+]
+
+
+@pytest.mark.parametrize("classical,rij,thetad,tk,z1,z2,rsavg,want", DEBYE_CORRELATION_KATS)
+def test_debye_correlation_kats(classical, rij, thetad, tk, z1, z2, rsavg, want):
+    assert abs(orc.debye_correlation(classical, rij, thetad, tk, z1, z2, rsavg) - want) < 1.0e-18
+
+
+def test_debye_waller_factor_kats():
+    """models.rs:116-169: the 2-leg Cu path and a 3-leg Cu-O-Cu triangle."""
+    cu = [[0.0, 0.0, 0.0], [2.55, 0.0, 0.0], [0.0, 0.0, 0.0]]
+    tri = [[0.0, 0.0, 0.0], [1.91, 0.25, 0.10], [2.60, 1.40, -0.20], [0.0, 0.0, 0.0]]
+    for classical, want in [(False, 5.620_717_932_013_852e-3), (True, 5.216_382_857_388_334e-3)]:
+        assert abs(orc.debye_waller_factor(classical, 300.0, 400.0, 2.7, cu, [29, 29, 29]) - want) < 1.0e-18
+    for classical, want in [(False, 2.623_124_881_997_499_5e-3), (True, 1.796_449_763_322_294e-3)]:
+        assert abs(orc.debye_waller_factor(classical, 180.0, 650.0, 2.3, tri, [29, 8, 29, 29]) - want) < 1.0e-18
+
+
+def test_debye_rejects_invalid_inputs():
+    """models.rs:99-113, 172-188: negative distance / temperature, Z = 0, zero-length leg."""
+    for args in [(False, -1.0, 400.0, 300.0, 29, 29, 2.7), (False, 1.0, 400.0, -1.0, 29, 29, 2.7),
+                 (True, 1.0, 400.0, 300.0, 29, 0, 2.7)]:
+        with pytest.raises(orc.OracleError):
+            orc.debye_correlation(*args)
+    with pytest.raises(orc.OracleError):
+        orc.debye_waller_factor(False, 300.0, 400.0, 2.7, [[0, 0, 0], [0, 0, 0]], [29, 29])
+
+
+def test_atomic_weights_table():
+    """FEFF's table (atomic/tables.rs:49-61): known elements, ends and range."""
+    f32 = lambda v: float(np.float32(v))
+    assert orc.atomic_weight(1) == f32(1.0079) and orc.atomic_weight(8) == f32(15.999)
+    assert orc.atomic_weight(29) == f32(63.55) and orc.atomic_weight(92) == f32(238.03)
+    assert orc.atomic_weight(139) == f32(392.0)
+    assert orc.atomic_weight(0) < 0 and orc.atomic_weight(140) < 0
+    w = [orc.atomic_weight(z) for z in range(1, 140)]
+    # the table rises with Z except at the physical inversions Ar/K, Co/Ni, Te/I,
+    # Th/Pa, U/Np, Pu/Am and FEFF's 270/269 for Bh/Hs: a shifted entry breaks this
+    inversions = [(z, z + 1) for z, (a, b) in enumerate(zip(w, w[1:]), start=1) if b < a]
+    assert inversions == [(18, 19), (27, 28), (52, 53), (90, 91), (92, 93), (94, 95), (107, 108)]
+
+
+DEBYE_KD = (9.0 * math.pi / 2.0) ** (1.0 / 3.0)
+
+
+def _debye_factor(thetad, z1, z2):
+    return 1.5 * float(np.float32(48.50846)) / (thetad * math.sqrt(orc.atomic_weight(z1) * orc.atomic_weight(z2)))
+
+
+@pytest.mark.parametrize("rij", [0.0, 0.5, 2.55, 5.0, 9.0])
+def test_debye_correlations_match_closed_forms(rij):
+    """The classical integral is 2 y Si(x) / x (2 y at x = 0), the quantum one
+    int_0^1 sin(w x) / x coth(w / 2y) dw: scipy's sici / quad, to the Romberg tolerance."""
+    from scipy.integrate import quad
+    from scipy.special import sici
+    thetad, tk, rs = 315.0, 300.0, 2.6
+    y = tk / thetad
+    x = DEBYE_KD / (rs * 0.529177249) * rij
+    fac = _debye_factor(thetad, 29, 8)
+    classical = fac * 2.0 * y * (sici(x)[0] / x if x > 0 else 1.0)
+    kern = (lambda w: math.sin(w * x) / x) if x > 0 else (lambda w: w)
+    quantum = fac * quad(lambda w: kern(w) / math.tanh(w / (2.0 * y)) if w > 0 else 2.0 * y, 0.0, 1.0,
+                         epsabs=0.0, epsrel=1e-13, limit=200)[0]
+    assert abs(orc.debye_correlation(True, rij, thetad, tk, 29, 8, rs) / classical - 1.0) < 1e-6
+    assert abs(orc.debye_correlation(False, rij, thetad, tk, 29, 8, rs) / quantum - 1.0) < 1e-6
+
+
+def test_debye_romberg_gives_up_on_far_pairs():
+    """sin(w x) at x ~ 700 needs more than 2^10 panels: IntegrationDidNotConverge."""
+    with pytest.raises(orc.OracleError) as e:
+        orc.debye_correlation(False, 500.0, 315.0, 300.0, 29, 8, 2.6)
+    assert e.value.code == 13
+
+
+@pytest.mark.parametrize("model", [0, 3])
+def test_debye_sigsqr_is_the_two_leg_path_closed_form(model):
+    """For the path a -> b -> a the leg sum reduces to c_aa + c_bb - 2 c_ab; the
+    pair table is [atom2, atom1] = f32(sig2g + that), zero on the diagonal."""
+    rng = np.random.default_rng(3)
+    pos = (rng.normal(size=(6, 3)) * 2.0).astype(np.float32)
+    z = np.array([29, 8, 29, 26, 8, 79], np.int32)
+    tk, thetad, rs, s2g = 250.0, 330.0, 2.4, np.float32(0.002)
+    sig = orc.debye_sigsqr(model, tk, thetad, rs, pos, z, s2g)
+    cl = model == 3
+    for a1 in range(6):
+        assert sig[a1, a1] == 0.0
+        for a2 in range(6):
+            if a1 == a2:
+                continue
+            r = float(np.linalg.norm(pos[a1].astype(np.float64) - pos[a2].astype(np.float64)))
+            caa = orc.debye_correlation(cl, 0.0, thetad, tk, int(z[a1]), int(z[a1]), rs)
+            cbb = orc.debye_correlation(cl, 0.0, thetad, tk, int(z[a2]), int(z[a2]), rs)
+            cab = orc.debye_correlation(cl, r, thetad, tk, int(z[a1]), int(z[a2]), rs)
+            want = float(s2g) + caa + cbb - 2.0 * cab
+            assert abs(float(sig[a2, a1]) - want) <= 2e-7 * want, (a1, a2)
