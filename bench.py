@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the LU trailing-update kernel, averaged over one factorisation's
 # launches, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script scaled by the
 # same counters read for a 1 GiB device copy (profiles/fms_cu_r8_gemm_v2_traffic.md).
-TRAFFIC_BYTES_PER_LAUNCH = {("cfg2", "mixed"): 9234017224}
+TRAFFIC_BYTES_PER_LAUNCH = {("cfg2", "mixed"): 11375332998}  # K = 192 trailing update, rocprofv3 --pmc
 
 
 def parse():
@@ -218,7 +218,7 @@ def main():
         "kernels_one_profiled_sweep": kernels,
     }
     if not args.no_fp64_compare and solver_id == 1:
-        r64 = run_solver(L, wl, 2, 1, 0, profile=False)
+        r64 = run_solver(L, wl, 2, 1, 1, profile=False)  # warmed like the main run
         v64 = r64["ne"] / r64["seconds"]
         d = [float(np.linalg.norm(main_res["scattering"][e] - r64["scattering"][e]) /
                    np.linalg.norm(r64["scattering"][e])) for e in range(ne)]

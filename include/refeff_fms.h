@@ -65,7 +65,14 @@ typedef enum refeff_fms_solver {
     REFEFF_FMS_SOLVER_GGBI = 3,    /* refeff minv=1 one-shot BiCGStab, in FP64 (refeff runs it in c32) */
     REFEFF_FMS_SOLVER_GGRM = 4,    /* minv=2 recursion (fms_recursion_scattering), in FP64 */
     REFEFF_FMS_SOLVER_GGGM = 5,    /* minv=3 Graves-Morris/Salam (fms_graves_morris_scattering), in FP64 */
-    REFEFF_FMS_SOLVER_GGTF = 6     /* other minv: TFQMR (fms_tfqmr_scattering), in FP64 */
+    REFEFF_FMS_SOLVER_GGTF = 6,    /* other minv: TFQMR (fms_tfqmr_scattering), in FP64 */
+    REFEFF_FMS_SOLVER_BICGSTAB = 7 /* converged BiCGSTAB per energy: FP32 MFMA matvecs, FP64 x and reliable
+                                      updates from the FP64 residual (relative residual 1e-10 per column);
+                                      energies it breaks down on or does not finish within the iteration
+                                      cap, and every lower energy after the first such one, are solved by
+                                      MIXED.  Same results as MIXED to the tolerance; a full scattering
+                                      request -> MIXED.  Replaces the LU of fms_lu_scattering
+                                      (fms/solvers.rs:366-484) where it converges */
 } refeff_fms_solver;
 
 /* ---------------------------------------------------------------------- */
@@ -174,6 +181,9 @@ typedef struct refeff_fms_stats {
     double max_correction;       /* largest estimated relative error left in x after refinement:
                                     d_s * min(1, d_s / d_{s-1}) from the last two corrections
                                     |dx|/|x| (mixed solver) */
+    int32_t krylov_energies;     /* BICGSTAB: energies BiCGSTAB solved */
+    int32_t krylov_iterations;   /* BICGSTAB: their iterations, summed */
+    int32_t lu_energies;         /* BICGSTAB: energies the mixed LU solved */
 } refeff_fms_stats;
 
 /* Build a plan (uploads geometry to the current HIP device).  Mirrors

@@ -22,6 +22,7 @@ STATUS_NAMES = {
 }
 
 SOLVER_AUTO, SOLVER_MIXED, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGRM, SOLVER_GGGM, SOLVER_GGTF = 0, 1, 2, 3, 4, 5, 6
+SOLVER_BICGSTAB = 7
 
 
 class FmsLibraryMissing(RuntimeError):
@@ -58,7 +59,8 @@ class PlanInfo(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("seconds_total", C.c_double), ("seconds_assemble", C.c_double), ("seconds_factor", C.c_double),
                 ("seconds_solve", C.c_double), ("refinement_sweeps", C.c_int32), ("batches", C.c_int32),
-                ("max_correction", C.c_double)]
+                ("max_correction", C.c_double), ("krylov_energies", C.c_int32), ("krylov_iterations", C.c_int32),
+                ("lu_energies", C.c_int32)]
 
 
 class Profile(C.Structure):

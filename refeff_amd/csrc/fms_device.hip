@@ -1942,6 +1942,230 @@ __global__ __launch_bounds__(256) void k_update(double2* X, const typename Cx<T>
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision BiCGSTAB (REFEFF_FMS_SOLVER_BICGSTAB): van der Vorst's
+// BiCGSTAB on A X = B for all right-hand-side columns of a batch of energies,
+// every column with its own scalars.  The two matvecs per iteration read A32
+// with FP32 complex MFMA; x accumulates in FP64, and "reliable updates"
+// (Sleijpen & van der Vorst 1996, as in mixed-precision lattice solvers,
+// Clark et al. 2010) replace the recursive residual by the FP64 residual
+// B - A64 X when it has dropped by a set factor, which also decides
+// convergence.  Column dots are summed in a fixed order: each 32-row block
+// leaves a partial sum, one reduction per energy adds them in block order.
+// Krylov vectors: KV[e][vec][npad][nrhs] float2.
+// ---------------------------------------------------------------------------
+constexpr int kBicgVecs = 6;
+enum BicgVec { kBvRh = 0, kBvR = 1, kBvP = 2, kBvV = 3, kBvS = 4, kBvT = 5 };
+
+struct BicgCol {
+    double2 rho, alpha, omega, beta;
+    double rr;   // recursive residual |r|^2
+    double bb;   // |b|^2
+    double rt;   // FP64 residual |b - A64 x|^2 at the last replacement
+    int active;  // still iterating
+    int flags;   // 1: breakdown (zero or non-finite scalar)
+};
+
+struct BicgArgs {
+    const float2* a32;
+    const double2* b64;
+    double2* x64;
+    const double2* r64;
+    float2* kv;
+    double2* pd;      // [e][nblk][nrhs] partial complex dots
+    double* pn;       // [e][nblk][nrhs] partial squared norms
+    BicgCol* col;     // [e][nrhs]
+    size_t sa, sx, skv;
+    int n, nrhs, nblk;
+    double tol2;      // squared relative residual tolerance
+};
+
+__device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {  // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// Y = A32 X for one 32-row block of one energy and 16-column chunk, X = KV vector
+// xin, Y -> KV vector yout; partial dots over the block: pd = sum conj(U) Y with
+// U = KV vector uvec, pn = sum |Y|^2.
+__global__ __launch_bounds__(512) void k_bicg_matvec(BicgArgs a, int xin, int yout, int uvec, const int* emap) {
+    constexpr int BS = kPanel, NC = 16;
+    __shared__ float2 red[4][BS][NC];
+    const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y), tid = threadIdx.x;
+    const int R0 = blockIdx.x * BS, c0 = blockIdx.z * NC;
+    const int ncol = min(NC, a.nrhs - c0);
+    float2* kv = a.kv + e * a.skv;
+    typename Mfma<float>::acc cr[2][1], ci[2][1];
+    row_block_product<float, 1>(a.a32 + e * a.sa, a.n, R0, 0, a.n / 16, kv + xin * a.sx, a.nrhs, c0, ncol, cr, ci);
+    reduce_waves<float, 1>(red, cr, ci);
+    float2* Y = kv + yout * a.sx;
+    for (int i = tid; i < BS * NC; i += 512) {
+        const int r = i / NC, c = i % NC;
+        if (c < ncol) Y[static_cast<size_t>(R0 + r) * a.nrhs + c0 + c] = red[1][r][c];
+    }
+    if (tid < ncol) {
+        const float2* U = kv + uvec * a.sx;
+        double2 d = make_double2(0.0, 0.0);
+        double q = 0.0;
+        for (int r = 0; r < BS; ++r) {
+            const double2 y = to_d(red[1][r][tid]);
+            d = cadd(d, cconj_mul(to_d(U[static_cast<size_t>(R0 + r) * a.nrhs + c0 + tid]), y));
+            q += y.x * y.x + y.y * y.y;
+        }
+        const size_t o = (static_cast<size_t>(e) * a.nblk + blockIdx.x) * a.nrhs + c0 + tid;
+        a.pd[o] = d;
+        a.pn[o] = q;
+    }
+}
+
+enum BicgVecMode { kBicgInit = 0, kBicgS = 1, kBicgXR = 2, kBicgP = 3, kBicgRepl = 4 };
+
+// Vector steps over one 32-row block (blockIdx.x) x 16-column chunk (blockIdx.z)
+// of one energy, with the block's partial column sums where the step needs them:
+//   init: x = 0, r = rh = p = f32(b); pd = (rh, r), pn = |b|^2 (FP64)
+//   S:    s = r - alpha v
+//   XR:   x += alpha p + omega s (FP64), r = s - omega t; pd = (rh, r), pn = |r|^2
+//   P:    p = r + beta (p - omega v)
+//   repl: r = f32(R64); pn = |R64|^2 (FP64)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bicg_vec(BicgArgs a, const int* emap) {
+    constexpr int BS = kPanel, NC = 16;
+    __shared__ double2 sd[BS][NC];
+    __shared__ double sq[BS][NC];
+    const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y), tid = threadIdx.x;
+    const int R0 = blockIdx.x * BS, c0 = blockIdx.z * NC;
+    const int ncol = min(NC, a.nrhs - c0);
+    float2* kv = a.kv + e * a.skv;
+    const BicgCol* col = a.col + static_cast<size_t>(e) * a.nrhs;
+    for (int i = tid; i < BS * NC; i += 256) {
+        const int r = i / NC, c = i % NC;
+        if (c >= ncol) continue;
+        const size_t o = static_cast<size_t>(R0 + r) * a.nrhs + c0 + c;
+        const BicgCol& cc = col[c0 + c];
+        if constexpr (MODE == kBicgInit) {
+            const double2 b = a.b64[e * a.sx + o];
+            const float2 f = make_float2(static_cast<float>(b.x), static_cast<float>(b.y));
+            kv[kBvR * a.sx + o] = f;
+            kv[kBvRh * a.sx + o] = f;
+            kv[kBvP * a.sx + o] = f;
+            a.x64[e * a.sx + o] = make_double2(0.0, 0.0);
+            const double2 fd = to_d(f);
+            sd[r][c] = cconj_mul(fd, fd);
+            sq[r][c] = b.x * b.x + b.y * b.y;
+        } else if constexpr (MODE == kBicgS) {
+            const float2 v = kv[kBvV * a.sx + o], rv = kv[kBvR * a.sx + o];
+            const float2 al = make_float2(static_cast<float>(cc.alpha.x), static_cast<float>(cc.alpha.y));
+            kv[kBvS * a.sx + o] = csub(rv, cmul(al, v));
+        } else if constexpr (MODE == kBicgXR) {
+            const float2 pv = kv[kBvP * a.sx + o], sv = kv[kBvS * a.sx + o], tv = kv[kBvT * a.sx + o];
+            double2 x = a.x64[e * a.sx + o];
+            x = cadd(x, cadd(cmul(cc.alpha, to_d(pv)), cmul(cc.omega, to_d(sv))));
+            a.x64[e * a.sx + o] = x;
+            const float2 om = make_float2(static_cast<float>(cc.omega.x), static_cast<float>(cc.omega.y));
+            const float2 rn = csub(sv, cmul(om, tv));
+            kv[kBvR * a.sx + o] = rn;
+            const double2 rd = to_d(rn);
+            sd[r][c] = cconj_mul(to_d(kv[kBvRh * a.sx + o]), rd);
+            sq[r][c] = rd.x * rd.x + rd.y * rd.y;
+        } else if constexpr (MODE == kBicgP) {
+            const float2 pv = kv[kBvP * a.sx + o], vv = kv[kBvV * a.sx + o], rv = kv[kBvR * a.sx + o];
+            const float2 be = make_float2(static_cast<float>(cc.beta.x), static_cast<float>(cc.beta.y));
+            const float2 om = make_float2(static_cast<float>(cc.omega.x), static_cast<float>(cc.omega.y));
+            kv[kBvP * a.sx + o] = cadd(rv, cmul(be, csub(pv, cmul(om, vv))));
+        } else {
+            const double2 rv = a.r64[e * a.sx + o];
+            kv[kBvR * a.sx + o] = make_float2(static_cast<float>(rv.x), static_cast<float>(rv.y));
+            sd[r][c] = make_double2(0.0, 0.0);
+            sq[r][c] = rv.x * rv.x + rv.y * rv.y;
+        }
+    }
+    if constexpr (MODE == kBicgInit || MODE == kBicgXR || MODE == kBicgRepl) {
+        __syncthreads();
+        if (tid < ncol) {
+            double2 d = make_double2(0.0, 0.0);
+            double q = 0.0;
+            for (int r = 0; r < BS; ++r) {
+                d = cadd(d, sd[r][tid]);
+                q += sq[r][tid];
+            }
+            const size_t o = (static_cast<size_t>(e) * a.nblk + blockIdx.x) * a.nrhs + c0 + tid;
+            a.pd[o] = d;
+            a.pn[o] = q;
+        }
+    }
+}
+
+__device__ __forceinline__ bool finite2(double2 v) { return isfinite(v.x) && isfinite(v.y); }
+
+enum BicgStage { kBicgStInit = 0, kBicgStAlpha = 1, kBicgStOmega = 2, kBicgStBeta = 3, kBicgStTrue = 4 };
+
+// Per-column scalars of one energy from the block partial sums, added in block
+// order; inactive columns get zero steps so they stay where they are.
+//   init:  rho = (rh, r), bb = |b|^2, rr = rt = bb, active = bb > 0
+//   alpha: alpha = rho / (rh, v)
+//   omega: omega = (t, s) / (t, t)           (pd holds (s, t) = conj((t, s)))
+//   beta:  rho' = (rh, r), beta = (rho'/rho)(alpha/omega), rho = rho', rr = |r|^2
+//   true:  rt = rr = |R64|^2; a column whose FP64 residual is within tol stops
+template <int STAGE>
+__global__ __launch_bounds__(64) void k_bicg_reduce(BicgArgs a, const int* emap) {
+    const int e = emap ? emap[blockIdx.x] : static_cast<int>(blockIdx.x);
+    for (int c = threadIdx.x; c < a.nrhs; c += 64) {
+        double2 d = make_double2(0.0, 0.0);
+        double q = 0.0;
+        const size_t base = static_cast<size_t>(e) * a.nblk * a.nrhs + c;
+        for (int b = 0; b < a.nblk; ++b) {
+            d = cadd(d, a.pd[base + static_cast<size_t>(b) * a.nrhs]);
+            q += a.pn[base + static_cast<size_t>(b) * a.nrhs];
+        }
+        BicgCol& cc = a.col[static_cast<size_t>(e) * a.nrhs + c];
+        const double2 zero = make_double2(0.0, 0.0);
+        if constexpr (STAGE == kBicgStInit) {
+            cc.rho = d;
+            cc.bb = q;
+            cc.rr = q;
+            cc.rt = q;
+            cc.alpha = cc.omega = cc.beta = zero;
+            cc.active = q > 0.0 ? 1 : 0;
+            cc.flags = 0;
+        } else if constexpr (STAGE == kBicgStTrue) {
+            cc.rt = q;
+            cc.rr = q;
+            if (cc.active && q <= a.tol2 * cc.bb) cc.active = 0;
+        } else {
+            if (!cc.active) {
+                cc.alpha = cc.omega = cc.beta = zero;
+                continue;
+            }
+            if constexpr (STAGE == kBicgStAlpha) {
+                if (d.x == 0.0 && d.y == 0.0) {
+                    cc.flags |= 1;
+                    cc.alpha = zero;
+                } else {
+                    cc.alpha = cdiv(cc.rho, d);
+                    if (!finite2(cc.alpha)) cc.flags |= 1;
+                }
+            } else if constexpr (STAGE == kBicgStOmega) {
+                if (!(q > 0.0)) {
+                    cc.flags |= 1;
+                    cc.omega = zero;
+                } else {
+                    cc.omega = make_double2(d.x / q, -d.y / q);
+                    if (!finite2(cc.omega) || (cc.omega.x == 0.0 && cc.omega.y == 0.0)) cc.flags |= 1;
+                }
+            } else {
+                cc.rr = q;
+                if ((d.x == 0.0 && d.y == 0.0) || (cc.rho.x == 0.0 && cc.rho.y == 0.0)) {
+                    cc.flags |= 1;
+                    cc.beta = zero;
+                } else {
+                    cc.beta = cmul(cdiv(d, cc.rho), cdiv(cc.alpha, cc.omega));
+                    if (!finite2(cc.beta)) cc.flags |= 1;
+                }
+                cc.rho = d;
+            }
+        }
+    }
+}
+
 // scattering[e][ip][row + ch*col] = X[e][rep[ip] + row][col0[ip] + col]
 // ---------------------------------------------------------------------------
 // FEFF ggbi (minv = 1): one-shot BiCGStab on M x = e_src with M = 1 - T G0
@@ -2631,6 +2855,17 @@ struct refeff_fms_plan {
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     refeff_fms_profile acc{};
+    // REFEFF_FMS_SOLVER_BICGSTAB: the plan runs as the mixed solver, with BiCGSTAB
+    // tried first per energy and the mixed LU for the energies it does not finish
+    bool bicg = false;
+    int bicg_maxit = 0;          // iterations per energy before the LU takes over
+    int bicg_batch = 32;         // energies per BiCGSTAB device batch
+    double bicg_tol = 1.0e-10;   // relative FP64 residual per column
+    double bicg_delta = 1.0e-2;  // reliable update once |r| < delta * max |r| since the last one
+    float2* d_kv = nullptr;      // Krylov vectors [e][kBicgVecs][npad][nrhs]
+    double2* d_pd = nullptr;     // block partial dots [e][npad/32][nrhs]
+    double* d_pn = nullptr;      // block partial norms [e][npad/32][nrhs]
+    BicgCol* d_bcol = nullptr;   // per-column scalars [e][nrhs]
 };
 
 namespace {
@@ -2648,6 +2883,7 @@ void free_workspace(refeff_fms_plan* p) {
     p->d_dinv = nullptr;
     dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_cand); dfree(p->d_iter); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
     dfree(p->d_out); dfree(p->d_norms);
+    dfree(p->d_kv); dfree(p->d_pd); dfree(p->d_pn); dfree(p->d_bcol);
     p->ws_batch = 0;
 }
 
@@ -2730,6 +2966,10 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
     b += 2 * p->npad * sizeof(int) + 2 * p->n * sizeof(double2);
     b += static_cast<size_t>(p->ch) * p->ch * p->npot * sizeof(double2);
     if (krylov_solver(p->solver)) b += static_cast<size_t>(kIterVecs) * nr * sizeof(double2);
+    if (p->bicg)
+        b += static_cast<size_t>(kBicgVecs) * nr * sizeof(float2) +
+             static_cast<size_t>(p->npad / kPanel) * p->nrhs * (sizeof(double2) + sizeof(double)) +
+             static_cast<size_t>(p->nrhs) * sizeof(BicgCol);
     return b;
 }
 
@@ -2749,7 +2989,12 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     if (mixed) ok = ok && alloc_calu(p, nb);
     if (krylov_solver(p->solver))
         ok = ok && dalloc(&p->d_iter, static_cast<size_t>(kIterVecs) * p->nrhs * p->npad * nb);
-    ok = ok && dalloc(&p->d_emap, static_cast<size_t>(nb)) &&
+    if (p->bicg) {
+        const size_t pb = static_cast<size_t>(p->npad / kPanel) * p->nrhs * nb;
+        ok = ok && dalloc(&p->d_kv, static_cast<size_t>(kBicgVecs) * nr * nb) && dalloc(&p->d_pd, pb) &&
+             dalloc(&p->d_pn, pb) && dalloc(&p->d_bcol, static_cast<size_t>(p->nrhs) * nb);
+    }
+    ok = ok && dalloc(&p->d_emap, 2 * static_cast<size_t>(nb)) &&
          dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
          dalloc(&p->d_info, nb) && dalloc(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb) &&
          dalloc(&p->d_k, static_cast<size_t>(p->nsp) * nb) &&
@@ -3116,10 +3361,136 @@ bool sync_groups(const std::vector<Group>& gs) {
     return ok;
 }
 
+// BiCGSTAB over the energies of one group, every right-hand-side column with
+// its own scalars.  The host reads the per-column scalars after every
+// iteration: an energy with a breakdown drops out (LU takes it), an energy
+// whose recursive residual fell by bicg_delta (or under the tolerance) gets a
+// reliable update R64 = B - A64 X, whose norm decides convergence.  ok[e] = 1:
+// X64 holds the solution; iters[e]: iterations run.
+int run_bicgstab(refeff_fms_plan* p, const Group& g, std::vector<int>& iters, std::vector<char>& ok) {
+    const int n = p->npad, nr = p->nrhs, nblk = n / kPanel, chunks = (nr + 15) / 16, nb = g.nb;
+    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
+    const size_t pb = static_cast<size_t>(nblk) * nr;
+    BicgArgs a{p->d_a32 + sa * g.e0, p->d_b64 + sx * g.e0, p->d_x64 + sx * g.e0, p->d_r64 + sx * g.e0,
+               p->d_kv + kBicgVecs * sx * g.e0, p->d_pd + pb * g.e0, p->d_pn + pb * g.e0,
+               p->d_bcol + static_cast<size_t>(nr) * g.e0, sa, sx, kBicgVecs * sx, n, nr, nblk,
+               p->bicg_tol * p->bicg_tol};
+    hipStream_t s = g.s;
+    std::vector<BicgCol> cols(static_cast<size_t>(nb) * nr);
+    auto pull = [&]() {
+        return hipMemcpyAsync(cols.data(), a.col, cols.size() * sizeof(BicgCol), hipMemcpyDeviceToHost, s) ==
+                   hipSuccess &&
+               hipStreamSynchronize(s) == hipSuccess;
+    };
+    const double nn = static_cast<double>(n) * n;
+    iters.assign(nb, 0);
+    ok.assign(nb, 0);
+    hipLaunchKernelGGL(k_bicg_vec<kBicgInit>, dim3(nblk, nb, chunks), dim3(256), 0, s, a, nullptr);
+    hipLaunchKernelGGL(k_bicg_reduce<kBicgStInit>, dim3(nb), dim3(64), 0, s, a, nullptr);
+    if (!pull()) return REFEFF_FMS_DEVICE_ERROR;
+    std::vector<double> maxrr(cols.size());
+    std::vector<int> live, next, repl;
+    for (int e = 0; e < nb; ++e) {
+        bool any = false;
+        for (int c = 0; c < nr; ++c) {
+            maxrr[static_cast<size_t>(e) * nr + c] = cols[static_cast<size_t>(e) * nr + c].rr;
+            any = any || cols[static_cast<size_t>(e) * nr + c].active;
+        }
+        if (any)
+            live.push_back(e);
+        else
+            ok[e] = 1;  // B = 0: X = 0
+    }
+    int* em_live = p->d_emap + 2 * g.e0;
+    int* em_repl = em_live + nb;
+    const double d2 = p->bicg_delta * p->bicg_delta, t2 = p->bicg_tol * p->bicg_tol;
+    for (int it = 1; it <= p->bicg_maxit && !live.empty(); ++it) {
+        const int m = static_cast<int>(live.size());
+        if (hipMemcpyAsync(em_live, live.data(), m * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
+            return REFEFF_FMS_DEVICE_ERROR;
+        const dim3 gmv(nblk, m, chunks), gv(nblk, m, chunks);
+        launch(p, s, REFEFF_FMS_K_SOLVE, 16.0 * nn * nr * m, 16.0 * nn * m, [&] {
+            hipLaunchKernelGGL(k_bicg_matvec, gmv, dim3(512), 0, s, a, static_cast<int>(kBvP), static_cast<int>(kBvV),
+                               static_cast<int>(kBvRh), em_live);
+            hipLaunchKernelGGL(k_bicg_reduce<kBicgStAlpha>, dim3(m), dim3(64), 0, s, a, em_live);
+            hipLaunchKernelGGL(k_bicg_vec<kBicgS>, gv, dim3(256), 0, s, a, em_live);
+            hipLaunchKernelGGL(k_bicg_matvec, gmv, dim3(512), 0, s, a, static_cast<int>(kBvS), static_cast<int>(kBvT),
+                               static_cast<int>(kBvS), em_live);
+            hipLaunchKernelGGL(k_bicg_reduce<kBicgStOmega>, dim3(m), dim3(64), 0, s, a, em_live);
+            hipLaunchKernelGGL(k_bicg_vec<kBicgXR>, gv, dim3(256), 0, s, a, em_live);
+            hipLaunchKernelGGL(k_bicg_reduce<kBicgStBeta>, dim3(m), dim3(64), 0, s, a, em_live);
+            hipLaunchKernelGGL(k_bicg_vec<kBicgP>, gv, dim3(256), 0, s, a, em_live);
+        });
+        if (!pull()) return REFEFF_FMS_DEVICE_ERROR;
+        next.clear();
+        repl.clear();
+        for (int e : live) {
+            iters[e] = it;
+            bool brk = false, trig = false;
+            for (int c = 0; c < nr; ++c) {
+                const size_t i = static_cast<size_t>(e) * nr + c;
+                const BicgCol& cc = cols[i];
+                if (!cc.active) continue;
+                if (cc.flags || !std::isfinite(cc.rr)) brk = true;
+                if (cc.rr <= d2 * maxrr[i] || cc.rr <= t2 * cc.bb) trig = true;
+                maxrr[i] = std::max(maxrr[i], cc.rr);
+            }
+            if (brk) continue;  // breakdown: the LU takes this energy
+            if (trig) repl.push_back(e);
+            next.push_back(e);
+        }
+        if (!repl.empty()) {
+            const int r = static_cast<int>(repl.size());
+            if (hipMemcpyAsync(em_repl, repl.data(), r * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
+                return REFEFF_FMS_DEVICE_ERROR;
+            double* gn = p->d_norms + 4 * static_cast<size_t>(g.e0);
+            launch(p, s, REFEFF_FMS_K_RESIDUAL, 8.0 * nn * nr * r, 16.0 * nn * r, [&] {
+                hipLaunchKernelGGL(k_residual_wg<1>, dim3(n / kPanel, r, chunks), dim3(512), 0, s, p->d_a64 + sa * g.e0,
+                                   sa, n, a.x64, a.b64, p->d_r64 + sx * g.e0, sx, nr, n, gn, em_repl);
+            });
+            hipLaunchKernelGGL(k_bicg_vec<kBicgRepl>, dim3(nblk, r, chunks), dim3(256), 0, s, a, em_repl);
+            hipLaunchKernelGGL(k_bicg_reduce<kBicgStTrue>, dim3(r), dim3(64), 0, s, a, em_repl);
+            if (!pull()) return REFEFF_FMS_DEVICE_ERROR;
+            for (int e : repl) {
+                bool any = false;
+                for (int c = 0; c < nr; ++c) {
+                    const size_t i = static_cast<size_t>(e) * nr + c;
+                    maxrr[i] = cols[i].rr;
+                    any = any || cols[i].active;
+                }
+                if (!any) ok[e] = 1;
+            }
+            std::vector<int> keep;
+            for (int e : next)
+                if (!ok[e]) keep.push_back(e);
+            next.swap(keep);
+        }
+        live.swap(next);
+    }
+    if (p->trace)
+        for (int e = 0; e < nb; ++e) {
+            double worst = 0.0;
+            for (int c = 0; c < nr; ++c) {
+                const BicgCol& cc = cols[static_cast<size_t>(e) * nr + c];
+                if (cc.bb > 0) worst = std::max(worst, std::sqrt(cc.rt / cc.bb));
+            }
+            std::fprintf(stderr, "refeff_fms: bicgstab energy %d: %s after %d iterations, FP64 residual %.3e\n", e,
+                         ok[e] ? "converged" : "left for the LU", iters[e], worst);
+        }
+    return REFEFF_FMS_OK;
+}
+
 // one device batch of nb energies; inputs already in p->d_k / p->d_ph.  The
 // batch is split into sub-batches on separate streams so that latency-bound
 // kernels of one sub-batch overlap MFMA work of another.
-int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status, refeff_fms_stats* st) {
+// BiCGSTAB batch outcome (solve_device_batch with a BicgOut: no LU)
+struct BicgOut {
+    std::vector<int> iters;
+    std::vector<char> ok;
+};
+
+int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status, refeff_fms_stats* st,
+                       BicgOut* bo = nullptr) {
     const int n = p->npad;
     const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * p->nrhs;
     const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
@@ -3131,7 +3502,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     const double t0 = now_s();
     std::vector<Group> gs;
     {
-        const int G = std::max(1, std::min(p->nstreams, nb));
+        const int G = bo ? 1 : std::max(1, std::min(p->nstreams, nb));
         int e0 = 0;
         for (int i = 0; i < G; ++i) {
             const int cnt = nb / G + (i < nb % G ? 1 : 0);
@@ -3183,6 +3554,11 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
             run_iterative(p, g);
             continue;
         }
+        if (bo) {
+            const int r = run_bicgstab(p, g, bo->iters, bo->ok);
+            if (r) return r;
+            continue;
+        }
         int* ipiv = p->d_ipiv + static_cast<size_t>(n) * g.e0;
         int* perm = p->d_perm + static_cast<size_t>(n) * g.e0;
         const size_t sd = 2 * static_cast<size_t>(n) * kPanel * g.e0;
@@ -3200,7 +3576,9 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     std::vector<double> norms(4 * static_cast<size_t>(nb));
     std::vector<double> est_err(nb, 0.0);  // estimated relative error left after refinement
     const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
-    if (mixed) {
+    if (bo) {
+        t_first = now_s();
+    } else if (mixed) {
         float2* w = static_cast<float2*>(p->d_w);
         for (const auto& g : gs)
             hipLaunchKernelGGL(k_fill_double, dim3((2 * sx * g.nb + 255) / 256), dim3(256), 0, g.s,
@@ -3364,6 +3742,10 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
                                         : REFEFF_FMS_SOLVER_GGTF;
     const bool iterative = solver == REFEFF_FMS_SOLVER_GGBI || krylov_solver(solver);
     if (full && iterative) return REFEFF_FMS_UNSUPPORTED;  // FullScatteringRequiresLu
+    // BiCGSTAB runs on the mixed plan (its fallback); a full scattering matrix
+    // (every G0 column a right-hand side) forces the LU
+    const bool bicg = solver == REFEFF_FMS_SOLVER_BICGSTAB && !full;
+    if (solver == REFEFF_FMS_SOLVER_BICGSTAB) solver = REFEFF_FMS_SOLVER_MIXED;
     if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64 && !iterative)
         return REFEFF_FMS_UNSUPPORTED;
     // InvalidTolerance (fms/solvers.rs:35-40, internals/iterative.rs:70-75)
@@ -3399,6 +3781,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->ch = (p->L + 1) * (p->L + 1) * p->nsp;
     p->npot = p->max_pot + 1;
     p->solver = solver;
+    p->bicg = bicg;
     p->rdirec = d->direct_cutoff;
     p->toler1 = d->convergence_tolerance;
     p->toler2 = d->zero_tolerance;
@@ -3489,6 +3872,16 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
     if (const char* e = std::getenv("REFEFF_FMS_LOOKAHEAD")) p->lookahead = std::atoi(e) != 0;
+    // BiCGSTAB iteration cap: about where its two matvecs per iteration (A32
+    // streamed twice, ~1.1 ms per energy at N = 9744 as measured) cost what
+    // the FP32 MFMA LU of the same energy costs (~23 ms at N = 9744)
+    p->bicg_maxit = std::max(8, std::min(400, p->npad / 256));
+    if (const char* e = std::getenv("REFEFF_FMS_BICG_MAXIT")) p->bicg_maxit = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("REFEFF_FMS_BICG_TOL")) {
+        const double v = std::atof(e);
+        if (v > 0.0 && v < 1.0) p->bicg_tol = v;
+    }
+    if (const char* e = std::getenv("REFEFF_FMS_BICG_BATCH")) p->bicg_batch = std::max(1, std::atoi(e));
     for (int i = 0; ok && i < p->nstreams; ++i)
         ok = hipStreamCreateWithFlags(&p->streams[i], hipStreamNonBlocking) == hipSuccess;
     for (int i = 0; ok && p->lookahead && i < p->nstreams; ++i)
@@ -3539,16 +3932,121 @@ int refeff_fms_plan_query(const refeff_fms_plan* p, refeff_fms_plan_info* info) 
     info->potential_start = p->pstart;
     info->potential_end = p->pend;
     info->rhs_count = p->nrhs;
-    info->solver = p->solver;
+    info->solver = p->bicg ? REFEFF_FMS_SOLVER_BICGSTAB : p->solver;
     info->batch = p->batch;
     info->lu_block = p->outer_block;
     return REFEFF_FMS_OK;
+}
+
+// REFEFF_FMS_SOLVER_BICGSTAB: energies in order of decreasing Re k^2 (where
+// the multiple-scattering series is weakest) in BiCGSTAB batches; an energy
+// BiCGSTAB breaks down on or does not finish within bicg_maxit iterations is
+// solved again by the mixed LU, and from the first such energy on every
+// remaining (lower) energy goes to the LU directly.  Outputs land at each
+// energy's own index.
+static int solve_bicg_common(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc,
+                             int32_t* status, refeff_fms_stats* st, bool device) {
+    const size_t kst = static_cast<size_t>(p->nsp);
+    const size_t pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+    const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
+    std::vector<double> kh;
+    if (device) {  // the order needs k on the host
+        kh.resize(2 * kst * ne);
+        if (check_hip(hipMemcpy(kh.data(), k, kh.size() * sizeof(double), hipMemcpyDeviceToHost))) return REFEFF_FMS_DEVICE_ERROR;
+    }
+    const double* kk = device ? kh.data() : k;
+    std::vector<int> order(ne);
+    for (int e = 0; e < ne; ++e) order[e] = e;
+    auto re_k2 = [&](int e) { return kk[2 * kst * e] * kk[2 * kst * e] - kk[2 * kst * e + 1] * kk[2 * kst * e + 1]; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return re_k2(a) > re_k2(b); });
+    const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    std::vector<double2> outh(sst * std::max(1, p->batch));
+    std::vector<int32_t> sb(std::max(1, p->batch));
+    int rc = REFEFF_FMS_OK;
+    // one device batch over the energies idx, outputs scattered to their indices
+    auto run = [&](const std::vector<int>& idx, BicgOut* bo) -> int {
+        const int nb = static_cast<int>(idx.size());
+        int r = ensure_workspace(p, nb);
+        if (r) return r;
+        for (int i = 0; i < nb; ++i) {
+            const int e = idx[i];
+            if (hipMemcpyAsync(p->d_k + kst * i, k + 2 * kst * e, 2 * kst * sizeof(double), kind, p->stream) != hipSuccess ||
+                hipMemcpyAsync(p->d_ph + pst * i, ph + 2 * pst * e, 2 * pst * sizeof(double), kind, p->stream) != hipSuccess)
+                return REFEFF_FMS_DEVICE_ERROR;
+        }
+        if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+        r = solve_device_batch(p, nb, p->d_out, sb.data(), st, bo);
+        if (r == REFEFF_FMS_DEVICE_ERROR) return r;
+        for (int i = 0; i < nb; ++i) {
+            if (bo && !sb[i] && !bo->ok[i]) continue;  // left for the LU
+            const int e = idx[i];
+            if (status) status[e] = sb[i];
+            if (sb[i] && !rc) rc = sb[i];
+            if (device) {
+                if (hipMemcpyAsync(reinterpret_cast<double2*>(sc) + sst * e, p->d_out + sst * i, sst * sizeof(double2),
+                                   hipMemcpyDeviceToDevice, p->stream) != hipSuccess)
+                    return REFEFF_FMS_DEVICE_ERROR;
+            }
+        }
+        if (!device) {
+            if (check_hip(hipMemcpy(outh.data(), p->d_out, sst * nb * sizeof(double2), hipMemcpyDeviceToHost)))
+                return REFEFF_FMS_DEVICE_ERROR;
+            for (int i = 0; i < nb; ++i) {
+                if (bo && !sb[i] && !bo->ok[i]) continue;
+                std::memcpy(sc + 2 * sst * idx[i], outh.data() + sst * i, sst * sizeof(double2));
+            }
+        } else if (hipStreamSynchronize(p->stream) != hipSuccess) {
+            return REFEFF_FMS_DEVICE_ERROR;
+        }
+        return REFEFF_FMS_OK;
+    };
+    std::vector<int> lu;
+    bool switched = false;
+    const int kb = std::max(1, std::min(p->batch, p->bicg_batch));
+    size_t pos = 0;
+    while (pos < order.size() && !switched) {
+        std::vector<int> idx(order.begin() + pos, order.begin() + std::min(order.size(), pos + kb));
+        pos += idx.size();
+        BicgOut bo;
+        const int r = run(idx, &bo);
+        if (r) return r;
+        for (size_t i = 0; i < idx.size(); ++i) {
+            if (sb[i]) continue;
+            if (bo.ok[i]) {
+                if (st) {
+                    st->krylov_energies += 1;
+                    st->krylov_iterations += bo.iters[i];
+                }
+            } else {
+                lu.push_back(idx[i]);
+                switched = true;
+            }
+        }
+    }
+    lu.insert(lu.end(), order.begin() + pos, order.end());
+    for (size_t q = 0; q < lu.size(); q += p->batch) {
+        std::vector<int> idx(lu.begin() + q, lu.begin() + std::min(lu.size(), q + p->batch));
+        const int r = run(idx, nullptr);
+        if (r) return r;
+        if (st) st->lu_energies += static_cast<int32_t>(idx.size());
+    }
+    return rc;
 }
 
 static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc, int32_t* status,
                         refeff_fms_stats* st, bool device, hipStream_t ext, double* full_out = nullptr) {
     if (!p || ne < 0 || (ne > 0 && (!k || !ph || !sc))) return REFEFF_FMS_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(p->mu);
+    if (p->bicg && !full_out) {
+        const double tb = now_s();
+        if (st) std::memset(st, 0, sizeof(*st));
+        if (device && ext) (void)hipStreamSynchronize(ext);
+        const int r = solve_bicg_common(p, ne, k, ph, sc, status, st, device);
+        if (r == REFEFF_FMS_DEVICE_ERROR) return r;
+        if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+        if (st) st->seconds_total = now_s() - tb;
+        return r;
+    }
     const double t0 = now_s();
     if (st) std::memset(st, 0, sizeof(*st));
     const size_t kst = static_cast<size_t>(p->nsp);

@@ -12,12 +12,13 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (SOLVER_AUTO, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGGM, SOLVER_GGRM, SOLVER_GGTF, SOLVER_MIXED,
+from ._lib import (SOLVER_AUTO, SOLVER_BICGSTAB, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGGM, SOLVER_GGRM, SOLVER_GGTF,
+                   SOLVER_MIXED,
                    FmsError, PlanDesc, PlanInfo, Stats, check, lib)
 
 __all__ = [
     "FmsError", "SOLVER_AUTO", "SOLVER_MIXED", "SOLVER_FP64", "SOLVER_GGBI", "SOLVER_GGRM", "SOLVER_GGGM",
-    "SOLVER_GGTF", "sort_atoms_by_radius",
+    "SOLVER_GGTF", "SOLVER_BICGSTAB", "sort_atoms_by_radius",
     "fms_yprep_cluster", "sort_representative_atoms", "construct_state_kets", "FmsRealSpacePlan",
     "fms_real_space_plan", "fms_real_space_spectrum", "transition_b_matrix_averaged", "mkgtr_green_trace",
     "debye_mean_square_displacements",

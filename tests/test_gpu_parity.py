@@ -396,3 +396,85 @@ def test_iterative_branches_cu_fcc_r52_match_oracle(minv):
     worst = iterative_compare(minv, wl.positions, wl.potentials, wl.wave_numbers()[sel], wl.phase_shifts()[sel],
                               raw=wl.raw_lmax, max_pot=1, toler2=0.02)
     banner(f"worst rel err {worst:.2e}")
+
+
+# ---------------------------------------------------------------------------
+# Converged BiCGSTAB (FP32 MFMA matvecs, FP64 reliable updates) with the mixed
+# LU behind it: every energy lands in its own slot within the gg gate.
+# ---------------------------------------------------------------------------
+
+def bicg_plan(wl, **kw):
+    return fms.FmsRealSpacePlan(wl.positions, wl.potentials, global_lmax=3, raw_potential_lmax=wl.raw_lmax,
+                                direct_cutoff=wl.rdirec, solver=fms.SOLVER_BICGSTAB, **kw)
+
+
+def test_bicgstab_cu_fcc_r52_matches_oracle(monkeypatch):
+    """Cu fcc R=5.2 (N=880), the whole 60-point mesh, given in scrambled order:
+    the upper energies converge under BiCGSTAB, the lowest ones go to the LU."""
+    monkeypatch.setenv("REFEFF_FMS_BICG_MAXIT", "150")
+    banner("BiCGSTAB Cu fcc R=5.2, 60 energies")
+    wl = W.fcc_cu(5.2, 60)
+    perm = np.random.default_rng(4).permutation(60)
+    k, ph = wl.wave_numbers()[perm], wl.phase_shifts()[perm]
+    ref, st = orc.Plan(wl.positions, wl.potentials, global_lmax=3, raw_lmax=wl.raw_lmax,
+                       rdirec=wl.rdirec).spectrum(k, ph, threads=8)
+    assert not st.any()
+    plan = bicg_plan(wl)
+    assert plan.info.solver == fms.SOLVER_BICGSTAB
+    got, status = plan.solve(k, ph)
+    s = plan.last_stats
+    banner(f"  BiCGSTAB energies {s.krylov_energies} ({s.krylov_iterations} iterations), LU energies {s.lu_energies}")
+    assert not status.any(), status
+    assert s.krylov_energies > 0 and s.lu_energies > 0 and s.krylov_energies + s.lu_energies == 60
+    errs = [rel_err(got[e], ref[e]) for e in range(60)]
+    banner(f"  worst rel err vs oracle {max(errs):.2e}")
+    assert max(errs) <= GG_TOL, errs
+    plan.close()
+
+
+def test_bicgstab_falls_back_to_lu_everywhere_under_a_tiny_cap(monkeypatch):
+    monkeypatch.setenv("REFEFF_FMS_BICG_MAXIT", "2")
+    banner("BiCGSTAB with a 2-iteration cap: all energies by the LU")
+    wl = W.fcc_cu(5.2, 6)
+    k, ph = wl.wave_numbers(), wl.phase_shifts()
+    plan = bicg_plan(wl)
+    got, status = plan.solve(k, ph)
+    assert not status.any()
+    assert plan.last_stats.krylov_energies == 0 and plan.last_stats.lu_energies == 6
+    mixed = fms.FmsRealSpacePlan(wl.positions, wl.potentials, global_lmax=3, raw_potential_lmax=wl.raw_lmax,
+                                 direct_cutoff=wl.rdirec, solver=fms.SOLVER_MIXED)
+    ref, st = mixed.solve(k, ph)
+    assert np.array_equal(got, ref)  # the same LU on the same energies
+    plan.close()
+    mixed.close()
+
+
+@pytest.mark.parametrize("nsp,lfms", [(2, 0), (1, 1)])
+def test_bicgstab_random_clusters_match_oracle(monkeypatch, nsp, lfms):
+    """More than 16 right-hand sides per energy (two spin channels, or every
+    potential's block with lfms=1): several 16-column chunks."""
+    monkeypatch.setenv("REFEFF_FMS_BICG_MAXIT", "300")
+    banner(f"BiCGSTAB random cluster nsp={nsp} lfms={lfms}")
+    pos, pot = random_cluster(12, 16)
+    kw = dict(global_lmax=3, max_potential=2, direct_cutoff=9.0, spin_channels=nsp, lfms=lfms)
+    k1 = np.sqrt(2 * (np.array([20.0, 28.0, 36.0, 45.0]) + 0.6j) / W.HARTREE_EV) / W.BOHR_ANGSTROM
+    k = np.stack([k1, k1 * 0.99], axis=1)[:, :nsp]
+    ph = phases_for(k1, 3, np.array([3, 3, 3]), nsp=nsp, seed=3, spin_orbit=nsp == 2)
+    oplan = orc.Plan(pos, pot, nsp=nsp, lfms=lfms, max_pot=2, global_lmax=3, raw_lmax=[3, 3, 3], rdirec=9.0)
+    ref, st = oplan.spectrum(k, ph, threads=8)
+    assert not st.any()
+    plan = fms.FmsRealSpacePlan(pos, pot, solver=fms.SOLVER_BICGSTAB, **kw)
+    assert plan.info.rhs_count > 16
+    got, status = plan.solve(k, ph)
+    assert not status.any()
+    assert plan.last_stats.krylov_energies == 4, (plan.last_stats.krylov_energies, plan.last_stats.lu_energies)
+    errs = [rel_err(got[e], ref[e]) for e in range(len(k))]
+    assert max(errs) <= GG_TOL, errs
+    plan.close()
+
+
+def test_bicgstab_full_scattering_request_uses_the_lu():
+    wl = W.fcc_cu(4.0, 2)
+    plan = bicg_plan(wl, full_scattering_matrix_requested=True)
+    assert plan.info.solver == fms.SOLVER_MIXED
+    plan.close()
