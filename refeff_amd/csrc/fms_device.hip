@@ -1497,6 +1497,414 @@ __global__ __launch_bounds__(256, (BK == 16 ? 4 : 2)) void k_gemm_v2(float2* Cb,
             }
 }
 
+// ---------------------------------------------------------------------------
+// Split-bf16 trailing update.  The FP32 LU only seeds the FP64 refinement, so
+// its trailing update may run on the bf16 MFMA (16x the FP32 MFMA rate on
+// gfx950) with each FP32 operand x split into NS bf16 parts x = h + m (+ l):
+// h = bf16(x), m = bf16(x - h), l = bf16(x - h - m); the product keeps the
+// cross terms of total order < NS (3 for NS = 2, about 2^-16 relative; 6 for
+// NS = 3, about FP32) and accumulates in FP32.  Complex products in the 3M
+// form: planes q = re, im, re + im.  k_split_rows / k_split_cols write the
+// parts once per update, A as [q][s][row][k] and B transposed as [q][s][col][k]
+// (k contiguous), so every MFMA operand is one 16-byte read.
+// ---------------------------------------------------------------------------
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+template <int NS>
+__device__ __forceinline__ void split_bf(float x, uint16_t (&o)[NS]) {
+    float r = x;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        o[s] = f2bf(r);
+        r -= bf2f(o[s]);
+    }
+}
+
+// A rows [0, M) x k [0, K) of a row-major complex panel -> P[e][q][s][row][k]
+template <int NS>
+__global__ __launch_bounds__(256) void k_split_rows(const float2* Ab, int lda, size_t sA, int M, int K, uint16_t* Pb,
+                                                    size_t sP) {
+    const int e = blockIdx.y;
+    const int kg = K / 8;
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= static_cast<size_t>(M) * kg) return;
+    const int row = static_cast<int>(idx / kg), k0 = static_cast<int>(idx % kg) * 8;
+    const float4* pa = reinterpret_cast<const float4*>(Ab + e * sA + static_cast<size_t>(row) * lda + k0);
+    uint16_t part[3][NS][8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const float4 v = pa[u];
+        const float re[2] = {v.x, v.z}, im[2] = {v.y, v.w};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint16_t t[NS];
+            split_bf<NS>(re[h], t);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) part[0][s][2 * u + h] = t[s];
+            split_bf<NS>(im[h], t);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) part[1][s][2 * u + h] = t[s];
+            split_bf<NS>(re[h] + im[h], t);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) part[2][s][2 * u + h] = t[s];
+        }
+    }
+    uint16_t* P = Pb + e * sP;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            uint4 w;
+            w.x = part[q][s][0] | (static_cast<uint32_t>(part[q][s][1]) << 16);
+            w.y = part[q][s][2] | (static_cast<uint32_t>(part[q][s][3]) << 16);
+            w.z = part[q][s][4] | (static_cast<uint32_t>(part[q][s][5]) << 16);
+            w.w = part[q][s][6] | (static_cast<uint32_t>(part[q][s][7]) << 16);
+            *reinterpret_cast<uint4*>(P + ((static_cast<size_t>(q * NS + s) * M + row) * K + k0)) = w;
+        }
+}
+
+// B k [0, K) x columns [0, N) of a row-major complex panel -> P[e][q][s][col][k]
+template <int NS>
+__global__ __launch_bounds__(256) void k_split_cols(const float2* Bb, int ldb, size_t sB, int N, int K, uint16_t* Pb,
+                                                    size_t sP) {
+    const int e = blockIdx.y;
+    const int kg = K / 8;
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= static_cast<size_t>(N) * kg) return;
+    const int col = static_cast<int>(idx % N), k0 = static_cast<int>(idx / N) * 8;
+    const float2* pb = Bb + e * sB + static_cast<size_t>(k0) * ldb + col;
+    uint16_t part[3][NS][8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const float2 v = pb[static_cast<size_t>(u) * ldb];
+        uint16_t t[NS];
+        split_bf<NS>(v.x, t);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) part[0][s][u] = t[s];
+        split_bf<NS>(v.y, t);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) part[1][s][u] = t[s];
+        split_bf<NS>(v.x + v.y, t);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) part[2][s][u] = t[s];
+    }
+    uint16_t* P = Pb + e * sP;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            uint4 w;
+            w.x = part[q][s][0] | (static_cast<uint32_t>(part[q][s][1]) << 16);
+            w.y = part[q][s][2] | (static_cast<uint32_t>(part[q][s][3]) << 16);
+            w.z = part[q][s][4] | (static_cast<uint32_t>(part[q][s][5]) << 16);
+            w.w = part[q][s][6] | (static_cast<uint32_t>(part[q][s][7]) << 16);
+            *reinterpret_cast<uint4*>(P + ((static_cast<size_t>(q * NS + s) * N + col) * K + k0)) = w;
+        }
+}
+
+// C (M x N, row-major complex) -= A B from the split planes: 64 x 64 tile per
+// 4-wave workgroup (32 x 32 per wave, 2 x 2 MFMA tiles), K in steps of 32 with
+// the next step's planes prefetched into registers while the current one
+// multiplies out of LDS (rows padded to 40 bf16).  M, N multiples of 64, K of 32.
+template <int NS>
+__global__ __launch_bounds__(256, (NS == 2 ? 2 : 1)) void k_gemm_bf(float2* Cb, int ldc, size_t sC, const uint16_t* PAb,
+                                                                    size_t sPA, const uint16_t* PBb, size_t sPB, int M,
+                                                                    int N, int K) {
+    constexpr int NP = 3 * NS, RS = 40;  // planes, padded LDS row length (bf16)
+    __shared__ uint4 lds_a[NP * 64 * RS / 8], lds_b[NP * 64 * RS / 8];
+    const int tilesN = N / 64;
+    const int tile = blockIdx.x, e = blockIdx.y;
+    const int m0 = (tile / tilesN) * 64, n0 = (tile % tilesN) * 64;
+    const uint16_t* PA = PAb + e * sPA;
+    const uint16_t* PB = PBb + e * sPB;
+    float2* C = Cb + e * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int lr = tid >> 2, lk = (tid & 3) * 8;  // load: row of the tile, k offset (8 bf16 = 16 B)
+    uint4 ra[NP], rb[NP];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            ra[p] = *reinterpret_cast<const uint4*>(PA + (static_cast<size_t>(p) * M + m0 + lr) * K + k0 + lk);
+            rb[p] = *reinterpret_cast<const uint4*>(PB + (static_cast<size_t>(p) * N + n0 + lr) * K + k0 + lk);
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            lds_a[((p * 64 + lr) * RS + lk) / 8] = ra[p];
+            lds_b[((p * 64 + lr) * RS + lk) / 8] = rb[p];
+        }
+    };
+    v4f acc[3][2][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[q][i][j] = v4f{0, 0, 0, 0};
+    const int orow = lane & 15, ok = (lane >> 4) * 8;  // MFMA operand: row/col and k offset
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        sstore();
+        __syncthreads();
+        if (k0 + 32 < K) gload(k0 + 32);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v8bf a[NS][2], b[NS][2];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const uint4 va = lds_a[(((q * NS + s) * 64 + wr * 32 + i * 16 + orow) * RS + ok) / 8];
+                    const uint4 vb = lds_b[(((q * NS + s) * 64 + wc * 32 + i * 16 + orow) * RS + ok) / 8];
+                    a[s][i] = __builtin_bit_cast(v8bf, va);
+                    b[s][i] = __builtin_bit_cast(v8bf, vb);
+                }
+            // cross terms of total order < NS, smallest first
+#pragma unroll
+            for (int ord = NS - 1; ord >= 0; --ord)
+#pragma unroll
+                for (int sa = 0; sa <= ord; ++sa) {
+                    const int sb = ord - sa;
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+                            acc[q][i][j] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[sb][j], acc[q][i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int col = n0 + wc * 32 + j * 16 + (lane & 15);
+                float2* c = C + static_cast<size_t>(row) * ldc + col;
+                const float p1 = acc[0][i][j][r], p2 = acc[1][i][j][r], p3 = acc[2][i][j][r];
+                float2 v = *c;
+                v.x -= p1 - p2;
+                v.y -= p3 - p1 - p2;
+                *c = v;
+            }
+}
+
+// fp16 two-part split with power-of-two row (A) and column (B) scales: every
+// value is divided by the power of two just above its row's (column's)
+// largest |re|, |im|, |re + im|, so the parts fit fp16 and x = s (h + m) to
+// about 2^-22 of the row maximum (h = f16(x / s), m = f16(x / s - h)); the
+// three terms h h', h m', m h' on the f16 MFMA then give an FP32-like update.
+// Per matrix (stride sS uint16) the buffer holds PA[q][part][row][k] (6 M K),
+// PB[q][part][col][k] (6 N K), then the scales sA[M], sB[N] as floats.
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x)); }
+__device__ __forceinline__ float h2f(uint16_t h) { return static_cast<float>(__builtin_bit_cast(_Float16, h)); }
+
+__device__ __forceinline__ float pow2_above(float mx, float* inv) {  // s = 2^e, mx / s in [0.5, 1)
+    if (!(mx > 0.0f)) {
+        *inv = 1.0f;
+        return 1.0f;
+    }
+    int ex;
+    (void)frexpf(mx, &ex);
+    *inv = ldexpf(1.0f, -ex);
+    return ldexpf(1.0f, ex);
+}
+
+struct HSplit {
+    uint16_t* base;  // [e] sS
+    size_t sS;
+    int M, N, K;
+    __device__ uint16_t* pa(int e) const { return base + e * sS; }
+    __device__ uint16_t* pb(int e) const { return base + e * sS + static_cast<size_t>(6) * M * K; }
+    __device__ float* sa(int e) const {
+        return reinterpret_cast<float*>(base + e * sS + static_cast<size_t>(6) * (M + N) * K);
+    }
+    __device__ float* sb(int e) const { return sa(e) + M; }
+};
+
+// A panel (M x K, row-major complex), one wave per row
+__global__ __launch_bounds__(256) void k_hsplit_rows(const float2* Ab, int lda, size_t sA, HSplit h) {
+    const int e = blockIdx.y, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= h.M) return;
+    const float2* a = Ab + e * sA + static_cast<size_t>(row) * lda;
+    float mx = 0.0f;
+    for (int k = lane; k < h.K; k += 64) {
+        const float2 v = a[k];
+        mx = fmaxf(mx, fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.x + v.y))));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    float inv;
+    const float s = pow2_above(mx, &inv);
+    uint16_t* P = h.pa(e);
+    for (int k = lane; k < h.K; k += 64) {
+        const float2 v = a[k];
+        const float x[3] = {v.x * inv, v.y * inv, (v.x + v.y) * inv};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const uint16_t hi = f2h(x[q]);
+            P[(static_cast<size_t>(2 * q) * h.M + row) * h.K + k] = hi;
+            P[(static_cast<size_t>(2 * q + 1) * h.M + row) * h.K + k] = f2h(x[q] - h2f(hi));
+        }
+    }
+    if (lane == 0) h.sa(e)[row] = s;
+}
+
+// B panel (K x N, row-major complex), one thread per column, written as
+// [q][part][col][k] in 8-value (16-byte) chunks
+__global__ __launch_bounds__(256) void k_hsplit_cols(const float2* Bb, int ldb, size_t sB, HSplit h) {
+    const int e = blockIdx.y;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= h.N) return;
+    const float2* b = Bb + e * sB + col;
+    float mx = 0.0f;
+    for (int k = 0; k < h.K; ++k) {
+        const float2 v = b[static_cast<size_t>(k) * ldb];
+        mx = fmaxf(mx, fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.x + v.y))));
+    }
+    float inv;
+    const float s = pow2_above(mx, &inv);
+    uint16_t* P = h.pb(e);
+    for (int k0 = 0; k0 < h.K; k0 += 8) {
+        uint32_t w[6][4];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            uint16_t part[2][6];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const float2 v = b[static_cast<size_t>(k0 + u + t) * ldb];
+                const float x[3] = {v.x * inv, v.y * inv, (v.x + v.y) * inv};
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    part[t][2 * q] = f2h(x[q]);
+                    part[t][2 * q + 1] = f2h(x[q] - h2f(part[t][2 * q]));
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 6; ++p) w[p][u / 2] = part[0][p] | (static_cast<uint32_t>(part[1][p]) << 16);
+        }
+#pragma unroll
+        for (int p = 0; p < 6; ++p)
+            *reinterpret_cast<uint4*>(P + (static_cast<size_t>(p) * h.N + col) * h.K + k0) =
+                uint4{w[p][0], w[p][1], w[p][2], w[p][3]};
+    }
+    h.sb(e)[col] = s;
+}
+
+// C (M x N, row-major complex) -= A B from the fp16 planes: 128 x 128 tile
+// per 8-wave workgroup (2 x 4 waves of 64 x 32, 4 x 2 MFMA tiles of 16x16x32;
+// two waves per SIMD, so one wave's LDS reads overlap the other's MFMAs), K in
+// steps of 32 with the next step prefetched into registers while the current
+// one multiplies out of LDS (rows padded to 40 halves).  Within a plane the
+// three terms go term-major, so consecutive MFMAs update different
+// accumulators.  M, N multiples of 64, K of 32; rows / columns past M / N
+// read as zero and are not written.
+__global__ __launch_bounds__(512, 1) void k_gemm_h2(float2* Cb, int ldc, size_t sC, HSplit h) {
+    constexpr int RS = 40, CH = RS / 8;  // padded row (halves), uint4 per row
+    constexpr int NLD = 6 * 128 * 4 / 512;  // 16-byte loads per thread per operand and step
+    __shared__ uint4 lds_a[6 * 128 * CH], lds_b[6 * 128 * CH];
+    const int M = h.M, N = h.N, K = h.K;
+    const int tilesN = (N + 127) / 128;
+    const int e = blockIdx.y, m0 = (blockIdx.x / tilesN) * 128, n0 = (blockIdx.x % tilesN) * 128;
+    const uint16_t* PA = h.pa(e);
+    const uint16_t* PB = h.pb(e);
+    float2* C = Cb + e * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    uint4 ra[NLD], rb[NLD];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int lin = i * 512 + tid, p = lin >> 9, rem = lin & 511, r = rem >> 2, kc = rem & 3;
+            const bool va = m0 + r < M, vb = n0 + r < N;
+            const uint4* sa = reinterpret_cast<const uint4*>(PA + (static_cast<size_t>(p) * M + (va ? m0 + r : 0)) * K + k0 + kc * 8);
+            const uint4* sb = reinterpret_cast<const uint4*>(PB + (static_cast<size_t>(p) * N + (vb ? n0 + r : 0)) * K + k0 + kc * 8);
+            const uint4 xa = *sa, xb = *sb;
+            ra[i] = va ? xa : uint4{0, 0, 0, 0};
+            rb[i] = vb ? xb : uint4{0, 0, 0, 0};
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int lin = i * 512 + tid, p = lin >> 9, rem = lin & 511, r = rem >> 2, kc = rem & 3;
+            lds_a[(p * 128 + r) * CH + kc] = ra[i];
+            lds_b[(p * 128 + r) * CH + kc] = rb[i];
+        }
+    };
+    v4f acc[3][4][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[q][i][j] = v4f{0, 0, 0, 0};
+    const int orow = lane & 15, okc = lane >> 4;
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        __syncthreads();
+        sstore();
+        __syncthreads();
+        if (k0 + 32 < K) gload(k0 + 32);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v8h a[2][4], b[2][2];
+#pragma unroll
+            for (int part = 0; part < 2; ++part) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    a[part][i] = __builtin_bit_cast(v8h, lds_a[((2 * q + part) * 128 + wr * 64 + i * 16 + orow) * CH + okc]);
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    b[part][j] = __builtin_bit_cast(v8h, lds_b[((2 * q + part) * 128 + wc * 32 + j * 16 + orow) * CH + okc]);
+            }
+            constexpr int ta[3] = {0, 1, 0}, tb[3] = {1, 0, 0};  // h m', m h', h h'
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[ta[t]][i], b[tb[t]][j], acc[q][i][j],
+                                                                              0, 0, 0);
+        }
+    }
+    const float* sa = h.sa(e);
+    const float* sb = h.sb(e);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const int col = n0 + wc * 32 + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    const float s = sa[row] * sb[col];
+                    const float p1 = acc[0][i][j][r], p2 = acc[1][i][j][r], p3 = acc[2][i][j][r];
+                    float2* c = C + static_cast<size_t>(row) * ldc + col;
+                    float2 v = *c;
+                    v.x -= s * (p1 - p2);
+                    v.y -= s * (p3 - p1 - p2);
+                    *c = v;
+                }
+            }
+}
+
 // C -= A * B, FP32 complex in the 3M form, with a larger register tile than
 // k_gemm_v2: each of the 4 waves computes (16 TI) x (16 TJ) of C, the
 // workgroup (32 TI) x (32 TJ).  K in steps of BK with register prefetch of the
@@ -2838,6 +3246,9 @@ struct refeff_fms_plan {
     bool trace = false;                 // REFEFF_FMS_TRACE: print refinement progress to stderr
     bool gemm_3m = true;                // REFEFF_FMS_GEMM_3M=0: FP32 trailing updates with four real products
     bool gemm_cpf = true;               // REFEFF_FMS_GEMM_CPF=0: C tile read in the epilogue
+    int gemm_bf16 = 0;                  // REFEFF_FMS_GEMM_BF16=2|3: trailing update on split-bf16 MFMA
+    uint16_t* d_split = nullptr;        // its operand planes, [e] split_stride bf16
+    size_t split_stride = 0;
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     bool lookahead = false;             // REFEFF_FMS_LOOKAHEAD=1: far trailing update on a second stream
     hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -2883,7 +3294,7 @@ void free_workspace(refeff_fms_plan* p) {
     p->d_dinv = nullptr;
     dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_cand); dfree(p->d_iter); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
     dfree(p->d_out); dfree(p->d_norms);
-    dfree(p->d_kv); dfree(p->d_pd); dfree(p->d_pn); dfree(p->d_bcol);
+    dfree(p->d_kv); dfree(p->d_pd); dfree(p->d_pn); dfree(p->d_bcol); dfree(p->d_split);
     p->ws_batch = 0;
 }
 
@@ -2966,6 +3377,9 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
     b += 2 * p->npad * sizeof(int) + 2 * p->n * sizeof(double2);
     b += static_cast<size_t>(p->ch) * p->ch * p->npot * sizeof(double2);
     if (krylov_solver(p->solver)) b += static_cast<size_t>(kIterVecs) * nr * sizeof(double2);
+    if (mixed && p->gemm_bf16)
+        b += (static_cast<size_t>(3) * (p->gemm_bf16 == 3 ? 3 : 2) * 2 * p->npad * p->outer_block +
+              4 * static_cast<size_t>(p->npad)) * sizeof(uint16_t);
     if (p->bicg)
         b += static_cast<size_t>(kBicgVecs) * nr * sizeof(float2) +
              static_cast<size_t>(p->npad / kPanel) * p->nrhs * (sizeof(double2) + sizeof(double)) +
@@ -2987,6 +3401,11 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     ok = ok && hipMalloc(&p->d_w, nr * nb * cs) == hipSuccess;
     ok = ok && hipMalloc(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs) == hipSuccess;
     if (mixed) ok = ok && alloc_calu(p, nb);
+    if (mixed && p->gemm_bf16) {
+        p->split_stride = static_cast<size_t>(3) * (p->gemm_bf16 == 3 ? 3 : 2) * 2 * p->npad * p->outer_block +
+                          4 * static_cast<size_t>(p->npad);
+        ok = ok && dalloc(&p->d_split, p->split_stride * nb);
+    }
     if (krylov_solver(p->solver))
         ok = ok && dalloc(&p->d_iter, static_cast<size_t>(kIterVecs) * p->nrhs * p->npad * nb);
     if (p->bicg) {
@@ -3063,6 +3482,34 @@ void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, typename Cx<T>::typ
     }
 }
 
+// Split-bf16 trailing update for nb matrices: the planes of A (M x K) and B
+// (K x N) go to split (per matrix: 3 NS (M + N) K bf16, stride ssplit), then
+// k_gemm_bf.  M, N multiples of 64, K of 32, 16-byte aligned A rows.
+void gemm_bf_launch(hipStream_t s, int ns, float2* C, int ldc, size_t sC, const float2* A, int lda, size_t sA,
+                    const float2* B, int ldb, size_t sB, int M, int N, int K, int nb, uint16_t* split, size_t ssplit) {
+    if (ns == 16) {  // fp16 two-part split with row / column scales
+        const HSplit hs{split, ssplit, M, N, K};
+        hipLaunchKernelGGL(k_hsplit_rows, dim3((M + 3) / 4, nb), dim3(256), 0, s, A, lda, sA, hs);
+        hipLaunchKernelGGL(k_hsplit_cols, dim3((N + 255) / 256, nb), dim3(256), 0, s, B, ldb, sB, hs);
+        hipLaunchKernelGGL(k_gemm_h2, dim3(((M + 127) / 128) * ((N + 127) / 128), nb), dim3(512), 0, s, C, ldc, sC, hs);
+        return;
+    }
+    uint16_t* pa = split;
+    uint16_t* pb = split + static_cast<size_t>(3) * ns * M * K;
+    const unsigned ga = static_cast<unsigned>((static_cast<size_t>(M) * (K / 8) + 255) / 256);
+    const unsigned gb = static_cast<unsigned>((static_cast<size_t>(N) * (K / 8) + 255) / 256);
+    const int tiles = (M / 64) * (N / 64);
+    if (ns == 2) {
+        hipLaunchKernelGGL(k_split_rows<2>, dim3(ga, nb), dim3(256), 0, s, A, lda, sA, M, K, pa, ssplit);
+        hipLaunchKernelGGL(k_split_cols<2>, dim3(gb, nb), dim3(256), 0, s, B, ldb, sB, N, K, pb, ssplit);
+        hipLaunchKernelGGL(k_gemm_bf<2>, dim3(tiles, nb), dim3(256), 0, s, C, ldc, sC, pa, ssplit, pb, ssplit, M, N, K);
+    } else {
+        hipLaunchKernelGGL(k_split_rows<3>, dim3(ga, nb), dim3(256), 0, s, A, lda, sA, M, K, pa, ssplit);
+        hipLaunchKernelGGL(k_split_cols<3>, dim3(gb, nb), dim3(256), 0, s, B, ldb, sB, N, K, pb, ssplit);
+        hipLaunchKernelGGL(k_gemm_bf<3>, dim3(tiles, nb), dim3(256), 0, s, C, ldc, sC, pa, ssplit, pb, ssplit, M, N, K);
+    }
+}
+
 template <typename T>
 void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int ldc, size_t sC,
               const typename Cx<T>::type* A, int lda, size_t sA, const typename Cx<T>::type* B, int ldb, size_t sB,
@@ -3075,6 +3522,20 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
     // aligned, N must be even and K a multiple of 16
     const bool v2 = sizeof(T) == 4 && (N % 2 == 0) && (K % 16 == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
                     ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % 16 == 0);
+    // the split-bf16 path for the trailing update (K = outer block)
+    const bool bf = sizeof(T) == 4 && p->gemm_bf16 && p->d_split && K > kPanel && M % 64 == 0 && N % 64 == 0 &&
+                    K % 32 == 0 && K <= p->outer_block && M <= p->npad && N <= p->npad && lda % 2 == 0 &&
+                    reinterpret_cast<uintptr_t>(A) % 16 == 0;
+    if (bf) {
+        const int terms = p->gemm_bf16 == 3 ? 6 : 3;
+        launch(p, g.s, REFEFF_FMS_K_GEMM, 3.0 * terms * 2.0 * M * N * K * nb,
+               (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
+                   gemm_bf_launch(g.s, p->gemm_bf16, reinterpret_cast<float2*>(C), ldc, sC,
+                                  reinterpret_cast<const float2*>(A), lda, sA, reinterpret_cast<const float2*>(B), ldb,
+                                  sB, M, N, K, nb, p->d_split + p->split_stride * g.e0, p->split_stride);
+               });
+        return;
+    }
     // flops as executed: 6 M N K real for the 3M form, 8 M N K otherwise
     launch(p, g.s, K > kPanel ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL,
            (v2 && p->gemm_3m ? 6.0 : 8.0) * M * N * K * nb,
@@ -3868,6 +4329,10 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->trace = std::getenv("REFEFF_FMS_TRACE") != nullptr;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_3M")) p->gemm_3m = std::atoi(e) != 0;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_CPF")) p->gemm_cpf = std::atoi(e) != 0;
+    if (const char* e = std::getenv("REFEFF_FMS_GEMM_BF16")) {
+        const int v = std::atoi(e);
+        p->gemm_bf16 = (v == 2 || v == 3 || v == 16) ? v : 0;  // bf16x2, bf16x3, fp16x2
+    }
     if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p->panel_calu = std::strcmp(e, "gepp") != 0;
     if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
@@ -4152,9 +4617,10 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
 int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, const double* A, const double* B,
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
-    const int kmul = (use_double == 0 || use_double == 3 || use_double == 5) ? 32 : 16;
-    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 11)
+    const int kmul = (use_double == 0 || use_double == 3 || use_double == 5 || use_double >= 12) ? 32 : 16;
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 14)
         return REFEFF_FMS_INVALID_ARGUMENT;
+    if (use_double >= 12 && (M % 64 || N % 64)) return REFEFF_FMS_INVALID_ARGUMENT;
     const int ld = std::max(std::max(M, N), K);
     const size_t sz = static_cast<size_t>(ld) * ld;
     int rc = REFEFF_FMS_OK;
@@ -4172,7 +4638,20 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
         if (!dupload(&da, ha.data(), sz) || !dupload(&db, hb.data(), sz) || !dupload(&dc, hc.data(), sz)) {
             rc = REFEFF_FMS_DEVICE_ERROR;
         } else {
-            if (use_double >= 3 && sizeof(T) == 4) {
+            if (use_double >= 12 && sizeof(T) == 4) {
+                // kinds 12 / 13: split-bf16 trailing update with 2 / 3 parts
+                const int ns = use_double == 12 ? 2 : use_double == 13 ? 3 : 16;
+                uint16_t* sp = nullptr;
+                const size_t ss = static_cast<size_t>(3) * (ns == 3 ? 3 : 2) * (M + N) * K + 2 * static_cast<size_t>(M + N);
+                if (!dalloc(&sp, ss)) {
+                    rc = REFEFF_FMS_DEVICE_ERROR;
+                } else {
+                    gemm_bf_launch(nullptr, ns, reinterpret_cast<float2*>(dc), ld, sz, reinterpret_cast<const float2*>(da),
+                                   ld, sz, reinterpret_cast<const float2*>(db), ld, sz, M, N, K, 1, sp, ss);
+                    (void)hipDeviceSynchronize();
+                    dfree(sp);
+                }
+            } else if (use_double >= 3 && sizeof(T) == 4) {
                 const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
                 auto* c2 = reinterpret_cast<float2*>(dc);
                 auto* a2 = reinterpret_cast<const float2*>(da);

@@ -50,7 +50,17 @@ def test_gemm_v2(kind, m, n, k):
     _gemm_case(kind, m, n, k)
 
 
-def _gemm_case(use_double, m, n, k):
+@pytest.mark.parametrize("kind,tol", [(12, 6e-5), (13, 2e-5), (14, 2e-5)])
+@pytest.mark.parametrize("m,n,k", [(64, 64, 32), (256, 256, 128), (320, 448, 64), (1024, 512, 256), (192, 640, 96)])
+def test_gemm_split_bf16(kind, tol, m, n, k):
+    """Trailing update on the bf16 MFMA from split operands: two parts (kind 12,
+    relative error of order 2^-17 per product) and three parts (kind 13, FP32-like);
+    on the f16 MFMA from two parts with power-of-two row / column scales (kind 14,
+    FP32-like, 128 x 128 tiles: the 64-multiple shapes leave partial tiles)."""
+    _gemm_case(kind, m, n, k, tol)
+
+
+def _gemm_case(use_double, m, n, k, tol=None):
     print(f"[gpu-kernels] gemm double={use_double} {m}x{n}x{k}", flush=True)
     rng = np.random.default_rng(m + n + k)
     a, b, c = crand(rng, m, k), crand(rng, k, n), crand(rng, m, n)
@@ -59,7 +69,8 @@ def _gemm_case(use_double, m, n, k):
     rc = _lib().refeff_fms_debug_gemm(use_double, m, n, k, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)),
                                       _p(out))
     assert rc == 0
-    tol = 1e-12 if use_double == 1 else 2e-5
+    if tol is None:
+        tol = 1e-12 if use_double == 1 else 2e-5
     assert out.shape == ref.shape
     assert np.abs(out - ref).max() <= tol * np.abs(ref).max()
 
