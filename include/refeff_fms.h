@@ -299,6 +299,17 @@ int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_w
  * traces return REFEFF_FMS_UNSUPPORTED. */
 int refeff_mkgtr_run_in_dir(const char* work_dir, int32_t* files_written, char* message, size_t message_len);
 
+/* LDOS gtrNN.bin from the FMS source handoffs.  Replaces
+ * write_ldos_gtr_bin_source_handoffs, crates/refeff-engine/src/fms.rs:1226-1247
+ * (with :1689-1760, 2095-2245): reads ldos.inp, fms.inp, phase.bin, geom.dat and
+ * global.inp; for every potential c solves the cluster centred on c's first atom
+ * with the ldos.inp FMS controls and writes gtrNN.bin with FEFF's fmsdos trace of
+ * potential c.  Writes nothing (status OK, *files_written = 0) where refeff
+ * leaves the work to its other LDOS paths; keeps a gtrNN.bin that already
+ * matches.  solver: as refeff_fms_run_in_dir. */
+int refeff_ldos_gtr_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_written, char* message,
+                               size_t message_len);
+
 /* ---------------------------------------------------------------------- */
 /* text codecs of the FEFF handoff files (crates/refeff-io)                 */
 /* ---------------------------------------------------------------------- */
@@ -432,6 +443,27 @@ int refeff_io_gg_slice_bin_parse(const char* bytes, size_t len, int32_t* dims, f
 int refeff_io_gg_diag_bin_bytes(int32_t energy_count, int32_t atom_count, int32_t dim, const float* values, char* out,
                                 size_t out_len, size_t* needed);
 int refeff_io_gg_diag_bin_parse(const char* bytes, size_t len, int32_t* dims, float* values);
+
+/* gtrNN.bin, the FMS -> LDOS trace handoff (binary: *needed = byte count).
+ * Replace gtr_bin_bytes / parse_gtr_bin, crates/refeff-io/src/gtr_bin.rs:86-190.
+ * header: [6] = ne, ne1, ne3, nph, ifms, angular channel count; values:
+ * [ne][nph+1][angular] complex, written as f32.  Parsing accepts either byte
+ * order; pass values = NULL to read the header only. */
+int refeff_io_gtr_bin_bytes(const int32_t* header, const double* values, char* out, size_t out_len, size_t* needed);
+int refeff_io_gtr_bin_parse(const char* bytes, size_t len, int32_t* header, double* values);
+
+/* ldos.inp.  Replaces LdosInput::parse_str / ldos_input_string,
+ * crates/refeff-io/src/ldos_input.rs:59-300. */
+typedef struct refeff_io_ldos_inp {
+    int32_t mldos, lfms2, ixc, ispin, minv, neldos, iscfxc;
+    double rfms2, emin, emax, eimag, rgrd;
+    double rdirec, toler1, toler2;
+    int32_t lmaxph_count;
+    int32_t lmaxph[64];
+    int32_t ldostype;
+} refeff_io_ldos_inp;
+int refeff_io_ldos_inp_parse(const char* text, refeff_io_ldos_inp* out);
+int refeff_io_ldos_inp_string(const refeff_io_ldos_inp* in, char* out, size_t out_len, size_t* needed);
 
 /* sig2.dat pair damping values (idwopt = 4): the third field of the first
  * pair_count data rows (pairs atom1 < atom2, atom1 outer), each finite and >= 0.

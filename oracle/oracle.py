@@ -415,3 +415,24 @@ def debye_sigsqr(model, tk, thetad, rsavg, positions, atomic_numbers, sig2g):
     _check(_lib.orc_debye_sigsqr(model, tk, thetad, rsavg, n, p.reshape(-1),
                                  np.ascontiguousarray(atomic_numbers, np.int32), sig2g, out), f"debye sigsqr model={model}")
     return out.reshape(n, n).T.copy()  # [atom2, atom1]
+
+
+def ldos_fmsdos_trace_grid(scattering, phases, angular_count, spin_index=0):
+    """FEFF fmsdos trace of the FMS scattering blocks, ldos_fmsdos_trace /
+    ldos_fmsdos_trace_grid (crates/refeff-core/src/density/ldos.rs:403-484):
+    for every energy, potential and l < angular_count,
+    sum_m gg[l^2 + m, l^2 + m] exp(2 i phase[spin, L + l]) / (2 l + 1), with L the
+    centre of the signed-l phase axis.
+    scattering: [ne, npot, ch, ch] (row, col); phases: [ne, nsp, 2L + 1, npot].
+    Returns [ne, npot, angular_count] complex128 (the gtrNN.bin order)."""
+    sc = np.asarray(scattering, np.complex128)
+    ph = np.asarray(phases, np.complex128)
+    ne, npot = sc.shape[0], sc.shape[1]
+    off = ph.shape[2] // 2
+    out = np.zeros((ne, npot, angular_count), np.complex128)
+    for e in range(ne):
+        for ip in range(npot):
+            for l in range(angular_count):
+                diag = sum(sc[e, ip, l * l + m, l * l + m] for m in range(2 * l + 1))
+                out[e, ip, l] = diag * np.exp(2j * ph[e, spin_index, off + l, ip]) / (2 * l + 1)
+    return out

@@ -96,22 +96,51 @@ float narrow_nonnegative(double v, const char* what) {
     return f;
 }
 
-// Controls the module stage supports: refeff fms.rs:971-1000 (finite radii,
-// sig2g >= 0, toler1 > 0, toler2 >= 0) plus what this library provides.
-void validate_controls(const rio::FmsInp& in) {
+std::string num(double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%g", v);
+    return buf;
+}
+
+// validate_supported_source_fms_controls (refeff engine fms.rs:975-1043): the
+// Debye controls (tk >= 0 and thetad > 0 for idwopt 0 / 3, tk >= 0 for 1 / 2,
+// no other idwopt >= 0 than 0-5), then finite radii, sig2g >= 0, toler1 > 0,
+// toler2 >= 0.
+void validate_source_controls(const rio::FmsInp& in) {
+    const std::string id = "FMS idwopt=" + std::to_string(in.idwopt);
+    if (in.idwopt == 0 || in.idwopt == 3 || in.idwopt == 1 || in.idwopt == 2) {
+        if (!std::isfinite(in.tk) || in.tk < 0.0)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, id + " source generation requires finite nonnegative temperature, got " + num(in.tk));
+        if ((in.idwopt == 0 || in.idwopt == 3) && (!std::isfinite(in.thetad) || in.thetad <= 0.0))
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT,
+                       id + " source generation requires positive finite Debye temperature, got " + num(in.thetad));
+    } else if (in.idwopt > 5) {
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT,
+                   "FMS source generation received unexpected idwopt=" + std::to_string(in.idwopt) + " Debye-Waller damping");
+    }
     if (!std::isfinite(in.rfms2) || !std::isfinite(in.rdirec) ||
         (in.do_fms != 0 && (in.rfms2 < 0.0 || in.rdirec < 0.0)))
         stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative cluster radii for full FMS");
     if (!std::isfinite(in.sig2g) || in.sig2g < 0.0)
-        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative global SIG2");
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative global SIG2, got " + num(in.sig2g));
     if (!std::isfinite(in.toler1) || in.toler1 <= 0.0)
-        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires positive finite toler1");
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires positive finite toler1, got " + num(in.toler1));
     if (!std::isfinite(in.toler2) || in.toler2 < 0.0)
-        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative toler2");
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation requires finite nonnegative toler2, got " + num(in.toler2));
+}
+
+// What this library provides on top: the spring (equation of motion,
+// recursion) and DMDW Debye-Waller models are not.
+void check_provided_debye(const rio::FmsInp& in) {
     if (in.idwopt == 1 || in.idwopt == 2 || in.idwopt == 5)
         stage_fail(REFEFF_FMS_UNSUPPORTED, "Debye-Waller model idwopt=" + std::to_string(in.idwopt) +
                                                " is not provided; only the global SIG2 (idwopt < 0), the correlated "
                                                "(0) and classical (3) Debye models and sig2.dat (4)");
+}
+
+void validate_controls(const rio::FmsInp& in) {
+    validate_source_controls(in);
+    check_provided_debye(in);
 }
 
 // Pair sigsqr(atom2, atom1) at atom2 + nat*atom1 for the cluster atoms:
@@ -196,28 +225,26 @@ struct FmsStageResult {
     rio::GgDiagBin diag;
 };
 
-// gg sections from phase.bin / geom.dat / global.inp: refeff
-// build_fms_source_outputs, crates/refeff-engine/src/fms.rs:3549-3695, with
-// fms_atoms_from_geom (:4286-4322), fms_mean_square_displacements (:4345-4433,
-// idwopt < 0, 0, 3 and 4), fms_wave_numbers (:4972-4985, kept
-// in FP64), fms_phase_shifts_for_energy (:5002-5054, kept in FP64) and
-// fms_gg_section_values (:2004-2047).
-std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsInp& in, int solver,
-                                         FmsStageResult& res) {
-    const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
-    const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
-    const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
-    if (rio::file_exists(join(dir, "reciprocal.inp")))
-        stage_fail(REFEFF_FMS_UNSUPPORTED, "reciprocal-space FMS (reciprocal.inp) is not provided");
-    if (rio::file_exists(join(dir, "hubbard.inp")) && rio::file_exists(join(dir, "v_hubbard.bin")))
-        stage_fail(REFEFF_FMS_UNSUPPORTED, "active Hubbard FMS sources are not provided");
+// One FMS solve over a work directory's cluster: f32 coordinates from
+// geom.dat, yprep around the first atom of `central` (fms_atoms_from_geom,
+// crates/refeff-engine/src/fms.rs:4286-4322), sortat when `sortat` (the FMS
+// stage; the LDOS caller keeps yprep order, :2095-2130), pair sigsqr
+// (fms_mean_square_displacements, :4345-4433, idwopt < 0, 0, 3, 4), wave
+// numbers and phase shifts of phase.bin kept in FP64 (fms_wave_numbers
+// :4972-4985, fms_phase_shifts_for_energy :5002-5054), then the plan and the
+// energy sweep on the GPU.  sc: [e][ip] ch x ch blocks, gg(row, col) at row + ch col.
+struct ClusterSolve {
+    int ne = 0, nsp = 1, npot = 0, ch = 0, L = 0, nat = 0, absorber = 0;
+    size_t nst = 0;
+    std::vector<double> sc, ph, full;
+};
+
+ClusterSolve solve_cluster(const std::string& dir, const rio::FmsInp& in, const rio::PhaseBin& phase,
+                           const rio::GlobalInp& global, const rio::GeomDat& geom, int solver, int central_pot,
+                           bool sortat, bool full_request, const std::string& what) {
+    ClusterSolve cs;
     const int npot = static_cast<int>(phase.potentials.size());
     const int max_pot = npot - 1;
-    if (geom.nph != max_pot)
-        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "geom.dat nph " + std::to_string(geom.nph) +
-                                                    " does not match phase.bin maximum potential " + std::to_string(max_pot));
-    if (static_cast<int>(in.lmaxph.size()) <= max_pot)
-        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp has too few lmaxph values for FMS generation");
     int global_lmax = 0;
     for (int ip = 0; ip <= max_pot; ++ip) {
         if (in.lmaxph[ip] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp lmaxph must be nonnegative");
@@ -228,8 +255,6 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
     const float radius = in.rfms2 >= 0.0 ? narrow_nonnegative(in.rfms2, "FMS cluster radius") : 0.0f;
     const float rdirec = in.rdirec >= 0.0 ? narrow_nonnegative(in.rdirec, "FMS direct cutoff") : 0.0f;
     if (geom.atoms.empty()) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
-
-    // cluster: f32 coordinates, yprep (absorber = first atom of potential 0), sortat
     const int nat_all = static_cast<int>(geom.atoms.size());
     std::vector<float> pos(3 * static_cast<size_t>(nat_all));
     std::vector<int32_t> pot(nat_all);
@@ -242,25 +267,19 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
     std::vector<float> cpos(pos.size());
     std::vector<int32_t> cpot(nat_all);
     int32_t nat = 0, central = 0, untruncated = 0;
-    check_status(refeff_fms_yprep_cluster(nat_all, pos.data(), pot.data(), 0, radius, nat_all, cpos.data(), cpot.data(),
-                                          &nat, &central, &untruncated),
+    check_status(refeff_fms_yprep_cluster(nat_all, pos.data(), pot.data(), central_pot, radius, nat_all, cpos.data(),
+                                          cpot.data(), &nat, &central, &untruncated),
                  "failed to select FMS cluster from geom.dat");
     if (in.do_fms != 0 && nat <= max_pot)
         stage_fail(REFEFF_FMS_MISSING_REPRESENTATIVE, "FMS cluster has " + std::to_string(nat) +
                                                           " atom(s), but full-potential generation needs representatives through potential " +
                                                           std::to_string(max_pot));
-    std::vector<int32_t> reps(npot);
-    check_status(refeff_fms_sort_representative_atoms(0, max_pot, nat, cpos.data(), cpot.data(), reps.data()),
-                 "failed to prepare FMS representative atoms from geom.dat");
-    if (cpot[0] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS absorber potential must be nonnegative");
-    const int absorber = cpot[0];
-
-    int nspu = phase.spin_count;
-    if (rio::file_exists(join(dir, ".dimensions.dat"))) {
-        nspu = rio::parse_dimensions_nspu(rio::read_file(join(dir, ".dimensions.dat")));
-        if (nspu <= 0 || nspu < phase.spin_count)
-            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, ".dimensions.dat nspu must be positive and at least the phase.bin spin count");
+    if (sortat) {
+        std::vector<int32_t> reps(npot);
+        check_status(refeff_fms_sort_representative_atoms(0, max_pot, nat, cpos.data(), cpot.data(), reps.data()),
+                     "failed to prepare FMS representative atoms from geom.dat");
     }
+    if (cpot[0] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS absorber potential must be nonnegative");
     const std::vector<float> sig = pair_sigsqr(dir, in, phase, nat, cpos, cpot);
 
     refeff_fms_plan_desc d{};
@@ -279,17 +298,18 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
     d.mean_square_displacements = sig.data();
     d.convergence_tolerance = static_cast<float>(in.toler1);
     d.zero_tolerance = static_cast<float>(in.toler2);
-    d.full_scattering_requested = in.save_gg_slice ? 1 : 0;
+    d.full_scattering_requested = full_request ? 1 : 0;
     d.solver = solver;
     d.max_batch = 0;
     refeff_fms_plan* plan = nullptr;
-    check_status(refeff_fms_plan_create(&d, &plan), "failed to prepare FMS real-space plan");
+    check_status(refeff_fms_plan_create(&d, &plan), "failed to prepare " + what + " real-space plan");
     refeff_fms_plan_info info{};
     int rc = refeff_fms_plan_query(plan, &info);
 
     const int ne = phase.energy_count, nsp = phase.spin_count, L = global_lmax;
     const int nl = 2 * L + 1;
-    std::vector<double> k(2 * static_cast<size_t>(ne) * nsp), ph(2 * static_cast<size_t>(ne) * nsp * nl * npot, 0.0);
+    std::vector<double> k(2 * static_cast<size_t>(ne) * nsp);
+    cs.ph.assign(2 * static_cast<size_t>(ne) * nsp * nl * npot, 0.0);
     for (int e = 0; e < ne; ++e)
         for (int s = 0; s < nsp; ++s) {
             const cplx w = std::sqrt(cplx(2.0, 0.0) * (phase.energy_grid[e] - phase.reference_energy[e * nsp + s])) / 0.529177249;
@@ -304,39 +324,80 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
                 for (int sl = -eff; sl <= eff; ++sl) {
                     const cplx v = pp.phase_shifts[(static_cast<size_t>(e) * pnl + (sl + pp.lmax)) * nsp + s];
                     const size_t o = ((static_cast<size_t>(e) * nsp + s) * nl + (sl + L)) * npot + ip;
-                    ph[2 * o] = v.real();
-                    ph[2 * o + 1] = v.imag();
+                    cs.ph[2 * o] = v.real();
+                    cs.ph[2 * o + 1] = v.imag();
                 }
     }
     const int ch = info.channel_count;
-    std::vector<double> sc(2 * static_cast<size_t>(ne) * npot * ch * ch, 0.0);
+    cs.sc.assign(2 * static_cast<size_t>(ne) * npot * ch * ch, 0.0);
     std::vector<int32_t> status(ne, 0);
-    std::vector<double> full;
-    const size_t nst = static_cast<size_t>(info.state_count);
+    cs.nst = static_cast<size_t>(info.state_count);
     // save_gg_slice: fixed-width atom blocks of nsp (global_lmax+1)^2 states
     // (engine fms.rs:4144-4173)
     const int bd = nsp * (L + 1) * (L + 1);
-    if (in.save_gg_slice && rc == REFEFF_FMS_OK) {
-        if (nst != static_cast<size_t>(nat) * bd) {
+    if (full_request && rc == REFEFF_FMS_OK) {
+        if (cs.nst != static_cast<size_t>(nat) * bd) {
             refeff_fms_plan_destroy(plan);
             stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS save_gg_slice requires fixed-width atom blocks: got " +
-                                                        std::to_string(nst) + " state(s), expected " +
+                                                        std::to_string(cs.nst) + " state(s), expected " +
                                                         std::to_string(static_cast<size_t>(nat) * bd) + " from " +
                                                         std::to_string(nat) + " atom(s) and block dimension " +
                                                         std::to_string(bd));
         }
-        full.assign(2 * static_cast<size_t>(ne) * nst * nst, 0.0);
-        rc = refeff_fms_solve_batch_full(plan, ne, k.data(), ph.data(), sc.data(), full.data(), status.data(), nullptr);
+        cs.full.assign(2 * static_cast<size_t>(ne) * cs.nst * cs.nst, 0.0);
+        rc = refeff_fms_solve_batch_full(plan, ne, k.data(), cs.ph.data(), cs.sc.data(), cs.full.data(), status.data(),
+                                         nullptr);
     } else if (rc == REFEFF_FMS_OK) {
-        rc = refeff_fms_solve_batch(plan, ne, k.data(), ph.data(), sc.data(), status.data(), nullptr);
+        rc = refeff_fms_solve_batch(plan, ne, k.data(), cs.ph.data(), cs.sc.data(), status.data(), nullptr);
     }
     refeff_fms_plan_destroy(plan);
     for (int e = 0; e < ne; ++e)
-        if (status[e]) stage_fail(status[e], "failed to solve FMS energy section " + std::to_string(e + 1));
-    check_status(rc, "failed to solve the FMS energy sweep");
+        if (status[e]) stage_fail(status[e], "failed to solve " + what + " energy section " + std::to_string(e + 1));
+    check_status(rc, "failed to solve the " + what + " energy sweep");
+    cs.ne = ne;
+    cs.nsp = nsp;
+    cs.npot = npot;
+    cs.ch = ch;
+    cs.L = L;
+    cs.nat = nat;
+    cs.absorber = cpot[0];
+    return cs;
+}
+
+// gg sections from phase.bin / geom.dat / global.inp: refeff
+// build_fms_source_outputs, crates/refeff-engine/src/fms.rs:3549-3695, over
+// solve_cluster (absorber = first atom of potential 0, sortat) and
+// fms_gg_section_values (:2004-2047).
+std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsInp& in, int solver,
+                                         FmsStageResult& res) {
+    const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
+    const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
+    const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
+    if (rio::file_exists(join(dir, "reciprocal.inp")))
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "reciprocal-space FMS (reciprocal.inp) is not provided");
+    if (rio::file_exists(join(dir, "hubbard.inp")) && rio::file_exists(join(dir, "v_hubbard.bin")))
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "active Hubbard FMS sources are not provided");
+    const int npot = static_cast<int>(phase.potentials.size());
+    const int max_pot = npot - 1;
+    if (geom.nph != max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "geom.dat nph " + std::to_string(geom.nph) +
+                                                    " does not match phase.bin maximum potential " + std::to_string(max_pot));
+    if (static_cast<int>(in.lmaxph.size()) <= max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp has too few lmaxph values for FMS generation");
+    int nspu = phase.spin_count;
+    if (rio::file_exists(join(dir, ".dimensions.dat"))) {
+        nspu = rio::parse_dimensions_nspu(rio::read_file(join(dir, ".dimensions.dat")));
+        if (nspu <= 0 || nspu < phase.spin_count)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, ".dimensions.dat nspu must be positive and at least the phase.bin spin count");
+    }
+    const ClusterSolve cs = solve_cluster(dir, in, phase, global, geom, solver, 0, true, in.save_gg_slice, "FMS");
+    const int ne = cs.ne, nsp = cs.nsp, L = cs.L, ch = cs.ch, nat = cs.nat, absorber = cs.absorber;
+    const size_t nst = cs.nst;
+    const std::vector<double>& sc = cs.sc;
     if (in.save_gg_slice) {
         // build_saved_fms_scattering_outputs (engine fms.rs:4154-4216): the
         // absorber rows of A^-1 G0 and every atom's diagonal block, as c32
+        const int bd = nsp * (L + 1) * (L + 1);
         res.saved_scattering = true;
         res.slice.energy_count = ne;
         res.slice.row_count = bd;
@@ -348,7 +409,7 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
         res.diag.values.resize(static_cast<size_t>(ne) * nat * bd * bd);
         auto at = [&](int e, size_t r, size_t c) {  // full is column-major per energy
             const size_t o = static_cast<size_t>(e) * nst * nst + r + nst * c;
-            return rio::cplxf(static_cast<float>(full[2 * o]), static_cast<float>(full[2 * o + 1]));
+            return rio::cplxf(static_cast<float>(cs.full[2 * o]), static_cast<float>(cs.full[2 * o + 1]));
         };
         for (int e = 0; e < ne; ++e) {
             for (int r = 0; r < bd; ++r)
@@ -477,6 +538,122 @@ int run_fms(const std::string& dir, int solver, int32_t* files_written) {
     return REFEFF_FMS_OK;
 }
 
+// LDOS gtrNN.bin from the FMS source handoffs: refeff
+// write_ldos_gtr_bin_source_handoffs (crates/refeff-engine/src/fms.rs:1226-1247),
+// build_ldos_gtr_bin_source_handoffs (:1689-1760) and
+// build_ldos_gtr_bin_for_central_potential (:2095-2245).  For every potential c
+// the cluster is centred on its first atom (yprep order kept) with the ldos.inp
+// FMS controls laid over fms.inp (:1941-1952), solved on the GPU, and traced as
+// FEFF fmsdos does (refeff-core density/ldos.rs:403-484): for l <= global lmax,
+// sum_m gg(l^2 + m, l^2 + m) exp(2 i delta_l) / (2 l + 1) of potential c, the
+// other potentials' columns zero.  A setup refeff leaves to its other paths
+// (spin, non-fmsdos LDOS, another energy grid, missing optional inputs) writes
+// no file; a gtrNN.bin already matching to 2e-6 + 1e-5 |x| is kept
+// (gtr_bin_matches_source_output, :1592-1628).
+int run_ldos_gtr(const std::string& dir, int solver, int32_t* files_written) {
+    if (files_written) *files_written = 0;
+    const std::string ldos_path = join(dir, "ldos.inp");
+    const rio::LdosInp ldos = rio::parse_ldos_inp(rio::read_file(ldos_path), ldos_path);
+    if (!(ldos.mldos == 1 && ldos.ispin == 0 && ldos.ldostype <= 0)) return REFEFF_FMS_OK;
+    if (!rio::file_exists(join(dir, "fms.inp")) || !rio::file_exists(join(dir, "phase.bin")) ||
+        !rio::file_exists(join(dir, "geom.dat")) || !rio::file_exists(join(dir, "global.inp")))
+        return REFEFF_FMS_OK;
+    rio::FmsInp in = rio::parse_fms_inp(rio::read_file(join(dir, "fms.inp")));
+    in.mfms = ldos.lfms2 != 0 ? 1 : 0;
+    in.minv = ldos.minv;
+    in.rfms2 = ldos.rfms2;
+    in.rdirec = ldos.rdirec;
+    in.toler1 = ldos.toler1;
+    in.toler2 = ldos.toler2;
+    in.lmaxph = ldos.lmaxph;
+    in.decomposition_channels = -1;
+    in.save_gg_slice = false;
+    in.do_fms = ldos.lfms2;
+    try {
+        validate_source_controls(in);
+    } catch (const Error&) {
+        return REFEFF_FMS_OK;
+    }
+    if (in.idwopt == 4 && !rio::file_exists(join(dir, "sig2.dat"))) return REFEFF_FMS_OK;
+    if ((in.idwopt == 1 || in.idwopt == 2) && !rio::file_exists(join(dir, "spring.inp"))) return REFEFF_FMS_OK;
+    if (in.idwopt == 5 && !rio::file_exists(join(dir, "dmdw.inp"))) return REFEFF_FMS_OK;
+    check_provided_debye(in);
+    if (rio::file_exists(join(dir, "hubbard.inp")) && rio::file_exists(join(dir, "v_hubbard.bin")))
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "active Hubbard LDOS FMS sources are not provided");
+    const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
+    // ldos_phase_energy_grid_matches_input (:1911-1939)
+    if (ldos.neldos <= 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "ldos.inp neldos must be positive");
+    if (static_cast<int>(phase.energy_grid.size()) != ldos.neldos) return REFEFF_FMS_OK;
+    const double step = ldos.neldos > 1 ? (ldos.emax - ldos.emin) / (ldos.neldos - 1) : 0.0;
+    for (int e = 0; e < ldos.neldos; ++e) {
+        const cplx g((ldos.emin + step * e) / 27.211396, ldos.eimag / 27.211396);
+        if (std::abs(phase.energy_grid[e] - g) > 1.0e-8) return REFEFF_FMS_OK;
+    }
+    const int npot = static_cast<int>(phase.potentials.size());
+    const int max_pot = npot - 1;
+    // phase_supports_fms_lmax (:4241-4258)
+    if (phase.spin_count != 1 || static_cast<int>(in.lmaxph.size()) <= max_pot) return REFEFF_FMS_OK;
+    for (int ip = 0; ip <= max_pot; ++ip)
+        if (in.lmaxph[ip] < 0 || in.lmaxph[ip] > phase.potentials[ip].lmax) return REFEFF_FMS_OK;
+    const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
+    if (global.do_nrixs != 0) return REFEFF_FMS_OK;
+    const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
+    if (geom.nph != max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "geom.dat nph " + std::to_string(geom.nph) +
+                                                    " does not match phase.bin maximum potential " + std::to_string(max_pot) +
+                                                    " for LDOS FMS generation");
+    int written = 0;
+    for (int c = 0; c <= max_pot; ++c) {
+        const ClusterSolve cs = solve_cluster(dir, in, phase, global, geom, solver, c, false, false,
+                                              "LDOS FMS central potential " + std::to_string(c));
+        const int ne = cs.ne, L = cs.L, ch = cs.ch, nl = 2 * L + 1;
+        rio::GtrBin g;
+        g.ne = ne;
+        g.ne1 = phase.main_energy_count;
+        g.ne3 = phase.auxiliary_energy_count;
+        g.nph = max_pot;
+        g.ifms = 2;
+        g.angular_count = L + 1;
+        g.values.assign(static_cast<size_t>(ne) * npot * (L + 1), cplx(0, 0));
+        for (int e = 0; e < ne; ++e) {
+            const double* blk = cs.sc.data() + 2 * (static_cast<size_t>(e) * npot + c) * ch * ch;
+            for (int l = 0; l <= L; ++l) {
+                cplx sum(0, 0);
+                for (int m = 0; m <= 2 * l; ++m) {
+                    const size_t o = static_cast<size_t>(l * l + m) * (ch + 1);  // diagonal (r, r): r + ch r
+                    sum += cplx(blk[2 * o], blk[2 * o + 1]);
+                }
+                const size_t po = (static_cast<size_t>(e) * 1 * nl + (L + l)) * npot + c;  // spin 0
+                const cplx delta(cs.ph[2 * po], cs.ph[2 * po + 1]);
+                g.values[(static_cast<size_t>(e) * npot + c) * (L + 1) + l] =
+                    sum * std::exp(cplx(0.0, 2.0) * delta) / static_cast<double>(2 * l + 1);
+            }
+        }
+        char name[32];
+        std::snprintf(name, sizeof(name), "gtr%02d.bin", c);
+        const std::string path = join(dir, name);
+        if (rio::file_exists(path)) {
+            bool same = false;
+            try {
+                const rio::GtrBin old = rio::parse_gtr_bin(rio::read_file(path));
+                same = old.ne == g.ne && old.ne1 == g.ne1 && old.ne3 == g.ne3 && old.nph == g.nph && old.ifms == g.ifms &&
+                       old.angular_count == g.angular_count && old.values.size() == g.values.size();
+                for (size_t i = 0; same && i < g.values.size(); ++i) {
+                    const auto close = [](double a, double b) { return std::fabs(a - b) <= 2.0e-6 + 1.0e-5 * std::fabs(b); };
+                    same = close(old.values[i].real(), g.values[i].real()) && close(old.values[i].imag(), g.values[i].imag());
+                }
+            } catch (const std::exception&) {
+                same = false;
+            }
+            if (same) continue;
+        }
+        rio::write_file(path, rio::gtr_bin_bytes(g));
+        ++written;
+    }
+    if (files_written) *files_written = written;
+    return REFEFF_FMS_OK;
+}
+
 // MKGTR over a cached gg: refeff run_mkgtr_in_dir / build_mkgtr_ordinary_outputs,
 // crates/refeff-engine/src/fms.rs:559-631, 5255-5330, 5528-5608, 5849-5918,
 // 5941-5963.  The gg sections stay in FP64 (refeff narrows them to c32).
@@ -590,6 +767,14 @@ int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_w
                           size_t message_len) {
     if (!work_dir) return REFEFF_FMS_INVALID_ARGUMENT;
     const int rc = guarded([&] { return run_fms(work_dir, solver, files_written); });
+    put_message(message, message_len, rc ? g_last_error : std::string());
+    return rc;
+}
+
+int refeff_ldos_gtr_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_written, char* message,
+                               size_t message_len) {
+    if (!work_dir) return REFEFF_FMS_INVALID_ARGUMENT;
+    const int rc = guarded([&] { return run_ldos_gtr(work_dir, solver, files_written); });
     put_message(message, message_len, rc ? g_last_error : std::string());
     return rc;
 }
@@ -999,6 +1184,95 @@ int refeff_io_gtr_dat_parse(const char* text, int32_t* count, double* energy, do
         if (energy) cput(en, energy);
         if (trace) cput(tr, trace);
         return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_gtr_bin_bytes(const int32_t* header, const double* values, char* out, size_t out_len, size_t* needed) {
+    if (!header || !values) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::GtrBin g;
+        g.ne = header[0];
+        g.ne1 = header[1];
+        g.ne3 = header[2];
+        g.nph = header[3];
+        g.ifms = header[4];
+        g.angular_count = header[5];
+        if (g.ne <= 0 || g.nph < 0 || g.angular_count <= 0) {
+            g_last_error = "gtrNN.bin ne, nph and the angular count must be positive";
+            return static_cast<int>(REFEFF_FMS_INVALID_ARGUMENT);
+        }
+        const size_t n = static_cast<size_t>(g.ne) * (g.nph + 1) * g.angular_count;
+        g.values.resize(n);
+        for (size_t i = 0; i < n; ++i) g.values[i] = cplx(values[2 * i], values[2 * i + 1]);
+        return emit_bytes(rio::gtr_bin_bytes(g), out, out_len, needed);
+    });
+}
+
+int refeff_io_gtr_bin_parse(const char* bytes, size_t len, int32_t* header, double* values) {
+    if (!bytes || !header) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::GtrBin g = rio::parse_gtr_bin(std::string(bytes, len));
+        const int32_t h[6] = {g.ne, g.ne1, g.ne3, g.nph, g.ifms, g.angular_count};
+        std::copy(h, h + 6, header);
+        if (values) cput(g.values, values);
+        return REFEFF_FMS_OK;
+    });
+}
+
+int refeff_io_ldos_inp_parse(const char* text, refeff_io_ldos_inp* out) {
+    if (!text || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        const rio::LdosInp in = rio::parse_ldos_inp(text, "ldos.inp");
+        if (in.lmaxph.size() > sizeof(out->lmaxph) / sizeof(out->lmaxph[0])) {
+            g_last_error = "ldos.inp has more lmaxph values than the C struct holds";
+            return static_cast<int>(REFEFF_FMS_INVALID_ARGUMENT);
+        }
+        *out = refeff_io_ldos_inp{};
+        out->mldos = in.mldos;
+        out->lfms2 = in.lfms2;
+        out->ixc = in.ixc;
+        out->ispin = in.ispin;
+        out->minv = in.minv;
+        out->neldos = in.neldos;
+        out->iscfxc = in.iscfxc;
+        out->rfms2 = in.rfms2;
+        out->emin = in.emin;
+        out->emax = in.emax;
+        out->eimag = in.eimag;
+        out->rgrd = in.rgrd;
+        out->rdirec = in.rdirec;
+        out->toler1 = in.toler1;
+        out->toler2 = in.toler2;
+        out->lmaxph_count = static_cast<int32_t>(in.lmaxph.size());
+        std::copy(in.lmaxph.begin(), in.lmaxph.end(), out->lmaxph);
+        out->ldostype = in.ldostype;
+        return static_cast<int>(REFEFF_FMS_OK);
+    });
+}
+
+int refeff_io_ldos_inp_string(const refeff_io_ldos_inp* in, char* out, size_t out_len, size_t* needed) {
+    if (!in || in->lmaxph_count < 0 || in->lmaxph_count > static_cast<int32_t>(sizeof(in->lmaxph) / sizeof(in->lmaxph[0])))
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    return guarded([&] {
+        rio::LdosInp l;
+        l.mldos = in->mldos;
+        l.lfms2 = in->lfms2;
+        l.ixc = in->ixc;
+        l.ispin = in->ispin;
+        l.minv = in->minv;
+        l.neldos = in->neldos;
+        l.iscfxc = in->iscfxc;
+        l.rfms2 = in->rfms2;
+        l.emin = in->emin;
+        l.emax = in->emax;
+        l.eimag = in->eimag;
+        l.rgrd = in->rgrd;
+        l.rdirec = in->rdirec;
+        l.toler1 = in->toler1;
+        l.toler2 = in->toler2;
+        l.lmaxph.assign(in->lmaxph, in->lmaxph + in->lmaxph_count);
+        l.ldostype = in->ldostype;
+        return emit(rio::ldos_inp_string(l), out, out_len, needed);
     });
 }
 

@@ -1194,6 +1194,167 @@ GgDiagBin parse_gg_diag_bin(const std::string& b) {
     return d;
 }
 
+// ---- gtrNN.bin (gtr_bin.rs:86-330) ----
+
+namespace {
+void validate_gtr_bin(const GtrBin& d) {
+    if (d.ne <= 0) invalid("gtrNN.bin ne must be positive");
+    if (d.nph < 0 || d.ne1 < 0 || d.ne3 < 0) invalid("gtrNN.bin header counts must be non-negative");
+    if (d.angular_count <= 0) invalid("gtrNN.bin needs at least one angular channel");
+    const size_t n = checked_mul(checked_mul(static_cast<size_t>(d.ne), static_cast<size_t>(d.nph) + 1), d.angular_count);
+    if (d.values.size() != n) invalid("gtrNN.bin value count does not match ne, nph and the angular channels");
+    if (d.ne1 > d.ne) invalid("gtrNN.bin ne1 " + std::to_string(d.ne1) + " exceeds ne " + std::to_string(d.ne));
+    for (size_t i = 0; i < d.values.size(); ++i)
+        if (!std::isfinite(d.values[i].real()) || !std::isfinite(d.values[i].imag()))
+            invalid("gtrNN.bin trace value " + std::to_string(i + 1) + " is not finite");
+}
+}  // namespace
+
+std::string gtr_bin_bytes(const GtrBin& d) {
+    validate_gtr_bin(d);
+    std::string head, pay, out;
+    for (int v : {d.ne, d.ne1, d.ne3, d.nph, d.ifms}) put_u32(head, static_cast<uint32_t>(v));
+    for (const cplx& v : d.values) {
+        const float re = static_cast<float>(v.real()), im = static_cast<float>(v.imag());
+        if (!std::isfinite(re) || !std::isfinite(im)) invalid("gtrNN.bin trace value does not fit in f32");
+        put_f32(pay, re);
+        put_f32(pay, im);
+    }
+    put_record(out, head);
+    put_record(out, pay);
+    return out;
+}
+
+GtrBin parse_gtr_bin(const std::string& b) {
+    const bool big = detect_big_endian(b, 20);
+    size_t pos = 0;
+    const std::string head = get_record(b, pos, big, "gtrNN.bin header");
+    if (head.size() != 20) invalid("gtrNN.bin header record must hold 20 bytes");
+    GtrBin d;
+    int* f[4] = {&d.ne, &d.ne1, &d.ne3, &d.nph};
+    const char* names[4] = {"ne", "ne1", "ne3", "nph"};
+    for (int i = 0; i < 4; ++i) {
+        const int32_t v = static_cast<int32_t>(get_u32(head, 4 * i, big));
+        if (v < 0) invalid(std::string("gtrNN.bin ") + names[i] + " must be non-negative");
+        *f[i] = v;
+    }
+    d.ifms = static_cast<int32_t>(get_u32(head, 16, big));
+    const std::string pay = get_record(b, pos, big, "gtrNN.bin trace");
+    if (pay.size() % 8) invalid("gtrNN.bin trace payload is not a whole number of complex values");
+    const size_t count = pay.size() / 8, ep = static_cast<size_t>(d.ne) * (static_cast<size_t>(d.nph) + 1);
+    if (ep == 0) invalid("gtrNN.bin ne and potential count must be positive");
+    if (count % ep) invalid("gtrNN.bin trace payload is not a whole number of angular channels");
+    d.angular_count = static_cast<int>(count / ep);
+    if (d.angular_count == 0) invalid("gtrNN.bin trace payload must include at least one angular channel");
+    const std::vector<cplxf> raw = complex_payload(pay, big, count, "gtrNN.bin");
+    if (pos != b.size()) invalid("gtrNN.bin has trailing bytes");
+    d.values.resize(count);
+    for (size_t i = 0; i < count; ++i) d.values[i] = cplx(raw[i].real(), raw[i].imag());
+    validate_gtr_bin(d);
+    return d;
+}
+
+// ---- ldos.inp (ldos_input.rs:59-300) ----
+
+LdosInp parse_ldos_inp(const std::string& text, const std::string& label) {
+    const std::vector<std::string> lines = split_lines(text);
+    size_t li = 0;
+    auto perr = [&](size_t line, const std::string& m) -> Error {
+        return Error(REFEFF_FMS_IO_ERROR, label + " line " + std::to_string(line) + ": " + m);
+    };
+    auto next = [&](const std::string& what) -> std::string {
+        if (li >= lines.size()) throw perr(0, "expected " + what);
+        return lines[li++];
+    };
+    auto header = [&](const char* expected) {
+        const std::string l = next(expected);
+        if (trim(l) != expected) throw perr(li, std::string("expected header \"") + expected + "\", found \"" + l + "\"");
+    };
+    auto ints = [&](size_t count, const std::string& what) {
+        const std::vector<std::string> f = fields(next(what));
+        if (f.size() < count) throw perr(li, what + " requires " + std::to_string(count) + " fields");
+        std::vector<int> v;
+        for (size_t i = 0; i < count; ++i) {
+            long long x;
+            if (!parse_long(f[i], x) || x < std::numeric_limits<int32_t>::min() || x > std::numeric_limits<int32_t>::max())
+                throw perr(li, "invalid numeric field \"" + f[i] + "\"");
+            v.push_back(static_cast<int>(x));
+        }
+        return v;
+    };
+    auto reals = [&](size_t count, const std::string& what) {
+        const std::vector<std::string> f = fields(next(what));
+        if (f.size() < count) throw perr(li, what + " requires " + std::to_string(count) + " fields");
+        std::vector<double> v;
+        for (size_t i = 0; i < count; ++i) {
+            double x;
+            if (!parse_double(f[i], x)) throw perr(li, "invalid numeric field \"" + f[i] + "\"");
+            v.push_back(x);
+        }
+        return v;
+    };
+    LdosInp in;
+    const std::string h0 = trim(next("LDOS control header"));
+    size_t nc;
+    if (h0 == "mldos, lfms2, ixc, ispin, minv, neldos, iscfxc") nc = 7;
+    else if (h0 == "mldos, lfms2, ixc, ispin, minv, neldos") nc = 6;
+    else throw perr(li, "expected LDOS control header, found \"" + lines[li - 1] + "\"");
+    const std::vector<int> c = ints(nc, "LDOS control line");
+    in.mldos = c[0];
+    in.lfms2 = c[1];
+    in.ixc = c[2];
+    in.ispin = c[3];
+    in.minv = c[4];
+    in.neldos = c[5];
+    in.iscfxc = nc == 7 ? c[6] : 0;
+    header("rfms2, emin, emax, eimag, rgrd");
+    const std::vector<double> m = reals(5, "LDOS mesh line");
+    in.rfms2 = m[0];
+    in.emin = m[1];
+    in.emax = m[2];
+    in.eimag = m[3];
+    in.rgrd = m[4];
+    header("rdirec, toler1, toler2");
+    const std::vector<double> fm = reals(3, "LDOS FMS line");
+    in.rdirec = fm[0];
+    in.toler1 = fm[1];
+    in.toler2 = fm[2];
+    header("lmaxph(0:nph)");
+    {
+        const std::vector<std::string> f = fields(next("LDOS lmaxph line"));
+        for (const std::string& t : f) {
+            long long x;
+            if (!parse_long(t, x) || x < std::numeric_limits<int32_t>::min() || x > std::numeric_limits<int32_t>::max())
+                throw perr(li, "invalid numeric field \"" + t + "\"");
+            in.lmaxph.push_back(static_cast<int>(x));
+        }
+    }
+    if (li < lines.size()) {  // optional ldostype
+        const std::string l = lines[li++];
+        if (trim(l) != "ldostype") throw perr(li, "expected header \"ldostype\", found \"" + l + "\"");
+        in.ldostype = ints(1, "LDOS type line")[0];
+    }
+    return in;
+}
+
+std::string ldos_inp_string(const LdosInp& in) {
+    for (double v : {in.rfms2, in.emin, in.emax, in.eimag, in.rgrd, in.rdirec, in.toler1, in.toler2})
+        if (!std::isfinite(v)) fail("ldos.inp values must be finite");
+    if (in.lmaxph.empty()) fail("LDOS input requires at least one lmaxph value");
+    std::string out = "mldos, lfms2, ixc, ispin, minv, neldos, iscfxc\n";
+    out += fmt("%4d%4d%4d%4d%4d %7d %4d\n", in.mldos, in.lfms2, in.ixc, in.ispin, in.minv, in.neldos, in.iscfxc);
+    out += "rfms2, emin, emax, eimag, rgrd\n";
+    out += fmt("%13.5f%13.5f", in.rfms2, in.emin);
+    out += in.emax < 0.0 ? fmt("%14.5f", in.emax) : fmt("%13.5f", in.emax);
+    out += fmt("%13.5f%13.5f\n", in.eimag, in.rgrd);
+    out += "rdirec, toler1, toler2\n";
+    out += fmt("%13.5f%13.5f%13.5f\n", in.rdirec, in.toler1, in.toler2);
+    out += " lmaxph(0:nph)\n";
+    for (int v : in.lmaxph) out += fmt("%4d", v);
+    out += "\nldostype\n" + fmt("%4d", in.ldostype) + "\n";
+    return out;
+}
+
 std::string read_file(const std::string& path) {
     std::ifstream f(path, std::ios::binary);
     if (!f) fail("failed to read " + path);

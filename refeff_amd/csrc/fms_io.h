@@ -150,6 +150,28 @@ GgSliceBin parse_gg_slice_bin(const std::string& bytes);
 std::string gg_diag_bin_bytes(const GgDiagBin& d);
 GgDiagBin parse_gg_diag_bin(const std::string& bytes);
 
+// gtrNN.bin (FMS -> LDOS trace handoff), crates/refeff-io/src/gtr_bin.rs:86-330:
+// record 1 = ne, ne1, ne3, nph, ifms (i32); record 2 = complex f32 traces in
+// (energy, potential, angular) order.  Written little-endian, read either way.
+struct GtrBin {
+    int ne = 0, ne1 = 0, ne3 = 0, nph = 0, ifms = 0;
+    int angular_count = 0;
+    std::vector<cplx> values;  // [e][ip][l], ip in [0, nph]
+};
+std::string gtr_bin_bytes(const GtrBin& d);
+GtrBin parse_gtr_bin(const std::string& bytes);
+
+// ldos.inp, crates/refeff-io/src/ldos_input.rs:59-300 (parser and renderer).
+struct LdosInp {
+    int mldos = 0, lfms2 = 0, ixc = 0, ispin = 0, minv = 0, neldos = 0, iscfxc = 0;
+    double rfms2 = 0, emin = 0, emax = 0, eimag = 0, rgrd = 0;
+    double rdirec = 0, toler1 = 0, toler2 = 0;
+    std::vector<int> lmaxph;
+    int ldostype = 0;
+};
+LdosInp parse_ldos_inp(const std::string& text, const std::string& label);
+std::string ldos_inp_string(const LdosInp& in);
+
 // str::lines(): split on '\n', drop one trailing '\r', no final empty line.
 std::vector<std::string> split_lines(const std::string& text);
 std::string read_file(const std::string& path);

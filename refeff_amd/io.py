@@ -84,6 +84,11 @@ def _L():
     L.refeff_io_gg_diag_bin_bytes.argtypes = [i32, i32, i32, vp, C.c_char_p, sz, szp]
     L.refeff_io_gg_diag_bin_parse.argtypes = [C.c_char_p, sz, i32p, vp]
     L.refeff_io_sig2_dat_parse.argtypes = [cp, sz, vp]
+    L.refeff_io_gtr_bin_bytes.argtypes = [i32p, vp, C.c_char_p, sz, szp]
+    L.refeff_io_gtr_bin_parse.argtypes = [C.c_char_p, sz, i32p, vp]
+    L.refeff_io_ldos_inp_parse.argtypes = [cp, C.POINTER(LdosInpC)]
+    L.refeff_io_ldos_inp_string.argtypes = [C.POINTER(LdosInpC), C.c_char_p, sz, szp]
+    L.refeff_ldos_gtr_run_in_dir.argtypes = [C.c_char_p, i32, i32p, C.c_char_p, sz]
     _bound = True
     return L
 
@@ -168,6 +173,14 @@ def fortran_zero_scaled_exp(value: float, width: int, precision: int) -> str:
 
 
 # --- fms.inp ---------------------------------------------------------------
+
+class LdosInpC(C.Structure):
+    _fields_ = [("mldos", C.c_int32), ("lfms2", C.c_int32), ("ixc", C.c_int32), ("ispin", C.c_int32),
+                ("minv", C.c_int32), ("neldos", C.c_int32), ("iscfxc", C.c_int32),
+                ("rfms2", C.c_double), ("emin", C.c_double), ("emax", C.c_double), ("eimag", C.c_double),
+                ("rgrd", C.c_double), ("rdirec", C.c_double), ("toler1", C.c_double), ("toler2", C.c_double),
+                ("lmaxph_count", C.c_int32), ("lmaxph", C.c_int32 * 64), ("ldostype", C.c_int32)]
+
 
 @dataclass
 class FmsInput:
@@ -488,3 +501,60 @@ def parse_sig2_dat(text: str, pair_count: int) -> np.ndarray:
     out = np.zeros(max(1, pair_count), np.float64)
     _check(_L().refeff_io_sig2_dat_parse(_enc(text), pair_count, _ptr(out)), "sig2.dat")
     return out[:pair_count]
+
+
+# --- ldos.inp (ldos_input.rs) and gtrNN.bin (gtr_bin.rs) ----------------------
+
+_LDOS_FIELDS = ["mldos", "lfms2", "ixc", "ispin", "minv", "neldos", "iscfxc", "rfms2", "emin", "emax", "eimag",
+                "rgrd", "rdirec", "toler1", "toler2", "ldostype"]
+
+
+def parse_ldos_input(text: str) -> dict:
+    """LdosInput::parse_str: the controls as a dict, lmaxph as a list."""
+    c = LdosInpC()
+    _check(_L().refeff_io_ldos_inp_parse(_enc(text), C.byref(c)), "ldos.inp")
+    d = {f: getattr(c, f) for f in _LDOS_FIELDS}
+    d["lmaxph"] = list(c.lmaxph[:c.lmaxph_count])
+    return d
+
+
+def ldos_input_string(d: dict) -> str:
+    """ldos_input_string: FEFF-layout ldos.inp text from parse_ldos_input's dict."""
+    c = LdosInpC()
+    for f in _LDOS_FIELDS:
+        setattr(c, f, d.get(f, 0))
+    lm = list(d["lmaxph"])
+    c.lmaxph_count = len(lm)
+    for i, v in enumerate(lm):
+        c.lmaxph[i] = v
+    return _text(lambda b, n, need: _L().refeff_io_ldos_inp_string(C.byref(c), b, n, need), "ldos_input_string")
+
+
+def gtr_bin_bytes(values, ne1: int, ne3: int, ifms: int) -> bytes:
+    """gtr_bin_bytes: values [ne, nph + 1, angular] complex."""
+    v = np.ascontiguousarray(values, np.complex128)
+    if v.ndim != 3:
+        raise ValueError("gtr values must be [ne, nph + 1, angular]")
+    ne, npot, ang = v.shape
+    h = np.array([ne, ne1, ne3, npot - 1, ifms, ang], np.int32)
+    return _bytes(lambda b, n, need: _L().refeff_io_gtr_bin_bytes(h.ctypes.data_as(C.POINTER(C.c_int32)), _ptr(v), b,
+                                                                  n, need), "gtr_bin_bytes")
+
+
+def parse_gtr_bin(data: bytes):
+    """parse_gtr_bin: -> (values [ne, nph + 1, angular] complex128 from f32, dict of ne, ne1, ne3, nph, ifms)."""
+    L, h = _L(), np.zeros(6, np.int32)
+    hp = h.ctypes.data_as(C.POINTER(C.c_int32))
+    _check(L.refeff_io_gtr_bin_parse(data, len(data), hp, None), "gtrNN.bin")
+    out = np.zeros((h[0], h[3] + 1, h[5]), np.complex128)
+    _check(L.refeff_io_gtr_bin_parse(data, len(data), hp, _ptr(out)), "gtrNN.bin")
+    return out, dict(ne=int(h[0]), ne1=int(h[1]), ne3=int(h[2]), nph=int(h[3]), ifms=int(h[4]))
+
+
+def run_ldos_gtr_in_dir(work_dir: str, solver: int = 0) -> int:
+    """LDOS gtrNN.bin from the FMS handoffs (engine fms.rs:1226-1247): files written."""
+    n, msg = C.c_int32(), C.create_string_buffer(1024)
+    rc = _L().refeff_ldos_gtr_run_in_dir(str(work_dir).encode(), solver, C.byref(n), msg, len(msg))
+    if rc != 0:
+        raise CodecError(rc, "run_ldos_gtr_in_dir", msg.value.decode(errors="replace"))
+    return n.value

@@ -234,3 +234,72 @@ def test_module_fms_sig2_dat_must_cover_every_pair(tmp_path):
     with pytest.raises(io.CodecError) as e:
         io.run_fms_in_dir(tmp_path, 0)
     assert "sig2.dat pair row(s), expected" in str(e.value)
+
+
+def make_ldos_workdir(d, lfms2, ne=6, radius=4.6, sig2g=0.003):
+    """LDOS on its own energy grid: phase.bin's mesh is the ldos.inp grid
+    (emin + i step + i eimag, eV), E_ref = 0."""
+    pts = W.lattice(W.FCC, 3.615, radius + 1.0)
+    pts = pts[np.argsort(np.linalg.norm(pts, axis=1), kind="stable")]
+    iph = np.ones(len(pts), np.int32)
+    iph[0] = 0
+    g = io.GeomDat(len(pts), 1, [1, 2], np.arange(1, len(pts) + 1), pts, iph, np.ones(len(pts), np.int32))
+    (d / "geom.dat").write_text(io.geom_dat_string(g))
+    emin, emax, eimag = 4.0, 36.5, 0.5
+    e_ev = emin + (emax - emin) / (ne - 1) * np.arange(ne) + 1j * eimag
+    em = e_ev / HARTREE
+    eref = np.zeros((ne, 1), np.complex128)
+    k = np.sqrt(2.0 * em) / A0
+    ph_model = W.synthetic_phases(k, 3, np.array([3, 3]))
+    pots = [io.PhaseBinPotential(3, 29, "Cu", ph_model[:, 0, :, ip][:, :, None].copy()) for ip in range(2)]
+    rkk = np.zeros((ne, 1, 8, 1), np.complex128)
+    pb = io.PhaseBinData(1, ne, ne, 0, 1, 1, 8, 8, 8, 1, (2.6, -0.14, -0.14), em, eref, pots, rkk)
+    (d / "phase.bin").write_text(io.phase_bin_string(pb))
+    fin = io.FmsInput(mfms=1, idwopt=-1, minv=0, rfms2=radius, rdirec=10.0, toler1=0.001, toler2=0.001, tk=0.0,
+                      thetad=0.0, sig2g=sig2g, lmaxph=[3, 3], decomposition_channels=-1, save_gg_slice=False,
+                      do_fms=1)
+    (d / "fms.inp").write_text(io.fms_input_string(fin))
+    (d / "global.inp").write_text(GLOBAL_INP)
+    (d / "ldos.inp").write_text(io.ldos_input_string(dict(
+        mldos=1, lfms2=lfms2, ixc=0, ispin=0, minv=0, neldos=ne, iscfxc=0, rfms2=radius, emin=emin, emax=emax,
+        eimag=eimag, rgrd=0.05, rdirec=10.0, toler1=0.001, toler2=0.001, lmaxph=[3, 3], ldostype=0)))
+    return pts, iph, k, ph_model, radius, sig2g
+
+
+@pytest.mark.parametrize("lfms2", [0, 1])
+def test_ldos_gtr_bin_matches_oracle(tmp_path, lfms2):
+    """gtrNN.bin for both central potentials (engine fms.rs:1226-1247, 2095-2245):
+    the cluster re-centred on each potential's first atom in yprep order, the FMS
+    solved with the ldos.inp controls, FEFF's fmsdos trace of that potential."""
+    banner(f"LDOS gtrNN.bin, lfms2 {lfms2}")
+    pts, iph, k, ph, radius, sig2g = make_ldos_workdir(tmp_path, lfms2)
+    assert io.run_ldos_gtr_in_dir(tmp_path, 0) == 2
+    for c in (0, 1):
+        got, h = io.parse_gtr_bin((tmp_path / f"gtr{c:02d}.bin").read_bytes())
+        assert h == dict(ne=len(k), ne1=len(k), ne3=0, nph=1, ifms=2) and got.shape == (len(k), 2, 4)
+        pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, c, np.float32(radius), len(iph))
+        nat = len(pot)
+        sig = np.full((nat, nat), np.float32(sig2g), np.float32)
+        np.fill_diagonal(sig, 0.0)
+        plan = orc.Plan(pos, pot, nsp=1, lfms=lfms2, max_pot=1, global_lmax=3, raw_lmax=[3, 3], rdirec=10.0,
+                        sigsqr=sig)
+        sc, st = plan.spectrum(k.reshape(-1, 1), ph, threads=8)
+        assert not st.any()
+        want = orc.ldos_fmsdos_trace_grid(sc, ph, 4)
+        want[:, 1 - c, :] = 0.0
+        assert not got[:, 1 - c, :].any()
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= 2e-6, (c, err)  # the file holds f32
+    assert io.run_ldos_gtr_in_dir(tmp_path, 0) == 0  # both files already match
+    (tmp_path / "gtr01.bin").write_bytes(io.gtr_bin_bytes(np.ones((len(k), 2, 4)), len(k), 0, 2))
+    assert io.run_ldos_gtr_in_dir(tmp_path, 0) == 1  # a stale one is rewritten
+    banner("gtr00.bin / gtr01.bin match the oracle's fmsdos trace")
+
+
+def test_ldos_gtr_needs_the_ldos_energy_grid(tmp_path):
+    banner("LDOS gtrNN.bin on a mismatched energy grid")
+    make_ldos_workdir(tmp_path, 0)
+    d = io.parse_ldos_input((tmp_path / "ldos.inp").read_text())
+    (tmp_path / "ldos.inp").write_text(io.ldos_input_string({**d, "emax": d["emax"] + 1.0}))
+    assert io.run_ldos_gtr_in_dir(tmp_path, 0) == 0
+    assert not list(tmp_path.glob("gtr*.bin"))

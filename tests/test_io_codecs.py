@@ -652,3 +652,101 @@ def test_sig2_dat_names_the_line():
     with pytest.raises(CodecError) as e:
         io.parse_sig2_dat("# header\n1 2 0.5 1\n1 3 -1 1\n", 2)
     assert "sig2.dat line 3" in str(e.value)
+
+
+# --------------------------------------------------------------------------
+# gtrNN.bin: crates/refeff-io/src/gtr_bin.rs tests (:454-578)
+# --------------------------------------------------------------------------
+
+def _sample_gtr():
+    v = np.zeros((2, 2, 2), np.complex128)
+    for e in range(2):
+        for p in range(2):
+            for a in range(2):
+                x = e + 0.25 * p + 0.125 * a
+                v[e, p, a] = x - 1j * x
+    return v
+
+
+def test_gtr_bin_roundtrips_refeff_sample():
+    v = _sample_gtr()
+    b = io.gtr_bin_bytes(v, 1, 0, 2)
+    assert len(b) == 100  # gtr_bin.rs:462: two records, 20 + 64 payload bytes
+    got, h = io.parse_gtr_bin(b)
+    assert h == dict(ne=2, ne1=1, ne3=0, nph=1, ifms=2)
+    assert np.array_equal(got, v)
+    assert got[1, 1, 1] == 1.375 - 1.375j
+
+
+def test_gtr_bin_reads_big_endian_and_rejects_bad_data():
+    b = io.gtr_bin_bytes(_sample_gtr(), 1, 0, 2)
+    # the same file written big-endian: every 4-byte word swapped
+    big = b"".join(b[i:i + 4][::-1] for i in range(0, len(b), 4))
+    got, h = io.parse_gtr_bin(big)
+    assert np.array_equal(got, _sample_gtr()) and h["ifms"] == 2
+    for bad in (b"", bytes([0]) + b[1:], b + b"\0", b[:-1]):
+        with pytest.raises(CodecError):
+            io.parse_gtr_bin(bad)
+    with pytest.raises(CodecError):
+        io.gtr_bin_bytes(np.full((1, 1, 1), np.nan + 0j), 0, 0, 2)
+    with pytest.raises(CodecError):
+        io.gtr_bin_bytes(_sample_gtr(), 3, 0, 2)  # ne1 > ne
+
+
+# --------------------------------------------------------------------------
+# ldos.inp: crates/refeff-io/src/ldos_input.rs tests (:386-441)
+# --------------------------------------------------------------------------
+
+LEGACY_LDOS_INP = """mldos, lfms2, ixc, ispin, minv, neldos
+   1   0   0   0   0     101
+rfms2, emin, emax, eimag, rgrd
+      4.51200    -25.00000     12.00000      0.01000      0.05000
+rdirec, toler1, toler2
+      9.40000      0.00100      0.00100
+ lmaxph(0:nph)
+   1   2   1
+"""
+
+
+def test_parses_legacy_ldos_input_without_iscfxc_or_ldostype():
+    d = io.parse_ldos_input(LEGACY_LDOS_INP)
+    assert (d["mldos"], d["neldos"], d["iscfxc"], d["ldostype"]) == (1, 101, 0, 0)
+    assert d["lmaxph"] == [1, 2, 1]
+    assert (d["rfms2"], d["emin"], d["emax"], d["eimag"], d["rgrd"]) == (4.512, -25.0, 12.0, 0.01, 0.05)
+    assert (d["rdirec"], d["toler1"], d["toler2"]) == (9.4, 0.001, 0.001)
+
+
+def test_renders_negative_emax_like_feff_reference():
+    d = dict(mldos=1, lfms2=0, ixc=0, ispin=0, minv=0, neldos=3, iscfxc=11, rfms2=-1.0, emin=-30.0, emax=-29.0,
+             eimag=0.1, rgrd=0.05, rdirec=-1.0, toler1=0.001, toler2=0.001, lmaxph=[3, 3], ldostype=0)
+    text = io.ldos_input_string(d)
+    assert text.splitlines()[3] == "     -1.00000    -30.00000     -29.00000      0.10000      0.05000"
+    back = io.parse_ldos_input(text)
+    assert back == d
+
+
+def test_ldos_input_rejects_bad_text_and_values():
+    for bad in (LEGACY_LDOS_INP.replace("mldos, lfms2", "mldos lfms2"), LEGACY_LDOS_INP.replace("   1   0   0   0   0     101", "   1   0   0"),
+                LEGACY_LDOS_INP.replace("4.51200", "4.5x"), LEGACY_LDOS_INP + "notldostype\n   0\n",
+                LEGACY_LDOS_INP.replace(" lmaxph(0:nph)\n   1   2   1\n", "")):
+        with pytest.raises(CodecError):
+            io.parse_ldos_input(bad)
+    d = io.parse_ldos_input(LEGACY_LDOS_INP)
+    with pytest.raises(CodecError):
+        io.ldos_input_string({**d, "rfms2": float("nan")})
+    with pytest.raises(CodecError):
+        io.ldos_input_string({**d, "lmaxph": []})
+
+
+def test_ldos_stage_writes_nothing_where_refeff_takes_another_path(tmp_path):
+    """write_ldos_gtr_bin_source_handoffs (engine fms.rs:1226-1247, 1689-1735): a
+    non-fmsdos LDOS (mldos != 1, spin, ldostype > 0) or missing handoffs leave
+    gtrNN.bin to refeff's other paths; nothing here touches the GPU."""
+    for change in ("   2   0   0   0   0     101", "   1   0   0   1   0     101"):
+        (tmp_path / "ldos.inp").write_text(LEGACY_LDOS_INP.replace("   1   0   0   0   0     101", change))
+        assert io.run_ldos_gtr_in_dir(tmp_path) == 0
+    (tmp_path / "ldos.inp").write_text(LEGACY_LDOS_INP)
+    assert io.run_ldos_gtr_in_dir(tmp_path) == 0  # no fms.inp / phase.bin / geom.dat / global.inp
+    (tmp_path / "ldos.inp").write_text(LEGACY_LDOS_INP + "ldostype\n   1\n")
+    assert io.run_ldos_gtr_in_dir(tmp_path) == 0
+    assert not list(tmp_path.glob("gtr*.bin"))

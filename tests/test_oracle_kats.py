@@ -689,3 +689,60 @@ def test_debye_sigsqr_is_the_two_leg_path_closed_form(model):
             cab = orc.debye_correlation(cl, r, thetad, tk, int(z[a1]), int(z[a2]), rs)
             want = float(s2g) + caa + cbb - 2.0 * cab
             assert abs(float(sig[a2, a1]) - want) <= 2e-7 * want, (a1, a2)
+
+
+# --------------------------------------------------------------------------
+# LDOS fmsdos trace: refeff density/tests.rs:1014-1135 (which diagonal
+# channels and which phase slot each (l, potential) uses)
+# --------------------------------------------------------------------------
+
+def _fmsdos_expected(diag, phase, l):
+    return diag * np.exp(2j * phase) / (2 * l + 1)
+
+
+def test_ldos_fmsdos_trace_kat():
+    sc = np.zeros((1, 2, 4, 4), np.complex128)
+    for ip, vals in enumerate([[1 + .5j, 2 - .25j, 3 + .75j, 4 - 1.25j], [-.5 + .25j, .25 + 1.5j, -1.25 + .5j, 2.5 - .75j]]):
+        for c, v in enumerate(vals):
+            sc[0, ip, c, c] = v
+    sc[0, 0, 1, 2] = 99.0  # off-diagonal entries never enter
+    ph = np.zeros((1, 1, 3, 2), np.complex128)
+    ph[0, 0, 1, 0], ph[0, 0, 2, 0] = 0.10 + 0.03j, -0.05 + 0.02j
+    ph[0, 0, 1, 1], ph[0, 0, 2, 1] = 0.07 - 0.01j, 0.02 + 0.04j
+    ph[0, 0, 0, :] = 5.0  # the l~ = -1 slot never enters
+    t = orc.ldos_fmsdos_trace_grid(sc, ph, 2)
+    assert t.shape == (1, 2, 2)
+    for got, want in [(t[0, 0, 0], _fmsdos_expected(1 + .5j, 0.10 + 0.03j, 0)),
+                      (t[0, 0, 1], _fmsdos_expected(9 - .75j, -0.05 + 0.02j, 1)),
+                      (t[0, 1, 0], _fmsdos_expected(-.5 + .25j, 0.07 - 0.01j, 0)),
+                      (t[0, 1, 1], _fmsdos_expected(1.5 + 1.25j, 0.02 + 0.04j, 1))]:
+        assert abs(got - want) < 1e-12
+
+
+def test_ldos_fmsdos_trace_grid_energy_order_kat():
+    sc = np.zeros((2, 2, 4, 4), np.complex128)
+    sc[0, 0, 0, 0], sc[0, 1, 1, 1], sc[0, 1, 2, 2], sc[0, 1, 3, 3] = 1 + .5j, .25 + 1.5j, -1.25 + .5j, 2.5 - .75j
+    sc[1, 0, 0, 0], sc[1, 1, 1, 1], sc[1, 1, 2, 2], sc[1, 1, 3, 3] = 1.25 - .5j, .5 - 1j, -2 + .25j, 1.25 + .75j
+    ph = np.zeros((2, 1, 3, 2), np.complex128)
+    ph[0, 0, 1, 0], ph[0, 0, 2, 1], ph[1, 0, 1, 0], ph[1, 0, 2, 1] = .10 + .03j, .02 + .04j, -.08 + .01j, .03 - .02j
+    t = orc.ldos_fmsdos_trace_grid(sc, ph, 2)
+    assert abs(t[0, 0, 0] - _fmsdos_expected(1 + .5j, .10 + .03j, 0)) < 1e-12
+    assert abs(t[0, 1, 1] - _fmsdos_expected(1.5 + 1.25j, .02 + .04j, 1)) < 1e-12
+    assert abs(t[1, 0, 0] - _fmsdos_expected(1.25 - .5j, -.08 + .01j, 0)) < 1e-12
+    assert abs(t[1, 1, 1] - _fmsdos_expected(-.25 + 0j, .03 - .02j, 1)) < 1e-12
+
+
+def test_ldos_fmsdos_trace_of_a_lone_atom_is_its_t_matrix_sum():
+    """One atom, no neighbours: gg is the free single-site block, zero, so the
+    trace vanishes; with gg = diag(t) it is t e^{2i delta} per l (closed form)."""
+    t0 = orc.ldos_fmsdos_trace_grid(np.zeros((1, 1, 9, 9)), np.full((1, 1, 5, 1), 0.3 + 0.1j), 3)
+    assert not t0.any()
+    delta = np.array([0.4 + 0.02j, 0.2 + 0.01j, 0.1 + 0.0j])
+    sc = np.zeros((1, 1, 9, 9), np.complex128)
+    for l in range(3):
+        for m in range(2 * l + 1):
+            sc[0, 0, l * l + m, l * l + m] = np.exp(1j * delta[l]) * np.sin(delta[l])
+    ph = np.zeros((1, 1, 5, 1), np.complex128)
+    ph[0, 0, 2:, 0] = delta
+    t = orc.ldos_fmsdos_trace_grid(sc, ph, 3)
+    assert np.allclose(t[0, 0], np.exp(3j * delta) * np.sin(delta), rtol=1e-14)
