@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip the plain FP64 LU comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle accuracy sample")
     ap.add_argument("--cpu-energies", type=int, default=6)
+    ap.add_argument("--caller", default=None, choices=["pot_scf"],
+                    help="time a caller of the FMS stage instead of the stage itself: pot_scf = the POT "
+                         "self-consistency traces (one JSON line of its own)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy (the reference kernel for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling under rocprofv3 --pmc)")
@@ -147,6 +150,79 @@ def cpu_baseline(wl, energies, gg_gpu):
                       f"LAPACK cgetrf/cgetrs complex64 (OpenBLAS, {cores} threads)"}, errs
 
 
+def pot_scf_case(d, rfms1, ne, lmaxsc):
+    """geom.dat of an fcc Cu crystal (a = 3.615 A, potential 0 at the origin, the
+    rest potential 1) and a POT SCF energy grid, reference energies and l-indexed
+    phase table in the shapes refeff's POT loop hands to the FMS traces."""
+    from refeff_amd import io, workloads as W
+    pts = W.lattice(W.FCC, 3.615, rfms1 + 3.0)
+    pts = pts[np.argsort(np.linalg.norm(pts, axis=1), kind="stable")]
+    iph = np.ones(len(pts), np.int32)
+    iph[0] = 0
+    g = io.GeomDat(len(pts), 1, [1, 2], np.arange(1, len(pts) + 1), pts, iph, np.ones(len(pts), np.int32))
+    with open(os.path.join(d, "geom.dat"), "w") as f:
+        f.write(io.geom_dat_string(g))
+    e = np.linspace(0.3, 1.6, ne) + 0.02j
+    refs = np.array([[0.05 * p - 1j * x.imag for p in range(2)] for x in e])
+    k = np.sqrt(2.0 * (e - refs[:, 1])) / 0.529177249
+    lmax = max(lmaxsc)
+    phl = np.ascontiguousarray(W.synthetic_phases(k, lmax, np.array(lmaxsc))[:, 0, lmax:, :])
+    return pts, iph, e, refs, k, phl
+
+
+def run_pot_scf_caller(args):
+    """POT SCF FMS traces (refeff build_pot_scf_fms_source_grid_handoff) over a
+    Cu cluster: whole calls timed, host preparation and geom.dat read included.
+    lfms1 = 1 solves one cluster around potential 0; lfms1 = 0 one per potential."""
+    import tempfile
+    from refeff_amd import io
+    from oracle import oracle as orc
+    rfms1, ne, lmaxsc, angular = 6.0, 48, [3, 3], 4
+    d = tempfile.mkdtemp(prefix="pot_scf_")
+    pts, iph, e, refs, k, phl = pot_scf_case(d, rfms1, ne, lmaxsc)
+    rates, traces = {}, {}
+    for lfms1 in (1, 0):
+        for _ in range(args.warmup):
+            io.pot_scf_fms_traces(d, lfms1, rfms1, lmaxsc, e, refs, phl, angular)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            traces[lfms1] = io.pot_scf_fms_traces(d, lfms1, rfms1, lmaxsc, e, refs, phl, angular)
+        rates[lfms1] = (time.perf_counter() - t0) / args.steps
+    nat = len(orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(rfms1), len(iph))[1])
+    line = {"metric": "POT SCF FMS trace energy-pts/s", "value": round(ne / rates[1], 3),
+            "unit": "energy-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(rates[1] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "c64 LU + f64 refinement",
+            "data": "synthetic (Cu fcc geom.dat, smooth model phase shifts)",
+            "config": {"workload": f"pot_scf Cu fcc rfms1={rfms1} ({nat} atoms, {nat * 16} states), "
+                                   f"{ne} energies, lmaxsc={lmaxsc}", "lfms1": 1},
+            "lfms1_0": {"value": round(ne / rates[0], 3), "unit": "energy-points/s",
+                        "ms_per_call": round(rates[0] * 1e3, 3), "clusters": 2}}
+    if not args.no_cpu:
+        cores = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        sample = sorted(set(np.linspace(0, ne - 1, cores).astype(int).tolist()))  # one energy per thread
+        cores = len(sample)
+        signed = np.zeros((len(sample), 1, 7, 2), np.complex128)
+        for ip in range(2):
+            for sl in range(-3, 4):
+                signed[:, 0, 3 + sl, ip] = phl[sample, abs(sl), ip]
+        pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(rfms1), len(iph))
+        pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
+        plan = orc.Plan(pos, pot, nsp=1, lfms=1, max_pot=1, global_lmax=3, raw_lmax=lmaxsc, rdirec=2 * rfms1,
+                        sigsqr=np.zeros((nat, nat)), toler1=0.0, toler2=0.0)
+        t0 = time.perf_counter()
+        sc, st = plan.spectrum(k[sample].reshape(-1, 1), signed, threads=cores)
+        t_cpu = time.perf_counter() - t0
+        want = orc.pot_scf_trace_grid(sc, phl[sample], angular)
+        got = traces[1][sample]
+        line["cpu_baseline"] = {"value": round(len(sample) / t_cpu, 4), "unit": "energy-points/s", "cores": cores,
+                                "kind": "port", "sample": f"{len(sample)} of {ne} energies, lfms1 = 1; oracle FP64 "
+                                                          f"G0/t assembly + LU, one energy per thread"}
+        line["accuracy"] = {"max_trace_rel_err_vs_fp64_oracle": float(np.abs(got - want).max() / np.abs(want).max()),
+                            "energies_checked": len(sample), "gate": 1e-8}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of a known size: rocprofv3 --pmc reads FETCH_SIZE / WRITE_SIZE
     for it, and the ratio to nbytes scales the counters of the FMS kernels."""
@@ -168,6 +244,9 @@ def main():
         sys.exit(2)
     from refeff_amd import lib, workloads as W
     L = lib()
+    if args.caller == "pot_scf":
+        run_pot_scf_caller(args)
+        return
     if args.copy_calibration:
         copy_calibration()
     wl = W.CONFIGS[args.config]()

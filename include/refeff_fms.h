@@ -284,8 +284,9 @@ int refeff_fms_mkgtr_trace(int32_t energy_count, int32_t channel_count, int32_t 
  * when fms.inp sets save_gg_slice.  Replaces run_fms_in_dir,
  * crates/refeff-engine/src/fms.rs:462-539, on its ordinary source path
  * (build_fms_source_outputs, :3549-3695) with the global SIG2 damping
- * (idwopt < 0); other Debye-Waller models, reciprocal-space and Hubbard
- * sources return REFEFF_FMS_UNSUPPORTED.  Without source files a cached
+ * (idwopt < 0), the correlated and classical Debye models (0, 3) or sig2.dat
+ * (4); the other Debye-Waller models, reciprocal-space FMS (reciprocal.inp
+ * ispace 0) and Hubbard sources return REFEFF_FMS_UNSUPPORTED.  Without source files a cached
  * gg.bin / gg.dat (and gg_slice.bin / gg_diag.bin) is re-emitted
  * canonically.  solver: a refeff_fms_solver.  *files_written receives the
  * file count; message receives a context string on failure. */
@@ -298,6 +299,26 @@ int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_w
  * with build_mkgtr_ordinary_outputs (:5528-5608); EELS, NRIXS and polarised
  * traces return REFEFF_FMS_UNSUPPORTED. */
 int refeff_mkgtr_run_in_dir(const char* work_dir, int32_t* files_written, char* message, size_t message_len);
+
+/* POT self-consistency FMS traces gtr(l, iph) on the POT energy grid.  Replaces
+ * the real-space branch of build_pot_scf_fms_source_grid_handoff,
+ * crates/refeff-engine/src/fms.rs:2650-2965 (with :3247-3275, 5118-5175 and
+ * refeff-io screen_dat.rs:645-690).  geom.dat is read from work_dir; the pot.inp
+ * controls come in as lfms1, rfms1 (cluster radius; direct cutoff 2 rfms1),
+ * nph and lmaxsc[nph+1].  lfms1 != 0: one cluster around potential 0 (sortat),
+ * every potential's block; otherwise one cluster per potential around its first
+ * atom.  No damping.  rfms1 <= 0 gives zero traces without a solve; otherwise a
+ * reciprocal.inp with ispace 0 (k-space FMS) returns REFEFF_FMS_UNSUPPORTED.
+ *   energies: [energy_count] complex Hartree
+ *   reference_energies: [energy_count][reference_count] complex; column nph sets k
+ *   phase_shifts: [energy_count][phase_l_count][nph+1] complex, indexed by l
+ *   traces: [energy_count][angular_count][nph+1] complex out,
+ *     sum_m gg(l^2 + m, l^2 + m) exp(2 i delta_l) / (2 l + 1) */
+int refeff_fms_pot_scf_traces(const char* work_dir, int32_t solver, int32_t lfms1, double rfms1, int32_t nph,
+                              const int32_t* lmaxsc, int32_t energy_count, const double* energies,
+                              int32_t reference_count, const double* reference_energies, int32_t phase_l_count,
+                              const double* phase_shifts, int32_t angular_count, double* traces, char* message,
+                              size_t message_len);
 
 /* LDOS gtrNN.bin from the FMS source handoffs.  Replaces
  * write_ldos_gtr_bin_source_handoffs, crates/refeff-engine/src/fms.rs:1226-1247

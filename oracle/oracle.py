@@ -417,6 +417,32 @@ def debye_sigsqr(model, tk, thetad, rsavg, positions, atomic_numbers, sig2g):
     return out.reshape(n, n).T.copy()  # [atom2, atom1]
 
 
+def cluster_green_trace(scattering, phase_shift, l):
+    """screen_fms_cluster_green_trace (crates/refeff-core/src/screen/response.rs:545-581):
+    sum over the 2l + 1 states l^2 .. (l+1)^2 - 1 of the scattering diagonal,
+    times exp(2 i phase_shift) / (2 l + 1).  scattering: [ch, ch] (row, col)."""
+    sc = np.asarray(scattering, np.complex128)
+    diag = sum(sc[st, st] for st in range(l * l, (l + 1) * (l + 1)))
+    return diag * np.exp(2j * complex(phase_shift)) / (2 * l + 1)
+
+
+def pot_scf_trace_grid(scattering, phases, angular_count):
+    """POT self-consistency FMS traces, pot_scf_fms_source_grid_handoff
+    (crates/refeff-io/src/screen_dat.rs:651-685): cluster_green_trace of every
+    energy's and potential's block with the l-indexed phase shift.
+    scattering: [ne, npot, ch, ch]; phases: [ne, nl, npot] (l-indexed).
+    Returns [ne, angular_count, npot] complex128."""
+    sc = np.asarray(scattering, np.complex128)
+    ph = np.asarray(phases, np.complex128)
+    ne, npot = sc.shape[0], sc.shape[1]
+    out = np.zeros((ne, angular_count, npot), np.complex128)
+    for e in range(ne):
+        for ip in range(npot):
+            for l in range(angular_count):
+                out[e, l, ip] = cluster_green_trace(sc[e, ip], ph[e, l, ip], l)
+    return out
+
+
 def ldos_fmsdos_trace_grid(scattering, phases, angular_count, spin_index=0):
     """FEFF fmsdos trace of the FMS scattering blocks, ldos_fmsdos_trace /
     ldos_fmsdos_trace_grid (crates/refeff-core/src/density/ldos.rs:403-484):

@@ -2,6 +2,7 @@
 // and the C ABI of the text codecs they use.
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -88,6 +89,30 @@ void check_status(int rc, const std::string& where) {
 std::string join(const std::string& dir, const char* name) {
     if (dir.empty()) return name;
     return dir.back() == '/' ? dir + name : dir + "/" + name;
+}
+
+// reciprocal.inp's ispace (refeff-io control_input/parser.rs:84-94: a header
+// line, then the value): 1 keeps the real-space solve, 0 asks for the k-space
+// one; refeff rejects any other value (engine fms.rs:771-785, 2742-2752).
+// Returns false when the file is absent.
+bool reciprocal_ispace(const std::string& dir, int* ispace) {
+    const std::string path = join(dir, "reciprocal.inp");
+    if (!rio::file_exists(path)) return false;
+    const std::string text = rio::read_file(path);
+    const size_t eol = text.find('\n');
+    const size_t end = eol == std::string::npos ? std::string::npos : text.find('\n', eol + 1);
+    const std::string line = eol == std::string::npos ? std::string() : text.substr(eol + 1, end - eol - 1);
+    const size_t b = line.find_first_not_of(" \t\r");
+    const size_t e = b == std::string::npos ? b : line.find_first_of(" \t\r", b);
+    const std::string tok = b == std::string::npos ? std::string() : line.substr(b, e - b);
+    char* stop = nullptr;
+    const long v = tok.empty() ? 0 : std::strtol(tok.c_str(), &stop, 10);
+    if (tok.empty() || *stop != '\0' || v < -2147483647L || v > 2147483647L)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "failed to parse " + path + ": expected the RECIPROCAL ispace line");
+    if (v != 0 && v != 1)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "reciprocal.inp ispace must be 0 or 1, got " + std::to_string(v));
+    *ispace = static_cast<int>(v);
+    return true;
 }
 
 float narrow_nonnegative(double v, const char* what) {
@@ -373,8 +398,9 @@ std::vector<rio::GgSection> fms_sections(const std::string& dir, const rio::FmsI
     const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
     const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
     const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
-    if (rio::file_exists(join(dir, "reciprocal.inp")))
-        stage_fail(REFEFF_FMS_UNSUPPORTED, "reciprocal-space FMS (reciprocal.inp) is not provided");
+    int ispace = 1;
+    if (reciprocal_ispace(dir, &ispace) && ispace == 0)
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "reciprocal-space FMS (reciprocal.inp ispace 0) is not provided");
     if (rio::file_exists(join(dir, "hubbard.inp")) && rio::file_exists(join(dir, "v_hubbard.bin")))
         stage_fail(REFEFF_FMS_UNSUPPORTED, "active Hubbard FMS sources are not provided");
     const int npot = static_cast<int>(phase.potentials.size());
@@ -654,6 +680,173 @@ int run_ldos_gtr(const std::string& dir, int solver, int32_t* files_written) {
     return REFEFF_FMS_OK;
 }
 
+// POT self-consistency FMS traces on the POT energy grid: the real-space branch
+// of refeff build_pot_scf_fms_source_grid_handoff (crates/refeff-engine/src/fms.rs:2650-2965)
+// with pot_scf_fms_input_from_pot (:3247-3275: rfms2 = rfms1, rdirec = 2 rfms1,
+// no damping, lmaxph = lmaxsc), pot_scf_fms_phase_shifts_for_energy
+// (:5118-5175: the l-indexed table spread over signed l) and
+// pot_scf_fms_source_grid_handoff (refeff-io screen_dat.rs:645-690).  lfms1 != 0:
+// one cluster around potential 0 (sortat), every potential's block; else one
+// cluster per potential c around its first atom (yprep order), block c only.
+// Wave numbers use the highest potential's reference energy.
+struct PotScfArgs {
+    int lfms1;
+    double rfms1;
+    int nph;
+    const int32_t* lmaxsc;
+    int ne;
+    const double* energies;    // [ne] complex, Hartree
+    int nref;
+    const double* references;  // [ne][nref] complex, Hartree
+    int nphl;
+    const double* phases;      // [ne][nphl][nph + 1] complex
+    int angular;
+    double* traces;            // [ne][angular][nph + 1] complex
+};
+
+int run_pot_scf(const std::string& dir, int solver, const PotScfArgs& a) {
+    const int npot = a.nph + 1, max_pot = a.nph;
+    if (a.ne <= 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF FMS source-grid generation requires at least one energy row");
+    if (a.angular <= 0)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF FMS source-grid generation requires at least one angular channel");
+    if (a.nph < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF FMS generation requires nonnegative nph");
+    if (a.nref < npot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF FMS reference-energy table cannot supply every potential");
+    for (int i = 0; i < 2 * a.ne; ++i)
+        if (!std::isfinite(a.energies[i])) stage_fail(REFEFF_FMS_NON_FINITE, "POT SCF FMS energies must be finite");
+    int L = 0;
+    for (int ip = 0; ip < npot; ++ip) {
+        if (a.lmaxsc[ip] < 0)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF lmaxsc(" + std::to_string(ip) +
+                                                        ") must be nonnegative for FMS source-grid generation");
+        L = std::max(L, a.lmaxsc[ip]);
+    }
+    if (a.angular > L + 1)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF requested " + std::to_string(a.angular) +
+                                                    " angular channel(s), but lmaxsc supports " + std::to_string(L + 1));
+    if (a.nphl < L + 1)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF FMS phase table has " + std::to_string(a.nphl) +
+                                                    " angular channel(s), expected at least " + std::to_string(L + 1));
+    std::fill(a.traces, a.traces + 2 * static_cast<size_t>(a.ne) * a.angular * npot, 0.0);
+    int ispace = 1;
+    const bool reciprocal = reciprocal_ispace(dir, &ispace);
+    if (!(a.rfms1 > 0.0)) return REFEFF_FMS_OK;  // build_zero_pot_scf_fms_source_grid_handoff
+    if (reciprocal && ispace == 0)
+        stage_fail(REFEFF_FMS_UNSUPPORTED, "reciprocal-space POT FMS (reciprocal.inp ispace 0) is not provided");
+    const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
+    if (geom.nph != max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "geom.dat nph " + std::to_string(geom.nph) +
+                                                    " does not match maximum potential " + std::to_string(max_pot) +
+                                                    " for POT SCF FMS generation");
+    const float radius = narrow_nonnegative(a.rfms1, "POT SCF FMS cluster radius");
+    const float rdirec = narrow_nonnegative(2.0 * a.rfms1, "POT SCF FMS direct cutoff");
+    const bool all = a.lfms1 != 0;
+    // wave numbers and signed-l phase shifts, shared by every cluster
+    const int nl = 2 * L + 1;
+    std::vector<double> k(2 * static_cast<size_t>(a.ne)), ph(2 * static_cast<size_t>(a.ne) * nl * npot, 0.0);
+    for (int e = 0; e < a.ne; ++e) {
+        const size_t ro = static_cast<size_t>(e) * a.nref + max_pot;
+        const cplx w = std::sqrt(cplx(2.0, 0.0) * (cplx(a.energies[2 * e], a.energies[2 * e + 1]) -
+                                                  cplx(a.references[2 * ro], a.references[2 * ro + 1]))) / 0.529177249;
+        k[2 * e] = w.real();
+        k[2 * e + 1] = w.imag();
+        for (int ip = 0; ip < npot; ++ip) {
+            const int eff = std::min(a.lmaxsc[ip], L);
+            for (int sl = -eff; sl <= eff; ++sl) {
+                const size_t src = (static_cast<size_t>(e) * a.nphl + std::abs(sl)) * npot + ip;
+                const size_t dst = (static_cast<size_t>(e) * nl + (sl + L)) * npot + ip;
+                ph[2 * dst] = a.phases[2 * src];
+                ph[2 * dst + 1] = a.phases[2 * src + 1];
+            }
+        }
+    }
+    const int nat_all = static_cast<int>(geom.atoms.size());
+    if (nat_all == 0) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
+    std::vector<float> pos(3 * static_cast<size_t>(nat_all));
+    std::vector<int32_t> pot(nat_all);
+    for (int i = 0; i < nat_all; ++i) {
+        pos[3 * i] = static_cast<float>(geom.atoms[i].x);
+        pos[3 * i + 1] = static_cast<float>(geom.atoms[i].y);
+        pos[3 * i + 2] = static_cast<float>(geom.atoms[i].z);
+        pot[i] = geom.atoms[i].iph;
+    }
+    const int ch = (L + 1) * (L + 1);
+    std::vector<double> sc_all(2 * static_cast<size_t>(a.ne) * npot * ch * ch, 0.0);  // [e][ip] blocks
+    for (int c = 0; c < (all ? 1 : npot); ++c) {
+        std::vector<float> cpos(pos.size());
+        std::vector<int32_t> cpot(nat_all);
+        int32_t nat = 0, central = 0, untruncated = 0;
+        check_status(refeff_fms_yprep_cluster(nat_all, pos.data(), pot.data(), c, radius, nat_all, cpos.data(),
+                                              cpot.data(), &nat, &central, &untruncated),
+                     "failed to select FMS cluster from geom.dat");
+        if (all && nat <= max_pot)
+            stage_fail(REFEFF_FMS_MISSING_REPRESENTATIVE, "FMS cluster has " + std::to_string(nat) +
+                                                              " atom(s), but full-potential generation needs representatives through potential " +
+                                                              std::to_string(max_pot));
+        if (all) {
+            std::vector<int32_t> reps(npot);
+            check_status(refeff_fms_sort_representative_atoms(c, max_pot, nat, cpos.data(), cpot.data(), reps.data()),
+                         "failed to prepare POT SCF FMS representative atoms from geom.dat");
+        }
+        if (nat <= 1) continue;  // FEFF fmsie calls FMS only for inclus > 1
+        refeff_fms_plan_desc d{};
+        d.lfms = all ? a.lfms1 : 0;
+        d.spin_channels = 1;
+        d.atom_count = nat;
+        d.positions = cpos.data();
+        d.potentials = cpot.data();
+        d.max_potential = max_pot;
+        d.global_lmax = L;
+        d.raw_potential_lmax = a.lmaxsc;
+        d.direct_cutoff = rdirec;
+        d.solver = solver;
+        refeff_fms_plan* plan = nullptr;
+        check_status(refeff_fms_plan_create(&d, &plan),
+                     "failed to prepare POT SCF FMS real-space plan for central potential " + std::to_string(c));
+        refeff_fms_plan_info info{};
+        int rc = refeff_fms_plan_query(plan, &info);
+        if (rc == REFEFF_FMS_OK && info.channel_count != ch) {
+            refeff_fms_plan_destroy(plan);
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "POT SCF FMS plan has " + std::to_string(info.channel_count) +
+                                                        " channel(s), expected " + std::to_string(ch));
+        }
+        std::vector<double> sc(2 * static_cast<size_t>(a.ne) * npot * ch * ch, 0.0);
+        std::vector<int32_t> status(a.ne, 0);
+        if (rc == REFEFF_FMS_OK)
+            rc = refeff_fms_solve_batch(plan, a.ne, k.data(), ph.data(), sc.data(), status.data(), nullptr);
+        refeff_fms_plan_destroy(plan);
+        for (int e = 0; e < a.ne; ++e)
+            if (status[e])
+                stage_fail(status[e], "failed to solve POT SCF FMS central potential " + std::to_string(c) +
+                                          " energy section " + std::to_string(e + 1));
+        check_status(rc, "failed to solve the POT SCF FMS energy sweep");
+        for (int e = 0; e < a.ne; ++e)
+            for (int ip = 0; ip < npot; ++ip) {
+                if (!all && ip != c) continue;
+                const size_t o = 2 * (static_cast<size_t>(e) * npot + ip) * ch * ch;
+                std::copy(sc.begin() + o, sc.begin() + o + 2 * ch * ch, sc_all.begin() + o);
+            }
+    }
+    for (int e = 0; e < a.ne; ++e)
+        for (int ip = 0; ip < npot; ++ip) {
+            const double* blk = sc_all.data() + 2 * (static_cast<size_t>(e) * npot + ip) * ch * ch;
+            for (int l = 0; l < a.angular; ++l) {
+                cplx sum(0, 0);
+                for (int st = l * l; st < (l + 1) * (l + 1); ++st) {
+                    const size_t o = static_cast<size_t>(st) * (ch + 1);
+                    sum += cplx(blk[2 * o], blk[2 * o + 1]);
+                }
+                const size_t po = (static_cast<size_t>(e) * a.nphl + l) * npot + ip;
+                const cplx v = sum * std::exp(cplx(0.0, 2.0) * cplx(a.phases[2 * po], a.phases[2 * po + 1])) /
+                               static_cast<double>(2 * l + 1);
+                const size_t to = (static_cast<size_t>(e) * a.angular + l) * npot + ip;
+                a.traces[2 * to] = v.real();
+                a.traces[2 * to + 1] = v.imag();
+            }
+        }
+    return REFEFF_FMS_OK;
+}
+
 // MKGTR over a cached gg: refeff run_mkgtr_in_dir / build_mkgtr_ordinary_outputs,
 // crates/refeff-engine/src/fms.rs:559-631, 5255-5330, 5528-5608, 5849-5918,
 // 5941-5963.  The gg sections stay in FP64 (refeff narrows them to c32).
@@ -775,6 +968,20 @@ int refeff_ldos_gtr_run_in_dir(const char* work_dir, int32_t solver, int32_t* fi
                                size_t message_len) {
     if (!work_dir) return REFEFF_FMS_INVALID_ARGUMENT;
     const int rc = guarded([&] { return run_ldos_gtr(work_dir, solver, files_written); });
+    put_message(message, message_len, rc ? g_last_error : std::string());
+    return rc;
+}
+
+int refeff_fms_pot_scf_traces(const char* work_dir, int32_t solver, int32_t lfms1, double rfms1, int32_t nph,
+                              const int32_t* lmaxsc, int32_t energy_count, const double* energies,
+                              int32_t reference_count, const double* reference_energies, int32_t phase_l_count,
+                              const double* phase_shifts, int32_t angular_count, double* traces, char* message,
+                              size_t message_len) {
+    if (!work_dir || !lmaxsc || !energies || !reference_energies || !phase_shifts || !traces)
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    const PotScfArgs a{lfms1, rfms1, nph, lmaxsc, energy_count, energies, reference_count, reference_energies,
+                       phase_l_count, phase_shifts, angular_count, traces};
+    const int rc = guarded([&] { return run_pot_scf(work_dir, solver, a); });
     put_message(message, message_len, rc ? g_last_error : std::string());
     return rc;
 }

@@ -89,6 +89,8 @@ def _L():
     L.refeff_io_ldos_inp_parse.argtypes = [cp, C.POINTER(LdosInpC)]
     L.refeff_io_ldos_inp_string.argtypes = [C.POINTER(LdosInpC), C.c_char_p, sz, szp]
     L.refeff_ldos_gtr_run_in_dir.argtypes = [C.c_char_p, i32, i32p, C.c_char_p, sz]
+    L.refeff_fms_pot_scf_traces.argtypes = [C.c_char_p, i32, i32, C.c_double, i32, i32p, i32, vp, i32, vp, i32, vp,
+                                            i32, vp, C.c_char_p, sz]
     _bound = True
     return L
 
@@ -558,3 +560,32 @@ def run_ldos_gtr_in_dir(work_dir: str, solver: int = 0) -> int:
     if rc != 0:
         raise CodecError(rc, "run_ldos_gtr_in_dir", msg.value.decode(errors="replace"))
     return n.value
+
+
+def pot_scf_fms_traces(work_dir: str, lfms1: int, rfms1: float, lmaxsc, energies, reference_energies,
+                       phase_shifts, angular_count: int, solver: int = 0):
+    """POT self-consistency FMS traces on the POT energy grid, solved on the GPU
+    over geom.dat in work_dir (refeff build_pot_scf_fms_source_grid_handoff,
+    engine fms.rs:2650-2965, real-space branch).  energies: [ne] complex Hartree;
+    reference_energies: [ne, >= nph+1] complex (column nph sets k); phase_shifts:
+    [ne, nl, nph+1] complex indexed by l.  Returns complex128 [ne, angular, nph+1]
+    with sum_m gg(l^2+m, l^2+m) exp(2 i delta_l) / (2 l + 1)."""
+    lmax = np.ascontiguousarray(lmaxsc, dtype=np.int32)
+    nph = lmax.shape[0] - 1
+    e = np.ascontiguousarray(energies, dtype=np.complex128).reshape(-1)
+    ne = e.shape[0]
+    ref = np.ascontiguousarray(reference_energies, dtype=np.complex128)
+    ph = np.ascontiguousarray(phase_shifts, dtype=np.complex128)
+    if ref.ndim != 2 or ref.shape[0] < ne or ph.ndim != 3 or ph.shape[0] < ne or ph.shape[2] != nph + 1:
+        raise CodecError(1, "pot_scf_fms_traces",
+                         f"table shapes {ref.shape} / {ph.shape} do not fit {ne} energies and {nph + 1} potentials")
+    ref, ph = np.ascontiguousarray(ref[:ne]), np.ascontiguousarray(ph[:ne])
+    out = np.zeros((max(1, ne), max(1, angular_count), nph + 1), np.complex128)
+    msg = C.create_string_buffer(1024)
+    rc = _L().refeff_fms_pot_scf_traces(str(work_dir).encode(), solver, int(lfms1), float(rfms1), nph,
+                                        lmax.ctypes.data_as(C.POINTER(C.c_int32)), ne, e.ctypes.data,
+                                        ref.shape[1], ref.ctypes.data, ph.shape[1], ph.ctypes.data,
+                                        int(angular_count), out.ctypes.data, msg, len(msg))
+    if rc != 0:
+        raise CodecError(rc, "pot_scf_fms_traces", msg.value.decode(errors="replace"))
+    return out[:ne, :angular_count]

@@ -131,3 +131,69 @@ def test_debye_sigsqr_validates_on_the_host():
                                                  "temperature": -5.0})
     assert one.shape == (1, 1) and one[0, 0] == 0.0
     assert fms.debye_mean_square_displacements(**{**base, "positions": pos[:0], "atomic_numbers": []}).shape == (0, 0)
+
+
+def _pot_scf_tables(ne=2, nl=2, npot=2):
+    """The shapes of refeff's POT SCF handoff tests (engine fms/tests.rs:1600-1625)."""
+    e = np.array([1.0 + 0.02j, 1.3 + 0.03j])[:ne]
+    refs = np.array([[0.05 * p - 1j * x.imag for p in range(npot)] for x in e])
+    ph = np.array([[[0.02 * (i + 1) + 0.03 * l - 0.004j * p for p in range(npot)] for l in range(nl)]
+                   for i in range(ne)])
+    return e, refs, ph
+
+
+def _pot_scf_geom(d, pts, iph, nph):
+    from refeff_amd import io
+    n = len(iph)
+    g = io.GeomDat(n, nph, list(range(1, nph + 2)), np.arange(1, n + 1), np.asarray(pts, float),
+                   np.asarray(iph, np.int32), np.zeros(n, np.int32))
+    (d / "geom.dat").write_text(io.geom_dat_string(g))
+
+
+def test_pot_scf_traces_host_paths(tmp_path):
+    """The POT SCF FMS paths that end before any solve (engine fms.rs:2663-2760,
+    2885-2889): a zero radius gives zero traces even beside a reciprocal.inp
+    (fms/tests.rs:1728-1771), a one-atom cluster is never solved and keeps zero
+    traces (fms/tests.rs:1690-1726), and the shape and cluster checks fail."""
+    from refeff_amd import io
+    e, refs, ph = _pot_scf_tables()
+    (tmp_path / "reciprocal.inp").write_text("ispace\n   0\n")
+    t = io.pot_scf_fms_traces(tmp_path, 1, 0.0, [1, 1], e, refs, ph, 2)
+    assert t.shape == (2, 2, 2) and not t.any()
+    with pytest.raises(io.CodecError) as err:
+        io.pot_scf_fms_traces(tmp_path, 1, 3.0, [1, 1], e, refs, ph, 2)
+    assert err.value.name == "UNSUPPORTED"
+    for text, msg in [("ispace\n   2\n", "ispace must be 0 or 1, got 2"), ("ispace\n\n", "RECIPROCAL ispace line")]:
+        (tmp_path / "reciprocal.inp").write_text(text)  # refeff validates ispace before the zero-radius exit
+        with pytest.raises(io.CodecError) as err:
+            io.pot_scf_fms_traces(tmp_path, 1, 0.0, [1, 1], e, refs, ph, 2)
+        assert msg in str(err.value)
+    (tmp_path / "reciprocal.inp").unlink()
+
+    one = tmp_path / "one"
+    one.mkdir()
+    _pot_scf_geom(one, [[0.0, 0.0, 0.0]], [0], 0)
+    e1, refs1, ph1 = _pot_scf_tables(npot=1)
+    for lfms1 in (0, 1):
+        t = io.pot_scf_fms_traces(one, lfms1, 3.0, [1], e1, refs1, ph1, 2)
+        assert t.shape == (2, 2, 1) and not t.any()
+
+    _pot_scf_geom(tmp_path, [[0.0, 0.0, 0.0], [1.4, 0.0, 0.0]], [0, 1], 1)
+    cases = [(dict(angular=3), "angular channel"),                        # lmaxsc supports 2
+             (dict(ph=ph[:, :1]), "phase table has 1 angular"),
+             (dict(refs=refs[:, :1]), "reference-energy table"),
+             (dict(lmaxsc=[1, -1]), "lmaxsc(1) must be nonnegative"),
+             (dict(rfms1=1.0, lfms1=1), "representatives through potential 1"),  # a lone absorber
+             (dict(e=e[:0]), "at least one energy row")]
+    for kw, text in cases:
+        a = dict(lfms1=0, rfms1=3.0, lmaxsc=[1, 1], e=e, refs=refs, ph=ph, angular=2)
+        a.update(kw)
+        with pytest.raises(io.CodecError) as err:
+            io.pot_scf_fms_traces(tmp_path, a["lfms1"], a["rfms1"], a["lmaxsc"], a["e"], a["refs"], a["ph"],
+                                  a["angular"])
+        assert text in str(err.value), (kw, str(err.value))
+    _pot_scf_geom(tmp_path, [[0.0, 0.0, 0.0], [1.4, 0.0, 0.0]], [0, 0], 0)
+    with pytest.raises(io.CodecError) as err:
+        io.pot_scf_fms_traces(tmp_path, 0, 3.0, [1, 1], e, refs, ph, 2)
+    assert "does not match maximum potential 1" in str(err.value)
+

@@ -49,6 +49,11 @@ nq,    imdff,   qaverage,   mixdff
 """
 
 
+# refeff rdinp's default reciprocal.inp (refeff-io control_input/tests.rs:400-409):
+# ispace 1, the real-space solve
+RECIPROCAL_REAL_SPACE = "ispace\n   1\n"
+
+
 def make_workdir(d, radius=4.6, ne=7, sig2g=0.003, save_gg_slice=False, lmaxph=(3, 3), idwopt=-1, tk=0.0,
                  thetad=0.0, z=(29, 29)):
     pts = W.lattice(W.FCC, 3.615, radius + 1.0)  # geom.dat holds more atoms than the FMS sphere
@@ -101,6 +106,8 @@ def oracle_gg(pts, iph, k, ph, radius, sig2g, sig=None):
 def test_module_fms_writes_gg_matching_oracle(tmp_path, solver):
     banner(f"module fms, solver {solver}")
     pts, iph, k, ph, rkk, em, radius, sig2g = make_workdir(tmp_path)
+    if solver == 2:
+        (tmp_path / "reciprocal.inp").write_text(RECIPROCAL_REAL_SPACE)  # keeps the real-space solve
     assert io.run_fms_in_dir(tmp_path, solver) == 3
     sections = io.parse_gg_bin((tmp_path / "gg.bin").read_text())
     assert (tmp_path / "gg.dat").read_text() == (tmp_path / "gg.bin").read_text()
@@ -227,6 +234,19 @@ def test_module_fms_debye_models_match_oracle(tmp_path, idwopt):
     banner(f"gg.bin with idwopt {idwopt} matches oracle, cluster {nat} atoms")
 
 
+def test_module_fms_reciprocal_ispace(tmp_path):
+    """reciprocal.inp ispace 0 asks for refeff's k-space FMS, which is not
+    provided; any value other than 0 or 1 is refeff's bail (engine fms.rs:771-785)."""
+    banner("module fms beside reciprocal.inp ispace 0 / 3")
+    make_workdir(tmp_path)
+    for text, name in [("ispace\n   0\n", "UNSUPPORTED"), ("ispace\n   3\n", "INVALID_ARGUMENT")]:
+        (tmp_path / "reciprocal.inp").write_text(text)
+        with pytest.raises(io.CodecError) as e:
+            io.run_fms_in_dir(tmp_path, 0)
+        assert e.value.name == name, str(e.value)
+    assert not (tmp_path / "gg.bin").exists()
+
+
 def test_module_fms_sig2_dat_must_cover_every_pair(tmp_path):
     banner("module fms, short sig2.dat")
     make_workdir(tmp_path, idwopt=4, tk=300.0, thetad=315.0)
@@ -303,3 +323,65 @@ def test_ldos_gtr_needs_the_ldos_energy_grid(tmp_path):
     (tmp_path / "ldos.inp").write_text(io.ldos_input_string({**d, "emax": d["emax"] + 1.0}))
     assert io.run_ldos_gtr_in_dir(tmp_path, 0) == 0
     assert not list(tmp_path.glob("gtr*.bin"))
+
+
+def make_pot_scf_case(d, rfms1=5.5, ne=5, lmaxsc=(3, 3)):
+    """geom.dat of an fcc Cu cluster (potential 0 at the origin) and the POT
+    SCF energy grid, reference energies and l-indexed phase table in the shapes
+    refeff's POT loop hands over (engine fms/tests.rs:1600-1625)."""
+    pts = W.lattice(W.FCC, 3.615, rfms1 + 3.0)
+    pts = pts[np.argsort(np.linalg.norm(pts, axis=1), kind="stable")]
+    iph = np.ones(len(pts), np.int32)
+    iph[0] = 0
+    g = io.GeomDat(len(pts), 1, [1, 2], np.arange(1, len(pts) + 1), pts, iph, np.ones(len(pts), np.int32))
+    (d / "geom.dat").write_text(io.geom_dat_string(g))
+    e = np.linspace(0.3, 1.6, ne) + 0.02j
+    refs = np.array([[0.05 * p - 1j * x.imag for p in range(2)] for x in e])
+    k = np.sqrt(2.0 * (e - refs[:, 1])) / A0
+    L = max(lmaxsc)
+    phl = np.ascontiguousarray(W.synthetic_phases(k, L, np.array(lmaxsc))[:, 0, L:, :])  # [ne, L + 1, 2], by l
+    return pts, iph, e, refs, k, phl
+
+
+def oracle_pot_scf(pts, iph, k, phl, rfms1, lfms1, lmaxsc, angular):
+    L = max(lmaxsc)
+    signed = np.zeros((len(k), 1, 2 * L + 1, 2), np.complex128)
+    for ip in range(2):
+        for sl in range(-lmaxsc[ip], lmaxsc[ip] + 1):
+            signed[:, 0, L + sl, ip] = phl[:, abs(sl), ip]
+    sc_all = np.zeros((len(k), 2, (L + 1) ** 2, (L + 1) ** 2), np.complex128)
+    for c in ([0] if lfms1 else [0, 1]):
+        pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, c, np.float32(rfms1), len(iph))
+        if lfms1:
+            pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
+        plan = orc.Plan(pos, pot, nsp=1, lfms=lfms1, max_pot=1, global_lmax=L, raw_lmax=list(lmaxsc),
+                        rdirec=2.0 * rfms1, sigsqr=np.zeros((len(pot), len(pot))), toler1=0.0, toler2=0.0)
+        sc, st = plan.spectrum(k.reshape(-1, 1), signed, threads=8)
+        assert not st.any()
+        if lfms1:
+            sc_all = sc
+        else:
+            sc_all[:, c] = sc[:, c]
+    return orc.pot_scf_trace_grid(sc_all, phl, angular)
+
+
+@pytest.mark.parametrize("lfms1,lmaxsc,angular", [(1, (3, 3), 4), (0, (3, 3), 4), (1, (3, 2), 3), (0, (2, 3), 4)])
+def test_pot_scf_traces_match_oracle(tmp_path, lfms1, lmaxsc, angular):
+    """POT SCF FMS traces (engine fms.rs:2650-2965): lfms1 != 0 solves one
+    cluster around potential 0 and projects every potential's block; lfms1 = 0
+    solves a cluster around each potential's first atom and keeps its own block.
+    No damping, direct cutoff 2 rfms1, k from the highest potential's reference
+    energy, the l-indexed phases spread over signed l up to each lmaxsc."""
+    banner(f"POT SCF traces lfms1={lfms1} lmaxsc={lmaxsc} angular={angular}")
+    pts, iph, e, refs, k, phl = make_pot_scf_case(tmp_path, lmaxsc=lmaxsc)
+    if lfms1:
+        (tmp_path / "reciprocal.inp").write_text(RECIPROCAL_REAL_SPACE)
+    got = io.pot_scf_fms_traces(tmp_path, lfms1, 5.5, list(lmaxsc), e, refs, phl, angular)
+    want = oracle_pot_scf(pts, iph, k, phl, 5.5, lfms1, lmaxsc, angular)
+    assert got.shape == want.shape == (len(e), angular, 2)
+    for ip in range(2):
+        for x in range(len(e)):
+            err = np.abs(got[x, :, ip] - want[x, :, ip]).max() / np.abs(want[x, :, ip]).max()
+            assert err <= 1e-8, (ip, x, err)
+    banner("  traces match the oracle to 1e-8")
+

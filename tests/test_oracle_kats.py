@@ -746,3 +746,38 @@ def test_ldos_fmsdos_trace_of_a_lone_atom_is_its_t_matrix_sum():
     ph[0, 0, 2:, 0] = delta
     t = orc.ldos_fmsdos_trace_grid(sc, ph, 3)
     assert np.allclose(t[0, 0], np.exp(3j * delta) * np.sin(delta), rtol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# POT SCF FMS trace: refeff screen/tests/response.rs:482-519 (KAT) and
+# refeff-io screen_dat.rs:651-685 (l-indexed phase, [e][l][iph] order)
+# --------------------------------------------------------------------------
+
+def test_cluster_green_trace_kat():
+    """refeff fms_cluster_green_trace_matches_feff_phase_trace_reference: the l = 1
+    trace of diag(9, 1 + 0.5i, -0.25 + 0.75i, 0.125 - 0.375i) at delta 0.2 + 0.05i."""
+    sc = np.diag([9.0, 1 + .5j, -.25 + .75j, .125 - .375j])
+    sc[0, 1] = sc[2, 3] = 7.0  # off-diagonal entries never enter
+    t = orc.cluster_green_trace(sc, 0.2 + 0.05j, 1)
+    assert abs(t.real - 0.140_306_297_914_067_32) <= 1e-14
+    assert abs(t.imag - 0.345_849_798_891_802_3) <= 1e-14
+    assert orc.cluster_green_trace(sc, 0.2 + 0.05j, 0) == pytest.approx(9.0 * np.exp(2j * (0.2 + 0.05j)), rel=1e-15)
+
+
+def test_pot_scf_trace_grid_is_the_fmsdos_trace_on_l_indexed_phases():
+    """The POT SCF projection and FEFF's fmsdos trace (pinned above) agree once the
+    l-indexed phases sit at the signed-l slot L + l, for every energy, potential
+    and l; the output is ordered [e][l][iph]."""
+    rng = np.random.default_rng(4)
+    ne, npot, L = 3, 2, 2
+    ch = (L + 1) ** 2
+    sc = rng.normal(size=(ne, npot, ch, ch)) + 1j * rng.normal(size=(ne, npot, ch, ch))
+    phl = rng.normal(size=(ne, L + 1, npot)) * 0.3 + 0.02j
+    signed = np.zeros((ne, 1, 2 * L + 1, npot), np.complex128)
+    signed[:, 0, L:, :] = phl
+    signed[:, 0, :L, :] = 9.0  # the negative slots never enter
+    got = orc.pot_scf_trace_grid(sc, phl, L + 1)
+    want = orc.ldos_fmsdos_trace_grid(sc, signed, L + 1)
+    assert got.shape == (ne, L + 1, npot)
+    assert np.allclose(got, want.transpose(0, 2, 1), rtol=1e-14, atol=0)
+
