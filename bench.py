@@ -41,9 +41,9 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip the plain FP64 LU comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle accuracy sample")
     ap.add_argument("--cpu-energies", type=int, default=6)
-    ap.add_argument("--caller", default=None, choices=["pot_scf"],
-                    help="time a caller of the FMS stage instead of the stage itself: pot_scf = the POT "
-                         "self-consistency traces (one JSON line of its own)")
+    ap.add_argument("--caller", default=None, choices=["pot_scf", "screen"],
+                    help="time a caller of the FMS stage instead of the stage itself (one JSON line of its "
+                         "own): pot_scf = the POT self-consistency traces, screen = SCREEN's absorber traces")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy (the reference kernel for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling under rocprofv3 --pmc)")
@@ -223,6 +223,67 @@ def run_pot_scf_caller(args):
     print(json.dumps(line), flush=True)
 
 
+def run_screen_caller(args):
+    """SCREEN's FMS cluster traces (refeff build_screen_fms_source_grid_handoff) over
+    a Cu cluster: whole calls timed, fms.inp / geom.dat reads and host preparation
+    included.  The correlated Debye model damps the pairs (sigma^2 on the GPU)."""
+    import tempfile
+    from refeff_amd import io
+    from oracle import oracle as orc
+    radius, rdirec, ne, lmaxph, angular = 6.0, 12.0, 64, [3, 3], 4
+    tk, thetad, rnrmav, z, sig2g = 300.0, 315.0, 2.6, [29, 29], 0.0
+    d = tempfile.mkdtemp(prefix="screen_")
+    pts, iph, _, _, _, _ = pot_scf_case(d, radius, 1, lmaxph)
+    fin = io.FmsInput(mfms=1, idwopt=0, minv=0, rfms2=radius, rdirec=rdirec, toler1=0.001, toler2=0.001, tk=tk,
+                      thetad=thetad, sig2g=sig2g, lmaxph=lmaxph, decomposition_channels=-1, save_gg_slice=False,
+                      do_fms=1)
+    with open(os.path.join(d, "fms.inp"), "w") as f:
+        f.write(io.fms_input_string(fin))
+    from refeff_amd import workloads as W
+    e = np.linspace(0.25, 1.6, ne) + 0.03j
+    k_bohr = np.sqrt(2.0 * e)
+    phl = np.ascontiguousarray(W.synthetic_phases(k_bohr / 0.529177249, 3, np.array(lmaxph))[:, 0, 3:, :])
+    call = lambda: io.screen_fms_traces(d, k_bohr, phl, angular, radius, rdirec, atomic_numbers=z,
+                                        average_norman_radius=rnrmav)
+    for _ in range(args.warmup):
+        call()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        got, _ = call()
+    dt = (time.perf_counter() - t0) / args.steps
+    pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(radius), len(iph))
+    pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
+    nat = len(pot)
+    line = {"metric": "SCREEN FMS trace energy-pts/s", "value": round(ne / dt, 3), "unit": "energy-points/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64 LU + f64 refinement",
+            "data": "synthetic (Cu fcc geom.dat, smooth model phase shifts)",
+            "config": {"workload": f"screen Cu fcc radius={radius} ({nat} atoms, {nat * 16} states), {ne} energies, "
+                                   f"lmaxph={lmaxph}, correlated Debye at {tk} K"}}
+    if not args.no_cpu:
+        cores = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        sample = sorted(set(np.linspace(0, ne - 1, cores).astype(int).tolist()))
+        signed = np.zeros((len(sample), 1, 7, 2), np.complex128)
+        for ip in range(2):
+            for sl in range(-3, 4):
+                signed[:, 0, 3 + sl, ip] = phl[sample, abs(sl), ip]
+        t0 = time.perf_counter()
+        sig = orc.debye_sigsqr(0, tk, thetad, rnrmav, pos, np.array(z, np.int32)[pot], np.float32(sig2g))
+        plan = orc.Plan(pos, pot, nsp=1, lfms=0, max_pot=1, global_lmax=3, raw_lmax=lmaxph, rdirec=rdirec,
+                        sigsqr=sig, toler1=0.001, toler2=0.001)
+        sc, st = plan.spectrum((k_bohr[sample] / 0.529177249).reshape(-1, 1), signed, threads=len(sample))
+        t_cpu = time.perf_counter() - t0
+        want = orc.pot_scf_trace_grid(sc, phl[sample], angular)[:, :, 0]
+        line["cpu_baseline"] = {"value": round(len(sample) / t_cpu, 4), "unit": "energy-points/s",
+                                "cores": len(sample), "kind": "port",
+                                "sample": f"{len(sample)} of {ne} energies; oracle Debye sigma^2 (1 thread) and FP64 "
+                                          f"G0/t assembly + LU, one energy per thread"}
+        line["accuracy"] = {"max_trace_rel_err_vs_fp64_oracle":
+                            float(np.abs(got[sample] - want).max() / np.abs(want).max()),
+                            "energies_checked": len(sample), "gate": 1e-8}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of a known size: rocprofv3 --pmc reads FETCH_SIZE / WRITE_SIZE
     for it, and the ratio to nbytes scales the counters of the FMS kernels."""
@@ -246,6 +307,9 @@ def main():
     L = lib()
     if args.caller == "pot_scf":
         run_pot_scf_caller(args)
+        return
+    if args.caller == "screen":
+        run_screen_caller(args)
         return
     if args.copy_calibration:
         copy_calibration()

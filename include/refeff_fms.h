@@ -320,6 +320,27 @@ int refeff_fms_pot_scf_traces(const char* work_dir, int32_t solver, int32_t lfms
                               const double* phase_shifts, int32_t angular_count, double* traces, char* message,
                               size_t message_len);
 
+/* SCREEN's FMS cluster Green's function traces for the absorbing potential.
+ * Replaces build_screen_fms_source_grid_handoff_with_potential_count,
+ * crates/refeff-engine/src/fms.rs:2464-2648 (both the phase.bin and the
+ * pot.bin variants, :2436-2462).  fms.inp and geom.dat are read from work_dir;
+ * when either is missing *produced is 0 and nothing else is written (refeff's
+ * None).  cluster_radius and direct_cutoff replace fms.inp's rfms2 and rdirec.
+ * The cluster is the sortat cluster around potential 0, damped as fms.inp's
+ * idwopt says (Debye models 0 and 3 take atomic_numbers[potential_count] and
+ * average_norman_radius from phase.bin or pot.bin; other models ignore them, and
+ * atomic_numbers may be NULL), solved with lfms 0 and fms.inp's tolerances.
+ *   wave_numbers_bohr: [energy_count] complex, 1/bohr
+ *   phase_shifts: [energy_count][phase_l_count][potential_count] complex, by l
+ *   traces: [energy_count][angular_count] complex out, the absorber's
+ *     sum_m gg(l^2 + m, l^2 + m) exp(2 i delta_l) / (2 l + 1)
+ *   *potential_index: the absorber's potential; *produced: 1 */
+int refeff_fms_screen_traces(const char* work_dir, int32_t solver, int32_t potential_count,
+                             const int32_t* atomic_numbers, double average_norman_radius, int32_t energy_count,
+                             const double* wave_numbers_bohr, int32_t phase_l_count, const double* phase_shifts,
+                             int32_t angular_count, double cluster_radius, double direct_cutoff, double* traces,
+                             int32_t* potential_index, int32_t* produced, char* message, size_t message_len);
+
 /* LDOS gtrNN.bin from the FMS source handoffs.  Replaces
  * write_ldos_gtr_bin_source_handoffs, crates/refeff-engine/src/fms.rs:1226-1247
  * (with :1689-1760, 2095-2245): reads ldos.inp, fms.inp, phase.bin, geom.dat and

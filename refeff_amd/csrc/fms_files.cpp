@@ -177,7 +177,24 @@ void validate_controls(const rio::FmsInp& in) {
 //               atom1 -> atom2 -> atom1, on the GPU (refeff_fms_debye_sigsqr), after
 //               validate_fms_debye_damping_metadata (:4879-4905) and
 //               fms_atom_atomic_number (:4913-4945).
-std::vector<float> pair_sigsqr(const std::string& dir, const rio::FmsInp& in, const rio::PhaseBin& phase, int nat,
+// What the Debye models need besides the cluster (refeff FmsDebyeDampingMetadata,
+// engine fms.rs:4879-4915): the average Norman radius and each potential's Z,
+// from phase.bin or from the caller (SCREEN's pot.bin variant).
+struct DebyeMeta {
+    double rnrmav = 0.0;
+    std::vector<int32_t> z;  // per potential
+    std::string table;       // named in errors
+};
+
+DebyeMeta debye_meta(const rio::PhaseBin& phase) {
+    DebyeMeta m;
+    m.rnrmav = phase.scalars[0];
+    for (const rio::PhasePotential& p : phase.potentials) m.z.push_back(static_cast<int32_t>(p.atomic_number));
+    m.table = "phase.bin";
+    return m;
+}
+
+std::vector<float> pair_sigsqr(const std::string& dir, const rio::FmsInp& in, const DebyeMeta& meta, int nat,
                                const std::vector<float>& cpos, const std::vector<int32_t>& cpot) {
     const float sig2g = narrow_nonnegative(in.sig2g, "FMS SIG2");
     std::vector<float> sig(static_cast<size_t>(nat) * nat, 0.0f);
@@ -207,12 +224,12 @@ std::vector<float> pair_sigsqr(const std::string& dir, const rio::FmsInp& in, co
         stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS source generation received unexpected idwopt=" +
                                                     std::to_string(in.idwopt) + " Debye-Waller damping");
     }
-    const double rnrmav = phase.scalars[0];
+    const double rnrmav = meta.rnrmav;
     if (!std::isfinite(rnrmav) || rnrmav <= 0.0)
         stage_fail(REFEFF_FMS_INVALID_ARGUMENT,
                    "FMS Debye damping requires a positive finite average Norman radius, got " + std::to_string(rnrmav));
-    for (size_t ip = 0; ip < phase.potentials.size(); ++ip)
-        if (phase.potentials[ip].atomic_number == 0)
+    for (size_t ip = 0; ip < meta.z.size(); ++ip)
+        if (meta.z[ip] <= 0)
             stage_fail(REFEFF_FMS_INVALID_ARGUMENT,
                        "FMS Debye damping requires a positive atomic number for potential " + std::to_string(ip));
     std::vector<int32_t> z(nat);
@@ -220,10 +237,10 @@ std::vector<float> pair_sigsqr(const std::string& dir, const rio::FmsInp& in, co
         if (cpot[i] < 0)
             stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS atom potential must be nonnegative for Debye damping, got " +
                                                         std::to_string(cpot[i]));
-        if (cpot[i] >= static_cast<int>(phase.potentials.size()))
+        if (cpot[i] >= static_cast<int>(meta.z.size()))
             stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS atom potential " + std::to_string(cpot[i]) +
-                                                        " is outside phase.bin potential table");
-        z[i] = phase.potentials[cpot[i]].atomic_number;
+                                                        " is outside " + meta.table + " potential table");
+        z[i] = meta.z[cpot[i]];
     }
     const char* what = in.idwopt == 0 ? "FMS correlated Debye-Waller pair damping"
                                       : "FMS classical Debye-Waller pair damping";
@@ -305,7 +322,7 @@ ClusterSolve solve_cluster(const std::string& dir, const rio::FmsInp& in, const 
                      "failed to prepare FMS representative atoms from geom.dat");
     }
     if (cpot[0] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS absorber potential must be nonnegative");
-    const std::vector<float> sig = pair_sigsqr(dir, in, phase, nat, cpos, cpot);
+    const std::vector<float> sig = pair_sigsqr(dir, in, debye_meta(phase), nat, cpos, cpot);
 
     refeff_fms_plan_desc d{};
     d.lfms = in.do_fms;
@@ -689,6 +706,36 @@ int run_ldos_gtr(const std::string& dir, int solver, int32_t* files_written) {
 // one cluster around potential 0 (sortat), every potential's block; else one
 // cluster per potential c around its first atom (yprep order), block c only.
 // Wave numbers use the highest potential's reference energy.
+// screen_fms_cluster_green_trace (refeff-core screen/response.rs:545-581): the
+// l-shell diagonal of one potential's gg block [ch][ch] (complex pairs),
+// times exp(2 i delta_l) / (2 l + 1).
+cplx cluster_green_trace(const double* blk, int ch, cplx delta, int l) {
+    cplx sum(0, 0);
+    for (int st = l * l; st < (l + 1) * (l + 1); ++st) {
+        const size_t o = static_cast<size_t>(st) * (ch + 1);
+        sum += cplx(blk[2 * o], blk[2 * o + 1]);
+    }
+    return sum * std::exp(cplx(0.0, 2.0) * delta) / static_cast<double>(2 * l + 1);
+}
+
+// The l-indexed phase table [ne][nphl][npot] spread over signed l up to each
+// potential's lmax (engine fms.rs:5056-5116, 5118-5175): [ne][2L+1][npot].
+std::vector<double> signed_l_phases(int ne, int nphl, int npot, int L, const int32_t* lmax, const double* phases) {
+    const int nl = 2 * L + 1;
+    std::vector<double> ph(2 * static_cast<size_t>(ne) * nl * npot, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int ip = 0; ip < npot; ++ip) {
+            const int eff = std::min(lmax[ip], L);
+            for (int sl = -eff; sl <= eff; ++sl) {
+                const size_t src = (static_cast<size_t>(e) * nphl + std::abs(sl)) * npot + ip;
+                const size_t dst = (static_cast<size_t>(e) * nl + (sl + L)) * npot + ip;
+                ph[2 * dst] = phases[2 * src];
+                ph[2 * dst + 1] = phases[2 * src + 1];
+            }
+        }
+    return ph;
+}
+
 struct PotScfArgs {
     int lfms1;
     double rfms1;
@@ -742,24 +789,15 @@ int run_pot_scf(const std::string& dir, int solver, const PotScfArgs& a) {
     const float rdirec = narrow_nonnegative(2.0 * a.rfms1, "POT SCF FMS direct cutoff");
     const bool all = a.lfms1 != 0;
     // wave numbers and signed-l phase shifts, shared by every cluster
-    const int nl = 2 * L + 1;
-    std::vector<double> k(2 * static_cast<size_t>(a.ne)), ph(2 * static_cast<size_t>(a.ne) * nl * npot, 0.0);
+    std::vector<double> k(2 * static_cast<size_t>(a.ne));
     for (int e = 0; e < a.ne; ++e) {
         const size_t ro = static_cast<size_t>(e) * a.nref + max_pot;
         const cplx w = std::sqrt(cplx(2.0, 0.0) * (cplx(a.energies[2 * e], a.energies[2 * e + 1]) -
                                                   cplx(a.references[2 * ro], a.references[2 * ro + 1]))) / 0.529177249;
         k[2 * e] = w.real();
         k[2 * e + 1] = w.imag();
-        for (int ip = 0; ip < npot; ++ip) {
-            const int eff = std::min(a.lmaxsc[ip], L);
-            for (int sl = -eff; sl <= eff; ++sl) {
-                const size_t src = (static_cast<size_t>(e) * a.nphl + std::abs(sl)) * npot + ip;
-                const size_t dst = (static_cast<size_t>(e) * nl + (sl + L)) * npot + ip;
-                ph[2 * dst] = a.phases[2 * src];
-                ph[2 * dst + 1] = a.phases[2 * src + 1];
-            }
-        }
     }
+    const std::vector<double> ph = signed_l_phases(a.ne, a.nphl, npot, L, a.lmaxsc, a.phases);
     const int nat_all = static_cast<int>(geom.atoms.size());
     if (nat_all == 0) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
     std::vector<float> pos(3 * static_cast<size_t>(nat_all));
@@ -831,19 +869,172 @@ int run_pot_scf(const std::string& dir, int solver, const PotScfArgs& a) {
         for (int ip = 0; ip < npot; ++ip) {
             const double* blk = sc_all.data() + 2 * (static_cast<size_t>(e) * npot + ip) * ch * ch;
             for (int l = 0; l < a.angular; ++l) {
-                cplx sum(0, 0);
-                for (int st = l * l; st < (l + 1) * (l + 1); ++st) {
-                    const size_t o = static_cast<size_t>(st) * (ch + 1);
-                    sum += cplx(blk[2 * o], blk[2 * o + 1]);
-                }
                 const size_t po = (static_cast<size_t>(e) * a.nphl + l) * npot + ip;
-                const cplx v = sum * std::exp(cplx(0.0, 2.0) * cplx(a.phases[2 * po], a.phases[2 * po + 1])) /
-                               static_cast<double>(2 * l + 1);
+                const cplx v = cluster_green_trace(blk, ch, cplx(a.phases[2 * po], a.phases[2 * po + 1]), l);
                 const size_t to = (static_cast<size_t>(e) * a.angular + l) * npot + ip;
                 a.traces[2 * to] = v.real();
                 a.traces[2 * to + 1] = v.imag();
             }
         }
+    return REFEFF_FMS_OK;
+}
+
+// SCREEN's FMS cluster Green's function traces: refeff
+// build_screen_fms_source_grid_handoff_with_potential_count (engine
+// fms.rs:2464-2648).  fms.inp and geom.dat come from the work directory (either
+// missing: nothing produced), rfms2 / rdirec from the caller; the cluster is the
+// sortat cluster around potential 0, damped as fms.inp's idwopt says, solved
+// with lfms 0 (the absorber's block only) and fms.inp's tolerances; k is the
+// caller's wave number in 1/bohr over a0.  The trace is the absorber's.
+struct ScreenArgs {
+    int npot;
+    const int32_t* z;          // [npot] atomic numbers, or null when no Debye model needs them
+    double rnrmav;
+    int ne;
+    const double* k_bohr;      // [ne] complex
+    int nphl;
+    const double* phases;      // [ne][nphl][npot] complex, l-indexed
+    int angular;
+    double radius, rdirec;
+    double* traces;            // [ne][angular] complex
+};
+
+int run_screen(const std::string& dir, int solver, const ScreenArgs& a, int32_t* potential_index, int32_t* produced) {
+    *produced = 0;
+    *potential_index = -1;
+    if (!rio::file_exists(join(dir, "fms.inp")) || !rio::file_exists(join(dir, "geom.dat"))) return REFEFF_FMS_OK;
+    rio::FmsInp in = rio::parse_fms_inp(rio::read_file(join(dir, "fms.inp")));
+    in.rfms2 = a.radius;
+    in.rdirec = a.rdirec;
+    const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
+    if (a.npot < 1) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "SCREEN FMS generation requires at least one potential");
+    const int npot = a.npot, max_pot = npot - 1;
+    if (geom.nph != max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "geom.dat nph " + std::to_string(geom.nph) +
+                                                    " does not match maximum potential " + std::to_string(max_pot) +
+                                                    " for SCREEN FMS generation");
+    if (a.ne < 0 || a.angular < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "SCREEN FMS counts must be nonnegative");
+    if (static_cast<int>(in.lmaxph.size()) <= max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp has " + std::to_string(in.lmaxph.size()) +
+                                                    " lmaxph value(s), expected at least " + std::to_string(npot) +
+                                                    " for FMS generation");
+    int L = 0;
+    for (int ip = 0; ip <= max_pot; ++ip) {
+        if (in.lmaxph[ip] < 0)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp lmaxph(" + std::to_string(ip) +
+                                                        ") must be nonnegative for FMS source generation, got " +
+                                                        std::to_string(in.lmaxph[ip]));
+        L = std::max(L, in.lmaxph[ip]);
+    }
+    if (a.angular > L + 1)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "screen.inp maxl requests " + std::to_string(a.angular) +
+                                                    " angular channel(s), but fms.inp lmaxph supports " +
+                                                    std::to_string(L + 1));
+    if (a.nphl < L + 1)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "SCREEN FMS source phase table has " + std::to_string(a.nphl) +
+                                                    " angular channel(s), expected at least " + std::to_string(L + 1));
+    // effective_fms_cluster_radius / effective_fms_direct_cutoff (engine fms.rs:4256-4284)
+    auto effective = [&](double v, const char* what, const char* full) {
+        if (v >= 0.0) return narrow_nonnegative(v, what);
+        if (in.do_fms == 0 && std::isfinite(v)) return 0.0f;
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, std::string("FMS source generation requires a nonnegative ") + full +
+                                                    " for full FMS, got " + num(v));
+    };
+    const float radius = effective(in.rfms2, "FMS cluster radius", "cluster radius");
+    const float rdirec = effective(in.rdirec, "FMS direct cutoff", "direct cutoff");
+    if (geom.atoms.empty()) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
+    const int nat_all = static_cast<int>(geom.atoms.size());
+    std::vector<float> pos(3 * static_cast<size_t>(nat_all));
+    std::vector<int32_t> pot(nat_all);
+    for (int i = 0; i < nat_all; ++i) {
+        pos[3 * i] = static_cast<float>(geom.atoms[i].x);
+        pos[3 * i + 1] = static_cast<float>(geom.atoms[i].y);
+        pos[3 * i + 2] = static_cast<float>(geom.atoms[i].z);
+        pot[i] = geom.atoms[i].iph;
+    }
+    std::vector<float> cpos(pos.size());
+    std::vector<int32_t> cpot(nat_all);
+    int32_t nat = 0, central = 0, untruncated = 0;
+    check_status(refeff_fms_yprep_cluster(nat_all, pos.data(), pot.data(), 0, radius, nat_all, cpos.data(),
+                                          cpot.data(), &nat, &central, &untruncated),
+                 "failed to select FMS cluster from geom.dat");
+    if (in.do_fms != 0 && nat <= max_pot)
+        stage_fail(REFEFF_FMS_MISSING_REPRESENTATIVE, "FMS cluster has " + std::to_string(nat) +
+                                                          " atom(s), but full-potential generation needs representatives through potential " +
+                                                          std::to_string(max_pot));
+    std::vector<int32_t> reps(npot);
+    check_status(refeff_fms_sort_representative_atoms(0, max_pot, nat, cpos.data(), cpot.data(), reps.data()),
+                 "failed to prepare SCREEN FMS representative atoms from geom.dat");
+    const int absorber = cpot[0];
+    if (absorber < 0 || absorber > max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS absorber potential must be within 0.." + std::to_string(max_pot));
+    check_provided_debye(in);
+    DebyeMeta meta;
+    meta.rnrmav = a.rnrmav;
+    meta.table = "the SCREEN potential table";
+    if (a.z) meta.z.assign(a.z, a.z + npot);
+    else if (in.idwopt == 0 || in.idwopt == 3)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS Debye damping requires each potential's atomic number");
+    const std::vector<float> sig = pair_sigsqr(dir, in, meta, nat, cpos, cpot);
+
+    refeff_fms_plan_desc d{};
+    d.lfms = 0;
+    d.minv = 0;
+    d.spin_channels = 1;
+    d.atom_count = nat;
+    d.positions = cpos.data();
+    d.potentials = cpot.data();
+    d.max_potential = max_pot;
+    d.global_lmax = L;
+    std::vector<int32_t> raw(in.lmaxph.begin(), in.lmaxph.begin() + npot);
+    d.raw_potential_lmax = raw.data();
+    d.direct_cutoff = rdirec;
+    d.mean_square_displacements = sig.data();
+    d.convergence_tolerance = static_cast<float>(in.toler1);
+    d.zero_tolerance = static_cast<float>(in.toler2);
+    d.solver = solver;
+    const int ch = (L + 1) * (L + 1);
+    std::fill(a.traces, a.traces + 2 * static_cast<size_t>(a.ne) * a.angular, 0.0);
+    if (a.ne == 0) {
+        *potential_index = absorber;
+        *produced = 1;
+        return REFEFF_FMS_OK;
+    }
+    std::vector<double> k(2 * static_cast<size_t>(a.ne));
+    for (int e = 0; e < a.ne; ++e) {
+        const cplx w = cplx(a.k_bohr[2 * e], a.k_bohr[2 * e + 1]) / 0.529177249;
+        k[2 * e] = w.real();
+        k[2 * e + 1] = w.imag();
+    }
+    const std::vector<double> ph = signed_l_phases(a.ne, a.nphl, npot, L, raw.data(), a.phases);
+    refeff_fms_plan* plan = nullptr;
+    check_status(refeff_fms_plan_create(&d, &plan), "failed to prepare SCREEN FMS real-space plan");
+    refeff_fms_plan_info info{};
+    int rc = refeff_fms_plan_query(plan, &info);
+    if (rc == REFEFF_FMS_OK && info.channel_count != ch) {
+        refeff_fms_plan_destroy(plan);
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "SCREEN FMS plan has " + std::to_string(info.channel_count) +
+                                                    " channel(s), expected " + std::to_string(ch));
+    }
+    std::vector<double> sc(2 * static_cast<size_t>(a.ne) * npot * ch * ch, 0.0);
+    std::vector<int32_t> status(a.ne, 0);
+    if (rc == REFEFF_FMS_OK)
+        rc = refeff_fms_solve_batch(plan, a.ne, k.data(), ph.data(), sc.data(), status.data(), nullptr);
+    refeff_fms_plan_destroy(plan);
+    for (int e = 0; e < a.ne; ++e)
+        if (status[e]) stage_fail(status[e], "failed to solve SCREEN FMS energy section " + std::to_string(e + 1));
+    check_status(rc, "failed to solve the SCREEN FMS energy sweep");
+    for (int e = 0; e < a.ne; ++e) {
+        const double* blk = sc.data() + 2 * (static_cast<size_t>(e) * npot + absorber) * ch * ch;
+        for (int l = 0; l < a.angular; ++l) {
+            const size_t po = (static_cast<size_t>(e) * a.nphl + l) * npot + absorber;
+            const cplx v = cluster_green_trace(blk, ch, cplx(a.phases[2 * po], a.phases[2 * po + 1]), l);
+            a.traces[2 * (static_cast<size_t>(e) * a.angular + l)] = v.real();
+            a.traces[2 * (static_cast<size_t>(e) * a.angular + l) + 1] = v.imag();
+        }
+    }
+    *potential_index = absorber;
+    *produced = 1;
     return REFEFF_FMS_OK;
 }
 
@@ -982,6 +1173,20 @@ int refeff_fms_pot_scf_traces(const char* work_dir, int32_t solver, int32_t lfms
     const PotScfArgs a{lfms1, rfms1, nph, lmaxsc, energy_count, energies, reference_count, reference_energies,
                        phase_l_count, phase_shifts, angular_count, traces};
     const int rc = guarded([&] { return run_pot_scf(work_dir, solver, a); });
+    put_message(message, message_len, rc ? g_last_error : std::string());
+    return rc;
+}
+
+int refeff_fms_screen_traces(const char* work_dir, int32_t solver, int32_t potential_count,
+                             const int32_t* atomic_numbers, double average_norman_radius, int32_t energy_count,
+                             const double* wave_numbers_bohr, int32_t phase_l_count, const double* phase_shifts,
+                             int32_t angular_count, double cluster_radius, double direct_cutoff, double* traces,
+                             int32_t* potential_index, int32_t* produced, char* message, size_t message_len) {
+    if (!work_dir || !traces || !potential_index || !produced || (energy_count > 0 && (!wave_numbers_bohr || !phase_shifts)))
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    const ScreenArgs a{potential_count, atomic_numbers, average_norman_radius, energy_count, wave_numbers_bohr,
+                       phase_l_count, phase_shifts, angular_count, cluster_radius, direct_cutoff, traces};
+    const int rc = guarded([&] { return run_screen(work_dir, solver, a, potential_index, produced); });
     put_message(message, message_len, rc ? g_last_error : std::string());
     return rc;
 }

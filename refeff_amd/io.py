@@ -91,6 +91,8 @@ def _L():
     L.refeff_ldos_gtr_run_in_dir.argtypes = [C.c_char_p, i32, i32p, C.c_char_p, sz]
     L.refeff_fms_pot_scf_traces.argtypes = [C.c_char_p, i32, i32, C.c_double, i32, i32p, i32, vp, i32, vp, i32, vp,
                                             i32, vp, C.c_char_p, sz]
+    L.refeff_fms_screen_traces.argtypes = [C.c_char_p, i32, i32, i32p, C.c_double, i32, vp, i32, vp, i32,
+                                           C.c_double, C.c_double, vp, i32p, i32p, C.c_char_p, sz]
     _bound = True
     return L
 
@@ -589,3 +591,39 @@ def pot_scf_fms_traces(work_dir: str, lfms1: int, rfms1: float, lmaxsc, energies
     if rc != 0:
         raise CodecError(rc, "pot_scf_fms_traces", msg.value.decode(errors="replace"))
     return out[:ne, :angular_count]
+
+
+def screen_fms_traces(work_dir: str, wave_numbers_bohr, phase_shifts, angular_count: int, cluster_radius: float,
+                      direct_cutoff: float, atomic_numbers=None, average_norman_radius: float = 0.0,
+                      solver: int = 0):
+    """SCREEN's FMS cluster Green's function traces of the absorbing potential,
+    solved on the GPU over fms.inp and geom.dat in work_dir (refeff
+    build_screen_fms_source_grid_handoff, engine fms.rs:2436-2648).
+    wave_numbers_bohr: [ne] complex; phase_shifts: [ne, nl, npot] complex by l;
+    atomic_numbers / average_norman_radius: the Debye-model metadata.  Returns
+    None when fms.inp or geom.dat is missing, else (traces complex128
+    [ne, angular], absorber potential index)."""
+    k = np.ascontiguousarray(wave_numbers_bohr, dtype=np.complex128).reshape(-1)
+    ne = k.shape[0]
+    ph = np.ascontiguousarray(phase_shifts, dtype=np.complex128)
+    if ph.ndim != 3 or ph.shape[0] < ne:
+        raise CodecError(1, "screen_fms_traces", f"phase table shape {ph.shape} does not fit {ne} energies")
+    ph = np.ascontiguousarray(ph[:ne])
+    npot = ph.shape[2]
+    z = None if atomic_numbers is None else np.ascontiguousarray(atomic_numbers, dtype=np.int32)
+    if z is not None and z.shape != (npot,):
+        raise CodecError(1, "screen_fms_traces", f"{z.shape} atomic numbers for {npot} potentials")
+    out = np.zeros((max(1, ne), max(1, angular_count)), np.complex128)
+    idx, produced = C.c_int32(), C.c_int32()
+    msg = C.create_string_buffer(1024)
+    rc = _L().refeff_fms_screen_traces(str(work_dir).encode(), solver, npot,
+                                       None if z is None else z.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       float(average_norman_radius), ne, k.ctypes.data, ph.shape[1], ph.ctypes.data,
+                                       int(angular_count), float(cluster_radius), float(direct_cutoff),
+                                       out.ctypes.data, C.byref(idx), C.byref(produced), msg, len(msg))
+    if rc != 0:
+        raise CodecError(rc, "screen_fms_traces", msg.value.decode(errors="replace"))
+    if not produced.value:
+        return None
+    return out[:ne, :angular_count], idx.value
+

@@ -197,3 +197,44 @@ def test_pot_scf_traces_host_paths(tmp_path):
         io.pot_scf_fms_traces(tmp_path, 0, 3.0, [1, 1], e, refs, ph, 2)
     assert "does not match maximum potential 1" in str(err.value)
 
+
+def test_screen_traces_host_paths(tmp_path):
+    """SCREEN's FMS traces up to the solve (engine fms.rs:2464-2540): no fms.inp or
+    geom.dat produces nothing (refeff's None); the shape, lmaxph and Debye-metadata
+    checks fail before any device work."""
+    from refeff_amd import io
+    k = np.array([1.0 + 0.02j, 1.2 + 0.02j])
+    ph = np.full((2, 4, 2), 0.1 + 0.01j)
+    assert io.screen_fms_traces(tmp_path, k, ph, 2, 4.0, 8.0) is None
+    _pot_scf_geom(tmp_path, [[0.0, 0.0, 0.0], [1.4, 0.0, 0.0]], [0, 1], 1)
+    assert io.screen_fms_traces(tmp_path, k, ph, 2, 4.0, 8.0) is None  # still no fms.inp
+
+    def fms_inp(**kw):
+        a = dict(mfms=1, idwopt=-1, minv=0, rfms2=4.0, rdirec=8.0, toler1=0.001, toler2=0.001, tk=0.0, thetad=0.0,
+                 sig2g=0.0, lmaxph=[3, 3], decomposition_channels=-1, save_gg_slice=False, do_fms=1)
+        a.update(kw)
+        (tmp_path / "fms.inp").write_text(io.fms_input_string(io.FmsInput(**a)))
+
+    cases = [(dict(), dict(angular=5), "screen.inp maxl requests 5"),
+             (dict(), dict(ph=ph[:, :3]), "phase table has 3 angular"),
+             (dict(lmaxph=[3, -1]), dict(), "lmaxph(1) must be nonnegative"),
+             (dict(idwopt=0, tk=300.0, thetad=315.0), dict(), "atomic number"),
+             (dict(idwopt=0, tk=300.0, thetad=315.0), dict(z=[29, 0], rs=2.6), "positive atomic number for potential 1"),
+             (dict(idwopt=3, tk=300.0, thetad=315.0), dict(z=[29, 29], rs=0.0), "average Norman radius"),
+             (dict(idwopt=1), dict(), "idwopt=1"),
+             (dict(), dict(radius=-1.0), "nonnegative cluster radius"),
+             (dict(), dict(radius=1.0), "representatives through potential 1")]
+    for inp, kw, text in cases:
+        fms_inp(**inp)
+        a = dict(angular=2, ph=ph, radius=4.0, z=None, rs=0.0)
+        a.update(kw)
+        with pytest.raises(io.CodecError) as err:
+            io.screen_fms_traces(tmp_path, k, a["ph"], a["angular"], a["radius"], 8.0, atomic_numbers=a["z"],
+                                 average_norman_radius=a["rs"])
+        assert text in str(err.value), (inp, kw, str(err.value))
+    fms_inp()
+    _pot_scf_geom(tmp_path, [[0.0, 0.0, 0.0], [1.4, 0.0, 0.0]], [0, 0], 0)
+    with pytest.raises(io.CodecError) as err:
+        io.screen_fms_traces(tmp_path, k, ph, 2, 4.0, 8.0)
+    assert "does not match maximum potential 1 for SCREEN" in str(err.value)
+

@@ -385,3 +385,60 @@ def test_pot_scf_traces_match_oracle(tmp_path, lfms1, lmaxsc, angular):
             assert err <= 1e-8, (ip, x, err)
     banner("  traces match the oracle to 1e-8")
 
+
+def make_screen_case(d, ne=6, lmaxph=(3, 3), idwopt=-1, tk=0.0, thetad=0.0, sig2g=0.003):
+    """fms.inp and geom.dat for SCREEN's FMS traces, with the energy grid's wave
+    numbers (1/bohr) and an l-indexed phase table as refeff's SCREEN loop hands them
+    over (engine fms.rs:70-77)."""
+    pts = W.lattice(W.FCC, 3.615, 8.0)
+    pts = pts[np.argsort(np.linalg.norm(pts, axis=1), kind="stable")]
+    iph = np.ones(len(pts), np.int32)
+    iph[0] = 0
+    g = io.GeomDat(len(pts), 1, [1, 2], np.arange(1, len(pts) + 1), pts, iph, np.ones(len(pts), np.int32))
+    (d / "geom.dat").write_text(io.geom_dat_string(g))
+    fin = io.FmsInput(mfms=1, idwopt=idwopt, minv=2, rfms2=-1.0, rdirec=-1.0, toler1=0.001, toler2=0.001, tk=tk,
+                      thetad=thetad, sig2g=sig2g, lmaxph=list(lmaxph), decomposition_channels=-1,
+                      save_gg_slice=False, do_fms=1)
+    (d / "fms.inp").write_text(io.fms_input_string(fin))
+    e = np.linspace(0.25, 1.4, ne) + 0.03j
+    k_bohr = np.sqrt(2.0 * e)
+    L = max(lmaxph)
+    phl = np.ascontiguousarray(W.synthetic_phases(k_bohr / A0, L, np.array(lmaxph))[:, 0, L:, :])
+    return pts, iph, k_bohr, phl, sig2g
+
+
+@pytest.mark.parametrize("lmaxph,idwopt", [((3, 3), -1), ((3, 2), -1), ((3, 3), 0)])
+def test_screen_traces_match_oracle(tmp_path, lmaxph, idwopt):
+    """SCREEN's FMS traces (engine fms.rs:2464-2648): the caller's radius and
+    cutoff replace fms.inp's, fms.inp's damping and tolerances stay, minv is
+    ignored (LU), lfms 0 solves the absorber's block and its trace is returned."""
+    banner(f"SCREEN traces lmaxph={lmaxph} idwopt={idwopt}")
+    tk, thetad, rnrmav, z = (300.0, 315.0, 2.6, [29, 29]) if idwopt == 0 else (0.0, 0.0, 0.0, None)
+    pts, iph, k_bohr, phl, sig2g = make_screen_case(tmp_path, lmaxph=lmaxph, idwopt=idwopt, tk=tk, thetad=thetad)
+    radius, rdirec = 5.2, 9.0
+    got, ip = io.screen_fms_traces(tmp_path, k_bohr, phl, 4, radius, rdirec, atomic_numbers=z,
+                                   average_norman_radius=rnrmav)
+    assert ip == 0 and got.shape == (len(k_bohr), 4)
+    pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(radius), len(iph))
+    pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
+    nat = len(pot)
+    if idwopt == 0:
+        sig = orc.debye_sigsqr(0, tk, thetad, rnrmav, pos, np.array(z, np.int32)[pot], np.float32(sig2g))
+    else:
+        sig = np.full((nat, nat), np.float32(sig2g), np.float32)
+        np.fill_diagonal(sig, 0.0)
+    L = max(lmaxph)
+    signed = np.zeros((len(k_bohr), 1, 2 * L + 1, 2), np.complex128)
+    for p in range(2):
+        for sl in range(-lmaxph[p], lmaxph[p] + 1):
+            signed[:, 0, L + sl, p] = phl[:, abs(sl), p]
+    plan = orc.Plan(pos, pot, nsp=1, lfms=0, max_pot=1, global_lmax=L, raw_lmax=list(lmaxph), rdirec=rdirec,
+                    sigsqr=sig, toler1=0.001, toler2=0.001)
+    sc, st = plan.spectrum((k_bohr / A0).reshape(-1, 1), signed, threads=8)
+    assert not st.any()
+    want = orc.pot_scf_trace_grid(sc, phl, 4)[:, :, 0]
+    for x in range(len(k_bohr)):
+        err = np.abs(got[x] - want[x]).max() / np.abs(want[x]).max()
+        assert err <= 1e-8, (x, err)
+    banner(f"  absorber traces match the oracle to 1e-8 ({nat} atoms)")
+
