@@ -300,6 +300,26 @@ int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_w
  * traces return REFEFF_FMS_UNSUPPORTED. */
 int refeff_mkgtr_run_in_dir(const char* work_dir, int32_t* files_written, char* message, size_t message_len);
 
+/* LDOS gtrNN.bin on the RHORRP energy grid that the LDOS stage hands over.
+ * Replaces write_ldos_gtr_bin_source_grid_handoff and
+ * has_supported_ldos_gtr_bin_source_grid_handoff,
+ * crates/refeff-engine/src/fms.rs:1632-1687 (with :1759-1906, 2049-2093,
+ * 2254-2434).  Reads ldos.inp, fms.inp, phase.bin, geom.dat, global.inp and an
+ * optional .dimensions.dat from work_dir; for every potential c solves the
+ * cluster around c's first atom under the ldos.inp controls with
+ * k = wave_numbers_bohr[e][c] / a0 and writes gtrNN.bin (FEFF fmsdos trace).
+ * ldos.inp's lmaxph is capped by phase_l_count - 1 and .dimensions.dat's lx.
+ *   wave_numbers_bohr: [energy_count][wave_number_columns] complex, 1/bohr
+ *   phase_shifts: [energy_count][phase_l_count][phase_potential_count] complex, by l
+ * *supported: 1 where refeff produces the files (0: another LDOS path, nothing
+ * written); dry_run != 0 stops there.  *files_written: gtrNN.bin files written.
+ * rfms2 <= 0 writes zero traces without a solve. */
+int refeff_ldos_gtr_source_grid_run_in_dir(const char* work_dir, int32_t solver, int32_t energy_count,
+                                           const double* wave_numbers_bohr, int32_t wave_number_columns,
+                                           int32_t phase_l_count, int32_t phase_potential_count,
+                                           const double* phase_shifts, int32_t dry_run, int32_t* supported,
+                                           int32_t* files_written, char* message, size_t message_len);
+
 /* POT self-consistency FMS traces gtr(l, iph) on the POT energy grid.  Replaces
  * the real-space branch of build_pot_scf_fms_source_grid_handoff,
  * crates/refeff-engine/src/fms.rs:2650-2965 (with :3247-3275, 5118-5175 and

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -275,6 +276,50 @@ struct FmsStageResult {
 // numbers and phase shifts of phase.bin kept in FP64 (fms_wave_numbers
 // :4972-4985, fms_phase_shifts_for_energy :5002-5054), then the plan and the
 // energy sweep on the GPU.  sc: [e][ip] ch x ch blocks, gg(row, col) at row + ch col.
+// screen_fms_cluster_green_trace (refeff-core screen/response.rs:545-581): the
+// l-shell diagonal of one potential's gg block [ch][ch] (complex pairs),
+// times exp(2 i delta_l) / (2 l + 1).
+cplx cluster_green_trace(const double* blk, int ch, cplx delta, int l) {
+    cplx sum(0, 0);
+    for (int st = l * l; st < (l + 1) * (l + 1); ++st) {
+        const size_t o = static_cast<size_t>(st) * (ch + 1);
+        sum += cplx(blk[2 * o], blk[2 * o + 1]);
+    }
+    return sum * std::exp(cplx(0.0, 2.0) * delta) / static_cast<double>(2 * l + 1);
+}
+
+// The l-indexed phase table [ne][nphl][npot] spread over signed l up to each
+// potential's lmax (engine fms.rs:5056-5116, 5118-5175): [ne][2L+1][npot].
+// src_npot: the table's potential stride (at least npot).
+std::vector<double> signed_l_phases(int ne, int nphl, int npot, int L, const int32_t* lmax, const double* phases,
+                                    int src_npot = -1) {
+    if (src_npot < 0) src_npot = npot;
+    const int nl = 2 * L + 1;
+    std::vector<double> ph(2 * static_cast<size_t>(ne) * nl * npot, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int ip = 0; ip < npot; ++ip) {
+            const int eff = std::min(lmax[ip], L);
+            for (int sl = -eff; sl <= eff; ++sl) {
+                const size_t src = (static_cast<size_t>(e) * nphl + std::abs(sl)) * src_npot + ip;
+                const size_t dst = (static_cast<size_t>(e) * nl + (sl + L)) * npot + ip;
+                ph[2 * dst] = phases[2 * src];
+                ph[2 * dst + 1] = phases[2 * src + 1];
+            }
+        }
+    return ph;
+}
+
+// An energy grid handed over by the caller in place of phase.bin's (the LDOS
+// RHORRP grid, engine fms.rs:1759-1906): wave numbers in 1/bohr per potential,
+// phase shifts indexed by l.
+struct SourceGrid {
+    int ne = 0;
+    const double* k_bohr = nullptr;  // [ne][nk] complex
+    int nk = 0;
+    const double* phases = nullptr;  // [ne][nphl][nphp] complex
+    int nphl = 0, nphp = 0;
+};
+
 struct ClusterSolve {
     int ne = 0, nsp = 1, npot = 0, ch = 0, L = 0, nat = 0, absorber = 0;
     size_t nst = 0;
@@ -283,14 +328,14 @@ struct ClusterSolve {
 
 ClusterSolve solve_cluster(const std::string& dir, const rio::FmsInp& in, const rio::PhaseBin& phase,
                            const rio::GlobalInp& global, const rio::GeomDat& geom, int solver, int central_pot,
-                           bool sortat, bool full_request, const std::string& what) {
+                           bool sortat, bool full_request, const std::string& what, const SourceGrid* grid = nullptr) {
     ClusterSolve cs;
     const int npot = static_cast<int>(phase.potentials.size());
     const int max_pot = npot - 1;
     int global_lmax = 0;
     for (int ip = 0; ip <= max_pot; ++ip) {
         if (in.lmaxph[ip] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp lmaxph must be nonnegative");
-        if (in.lmaxph[ip] > phase.potentials[ip].lmax)
+        if (!grid && in.lmaxph[ip] > phase.potentials[ip].lmax)
             stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "fms.inp lmaxph exceeds the phase.bin lmax of potential " + std::to_string(ip));
         global_lmax = std::max(global_lmax, in.lmaxph[ip]);
     }
@@ -348,17 +393,27 @@ ClusterSolve solve_cluster(const std::string& dir, const rio::FmsInp& in, const 
     refeff_fms_plan_info info{};
     int rc = refeff_fms_plan_query(plan, &info);
 
-    const int ne = phase.energy_count, nsp = phase.spin_count, L = global_lmax;
+    const int ne = grid ? grid->ne : phase.energy_count, nsp = grid ? 1 : phase.spin_count, L = global_lmax;
     const int nl = 2 * L + 1;
     std::vector<double> k(2 * static_cast<size_t>(ne) * nsp);
     cs.ph.assign(2 * static_cast<size_t>(ne) * nsp * nl * npot, 0.0);
-    for (int e = 0; e < ne; ++e)
+    for (int e = 0; grid && e < ne; ++e) {  // ldos_source_fms_wave_numbers (engine fms.rs:4987-5000)
+        const size_t o = static_cast<size_t>(e) * grid->nk + central_pot;
+        const cplx w = cplx(grid->k_bohr[2 * o], grid->k_bohr[2 * o + 1]) / 0.529177249;
+        k[2 * e] = w.real();
+        k[2 * e + 1] = w.imag();
+    }
+    if (grid) {
+        std::vector<int32_t> lm(in.lmaxph.begin(), in.lmaxph.begin() + npot);
+        cs.ph = signed_l_phases(ne, grid->nphl, npot, L, lm.data(), grid->phases, grid->nphp);
+    }
+    for (int e = 0; !grid && e < ne; ++e)
         for (int s = 0; s < nsp; ++s) {
             const cplx w = std::sqrt(cplx(2.0, 0.0) * (phase.energy_grid[e] - phase.reference_energy[e * nsp + s])) / 0.529177249;
             k[2 * (e * nsp + s)] = w.real();
             k[2 * (e * nsp + s) + 1] = w.imag();
         }
-    for (int ip = 0; ip < npot; ++ip) {
+    for (int ip = 0; !grid && ip < npot; ++ip) {
         const rio::PhasePotential& pp = phase.potentials[ip];
         const int eff = std::min(in.lmaxph[ip], L), pnl = 2 * pp.lmax + 1;
         for (int e = 0; e < ne; ++e)
@@ -593,6 +648,54 @@ int run_fms(const std::string& dir, int solver, int32_t* files_written) {
 // (spin, non-fmsdos LDOS, another energy grid, missing optional inputs) writes
 // no file; a gtrNN.bin already matching to 2e-6 + 1e-5 |x| is kept
 // (gtr_bin_matches_source_output, :1592-1628).
+// FEFF fmsdos over central potential c's block (refeff ldos_fmsdos_trace_grid,
+// core density/ldos.rs:403-484, with the other potentials zeroed), packaged as
+// gtrNN.bin (refeff-io gtr_bin.rs:200-217, fms mode 2).
+rio::GtrBin fmsdos_gtr(const ClusterSolve& cs, int c, int max_pot, int ne1, int ne3) {
+    const int ne = cs.ne, L = cs.L, ch = cs.ch, nl = 2 * L + 1, npot = max_pot + 1;
+    rio::GtrBin g;
+    g.ne = ne;
+    g.ne1 = ne1;
+    g.ne3 = ne3;
+    g.nph = max_pot;
+    g.ifms = 2;
+    g.angular_count = L + 1;
+    g.values.assign(static_cast<size_t>(ne) * npot * (L + 1), cplx(0, 0));
+    for (int e = 0; e < ne; ++e) {
+        const double* blk = cs.sc.data() + 2 * (static_cast<size_t>(e) * npot + c) * ch * ch;
+        for (int l = 0; l <= L; ++l) {
+            const size_t po = (static_cast<size_t>(e) * 1 * nl + (L + l)) * npot + c;  // spin 0, slot L + l
+            g.values[(static_cast<size_t>(e) * npot + c) * (L + 1) + l] =
+                cluster_green_trace(blk, ch, cplx(cs.ph[2 * po], cs.ph[2 * po + 1]), l);
+        }
+    }
+    return g;
+}
+
+// .dimensions.dat's lx (refeff-io structure_output.rs:428-441: the first
+// non-empty line holds nclusx, lx, nphu, nspu).  Returns false when absent.
+bool dimensions_lx(const std::string& dir, int* lx) {
+    const std::string path = join(dir, ".dimensions.dat");
+    if (!rio::file_exists(path)) return false;
+    std::istringstream text(rio::read_file(path));
+    std::string line;
+    while (std::getline(text, line) && line.find_first_not_of(" \t\r") == std::string::npos) {
+    }
+    std::istringstream f(line);
+    long v[4];
+    for (long& x : v) {
+        std::string tok;
+        char* stop = nullptr;
+        if (!(f >> tok) || (x = std::strtol(tok.c_str(), &stop, 10), *stop != '\0'))
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "failed to parse " + path + ": .dimensions.dat requires 4 integer fields");
+    }
+    if (v[1] < 0)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, path + " lx must be nonnegative for LDOS FMS source-grid generation, got " +
+                                                    std::to_string(v[1]));
+    *lx = static_cast<int>(std::min<long>(v[1], 1 << 20));
+    return true;
+}
+
 int run_ldos_gtr(const std::string& dir, int solver, int32_t* files_written) {
     if (files_written) *files_written = 0;
     const std::string ldos_path = join(dir, "ldos.inp");
@@ -649,29 +752,7 @@ int run_ldos_gtr(const std::string& dir, int solver, int32_t* files_written) {
     for (int c = 0; c <= max_pot; ++c) {
         const ClusterSolve cs = solve_cluster(dir, in, phase, global, geom, solver, c, false, false,
                                               "LDOS FMS central potential " + std::to_string(c));
-        const int ne = cs.ne, L = cs.L, ch = cs.ch, nl = 2 * L + 1;
-        rio::GtrBin g;
-        g.ne = ne;
-        g.ne1 = phase.main_energy_count;
-        g.ne3 = phase.auxiliary_energy_count;
-        g.nph = max_pot;
-        g.ifms = 2;
-        g.angular_count = L + 1;
-        g.values.assign(static_cast<size_t>(ne) * npot * (L + 1), cplx(0, 0));
-        for (int e = 0; e < ne; ++e) {
-            const double* blk = cs.sc.data() + 2 * (static_cast<size_t>(e) * npot + c) * ch * ch;
-            for (int l = 0; l <= L; ++l) {
-                cplx sum(0, 0);
-                for (int m = 0; m <= 2 * l; ++m) {
-                    const size_t o = static_cast<size_t>(l * l + m) * (ch + 1);  // diagonal (r, r): r + ch r
-                    sum += cplx(blk[2 * o], blk[2 * o + 1]);
-                }
-                const size_t po = (static_cast<size_t>(e) * 1 * nl + (L + l)) * npot + c;  // spin 0
-                const cplx delta(cs.ph[2 * po], cs.ph[2 * po + 1]);
-                g.values[(static_cast<size_t>(e) * npot + c) * (L + 1) + l] =
-                    sum * std::exp(cplx(0.0, 2.0) * delta) / static_cast<double>(2 * l + 1);
-            }
-        }
+        const rio::GtrBin g = fmsdos_gtr(cs, c, max_pot, phase.main_energy_count, phase.auxiliary_energy_count);
         char name[32];
         std::snprintf(name, sizeof(name), "gtr%02d.bin", c);
         const std::string path = join(dir, name);
@@ -697,6 +778,120 @@ int run_ldos_gtr(const std::string& dir, int solver, int32_t* files_written) {
     return REFEFF_FMS_OK;
 }
 
+// LDOS gtrNN.bin on the RHORRP energy grid the LDOS stage hands over: refeff
+// write_ldos_gtr_bin_source_grid_handoff / ldos_gtr_bin_source_grid_setup
+// (engine fms.rs:1632-1687, 1759-1906) with ldos_source_grid_effective_fms_input
+// (:2049-2093) and build_ldos_gtr_bin_for_source_grid_central_potential
+// (:2254-2434).  The solve per central potential is the phase.bin variant's, with
+// the caller's wave numbers (column c, 1/bohr) and l-indexed phase shifts, and
+// lmaxph capped by the phase table and .dimensions.dat.  *supported is refeff's
+// has_supported_ldos_gtr_bin_source_grid_handoff; with dry_run nothing is solved.
+struct LdosGridArgs {
+    int ne;
+    const double* k_bohr;  // [ne][nk] complex
+    int nk;
+    const double* phases;  // [ne][nphl][nphp] complex
+    int nphl, nphp;
+    bool dry_run;
+};
+
+int run_ldos_gtr_source_grid(const std::string& dir, int solver, const LdosGridArgs& a, int32_t* supported,
+                             int32_t* files_written) {
+    *supported = 0;
+    *files_written = 0;
+    const std::string ldos_path = join(dir, "ldos.inp");
+    const rio::LdosInp ldos = rio::parse_ldos_inp(rio::read_file(ldos_path), ldos_path);
+    if (!(ldos.mldos == 1 && ldos.ispin == 0 && ldos.ldostype <= 0)) return REFEFF_FMS_OK;
+    if (a.ne <= 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "LDOS FMS source grid requires at least one energy point");
+    if (!rio::file_exists(join(dir, "fms.inp")) || !rio::file_exists(join(dir, "phase.bin")) ||
+        !rio::file_exists(join(dir, "geom.dat")) || !rio::file_exists(join(dir, "global.inp")))
+        return REFEFF_FMS_OK;
+    rio::FmsInp in = rio::parse_fms_inp(rio::read_file(join(dir, "fms.inp")));
+    in.mfms = ldos.lfms2 != 0 ? 1 : 0;
+    in.minv = ldos.minv;
+    in.rfms2 = ldos.rfms2;
+    in.rdirec = ldos.rdirec;
+    in.toler1 = ldos.toler1;
+    in.toler2 = ldos.toler2;
+    in.lmaxph = ldos.lmaxph;
+    in.decomposition_channels = -1;
+    in.save_gg_slice = false;
+    in.do_fms = ldos.lfms2;
+    const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
+    if (phase.spin_count != 1) return REFEFF_FMS_OK;
+    const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
+    if (global.do_nrixs != 0 || global.ispin != 0) return REFEFF_FMS_OK;
+    const rio::GeomDat geom = rio::parse_geom_dat(rio::read_file(join(dir, "geom.dat")));
+    const int npot = static_cast<int>(phase.potentials.size());
+    if (npot < 1) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "phase.bin requires at least one potential for LDOS FMS generation");
+    const int max_pot = npot - 1;
+    if (geom.nph != max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "geom.dat nph " + std::to_string(geom.nph) +
+                                                    " does not match phase.bin maximum potential " + std::to_string(max_pot) +
+                                                    " for LDOS FMS generation");
+    if (a.nphp <= max_pot || a.nk <= max_pot) return REFEFF_FMS_OK;
+    int lx = 0;
+    const bool have_lx = dimensions_lx(dir, &lx);
+    if (a.nphl <= 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "LDOS RHORRP phase-shift table requires at least one angular channel");
+    if (static_cast<int>(in.lmaxph.size()) <= max_pot)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "ldos.inp has " + std::to_string(in.lmaxph.size()) +
+                                                    " lmaxph value(s), expected at least " + std::to_string(npot) +
+                                                    " for LDOS FMS generation");
+    for (int ip = 0; ip <= max_pot; ++ip) {
+        if (in.lmaxph[ip] < 0)
+            stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "ldos.inp lmaxph(" + std::to_string(ip) +
+                                                        ") must be nonnegative for LDOS FMS source-grid generation, got " +
+                                                        std::to_string(in.lmaxph[ip]));
+        int cap = a.nphl - 1;
+        if (have_lx) cap = std::min(cap, lx);
+        in.lmaxph[ip] = std::min(in.lmaxph[ip], cap);
+    }
+    const bool zero = std::isfinite(in.rfms2) && in.rfms2 <= 0.0;
+    if (!zero) {
+        try {
+            validate_source_controls(in);
+        } catch (const Error&) {
+            return REFEFF_FMS_OK;
+        }
+        if (in.idwopt == 4 && !rio::file_exists(join(dir, "sig2.dat"))) return REFEFF_FMS_OK;
+        if ((in.idwopt == 1 || in.idwopt == 2) && !rio::file_exists(join(dir, "spring.inp"))) return REFEFF_FMS_OK;
+        if (in.idwopt == 5 && !rio::file_exists(join(dir, "dmdw.inp"))) return REFEFF_FMS_OK;
+    }
+    *supported = 1;
+    if (a.dry_run) return REFEFF_FMS_OK;
+    if (!zero) check_provided_debye(in);
+    int L = 0;
+    for (int ip = 0; ip <= max_pot; ++ip) L = std::max(L, in.lmaxph[ip]);
+    SourceGrid grid;
+    grid.ne = a.ne;
+    grid.k_bohr = a.k_bohr;
+    grid.nk = a.nk;
+    grid.phases = a.phases;
+    grid.nphl = a.nphl;
+    grid.nphp = a.nphp;
+    for (int c = 0; c <= max_pot; ++c) {
+        rio::GtrBin g;
+        if (zero) {  // build_zero_ldos_gtr_bin_source_grid (:2254-2272)
+            g.ne = a.ne;
+            g.ne1 = a.ne;
+            g.ne3 = 0;
+            g.nph = max_pot;
+            g.ifms = 2;
+            g.angular_count = L + 1;
+            g.values.assign(static_cast<size_t>(a.ne) * npot * (L + 1), cplx(0, 0));
+        } else {
+            const ClusterSolve cs = solve_cluster(dir, in, phase, global, geom, solver, c, false, false,
+                                                  "LDOS FMS source-grid central potential " + std::to_string(c), &grid);
+            g = fmsdos_gtr(cs, c, max_pot, a.ne, 0);
+        }
+        char name[32];
+        std::snprintf(name, sizeof(name), "gtr%02d.bin", c);
+        rio::write_file(join(dir, name), rio::gtr_bin_bytes(g));
+        ++*files_written;
+    }
+    return REFEFF_FMS_OK;
+}
+
 // POT self-consistency FMS traces on the POT energy grid: the real-space branch
 // of refeff build_pot_scf_fms_source_grid_handoff (crates/refeff-engine/src/fms.rs:2650-2965)
 // with pot_scf_fms_input_from_pot (:3247-3275: rfms2 = rfms1, rdirec = 2 rfms1,
@@ -706,36 +901,6 @@ int run_ldos_gtr(const std::string& dir, int solver, int32_t* files_written) {
 // one cluster around potential 0 (sortat), every potential's block; else one
 // cluster per potential c around its first atom (yprep order), block c only.
 // Wave numbers use the highest potential's reference energy.
-// screen_fms_cluster_green_trace (refeff-core screen/response.rs:545-581): the
-// l-shell diagonal of one potential's gg block [ch][ch] (complex pairs),
-// times exp(2 i delta_l) / (2 l + 1).
-cplx cluster_green_trace(const double* blk, int ch, cplx delta, int l) {
-    cplx sum(0, 0);
-    for (int st = l * l; st < (l + 1) * (l + 1); ++st) {
-        const size_t o = static_cast<size_t>(st) * (ch + 1);
-        sum += cplx(blk[2 * o], blk[2 * o + 1]);
-    }
-    return sum * std::exp(cplx(0.0, 2.0) * delta) / static_cast<double>(2 * l + 1);
-}
-
-// The l-indexed phase table [ne][nphl][npot] spread over signed l up to each
-// potential's lmax (engine fms.rs:5056-5116, 5118-5175): [ne][2L+1][npot].
-std::vector<double> signed_l_phases(int ne, int nphl, int npot, int L, const int32_t* lmax, const double* phases) {
-    const int nl = 2 * L + 1;
-    std::vector<double> ph(2 * static_cast<size_t>(ne) * nl * npot, 0.0);
-    for (int e = 0; e < ne; ++e)
-        for (int ip = 0; ip < npot; ++ip) {
-            const int eff = std::min(lmax[ip], L);
-            for (int sl = -eff; sl <= eff; ++sl) {
-                const size_t src = (static_cast<size_t>(e) * nphl + std::abs(sl)) * npot + ip;
-                const size_t dst = (static_cast<size_t>(e) * nl + (sl + L)) * npot + ip;
-                ph[2 * dst] = phases[2 * src];
-                ph[2 * dst + 1] = phases[2 * src + 1];
-            }
-        }
-    return ph;
-}
-
 struct PotScfArgs {
     int lfms1;
     double rfms1;
@@ -1187,6 +1352,20 @@ int refeff_fms_screen_traces(const char* work_dir, int32_t solver, int32_t poten
     const ScreenArgs a{potential_count, atomic_numbers, average_norman_radius, energy_count, wave_numbers_bohr,
                        phase_l_count, phase_shifts, angular_count, cluster_radius, direct_cutoff, traces};
     const int rc = guarded([&] { return run_screen(work_dir, solver, a, potential_index, produced); });
+    put_message(message, message_len, rc ? g_last_error : std::string());
+    return rc;
+}
+
+int refeff_ldos_gtr_source_grid_run_in_dir(const char* work_dir, int32_t solver, int32_t energy_count,
+                                           const double* wave_numbers_bohr, int32_t wave_number_columns,
+                                           int32_t phase_l_count, int32_t phase_potential_count,
+                                           const double* phase_shifts, int32_t dry_run, int32_t* supported,
+                                           int32_t* files_written, char* message, size_t message_len) {
+    if (!work_dir || !supported || !files_written || (energy_count > 0 && (!wave_numbers_bohr || !phase_shifts)))
+        return REFEFF_FMS_INVALID_ARGUMENT;
+    const LdosGridArgs a{energy_count, wave_numbers_bohr, wave_number_columns, phase_shifts, phase_l_count,
+                         phase_potential_count, dry_run != 0};
+    const int rc = guarded([&] { return run_ldos_gtr_source_grid(work_dir, solver, a, supported, files_written); });
     put_message(message, message_len, rc ? g_last_error : std::string());
     return rc;
 }

@@ -91,6 +91,8 @@ def _L():
     L.refeff_ldos_gtr_run_in_dir.argtypes = [C.c_char_p, i32, i32p, C.c_char_p, sz]
     L.refeff_fms_pot_scf_traces.argtypes = [C.c_char_p, i32, i32, C.c_double, i32, i32p, i32, vp, i32, vp, i32, vp,
                                             i32, vp, C.c_char_p, sz]
+    L.refeff_ldos_gtr_source_grid_run_in_dir.argtypes = [C.c_char_p, i32, i32, vp, i32, i32, i32, vp, i32, i32p,
+                                                         i32p, C.c_char_p, sz]
     L.refeff_fms_screen_traces.argtypes = [C.c_char_p, i32, i32, i32p, C.c_double, i32, vp, i32, vp, i32,
                                            C.c_double, C.c_double, vp, i32p, i32p, C.c_char_p, sz]
     _bound = True
@@ -562,6 +564,26 @@ def run_ldos_gtr_in_dir(work_dir: str, solver: int = 0) -> int:
     if rc != 0:
         raise CodecError(rc, "run_ldos_gtr_in_dir", msg.value.decode(errors="replace"))
     return n.value
+
+
+def run_ldos_gtr_source_grid_in_dir(work_dir: str, wave_numbers_bohr, phase_shifts, solver: int = 0,
+                                    dry_run: bool = False):
+    """LDOS gtrNN.bin on the caller's RHORRP grid (engine fms.rs:1632-1687).
+    wave_numbers_bohr: [ne, >= npot] complex (column c is central potential c's);
+    phase_shifts: [ne, nl, >= npot] complex by l.  Returns (supported, files
+    written); supported False means refeff takes another LDOS path."""
+    k = np.ascontiguousarray(wave_numbers_bohr, dtype=np.complex128)
+    ph = np.ascontiguousarray(phase_shifts, dtype=np.complex128)
+    if k.ndim != 2 or ph.ndim != 3 or ph.shape[0] != k.shape[0]:
+        raise CodecError(1, "run_ldos_gtr_source_grid_in_dir",
+                         f"wave-number grid {k.shape} and phase grid {ph.shape} must share the energy rows")
+    sup, n, msg = C.c_int32(), C.c_int32(), C.create_string_buffer(1024)
+    rc = _L().refeff_ldos_gtr_source_grid_run_in_dir(str(work_dir).encode(), solver, k.shape[0], k.ctypes.data,
+                                                     k.shape[1], ph.shape[1], ph.shape[2], ph.ctypes.data,
+                                                     int(bool(dry_run)), C.byref(sup), C.byref(n), msg, len(msg))
+    if rc != 0:
+        raise CodecError(rc, "run_ldos_gtr_source_grid_in_dir", msg.value.decode(errors="replace"))
+    return bool(sup.value), n.value
 
 
 def pot_scf_fms_traces(work_dir: str, lfms1: int, rfms1: float, lmaxsc, energies, reference_energies,

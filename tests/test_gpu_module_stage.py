@@ -442,3 +442,67 @@ def test_screen_traces_match_oracle(tmp_path, lmaxph, idwopt):
         assert err <= 1e-8, (x, err)
     banner(f"  absorber traces match the oracle to 1e-8 ({nat} atoms)")
 
+
+def rhorrp_grid(ne=5, L=3):
+    """A RHORRP-style LDOS grid, not phase.bin's: per-potential wave numbers (1/bohr)
+    and an l-indexed phase table."""
+    e = np.linspace(0.3, 1.2, ne) + 0.02j
+    k_bohr = np.stack([np.sqrt(2.0 * e), np.sqrt(2.0 * (e - 0.04))], axis=1)  # [ne, 2]
+    phl = np.ascontiguousarray(W.synthetic_phases(k_bohr[:, 0] / A0, L, np.array([L, L]))[:, 0, L:, :])
+    return k_bohr, phl
+
+
+@pytest.mark.parametrize("lfms2,lx", [(0, None), (1, None), (1, 2)])
+def test_ldos_gtr_source_grid_matches_oracle(tmp_path, lfms2, lx):
+    """gtrNN.bin on the LDOS stage's own RHORRP grid (engine fms.rs:1632-1687,
+    2274-2434): central potential c's wave numbers from column c, the l-indexed
+    phases spread over signed l, ldos.inp's lmaxph capped by .dimensions.dat."""
+    banner(f"LDOS source-grid gtrNN.bin lfms2={lfms2} lx={lx}")
+    pts, iph, _, _, radius, sig2g = make_ldos_workdir(tmp_path, lfms2)
+    if lx is not None:
+        (tmp_path / ".dimensions.dat").write_text(f"{200:12d}{lx:12d}{2:12d}{1:12d}\n")
+    k_bohr, phl = rhorrp_grid()
+    assert io.run_ldos_gtr_source_grid_in_dir(tmp_path, k_bohr, phl, dry_run=True) == (True, 0)
+    assert not list(tmp_path.glob("gtr*.bin"))
+    assert io.run_ldos_gtr_source_grid_in_dir(tmp_path, k_bohr, phl) == (True, 2)
+    L = 3 if lx is None else lx
+    for c in (0, 1):
+        got, h = io.parse_gtr_bin((tmp_path / f"gtr{c:02d}.bin").read_bytes())
+        assert h == dict(ne=len(k_bohr), ne1=len(k_bohr), ne3=0, nph=1, ifms=2) and got.shape == (len(k_bohr), 2, L + 1)
+        pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, c, np.float32(radius), len(iph))
+        nat = len(pot)
+        sig = np.full((nat, nat), np.float32(sig2g), np.float32)
+        np.fill_diagonal(sig, 0.0)
+        signed = np.zeros((len(k_bohr), 1, 2 * L + 1, 2), np.complex128)
+        for sl in range(-L, L + 1):
+            signed[:, 0, L + sl, :] = phl[:, abs(sl), :]
+        plan = orc.Plan(pos, pot, nsp=1, lfms=lfms2, max_pot=1, global_lmax=L, raw_lmax=[L, L], rdirec=10.0,
+                        sigsqr=sig)
+        sc, st = plan.spectrum((k_bohr[:, c] / A0).reshape(-1, 1), signed, threads=8)
+        assert not st.any()
+        want = orc.ldos_fmsdos_trace_grid(sc, signed, L + 1)
+        want[:, 1 - c, :] = 0.0
+        assert not got[:, 1 - c, :].any()
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= 2e-6, (c, err)  # the file holds f32
+    banner("  gtr00.bin / gtr01.bin match the oracle on the RHORRP grid")
+
+
+def test_ldos_gtr_source_grid_zero_radius_and_other_paths(tmp_path):
+    """rfms2 <= 0 writes zero traces without a solve (engine fms.rs:1778-1790,
+    2254-2272); a non-fmsdos ldostype or a spin LDOS is another LDOS path."""
+    banner("LDOS source-grid zero radius / unsupported controls")
+    make_ldos_workdir(tmp_path, 1)
+    k_bohr, phl = rhorrp_grid()
+    d = io.parse_ldos_input((tmp_path / "ldos.inp").read_text())
+    (tmp_path / "ldos.inp").write_text(io.ldos_input_string({**d, "rfms2": 0.0}))
+    assert io.run_ldos_gtr_source_grid_in_dir(tmp_path, k_bohr, phl) == (True, 2)
+    for c in (0, 1):
+        got, h = io.parse_gtr_bin((tmp_path / f"gtr{c:02d}.bin").read_bytes())
+        assert got.shape == (len(k_bohr), 2, 4) and not got.any()
+    for bad in ({"ldostype": 1}, {"ispin": 1}, {"mldos": 0}):
+        (tmp_path / "ldos.inp").write_text(io.ldos_input_string({**d, **bad}))
+        assert io.run_ldos_gtr_source_grid_in_dir(tmp_path, k_bohr, phl) == (False, 0)
+    (tmp_path / "ldos.inp").write_text(io.ldos_input_string(d))
+    assert io.run_ldos_gtr_source_grid_in_dir(tmp_path, k_bohr[:, :1], phl) == (False, 0)  # one k column
+
