@@ -53,10 +53,22 @@ def parse():
 GEMM_CLASS = 4  # REFEFF_FMS_K_GEMM: the LU trailing update
 
 
+def mkgtr_inputs(wl, ne):
+    """K-edge orientation-averaged bmat and seeded synthetic radial matrix elements
+    for the MKGTR trace gtr(ie), the FMS part of xmu (refeff ff2x.rs:1306-1333)."""
+    from refeff_amd import fms
+    lind, bmat = fms.transition_b_matrix_averaged(1, 0, wl.global_lmax, 0, 1)
+    rng = np.random.default_rng(17)
+    rkk = (rng.normal(size=(ne, 8, 1)) + 1j * rng.normal(size=(ne, 8, 1))) * 0.3
+    return lind, bmat, rkk
+
+
 def run_solver(lib, wl, solver_id, steps, warmup, profile):
-    """Times `steps` sweeps.  With profile=True the timed region brackets only the
-    trailing-update launches with HIP events (the roofline kernel), and one
-    extra, untimed sweep records every kernel class for the breakdown."""
+    """Times `steps` sweeps.  A sweep is the solve of every energy with inputs in
+    HBM, the absorber's gg copied to the host and its MKGTR trace gtr(ie).  With
+    profile=True HIP events also bracket the trailing-update launches (the
+    roofline kernel), and one extra, untimed sweep records every kernel class
+    for the breakdown."""
     from refeff_amd import fms
     from refeff_amd._lib import Profile, Stats, check
     from refeff_amd.hip import DeviceArray, synchronize
@@ -73,11 +85,17 @@ def run_solver(lib, wl, solver_id, steps, warmup, profile):
     status = np.zeros(ne, np.int32)
     st = Stats()
     i32p = status.ctypes.data_as(C.POINTER(C.c_int32))
+    absorber = int(wl.potentials[0])
+    lind, bmat, rkk = mkgtr_inputs(wl, ne)
+    host = np.zeros((ne, npot, ch, ch), np.complex128)  # (col, row) per block
+    gtr = np.zeros(ne, np.complex128)
 
     def step():
         rc = lib.refeff_fms_solve_batch_device(plan.handle, ne, dk.ptr, dph.ptr, dout.ptr, i32p, C.byref(st), None)
         if rc != 0 and not status.any():
             check(rc, "solve_batch_device")
+        dout.download(host)
+        gtr[:] = fms.mkgtr_green_trace(host[:, absorber].transpose(0, 2, 1), rkk, lind, bmat, wl.global_lmax, 1)
 
     for _ in range(warmup):
         step()
@@ -103,7 +121,7 @@ def run_solver(lib, wl, solver_id, steps, warmup, profile):
     out = np.zeros((ne, npot, ch, ch), np.complex128)
     dout.download(out)
     sc = out.transpose(0, 1, 3, 2).copy()
-    res = dict(seconds=t1 - t0, ne=ne, status=status.copy(), scattering=sc, stats=st, profile=prof,
+    res = dict(seconds=t1 - t0, ne=ne, status=status.copy(), scattering=sc, gtr=gtr.copy(), stats=st, profile=prof,
                breakdown=breakdown, n=plan.info.state_count, npad=plan.info.padded_order, batch=plan.info.batch,
                lu_block=plan.info.lu_block)
     for d in (dk, dph, dout):
@@ -127,7 +145,7 @@ def cpu_baseline(wl, energies, gg_gpu):
     off = rep[int(wl.potentials[0])]
     nb = (min(wl.raw_lmax[wl.potentials[0]], wl.global_lmax) + 1) ** 2
     t_c32 = 0.0
-    errs = []
+    errs, refs = [], []
     for e in energies:
         t0 = time.perf_counter()
         g0 = plan.g0(k[e])
@@ -140,6 +158,7 @@ def cpu_baseline(wl, energies, gg_gpu):
         lu64, piv64, _ = lapack.zgetrf(a)
         x64, _ = lapack.zgetrs(lu64, piv64, rhs)
         ref = x64[off:off + nb, :]
+        refs.append(ref)
         got = gg_gpu[e, int(wl.potentials[0]), :nb, :nb]
         errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
         if info or info2:
@@ -147,7 +166,7 @@ def cpu_baseline(wl, energies, gg_gpu):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": len(energies) / t_c32, "unit": "energy-points/s", "cores": cores, "kind": "port",
             "sample": f"{len(energies)} of {len(k)} energies; oracle FP64 G0/t assembly (1 thread) + "
-                      f"LAPACK cgetrf/cgetrs complex64 (OpenBLAS, {cores} threads)"}, errs
+                      f"LAPACK cgetrf/cgetrs complex64 (OpenBLAS, {cores} threads)"}, errs, refs
 
 
 def pot_scf_case(d, rfms1, ne, lmaxsc):
@@ -370,10 +389,19 @@ def main():
                            "max_gg_rel_diff_mixed_vs_fp64": max(d)}
     if not args.no_cpu:
         sample = sorted(set(np.linspace(0, ne - 1, args.cpu_energies).astype(int).tolist()))
-        cb, errs = cpu_baseline(wl, sample, main_res["scattering"])
+        cb, errs, refs = cpu_baseline(wl, sample, main_res["scattering"])
         line["cpu_baseline"] = cb
-        line["accuracy"] = {"max_gg_rel_err_vs_fp64_oracle": max(errs), "energies_checked": len(sample),
-                            "gate": 1e-8}
+        # the xmu proxy: the FMS part of xmu is gtr * S02 (refeff ff2x.rs:1306-1333)
+        from oracle import oracle as orc
+        lind, bmat, rkk = mkgtr_inputs(wl, ne)
+        nb = refs[0].shape[0]
+        want = orc.mkgtr_trace(np.array(refs), rkk[sample], lind, bmat, wl.global_lmax, 1) if nb == (
+            wl.global_lmax + 1) ** 2 else None
+        gtr_err = (float(np.max(np.abs(main_res["gtr"][sample] - want) / np.abs(want)))
+                   if want is not None else None)
+        line["accuracy"] = {"max_gg_rel_err_vs_fp64_oracle": max(errs),
+                            "max_gtr_rel_err_vs_fp64_oracle": gtr_err, "energies_checked": len(sample),
+                            "gate": {"gg": 1e-8, "gtr (xmu proxy)": 1e-6}}
     print(json.dumps(line), flush=True)
 
 
