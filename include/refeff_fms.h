@@ -267,6 +267,16 @@ int refeff_fms_plan_profile_read(refeff_fms_plan* plan, refeff_fms_profile* out)
 int refeff_fms_transition_bmat_averaged(int32_t ihole, int32_t multipole, int32_t lmax, int32_t spin,
                                         int32_t spin_channels, int32_t* lind, double* bmat);
 
+/* FEFF bcoef for any polarization: transition_b_matrix,
+ * crates/refeff-core/src/angular/transition.rs:9-482.  ipol = 0 is the
+ * averaged matrix; otherwise ptz holds the polarization tensor as 3 x 3
+ * complex (re, im pairs) with ptz[p+1][p'+1], angks is the spin-axis angle.
+ * trace_orbital folds the m-diagonal onto m = 0 within equal l and spin; spin
+ * and spin_channels select the ispin fold.  Layout of lind and bmat as above. */
+int refeff_fms_transition_bmat(int32_t ihole, int32_t multipole, int32_t lmax, int32_t ipol, const double* ptz,
+                               int32_t trace_orbital, int32_t spin, int32_t spin_channels, double angks,
+                               int32_t* lind, double* bmat);
+
 /* gtr(ie) = sum gg bmat rkk rkk.  Replaces mkgtr_green_trace,
  * crates/refeff-core/src/fms/mkgtr.rs:790-953, keeping gg in FP64.
  *   gg:  [ne][nch][nch] complex, (row, col) at row + nch*col
@@ -294,10 +304,11 @@ int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_w
                           size_t message_len);
 
 /* Reads gg.bin, fms.inp, global.inp, phase.bin and writes fms.bin and gtr.dat
- * (ordinary orientation-averaged trace, ipol = 0) and appends the MKGTR lines
- * to log3.dat.  Replaces run_mkgtr_in_dir, crates/refeff-engine/src/fms.rs:559-631
- * with build_mkgtr_ordinary_outputs (:5528-5608); EELS, NRIXS and polarised
- * traces return REFEFF_FMS_UNSUPPORTED. */
+ * (the ordinary trace: orientation averaged for ipol = 0, else with
+ * global.inp's polarization tensor and spin angle) and appends the MKGTR
+ * lines to log3.dat.  Replaces run_mkgtr_in_dir,
+ * crates/refeff-engine/src/fms.rs:559-631 with build_mkgtr_ordinary_outputs
+ * (:5528-5608); EELS and NRIXS traces return REFEFF_FMS_UNSUPPORTED. */
 int refeff_mkgtr_run_in_dir(const char* work_dir, int32_t* files_written, char* message, size_t message_len);
 
 /* LDOS gtrNN.bin on the RHORRP energy grid that the LDOS stage hands over.

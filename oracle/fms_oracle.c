@@ -1303,6 +1303,225 @@ void orc_bmat_averaged(int kappa, int multipole, int lmax, int spin, int nsp, in
 #undef B
 }
 
+/* FEFF rotwig, refeff crates/refeff-core/src/angular/wigner.rs:101-190: the
+ * Wigner small-d element d^{jj/scale}_{m1/scale, m2/scale}(beta), reduced to
+ * m1' >= |m2'| by the same symmetries and summed with log factorials. */
+static double orc_rotwig(double beta, int jj, int m1, int m2, int scale) {
+    int m1p, m2p;
+    double b = beta, sign = 1.0;
+    if (m1 >= 0 && abs(m1) >= abs(m2)) {
+        m1p = m1; m2p = m2;
+    } else if (m2 >= 0 && abs(m2) >= abs(m1)) {
+        m1p = m2; m2p = m1; b = -beta;
+    } else if (m1 <= 0 && abs(m1) >= abs(m2)) {
+        m1p = -m1; m2p = -m2;
+        sign = (((m1 - m2) / scale) % 2 == 0) ? 1.0 : -1.0;
+    } else {
+        m1p = -m2; m2p = -m1; b = -beta;
+        sign = (((m2 - m1) / scale) % 2 == 0) ? 1.0 : -1.0;
+    }
+    double zeta = cos(b / 2.0), eta = sin(b / 2.0), total = 0.0;
+    for (int k = m1p - m2p; k <= jj - m2p; k += scale) {
+        int a[8] = {(jj + m1p) / scale, (jj - m1p) / scale, (jj + m2p) / scale, (jj - m2p) / scale,
+                    (jj + m1p - k) / scale, (jj - m2p - k) / scale, k / scale, (m2p - m1p + k) / scale};
+        int zp = (2 * jj + m1p - m2p - 2 * k) / scale, ep = (2 * k - m1p + m2p) / scale;
+        double f = 0.0;
+        for (int i = 0; i < 4; ++i) f += lgamma(a[i] + 1.0) / 2.0;
+        for (int i = 4; i < 8; ++i) f -= lgamma(a[i] + 1.0);
+        double c = (((k / scale) % 2) == 0 ? 1.0 : -1.0) * exp(f);
+        total += c * pow(zeta, zp) * pow(eta, ep);
+    }
+    return sign * total;
+}
+
+/* FEFF bcoef slots, refeff angular/transition.rs:97-138: j, l and kappa of
+ * the 8 transition slots (3 dipole-like, 5 quadrupole-like). */
+static void orc_transition_slots(int kappa, int multipole, int lmax, int* jind, int* lind, int* kiind) {
+    for (int k = -1; k <= 1; ++k) {
+        int kap = kappa + k;
+        if (k == 0) kap = -kap;
+        int jk = abs(kap), lk = kap <= 0 ? abs(kap) - 1 : kap;
+        if (lk > lmax) { jk = 0; lk = -1; kap = 0; }
+        jind[k + 1] = jk; lind[k + 1] = lk; kiind[k + 1] = kap;
+    }
+    for (int k = -2; k <= 2; ++k) {
+        int jk = abs(kappa) + k;
+        if (jk <= 0) jk = 0;
+        int kap = jk;
+        if (kappa < 0 && abs(k) != 1) kap = -jk;
+        if (kappa > 0 && abs(k) == 1) kap = -jk;
+        int lk = kap <= 0 ? -kap - 1 : kap;
+        if (lk > lmax || multipole == 0 || (multipole == 1 && abs(k) == 2)) { jk = 0; lk = -1; kap = 0; }
+        jind[k + 5] = jk; lind[k + 5] = lk; kiind[k + 5] = kap;
+    }
+}
+
+/* FEFF bcoef with any polarization, refeff transition_b_matrix
+ * (angular/transition.rs:9-70): averaged (:140-167) or polarized
+ * (:169-398) matrix, the optional orbital trace (:400-429) and the ispin fold
+ * (:431-482).  ptz: [3][3] complex, ptz[p+1][p'+1]; bmat as orc_bmat_averaged;
+ * kiind may be NULL. */
+int orc_bmat_general(int kappa, int multipole, int lmax, int ipol, const double* ptz_in, int trace_orbital, int spin,
+                     int nsp, double angks, int* lind, int* kiind_out, double* bmat_out) {
+    int nm = 2 * lmax + 1, ne = nm + 1;
+    int jind[8], kiind[8];
+    orc_transition_slots(kappa, multipole, lmax, jind, lind, kiind);
+    if (kiind_out) memcpy(kiind_out, kiind, sizeof(kiind));
+    cplx* b = (cplx*)bmat_out;
+    const cplx* ptz = (const cplx*)ptz_in;
+    memset(b, 0, sizeof(cplx) * (size_t)nm * 2 * 8 * nm * 2 * 8);
+#define B(ml2, ms2, k2, ml1, ms1, k1) \
+    b[(ml2) + (size_t)nm * ((ms2) + 2 * ((k2) + 8 * ((ml1) + (size_t)nm * ((ms1) + 2 * (size_t)(k1)))))]
+#define EI(m) ((m) + lmax) /* extended magnetic index: -lmax .. lmax+1 */
+    if (ipol == 0) {
+        for (int t = 0; t < 8; ++t) {
+            int l = lind[t];
+            if (l < 0) continue;
+            double deg = (multipole == 2 && t >= 3) ? 5.0 : 3.0;
+            double v = 0.5 / (2.0 * l + 1.0) / deg;
+            if (t < 3) v = -v;
+            for (int s = 0; s <= 1; ++s)
+                for (int m = -l; m <= l; ++m) B(m + lmax, s, t, m + lmax, s, t) = v;
+        }
+    } else {
+        double* t3j = calloc((size_t)8 * 2 * ne, sizeof(double));
+        double* x3j = calloc((size_t)8 * 3 * ne, sizeof(double));
+        double* qmat = calloc((size_t)ne * nm * 2 * 8, sizeof(double));
+        cplx* pmat = calloc((size_t)ne * 8 * ne * 8, sizeof(cplx));
+        cplx* tmat = calloc((size_t)ne * 8 * nm * 2 * 8, sizeof(cplx));
+        if (!t3j || !x3j || !qmat || !pmat || !tmat) {
+            free(t3j); free(x3j); free(qmat); free(pmat); free(tmat);
+            return ORC_INVALID_ARGUMENT;
+        }
+        for (int t = 0; t < 8; ++t) {
+            int j = jind[t];
+            if (j <= 0) continue;
+            for (int mp = -j + 1; mp <= j; ++mp) {
+                for (int s = 0; s <= 1; ++s) {
+                    int j1 = 2 * lind[t], j2 = 1, j3 = 2 * j - 1, m1 = 2 * (mp - s), m2 = 2 * s - 1;
+                    double v = sqrt((double)(j3 + 1)) * orc_wigner_3j_doubled(j1, j2, j3, m1, m2);
+                    if (((j2 - j1 - m1 - m2) / 2) % 2 != 0) v = -v;
+                    t3j[(t * 2 + s) * ne + EI(mp)] = v;
+                }
+                for (int p = -1; p <= 1; ++p) {
+                    int j2 = (t >= 3 && multipole == 2) ? 4 : 2;
+                    x3j[(t * 3 + p + 1) * ne + EI(mp)] =
+                        orc_wigner_3j_doubled(2 * j - 1, j2, 2 * abs(kappa) - 1, -2 * mp + 1, 2 * p);
+                }
+            }
+        }
+#define Q(mj, ml, s, t) qmat[((EI(mj) * nm + ((ml) + lmax)) * 2 + (s)) * 8 + (t)]
+#define P(m1, t1, m2, t2) pmat[((EI(m1) * 8 + (t1)) * ne + EI(m2)) * 8 + (t2)]
+#define T(mj, t2, ml, s, t1) tmat[(((EI(mj) * 8 + (t2)) * nm + ((ml) + lmax)) * 2 + (s)) * 8 + (t1)]
+        for (int t = 0; t < 8; ++t) {
+            int l = lind[t], j = jind[t];
+            if (l < 0 || j <= 0) continue;
+            for (int s = 0; s <= 1; ++s)
+                for (int ml = -l; ml <= l; ++ml)
+                    for (int mj = -j + 1; mj <= j; ++mj) {
+                        int mp = ml + s;
+                        double rot = orc_rotwig(angks, 2 * j - 1, 2 * mj - 1, 2 * mp - 1, 2);
+                        Q(mj, ml, s, t) = rot * t3j[(t * 2 + s) * ne + EI(mp)];
+                    }
+        }
+        for (int t2 = 0; t2 < 8; ++t2) {
+            int j2 = jind[t2];
+            if (j2 <= 0) continue;
+            for (int m2 = -j2 + 1; m2 <= j2; ++m2)
+                for (int t1 = 0; t1 < 8; ++t1) {
+                    int j1 = jind[t1];
+                    if (j1 <= 0) continue;
+                    for (int m1 = -j1 + 1; m1 <= j1; ++m1) {
+                        if (abs(m2 - m1) > 2) continue;
+                        cplx v = 0.0;
+                        for (int pp = -1; pp <= 1; ++pp)
+                            for (int p = -1; p <= 1; ++p) {
+                                if (m1 - p != m2 - pp) continue;
+                                double sg = 1.0;
+                                if (multipole == 1 && p > 0 && t1 >= 3) sg = -sg;
+                                if (multipole == 1 && pp > 0 && t2 >= 3) sg = -sg;
+                                v += ptz[(p + 1) * 3 + (pp + 1)] *
+                                     (sg * x3j[(t1 * 3 + p + 1) * ne + EI(m1)] * x3j[(t2 * 3 + pp + 1) * ne + EI(m2)]);
+                            }
+                        double sg = 1.0;
+                        if ((jind[t1] - jind[t2]) % 2 != 0) sg = -sg;
+                        if (t2 < 3) sg = -sg;
+                        int ex = ((lind[t2] - lind[t1]) % 4 + 4) % 4;
+                        cplx ip = ex == 0 ? 1.0 : ex == 1 ? I : ex == 2 ? -1.0 : -I;
+                        P(m1, t1, m2, t2) = v * sg * ip;
+                    }
+                }
+        }
+        for (int t1 = 0; t1 < 8; ++t1) {
+            int l1 = lind[t1], j1 = jind[t1];
+            if (l1 < 0 || j1 <= 0) continue;
+            for (int s = 0; s <= 1; ++s)
+                for (int ml = -l1; ml <= l1; ++ml)
+                    for (int t2 = 0; t2 < 8; ++t2) {
+                        int j2 = jind[t2];
+                        if (j2 <= 0) continue;
+                        for (int mj = -j2 + 1; mj <= j2; ++mj) {
+                            cplx v = 0.0;
+                            for (int mp = -j1 + 1; mp <= j1; ++mp) v += P(mj, t2, mp, t1) * Q(mp, ml, s, t1);
+                            T(mj, t2, ml, s, t1) = v;
+                        }
+                    }
+        }
+        for (int t1 = 0; t1 < 8; ++t1) {
+            int l1 = lind[t1];
+            if (l1 < 0) continue;
+            for (int s1 = 0; s1 <= 1; ++s1)
+                for (int ml1 = -l1; ml1 <= l1; ++ml1)
+                    for (int t2 = 0; t2 < 8; ++t2) {
+                        int l2 = lind[t2], j2 = jind[t2];
+                        if (l2 < 0 || j2 <= 0) continue;
+                        for (int s2 = 0; s2 <= 1; ++s2)
+                            for (int ml2 = -l2; ml2 <= l2; ++ml2) {
+                                cplx v = 0.0;
+                                for (int mj = -j2 + 1; mj <= j2; ++mj) v += Q(mj, ml2, s2, t2) * T(mj, t2, ml1, s1, t1);
+                                B(ml2 + lmax, s2, t2, ml1 + lmax, s1, t1) = v;
+                            }
+                    }
+        }
+#undef Q
+#undef P
+#undef T
+        free(t3j); free(x3j); free(qmat); free(pmat); free(tmat);
+    }
+    if (trace_orbital) {
+        int z = lmax;
+        for (int t1 = 0; t1 < 8; ++t1)
+            for (int s1 = 0; s1 <= 1; ++s1)
+                for (int t2 = 0; t2 < 8; ++t2)
+                    for (int s2 = 0; s2 <= 1; ++s2) {
+                        if (lind[t1] != lind[t2] || s1 != s2) {
+                            B(z, s2, t2, z, s1, t1) = 0.0;
+                        } else {
+                            for (int m = 1; m <= lind[t1]; ++m)
+                                B(z, s1, t2, z, s1, t1) += B(lmax - m, s1, t2, lmax - m, s1, t1) +
+                                                           B(lmax + m, s1, t2, lmax + m, s1, t1);
+                        }
+                    }
+    }
+    if (spin == 0 || spin == 2 || (spin == 1 && nsp == 1)) {
+        for (int t1 = 0; t1 < 8; ++t1) {
+            if (lind[t1] < 0) continue;
+            for (int t2 = 0; t2 < 8; ++t2) {
+                if (lind[t2] < 0) continue;
+                for (int m1 = -lind[t1]; m1 <= lind[t1]; ++m1)
+                    for (int m2 = -lind[t2]; m2 <= lind[t2]; ++m2) {
+                        cplx up = B(m2 + lmax, 1, t2, m1 + lmax, 1, t1);
+                        if (spin == 0) B(m2 + lmax, 0, t2, m1 + lmax, 0, t1) += up;
+                        else B(m2 + lmax, 0, t2, m1 + lmax, 0, t1) = up;
+                    }
+            }
+        }
+    }
+#undef EI
+#undef B
+    return ORC_OK;
+}
+
 /* gtr(ie) = sum gg[(l1 m1 s1),(l2 m2 s2)] bmat(m2,s2,k2,m1,s1,k1) rkk(ie,k1,s1) rkk(ie,k2,s2)
  * gg: [ne][nch][nch] complex with (row, col) at row + nch*col (FP64 kept:
  * the narrowing refeff applies at crates/refeff-engine/src/fms.rs:5912 is

@@ -80,6 +80,8 @@ _lib.orc_fms_spectrum.argtypes = [C.POINTER(OrcPlan), C.c_int, _f64p, _f64p, _f6
 _lib.orc_core_hole_kappa.argtypes = [C.c_int, C.POINTER(C.c_int)]
 _lib.orc_transition_lind.argtypes = [C.c_int, C.c_int, C.c_int, _i32p]
 _lib.orc_bmat_averaged.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _f64p]
+_lib.orc_bmat_general.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _f64p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                  _i32p, _i32p, _f64p]
 _lib.orc_mkgtr_trace.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _f64p, _f64p, _f64p, _f64p]
 
 
@@ -287,6 +289,29 @@ def bmat_averaged(kappa, multipole, lmax, spin, nsp):
     out = np.zeros(2 * nm * 2 * 8 * nm * 2 * 8)
     _lib.orc_bmat_averaged(kappa, multipole, lmax, spin, nsp, lind, out)
     return lind, out.view(np.complex128).copy()
+
+
+def bmat_general(kappa, multipole, lmax, ipol, ptz, trace_orbital, spin, nsp, angks):
+    """FEFF bcoef for any polarization (refeff transition_b_matrix,
+    angular/transition.rs:9-482).  ptz: [3, 3] complex, ptz[p + 1, p' + 1].
+    Returns (lind, kiind, bmat flat complex in (ml2, ms2, k2, ml1, ms1, k1)
+    order, ml2 fastest)."""
+    nm = 2 * lmax + 1
+    lind = np.zeros(8, np.int32)
+    kiind = np.zeros(8, np.int32)
+    out = np.zeros(2 * nm * 2 * 8 * nm * 2 * 8)
+    t = np.ascontiguousarray(np.asarray(ptz, np.complex128).reshape(3, 3))
+    _check(_lib.orc_bmat_general(kappa, multipole, lmax, ipol, t.view(np.float64).reshape(-1), int(trace_orbital),
+                                 spin, nsp, float(angks), lind, kiind, out), "bcoef")
+    return lind, kiind, out.view(np.complex128).copy()
+
+
+def bmat_value(bmat_flat, lmax, ml2, ms2, k2, ml1, ms1, k1):
+    """refeff TransitionBMatrix::value (angular/types.rs:166-195): signed m,
+    transition slots counted from 1."""
+    nm = 2 * lmax + 1
+    i = (ml2 + lmax) + nm * (ms2 + 2 * ((k2 - 1) + 8 * ((ml1 + lmax) + nm * (ms1 + 2 * (k1 - 1)))))
+    return bmat_flat[i]
 
 
 def mkgtr_trace(gg, rkk, lind, bmat_flat, lmax, nsp):

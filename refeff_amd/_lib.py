@@ -100,6 +100,7 @@ def lib():
     L.refeff_fms_solve_batch_device.argtypes = [vp, i32, vp, vp, vp, i32p, C.POINTER(Stats), vp]
     L.refeff_fms_solve_batch_full.argtypes = [vp, i32, f64p, f64p, f64p, f64p, i32p, C.POINTER(Stats)]
     L.refeff_fms_transition_bmat_averaged.argtypes = [i32, i32, i32, i32, i32, i32p, f64p]
+    L.refeff_fms_transition_bmat.argtypes = [i32, i32, i32, i32, f64p, i32, i32, i32, C.c_double, i32p, f64p]
     L.refeff_fms_mkgtr_trace.argtypes = [i32, i32, i32, i32, i32p, f64p, f64p, f64p, f64p]
     L.refeff_fms_run_in_dir.argtypes = [C.c_char_p, i32, i32p, C.c_char_p, C.c_size_t]
     L.refeff_mkgtr_run_in_dir.argtypes = [C.c_char_p, i32p, C.c_char_p, C.c_size_t]

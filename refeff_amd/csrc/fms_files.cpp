@@ -1221,8 +1221,6 @@ int run_mkgtr(const std::string& dir, int32_t* files_written) {
             stage_fail(REFEFF_FMS_UNSUPPORTED, "EELS MKGTR projections (eels.inp) are not provided");
         const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
         if (global.do_nrixs == 1) stage_fail(REFEFF_FMS_UNSUPPORTED, "NRIXS/JAS MKGTR traces are not provided");
-        if (global.ipol != 0)
-            stage_fail(REFEFF_FMS_UNSUPPORTED, "polarised MKGTR (ipol != 0) is not provided; orientation average only");
         const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
         const std::vector<rio::GgSection> gg = rio::parse_gg_dat(rio::read_file(rio::file_exists(bin) ? bin : dat));
         if (in.lmaxph.empty() || in.lmaxph[0] < 0) stage_fail(REFEFF_FMS_INVALID_ARGUMENT, "FMS lmaxph(0) must be nonnegative for MKGTR");
@@ -1239,7 +1237,9 @@ int run_mkgtr(const std::string& dir, int32_t* files_written) {
         std::vector<int32_t> lind(8);
         const int nm = 2 * lmax + 1;
         std::vector<double> bmat(2 * static_cast<size_t>(nm) * 2 * 8 * nm * 2 * 8);
-        check_status(refeff_fms_transition_bmat_averaged(phase.ihole, global.le2, lmax, global.ispin, nsp, lind.data(), bmat.data()),
+        // polarization_tensor (engine fms.rs:5871-5881): rows of (re, im) pairs
+        check_status(refeff_fms_transition_bmat(phase.ihole, global.le2, lmax, global.ipol, &global.polarization_tensor[0][0],
+                                                0, global.ispin, nsp, global.angks, lind.data(), bmat.data()),
                      "failed to build MKGTR transition B matrix");
         std::vector<double> g(2 * static_cast<size_t>(ne) * nch * nch), rkk(2 * static_cast<size_t>(ne) * 8 * active, 0.0);
         for (int e = 0; e < ne; ++e)

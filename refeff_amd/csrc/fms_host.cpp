@@ -4,6 +4,7 @@
 // in fms_device.hip.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -104,6 +105,130 @@ int build_state_table(int nsp, int nat, const int32_t* atom_pot, int max_pot, co
     }
     return REFEFF_FMS_OK;
 }
+
+namespace {
+
+// FEFF setkap: the core hole's Dirac kappa (refeff core_hole.rs:66-80); 0 for
+// an unknown ihole.
+int ihole_kappa(int ihole) {
+    switch (ihole) {
+    case 1: case 2: case 5: case 10: case 17: case 24: case 29: return -1;
+    case 3: case 6: case 11: case 18: case 25: case 30: return 1;
+    case 4: case 7: case 12: case 19: case 26: return -2;
+    case 8: case 13: case 20: case 27: return 2;
+    case 9: case 14: case 21: case 28: return -3;
+    case 15: case 22: return 3;
+    case 16: case 23: return -4;
+    default: return 0;
+    }
+}
+
+// Wigner small-d d^j_{m1 m2}(beta) with doubled arguments (j = j2 / 2, ...),
+// from Wigner's sum over k (Edmonds, Angular Momentum in Quantum Mechanics,
+// eq. 4.1.15); FEFF rotwig is the same function (refeff angular/wigner.rs:101).
+double small_d_doubled(int j2, int m1_2, int m2_2, double beta) {
+    const int jpm1 = (j2 + m1_2) / 2, jmm1 = (j2 - m1_2) / 2, jpm2 = (j2 + m2_2) / 2, jmm2 = (j2 - m2_2) / 2;
+    const int dm = (m1_2 - m2_2) / 2;  // m1 - m2
+    const double c = std::cos(beta / 2.0), s = std::sin(beta / 2.0);
+    const double pre = 0.5 * (std::lgamma(jpm1 + 1.0) + std::lgamma(jmm1 + 1.0) + std::lgamma(jpm2 + 1.0) +
+                              std::lgamma(jmm2 + 1.0));
+    double sum = 0.0;
+    for (int k = std::max(0, -dm); k <= std::min(jpm2, jmm1); ++k) {
+        // (j + m2 - k)! k! (j - m1 - k)! (m1 - m2 + k)!
+        const double den = std::lgamma(jpm2 - k + 1.0) + std::lgamma(k + 1.0) + std::lgamma(jmm1 - k + 1.0) +
+                           std::lgamma(dm + k + 1.0);
+        const int pc = jpm2 + jmm1 - 2 * k, ps = dm + 2 * k;  // exponents of cos and sin
+        const double term = std::exp(pre - den) * std::pow(c, pc) * std::pow(s, ps);
+        sum += ((dm + k) % 2 == 0) ? term : -term;
+    }
+    return sum;
+}
+
+// The polarized bcoef matrix (refeff angular/transition.rs:169-398).  With
+// slot t = (j_t, l_t) and half-integer projections written as mj - 1/2:
+//   c(t, s, mp) = sqrt(2 j_t) (l_t, 1/2, j_t - 1/2; mp - s, s - 1/2, .)  [t3j]
+//   q(t, s, ml, mj) = d^{j_t - 1/2}_{mj - 1/2, ml + s - 1/2}(angks) c(t, s, ml + s)
+//   x(t, p, mj) = (j_t - 1/2, k_t, |kappa| - 1/2; 1/2 - mj, p, .)    [x3j]
+//   P(ma, ta; mb, tb) = sgn i^{l_tb - l_ta} sum_{p, p'} e(p, p') x(ta, p, ma) x(tb, p', mb)
+//                       over ma - p = mb - p'
+//   b(ml2 s2 t2; ml1 s1 t1) = sum_{mj, mp} q(t2, s2, ml2, mj) P(mj, t2; mp, t1) q(t1, s1, ml1, mp)
+void polarized_bmat(int kappa, int multipole, int lmax, const double* ptz, double beta, const int32_t* lind,
+                    double* bmat) {
+    int jind[8];
+    for (int k = -1; k <= 1; ++k) {
+        const int kap = (k == 0) ? -(kappa + k) : kappa + k;
+        jind[k + 1] = lind[k + 1] < 0 ? 0 : std::abs(kap);
+    }
+    for (int k = -2; k <= 2; ++k) jind[k + 5] = lind[k + 5] < 0 ? 0 : std::max(std::abs(kappa) + k, 0);
+    const int nm = 2 * lmax + 1, nx = nm + 1;  // mj runs over -lmax .. lmax + 1
+    auto tensor = [&](int p, int pp) { return std::complex<double>(ptz[2 * (3 * (p + 1) + pp + 1)], ptz[2 * (3 * (p + 1) + pp + 1) + 1]); };
+    std::vector<double> q(static_cast<size_t>(8) * 2 * nm * nx, 0.0), x(static_cast<size_t>(8) * 3 * nx, 0.0);
+    auto Q = [&](int t, int s, int ml, int mj) -> double& { return q[((static_cast<size_t>(t) * 2 + s) * nm + ml + lmax) * nx + mj + lmax]; };
+    auto X = [&](int t, int p, int mj) -> double& { return x[(static_cast<size_t>(t) * 3 + p + 1) * nx + mj + lmax]; };
+    for (int t = 0; t < 8; ++t) {
+        const int j = jind[t], l = lind[t];
+        if (j <= 0) continue;
+        const int k2 = (t >= 3 && multipole == 2) ? 4 : 2;
+        for (int mj = -j + 1; mj <= j; ++mj)
+            for (int p = -1; p <= 1; ++p)
+                X(t, p, mj) = wigner_3j_doubled(2 * j - 1, k2, 2 * std::abs(kappa) - 1, 1 - 2 * mj, 2 * p);
+        if (l < 0) continue;
+        for (int s = 0; s < 2; ++s)
+            for (int ml = -l; ml <= l; ++ml) {
+                const int mp = ml + s;
+                const int j1 = 2 * l, m1 = 2 * (mp - s), m2 = 2 * s - 1;
+                double c = std::sqrt(2.0 * j) * wigner_3j_doubled(j1, 1, 2 * j - 1, m1, m2);
+                if (((1 - j1 - m1 - m2) / 2) % 2 != 0) c = -c;
+                for (int mj = -j + 1; mj <= j; ++mj) Q(t, s, ml, mj) = small_d_doubled(2 * j - 1, 2 * mj - 1, 2 * mp - 1, beta) * c;
+            }
+    }
+    auto P = [&](int ma, int ta, int mb, int tb) {
+        std::complex<double> v = 0.0;
+        if (std::abs(mb - ma) > 2) return v;
+        for (int p = -1; p <= 1; ++p) {
+            const int pp = mb - ma + p;
+            if (pp < -1 || pp > 1) continue;
+            double sg = 1.0;
+            if (multipole == 1 && p > 0 && ta >= 3) sg = -sg;
+            if (multipole == 1 && pp > 0 && tb >= 3) sg = -sg;
+            v += tensor(p, pp) * (sg * X(ta, p, ma) * X(tb, pp, mb));
+        }
+        double sg = ((jind[ta] - jind[tb]) % 2 != 0) ? -1.0 : 1.0;
+        if (tb < 3) sg = -sg;
+        static const std::complex<double> ipow[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+        return v * sg * ipow[(((lind[tb] - lind[ta]) % 4) + 4) % 4];
+    };
+    std::complex<double>* b = reinterpret_cast<std::complex<double>*>(bmat);
+    std::fill(b, b + static_cast<size_t>(nm) * 2 * 8 * nm * 2 * 8, std::complex<double>(0.0, 0.0));
+    for (int t1 = 0; t1 < 8; ++t1) {
+        if (lind[t1] < 0 || jind[t1] <= 0) continue;
+        for (int t2 = 0; t2 < 8; ++t2) {
+            if (lind[t2] < 0 || jind[t2] <= 0) continue;
+            // pq[mj][s1][ml1] = sum_mp P(mj, t2; mp, t1) q(t1, s1, ml1, mp)
+            std::vector<std::complex<double>> pq(static_cast<size_t>(nx) * 2 * nm, 0.0);
+            for (int mj = -jind[t2] + 1; mj <= jind[t2]; ++mj)
+                for (int mp = -jind[t1] + 1; mp <= jind[t1]; ++mp) {
+                    const std::complex<double> pv = P(mj, t2, mp, t1);
+                    if (pv == 0.0) continue;
+                    for (int s1 = 0; s1 < 2; ++s1)
+                        for (int ml1 = -lind[t1]; ml1 <= lind[t1]; ++ml1)
+                            pq[(static_cast<size_t>(mj + lmax) * 2 + s1) * nm + ml1 + lmax] += pv * Q(t1, s1, ml1, mp);
+                }
+            for (int s1 = 0; s1 < 2; ++s1)
+                for (int ml1 = -lind[t1]; ml1 <= lind[t1]; ++ml1)
+                    for (int s2 = 0; s2 < 2; ++s2)
+                        for (int ml2 = -lind[t2]; ml2 <= lind[t2]; ++ml2) {
+                            std::complex<double> v = 0.0;
+                            for (int mj = -jind[t2] + 1; mj <= jind[t2]; ++mj)
+                                v += Q(t2, s2, ml2, mj) * pq[(static_cast<size_t>(mj + lmax) * 2 + s1) * nm + ml1 + lmax];
+                            b[(ml2 + lmax) + static_cast<size_t>(nm) *
+                                                 (s2 + 2 * (t2 + 8 * ((ml1 + lmax) + static_cast<size_t>(nm) * (s1 + 2 * static_cast<size_t>(t1)))))] = v;
+                        }
+        }
+    }
+}
+
+}  // namespace
 
 }  // namespace refeff
 
@@ -265,17 +390,8 @@ int refeff_fms_transition_bmat_averaged(int32_t ihole, int32_t multipole, int32_
                                         int32_t spin_channels, int32_t* lind, double* bmat) {
     // FEFF setkap (refeff core_hole.rs:66-80), bcoef slots and the averaged
     // matrix with the ispin fold (refeff angular/transition.rs:9-167, 431-482)
-    int kappa;
-    switch (ihole) {
-    case 1: case 2: case 5: case 10: case 17: case 24: case 29: kappa = -1; break;
-    case 3: case 6: case 11: case 18: case 25: case 30: kappa = 1; break;
-    case 4: case 7: case 12: case 19: case 26: kappa = -2; break;
-    case 8: case 13: case 20: case 27: kappa = 2; break;
-    case 9: case 14: case 21: case 28: kappa = -3; break;
-    case 15: case 22: kappa = 3; break;
-    case 16: case 23: kappa = -4; break;
-    default: return REFEFF_FMS_INVALID_ARGUMENT;
-    }
+    const int kappa = ihole_kappa(ihole);
+    if (kappa == 0) return REFEFF_FMS_INVALID_ARGUMENT;
     if (lmax < 0 || !lind || !bmat) return REFEFF_FMS_INVALID_ARGUMENT;
     for (int k = -1; k <= 1; ++k) {
         int kap = kappa + k;
@@ -329,6 +445,58 @@ int refeff_fms_transition_bmat_averaged(int32_t ihole, int32_t multipole, int32_
                     }
             }
         }
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_transition_bmat(int32_t ihole, int32_t multipole, int32_t lmax, int32_t ipol, const double* ptz,
+                               int32_t trace_orbital, int32_t spin, int32_t spin_channels, double angks,
+                               int32_t* lind, double* bmat) {
+    // FEFF bcoef for any polarization (refeff angular/transition.rs:9-482):
+    // the averaged matrix, or the polarized one built from the dipole /
+    // quadrupole 3j couplings, the polarization tensor and the spin-axis
+    // rotation, then the optional orbital trace and the ispin fold.
+    // spin 3: the averaged matrix and the slots without a fold
+    const int rc = refeff_fms_transition_bmat_averaged(ihole, multipole, lmax, 3, spin_channels, lind, bmat);
+    if (rc != REFEFF_FMS_OK) return rc;
+    if (ipol != 0) {
+        if (!ptz) return REFEFF_FMS_INVALID_ARGUMENT;
+        for (int i = 0; i < 18; ++i)
+            if (!std::isfinite(ptz[i])) return REFEFF_FMS_NON_FINITE;
+        if (!std::isfinite(angks)) return REFEFF_FMS_NON_FINITE;
+        polarized_bmat(ihole_kappa(ihole), multipole, lmax, ptz, angks, lind, bmat);
+    }
+    const int nm = 2 * lmax + 1;
+    auto at = [&](int ml2, int ms2, int k2, int ml1, int ms1, int k1) -> std::complex<double>& {
+        const size_t i = (ml2 + lmax) + static_cast<size_t>(nm) *
+                                            (ms2 + 2 * (k2 + 8 * ((ml1 + lmax) + static_cast<size_t>(nm) * (ms1 + 2 * static_cast<size_t>(k1)))));
+        return reinterpret_cast<std::complex<double>*>(bmat)[i];
+    };
+    if (trace_orbital) {  // (:400-429): the m-diagonal folded onto m = 0 within equal l and spin
+        for (int k1 = 0; k1 < 8; ++k1)
+            for (int s1 = 0; s1 < 2; ++s1)
+                for (int k2 = 0; k2 < 8; ++k2)
+                    for (int s2 = 0; s2 < 2; ++s2) {
+                        if (lind[k1] != lind[k2] || s1 != s2) {
+                            at(0, s2, k2, 0, s1, k1) = 0.0;
+                            continue;
+                        }
+                        for (int m = 1; m <= lind[k1]; ++m)
+                            at(0, s1, k2, 0, s1, k1) += at(-m, s1, k2, -m, s1, k1) + at(m, s1, k2, m, s1, k1);
+                    }
+    }
+    // ispin fold (:431-482): spin 0 adds the up block onto the down one; spin 2,
+    // or spin 1 with a single channel, copies it there
+    if (spin == 0 || spin == 2 || (spin == 1 && spin_channels == 1))
+        for (int k1 = 0; k1 < 8; ++k1)
+            for (int k2 = 0; k2 < 8; ++k2) {
+                if (lind[k1] < 0 || lind[k2] < 0) continue;
+                for (int m1 = -lind[k1]; m1 <= lind[k1]; ++m1)
+                    for (int m2 = -lind[k2]; m2 <= lind[k2]; ++m2) {
+                        const std::complex<double> up = at(m2, 1, k2, m1, 1, k1);
+                        if (spin == 0) at(m2, 0, k2, m1, 0, k1) += up;
+                        else at(m2, 0, k2, m1, 0, k1) = up;
+                    }
+            }
     return REFEFF_FMS_OK;
 }
 

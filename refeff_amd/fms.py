@@ -20,7 +20,8 @@ __all__ = [
     "FmsError", "SOLVER_AUTO", "SOLVER_MIXED", "SOLVER_FP64", "SOLVER_GGBI", "SOLVER_GGRM", "SOLVER_GGGM",
     "SOLVER_GGTF", "SOLVER_BICGSTAB", "sort_atoms_by_radius",
     "fms_yprep_cluster", "sort_representative_atoms", "construct_state_kets", "FmsRealSpacePlan",
-    "fms_real_space_plan", "fms_real_space_spectrum", "transition_b_matrix_averaged", "mkgtr_green_trace",
+    "fms_real_space_plan", "fms_real_space_spectrum", "transition_b_matrix_averaged", "transition_b_matrix",
+    "mkgtr_green_trace",
     "debye_mean_square_displacements",
 ]
 
@@ -237,6 +238,20 @@ def transition_b_matrix_averaged(ihole: int, multipole: int, lmax: int, spin: in
     b = np.zeros(nm * 2 * 8 * nm * 2 * 8, np.complex128)
     check(lib().refeff_fms_transition_bmat_averaged(ihole, multipole, lmax, spin, spin_channels, _i32p(lind),
                                                     _f64p(b.view(np.float64))), "transition_b_matrix")
+    return lind, b
+
+
+def transition_b_matrix(ihole: int, multipole: int, lmax: int, polarization: int, polarization_tensor,
+                        trace_orbital: bool, spin: int, spin_channels: int, spin_vector_angle: float):
+    """FEFF bcoef for any polarization (refeff transition_b_matrix,
+    angular/transition.rs:9-482): returns (lind, bmat flat complex)."""
+    nm = 2 * lmax + 1
+    lind = np.zeros(8, np.int32)
+    b = np.zeros(nm * 2 * 8 * nm * 2 * 8, np.complex128)
+    t = np.ascontiguousarray(np.asarray(polarization_tensor, np.complex128).reshape(3, 3))
+    check(lib().refeff_fms_transition_bmat(ihole, multipole, lmax, polarization, _f64p(t.view(np.float64)),
+                                           int(bool(trace_orbital)), spin, spin_channels, float(spin_vector_angle),
+                                           _i32p(lind), _f64p(b.view(np.float64))), "transition_b_matrix")
     return lind, b
 
 

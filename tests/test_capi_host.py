@@ -238,3 +238,43 @@ def test_screen_traces_host_paths(tmp_path):
         io.screen_fms_traces(tmp_path, k, ph, 2, 4.0, 8.0)
     assert "does not match maximum potential 1 for SCREEN" in str(err.value)
 
+
+@pytest.mark.parametrize("case", ["averaged", "polarized", "traced", "spin_folded"])
+def test_transition_bmat_matches_refeff_kats_and_oracle(case):
+    """The library's bcoef against refeff's known answers (angular/tests.rs:285-376)
+    and the oracle's restatement, on refeff's sample input (K edge, E1 + E2,
+    lmax 3, angks 0.3)."""
+    kw = {"averaged": dict(ipol=0), "polarized": {}, "traced": dict(trace_orbital=1), "spin_folded": dict(spin=0)}[case]
+    a = dict(ipol=1, trace_orbital=0, spin=1, nsp=1, angks=0.3)
+    a.update(kw)
+    lind, b = fms.transition_b_matrix(1, 2, 3, a["ipol"], _BCOEF_PTZ, a["trace_orbital"], a["spin"], a["nsp"],
+                                      a["angks"])
+    olind, _, ob = orc.bmat_general(-1, 2, 3, a["ipol"], _BCOEF_PTZ, a["trace_orbital"], a["spin"], a["nsp"], a["angks"])
+    assert list(lind) == list(olind) == [1, 1, -1, -1, -1, 0, 2, 2]
+    assert np.abs(b - ob).max() <= 1e-14
+    total = {"averaged": -0.06666666666666678, "polarized": -0.1498383428670292 + 1.0054312926918387j,
+             "traced": -0.31619310063452616 + 0.8078421709741431j,
+             "spin_folded": -0.2170418650429801 + 1.3409109998393136j}[case]
+    assert abs(b.sum() - total) < 1e-12
+    assert int(np.sum(np.abs(b) > 1e-14)) == {"averaged": 34, "polarized": 776, "traced": 728, "spin_folded": 828}[case]
+
+
+_BCOEF_PTZ = np.array([[0.20 - 0.05j, -0.10 + 0.04j, 0.03 + 0.02j],
+                       [0.11 - 0.07j, 0.50 + 0.00j, -0.08 + 0.09j],
+                       [0.06 + 0.01j, 0.13 - 0.02j, 0.17 + 0.03j]])
+
+
+def test_transition_bmat_matches_oracle_across_edges():
+    """Random tensors and spin angles over L1..M5-type holes, E1 / E1+E2, both
+    spin folds: library and oracle bcoef agree to rounding."""
+    rng = np.random.default_rng(3)
+    for ihole, kappa in [(1, -1), (3, 1), (4, -2), (8, 2), (9, -3)]:
+        for multipole in (1, 2):
+            for spin, nsp in ((0, 1), (2, 2)):
+                t = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+                ang = float(rng.uniform(-3, 3))
+                lind, b = fms.transition_b_matrix(ihole, multipole, 3, 1, t, 0, spin, nsp, ang)
+                olind, _, ob = orc.bmat_general(kappa, multipole, 3, 1, t, 0, spin, nsp, ang)
+                assert list(lind) == list(olind)
+                assert np.abs(b - ob).max() <= 1e-13 * max(1.0, np.abs(ob).max()), (ihole, multipole, spin)
+

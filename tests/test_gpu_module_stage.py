@@ -149,6 +149,43 @@ def test_module_mkgtr_writes_trace_matching_oracle(tmp_path):
     banner("fms.bin / gtr.dat match oracle trace")
 
 
+POLARIZED_TENSOR = np.array([[0.20 - 0.05j, -0.10 + 0.04j, 0.03 + 0.02j],
+                             [0.11 - 0.07j, 0.50 + 0.00j, -0.08 + 0.09j],
+                             [0.06 + 0.01j, 0.13 - 0.02j, 0.17 + 0.03j]])
+
+
+def polarized_global_inp(ipol, le2, angks, tensor):
+    rows = "\n".join("".join(f"{v:13.5f}" for z in row for v in (z.real, z.imag)) for row in tensor)
+    text = GLOBAL_INP.replace("    0    0    0      0.0000      0.0000    0    0   -1   -1",
+                              f"    {ipol}    0    {le2}      0.0000      {angks:.4f}    0    0   -1   -1")
+    start = text.index(" polarization tensor\n") + len(" polarization tensor\n")
+    end = text.index("evnorm")
+    return text[:start] + rows + "\n" + text[end:]
+
+
+@pytest.mark.parametrize("le2", [0, 2])
+def test_module_mkgtr_polarized_trace_matches_oracle(tmp_path, le2):
+    """ipol = 1: the MKGTR trace with global.inp's polarization tensor and spin
+    angle (engine fms.rs:5528-5581, core angular/transition.rs:169-398), E1 and
+    E1 + E2, against the oracle's bcoef and trace over the written gg."""
+    banner(f"module mkgtr, ipol 1, le2 {le2}")
+    pts, iph, k, ph, rkk, em, radius, sig2g = make_workdir(tmp_path)
+    (tmp_path / "global.inp").write_text(polarized_global_inp(1, le2, 0.3, POLARIZED_TENSOR))
+    g = io.parse_global_input((tmp_path / "global.inp").read_text())
+    assert g["ipol"] == 1 and g["le2"] == le2 and abs(g["angks"] - 0.3) < 1e-12
+    assert np.allclose(np.array(g["polarization_tensor"]).reshape(3, 3, 2) @ [1, 1j], POLARIZED_TENSOR)
+    io.run_fms_in_dir(tmp_path, 0)
+    assert io.run_mkgtr_in_dir(tmp_path) == 3
+    fb = io.parse_fms_bin((tmp_path / "fms.bin").read_text())
+    gg = np.stack([s.values for s in io.parse_gg_bin((tmp_path / "gg.bin").read_text())])
+    lind, _, b = orc.bmat_general(orc.core_hole_kappa(1), le2, 3, 1, POLARIZED_TENSOR, 0, 0, 1, 0.3)
+    ref = orc.mkgtr_trace(gg, rkk[:, 0], lind, b, 3, 1)
+    _, _, b_avg = orc.bmat_general(orc.core_hole_kappa(1), le2, 3, 0, POLARIZED_TENSOR, 0, 0, 1, 0.3)
+    assert np.abs(orc.mkgtr_trace(gg, rkk[:, 0], lind, b_avg, 3, 1) - ref).max() > 1e-3 * np.abs(ref).max()
+    assert np.all(np.abs(fb.spectra[0] - ref) <= 1e-10 * np.abs(ref).max())  # PAD(8) precision
+    banner("polarized fms.bin matches the oracle trace")
+
+
 @pytest.mark.parametrize("solver", [0, 2])
 def test_module_fms_save_gg_slice_writes_slice_and_diag(tmp_path, solver):
     """save_gg_slice: gg_slice.bin holds the absorber rows of A^-1 G0 and gg_diag.bin

@@ -781,3 +781,71 @@ def test_pot_scf_trace_grid_is_the_fmsdos_trace_on_l_indexed_phases():
     assert got.shape == (ne, L + 1, npot)
     assert np.allclose(got, want.transpose(0, 2, 1), rtol=1e-14, atol=0)
 
+
+# --------------------------------------------------------------------------
+# FEFF bcoef with polarization: refeff angular/tests.rs:285-376 (averaged,
+# polarized, traced and spin-folded KATs on one sample input, :946-977)
+# --------------------------------------------------------------------------
+
+BCOEF_PTZ = np.array([[0.20 - 0.05j, -0.10 + 0.04j, 0.03 + 0.02j],
+                      [0.11 - 0.07j, 0.50 + 0.00j, -0.08 + 0.09j],
+                      [0.06 + 0.01j, 0.13 - 0.02j, 0.17 + 0.03j]])
+BCOEF_AT = [(0, 0, 1, 0, 0, 1), (1, 0, 2, -1, 0, 2), (0, 1, 4, 0, 1, 4), (-1, 0, 5, 1, 1, 6), (0, 0, 7, 0, 0, 7),
+            (0, 0, 8, 0, 0, 8)]
+
+
+def _bcoef(**kw):
+    a = dict(kappa=-1, multipole=2, lmax=3, ipol=1, ptz=BCOEF_PTZ, trace_orbital=0, spin=1, nsp=1, angks=0.3)
+    a.update(kw)
+    return orc.bmat_general(**a)
+
+
+def _check_bcoef(b, values, total, nonzero):
+    for at, want in zip(BCOEF_AT, values):
+        assert abs(orc.bmat_value(b, 3, *at) - want) < 1e-12, (at, orc.bmat_value(b, 3, *at), want)
+    assert abs(b.sum() - total) < 1e-12, (b.sum(), total)
+    assert int(np.sum(np.abs(b) > 1e-14)) == nonzero
+
+
+@pytest.mark.parametrize("case", ["averaged", "polarized", "traced", "spin_folded"])
+def test_bcoef_kats(case):
+    kw = {"averaged": dict(ipol=0), "polarized": {}, "traced": dict(trace_orbital=1), "spin_folded": dict(spin=0)}[case]
+    lind, kiind, b = _bcoef(**kw)
+    assert list(kiind) == [-2, 1, 0, 0, 0, -1, 2, -3]
+    assert list(lind) == [1, 1, -1, -1, -1, 0, 2, 2]
+    want = {
+        "averaged": ([-0.05555555555555555, 0, 0, 0, 0.02, 0.02], -0.06666666666666678, 34),
+        "polarized": ([-0.06273441744900005 - 0.002816883591316175j, 0, 0, 0,
+                       0.028021517059414094 + 0.0012286136649168643j, 0.032783586221432195 + 0.001621399287040505j],
+                      -0.1498383428670292 + 1.0054312926918387j, 776),
+        "traced": ([-0.13927806177774438 - 0.006120603534129609j, 0, 0, 0,
+                    0.08851128918371495 - 0.0038584833449001833j, 0.08464910571422107 + 0.0008909740920669523j],
+                   -0.31619310063452616 + 0.8078421709741431j, 728),
+        "spin_folded": ([-0.12775761018690238 - 0.0018521924356471728j, 0, 0, 0,
+                         0.05865146715590615 - 0.0005346566843031169j, 0.06730612780091635 + 0.0012482092313231155j],
+                        -0.2170418650429801 + 1.3409109998393136j, 828),
+    }[case]
+    _check_bcoef(b, *want)
+
+
+def test_bcoef_averaged_branch_is_the_closed_form():
+    """ipol = 0 of the general bcoef is the averaged closed form already pinned,
+    for every core hole, multipole and spin fold."""
+    for kappa in (-1, 1, -2, 2, -3):
+        for multipole in (0, 1, 2):
+            for spin, nsp in ((0, 1), (1, 1), (1, 2), (2, 2)):
+                lind, _, b = orc.bmat_general(kappa, multipole, 3, 0, np.zeros((3, 3)), 0, spin, nsp, 0.0)
+                lind2, b2 = orc.bmat_averaged(kappa, multipole, 3, spin, nsp)
+                assert list(lind) == list(lind2) and np.array_equal(b, b2)
+
+
+def test_rotwig_is_wigner_small_d():
+    """The spin rotation enters only through FEFF rotwig: with angks = 0 it is the
+    identity, so a rotated and an unrotated bcoef differ, and angks = 2 pi flips
+    the sign of every half-integer d, which cancels in qmat's product of two."""
+    _, _, b0 = _bcoef(angks=0.0)
+    _, _, b1 = _bcoef(angks=0.3)
+    _, _, b2 = _bcoef(angks=2 * np.pi)
+    assert np.abs(b0 - b1).max() > 1e-3
+    assert np.allclose(b0, b2, atol=1e-13)
+
