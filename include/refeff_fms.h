@@ -308,7 +308,9 @@ int refeff_fms_run_in_dir(const char* work_dir, int32_t solver, int32_t* files_w
  * global.inp's polarization tensor and spin angle) and appends the MKGTR
  * lines to log3.dat.  Replaces run_mkgtr_in_dir,
  * crates/refeff-engine/src/fms.rs:559-631 with build_mkgtr_ordinary_outputs
- * (:5528-5608); EELS and NRIXS traces return REFEFF_FMS_UNSUPPORTED. */
+ * (:5528-5608).  With eels.inp asking for ELNES, fms.bin holds one spectrum
+ * per Cartesian polarization selector (:5610-5677) and gtr.dat the first.
+ * NRIXS traces return REFEFF_FMS_UNSUPPORTED. */
 int refeff_mkgtr_run_in_dir(const char* work_dir, int32_t* files_written, char* message, size_t message_len);
 
 /* LDOS gtrNN.bin on the RHORRP energy grid that the LDOS stage hands over.

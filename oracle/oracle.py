@@ -306,6 +306,28 @@ def bmat_general(kappa, multipole, lmax, ipol, ptz, trace_orbital, spin, nsp, an
     return lind, kiind, out.view(np.complex128).copy()
 
 
+def cartesian_polarization_tensor(selector):
+    """FEFF iniptz, Cartesian mode (refeff core angular/polarization.rs:8-101):
+    [3, 3] complex indexed [p + 1, p' + 1]; selector 10 is the average."""
+    t = np.zeros((3, 3), np.complex128)
+    h, r = 0.5, 1.0 / np.sqrt(2.0)
+    entries = {
+        1: [(1, 1, h), (-1, -1, h), (-1, 1, -h), (1, -1, -h)],
+        2: [(1, 1, 1j * h), (-1, -1, -1j * h), (-1, 1, -1j * h), (1, -1, 1j * h)],
+        3: [(-1, 0, r), (1, 0, -r)],
+        4: [(1, 1, -1j * h), (-1, -1, 1j * h), (-1, 1, -1j * h), (1, -1, 1j * h)],
+        5: [(1, 1, h), (-1, -1, h), (-1, 1, h), (1, -1, h)],
+        6: [(-1, 0, -1j * r), (1, 0, -1j * r)],
+        7: [(0, -1, r), (0, 1, -r)],
+        8: [(0, -1, 1j * r), (0, 1, 1j * r)],
+        9: [(0, 0, 1.0)],
+        10: [(-1, -1, 1 / 3), (0, 0, 1 / 3), (1, 1, 1 / 3)],
+    }[selector]
+    for p, pp, v in entries:
+        t[p + 1, pp + 1] = v
+    return t
+
+
 def bmat_value(bmat_flat, lmax, ml2, ms2, k2, ml1, ms1, k1):
     """refeff TransitionBMatrix::value (angular/types.rs:166-195): signed m,
     transition slots counted from 1."""

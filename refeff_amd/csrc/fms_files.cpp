@@ -1203,6 +1203,47 @@ int run_screen(const std::string& dir, int solver, const ScreenArgs& a, int32_t*
     return REFEFF_FMS_OK;
 }
 
+// FEFF iniptz's Cartesian tensors (refeff core angular/polarization.rs:37-101):
+// selector 3 a + b + 1 pairs the directions a, b of x, y, z; 10 is the
+// orientation average.  Entries [p+1][p'+1] as (re, im), p, p' in -1, 0, 1.
+void cartesian_tensor(int selector, double* t) {
+    std::fill(t, t + 18, 0.0);
+    const double h = 0.5, r = std::sqrt(0.5);
+    auto set = [&](int p, int pp, double re, double im) {
+        t[2 * (3 * (p + 1) + pp + 1)] = re;
+        t[2 * (3 * (p + 1) + pp + 1) + 1] = im;
+    };
+    switch (selector) {
+    case 1: set(1, 1, h, 0); set(-1, -1, h, 0); set(-1, 1, -h, 0); set(1, -1, -h, 0); break;
+    case 2: set(1, 1, 0, h); set(-1, -1, 0, -h); set(-1, 1, 0, -h); set(1, -1, 0, h); break;
+    case 3: set(-1, 0, r, 0); set(1, 0, -r, 0); break;
+    case 4: set(1, 1, 0, -h); set(-1, -1, 0, h); set(-1, 1, 0, -h); set(1, -1, 0, h); break;
+    case 5: set(1, 1, h, 0); set(-1, -1, h, 0); set(-1, 1, h, 0); set(1, -1, h, 0); break;
+    case 6: set(-1, 0, 0, -r); set(1, 0, 0, -r); break;
+    case 7: set(0, -1, r, 0); set(0, 1, -r, 0); break;
+    case 8: set(0, -1, 0, r); set(0, 1, 0, r); break;
+    case 9: set(0, 0, 1, 0); break;
+    default: for (int p = -1; p <= 1; ++p) set(p, p, 1.0 / 3.0, 0); break;
+    }
+}
+
+// MKGTR's EELS projections (engine fms.rs:5610-5649): with eels.inp asking for
+// ELNES, one Cartesian tensor per selector min, min + step, ..., max.
+bool eels_selectors(const std::string& dir, std::vector<int>& out) {
+    out.clear();
+    const std::string path = join(dir, "eels.inp");
+    if (!rio::file_exists(path)) return false;
+    const rio::EelsInp e = rio::parse_eels_inp(rio::read_file(path), path);
+    if (e.calculation_mode == 0) return false;
+    if (e.pol_min < 1 || e.pol_min > 10 || e.pol_max < 1 || e.pol_max > 10 || e.pol_min > e.pol_max || e.pol_step <= 0)
+        stage_fail(REFEFF_FMS_INVALID_ARGUMENT,
+                   "MKGTR EELS polarization range must satisfy 1 <= min <= max <= 10 and step > 0, got min=" +
+                       std::to_string(e.pol_min) + ", step=" + std::to_string(e.pol_step) + ", max=" +
+                       std::to_string(e.pol_max));
+    for (int s = e.pol_min; s <= e.pol_max; s += e.pol_step) out.push_back(s);
+    return true;
+}
+
 // MKGTR over a cached gg: refeff run_mkgtr_in_dir / build_mkgtr_ordinary_outputs,
 // crates/refeff-engine/src/fms.rs:559-631, 5255-5330, 5528-5608, 5849-5918,
 // 5941-5963.  The gg sections stay in FP64 (refeff narrows them to c32).
@@ -1216,9 +1257,11 @@ int run_mkgtr(const std::string& dir, int32_t* files_written) {
     int written = 0;
     const std::string fms_path = join(dir, "fms.bin"), gtr_path = join(dir, "gtr.dat");
     const bool have_sources = rio::file_exists(join(dir, "phase.bin")) && rio::file_exists(join(dir, "global.inp"));
-    if (have_sources && (!rio::file_exists(fms_path) || !rio::file_exists(gtr_path))) {
-        if (rio::file_exists(join(dir, "eels.inp")))
-            stage_fail(REFEFF_FMS_UNSUPPORTED, "EELS MKGTR projections (eels.inp) are not provided");
+    std::vector<int> selectors;
+    // EELS replaces the one global.inp tensor by one Cartesian tensor per
+    // selector, and its traces are always rebuilt from gg (engine fms.rs:5268-5316)
+    const bool eels = have_sources && eels_selectors(dir, selectors);
+    if (have_sources && (!rio::file_exists(fms_path) || !rio::file_exists(gtr_path) || eels)) {
         const rio::GlobalInp global = rio::parse_global_inp(rio::read_file(join(dir, "global.inp")));
         if (global.do_nrixs == 1) stage_fail(REFEFF_FMS_UNSUPPORTED, "NRIXS/JAS MKGTR traces are not provided");
         const rio::PhaseBin phase = rio::parse_phase_bin(rio::read_file(join(dir, "phase.bin")));
@@ -1236,11 +1279,18 @@ int run_mkgtr(const std::string& dir, int32_t* files_written) {
         const int nsp = phase.spin_count;
         std::vector<int32_t> lind(8);
         const int nm = 2 * lmax + 1;
-        std::vector<double> bmat(2 * static_cast<size_t>(nm) * 2 * 8 * nm * 2 * 8);
-        // polarization_tensor (engine fms.rs:5871-5881): rows of (re, im) pairs
-        check_status(refeff_fms_transition_bmat(phase.ihole, global.le2, lmax, global.ipol, &global.polarization_tensor[0][0],
-                                                0, global.ispin, nsp, global.angks, lind.data(), bmat.data()),
-                     "failed to build MKGTR transition B matrix");
+        const size_t bsize = 2 * static_cast<size_t>(nm) * 2 * 8 * nm * 2 * 8;
+        const int nspec = eels ? static_cast<int>(selectors.size()) : 1;
+        std::vector<double> bmat(bsize * nspec);
+        for (int sp = 0; sp < nspec; ++sp) {
+            // polarization_tensor (engine fms.rs:5871-5881): rows of (re, im) pairs
+            double tensor[18];
+            if (eels) cartesian_tensor(selectors[sp], tensor);
+            else std::copy(&global.polarization_tensor[0][0], &global.polarization_tensor[0][0] + 18, tensor);
+            check_status(refeff_fms_transition_bmat(phase.ihole, global.le2, lmax, eels ? 1 : global.ipol, tensor, 0,
+                                                    global.ispin, nsp, global.angks, lind.data(), bmat.data() + bsize * sp),
+                         "failed to build MKGTR transition B matrix");
+        }
         std::vector<double> g(2 * static_cast<size_t>(ne) * nch * nch), rkk(2 * static_cast<size_t>(ne) * 8 * active, 0.0);
         for (int e = 0; e < ne; ++e)
             for (int r = 0; r < nch; ++r)
@@ -1259,9 +1309,11 @@ int run_mkgtr(const std::string& dir, int32_t* files_written) {
                     rkk[2 * o] = v.real();
                     rkk[2 * o + 1] = v.imag();
                 }
-        std::vector<double> gtr(2 * static_cast<size_t>(ne));
-        check_status(refeff_fms_mkgtr_trace(ne, nch, active, lmax, lind.data(), bmat.data(), g.data(), rkk.data(), gtr.data()),
-                     "failed to fold cached gg matrices into MKGTR trace");
+        std::vector<double> gtr(2 * static_cast<size_t>(ne) * nspec);
+        for (int sp = 0; sp < nspec; ++sp)
+            check_status(refeff_fms_mkgtr_trace(ne, nch, active, lmax, lind.data(), bmat.data() + bsize * sp, g.data(),
+                                                rkk.data(), gtr.data() + 2 * static_cast<size_t>(ne) * sp),
+                         "failed to fold cached gg matrices into MKGTR trace");
         rio::FmsBin f;
         f.cluster_radius = in.rfms2;
         f.energy_count = ne;
@@ -1270,14 +1322,15 @@ int run_mkgtr(const std::string& dir, int32_t* files_written) {
         f.highest_potential_index = static_cast<int>(phase.potentials.size()) - 1;
         f.pad_width = phase.pad_width;
         f.declared_spectrum_count = 0;
-        f.spectrum_count = 1;
-        f.spectra = cvec(gtr.data(), ne);
-        if (!rio::file_exists(fms_path)) {
+        f.spectrum_count = nspec;
+        f.spectra = cvec(gtr.data(), static_cast<size_t>(ne) * nspec);
+        const std::vector<cplx> first(f.spectra.begin(), f.spectra.begin() + ne);
+        if (!rio::file_exists(fms_path) || eels) {
             rio::write_file(fms_path, rio::fms_bin_string(f));
             ++written;
         }
-        if (!rio::file_exists(gtr_path)) {
-            rio::write_file(gtr_path, rio::gtr_dat_string(phase.energy_grid, f.spectra));
+        if (!rio::file_exists(gtr_path) || eels) {
+            rio::write_file(gtr_path, rio::gtr_dat_string(phase.energy_grid, first));
             ++written;
         }
     } else {

@@ -1256,6 +1256,77 @@ GtrBin parse_gtr_bin(const std::string& b) {
 
 // ---- ldos.inp (ldos_input.rs:59-300) ----
 
+EelsInp parse_eels_inp(const std::string& text, const std::string& label) {
+    const std::vector<std::string> lines = split_lines(text);
+    size_t li = 0;
+    auto perr = [&](size_t line, const std::string& m) -> Error {
+        return Error(REFEFF_FMS_IO_ERROR, label + " line " + std::to_string(line) + ": " + m);
+    };
+    auto next = [&](const std::string& what) -> std::string {
+        if (li >= lines.size()) throw perr(0, "expected " + what);
+        return lines[li++];
+    };
+    auto header = [&](const char* expected) {
+        const std::string l = next(expected);
+        if (trim(l) != expected) throw perr(li, std::string("expected header \"") + expected + "\", found \"" + l + "\"");
+    };
+    auto values = [&](size_t count, const std::string& what, bool integer) {
+        const std::vector<std::string> f = fields(next(what));
+        if (f.size() < count) throw perr(li, what + " requires " + std::to_string(count) + " fields");
+        std::vector<double> v;
+        for (size_t i = 0; i < count; ++i) {
+            long long n;
+            double x;
+            if (integer) {
+                if (!parse_long(f[i], n) || n < std::numeric_limits<int32_t>::min() || n > std::numeric_limits<int32_t>::max())
+                    throw perr(li, "invalid numeric field \"" + f[i] + "\"");
+                x = static_cast<double>(n);
+            } else if (!parse_double(f[i], x)) {
+                throw perr(li, "invalid numeric field \"" + f[i] + "\"");
+            }
+            v.push_back(x);
+        }
+        return v;
+    };
+    EelsInp e;
+    header("calculate ELNES?");
+    e.calculation_mode = static_cast<int>(values(1, "EELS calculate line", true)[0]);
+    header("average? relativistic? cross-terms? Which input?");
+    std::vector<double> v = values(5, "EELS control line", true);
+    e.average = static_cast<int>(v[0]);
+    e.relativistic = static_cast<int>(v[1]);
+    e.cross_terms = static_cast<int>(v[2]);
+    e.input = static_cast<int>(v[3]);
+    e.spectrum_column = static_cast<int>(v[4]);
+    header("polarizations to be used ; min step max");
+    v = values(3, "EELS polarization line", true);
+    e.pol_min = static_cast<int>(v[0]);
+    e.pol_step = static_cast<int>(v[1]);
+    e.pol_max = static_cast<int>(v[2]);
+    header("beam energy in eV");
+    e.beam_energy = values(1, "EELS beam-energy line", false)[0];
+    header("beam direction in arbitrary units");
+    v = values(3, "EELS beam-direction line", false);
+    std::copy(v.begin(), v.end(), e.beam_direction);
+    header("collection and convergence semiangle in rad");
+    v = values(2, "EELS angle line", false);
+    e.collection = v[0];
+    e.convergence = v[1];
+    header("qmesh - radial and angular grid size");
+    v = values(2, "EELS qmesh line", true);
+    e.qmesh_radial = static_cast<int>(v[0]);
+    e.qmesh_angular = static_cast<int>(v[1]);
+    header("detector positions - two angles in rad");
+    v = values(2, "EELS detector line", false);
+    e.detector[0] = v[0];
+    e.detector[1] = v[1];
+    header("calculate magic angle if magic=1");
+    e.magic = static_cast<int>(values(1, "EELS magic line", true)[0]);
+    header("energy for magic angle - eV above threshold");
+    e.magic_energy = values(1, "EELS magic-energy line", false)[0];
+    return e;
+}
+
 LdosInp parse_ldos_inp(const std::string& text, const std::string& label) {
     const std::vector<std::string> lines = split_lines(text);
     size_t li = 0;

@@ -161,6 +161,20 @@ struct GtrBin {
 std::string gtr_bin_bytes(const GtrBin& d);
 GtrBin parse_gtr_bin(const std::string& bytes);
 
+// eels.inp, crates/refeff-io/src/eels_input.rs:195-300 (parser): the fixed
+// header lines and their values.  calculation_mode != 0 asks for ELNES.
+struct EelsInp {
+    int calculation_mode = 0;
+    int average = 0, relativistic = 0, cross_terms = 0, input = 0, spectrum_column = 0;
+    int pol_min = 0, pol_step = 0, pol_max = 0;
+    double beam_energy = 0, beam_direction[3] = {0, 0, 0}, collection = 0, convergence = 0;
+    int qmesh_radial = 0, qmesh_angular = 0;
+    double detector[2] = {0, 0};
+    int magic = 0;
+    double magic_energy = 0;
+};
+EelsInp parse_eels_inp(const std::string& text, const std::string& label);
+
 // ldos.inp, crates/refeff-io/src/ldos_input.rs:59-300 (parser and renderer).
 struct LdosInp {
     int mldos = 0, lfms2 = 0, ixc = 0, ispin = 0, minv = 0, neldos = 0, iscfxc = 0;

@@ -9,6 +9,7 @@ and the phase shifts from the arrays written into phase.bin (PAD at npack 8
 keeps ~1e-12 relative), σ² = sig2g between distinct atoms (or the oracle's
 Debye / sig2.dat σ² for idwopt 0, 3, 4).  gg.bin's E20.10
 fields carry 10 significant digits, so the gate is the north-star 1e-8."""
+import dataclasses
 import sys
 
 import numpy as np
@@ -184,6 +185,45 @@ def test_module_mkgtr_polarized_trace_matches_oracle(tmp_path, le2):
     assert np.abs(orc.mkgtr_trace(gg, rkk[:, 0], lind, b_avg, 3, 1) - ref).max() > 1e-3 * np.abs(ref).max()
     assert np.all(np.abs(fb.spectra[0] - ref) <= 1e-10 * np.abs(ref).max())  # PAD(8) precision
     banner("polarized fms.bin matches the oracle trace")
+
+
+def eels_inp(mode, pmin, pstep, pmax):
+    return (f"calculate ELNES?\n{mode:4d}\naverage? relativistic? cross-terms? Which input?\n"
+            f"{0:4d}{1:4d}{1:4d}{1:4d}{4:4d}\npolarizations to be used ; min step max\n{pmin:4d}{pstep:4d}{pmax:4d}\n"
+            "beam energy in eV\n 300000.00000\nbeam direction in arbitrary units\n      0.00000      1.00000"
+            "      0.00000\ncollection and convergence semiangle in rad\n      0.00240      0.00000\n"
+            "qmesh - radial and angular grid size\n   5   3\ndetector positions - two angles in rad\n"
+            "      0.00000      0.00000\ncalculate magic angle if magic=1\n   0\n"
+            "energy for magic angle - eV above threshold\n      0.00000\n")
+
+
+def test_module_mkgtr_eels_spectra_match_oracle(tmp_path):
+    """eels.inp with ELNES on: one trace per Cartesian tensor selector
+    (engine fms.rs:5528-5560, 5610-5677), written as fms.bin spectra; gtr.dat holds
+    the first.  The traces are rebuilt from gg on every MKGTR run."""
+    banner("module mkgtr, EELS selectors 1, 5, 9")
+    pts, iph, k, ph, rkk, em, radius, sig2g = make_workdir(tmp_path)
+    (tmp_path / "eels.inp").write_text(eels_inp(1, 1, 4, 9))
+    io.run_fms_in_dir(tmp_path, 0)
+    assert io.run_mkgtr_in_dir(tmp_path) == 3
+    fb = io.parse_fms_bin((tmp_path / "fms.bin").read_text())
+    assert fb.spectrum_count() == 3 and fb.declared_spectrum_count == 0
+    gg = np.stack([s.values for s in io.parse_gg_bin((tmp_path / "gg.bin").read_text())])
+    for i, sel in enumerate((1, 5, 9)):
+        lind, _, b = orc.bmat_general(orc.core_hole_kappa(1), 0, 3, 1, orc.cartesian_polarization_tensor(sel), 0, 0, 1,
+                                      0.0)
+        ref = orc.mkgtr_trace(gg, rkk[:, 0], lind, b, 3, 1)
+        assert np.all(np.abs(fb.spectra[i] - ref) <= 1e-10 * np.abs(ref).max()), sel
+    _, t_gtr = io.parse_gtr_dat((tmp_path / "gtr.dat").read_text())
+    assert np.all(np.abs(t_gtr - fb.spectra[0]) <= 1e-6 * np.maximum(np.abs(fb.spectra[0]), 1))
+    (tmp_path / "fms.bin").write_text(io.fms_bin_string(dataclasses.replace(fb, spectra=fb.spectra[:1])))
+    assert io.run_mkgtr_in_dir(tmp_path) == 3  # a one-spectrum cache is rebuilt
+    assert io.parse_fms_bin((tmp_path / "fms.bin").read_text()).spectrum_count() == 3
+    (tmp_path / "eels.inp").write_text(eels_inp(1, 3, 1, 11))
+    with pytest.raises(io.CodecError) as e:
+        io.run_mkgtr_in_dir(tmp_path)
+    assert "1 <= min <= max <= 10" in str(e.value)
+    banner("EELS spectra match the oracle traces")
 
 
 @pytest.mark.parametrize("solver", [0, 2])

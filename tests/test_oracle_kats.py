@@ -849,3 +849,41 @@ def test_rotwig_is_wigner_small_d():
     assert np.abs(b0 - b1).max() > 1e-3
     assert np.allclose(b0, b2, atol=1e-13)
 
+
+def test_cartesian_polarization_tensor_kats():
+    """refeff builds_feff_cartesian_polarization_tensors (angular/tests.rs:191-263):
+    every non-zero entry of the ten tensors, and nothing else."""
+    h, r = 0.5, 1.0 / np.sqrt(2.0)
+    kat = {1: [(-1, -1, h), (-1, 1, -h), (1, -1, -h), (1, 1, h)],
+           2: [(-1, -1, -0.5j), (-1, 1, -0.5j), (1, -1, 0.5j), (1, 1, 0.5j)],
+           3: [(-1, 0, r), (1, 0, -r)],
+           4: [(-1, -1, 0.5j), (-1, 1, -0.5j), (1, -1, 0.5j), (1, 1, -0.5j)],
+           5: [(-1, -1, h), (-1, 1, h), (1, -1, h), (1, 1, h)],
+           6: [(-1, 0, -1j * r), (1, 0, -1j * r)],
+           7: [(0, -1, r), (0, 1, -r)],
+           8: [(0, -1, 1j * r), (0, 1, 1j * r)],
+           9: [(0, 0, 1.0)],
+           10: [(-1, -1, 1 / 3), (0, 0, 1 / 3), (1, 1, 1 / 3)]}
+    for sel, entries in kat.items():
+        want = np.zeros((3, 3), np.complex128)
+        for p, pp, v in entries:
+            want[p + 1, pp + 1] = v
+        assert np.abs(orc.cartesian_polarization_tensor(sel) - want).max() < 1e-15, sel
+
+
+def test_cartesian_polarization_tensor_is_a_direction_product():
+    """Selectors 3a + b + 1 of the x and z directions are conj(a_p) b_p' in the
+    spherical basis v_{+1} = -(v_x + i v_y)/sqrt 2, v_0 = v_z, v_{-1} = (v_x - i v_y)/sqrt 2
+    (the y-z couplings 6 and 8 carry FEFF's opposite sign); the trace of 10 is 1."""
+    def sph(v):
+        x, y, z = v
+        return np.array([(x - 1j * y) / np.sqrt(2), z, -(x + 1j * y) / np.sqrt(2)])  # p = -1, 0, 1
+    axes = np.eye(3)
+    for a in range(3):
+        for b in range(3):
+            sel = 3 * a + b + 1
+            t = np.outer(np.conj(sph(axes[a])), sph(axes[b]))
+            sign = -1.0 if sel in (6, 8) else 1.0
+            assert np.abs(orc.cartesian_polarization_tensor(sel) - sign * t).max() < 1e-15, sel
+    assert abs(np.trace(orc.cartesian_polarization_tensor(10)) - 1.0) < 1e-15
+
