@@ -320,6 +320,19 @@ struct SourceGrid {
     int nphl = 0, nphp = 0;
 };
 
+// geom.dat's atoms as the f32 coordinates and potentials the cluster
+// selection takes (refeff fms_atoms_from_geom, engine fms.rs:4286-4300).
+void geom_arrays(const rio::GeomDat& geom, std::vector<float>& pos, std::vector<int32_t>& pot) {
+    pos.resize(3 * geom.atoms.size());
+    pot.resize(geom.atoms.size());
+    for (size_t i = 0; i < geom.atoms.size(); ++i) {
+        pos[3 * i] = static_cast<float>(geom.atoms[i].x);
+        pos[3 * i + 1] = static_cast<float>(geom.atoms[i].y);
+        pos[3 * i + 2] = static_cast<float>(geom.atoms[i].z);
+        pot[i] = geom.atoms[i].iph;
+    }
+}
+
 struct ClusterSolve {
     int ne = 0, nsp = 1, npot = 0, ch = 0, L = 0, nat = 0, absorber = 0;
     size_t nst = 0;
@@ -343,14 +356,9 @@ ClusterSolve solve_cluster(const std::string& dir, const rio::FmsInp& in, const 
     const float rdirec = in.rdirec >= 0.0 ? narrow_nonnegative(in.rdirec, "FMS direct cutoff") : 0.0f;
     if (geom.atoms.empty()) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
     const int nat_all = static_cast<int>(geom.atoms.size());
-    std::vector<float> pos(3 * static_cast<size_t>(nat_all));
-    std::vector<int32_t> pot(nat_all);
-    for (int i = 0; i < nat_all; ++i) {
-        pos[3 * i] = static_cast<float>(geom.atoms[i].x);
-        pos[3 * i + 1] = static_cast<float>(geom.atoms[i].y);
-        pos[3 * i + 2] = static_cast<float>(geom.atoms[i].z);
-        pot[i] = geom.atoms[i].iph;
-    }
+    std::vector<float> pos;
+    std::vector<int32_t> pot;
+    geom_arrays(geom, pos, pot);
     std::vector<float> cpos(pos.size());
     std::vector<int32_t> cpot(nat_all);
     int32_t nat = 0, central = 0, untruncated = 0;
@@ -965,14 +973,9 @@ int run_pot_scf(const std::string& dir, int solver, const PotScfArgs& a) {
     const std::vector<double> ph = signed_l_phases(a.ne, a.nphl, npot, L, a.lmaxsc, a.phases);
     const int nat_all = static_cast<int>(geom.atoms.size());
     if (nat_all == 0) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
-    std::vector<float> pos(3 * static_cast<size_t>(nat_all));
-    std::vector<int32_t> pot(nat_all);
-    for (int i = 0; i < nat_all; ++i) {
-        pos[3 * i] = static_cast<float>(geom.atoms[i].x);
-        pos[3 * i + 1] = static_cast<float>(geom.atoms[i].y);
-        pos[3 * i + 2] = static_cast<float>(geom.atoms[i].z);
-        pot[i] = geom.atoms[i].iph;
-    }
+    std::vector<float> pos;
+    std::vector<int32_t> pot;
+    geom_arrays(geom, pos, pot);
     const int ch = (L + 1) * (L + 1);
     std::vector<double> sc_all(2 * static_cast<size_t>(a.ne) * npot * ch * ch, 0.0);  // [e][ip] blocks
     for (int c = 0; c < (all ? 1 : npot); ++c) {
@@ -1109,14 +1112,9 @@ int run_screen(const std::string& dir, int solver, const ScreenArgs& a, int32_t*
     const float rdirec = effective(in.rdirec, "FMS direct cutoff", "direct cutoff");
     if (geom.atoms.empty()) stage_fail(REFEFF_FMS_EMPTY_CLUSTER, "geom.dat requires at least one atom for FMS generation");
     const int nat_all = static_cast<int>(geom.atoms.size());
-    std::vector<float> pos(3 * static_cast<size_t>(nat_all));
-    std::vector<int32_t> pot(nat_all);
-    for (int i = 0; i < nat_all; ++i) {
-        pos[3 * i] = static_cast<float>(geom.atoms[i].x);
-        pos[3 * i + 1] = static_cast<float>(geom.atoms[i].y);
-        pos[3 * i + 2] = static_cast<float>(geom.atoms[i].z);
-        pot[i] = geom.atoms[i].iph;
-    }
+    std::vector<float> pos;
+    std::vector<int32_t> pot;
+    geom_arrays(geom, pos, pot);
     std::vector<float> cpos(pos.size());
     std::vector<int32_t> cpot(nat_all);
     int32_t nat = 0, central = 0, untruncated = 0;
