@@ -781,6 +781,206 @@ __global__ __launch_bounds__(1024) void k_panel_rb(typename Cx<T>::type* Ab, siz
     }
 }
 
+// The same panel factorisation with one barrier per column step (below); used
+// where Q <= 4 keeps it free of spills.  Register-blocked panel factorisation
+// of rows [j0, n) x columns [j0, j0+kPanel)
+// with partial pivoting (same pivots and info as LAPACK's unblocked getf2
+// order: first maximum of |re|+|im|).  Thread t owns rows j0 + t + 1024 q,
+// q < Q (host guarantees n - j0 <= 1024 Q).  The panel is split into
+// sub-panels of W columns held in registers: a column step is a block argmax
+// plus one exchange of the pivot row through LDS, with no memory traffic.
+// After each sub-panel its interchanges and unit-lower solve are applied to
+// the panel columns right of it and one rank-W pass updates the rows below.
+template <typename T, int W, int Q>
+__global__ __launch_bounds__(1024) void k_panel_rb1(typename Cx<T>::type* Ab, size_t sa, int lda, int* ipivb,
+                                                   int* info, int j0, int n) {
+    using T2 = typename Cx<T>::type;
+    static_assert(kPanel % W == 0, "sub-panel width must divide the panel");
+    const int e = blockIdx.x, tid = threadIdx.x;
+    T2* A = Ab + e * sa;
+    int* ipiv = ipivb + static_cast<size_t>(e) * n;
+    __shared__ T2 wrow[2][16][W], crow2[2][W];
+    __shared__ T2 u12[W][kPanel];
+    __shared__ T wbest2[2][16];
+    __shared__ int widx2[2][16];
+    __shared__ int pv[W];
+    const int jend = j0 + kPanel;
+    T2 P[Q][W];
+    for (int c0 = j0; c0 < jend; c0 += W) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = j0 + tid + 1024 * q;
+            if (r < n) {
+                const T2* src = A + static_cast<size_t>(r) * lda + c0;
+#pragma unroll
+                for (int c = 0; c < W; ++c) P[q][c] = src[c];
+            }
+        }
+        // candidate pivot of this thread's rows for the first column
+        T best = -1;
+        int bi = c0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = j0 + tid + 1024 * q;
+            if (r >= c0 && r < n) {
+                const T m = fabs(P[q][0].x) + fabs(P[q][0].y);
+                if (m > best) {
+                    best = m;
+                    bi = r;
+                }
+            }
+        }
+        // One barrier per column: each wave reduces its candidates and its
+        // best row goes to LDS with the row at the pivot position; after the
+        // barrier every wave finishes the argmax over the 16 wave results
+        // itself.  Buffers alternate by column so the next column's writes
+        // cannot meet this column's reads.
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const int col = c0 + j, buf = j & 1;
+            const int lane = tid & 63, wave = tid >> 6;
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                wbest2[buf][wave] = best;
+                widx2[buf][wave] = bi;
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int r = j0 + tid + 1024 * q;
+                if (best > 0 && r == bi)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) wrow[buf][wave][c] = P[q][c];
+                if (r == col)
+#pragma unroll
+                    for (int c = 0; c < W; ++c) crow2[buf][c] = P[q][c];
+            }
+            __syncthreads();
+            T bb = lane < 16 ? wbest2[buf][lane] : T(-1);
+            int ii = lane < 16 ? widx2[buf][lane] : 0x7fffffff;
+            for (int off = 8; off > 0; off >>= 1) {
+                const T ob = __shfl_xor(bb, off, 64);
+                const int oi = __shfl_xor(ii, off, 64);
+                if (ob > bb || (ob == bb && oi < ii)) {
+                    bb = ob;
+                    ii = oi;
+                }
+            }
+            bb = __shfl(bb, 0, 64);
+            ii = __shfl(ii, 0, 64);
+            const bool singular = !(bb > 0);
+            const int p = singular ? col : ii;  // exact zero column: no interchange, LAPACK-style info
+            const T2* prow = singular ? crow2[buf] : wrow[buf][((p - j0) & 1023) >> 6];
+            if (tid == 0) {
+                ipiv[col] = p;
+                pv[j] = p;
+                if (singular && info[e] == 0) info[e] = col + 1;
+            }
+            if (p != col) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int r = j0 + tid + 1024 * q;
+                    if (r == p)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) P[q][c] = crow2[buf][c];
+                    if (r == col)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) P[q][c] = prow[c];
+                }
+            }
+            const T2 d = prow[j];
+            const bool nz = (d.x != 0 || d.y != 0);
+            const T2 inv = nz ? cdiv(T2{1, 0}, d) : T2{0, 0};
+            best = -1;
+            bi = col + 1;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int r = j0 + tid + 1024 * q;
+                if (r > col && r < n) {
+                    const T2 l = nz ? cmul(P[q][j], inv) : P[q][j];
+                    P[q][j] = l;
+#pragma unroll
+                    for (int c = j + 1; c < W; ++c) P[q][c] = cfms(P[q][c], l, prow[c]);
+                    if (j + 1 < W) {
+                        const T m = fabs(P[q][j + 1].x) + fabs(P[q][j + 1].y);
+                        if (m > best) {
+                            best = m;
+                            bi = r;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = j0 + tid + 1024 * q;
+            if (r >= c0 && r < n) {
+                T2* dst = A + static_cast<size_t>(r) * lda + c0;
+#pragma unroll
+                for (int c = 0; c < W; ++c) dst[c] = P[q][c];
+            }
+        }
+        __syncthreads();
+        // this sub-panel's interchanges on the panel columns left and right of
+        // it; right of it also the unit-lower solve with its diagonal block
+        const int R = jend - (c0 + W);
+        if (tid < kPanel && (j0 + tid < c0 || j0 + tid >= c0 + W)) {
+            const int c = j0 + tid;
+            for (int jj = 0; jj < W; ++jj) {
+                const int pr = pv[jj];
+                if (pr != c0 + jj) {
+                    T2* a0 = A + static_cast<size_t>(c0 + jj) * lda + c;
+                    T2* a1 = A + static_cast<size_t>(pr) * lda + c;
+                    const T2 x = *a0;
+                    *a0 = *a1;
+                    *a1 = x;
+                }
+            }
+            if (c >= c0 + W) {
+                T2 u[W];
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    const T2* lrow = A + static_cast<size_t>(c0 + i) * lda;
+                    u[i] = lrow[c];
+#pragma unroll
+                    for (int k = 0; k < i; ++k) u[i] = cfms(u[i], lrow[c0 + k], u[k]);
+                }
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    A[static_cast<size_t>(c0 + i) * lda + c] = u[i];
+                    u12[i][c - (c0 + W)] = u[i];
+                }
+            }
+        }
+        if (R == 0) break;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = j0 + tid + 1024 * q;
+            if (r >= c0 + W && r < n) {
+                T2* row = A + static_cast<size_t>(r) * lda + c0 + W;
+                for (int cb = 0; cb < R; cb += W) {
+                    T2 v[W];
+#pragma unroll
+                    for (int c = 0; c < W; ++c) v[c] = row[cb + c];
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+#pragma unroll
+                        for (int c = 0; c < W; ++c) v[c] = cfms(v[c], P[q][i], u12[i][cb + c]);
+#pragma unroll
+                    for (int c = 0; c < W; ++c) row[cb + c] = v[c];
+                }
+            }
+        }
+    }
+}
+
 // ---- Tournament-pivoting panel (CALU), FP32 LU of the mixed solver ----
 //
 // Panel j0 (rows [j0, n) x kPanel columns) in three steps, spread over many
@@ -3457,9 +3657,9 @@ void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, typename Cx<T>::typ
                 hipLaunchKernelGGL(k_calu_apply<float>, dim3((m - kPanel + 255) / 256, nb), dim3(256), 0, s, A, sa, lda,
                                    j0, n);
         } else if (m <= 3072)
-            hipLaunchKernelGGL((k_panel_rb<T, 8, 3>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+            hipLaunchKernelGGL((k_panel_rb1<T, 8, 3>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 4096)
-            hipLaunchKernelGGL((k_panel_rb<T, 4, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+            hipLaunchKernelGGL((k_panel_rb1<T, 4, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 8192)
             hipLaunchKernelGGL((k_panel_rb<T, 2, 8>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 10240)
@@ -3470,9 +3670,9 @@ void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, typename Cx<T>::typ
             hipLaunchKernelGGL(k_panel<T>, grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
     } else {
         if (m <= 2048)
-            hipLaunchKernelGGL((k_panel_rb<T, 4, 2>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+            hipLaunchKernelGGL((k_panel_rb1<T, 4, 2>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 4096)
-            hipLaunchKernelGGL((k_panel_rb<T, 2, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
+            hipLaunchKernelGGL((k_panel_rb1<T, 2, 4>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 8192)
             hipLaunchKernelGGL((k_panel_rb<T, 1, 8>), grid, block, 0, s, A, sa, lda, ipiv, info, j0, n);
         else if (m <= 10240)
