@@ -515,6 +515,33 @@ def test_module_stage_rejects_unprovided_debye_models_before_solving(tmp_path, i
     assert e.value.name == "UNSUPPORTED" and f"idwopt={idwopt}" in str(e.value)
 
 
+def test_module_mkgtr_reads_eels_inp_before_any_device_work(tmp_path):
+    """eels.inp is parsed header by header (refeff-io eels_input.rs:195-262) and
+    its polarization range checked (engine fms.rs:5619-5634) before MKGTR reads
+    phase.bin or solves anything."""
+    _write(tmp_path, "fms.inp", LEGACY_FMS_INP)
+    _write(tmp_path, "gg.bin", GG_DAT)
+    for name in ("phase.bin", "global.inp"):
+        _write(tmp_path, name, "")
+    good = ("calculate ELNES?\n   1\naverage? relativistic? cross-terms? Which input?\n   0   1   1   1   4\n"
+            "polarizations to be used ; min step max\n   1   1   9\nbeam energy in eV\n 300000.00000\n"
+            "beam direction in arbitrary units\n      0.00000      1.00000      0.00000\n"
+            "collection and convergence semiangle in rad\n      0.00240      0.00000\n"
+            "qmesh - radial and angular grid size\n   5   3\ndetector positions - two angles in rad\n"
+            "      0.00000      0.00000\ncalculate magic angle if magic=1\n   0\n"
+            "energy for magic angle - eV above threshold\n      0.00000\n")
+    cases = [(good.replace("calculate ELNES?", "calculate ELNES"), "expected header"),
+             (good.replace("   0   1   1   1   4", "   0   1   1"), "EELS control line requires 5 fields"),
+             (good.replace("   1   1   9", "   0   1   9"), "1 <= min <= max <= 10"),
+             (good.replace("   1   1   9", "   2   0   9"), "step > 0"),
+             (good.rsplit("energy for magic", 1)[0], "expected")]
+    for text, msg in cases:
+        _write(tmp_path, "eels.inp", text)
+        with pytest.raises(CodecError) as e:
+            io.run_mkgtr_in_dir(tmp_path)
+        assert msg in str(e.value), (msg, str(e.value))
+
+
 # --- gg_slice.bin / gg_diag.bin: crates/refeff-io/src/rhorrp_gg_bin.rs tests --
 
 # FEFF reference bytes of rhorrp_gg_bin.rs:785-809 (FEFF_GG_SLICE_BYTES, FEFF_GG_DIAG_BYTES)
