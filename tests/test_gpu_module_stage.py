@@ -484,24 +484,33 @@ def make_screen_case(d, ne=6, lmaxph=(3, 3), idwopt=-1, tk=0.0, thetad=0.0, sig2
     return pts, iph, k_bohr, phl, sig2g
 
 
-@pytest.mark.parametrize("lmaxph,idwopt", [((3, 3), -1), ((3, 2), -1), ((3, 3), 0)])
+@pytest.mark.parametrize("lmaxph,idwopt", [((3, 3), -1), ((3, 2), -1), ((3, 3), 0), ((3, 3), 3), ((3, 3), 4)])
 def test_screen_traces_match_oracle(tmp_path, lmaxph, idwopt):
     """SCREEN's FMS traces (engine fms.rs:2464-2648): the caller's radius and
     cutoff replace fms.inp's, fms.inp's damping and tolerances stay, minv is
     ignored (LU), lfms 0 solves the absorber's block and its trace is returned."""
     banner(f"SCREEN traces lmaxph={lmaxph} idwopt={idwopt}")
-    tk, thetad, rnrmav, z = (300.0, 315.0, 2.6, [29, 29]) if idwopt == 0 else (0.0, 0.0, 0.0, None)
+    tk, thetad, rnrmav, z = (300.0, 315.0, 2.6, [29, 8]) if idwopt in (0, 3) else (0.0, 0.0, 0.0, None)
     pts, iph, k_bohr, phl, sig2g = make_screen_case(tmp_path, lmaxph=lmaxph, idwopt=idwopt, tk=tk, thetad=thetad)
     radius, rdirec = 5.2, 9.0
-    got, ip = io.screen_fms_traces(tmp_path, k_bohr, phl, 4, radius, rdirec, atomic_numbers=z,
-                                   average_norman_radius=rnrmav)
-    assert ip == 0 and got.shape == (len(k_bohr), 4)
     pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(radius), len(iph))
     pos, pot, _ = orc.sort_representative_atoms(0, 1, pos, pot)
     nat = len(pot)
-    if idwopt == 0:
-        sig = orc.debye_sigsqr(0, tk, thetad, rnrmav, pos, np.array(z, np.int32)[pot], np.float32(sig2g))
-    else:
+    if idwopt == 4:  # sig2.dat rows of the cluster's pairs (engine fms.rs:4353-4358)
+        vals = np.random.default_rng(23).uniform(0.0, 0.01, nat * (nat - 1) // 2)
+        rows, sig, i = ["# atom1 atom2 sig2 amplitude"], np.zeros((nat, nat), np.float32), 0
+        for a1 in range(nat - 1):
+            for a2 in range(a1 + 1, nat):
+                rows.append(f"{a1 + 1} {a2 + 1} {float(vals[i])!r} 1.0")
+                sig[a2, a1] = sig[a1, a2] = np.float32(float(np.float32(sig2g)) + vals[i])
+                i += 1
+        (tmp_path / "sig2.dat").write_text("\n".join(rows) + "\n")
+    got, ip = io.screen_fms_traces(tmp_path, k_bohr, phl, 4, radius, rdirec, atomic_numbers=z,
+                                   average_norman_radius=rnrmav)
+    assert ip == 0 and got.shape == (len(k_bohr), 4)
+    if idwopt in (0, 3):
+        sig = orc.debye_sigsqr(idwopt, tk, thetad, rnrmav, pos, np.array(z, np.int32)[pot], np.float32(sig2g))
+    elif idwopt < 0:
         sig = np.full((nat, nat), np.float32(sig2g), np.float32)
         np.fill_diagonal(sig, 0.0)
     L = max(lmaxph)
