@@ -3438,6 +3438,7 @@ struct refeff_fms_plan {
     int* d_cand = nullptr;              // tournament-pivoting candidates: two [e][calu_cap] lists
     int calu_cap = 0;                   // candidate list length per energy
     bool panel_calu = true;             // REFEFF_FMS_PANEL=gepp: FP32 panels by partial pivoting
+    bool lu_split = false;              // REFEFF_FMS_LU_SPLIT=1: outer blocks factored as two halves
     int calu_min_rows = 3072;           // tournament pivoting for panels of at least this many rows
     double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_norms = nullptr;
@@ -3796,14 +3797,40 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
     bool far_pending = false;
     for (int k0 = 0; k0 < n; k0 += kOuter) {
         const int kend = std::min(k0 + kOuter, n);
-        for (int j0 = k0; j0 < kend; j0 += kPanel) {
+        auto panel = [&](int j0) {
             const double m = n - j0;
             launch(p, g.s, REFEFF_FMS_K_PANEL, 8.0 * m * kPanel * kPanel / 2 * nb, 2.0 * m * kPanel * cb * nb, [&] {
                 launch_panel<T>(p, g.s, nb, A, sa, n, ipiv, info, j0, n);
             });
-            swap_trsm<T>(p, g, A, ipiv, j0, k0, kend, 1, j0 + kPanel);
+        };
+        // In-block steps.  Split: the block's left half is factored first with
+        // its right half treated like the far columns (interchanges only, then
+        // its U12 rows in 32-row steps and one K = half update of the rows
+        // below), so the right half is read and written once per half instead
+        // of once per 32-column panel.
+        const int half = ((kend - k0) / 2 / kPanel) * kPanel;
+        const bool split = p->lu_split && half >= 2 * kPanel && kend - k0 - half >= kPanel;
+        const int km = split ? k0 + half : kend;
+        for (int j0 = k0; j0 < km; j0 += kPanel) {
+            panel(j0);
+            swap_trsm<T>(p, g, A, ipiv, j0, k0, km, 1, j0 + kPanel);
+            if (split) swap_trsm<T>(p, g, A, ipiv, j0, km, kend, 1, kend);
             gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, j0 + kPanel), n,
-                        sa, n - j0 - kPanel, kend - j0 - kPanel, kPanel);
+                        sa, n - j0 - kPanel, km - j0 - kPanel, kPanel);
+        }
+        if (split) {
+            for (int j0 = k0; j0 < km; j0 += kPanel) {
+                swap_trsm<T>(p, g, A, ipiv, j0, km, kend, 0, km);
+                gemm_sub<T>(p, g, at(j0 + kPanel, km), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, km), n, sa,
+                            km - j0 - kPanel, kend - km, kPanel);
+            }
+            gemm_sub<T>(p, g, at(km, km), n, sa, at(km, k0), n, sa, at(k0, km), n, sa, n - km, kend - km, half);
+            for (int j0 = km; j0 < kend; j0 += kPanel) {
+                panel(j0);
+                swap_trsm<T>(p, g, A, ipiv, j0, k0, kend, 1, j0 + kPanel);
+                gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, j0 + kPanel),
+                            n, sa, n - j0 - kPanel, kend - j0 - kPanel, kPanel);
+            }
         }
         if (far_pending) {
             (void)hipStreamWaitEvent(g.s, g.ev_far, 0);
@@ -4451,6 +4478,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     // block halves the passes over the trailing matrix but grows the in-block
     // K = 32 work (N = 2832: 192 best; N = 9744: 256 best)
     p->outer_block = p->npad >= 4096 ? 256 : p->npad >= 2048 ? 192 : 128;
+    if (const char* e = std::getenv("REFEFF_FMS_LU_SPLIT")) p->lu_split = std::atoi(e) != 0;
     if (const char* ob = std::getenv("REFEFF_FMS_LU_BLOCK")) {
         const int v = std::atoi(ob);
         if (v >= kPanel && v % kPanel == 0 && v <= 1024) p->outer_block = v;
