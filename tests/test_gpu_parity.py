@@ -188,13 +188,15 @@ def test_mkgtr_trace_on_gpu_matches_oracle():
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("bf16", ["0", "2", "3", "16"])
-def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16):
+@pytest.mark.parametrize("bf16,split", [("0", "0"), ("2", "0"), ("3", "0"), ("16", "0"), ("0", "1")])
+def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16, split):
     """Cu fcc R=8.0 (177 atoms, N=2832) at full size: the two device solvers
     (FP32 LU + FP64 refinement vs plain FP64 LU) agree to the gg gate, also with
-    the trailing updates on split-bf16 MFMA (two or three parts)."""
+    the trailing updates on split-bf16 MFMA (two or three parts) and with the
+    outer blocks factored as two halves (both LUs)."""
     monkeypatch.setenv("REFEFF_FMS_GEMM_BF16", bf16)
-    banner(f"Cu fcc R=8.0 N=2832 mixed (bf16 split {bf16}) vs fp64 (6 energies)")
+    monkeypatch.setenv("REFEFF_FMS_LU_SPLIT", split)
+    banner(f"Cu fcc R=8.0 N=2832 mixed (bf16 split {bf16}, LU halves {split}) vs fp64 (6 energies)")
     wl = W.fcc_cu(8.0, 120)
     sel = [0, 20, 47, 80, 101, 119]
     k, ph = wl.wave_numbers()[sel], wl.phase_shifts()[sel]
