@@ -41,9 +41,10 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip the plain FP64 LU comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle accuracy sample")
     ap.add_argument("--cpu-energies", type=int, default=6)
-    ap.add_argument("--caller", default=None, choices=["pot_scf", "screen"],
+    ap.add_argument("--caller", default=None, choices=["pot_scf", "screen", "ldos"],
                     help="time a caller of the FMS stage instead of the stage itself (one JSON line of its "
-                         "own): pot_scf = the POT self-consistency traces, screen = SCREEN's absorber traces")
+                         "own): pot_scf = the POT self-consistency traces, screen = SCREEN's absorber traces, "
+                         "ldos = the LDOS gtrNN.bin files on the LDOS stage's own energy grid")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy (the reference kernel for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling under rocprofv3 --pmc)")
@@ -303,6 +304,96 @@ def run_screen_caller(args):
     print(json.dumps(line), flush=True)
 
 
+# global.inp of an orientation-averaged, spin-unpolarized K-edge run (the layout
+# refeff-io's global_input.rs writes)
+GLOBAL_INP = """\
+ nabs, iphabs - CFAVERAGE data
+       1       0 100000.00000
+ ipol, ispin, le2, elpty, angks, l2lp, do_nrixs, ldecmx, lj
+    0    0    0      0.0000      0.0000    0    0   -1   -1
+evec\t\t  xivec \t   spvec
+      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000
+ polarization tensor
+      0.33333      0.00000      0.00000      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.33333      0.00000      0.00000      0.00000
+      0.00000      0.00000      0.00000      0.00000      0.33333      0.00000
+evnorm, xivnorm, spvnorm - only used for nrixs
+      0.00000      0.00000      0.00000
+nq,    imdff,   qaverage,   mixdff
+           0           0 T F
+ q-vectors : qx, qy, qz, q(norm), weight, qcosth, qsinth, qcosfi, qsinfi
+"""
+
+
+def run_ldos_caller(args):
+    """LDOS gtrNN.bin on the LDOS stage's RHORRP grid (refeff
+    write_ldos_gtr_bin_source_grid_handoff): one cluster per potential, whole
+    calls timed including the file reads and the two gtrNN.bin writes."""
+    import tempfile
+    from refeff_amd import io, workloads as W
+    from oracle import oracle as orc
+    radius, rdirec, ne, sig2g, a0 = 6.0, 12.0, 48, 0.003, 0.529177249
+    d = tempfile.mkdtemp(prefix="ldos_")
+    pts, iph, _, _, _, _ = pot_scf_case(d, radius, 1, [3, 3])
+    e_ph = np.linspace(0.3, 1.6, 8) + 0.02j  # phase.bin's own grid: metadata only on this path
+    ph_model = W.synthetic_phases(np.sqrt(2.0 * e_ph) / a0, 3, np.array([3, 3]))
+    pots = [io.PhaseBinPotential(3, 29, "Cu", ph_model[:, 0, :, ip][:, :, None].copy()) for ip in range(2)]
+    pb = io.PhaseBinData(1, 8, 8, 0, 1, 1, 8, 8, 8, 1, (2.6, -0.14, -0.14), e_ph, np.zeros((8, 1), np.complex128),
+                         pots, np.zeros((8, 1, 8, 1), np.complex128))
+    fin = io.FmsInput(mfms=1, idwopt=-1, minv=0, rfms2=radius, rdirec=rdirec, toler1=0.001, toler2=0.001, tk=0.0,
+                      thetad=0.0, sig2g=sig2g, lmaxph=[3, 3], decomposition_channels=-1, save_gg_slice=False,
+                      do_fms=1)
+    ldos = dict(mldos=1, lfms2=0, ixc=0, ispin=0, minv=0, neldos=ne, iscfxc=0, rfms2=radius, emin=-10.0, emax=30.0,
+                eimag=0.5, rgrd=0.05, rdirec=rdirec, toler1=0.001, toler2=0.001, lmaxph=[3, 3], ldostype=0)
+    for name, text in [("phase.bin", io.phase_bin_string(pb)), ("fms.inp", io.fms_input_string(fin)),
+                       ("global.inp", GLOBAL_INP), ("ldos.inp", io.ldos_input_string(ldos))]:
+        with open(os.path.join(d, name), "w") as f:
+            f.write(text)
+    e = np.linspace(0.3, 1.6, ne) + 0.02j
+    k_bohr = np.stack([np.sqrt(2.0 * e), np.sqrt(2.0 * (e - 0.04))], axis=1)
+    phl = np.ascontiguousarray(W.synthetic_phases(k_bohr[:, 0] / a0, 3, np.array([3, 3]))[:, 0, 3:, :])
+    for _ in range(args.warmup):
+        io.run_ldos_gtr_source_grid_in_dir(d, k_bohr, phl)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        supported, written = io.run_ldos_gtr_source_grid_in_dir(d, k_bohr, phl)
+    dt = (time.perf_counter() - t0) / args.steps
+    assert supported and written == 2
+    line = {"metric": "LDOS gtrNN.bin energy-pts/s", "value": round(ne / dt, 3), "unit": "energy-points/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64 LU + f64 refinement",
+            "data": "synthetic (Cu fcc geom.dat, smooth model phase shifts)",
+            "config": {"workload": f"ldos Cu fcc rfms2={radius}, 2 central potentials, {ne} energies, lmaxph=[3, 3], "
+                                   "lfms2=0"}}
+    if not args.no_cpu:
+        cores = min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        sample = sorted(set(np.linspace(0, ne - 1, cores).astype(int).tolist()))
+        signed = np.zeros((len(sample), 1, 7, 2), np.complex128)
+        for sl in range(-3, 4):
+            signed[:, 0, 3 + sl, :] = phl[sample, abs(sl), :]
+        t0 = time.perf_counter()
+        pos, pot, _, _ = orc.yprep_cluster(pts.astype(np.float32), iph, 0, np.float32(radius), len(iph))
+        nat = len(pot)
+        sig = np.full((nat, nat), np.float32(sig2g), np.float32)
+        np.fill_diagonal(sig, 0.0)
+        plan = orc.Plan(pos, pot, nsp=1, lfms=0, max_pot=1, global_lmax=3, raw_lmax=[3, 3], rdirec=rdirec,
+                        sigsqr=sig)
+        sc, st = plan.spectrum((k_bohr[sample, 0] / a0).reshape(-1, 1), signed, threads=len(sample))
+        t_cpu = time.perf_counter() - t0
+        want = orc.ldos_fmsdos_trace_grid(sc, signed, 4)[:, 0, :]
+        with open(os.path.join(d, "gtr00.bin"), "rb") as f:
+            got = io.parse_gtr_bin(f.read())[0][sample, 0, :]
+        line["cpu_baseline"] = {"value": round(len(sample) / t_cpu, 4), "unit": "energy-points/s",
+                                "cores": len(sample), "kind": "port",
+                                "sample": f"{len(sample)} of {ne} energies, central potential 0 only; oracle FP64 "
+                                          f"G0/t assembly + LU, one energy per thread"}
+        line["accuracy"] = {"max_gtr00_rel_err_vs_fp64_oracle": float(np.abs(got - want).max() / np.abs(want).max()),
+                            "energies_checked": len(sample), "gate": 2e-6, "note": "gtrNN.bin holds f32"}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of a known size: rocprofv3 --pmc reads FETCH_SIZE / WRITE_SIZE
     for it, and the ratio to nbytes scales the counters of the FMS kernels."""
@@ -329,6 +420,9 @@ def main():
         return
     if args.caller == "screen":
         run_screen_caller(args)
+        return
+    if args.caller == "ldos":
+        run_ldos_caller(args)
         return
     if args.copy_calibration:
         copy_calibration()
