@@ -410,8 +410,8 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one MI355X only; multi-GPU sharding over energies is "
-                                   "described in DESIGN.md and not run", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark measures one MI355X only; sharding the energies over several "
+                                   "GPUs is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
     from refeff_amd import lib, workloads as W
     L = lib()

@@ -8,7 +8,7 @@ yprep/sortat cluster from the same f32 geometry, k = sqrt(2 (E - E_ref)) / a0
 and the phase shifts from the arrays written into phase.bin (PAD at npack 8
 keeps ~1e-12 relative), σ² = sig2g between distinct atoms (or the oracle's
 Debye / sig2.dat σ² for idwopt 0, 3, 4).  gg.bin's E20.10
-fields carry 10 significant digits, so the gate is the north-star 1e-8."""
+fields carry 10 significant digits, so the gate is the gg accuracy gate, 1e-8."""
 import dataclasses
 import sys
 

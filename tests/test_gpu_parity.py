@@ -1,7 +1,7 @@
 """GPU parity: the HIP path of librefeff_fms.so against the FP64 oracle.
 
 Bar: relative Frobenius error of every energy's gg block <= 1e-8 against the
-oracle on the same inputs (the north-star gg.bin gate), for both the mixed
+oracle on the same inputs (the gg.bin accuracy gate), for both the mixed
 FP32-LU + FP64-refinement solver and the plain FP64 LU.  Larger clusters are
 checked through size-independent properties (mixed vs FP64 agreement, batch
 independence)."""
