@@ -443,6 +443,57 @@ int orc_rehr_albers(int lx, int lmaxp1, int mmaxp1, double rho_re, double rho_im
 }
 
 /* ------------------------------------------------------------------------ */
+/* z-axis propagator: FEFF xgllm, refeff fms/xgllm.rs:9-57 (and the inlined    */
+/* copy in fms/pairs.rs:519-553).  For a bond along z, the Rehr-Albers term    */
+/*   g_mu(l1, l2) = sum_{nu=0}^{min(l1, l2-mu)} [(2 l1 + 1) c_{l1}^nu / N_{mu l1}]  */
+/*                  * [(-1)^mu N_{mu l2} c_{l2}^{mu+nu}],   0 when mu > l2,     */
+/* with c = the xclmz table above (clm(il, im) at il + (lx+2)*im) and        */
+/* N_{mu l} = sqrt((2l+1)(l-mu)!/(l+mu)!).  Every G0 element of the oracle goes */
+/* through this one function.                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Test-only planted mistake (CPU): 1 swaps N_{mu l1} and N_{mu l2} in the
+ * weight above, so the tests can show that their pins reject that mistake. */
+static int orc_fault = 0;
+void orc_set_fault(int fault) { orc_fault = fault; }
+
+static cplx z_axis_propagator(int lx, const cplx* clm, int mu, int l1, int l2) {
+    if (mu > l2) return 0.0;
+    int rows = lx + 2;
+    int numax = l1 < (l2 - mu) ? l1 : (l2 - mu);
+    double sgn = (mu % 2 == 0) ? 1.0 : -1.0;
+    double n1 = orc_xnlm(l1, mu), n2 = orc_xnlm(l2, mu);
+    if (orc_fault == 1) {
+        double t = n1;
+        n1 = n2;
+        n2 = t;
+    }
+    cplx g = 0.0;
+    for (int nu = 0; nu <= numax; ++nu) {
+        cplx gamtl = (double)(2 * l1 + 1) * clm[l1 + (size_t)rows * nu] / n1;
+        cplx gam = clm[l2 + (size_t)rows * (mu + nu)] * (sgn * n2);
+        g += gamtl * gam;
+    }
+    return g;
+}
+
+/* g_mu(l1, l2) at rho from a fresh xclmz(lx, lmaxp1, mmaxp1, rho) table. */
+int orc_z_axis_propagator(int lx, int lmaxp1, int mmaxp1, double rho_re, double rho_im, int mu, int l1, int l2,
+                          double* out) {
+    if (mu < 0 || l1 < 0 || l2 < 0 || mu > l1 || l1 > lx || l2 > lx) return ORC_INVALID_ARGUMENT;
+    cplx* clm = malloc(sizeof(cplx) * (size_t)(lx + 2) * (2 * lx + 3));
+    if (!clm) return ORC_NO_MEMORY;
+    int rc = orc_rehr_albers(lx, lmaxp1, mmaxp1, rho_re, rho_im, (double*)clm);
+    if (!rc) {
+        cplx g = z_axis_propagator(lx, clm, mu, l1, l2);
+        out[0] = creal(g);
+        out[1] = cimag(g);
+    }
+    free(clm);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* plan                                                                       */
 /* ------------------------------------------------------------------------ */
 
@@ -521,9 +572,6 @@ static int g0_matrix(const orc_plan* p, const orc_states* st, const cplx* k, cpl
     cplx* dbw = malloc(sizeof(cplx) * dsz);
     cplx* clm = malloc(sizeof(cplx) * (size_t)(L + 2) * (2 * L + 3));
     if (!dfw || !dbw || !clm) { free(first); free(dfw); free(dbw); free(clm); return ORC_NO_MEMORY; }
-    double xn[ORC_LX + 1][ORC_LX + 1];
-    for (int mu = 0; mu <= L; ++mu)
-        for (int l = 0; l <= L; ++l) xn[mu][l] = orc_xnlm(l, mu);
     float cut2 = p->rdirec * p->rdirec;
     int rc = ORC_OK;
     for (int a = 0; a < nat && rc == ORC_OK; ++a) {
@@ -565,14 +613,7 @@ static int g0_matrix(const orc_plan* p, const orc_states* st, const cplx* k, cpl
                                 for (int mu = -l1; mu <= l1; ++mu) {
                                     int amu = abs(mu);
                                     if (amu > l2) continue;
-                                    int numax = l1 < (l2 - amu) ? l1 : (l2 - amu);
-                                    cplx g = 0.0;
-                                    double sgn = (amu % 2 == 0) ? 1.0 : -1.0;
-                                    for (int nu = 0; nu <= numax; ++nu) {
-                                        cplx gamtl = (double)(2 * l1 + 1) * clm[l1 + (size_t)(L + 2) * nu] / xn[amu][l1];
-                                        cplx gam = clm[l2 + (size_t)(L + 2) * (amu + nu)] * (sgn * xn[amu][l2]);
-                                        g += gamtl * gam;
-                                    }
+                                    cplx g = z_axis_propagator(L, clm, amu, l1, l2);
                                     cplx bw = dbw[((size_t)l1 * nm + (m1 + L)) * nm + (mu + L)];
                                     cplx fw = dfw[((size_t)l2 * nm + (mu + L)) * nm + (m2 + L)];
                                     sum += bw * g * fw;
@@ -1588,12 +1629,7 @@ int orc_g0_element(int lmax, int l1, int m1, int l2, int m2, double rho_re, doub
     for (int mu = -l1; mu <= l1; ++mu) {
         int amu = abs(mu);
         if (amu > l2) continue;
-        int numax = l1 < (l2 - amu) ? l1 : (l2 - amu);
-        double sgn = (amu % 2 == 0) ? 1.0 : -1.0;
-        cplx g = 0.0;
-        for (int nu = 0; nu <= numax; ++nu)
-            g += (double)(2 * l1 + 1) * clm[l1 + (size_t)(lmax + 2) * nu] / orc_xnlm(l1, amu) *
-                 (clm[l2 + (size_t)(lmax + 2) * (amu + nu)] * (sgn * orc_xnlm(l2, amu)));
+        cplx g = z_axis_propagator(lmax, clm, amu, l1, l2);
         sum += dbw[((size_t)l1 * nm + (m1 + lmax)) * nm + (mu + lmax)] * g *
                dfw[((size_t)l2 * nm + (mu + lmax)) * nm + (m2 + lmax)];
     }

@@ -171,6 +171,28 @@ def rehr_albers(lx, lmaxp1, mmaxp1, rho):
     return out.view(np.complex128).reshape(cols, rows).T.copy()
 
 
+_lib.orc_z_axis_propagator.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       _f64p]
+_lib.orc_set_fault.argtypes = [C.c_int]
+
+
+def z_axis_propagator(lx, lmaxp1, mmaxp1, rho, mu, l1, l2):
+    """FEFF xgllm g_mu(l1, l2) on a fresh xclmz(lx, lmaxp1, mmaxp1, rho) table."""
+    out = np.zeros(2)
+    _check(_lib.orc_z_axis_propagator(lx, lmaxp1, mmaxp1, float(np.real(rho)), float(np.imag(rho)), mu, l1, l2, out),
+           "xgllm")
+    return complex(out[0], out[1])
+
+
+FAULT_NONE, FAULT_SWAP_NORMS = 0, 1
+
+
+def set_fault(fault: int) -> None:
+    """Plant a known mistake in the oracle (tests only): FAULT_SWAP_NORMS swaps
+    N_{mu l1} and N_{mu l2} in the z-axis propagator's weight."""
+    _lib.orc_set_fault(int(fault))
+
+
 def state_kets(nsp, atom_pot, raw_lmax, global_lmax):
     atom_pot = np.ascontiguousarray(atom_pot, dtype=np.int32)
     raw = np.ascontiguousarray(raw_lmax, dtype=np.int32)

@@ -234,6 +234,100 @@ def test_pair_tables_kat():
     close(tab[(2, 1)][2, 1], -0.15384735 + 0.9149786j)
 
 
+# --------------------------------------------------------------------------
+# z-axis propagator (xgllm): refeff fms/tests/xgllm.rs:4-55, on the xclmz table
+# of fms/tests.rs:50-60 (lx 3, lmaxp1 = mmaxp1 = 4, rho = 1.25 + 0.4i)
+# --------------------------------------------------------------------------
+
+XGLLM_RHO = 1.25 + 0.4j
+
+
+def test_xgllm_kat():
+    close(orc.z_axis_propagator(3, 4, 4, XGLLM_RHO, 0, 2, 3), 415.5469 - 1006.2809j)
+    close(orc.z_axis_propagator(3, 4, 4, XGLLM_RHO, 1, 2, 3), -307.4973 + 722.4695j)
+    close(orc.z_axis_propagator(3, 4, 4, XGLLM_RHO, 2, 2, 3), 115.08963 - 235.94589j)
+    # empty sum: mu > l2 (xgllm.rs:35-55)
+    assert orc.z_axis_propagator(3, 4, 4, XGLLM_RHO, 2, 3, 1) == 0
+
+
+def _translation_coefficient(l1, l2, mu, k, dist):
+    """Addition theorem by quadrature: h_l2(k|r - R|) Y_{l2 mu}(r - R), R = dist * z,
+    expanded about the origin as sum_l1 A_{l1} j_l1(k r) Y_{l1 mu}(r^) (|r| < dist).
+    A_{l1} is projected out on a small sphere with Gauss-Legendre in theta (the
+    e^{i mu phi} factor separates).  Independent of FEFF's Rehr-Albers algebra."""
+    from scipy.special import spherical_jn, spherical_yn, sph_harm_y
+    r = 0.05 * dist
+    x, w = np.polynomial.legendre.leggauss(80)
+    th = np.arccos(x)
+    dz, dx = r * np.cos(th) - dist, r * np.sin(th)
+    d = np.hypot(dx, dz)
+    h = spherical_jn(l2, k * d) + 1j * spherical_yn(l2, k * d)
+    f = h * sph_harm_y(l2, mu, np.arccos(dz / d), 0.0)
+    return 2 * np.pi * np.sum(w * f * np.conj(sph_harm_y(l1, mu, th, 0.0))) / spherical_jn(l1, k * r)
+
+
+def test_xgllm_equals_addition_theorem():
+    """e^{i rho}/rho g_mu(l1, l2) is the exact free-propagator translation
+    coefficient along z times i^(1 + l1 - l2), for every l1, l2 <= 4 and mu: the
+    i^l phases are FEFF's basis convention (l1 = l2 = 2 is refeff's G0 KAT,
+    pairs.rs:120-163), the magnitudes are fixed by the addition theorem alone."""
+    for k, dist in ((1.3 + 0.2j, 2.1), (2.9 + 0.05j, 2.55), (0.7 + 0.9j, 1.8)):
+        rho = k * dist
+        for l1 in range(5):
+            for l2 in range(5):
+                for mu in range(min(l1, l2) + 1):
+                    g = orc.z_axis_propagator(4, 5, 5, rho, mu, l1, l2) * np.exp(1j * rho) / rho
+                    a = _translation_coefficient(l1, l2, mu, k, dist) * 1j ** (1 + l1 - l2)
+                    assert abs(g - a) <= 1e-7 * abs(a), (l1, l2, mu, g, a)
+
+
+def _g0_reciprocity_error(plan, k):
+    """max |G0[(a,l1,m1),(b,l2,m2)] - (-1)^(l1+l2+m1+m2) G0[(b,l2,-m2),(a,l1,-m1)]| / max|G0|.
+    The outgoing free Green's function is symmetric, G(r, r') = G(r', r); in a
+    basis of Y_lm on one side and Y*_lm = (-1)^m Y_l,-m on the other, with
+    R_ab = -R_ba contributing the parity (-1)^(l1+l2), that is this relation."""
+    g = plan.g0(k)
+    a, l, m, s, _ = plan.states()
+    idx = {(int(a[i]), int(l[i]), int(m[i]), int(s[i])): i for i in range(len(a))}
+    err = 0.0
+    for (A, l1, m1, s1), i in idx.items():
+        for (B, l2, m2, s2), j in idx.items():
+            sign = (-1) ** (l1 + l2 + m1 + m2)
+            err = max(err, abs(g[i, j] - sign * g[idx[(B, l2, -m2, s2)], idx[(A, l1, -m1, s1)]]))
+    return err / np.abs(g).max()
+
+
+def _reciprocity_cluster():
+    rng = np.random.default_rng(11)
+    pos = np.concatenate([[[0, 0, 0]], rng.normal(size=(4, 3)) * 1.6]).astype(np.float32)
+    return orc.Plan(pos, [0, 1, 2, 1, 2], global_lmax=4, raw_lmax=[4, 3, 2])
+
+
+def test_g0_reciprocity_with_unequal_l():
+    plan = _reciprocity_cluster()
+    for k in (1.7 + 0.2j, 3.1 + 0.4j):
+        assert _g0_reciprocity_error(plan, k) < 1e-13
+
+
+def test_pins_reject_swapped_norms():
+    """A planted mistake in the oracle -- N_{mu l1} and N_{mu l2} swapped in the
+    z-axis weight, invisible at l1 = l2 -- fails the KAT, the addition theorem
+    and reciprocity."""
+    orc.set_fault(orc.FAULT_SWAP_NORMS)
+    try:
+        assert abs(orc.z_axis_propagator(3, 4, 4, XGLLM_RHO, 0, 2, 3) - (415.5469 - 1006.2809j)) > 1.0
+        rho = (1.3 + 0.2j) * 2.1
+        g = orc.z_axis_propagator(4, 5, 5, rho, 1, 3, 1) * np.exp(1j * rho) / rho
+        a = _translation_coefficient(3, 1, 1, 1.3 + 0.2j, 2.1) * 1j ** 3
+        assert abs(g - a) > 0.1 * abs(a)
+        assert _g0_reciprocity_error(_reciprocity_cluster(), 1.7 + 0.2j) > 1e-3
+        # the l1 = l2 KAT cannot see it
+        close(orc.g0_element(2, 2, 1, 2, -1, 3.0 * (1.2 + 0.3j), 1.2 + 0.3j, 0.05, 0.7, 0.7, 1.1),
+              -0.13408086 + 0.11388555j)
+    finally:
+        orc.set_fault(orc.FAULT_NONE)
+
+
 def test_free_propagator_element_kat():
     k = 1.2 + 0.3j
     v = orc.g0_element(2, 2, 1, 2, -1, 3.0 * k, k, 0.05, 0.7, 0.7, 1.1)
