@@ -156,6 +156,12 @@ typedef struct refeff_fms_plan_desc {
     int32_t full_scattering_requested;  /* save_gg_slice: also solve A^-1 G0 for every G0 column (forces LU) */
     int32_t solver;                     /* refeff_fms_solver */
     int32_t max_batch;                  /* energies per device batch; 0 = sized from free device memory */
+    int32_t state_capacity;             /* istatx-style state-ket capacity (FmsRealSpacePlanInput::state_capacity,
+                                           types.rs:126); 0 = no limit.  A larger state count fails with
+                                           REFEFF_FMS_INVALID_ARGUMENT, variant REFEFF_FMS_E_STATE_CAPACITY_EXCEEDED */
+    const uint8_t* calculated_l;        /* FEFF lcalc(l), [global_lmax+1] (types.rs:140); NULL = every l.  The
+                                           iterative branches (GGBI/GGRM/GGGM/GGTF) leave the source columns of
+                                           masked l at zero (internals/iterative.rs:138-143); the LU ignores it */
 } refeff_fms_plan_desc;
 
 typedef struct refeff_fms_plan_info {
@@ -173,7 +179,9 @@ typedef struct refeff_fms_plan_info {
 
 typedef struct refeff_fms_stats {
     double seconds_total;        /* wall time of the call */
-    double seconds_assemble;     /* t-matrix + G0 + (1 - G0 t) build */
+    double seconds_assemble;     /* t-matrix + G0 + (1 - G0 t) build: device time on the first energy
+                                    sub-batch's stream (all of it with one stream, the default);
+                                    seconds_factor excludes it */
     double seconds_factor;       /* LU */
     double seconds_solve;        /* triangular solves + refinement */
     int32_t refinement_sweeps;   /* largest number of FP64 refinement sweeps over the energies */
@@ -185,6 +193,54 @@ typedef struct refeff_fms_stats {
     int32_t krylov_iterations;   /* BICGSTAB: their iterations, summed */
     int32_t lu_energies;         /* BICGSTAB: energies the mixed LU solved */
 } refeff_fms_stats;
+
+/* Typed failure detail.  The status codes above are the FmsError families; this
+ * record carries the variant and its payload (crates/refeff-core/src/fms/types.rs:830-981,
+ * crates/refeff-linalg/src/error.rs:1-42), so a Rust wrapper can rebuild the
+ * exact FmsError value. */
+typedef enum refeff_fms_error_variant {
+    REFEFF_FMS_E_NONE = 0,
+    REFEFF_FMS_E_INVALID_ANGULAR_LIMIT = 1,      /* InvalidAngularLimit{name, value, lx}: values = value, lx */
+    REFEFF_FMS_E_EMPTY_CLUSTER = 2,              /* EmptyCluster */
+    REFEFF_FMS_E_NON_FINITE_COORDINATE = 3,      /* NonFiniteCoordinate{atom, axis}: values = atom, axis */
+    REFEFF_FMS_E_POTENTIAL_OUT_OF_RANGE = 4,     /* PotentialOutOfRange{potential, max_potential} */
+    REFEFF_FMS_E_NON_FINITE_MSD = 5,             /* NonFiniteMeanSquareDisplacement */
+    REFEFF_FMS_E_INVALID_DIRECT_CUTOFF = 6,      /* InvalidDirectCutoff */
+    REFEFF_FMS_E_INVALID_TOLERANCE = 7,          /* InvalidTolerance{name, value}: name, real_value */
+    REFEFF_FMS_E_INVALID_SPIN_CHANNEL_COUNT = 8, /* InvalidSpinChannelCount{value} */
+    REFEFF_FMS_E_STATE_CAPACITY_EXCEEDED = 9,    /* StateCapacityExceeded{capacity} */
+    REFEFF_FMS_E_MISSING_REPRESENTATIVE = 10,    /* MissingRepresentativePotential{potential} */
+    REFEFF_FMS_E_FULL_SCATTERING_REQUIRES_LU = 11, /* FullScatteringRequiresLu{method}: values = minv */
+    REFEFF_FMS_E_NON_FINITE_WAVE_NUMBER = 12,    /* NonFiniteWaveNumber */
+    REFEFF_FMS_E_NON_FINITE_PHASE_SHIFT = 13,    /* NonFinitePhaseShift{spin, angular_momentum, potential} */
+    REFEFF_FMS_E_SINGULAR_MATRIX = 14,           /* LinearAlgebra(SingularMatrix{pivot}): values = pivot (0-based) */
+    REFEFF_FMS_E_ITERATIVE_BREAKDOWN = 15,       /* IterativeSolverBreakdown{solver, step}: name = solver */
+    REFEFF_FMS_E_ITERATIVE_NO_CONVERGENCE = 16,  /* IterativeSolverNoConvergence{solver, restarts} */
+    REFEFF_FMS_E_REFINEMENT_NO_CONVERGENCE = 17, /* no refeff counterpart: FP64 refinement left an estimated
+                                                    relative error above 1e-9; values = sweeps, real_value = error */
+    REFEFF_FMS_E_UNSUPPORTED = 18,               /* a branch this library does not provide */
+    REFEFF_FMS_E_DEVICE = 19,                    /* HIP runtime failure, no device, or no memory */
+    REFEFF_FMS_E_INVALID_ARGUMENT = 20           /* a malformed call (NULL pointer, negative count, ...) */
+} refeff_fms_error_variant;
+
+typedef struct refeff_fms_error {
+    int32_t status;       /* refeff_fms_status family */
+    int32_t variant;      /* refeff_fms_error_variant */
+    int64_t values[3];    /* integer payload fields, in the order the variant lists them */
+    double real_value;    /* floating-point payload (InvalidTolerance value, refinement error) */
+    char name[24];        /* &'static str payload (tolerance or solver name, "lfms" ...), NUL-terminated */
+    char message[160];    /* refeff's Display text for the variant */
+} refeff_fms_error;
+
+/* Detail of the last failing refeff_fms_plan_create on the calling thread
+ * (REFEFF_FMS_E_NONE after a success). */
+int refeff_fms_last_error(refeff_fms_error* out);
+
+/* Detail of energy `energy` (index into the last solve call's energies) for
+ * the last refeff_fms_solve_batch* call on this plan: the per-energy
+ * Vec<Result<FmsRealSpaceEnergyResult, FmsError>> of fms_real_space_spectrum
+ * (driver.rs:231-236).  REFEFF_FMS_E_NONE for an energy that succeeded. */
+int refeff_fms_plan_energy_error(const refeff_fms_plan* plan, int32_t energy, refeff_fms_error* out);
 
 /* Build a plan (uploads geometry to the current HIP device).  Mirrors
  * fms_real_space_plan, crates/refeff-core/src/fms/driver.rs:105-140. */

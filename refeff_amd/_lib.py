@@ -12,6 +12,7 @@ import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librefeff_fms.so")
+TEST_LIB_PATH = os.path.join(HERE, "librefeff_fms_test.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "refeff_fms.h")
 
 STATUS_NAMES = {
@@ -30,13 +31,18 @@ class FmsLibraryMissing(RuntimeError):
 
 
 class FmsError(RuntimeError):
-    """Non-zero refeff_fms_status (mirrors refeff's FmsError families)."""
+    """Non-zero refeff_fms_status (mirrors refeff's FmsError families); ``variant``
+    and ``payload`` carry the FmsError variant when the library reported one."""
 
-    def __init__(self, code: int, where: str, detail: str = ""):
+    def __init__(self, code: int, where: str, detail: str = "", error=None):
         name = STATUS_NAMES.get(code, str(code))
+        if error is not None and error.variant and not detail:
+            detail = error.message.decode()
         super().__init__(f"{where}: {name}" + (f" ({detail})" if detail else ""))
         self.code = code
         self.name = name
+        self.variant = ERROR_VARIANTS.get(error.variant, str(error.variant)) if error is not None else None
+        self.payload = (tuple(error.values), error.real_value, error.name.decode()) if error is not None else None
 
 
 class PlanDesc(C.Structure):
@@ -47,7 +53,24 @@ class PlanDesc(C.Structure):
         ("direct_cutoff", C.c_float), ("mean_square_displacements", C.c_void_p),
         ("convergence_tolerance", C.c_float), ("zero_tolerance", C.c_float),
         ("full_scattering_requested", C.c_int32), ("solver", C.c_int32), ("max_batch", C.c_int32),
+        ("state_capacity", C.c_int32), ("calculated_l", C.c_void_p),
     ]
+
+
+class ErrorDetail(C.Structure):
+    """refeff_fms_error: the FmsError variant and payload behind a status."""
+    _fields_ = [("status", C.c_int32), ("variant", C.c_int32), ("values", C.c_int64 * 3), ("real_value", C.c_double),
+                ("name", C.c_char * 24), ("message", C.c_char * 160)]
+
+
+ERROR_VARIANTS = {
+    0: "None", 1: "InvalidAngularLimit", 2: "EmptyCluster", 3: "NonFiniteCoordinate", 4: "PotentialOutOfRange",
+    5: "NonFiniteMeanSquareDisplacement", 6: "InvalidDirectCutoff", 7: "InvalidTolerance",
+    8: "InvalidSpinChannelCount", 9: "StateCapacityExceeded", 10: "MissingRepresentativePotential",
+    11: "FullScatteringRequiresLu", 12: "NonFiniteWaveNumber", 13: "NonFinitePhaseShift", 14: "SingularMatrix",
+    15: "IterativeSolverBreakdown", 16: "IterativeSolverNoConvergence", 17: "RefinementNoConvergence",
+    18: "Unsupported", 19: "Device", 20: "InvalidArgument",
+}
 
 
 class PlanInfo(C.Structure):
@@ -93,6 +116,8 @@ def lib():
                                           i32p]
     L.refeff_fms_state_kets.argtypes = [i32, i32, i32p, i32, i32p, i32, i32p, i32p, i32p, i32p, i32p, i32p]
     L.refeff_fms_plan_create.argtypes = [C.POINTER(PlanDesc), C.POINTER(vp)]
+    L.refeff_fms_last_error.argtypes = [C.POINTER(ErrorDetail)]
+    L.refeff_fms_plan_energy_error.argtypes = [vp, i32, C.POINTER(ErrorDetail)]
     L.refeff_fms_plan_destroy.argtypes = [vp]
     L.refeff_fms_plan_destroy.restype = None
     L.refeff_fms_plan_query.argtypes = [vp, C.POINTER(PlanInfo)]
@@ -111,6 +136,25 @@ def lib():
     return L
 
 
-def check(rc: int, where: str, detail: str = ""):
+_test_lib = None
+
+
+def test_lib():
+    """The test build of the library (adds the kernel-level debug entry points)."""
+    global _test_lib
+    if _test_lib is None:
+        if not os.path.exists(TEST_LIB_PATH):
+            raise FmsLibraryMissing(f"{TEST_LIB_PATH} is not built; run `python refeff_amd/build.py`")
+        _test_lib = C.CDLL(TEST_LIB_PATH)
+    return _test_lib
+
+
+def check(rc: int, where: str, detail: str = "", error=None):
     if rc != 0:
-        raise FmsError(rc, where, detail)
+        raise FmsError(rc, where, detail, error)
+
+
+def last_error() -> ErrorDetail:
+    e = ErrorDetail()
+    lib().refeff_fms_last_error(C.byref(e))
+    return e

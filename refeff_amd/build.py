@@ -1,4 +1,8 @@
-"""Builds librefeff_fms.so (host C++ + gfx950 HIP kernels) in-tree with hipcc."""
+"""Builds librefeff_fms.so (host C++ + gfx950 HIP kernels) in-tree with hipcc.
+
+Also builds librefeff_fms_test.so: the same sources with REFEFF_FMS_TEST_HOOKS,
+which adds the kernel-level test entry points (single-matrix LU and GEMM)
+that the product library does not export."""
 from __future__ import annotations
 
 import os
@@ -8,6 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librefeff_fms.so")
+TEST_LIB = os.path.join(HERE, "librefeff_fms_test.so")
 SOURCES = ["fms_host.cpp", "fms_device.hip", "fms_debye.hip", "fms_files.cpp", "fms_io.cpp"]
 
 HIPCC_FLAGS = [
@@ -27,12 +32,17 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    jobs = []
+    for out, extra in ((LIB, []), (TEST_LIB, ["-DREFEFF_FMS_TEST_HOOKS"])):
+        cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-o", out + ".tmp", *srcs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((out, subprocess.Popen(cmd)))
+    for out, proc in jobs:
+        if proc.wait() != 0:
+            raise subprocess.CalledProcessError(proc.returncode, f"hipcc -> {out}")
+    for out, _ in jobs:
+        os.replace(out + ".tmp", out)
     return LIB
 
 

@@ -460,6 +460,7 @@ __global__ void k_fill_int(int* p, int v, size_t n) {
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
         p[i] = v;
 }
+#ifdef REFEFF_FMS_TEST_HOOKS
 // x[i] = a hash of (i, seed) mapped to [-1, 1): benchmark operands only.
 __global__ __launch_bounds__(256) void k_fill_hash(float* x, size_t n, unsigned seed) {
     for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
@@ -470,6 +471,7 @@ __global__ __launch_bounds__(256) void k_fill_hash(float* x, size_t n, unsigned 
         x[i] = static_cast<float>(h & 0xFFFFFF) * (2.0f / 16777216.0f) - 1.0f;
     }
 }
+#endif
 
 __global__ void k_fill_double(double* p, double v, size_t n) {
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
@@ -1404,111 +1406,6 @@ __global__ __launch_bounds__(256) void k_gemm_sub(typename Cx<T>::type* Cb, int 
             }
 }
 
-// C -= A * B for large FP32 complex updates: 128x128 tile, 4 waves of 64x64,
-// K in steps of 16 with two LDS stages (global loads of step k+1 overlap the
-// MFMAs of step k).  M, N, K multiples of 128, 128, 16 are not required: edge
-// tiles are guarded.
-__global__ __launch_bounds__(256) void k_gemm_big(float2* Cb, int ldc, size_t sC, const float2* Ab_, int lda,
-                                                  size_t sA, const float2* Bb_, int ldb, size_t sB, int M, int N,
-                                                  int K) {
-    constexpr int BM = 128, BN = 128, BK = 16, LDA = BM + 16, LDB = BN + 16;
-    __shared__ float as_re[2][BK][LDA], as_im[2][BK][LDA];
-    __shared__ float bs_re[2][BK][LDB], bs_im[2][BK][LDB];
-    const int e = blockIdx.y;
-    const int tilesN = (N + BN - 1) / BN;
-    const int m0 = (blockIdx.x / tilesN) * BM, n0 = (blockIdx.x % tilesN) * BN;
-    const float2* A = Ab_ + e * sA;
-    const float2* B = Bb_ + e * sB;
-    float2* C = Cb + e * sC;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    // global load mapping: A rows (tid/2), 8 k each; B k-row (tid/16), 8 cols each
-    const int ar = tid >> 1, ak = (tid & 1) * 8;
-    const int bk = tid >> 4, bc = (tid & 15) * 8;
-    float2 ra[8], rb[8];
-    auto gload = [&](int k0) {
-        const int gr = m0 + ar;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) ra[i] = (gr < M) ? A[static_cast<size_t>(gr) * lda + k0 + ak + i] : float2{0, 0};
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int gc = n0 + bc + i;
-            rb[i] = (gc < N) ? B[static_cast<size_t>(k0 + bk) * ldb + gc] : float2{0, 0};
-        }
-    };
-    auto sstore = [&](int st) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            as_re[st][ak + i][ar] = ra[i].x;
-            as_im[st][ak + i][ar] = ra[i].y;
-            bs_re[st][bk][bc + i] = rb[i].x;
-            bs_im[st][bk][bc + i] = rb[i].y;
-        }
-    };
-    v4f cr[4][4], ci[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            cr[i][j] = v4f{0, 0, 0, 0};
-            ci[i][j] = v4f{0, 0, 0, 0};
-        }
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    int st = 0;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        const bool more = k0 + BK < K;
-        if (more) gload(k0 + BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int kr = kk + (lane >> 4), ix = lane & 15;
-            float ar_[4], ai_[4], nai[4], br_[4], bi_[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                ar_[i] = as_re[st][kr][wr * 64 + i * 16 + ix];
-                ai_[i] = as_im[st][kr][wr * 64 + i * 16 + ix];
-                nai[i] = -ai_[i];
-                br_[i] = bs_re[st][kr][wc * 64 + i * 16 + ix];
-                bi_[i] = bs_im[st][kr][wc * 64 + i * 16 + ix];
-            }
-            // two passes so that back-to-back MFMAs never share an accumulator
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar_[i], br_[j], cr[i][j], 0, 0, 0);
-                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar_[i], bi_[j], ci[i][j], 0, 0, 0);
-                }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], bi_[j], cr[i][j], 0, 0, 0);
-                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai_[i], br_[j], ci[i][j], 0, 0, 0);
-                }
-        }
-        if (more) sstore(st ^ 1);
-        __syncthreads();
-        st ^= 1;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
-                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
-                if (row < M && col < N) {
-                    float2* q = C + static_cast<size_t>(row) * ldc + col;
-                    float2 v = *q;
-                    v.x -= cr[i][j][r];
-                    v.y -= ci[i][j][r];
-                    *q = v;
-                }
-            }
-}
 
 // Workgroup -> (energy, tile) with the 8 XCDs in mind: the dispatcher deals
 // linear workgroup ids round-robin to the XCDs, so id L runs on XCD L % 8.
@@ -2105,129 +2002,6 @@ __global__ __launch_bounds__(512, 1) void k_gemm_h2(float2* Cb, int ldc, size_t 
             }
 }
 
-// C -= A * B, FP32 complex in the 3M form, with a larger register tile than
-// k_gemm_v2: each of the 4 waves computes (16 TI) x (16 TJ) of C, the
-// workgroup (32 TI) x (32 TJ).  K in steps of BK with register prefetch of the
-// next step and two LDS stages; re and im are kept in separate LDS planes,
-// A transposed to [k][m], B as [k][n], both padded to a row length = 16
-// (mod 64) so the MFMA operand reads are free of bank conflicts.
-// Per K step of 4 a wave reads 2 TI + 2 TJ values and issues 3 TI TJ MFMAs.
-// Requirements (host-checked): N even, K % BK == 0, 16-byte aligned rows.
-template <int TI, int TJ, int BK>
-__global__ __launch_bounds__(256, 2) void k_gemm_v3(float2* Cb, int ldc, size_t sC, const float2* Ab_, int lda,
-                                                    size_t sA, const float2* Bb_, int ldb, size_t sB, int M, int N,
-                                                    int K) {
-    constexpr int BM = 32 * TI, BN = 32 * TJ;
-    constexpr int LDA = BM + 16, LDB = BN + 16;
-    constexpr int APT = BM * BK / 256;      // complex values of A per thread per step
-    constexpr int TPR = BK / APT;           // threads per A row
-    constexpr int BPT = BN * BK / 256;      // complex values of B per thread per step
-    constexpr int TPB = BN / BPT;           // threads per B row
-    static_assert(APT % 2 == 0 && BPT % 2 == 0 && TPR >= 1 && TPB >= 1, "16-byte loads");
-    __shared__ float as_re[2][BK][LDA], as_im[2][BK][LDA];
-    __shared__ float bs_re[2][BK][LDB], bs_im[2][BK][LDB];
-    const int tilesN = (N + BN - 1) / BN;
-    const int tile = blockIdx.x, e = blockIdx.y;
-    const int m0 = (tile / tilesN) * BM, n0 = (tile % tilesN) * BN;
-    const float2* A = Ab_ + e * sA;
-    const float2* B = Bb_ + e * sB;
-    float2* C = Cb + e * sC;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int ar = tid / TPR, ak = (tid % TPR) * APT;
-    const int bk = tid / TPB, bc = (tid % TPB) * BPT;
-    float4 ra[APT / 2], rb[BPT / 2];
-    auto gload = [&](int k0) {
-        const int gr = m0 + ar;
-        const float4* pa = reinterpret_cast<const float4*>(A + static_cast<size_t>(gr) * lda + k0 + ak);
-#pragma unroll
-        for (int i = 0; i < APT / 2; ++i) ra[i] = (gr < M) ? pa[i] : float4{0, 0, 0, 0};
-        const int gc = n0 + bc;
-        const float4* pb = reinterpret_cast<const float4*>(B + static_cast<size_t>(k0 + bk) * ldb + gc);
-#pragma unroll
-        for (int i = 0; i < BPT / 2; ++i) rb[i] = (gc + 2 * i < N) ? pb[i] : float4{0, 0, 0, 0};
-    };
-    auto sstore = [&](int st) {
-#pragma unroll
-        for (int i = 0; i < APT / 2; ++i) {
-            as_re[st][ak + 2 * i][ar] = ra[i].x;
-            as_im[st][ak + 2 * i][ar] = ra[i].y;
-            as_re[st][ak + 2 * i + 1][ar] = ra[i].z;
-            as_im[st][ak + 2 * i + 1][ar] = ra[i].w;
-        }
-#pragma unroll
-        for (int i = 0; i < BPT / 2; ++i) {
-            bs_re[st][bk][bc + 2 * i] = rb[i].x;
-            bs_im[st][bk][bc + 2 * i] = rb[i].y;
-            bs_re[st][bk][bc + 2 * i + 1] = rb[i].z;
-            bs_im[st][bk][bc + 2 * i + 1] = rb[i].w;
-        }
-    };
-    v4f p1[TI][TJ], p2[TI][TJ], p3[TI][TJ];
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j) {
-            p1[i][j] = v4f{0, 0, 0, 0};
-            p2[i][j] = v4f{0, 0, 0, 0};
-            p3[i][j] = v4f{0, 0, 0, 0};
-        }
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    int st = 0;
-    const int ix = lane & 15, q = lane >> 4;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        const bool more = k0 + BK < K;
-        if (more) gload(k0 + BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int kr = kk + q;
-            float xr[TI], xi[TI], xs[TI], yr[TJ], yi[TJ], ys[TJ];
-#pragma unroll
-            for (int i = 0; i < TI; ++i) {
-                xr[i] = as_re[st][kr][wr * 16 * TI + i * 16 + ix];
-                xi[i] = as_im[st][kr][wr * 16 * TI + i * 16 + ix];
-                xs[i] = xr[i] + xi[i];
-            }
-#pragma unroll
-            for (int j = 0; j < TJ; ++j) {
-                yr[j] = bs_re[st][kr][wc * 16 * TJ + j * 16 + ix];
-                yi[j] = bs_im[st][kr][wc * 16 * TJ + j * 16 + ix];
-                ys[j] = yr[j] + yi[j];
-            }
-#pragma unroll
-            for (int i = 0; i < TI; ++i)
-#pragma unroll
-                for (int j = 0; j < TJ; ++j) {
-                    p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[i], yr[j], p1[i][j], 0, 0, 0);
-                    p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[i], yi[j], p2[i][j], 0, 0, 0);
-                    p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xs[i], ys[j], p3[i][j], 0, 0, 0);
-                }
-        }
-        if (more) sstore(st ^ 1);
-        __syncthreads();
-        st ^= 1;
-    }
-    // C -= (P1 - P2) + i (P3 - P1 - P2)
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wr * 16 * TI + i * 16 + 4 * q + r;
-                const int col = n0 + wc * 16 * TJ + j * 16 + ix;
-                if (row < M && col < N) {
-                    float2* c = C + static_cast<size_t>(row) * ldc + col;
-                    float2 v = *c;
-                    const float a = p1[i][j][r], b = p2[i][j][r];
-                    v.x -= a - b;
-                    v.y -= p3[i][j][r] - a - b;
-                    *c = v;
-                }
-            }
-}
 
 // final row permutation of the LU: perm[i] = original row now at position i
 __global__ __launch_bounds__(256) void k_perm(const int* ipivb, int* permb, int n) {
@@ -3348,6 +3122,8 @@ struct ExtractArgs {
     const int* col0;
     const int* ipart;
     double2* out;
+    const int* sl;   // state l (FEFF lcalc mask over the source column's l)
+    unsigned lmask;
 };
 __global__ __launch_bounds__(256) void k_extract(ExtractArgs a) {
     const int e = blockIdx.y;
@@ -3357,7 +3133,8 @@ __global__ __launch_bounds__(256) void k_extract(ExtractArgs a) {
         const int rem = static_cast<int>(idx % (static_cast<size_t>(a.ch) * a.ch));
         const int col = rem / a.ch, row = rem % a.ch;
         double2 v = make_double2(0.0, 0.0);
-        if (ip >= a.pstart && ip <= a.pend && row < a.ipart[ip] && col < a.ipart[ip])
+        if (ip >= a.pstart && ip <= a.pend && row < a.ipart[ip] && col < a.ipart[ip] &&
+            ((a.lmask >> a.sl[a.rep[ip] + col]) & 1u))
             v = a.X[e * a.sx + static_cast<size_t>(a.rep[ip] + row) * a.nrhs + a.col0[ip] + col];
         a.out[e * per + idx] = v;
     }
@@ -3454,7 +3231,11 @@ struct refeff_fms_plan {
     bool lookahead = false;             // REFEFF_FMS_LOOKAHEAD=1: far trailing update on a second stream
     hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t aux_ev[4][2] = {};
+    hipEvent_t ev_asm[2] = {nullptr, nullptr};  // brackets the assembly for refeff_fms_stats
     std::mutex mu;
+    unsigned lcalc_mask = ~0u;                   // FEFF lcalc: bit l set = source columns of l solved (iterative)
+    std::vector<int> h_sl, h_ss, h_spot;         // host copies of the state table (l, spin, potential)
+    std::vector<refeff_fms_error> energy_err;    // per-energy detail of the last solve call
     // kernel-level instrumentation (refeff_fms_plan_profile)
     struct Rec {
         hipEvent_t a, b;
@@ -4177,8 +3958,117 @@ struct BicgOut {
     std::vector<char> ok;
 };
 
+// Typed failure detail (refeff FmsError variants, fms/types.rs:830-981).
+refeff_fms_error make_error(int status, int variant, int64_t v0 = 0, int64_t v1 = 0, int64_t v2 = 0,
+                            double real = 0.0, const char* name = "") {
+    refeff_fms_error e{};
+    e.status = status;
+    e.variant = variant;
+    e.values[0] = v0;
+    e.values[1] = v1;
+    e.values[2] = v2;
+    e.real_value = real;
+    std::snprintf(e.name, sizeof e.name, "%s", name);
+    char* m = e.message;
+    const size_t ml = sizeof e.message;
+    switch (variant) {
+    case REFEFF_FMS_E_NONE: m[0] = 0; break;
+    case REFEFF_FMS_E_INVALID_ANGULAR_LIMIT:
+        std::snprintf(m, ml, "%s=%lld is invalid for lx=%lld", name, (long long)v0, (long long)v1); break;
+    case REFEFF_FMS_E_EMPTY_CLUSTER: std::snprintf(m, ml, "FMS cluster must contain at least one atom"); break;
+    case REFEFF_FMS_E_NON_FINITE_COORDINATE:
+        std::snprintf(m, ml, "atom %lld coordinate axis %lld must be finite", (long long)v0, (long long)v1); break;
+    case REFEFF_FMS_E_POTENTIAL_OUT_OF_RANGE:
+        std::snprintf(m, ml, "potential %lld is outside 0..=%lld", (long long)v0, (long long)v1); break;
+    case REFEFF_FMS_E_NON_FINITE_MSD: std::snprintf(m, ml, "mean-square displacement must be finite"); break;
+    case REFEFF_FMS_E_INVALID_DIRECT_CUTOFF:
+        std::snprintf(m, ml, "direct FMS cutoff must be finite and nonnegative"); break;
+    case REFEFF_FMS_E_INVALID_TOLERANCE:
+        std::snprintf(m, ml, "%s tolerance must be finite and nonnegative, got %g", name, real); break;
+    case REFEFF_FMS_E_INVALID_SPIN_CHANNEL_COUNT:
+        std::snprintf(m, ml, "invalid spin channel count %lld; expected 1 or 2", (long long)v0); break;
+    case REFEFF_FMS_E_STATE_CAPACITY_EXCEEDED:
+        std::snprintf(m, ml, "state-ket count exceeded capacity %lld", (long long)v0); break;
+    case REFEFF_FMS_E_MISSING_REPRESENTATIVE:
+        std::snprintf(m, ml, "missing representative state offset for potential %lld", (long long)v0); break;
+    case REFEFF_FMS_E_FULL_SCATTERING_REQUIRES_LU:
+        std::snprintf(m, ml, "full FMS scattering matrix requires LU inversion, got minv=%lld", (long long)v0); break;
+    case REFEFF_FMS_E_NON_FINITE_WAVE_NUMBER: std::snprintf(m, ml, "wave number must be finite"); break;
+    case REFEFF_FMS_E_NON_FINITE_PHASE_SHIFT:
+        std::snprintf(m, ml, "xphase(spin=%lld, l=%lld, potential=%lld) must be finite", (long long)v0, (long long)v1,
+                      (long long)v2); break;
+    case REFEFF_FMS_E_SINGULAR_MATRIX:
+        std::snprintf(m, ml, "linear algebra failure: matrix is singular at pivot %lld", (long long)v0); break;
+    case REFEFF_FMS_E_ITERATIVE_BREAKDOWN: std::snprintf(m, ml, "%s solver breakdown at iteration", name); break;
+    case REFEFF_FMS_E_ITERATIVE_NO_CONVERGENCE:
+        std::snprintf(m, ml, "%s solver did not converge after %lld restarts", name, (long long)v0); break;
+    case REFEFF_FMS_E_REFINEMENT_NO_CONVERGENCE:
+        std::snprintf(m, ml, "FP64 refinement left an estimated relative error %.3e after %lld sweeps", real,
+                      (long long)v0); break;
+    case REFEFF_FMS_E_UNSUPPORTED: std::snprintf(m, ml, "%s is not provided by this library", name); break;
+    case REFEFF_FMS_E_DEVICE: std::snprintf(m, ml, "HIP device failure (%s)", name); break;
+    default: std::snprintf(m, ml, "invalid argument (%s)", name); break;
+    }
+    return e;
+}
+
+thread_local refeff_fms_error t_last_error{};
+
+int fail(int status, int variant, int64_t v0 = 0, int64_t v1 = 0, int64_t v2 = 0, double real = 0.0,
+         const char* name = "") {
+    t_last_error = make_error(status, variant, v0, v1, v2, real, name);
+    return status;
+}
+
+const char* solver_name(int solver) {
+    switch (solver) {
+    case REFEFF_FMS_SOLVER_GGBI: return "ggbi";
+    case REFEFF_FMS_SOLVER_GGRM: return "ggrm";
+    case REFEFF_FMS_SOLVER_GGGM: return "gggm";
+    case REFEFF_FMS_SOLVER_GGTF: return "ggtf";
+    case REFEFF_FMS_SOLVER_FP64: return "fp64 lu";
+    default: return "mixed lu";
+    }
+}
+
+// Per-energy failure detail: the FmsError variant behind an energy's status.
+// Non-finite inputs are located on the host from the energy's (small) k and
+// phase slices, in refeff's order: the wave numbers (pairs.rs:242-260), then
+// the phase shifts of the states in state order (t_matrix.rs:149-192).
+refeff_fms_error energy_detail(refeff_fms_plan* p, int e, int info, int status, int sweeps, double est) {
+    if (status == REFEFF_FMS_OK) return refeff_fms_error{};
+    if (status == REFEFF_FMS_NON_FINITE) {
+        const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+        std::vector<double2> kh(kst), ph(pst);
+        if (hipMemcpy(kh.data(), p->d_k + kst * e, kst * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(ph.data(), p->d_ph + pst * e, pst * sizeof(double2), hipMemcpyDeviceToHost) != hipSuccess)
+            return make_error(status, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "error detail");
+        auto fin = [](double2 v) { return std::isfinite(v.x) && std::isfinite(v.y); };
+        for (const auto& v : kh)
+            if (!fin(v)) return make_error(status, REFEFF_FMS_E_NON_FINITE_WAVE_NUMBER);
+        const int nl = 2 * p->L + 1;
+        auto at = [&](int s, int ls, int pot) { return ph[(static_cast<size_t>(s - 1) * nl + (ls + p->L)) * p->npot + pot]; };
+        const bool so = !(p->nsp == 1 && p->ispin == 0);
+        for (size_t i = 0; i < p->h_sl.size(); ++i) {
+            const int l = p->h_sl[i], sp = p->h_ss[i], pot = p->h_spot[i];
+            if (!fin(at(sp, l, pot))) return make_error(status, REFEFF_FMS_E_NON_FINITE_PHASE_SHIFT, sp, l, pot);
+            if (so && !fin(at(sp, -l, pot))) return make_error(status, REFEFF_FMS_E_NON_FINITE_PHASE_SHIFT, sp, -l, pot);
+        }
+        return make_error(status, REFEFF_FMS_E_NON_FINITE_PHASE_SHIFT, 0, 0, 0, 0.0, "t-matrix");
+    }
+    if (status == REFEFF_FMS_SINGULAR) return make_error(status, REFEFF_FMS_E_SINGULAR_MATRIX, (info & 0x0FFFFFFF) - 1);
+    if (status == REFEFF_FMS_BREAKDOWN)
+        return make_error(status, REFEFF_FMS_E_ITERATIVE_BREAKDOWN, 0, 0, 0, 0.0, solver_name(p->solver));
+    if (status == REFEFF_FMS_NO_CONVERGENCE) {
+        if (p->solver == REFEFF_FMS_SOLVER_MIXED)
+            return make_error(status, REFEFF_FMS_E_REFINEMENT_NO_CONVERGENCE, sweeps, 0, 0, est);
+        return make_error(status, REFEFF_FMS_E_ITERATIVE_NO_CONVERGENCE, 128, 0, 0, 0.0, solver_name(p->solver));
+    }
+    return make_error(status, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "solve");
+}
+
 int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status, refeff_fms_stats* st,
-                       BicgOut* bo = nullptr) {
+                       BicgOut* bo = nullptr, int ebase = 0, const int* eidx = nullptr) {
     const int n = p->npad;
     const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * p->nrhs;
     const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
@@ -4205,6 +4095,8 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         }
     }
     const double n2 = static_cast<double>(p->n) * p->n;
+    // assembly time (t-matrix, G0 build, padding) of the first sub-batch, by events
+    if (st && p->ev_asm[0]) (void)hipEventRecord(p->ev_asm[0], gs.front().s);
     for (const auto& g : gs) {
         hipLaunchKernelGGL(k_fill_int, dim3((g.nb + 255) / 256), dim3(256), 0, g.s, p->d_info + g.e0, 0,
                            static_cast<size_t>(g.nb));
@@ -4234,6 +4126,8 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                 hipLaunchKernelGGL(k_pad<float>, dim3(n, g.nb), dim3(256), 0, g.s, p->d_a32 + sa * g.e0, sa, n, p->n,
                                    static_cast<double2*>(nullptr), sx, p->nrhs);
         }
+        if (hipGetLastError() != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+        if (st && &g == &gs.front() && p->ev_asm[1]) (void)hipEventRecord(p->ev_asm[1], g.s);
         if (ggbi) {
             run_ggbi(p, g);
             continue;
@@ -4269,8 +4163,9 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     } else if (mixed) {
         float2* w = static_cast<float2*>(p->d_w);
         for (const auto& g : gs)
-            hipLaunchKernelGGL(k_fill_double, dim3((2 * sx * g.nb + 255) / 256), dim3(256), 0, g.s,
-                               reinterpret_cast<double*>(p->d_x64 + sx * g.e0), 0.0, 2 * sx * g.nb);
+            hipLaunchKernelGGL(k_fill_double, dim3(std::max<size_t>(1, std::min<size_t>(65535, (2 * sx * g.nb + 255) / 256))),
+                               dim3(256), 0, g.s, reinterpret_cast<double*>(p->d_x64 + sx * g.e0), 0.0, 2 * sx * g.nb);
+        if (hipGetLastError() != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
         const int max_sweeps = 8;
         std::vector<double> prev_rel(nb, 1.0e300);
         // Sweeps after the first run over the energies still refining only:
@@ -4376,8 +4271,9 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     std::vector<int> info(nb);
     const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
     for (const auto& g : gs) {
+        // lcalc applies to the iterative branches only (the LU ignores it, fms/solvers.rs:80-86)
         ExtractArgs ea{(cut_g0 ? p->d_r64 : p->d_x64) + sx * g.e0, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0,
-                       p->d_ipart, out + sst * g.e0};
+                       p->d_ipart, out + sst * g.e0, p->d_sl, cut_g0 ? p->lcalc_mask : ~0u};
         hipLaunchKernelGGL(k_extract, dim3(std::max<size_t>(1, std::min<size_t>(64, (per + 255) / 256)), g.nb),
                            dim3(256), 0, g.s, ea);
         (void)hipMemcpyAsync(info.data() + g.e0, p->d_info + g.e0, g.nb * sizeof(int), hipMemcpyDeviceToHost, g.s);
@@ -4392,8 +4288,15 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         }
         if (status) status[e] = s;
         if (s && !rc) rc = s;
+        const size_t slot = static_cast<size_t>(eidx ? eidx[e] : ebase + e);
+        if (slot < p->energy_err.size()) p->energy_err[slot] = energy_detail(p, e, info[e], s, sweeps, est_err[e]);
     }
     if (st) {
+        float asm_ms = 0.f;
+        if (p->ev_asm[0] && p->ev_asm[1] && hipEventElapsedTime(&asm_ms, p->ev_asm[0], p->ev_asm[1]) == hipSuccess) {
+            st->seconds_assemble += 1.0e-3 * asm_ms;
+            st->seconds_factor -= 1.0e-3 * asm_ms;
+        }
         st->seconds_factor += t_first - t0;
         st->seconds_solve += now_s() - t_first;
         st->refinement_sweeps = std::max(st->refinement_sweeps, sweeps);
@@ -4409,14 +4312,36 @@ int check_hip(hipError_t e) { return e == hipSuccess ? REFEFF_FMS_OK : REFEFF_FM
 
 extern "C" {
 
-int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out) {
-    if (!d || !out) return REFEFF_FMS_INVALID_ARGUMENT;
-    *out = nullptr;
-    if (d->spin_channels < 1 || d->spin_channels > 2) return REFEFF_FMS_INVALID_SPIN_COUNT;
-    if (d->atom_count <= 0 || !d->positions || !d->potentials) return REFEFF_FMS_EMPTY_CLUSTER;
-    if (d->global_lmax < 0 || d->global_lmax > kMaxL || d->max_potential < 0 || !d->raw_potential_lmax)
+int refeff_fms_last_error(refeff_fms_error* out) {
+    if (!out) return REFEFF_FMS_INVALID_ARGUMENT;
+    *out = t_last_error;
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_energy_error(const refeff_fms_plan* p, int32_t energy, refeff_fms_error* out) {
+    if (!p || !out || energy < 0 || static_cast<size_t>(energy) >= p->energy_err.size())
         return REFEFF_FMS_INVALID_ARGUMENT;
-    if (!std::isfinite(d->direct_cutoff) || d->direct_cutoff < 0.f) return REFEFF_FMS_INVALID_ARGUMENT;
+    *out = p->energy_err[energy];
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out) {
+    t_last_error = refeff_fms_error{};
+    if (!d || !out) return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_ARGUMENT, 0, 0, 0, 0.0, "NULL");
+    *out = nullptr;
+    if (d->spin_channels < 1 || d->spin_channels > 2)
+        return fail(REFEFF_FMS_INVALID_SPIN_COUNT, REFEFF_FMS_E_INVALID_SPIN_CHANNEL_COUNT, d->spin_channels);
+    if (d->atom_count <= 0 || !d->positions || !d->potentials)
+        return fail(REFEFF_FMS_EMPTY_CLUSTER, REFEFF_FMS_E_EMPTY_CLUSTER);
+    if (d->global_lmax < 0 || d->global_lmax > kMaxL)
+        return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_ANGULAR_LIMIT, d->global_lmax, kMaxL, 0, 0.0,
+                    "global_lmax");
+    if (d->max_potential < 0 || !d->raw_potential_lmax)
+        return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_ARGUMENT, 0, 0, 0, 0.0, "max_potential");
+    if (!std::isfinite(d->direct_cutoff) || d->direct_cutoff < 0.f)
+        return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_DIRECT_CUTOFF);
+    if (d->state_capacity < 0)
+        return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_ARGUMENT, 0, 0, 0, 0.0, "state_capacity");
     const bool full = d->full_scattering_requested != 0;
     int solver = d->solver;
     // refeff fms_scattering_method_selection (fms/driver.rs:72-95): minv 0 -> LU,
@@ -4429,29 +4354,40 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
                  : d->minv == 3         ? REFEFF_FMS_SOLVER_GGGM
                                         : REFEFF_FMS_SOLVER_GGTF;
     const bool iterative = solver == REFEFF_FMS_SOLVER_GGBI || krylov_solver(solver);
-    if (full && iterative) return REFEFF_FMS_UNSUPPORTED;  // FullScatteringRequiresLu
+    if (full && iterative)
+        return fail(REFEFF_FMS_UNSUPPORTED, REFEFF_FMS_E_FULL_SCATTERING_REQUIRES_LU, d->minv);
     // BiCGSTAB runs on the mixed plan (its fallback); a full scattering matrix
     // (every G0 column a right-hand side) forces the LU
     const bool bicg = solver == REFEFF_FMS_SOLVER_BICGSTAB && !full;
     if (solver == REFEFF_FMS_SOLVER_BICGSTAB) solver = REFEFF_FMS_SOLVER_MIXED;
     if (solver != REFEFF_FMS_SOLVER_MIXED && solver != REFEFF_FMS_SOLVER_FP64 && !iterative)
-        return REFEFF_FMS_UNSUPPORTED;
+        return fail(REFEFF_FMS_UNSUPPORTED, REFEFF_FMS_E_UNSUPPORTED, d->solver, 0, 0, 0.0, "solver");
     // InvalidTolerance (fms/solvers.rs:35-40, internals/iterative.rs:70-75)
-    if (iterative && (!std::isfinite(d->zero_tolerance) || d->zero_tolerance < 0.f ||
-                      !std::isfinite(d->convergence_tolerance) || d->convergence_tolerance < 0.f))
-        return REFEFF_FMS_INVALID_ARGUMENT;
+    if (iterative && (!std::isfinite(d->zero_tolerance) || d->zero_tolerance < 0.f))
+        return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_TOLERANCE, 0, 0, 0, d->zero_tolerance, "toler2");
+    if (iterative && (!std::isfinite(d->convergence_tolerance) || d->convergence_tolerance < 0.f))
+        return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_TOLERANCE, 0, 0, 0, d->convergence_tolerance,
+                    "toler1");
     for (int i = 0; i < 3 * d->atom_count; ++i)
-        if (!std::isfinite(d->positions[i])) return REFEFF_FMS_NON_FINITE;
+        if (!std::isfinite(d->positions[i]))
+            return fail(REFEFF_FMS_NON_FINITE, REFEFF_FMS_E_NON_FINITE_COORDINATE, i / 3, i % 3);
     for (int a = 0; a < d->atom_count; ++a)
-        if (d->potentials[a] < 0 || d->potentials[a] > d->max_potential) return REFEFF_FMS_POTENTIAL_OUT_OF_RANGE;
+        if (d->potentials[a] < 0 || d->potentials[a] > d->max_potential)
+            return fail(REFEFF_FMS_POTENTIAL_OUT_OF_RANGE, REFEFF_FMS_E_POTENTIAL_OUT_OF_RANGE, d->potentials[a],
+                        d->max_potential);
     if (d->mean_square_displacements)
         for (size_t i = 0; i < static_cast<size_t>(d->atom_count) * d->atom_count; ++i)
-            if (!std::isfinite(d->mean_square_displacements[i])) return REFEFF_FMS_NON_FINITE;
+            if (!std::isfinite(d->mean_square_displacements[i]))
+                return fail(REFEFF_FMS_NON_FINITE, REFEFF_FMS_E_NON_FINITE_MSD);
 
     StateTable stt;
     int rc = build_state_table(d->spin_channels, d->atom_count, d->potentials, d->max_potential,
                                d->raw_potential_lmax, d->global_lmax, &stt);
-    if (rc) return rc;
+    if (rc == REFEFF_FMS_POTENTIAL_OUT_OF_RANGE) return fail(rc, REFEFF_FMS_E_POTENTIAL_OUT_OF_RANGE);
+    if (rc) return fail(rc, REFEFF_FMS_E_INVALID_ARGUMENT, 0, 0, 0, 0.0, "state kets");
+    // istatx capacity (state.rs:97-148: more states than the capacity fail)
+    if (d->state_capacity > 0 && static_cast<int>(stt.atom.size()) > d->state_capacity)
+        return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_STATE_CAPACITY_EXCEEDED, d->state_capacity);
     auto* p = new refeff_fms_plan();
     p->nat = d->atom_count;
     p->nsp = d->spin_channels;
@@ -4464,8 +4400,18 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->npad = ((p->n + 63) / 64) * 64;
     if (p->npad > 16384) {  // row permutation is built in LDS (64 KiB)
         delete p;
-        return REFEFF_FMS_UNSUPPORTED;
+        return fail(REFEFF_FMS_UNSUPPORTED, REFEFF_FMS_E_UNSUPPORTED, 16384, 0, 0, 0.0, "a padded order above 16384");
     }
+    p->lcalc_mask = ~0u;
+    if (d->calculated_l) {
+        p->lcalc_mask = 0u;
+        for (int l = 0; l <= d->global_lmax; ++l)
+            if (d->calculated_l[l]) p->lcalc_mask |= 1u << l;
+    }
+    p->h_sl = stt.l;
+    p->h_ss = stt.spin;
+    p->h_spot.resize(stt.atom.size());
+    for (size_t i = 0; i < stt.atom.size(); ++i) p->h_spot[i] = d->potentials[stt.atom[i]];
     p->ch = (p->L + 1) * (p->L + 1) * p->nsp;
     p->npot = p->max_pot + 1;
     p->solver = solver;
@@ -4503,7 +4449,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     for (int ip = p->pstart; ip <= p->pend; ++ip) {
         if (p->rep[ip] < 0) {
             delete p;
-            return REFEFF_FMS_MISSING_REPRESENTATIVE;
+            return fail(REFEFF_FMS_MISSING_REPRESENTATIVE, REFEFF_FMS_E_MISSING_REPRESENTATIVE, ip);
         }
         const int lm = clamp_lipotx(d->raw_potential_lmax[ip], p->L);
         p->ipart[ip] = (lm + 1) * (lm + 1) * p->nsp;
@@ -4540,7 +4486,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
         delete p;
-        return REFEFF_FMS_DEVICE_ERROR;
+        return fail(REFEFF_FMS_DEVICE_ERROR, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "no HIP device");
     }
     bool ok = dupload(&p->d_pos, d->positions, 3 * static_cast<size_t>(p->nat)) &&
               dupload(&p->d_sig, sig.data(), sig.size()) &&
@@ -4553,7 +4499,8 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
               dupload(&p->d_rhs_src, rhs_src.data(), rhs_src.size()) && dupload(&p->d_rhs_pot, rhs_pot.data(), rhs_pot.size()) &&
               dupload(&p->d_ipart, p->ipart.data(), p->ipart.size()) && dupload(&p->d_xnlm, xn.data(), xn.size()) &&
               dupload(&p->d_so_plus, sp.data(), sp.size()) && dupload(&p->d_so_minus, sm.data(), sm.size()) &&
-              hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
+              hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&p->ev_asm[0]) == hipSuccess && hipEventCreate(&p->ev_asm[1]) == hipSuccess;
     p->trace = std::getenv("REFEFF_FMS_TRACE") != nullptr;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_3M")) p->gemm_3m = std::atoi(e) != 0;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_CPF")) p->gemm_cpf = std::atoi(e) != 0;
@@ -4584,7 +4531,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (!ok) {
         refeff_fms_plan_destroy(p);
         (void)hipGetLastError();
-        return REFEFF_FMS_DEVICE_ERROR;
+        return fail(REFEFF_FMS_DEVICE_ERROR, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "plan upload");
     }
     // batch: bounded by free device memory (70%) and the caller's limit
     size_t free_b = 0, total_b = 0;
@@ -4605,6 +4552,8 @@ void refeff_fms_plan_destroy(refeff_fms_plan* p) {
     dfree(p->d_sl); dfree(p->d_sm); dfree(p->d_ss); dfree(p->d_spot); dfree(p->d_rep); dfree(p->d_col0);
     dfree(p->d_ipart); dfree(p->d_rhs_src); dfree(p->d_rhs_pot); dfree(p->d_xnlm); dfree(p->d_so_plus); dfree(p->d_so_minus);
     for (hipEvent_t ev : p->pool) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : p->ev_asm)
+        if (ev) (void)hipEventDestroy(ev);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     for (hipStream_t s : p->streams)
         if (s) (void)hipStreamDestroy(s);
@@ -4668,7 +4617,7 @@ static int solve_bicg_common(refeff_fms_plan* p, int32_t ne, const double* k, co
                 return REFEFF_FMS_DEVICE_ERROR;
         }
         if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
-        r = solve_device_batch(p, nb, p->d_out, sb.data(), st, bo);
+        r = solve_device_batch(p, nb, p->d_out, sb.data(), st, bo, 0, idx.data());
         if (r == REFEFF_FMS_DEVICE_ERROR) return r;
         for (int i = 0; i < nb; ++i) {
             if (bo && !sb[i] && !bo->ok[i]) continue;  // left for the LU
@@ -4730,6 +4679,7 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
                         refeff_fms_stats* st, bool device, hipStream_t ext, double* full_out = nullptr) {
     if (!p || ne < 0 || (ne > 0 && (!k || !ph || !sc))) return REFEFF_FMS_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(p->mu);
+    p->energy_err.assign(static_cast<size_t>(ne), refeff_fms_error{});
     if (p->bicg && !full_out) {
         const double tb = now_s();
         if (st) std::memset(st, 0, sizeof(*st));
@@ -4758,7 +4708,7 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
         // the sub-batch streams read the inputs
         if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
         double2* out = device ? reinterpret_cast<double2*>(sc) + sst * e0 : p->d_out;
-        r = solve_device_batch(p, nb, out, status ? status + e0 : nullptr, st);
+        r = solve_device_batch(p, nb, out, status ? status + e0 : nullptr, st, nullptr, e0);
         if (r == REFEFF_FMS_DEVICE_ERROR) return r;
         if (r && !rc) rc = r;
         if (!device) {
@@ -4802,8 +4752,11 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* p, int32_t ne, const double* k,
     return solve_common(p, ne, k, ph, sc, status, st, false, nullptr, full_scattering);
 }
 
-// Kernel-level test hooks (not part of the public header): LU and GEMM of
-// single host matrices through the same device kernels the batched sweep uses.
+#ifdef REFEFF_FMS_TEST_HOOKS
+// Kernel-level test hooks, compiled only into the test library
+// (librefeff_fms_test.so, REFEFF_FMS_TEST_HOOKS): LU and GEMM of single host
+// matrices through the same device kernels the batched sweep uses.  The
+// product library exports only what include/refeff_fms.h declares.
 int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv, int32_t* info) {
     if (n <= 0 || n % 64 || !a || !ipiv) return REFEFF_FMS_INVALID_ARGUMENT;
     refeff_fms_plan p;
@@ -4846,7 +4799,8 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
     const int kmul = (use_double == 0 || use_double == 3 || use_double == 5 || use_double >= 12) ? 32 : 16;
-    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 14)
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 14 ||
+        use_double == 2 || (use_double >= 9 && use_double <= 11))
         return REFEFF_FMS_INVALID_ARGUMENT;
     if (use_double >= 12 && (M % 64 || N % 64)) return REFEFF_FMS_INVALID_ARGUMENT;
     const int ld = std::max(std::max(M, N), K);
@@ -4894,19 +4848,8 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                     hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
                 else if (use_double == 7)
                     hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
-                else if (use_double == 8)
-                    hipLaunchKernelGGL((k_gemm_v2<16, false, true, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
-                else if (use_double == 9)
-                    hipLaunchKernelGGL((k_gemm_v3<4, 2, 16>), dim3(((M + 127) / 128) * ((N + 63) / 64), 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
-                else if (use_double == 10)
-                    hipLaunchKernelGGL((k_gemm_v3<4, 4, 8>), dim3(((M + 127) / 128) * ((N + 127) / 128), 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
                 else
-                    hipLaunchKernelGGL((k_gemm_v3<2, 4, 16>), dim3(((M + 63) / 64) * ((N + 127) / 128), 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
-            } else if (use_double == 2 && sizeof(T) == 4) {
-                const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
-                hipLaunchKernelGGL(k_gemm_big, dim3(tiles, 1), dim3(256), 0, nullptr, reinterpret_cast<float2*>(dc), ld,
-                                   sz, reinterpret_cast<const float2*>(da), ld, sz, reinterpret_cast<const float2*>(db),
-                                   ld, sz, M, N, K);
+                    hipLaunchKernelGGL((k_gemm_v2<16, false, true, true>), dim3(tiles, 1), dim3(256), 0, nullptr, c2, ld, sz, a2, ld, sz, b2, ld, sz, M, N, K);
             } else {
                 const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
                 hipLaunchKernelGGL(k_gemm_sub<T>, dim3(tiles, 1), dim3(256), 0, nullptr, dc, ld, sz, da, ld, sz, db, ld,
@@ -4928,8 +4871,8 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
     return rc;
 }
 
-// Times `iters` launches of one GEMM kernel on a batch of zero-filled
-// matrices (kernel 0: 64x64 FP32, 1: 64x64 FP64, 2: 128x128 FP32).
+// Times `iters` launches of one GEMM kernel on a batch of pseudo-random
+// matrices (kernel 0: 64x64 FP32, 1: 64x64 FP64, 3-8: the k_gemm_v2 variants).
 int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K, int32_t batch, int32_t iters,
                                 double* ms) {
     if (M <= 0 || N <= 0 || K <= 0 || K % 32 || batch <= 0 || iters <= 0 || !ms) return REFEFF_FMS_INVALID_ARGUMENT;
@@ -4960,7 +4903,7 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     auto one = [&]() {
-        if (kernel >= 3 && kernel <= 11) {
+        if (kernel >= 3 && kernel <= 8) {
             const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
             auto* c2 = static_cast<float2*>(dc);
             auto* a2 = static_cast<const float2*>(da);
@@ -4975,18 +4918,8 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
                 hipLaunchKernelGGL((k_gemm_v2<16, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
             else if (kernel == 7)
                 hipLaunchKernelGGL((k_gemm_v2<16, false, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
-            else if (kernel == 8)
-                hipLaunchKernelGGL((k_gemm_v2<16, false, true, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
-            else if (kernel == 9)
-                hipLaunchKernelGGL((k_gemm_v3<4, 2, 16>), dim3(((M + 127) / 128) * ((N + 63) / 64), batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
-            else if (kernel == 10)
-                hipLaunchKernelGGL((k_gemm_v3<4, 4, 8>), dim3(((M + 127) / 128) * ((N + 127) / 128), batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
             else
-                hipLaunchKernelGGL((k_gemm_v3<2, 4, 16>), dim3(((M + 63) / 64) * ((N + 127) / 128), batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
-        } else if (kernel == 2) {
-            const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
-            hipLaunchKernelGGL(k_gemm_big, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<float2*>(dc), ld, per,
-                               static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K);
+                hipLaunchKernelGGL((k_gemm_v2<16, false, true, true>), dim3(tiles, batch), dim3(256), 0, nullptr, c2, ld, per, a2, ld, per, b2, ld, per, M, N, K);
         } else if (kernel == 1) {
             const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
             hipLaunchKernelGGL(k_gemm_sub<double>, dim3(tiles, batch), dim3(256), 0, nullptr, static_cast<double2*>(dc),
@@ -5014,6 +4947,8 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     (void)hipFree(dc);
     return rc;
 }
+
+#endif  // REFEFF_FMS_TEST_HOOKS
 
 int refeff_fms_plan_profile(refeff_fms_plan* p, int32_t enable) {
     if (!p) return REFEFF_FMS_INVALID_ARGUMENT;

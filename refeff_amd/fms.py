@@ -14,7 +14,7 @@ import numpy as np
 
 from ._lib import (SOLVER_AUTO, SOLVER_BICGSTAB, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGGM, SOLVER_GGRM, SOLVER_GGTF,
                    SOLVER_MIXED,
-                   FmsError, PlanDesc, PlanInfo, Stats, check, lib)
+                   ERROR_VARIANTS, ErrorDetail, FmsError, PlanDesc, PlanInfo, Stats, check, last_error, lib)
 
 __all__ = [
     "FmsError", "SOLVER_AUTO", "SOLVER_MIXED", "SOLVER_FP64", "SOLVER_GGBI", "SOLVER_GGRM", "SOLVER_GGGM",
@@ -128,7 +128,8 @@ class FmsRealSpacePlan:
     def __init__(self, positions, potentials, *, lfms=0, minv=0, spin_channels=1, spin_selector=0,
                  max_potential=None, global_lmax=3, raw_potential_lmax=None, direct_cutoff=1.0e9,
                  mean_square_displacements=None, convergence_tolerance=1e-3, zero_tolerance=1e-3,
-                 full_scattering_matrix_requested=False, solver=SOLVER_AUTO, max_batch=0):
+                 full_scattering_matrix_requested=False, solver=SOLVER_AUTO, max_batch=0, state_capacity=None,
+                 calculated_l=None):
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.potentials = np.ascontiguousarray(potentials, dtype=np.int32)
         nat = len(self.potentials)
@@ -149,8 +150,16 @@ class FmsRealSpacePlan:
                         self.raw_potential_lmax.ctypes.data, direct_cutoff,
                         sig.ctypes.data if sig is not None else None, convergence_tolerance, zero_tolerance,
                         1 if full_scattering_matrix_requested else 0, solver, max_batch)
+        desc.state_capacity = 0 if state_capacity is None else int(state_capacity)
+        self._lcalc = None
+        if calculated_l is not None:
+            self._lcalc = np.ascontiguousarray(np.asarray(calculated_l, bool).astype(np.uint8))
+            if len(self._lcalc) < global_lmax + 1:
+                raise ValueError("calculated_l needs global_lmax + 1 entries")
+            desc.calculated_l = self._lcalc.ctypes.data
         self._h = C.c_void_p()
-        check(lib().refeff_fms_plan_create(C.byref(desc), C.byref(self._h)), "fms_real_space_plan")
+        rc = lib().refeff_fms_plan_create(C.byref(desc), C.byref(self._h))
+        check(rc, "fms_real_space_plan", error=last_error() if rc else None)
         info = PlanInfo()
         check(lib().refeff_fms_plan_query(self._h, C.byref(info)), "plan_query")
         self.info = info
@@ -178,6 +187,13 @@ class FmsRealSpacePlan:
             self.close()
         except Exception:
             pass
+
+    def energy_error(self, energy: int):
+        """FmsError detail of one energy of the last solve: (variant, values, real_value, name, message)."""
+        e = ErrorDetail()
+        check(lib().refeff_fms_plan_energy_error(self._h, int(energy), C.byref(e)), "plan_energy_error")
+        return ERROR_VARIANTS.get(e.variant, str(e.variant)), tuple(e.values), e.real_value, e.name.decode(), \
+            e.message.decode()
 
     def solve(self, wave_numbers, phase_shifts):
         """wave_numbers [ne, nsp] complex, phase_shifts [ne, nsp, 2L+1, npot] complex ->

@@ -18,6 +18,19 @@ def test_library_exports_every_header_symbol():
         assert hasattr(lib, n), n
 
 
+def test_library_exports_nothing_beyond_the_header():
+    """The product library's refeff_* entry points are exactly the header's; the
+    kernel-level debug hooks live only in the test build (librefeff_fms_test.so)."""
+    import subprocess
+    from refeff_amd import _lib as rl
+    out = subprocess.run(["nm", "-D", "--defined-only", rl.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = {ln.split()[-1] for ln in out.stdout.splitlines() if ln.split() and ln.split()[-1].startswith("refeff_")}
+    assert exported == set(refeff_amd.header_functions()), exported ^ set(refeff_amd.header_functions())
+    test_out = subprocess.run(["nm", "-D", "--defined-only", rl.TEST_LIB_PATH], capture_output=True, text=True,
+                              check=True).stdout
+    assert "refeff_fms_debug_lu" in test_out and "refeff_fms_debug_gemm" in test_out
+
+
 def test_version():
     assert b"gfx950" in refeff_amd.lib().refeff_fms_version()
 
@@ -278,3 +291,45 @@ def test_transition_bmat_matches_oracle_across_edges():
                 assert list(lind) == list(olind)
                 assert np.abs(b - ob).max() <= 1e-13 * max(1.0, np.abs(ob).max()), (ihole, multipole, spin)
 
+
+
+# --------------------------------------------------------------------------
+# typed failure detail of plan creation (FmsError variants, fms/types.rs:830-981);
+# every case below fails before any device call
+# --------------------------------------------------------------------------
+
+def _plan_error(**kw):
+    pos = kw.pop("positions", [[0, 0, 0], [1.8, 0, 0], [0, 1.8, 0]])
+    pot = kw.pop("potentials", [0, 1, 1])
+    with pytest.raises(fms.FmsError) as ei:
+        fms.FmsRealSpacePlan(pos, pot, **kw)
+    return ei.value
+
+
+def test_plan_error_non_finite_coordinate_names_atom_and_axis():
+    e = _plan_error(positions=[[0, 0, 0], [1.8, 0, 0], [0, float("nan"), 0]])
+    assert e.name == "NON_FINITE" and e.variant == "NonFiniteCoordinate"
+    assert e.payload[0][:2] == (2, 1)
+    assert "atom 2 coordinate axis 1 must be finite" in str(e)
+
+
+def test_plan_error_state_capacity():
+    e = _plan_error(state_capacity=40)  # 3 atoms x 16 states
+    assert e.variant == "StateCapacityExceeded" and e.payload[0][0] == 40
+
+
+def test_plan_error_tolerance_and_spin_and_full_scattering():
+    e = _plan_error(minv=2, zero_tolerance=float("nan"))
+    assert e.variant == "InvalidTolerance" and e.payload[2] == "toler2"
+    e = _plan_error(spin_channels=3)
+    assert e.name == "INVALID_SPIN_COUNT" and e.variant == "InvalidSpinChannelCount" and e.payload[0][0] == 3
+    e = _plan_error(minv=1, solver=fms.SOLVER_GGBI, full_scattering_matrix_requested=True)
+    assert e.variant == "FullScatteringRequiresLu" and e.payload[0][0] == 1
+    e = _plan_error(potentials=[0, 1, 4], max_potential=2)
+    assert e.variant == "PotentialOutOfRange" and e.payload[0][:2] == (4, 2)
+
+
+def test_plan_error_missing_representative():
+    e = _plan_error(potentials=[0, 1, 1], max_potential=2, lfms=1)
+    assert e.name == "MISSING_REPRESENTATIVE" and e.variant == "MissingRepresentativePotential"
+    assert e.payload[0][0] == 2
