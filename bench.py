@@ -2,10 +2,11 @@
 """FMS energy-points/s on one MI355X.
 
 A "step" is one pass of the FMS stage over every energy point of the workload
-(G0 build, 1 - G0 t assembly, LU + solve (+ FP64 refinement), gg extraction),
-with the wave numbers and phase shifts already resident in HBM.  Prints one
-JSON line.  The default workload is the mixed-precision configuration:
-Cu fcc a=3.615 A, R=8.0 A (177 atoms, N=2832), 120 complex energies.
+(G0 build, 1 - G0 t assembly, LU + solve (+ FP64 refinement), gg extraction,
+the absorber's gg on the host and its MKGTR trace), with the wave numbers and
+phase shifts already resident in HBM.  Prints one JSON line.  The default
+workload is the 600-atom configuration the metric's >= 5x target is quoted on:
+bcc Fe a=2.87 A, R=12 A (609 atoms, N=9744), 200 complex energies.
 """
 from __future__ import annotations
 
@@ -25,10 +26,39 @@ sys.path.insert(0, ROOT)
 METRIC = "FMS energy-pts/s at ≤1e-6 xmu rel err; mixed+learned vs plain FP64 LU"
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the LU trailing-update kernel, averaged over one factorisation's
-# launches, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script scaled by the
-# same counters read for a 1 GiB device copy (profiles/fms_cu_r8_gemm_v2_traffic.md).
-TRAFFIC_BYTES_PER_LAUNCH = {("cfg2", "mixed"): 11375332998}  # K = 192 trailing update, rocprofv3 --pmc
+
+
+def traffic_bytes_per_launch(config, solver):
+    """HBM bytes per launch of the LU trailing-update kernel, averaged over one
+    factorisation's launches: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this
+    script, each scaled by the same counter read for a 1 GiB device copy.  Read from
+    the committed summaries under profiles/ (a line `bench.traffic config=.. solver=..
+    bytes_per_launch=..`); None where no summary covers the workload."""
+    pat = re.compile(r"^bench\.traffic config=(\S+) solver=(\S+) bytes_per_launch=(\d+)\s*$")
+    pdir = os.path.join(ROOT, "profiles")
+    for name in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        if not name.endswith((".md", ".txt")):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for ln in f:
+                m = pat.match(ln)
+                if m and m.group(1) == config and m.group(2) == solver:
+                    return int(m.group(3))
+    return None
+
+
+def host_cpu():
+    """(model name, logical cores) of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
 
 
 def parse():
@@ -36,11 +66,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5"])
+    ap.add_argument("--config", default="cfg4", choices=["cfg1", "cfg2", "cfg4", "cfg5"])
     ap.add_argument("--solver", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-fp64-compare", action="store_true", help="skip the plain FP64 LU comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle accuracy sample")
-    ap.add_argument("--cpu-energies", type=int, default=6)
+    ap.add_argument("--cpu-energies", type=int, default=None,
+                    help="energies in the CPU baseline / oracle accuracy sample (default: 2 at N > 6000, else 6)")
     ap.add_argument("--caller", default=None, choices=["pot_scf", "screen", "ldos"],
                     help="time a caller of the FMS stage instead of the stage itself (one JSON line of its "
                          "own): pot_scf = the POT self-consistency traces, screen = SCREEN's absorber traces, "
@@ -132,9 +163,11 @@ def run_solver(lib, wl, solver_id, steps, warmup, profile):
 
 
 def cpu_baseline(wl, energies, gg_gpu):
-    """FP64 oracle G0/t assembly + LAPACK LU on the host cores for a bounded sample.
+    """FP64 oracle G0/t assembly + LAPACK LU on all host cores for a bounded sample.
     Times the complex-single LU (refeff's faer path is c32) and uses the FP64
-    solve as the accuracy reference for the GPU result."""
+    solve as the accuracy reference for the GPU result.  Like refeff's rayon + faer
+    path (fms/driver.rs:231-236, lu.rs:385-397) every host core works: the oracle's
+    G0 assembly runs its atom rows in OpenMP threads, the LU in threaded OpenBLAS."""
     import scipy.linalg.lapack as lapack
     from oracle import oracle as orc
 
@@ -164,10 +197,13 @@ def cpu_baseline(wl, energies, gg_gpu):
         errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
         if info or info2:
             raise RuntimeError("LAPACK reported a singular matrix")
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, nproc = host_cpu()
+    cores = int(os.environ.get("OMP_NUM_THREADS", nproc))
     return {"value": len(energies) / t_c32, "unit": "energy-points/s", "cores": cores, "kind": "port",
-            "sample": f"{len(energies)} of {len(k)} energies; oracle FP64 G0/t assembly (1 thread) + "
-                      f"LAPACK cgetrf/cgetrs complex64 (OpenBLAS, {cores} threads)"}, errs, refs
+            "cpu_model": model, "nproc": nproc,
+            "sample": f"{len(energies)} of {len(k)} energies; oracle FP64 G0/t assembly ({cores} OpenMP threads) + "
+                      f"LAPACK cgetrf/cgetrs complex64 (OpenBLAS, {cores} threads); refeff itself (Rust/faer) "
+                      f"cannot be built here"}, errs, refs
 
 
 def pot_scf_case(d, rfms1, ne, lmaxsc):
@@ -452,6 +488,8 @@ def main():
                       if bd.milliseconds[i] > 0 else 0.0}
                for i, name in enumerate(["tmatrix", "build", "panel", "swap_trsm", "gemm", "solve", "residual",
                                          "other", "gemm_small"])}
+    method = ("complex FP32 MFMA LU (3M trailing update) + FP64 iterative refinement, energies batched"
+              if solver_id == 1 else "plain complex FP64 MFMA LU")
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "energy-points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(main_res["seconds"] / args.steps * 1e3, 3),
@@ -459,7 +497,8 @@ def main():
         "dtype": "c64 LU + f64 refinement" if solver_id == 1 else "c128",
         "data": "synthetic (seeded lattice cluster, smooth model phase shifts)",
         "config": {"workload": wl.name, "atoms": wl.nat, "states": n, "padded": main_res["npad"], "energies": ne,
-                   "solver": args.solver, "batch": main_res["batch"], "lu_block": kb,
+                   "solver": args.solver, "method": method, "learned_preconditioner": False,
+                   "batch": main_res["batch"], "lu_block": kb,
                    "refinement_sweeps": int(main_res["stats"].refinement_sweeps)},
         "roofline": {"bound": "mfma",
                      "kernel": ((f"k_gemm_v2<16, 3M> (LU trailing update, K={kb}; 3 real MFMA products per "
@@ -468,7 +507,7 @@ def main():
                                 f"k_gemm_sub<double> (LU trailing update, K={kb})"),
                      "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_TFLOPS[prec], 4),
-                     "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((args.config, args.solver)),
+                     "traffic": traffic_bytes_per_launch(args.config, args.solver),
                      "launches": int(prof.launches[gemm]), "avg_launch_ms": round(g_ms / max(1, prof.launches[gemm]), 4),
                      "complex_equiv_tflops": round(achieved * (8.0 / 6.0 if gemm_3m else 1.0), 3)},
         "kernels_one_profiled_sweep": kernels,
@@ -482,7 +521,8 @@ def main():
                            "speedup_mixed_vs_fp64": round(value / v64, 3),
                            "max_gg_rel_diff_mixed_vs_fp64": max(d)}
     if not args.no_cpu:
-        sample = sorted(set(np.linspace(0, ne - 1, args.cpu_energies).astype(int).tolist()))
+        ncpu = args.cpu_energies if args.cpu_energies else (2 if n > 6000 else 6)
+        sample = sorted(set(np.linspace(0, ne - 1, ncpu).astype(int).tolist()))
         cb, errs, refs = cpu_baseline(wl, sample, main_res["scattering"])
         line["cpu_baseline"] = cb
         # the xmu proxy: the FMS part of xmu is gtr * S02 (refeff ff2x.rs:1306-1333)

@@ -568,13 +568,18 @@ static int g0_matrix(const orc_plan* p, const orc_states* st, const cplx* k, cpl
         first[nat] = s;
     }
     size_t dsz = (size_t)nm * nm * (L + 1);
+    float cut2 = p->rdirec * p->rdirec;
+    /* atom rows in parallel (OpenMP; single-threaded when called from inside a
+     * parallel region); the error of the lowest failing atom row is returned */
+    int rc = ORC_OK, rc_atom = nat, rc_first = ORC_OK;
+#pragma omp parallel
+    {
     cplx* dfw = malloc(sizeof(cplx) * dsz);
     cplx* dbw = malloc(sizeof(cplx) * dsz);
     cplx* clm = malloc(sizeof(cplx) * (size_t)(L + 2) * (2 * L + 3));
-    if (!dfw || !dbw || !clm) { free(first); free(dfw); free(dbw); free(clm); return ORC_NO_MEMORY; }
-    float cut2 = p->rdirec * p->rdirec;
-    int rc = ORC_OK;
-    for (int a = 0; a < nat && rc == ORC_OK; ++a) {
+#pragma omp for schedule(dynamic, 1)
+    for (int a = 0; a < nat; ++a) {
+        int rc = (dfw && dbw && clm) ? ORC_OK : ORC_NO_MEMORY;
         for (int b = 0; b < nat && rc == ORC_OK; ++b) {
             if (a == b) continue;
             /* cutoff evaluated in single precision as refeff does */
@@ -623,8 +628,20 @@ static int g0_matrix(const orc_plan* p, const orc_states* st, const cplx* k, cpl
                     }
             }
         }
+        if (rc) {
+#pragma omp critical(orc_g0_rc)
+            {
+                if (a < rc_atom) {
+                    rc_atom = a;
+                    rc_first = rc;
+                }
+            }
+        }
     }
-    free(first); free(dfw); free(dbw); free(clm);
+    free(dfw); free(dbw); free(clm);
+    }
+    rc = rc_first;
+    free(first);
     return rc;
 }
 
