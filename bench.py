@@ -61,6 +61,11 @@ def host_cpu():
     return model, os.cpu_count() or 1
 
 
+def progress(msg):
+    """One progress line on stderr per finished piece of work (steps, legs, samples)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,15 +134,18 @@ def run_solver(lib, wl, solver_id, steps, warmup, profile):
         dout.download(host)
         gtr[:] = fms.mkgtr_green_trace(host[:, absorber].transpose(0, 2, 1), rkk, lind, bmat, wl.global_lmax, 1)
 
-    for _ in range(warmup):
+    for i in range(warmup):
         step()
+        progress(f"warmup step {i + 1}/{warmup} done (solver {solver_id})")
     synchronize()
     if profile:
         check(lib.refeff_fms_plan_profile(plan.handle, 1 << GEMM_CLASS), "profile")
     synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         step()
+        if steps <= 10 or (i + 1) % 5 == 0:
+            progress(f"timed step {i + 1}/{steps} enqueued+done (solver {solver_id})")
     synchronize()
     t1 = time.perf_counter()
     prof = breakdown = None
@@ -191,6 +199,7 @@ def cpu_baseline(wl, energies, gg_gpu):
         t_c32 += time.perf_counter() - t0
         lu64, piv64, _ = lapack.zgetrf(a)
         x64, _ = lapack.zgetrs(lu64, piv64, rhs)
+        progress(f"cpu baseline / oracle energy {e} done")
         ref = x64[off:off + nb, :]
         refs.append(ref)
         got = gg_gpu[e, int(wl.potentials[0]), :nb, :nb]
