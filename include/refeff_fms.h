@@ -269,6 +269,22 @@ int refeff_fms_solve_batch_device(refeff_fms_plan* plan, int32_t energy_count, c
                                   const double* d_phase_shifts, double* d_scattering, int32_t* status,
                                   refeff_fms_stats* stats, void* stream);
 
+/* No counterpart in refeff: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same as refeff_fms_solve_batch_device, but the batch workspace
+ * comes from a fixed per-device scratch of 256 MiB inside the library instead
+ * of hipMalloc: the call allocates nothing.  Energies run in batches of as many
+ * as the scratch holds (at most the plan's batch); a cluster whose single-energy
+ * workspace exceeds 256 MiB (about N > 3300 for the mixed solver, N > 4000 for
+ * FP64) fails with REFEFF_FMS_NO_MEMORY and refeff_fms_last_error describes it.
+ * Mixed and FP64 LU plans only (REFEFF_FMS_UNSUPPORTED otherwise).  All calls on
+ * one device share that scratch: calls within a process are serialised by a
+ * per-device lock; the scratch keeps the last call's contents (every buffer is
+ * rewritten before it is read). */
+int refeff_fms_solve_batch_device_noalloc(refeff_fms_plan* plan, int32_t energy_count, const double* d_wave_numbers,
+                                          const double* d_phase_shifts, double* d_scattering, int32_t* status,
+                                          refeff_fms_stats* stats, void* stream);
+
 /* Same as refeff_fms_solve_batch, and also returns full_scattering = A^-1 G0
  * (fms_lu_scattering with calculate_full_scattering, solvers.rs:473-477, as
  * engine fms.rs:3659-3665 collects it for save_gg_slice).  The plan must have

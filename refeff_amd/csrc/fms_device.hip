@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -3234,6 +3235,7 @@ struct refeff_fms_plan {
     hipEvent_t ev_asm[2] = {nullptr, nullptr};  // brackets the assembly for refeff_fms_stats
     std::mutex mu;
     unsigned lcalc_mask = ~0u;                   // FEFF lcalc: bit l set = source columns of l solved (iterative)
+    int device = 0;                              // HIP device the plan was created on
     std::vector<int> h_sl, h_ss, h_spot;         // host copies of the state table (l, spin, potential)
     std::vector<refeff_fms_error> energy_err;    // per-energy detail of the last solve call
     // kernel-level instrumentation (refeff_fms_plan_profile)
@@ -3369,38 +3371,55 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
     return b;
 }
 
-int ensure_workspace(refeff_fms_plan* p, int nb) {
-    if (p->ws_batch >= nb) return REFEFF_FMS_OK;
-    free_workspace(p);
+// Every batch buffer for nb energies, in one fixed order.  take(slot, bytes)
+// binds one buffer: hipMalloc for the plan's own workspace, a bump pointer in
+// the no-allocation entry point's scratch.
+template <typename Take>
+bool workspace_layout(refeff_fms_plan* p, int nb, Take&& take) {
     const size_t n2 = static_cast<size_t>(p->npad) * p->npad;
     const size_t nr = static_cast<size_t>(p->npad) * p->nrhs;
     const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
-    bool ok = true;
-    if (mixed) ok = ok && dalloc(&p->d_a32, n2 * nb);
-    ok = ok && dalloc(&p->d_a64, n2 * nb) && dalloc(&p->d_b64, nr * nb) && dalloc(&p->d_x64, nr * nb) &&
-         dalloc(&p->d_r64, nr * nb);
     const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
-    ok = ok && hipMalloc(&p->d_w, nr * nb * cs) == hipSuccess;
-    ok = ok && hipMalloc(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs) == hipSuccess;
-    if (mixed) ok = ok && alloc_calu(p, nb);
+    auto t = [&](auto** slot, size_t bytes) { return take(reinterpret_cast<void**>(slot), bytes); };
+    bool ok = true;
+    if (mixed) ok = ok && t(&p->d_a32, n2 * nb * sizeof(float2));
+    ok = ok && t(&p->d_a64, n2 * nb * sizeof(double2)) && t(&p->d_b64, nr * nb * sizeof(double2)) &&
+         t(&p->d_x64, nr * nb * sizeof(double2)) && t(&p->d_r64, nr * nb * sizeof(double2));
+    ok = ok && t(&p->d_w, nr * nb * cs) && t(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs);
+    if (mixed) {
+        p->calu_cap = ((p->npad + kSelRows - 1) / kSelRows) * kPanel;
+        ok = ok && t(&p->d_cand, 2 * static_cast<size_t>(p->calu_cap) * nb * sizeof(int));
+    }
     if (mixed && p->gemm_bf16) {
         p->split_stride = static_cast<size_t>(3) * (p->gemm_bf16 == 3 ? 3 : 2) * 2 * p->npad * p->outer_block +
                           4 * static_cast<size_t>(p->npad);
-        ok = ok && dalloc(&p->d_split, p->split_stride * nb);
+        ok = ok && t(&p->d_split, p->split_stride * nb * sizeof(uint16_t));
     }
     if (krylov_solver(p->solver))
-        ok = ok && dalloc(&p->d_iter, static_cast<size_t>(kIterVecs) * p->nrhs * p->npad * nb);
+        ok = ok && t(&p->d_iter, static_cast<size_t>(kIterVecs) * p->nrhs * p->npad * nb * sizeof(double2));
     if (p->bicg) {
         const size_t pb = static_cast<size_t>(p->npad / kPanel) * p->nrhs * nb;
-        ok = ok && dalloc(&p->d_kv, static_cast<size_t>(kBicgVecs) * nr * nb) && dalloc(&p->d_pd, pb) &&
-             dalloc(&p->d_pn, pb) && dalloc(&p->d_bcol, static_cast<size_t>(p->nrhs) * nb);
+        ok = ok && t(&p->d_kv, static_cast<size_t>(kBicgVecs) * nr * nb * sizeof(float2)) &&
+             t(&p->d_pd, pb * sizeof(double2)) && t(&p->d_pn, pb * sizeof(double)) &&
+             t(&p->d_bcol, static_cast<size_t>(p->nrhs) * nb * sizeof(BicgCol));
     }
-    ok = ok && dalloc(&p->d_emap, 2 * static_cast<size_t>(nb)) &&
-         dalloc(&p->d_ipiv, static_cast<size_t>(p->npad) * nb) && dalloc(&p->d_perm, static_cast<size_t>(p->npad) * nb) &&
-         dalloc(&p->d_info, nb) && dalloc(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb) &&
-         dalloc(&p->d_k, static_cast<size_t>(p->nsp) * nb) &&
-         dalloc(&p->d_ph, static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot * nb) &&
-         dalloc(&p->d_out, static_cast<size_t>(p->ch) * p->ch * p->npot * nb) && dalloc(&p->d_norms, 4 * static_cast<size_t>(nb));
+    ok = ok && t(&p->d_emap, 2 * static_cast<size_t>(nb) * sizeof(int)) &&
+         t(&p->d_ipiv, static_cast<size_t>(p->npad) * nb * sizeof(int)) &&
+         t(&p->d_perm, static_cast<size_t>(p->npad) * nb * sizeof(int)) && t(&p->d_info, nb * sizeof(int)) &&
+         t(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb * sizeof(double2)) &&
+         t(&p->d_k, static_cast<size_t>(p->nsp) * nb * sizeof(double2)) &&
+         t(&p->d_ph, static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot * nb * sizeof(double2)) &&
+         t(&p->d_out, static_cast<size_t>(p->ch) * p->ch * p->npot * nb * sizeof(double2)) &&
+         t(&p->d_norms, 4 * static_cast<size_t>(nb) * sizeof(double));
+    return ok;
+}
+
+int ensure_workspace(refeff_fms_plan* p, int nb) {
+    if (p->ws_batch >= nb) return REFEFF_FMS_OK;
+    free_workspace(p);
+    const bool ok = workspace_layout(p, nb, [](void** slot, size_t bytes) {
+        return hipMalloc(slot, std::max<size_t>(bytes, 16)) == hipSuccess;
+    });
     if (!ok) {
         free_workspace(p);
         (void)hipGetLastError();
@@ -3408,6 +3427,23 @@ int ensure_workspace(refeff_fms_plan* p, int nb) {
     }
     p->ws_batch = nb;
     return REFEFF_FMS_OK;
+}
+
+// Bytes the layout of nb energies takes when every buffer starts on a 256-byte
+// boundary (the no-allocation scratch carving).
+size_t carved_bytes(refeff_fms_plan* p, int nb) {
+    refeff_fms_plan probe_fields;  // receives throw-away slot values
+    size_t off = 0;
+    // the layout writes its slots into p; run it on a copy of the sizing fields
+    probe_fields.npad = p->npad; probe_fields.n = p->n; probe_fields.nrhs = p->nrhs; probe_fields.nsp = p->nsp;
+    probe_fields.L = p->L; probe_fields.npot = p->npot; probe_fields.ch = p->ch; probe_fields.solver = p->solver;
+    probe_fields.gemm_bf16 = p->gemm_bf16; probe_fields.outer_block = p->outer_block; probe_fields.bicg = p->bicg;
+    workspace_layout(&probe_fields, nb, [&](void** slot, size_t bytes) {
+        *slot = nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return true;
+    });
+    return off;
 }
 
 // Register-blocked panel when the panel's rows fit the register tiles without
@@ -4484,7 +4520,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         std::copy(d->mean_square_displacements, d->mean_square_displacements + sig.size(), sig.begin());
 
     int dev_count = 0;
-    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0 || hipGetDevice(&p->device) != hipSuccess) {
         delete p;
         return fail(REFEFF_FMS_DEVICE_ERROR, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "no HIP device");
     }
@@ -4675,11 +4711,75 @@ static int solve_bicg_common(refeff_fms_plan* p, int32_t ne, const double* k, co
     return rc;
 }
 
+// The plan's workspace pointers, saved while a call runs on the no-allocation
+// scratch and put back afterwards.
+struct WorkspaceSnapshot {
+#define REFEFF_WS_FIELDS(X)                                                                                  \
+    X(d_a32) X(d_a64) X(d_b64) X(d_x64) X(d_r64) X(d_w) X(d_dinv) X(d_cand) X(d_split) X(d_iter) X(d_kv)     \
+    X(d_pd) X(d_pn) X(d_bcol) X(d_emap) X(d_ipiv) X(d_perm) X(d_info) X(d_tmat) X(d_k) X(d_ph) X(d_out)      \
+    X(d_norms)
+#define REFEFF_WS_DECL(f) decltype(refeff_fms_plan::f) f;
+    REFEFF_WS_FIELDS(REFEFF_WS_DECL)
+#undef REFEFF_WS_DECL
+    int ws_batch, calu_cap;
+    size_t split_stride;
+    refeff_fms_plan* p;
+    explicit WorkspaceSnapshot(refeff_fms_plan* q) : p(q) {
+#define REFEFF_WS_SAVE(f) f = q->f;
+        REFEFF_WS_FIELDS(REFEFF_WS_SAVE)
+#undef REFEFF_WS_SAVE
+        ws_batch = q->ws_batch;
+        calu_cap = q->calu_cap;
+        split_stride = q->split_stride;
+    }
+    ~WorkspaceSnapshot() {
+#define REFEFF_WS_LOAD(f) p->f = f;
+        REFEFF_WS_FIELDS(REFEFF_WS_LOAD)
+#undef REFEFF_WS_LOAD
+        p->ws_batch = ws_batch;
+        p->calu_cap = calu_cap;
+        p->split_stride = split_stride;
+    }
+#undef REFEFF_WS_FIELDS
+};
+
+// Device scratch for the no-allocation entry point (carving base and size).
+struct ScratchArena {
+    unsigned char* base = nullptr;
+    size_t bytes = 0;
+};
+
 static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc, int32_t* status,
-                        refeff_fms_stats* st, bool device, hipStream_t ext, double* full_out = nullptr) {
+                        refeff_fms_stats* st, bool device, hipStream_t ext, double* full_out = nullptr,
+                        const ScratchArena* arena = nullptr) {
     if (!p || ne < 0 || (ne > 0 && (!k || !ph || !sc))) return REFEFF_FMS_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(p->mu);
     p->energy_err.assign(static_cast<size_t>(ne), refeff_fms_error{});
+    // no-allocation call: energies per batch as many as the scratch holds
+    // (at most the plan's batch), buffers carved from it, the plan's own
+    // workspace untouched and restored on every return
+    int batch = p->batch;
+    std::unique_ptr<WorkspaceSnapshot> keep;
+    if (arena) {
+        while (batch > 1 && carved_bytes(p, batch) > arena->bytes) batch = std::max(1, batch / 2);
+        while (batch < p->batch && carved_bytes(p, batch + 1) <= arena->bytes) ++batch;
+        if (carved_bytes(p, 1) > arena->bytes) {
+            t_last_error = make_error(REFEFF_FMS_NO_MEMORY, REFEFF_FMS_E_DEVICE, static_cast<int64_t>(carved_bytes(p, 1)),
+                                      static_cast<int64_t>(arena->bytes), 0, 0.0, "scratch too small");
+            std::snprintf(t_last_error.message, sizeof t_last_error.message,
+                          "one energy needs %zu bytes of workspace; the no-allocation scratch holds %zu",
+                          carved_bytes(p, 1), arena->bytes);
+            return REFEFF_FMS_NO_MEMORY;
+        }
+        keep = std::make_unique<WorkspaceSnapshot>(p);
+        size_t off = 0;
+        workspace_layout(p, batch, [&](void** slot, size_t bytes) {
+            *slot = arena->base + off;
+            off += (bytes + 255) / 256 * 256;
+            return true;
+        });
+        p->ws_batch = batch;
+    }
     if (p->bicg && !full_out) {
         const double tb = now_s();
         if (st) std::memset(st, 0, sizeof(*st));
@@ -4697,9 +4797,9 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
     const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
     int rc = REFEFF_FMS_OK;
     if (device && ext) (void)hipStreamSynchronize(ext);
-    for (int e0 = 0; e0 < ne; e0 += p->batch) {
-        const int nb = std::min(p->batch, ne - e0);
-        int r = ensure_workspace(p, nb);
+    for (int e0 = 0; e0 < ne; e0 += batch) {
+        const int nb = std::min(batch, ne - e0);
+        int r = arena ? REFEFF_FMS_OK : ensure_workspace(p, nb);
         if (r) return r;
         const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         if (hipMemcpyAsync(p->d_k, k + 2 * kst * e0, 2 * kst * nb * sizeof(double), kind, p->stream) != hipSuccess ||
@@ -4744,6 +4844,58 @@ int refeff_fms_solve_batch(refeff_fms_plan* p, int32_t ne, const double* k, cons
 int refeff_fms_solve_batch_device(refeff_fms_plan* p, int32_t ne, const double* dk, const double* dph, double* dsc,
                                   int32_t* status, refeff_fms_stats* st, void* stream) {
     return solve_common(p, ne, dk, dph, dsc, status, st, true, static_cast<hipStream_t>(stream));
+}
+
+// ---- no-allocation entry point ----------------------------------------------
+// The batch workspace of refeff_fms_solve_batch_device_noalloc lives in this
+// file-scope device array.  Host code never uses its name as an address: each
+// device's copy is located once with hipGetSymbolAddress (device current) and
+// that pointer is carved into the batch buffers the kernels receive.
+constexpr size_t kNoAllocScratchBytes = static_cast<size_t>(256) << 20;  // 256 MiB per device
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_setup;
+static unsigned char* g_scratch_ptr[kNoAllocMaxDevices] = {};
+static std::mutex g_scratch_use[kNoAllocMaxDevices];  // one call at a time per device in this process
+
+static int scratch_for_device(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return REFEFF_FMS_DEVICE_ERROR;
+    std::lock_guard<std::mutex> lock(g_scratch_setup);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return REFEFF_FMS_DEVICE_ERROR;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_solve_batch_device_noalloc(refeff_fms_plan* p, int32_t ne, const double* dk, const double* dph,
+                                          double* dsc, int32_t* status, refeff_fms_stats* st, void* stream) {
+    t_last_error = refeff_fms_error{};
+    if (!p) return fail(REFEFF_FMS_INVALID_ARGUMENT, REFEFF_FMS_E_INVALID_ARGUMENT, 0, 0, 0, 0.0, "NULL plan");
+    if (p->bicg || p->full || (p->solver != REFEFF_FMS_SOLVER_MIXED && p->solver != REFEFF_FMS_SOLVER_FP64))
+        return fail(REFEFF_FMS_UNSUPPORTED, REFEFF_FMS_E_UNSUPPORTED, p->solver, 0, 0, 0.0,
+                    "the no-allocation solve of a non-LU or full-scattering plan");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return fail(REFEFF_FMS_DEVICE_ERROR, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "hipGetDevice");
+    const bool switch_dev = cur != p->device;
+    if (switch_dev && hipSetDevice(p->device) != hipSuccess)
+        return fail(REFEFF_FMS_DEVICE_ERROR, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "hipSetDevice");
+    unsigned char* base = nullptr;
+    int rc = scratch_for_device(p->device, &base);
+    if (rc) {
+        rc = fail(rc, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "scratch symbol");
+    } else {
+        std::lock_guard<std::mutex> use(g_scratch_use[p->device]);
+        const ScratchArena arena{base, kNoAllocScratchBytes};
+        rc = solve_common(p, ne, dk, dph, dsc, status, st, true, static_cast<hipStream_t>(stream), nullptr, &arena);
+    }
+    if (switch_dev) (void)hipSetDevice(cur);
+    return rc;
 }
 
 int refeff_fms_solve_batch_full(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc,
