@@ -81,6 +81,10 @@ def parse():
                     help="time a caller of the FMS stage instead of the stage itself (one JSON line of its "
                          "own): pot_scf = the POT self-consistency traces, screen = SCREEN's absorber traces, "
                          "ldos = the LDOS gtrNN.bin files on the LDOS stage's own energy grid")
+    ap.add_argument("--learned", action="store_true",
+                    help="measure the BiCGSTAB preconditioners on the CuNiPd alloy R=10 workload (one JSON line of "
+                         "its own): the network trained on 2000 LU solves (125 energies x 16 right-hand sides) vs "
+                         "block-Jacobi and no preconditioner, iterations and time, against the mixed LU")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy (the reference kernel for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling under rocprofv3 --pmc)")
@@ -439,6 +443,101 @@ def run_ldos_caller(args):
     print(json.dumps(line), flush=True)
 
 
+def run_learned(args):
+    """BiCGSTAB with no preconditioner, neighbour-shell block-Jacobi and the learned
+    preconditioner on the alloy workload (CuNiPd fcc R=10, 150 energies).  The
+    network is trained on 2000 LU solves: the mixed LU's 16 right-hand sides at 125
+    energies of a second mesh over the same range, shifted by 0.1 eV, so no
+    evaluation energy is a training energy.  Two evaluations, each checked against
+    the mixed LU's gg: the whole 150-energy sweep with the solver's own iteration
+    cap (energies it does not finish go to the LU), and the upper half of the mesh
+    (energies 75..149, where BiCGSTAB converges) with a cap of 300 iterations."""
+    from refeff_amd import fms, workloads as W
+    from refeff_amd.hip import synchronize
+    wl = W.CONFIGS["cfg5"]()
+    k, ph = wl.wave_numbers(), wl.phase_shifts()
+    ne = len(k)
+    tr = W.alloy(10.0, 125)
+    tr.energies_ev = (W.FERMI_ABOVE_REFERENCE_EV + np.linspace(-10.0, 40.0, 125) + 0.1 +
+                      1j * np.linspace(0.1, 2.0, 125))
+    k_tr, ph_tr = tr.wave_numbers(), tr.phase_shifts()
+    kw = dict(global_lmax=wl.global_lmax, raw_potential_lmax=wl.raw_lmax, direct_cutoff=wl.rdirec,
+              mean_square_displacements=wl.sigsqr(), lfms=wl.lfms)
+    lu_plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, solver=fms.SOLVER_MIXED, **kw)
+    lu_plan.solve(k[:4], ph[:4])  # warm
+    synchronize()
+    t0 = time.perf_counter()
+    lu, st = lu_plan.solve(k, ph)
+    t_lu = time.perf_counter() - t0
+    upper = list(range(75, ne))
+    t0 = time.perf_counter()
+    lu_plan.solve(k[upper], ph[upper])
+    t_lu_upper = time.perf_counter() - t0
+    lu_plan.close()
+    progress(f"mixed LU over {ne} energies: {t_lu:.2f} s; over the upper {len(upper)}: {t_lu_upper:.2f} s")
+    rcut, epochs, lr = 3.7, 400, 3e-3
+    net_plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, solver=fms.SOLVER_MIXED, **kw)
+    net = fms.LearnedPreconditioner(net_plan, rcut, seed=11)
+    theta0 = net.params()
+    t0 = time.perf_counter()
+    st_tr = net.collect(k_tr, ph_tr)
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = net.train(epochs, lr)
+    t_train = time.perf_counter() - t0
+    theta = net.params()
+    progress(f"collected {len(k_tr)} energies ({int((st_tr == 0).sum())} ok) in {t_collect:.2f} s, trained {epochs} "
+             f"epochs in {t_train:.2f} s (loss {hist[0]:.4f} -> {hist[-1]:.4f})")
+    net.close()
+    net_plan.close()
+    runs = {}
+    for part, sel, cap in (("sweep", list(range(ne)), None), ("upper_cap300", upper, 300)):
+        if cap:
+            os.environ["REFEFF_FMS_BICG_MAXIT"] = str(cap)
+        else:
+            os.environ.pop("REFEFF_FMS_BICG_MAXIT", None)
+        for name, kind, params in (("none", fms.PRECOND_NONE, None), ("block_jacobi", fms.PRECOND_BLOCK_JACOBI, None),
+                                   ("network_untrained", fms.PRECOND_LEARNED, theta0),
+                                   ("network_trained", fms.PRECOND_LEARNED, theta)):
+            plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, solver=fms.SOLVER_BICGSTAB, **kw)
+            n2 = None
+            if params is not None:
+                n2 = fms.LearnedPreconditioner(plan, rcut, seed=11)
+                n2.set_params(params)
+            plan.set_preconditioner(kind, n2)
+            plan.solve(k[sel[-2:]], ph[sel[-2:]])  # warm
+            synchronize()
+            t0 = time.perf_counter()
+            sc, st = plan.solve(k[sel], ph[sel])
+            dt = time.perf_counter() - t0
+            s_ = plan.last_stats
+            err = max(float(np.linalg.norm(sc[i] - lu[e]) / np.linalg.norm(lu[e])) for i, e in enumerate(sel))
+            key = f"{part}_{name}"
+            runs[key] = {"energies": len(sel), "krylov_energies": int(s_.krylov_energies),
+                         "krylov_iterations": int(s_.krylov_iterations),
+                         "iterations_per_krylov_energy": round(s_.krylov_iterations / max(1, s_.krylov_energies), 2),
+                         "lu_energies": int(s_.lu_energies), "seconds": round(dt, 4),
+                         "energy_points_per_s": round(len(sel) / dt, 3), "max_gg_rel_diff_vs_mixed_lu": err,
+                         "status_nonzero": int(np.count_nonzero(st))}
+            progress(f"{key}: {runs[key]}")
+            plan.close()
+            if n2:
+                n2.close()
+    os.environ.pop("REFEFF_FMS_BICG_MAXIT", None)
+    line = {"metric": "BiCGSTAB iterations and energy-points/s with the learned preconditioner",
+            "workload": wl.name, "atoms": wl.nat, "states": wl.state_count, "energies": ne,
+            "mixed_lu": {"sweep_seconds": round(t_lu, 4), "sweep_energy_points_per_s": round(ne / t_lu, 3),
+                         "upper_seconds": round(t_lu_upper, 4),
+                         "upper_energy_points_per_s": round(len(upper) / t_lu_upper, 3)},
+            "training": {"energies": len(k_tr), "right_hand_sides": len(k_tr) * 16,
+                         "mesh": "125 points, Re E - E_F = -10..40 eV + 0.1 eV, Im 0.1..2 eV",
+                         "collect_seconds": round(t_collect, 3), "epochs": epochs, "learning_rate": lr,
+                         "train_seconds": round(t_train, 3), "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                         "neighbor_cutoff": rcut},
+            "runs": runs}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of a known size: rocprofv3 --pmc reads FETCH_SIZE / WRITE_SIZE
     for it, and the ratio to nbytes scales the counters of the FMS kernels."""
@@ -468,6 +567,9 @@ def main():
         return
     if args.caller == "ldos":
         run_ldos_caller(args)
+        return
+    if args.learned:
+        run_learned(args)
         return
     if args.copy_calibration:
         copy_calibration()

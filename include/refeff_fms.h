@@ -296,6 +296,54 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, con
                                 int32_t* status, refeff_fms_stats* stats);
 
 /* ---------------------------------------------------------------------- */
+/* BiCGSTAB preconditioners (REFEFF_FMS_SOLVER_BICGSTAB plans)              */
+/* ---------------------------------------------------------------------- */
+
+/* No counterpart in refeff: refeff's iterative branches are FEFF's one-shot
+ * and restarted solvers without a preconditioner.  These change only the
+ * iteration count of the converged BiCGSTAB path; convergence is judged by the
+ * FP64 residual and the mixed LU stays the fallback, so results are the LU's
+ * (crates/refeff-core/src/fms/solvers.rs:366-484) to the solver tolerance. */
+typedef enum refeff_fms_precond {
+    REFEFF_FMS_PRECOND_NONE = 0,
+    REFEFF_FMS_PRECOND_BLOCK_JACOBI = 1, /* inverses of A on neighbour-shell atom groups (<= 64 states), FP32 */
+    REFEFF_FMS_PRECOND_LEARNED = 2       /* M = (I + N_1)(I + N_2): N_h applies per-channel weights, predicted by a
+                                            graph network over the atom blocks, to the neighbour blocks of G0 t */
+} refeff_fms_precond;
+
+/* The learned preconditioner: a small graph network over the atoms of one
+ * cluster (edges: atom pairs closer than neighbor_cutoff), trained on the GPU
+ * by this library's own forward / backward kernels on exact solves of the
+ * mixed LU.  Global lmax <= 3 and one spin channel (REFEFF_FMS_UNSUPPORTED
+ * otherwise).  Parameters start from a fixed function of seed. */
+typedef struct refeff_fms_gnn refeff_fms_gnn;
+int refeff_fms_gnn_create(const refeff_fms_plan* plan, float neighbor_cutoff, uint64_t seed, refeff_fms_gnn** gnn);
+void refeff_fms_gnn_destroy(refeff_fms_gnn* gnn);
+/* param_count: network parameters (float); edge_count: graph edges; samples: training energies held */
+int refeff_fms_gnn_info(const refeff_fms_gnn* gnn, int32_t* param_count, int32_t* edge_count, int32_t* samples);
+int refeff_fms_gnn_get_params(const refeff_fms_gnn* gnn, float* params);
+int refeff_fms_gnn_set_params(refeff_fms_gnn* gnn, const float* params);
+/* Solve energy_count energies (host arrays, layout of refeff_fms_solve_batch) with the
+ * plan's mixed LU and keep each energy's neighbour blocks of G0 t, right-hand sides and
+ * solutions as training samples (energies with a non-OK status are held but get no weight).
+ * The plan must be built on the gnn's cluster. */
+int refeff_fms_gnn_collect(refeff_fms_gnn* gnn, refeff_fms_plan* plan, int32_t energy_count, const double* wave_numbers,
+                           const double* phase_shifts, int32_t* status);
+/* Loss (mean over held samples and columns of log(1 + |M b - x|^2 / |x|^2)) and its gradient
+ * [param_count] at the current parameters. */
+int refeff_fms_gnn_loss_grad(refeff_fms_gnn* gnn, double* loss, float* grad);
+/* epochs full-batch Adam steps (learning_rate); loss_history: [epochs] losses before each step, or NULL. */
+int refeff_fms_gnn_train(refeff_fms_gnn* gnn, int32_t epochs, float learning_rate, double* loss_history);
+/* Network output w of held sample `sample`: [edge_count][2 heads][16 channels] complex (float pairs),
+ * edges in (source atom, destination atom) order. */
+int refeff_fms_gnn_weights(refeff_fms_gnn* gnn, int32_t sample, float* weights);
+/* y = M v for held sample `sample` (its blocks); v, y: [npad][rhs_count] complex doubles. */
+int refeff_fms_gnn_apply(refeff_fms_gnn* gnn, int32_t sample, const double* v, double* y);
+/* Select the BiCGSTAB preconditioner of a plan (gnn: required for LEARNED, built on the
+ * plan's cluster; the plan keeps a reference, so destroy the plan first). */
+int refeff_fms_plan_set_preconditioner(refeff_fms_plan* plan, int32_t kind, refeff_fms_gnn* gnn);
+
+/* ---------------------------------------------------------------------- */
 /* measurement hooks                                                        */
 /* ---------------------------------------------------------------------- */
 

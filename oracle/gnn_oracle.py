@@ -20,7 +20,9 @@ Definitions (one energy, one spin channel, global lmax L <= 3):
   weights    w^h_ab[c] = base_h + (o_ab[h, c, re] + i o_ab[h, c, im]),  base = (e^{i pi/3}, e^{-i pi/3})
   operator   (N_h v)_a = sum_{b in N(a)} (w^h_ab[c(i,j)] Q_ij) v_j,  c(i,j) = 4 l_i + l_j,  Q_ab = (G0 t)_ab = -A_ab
   M          = (I + N_1)(I + N_2):  u = v + N_2 v,  y = u + N_1 u
-  loss       sum over samples (energy, column) of |M b - x|^2 / |x|^2, divided by the sample count
+  loss       mean over samples (energy, column) of log(1 + |M b - x|^2 / |x|^2): energies where
+             the local operator cannot approximate A^-1 (ratios of 1e30 and more) do not
+             swamp the ones it can
 With zero network outputs M = (I + e^{i pi/3} Q_near)(I + e^{-i pi/3} Q_near), which
 is I + Q_near + Q_near^2 (the second-order Neumann series of A^-1 on the neighbour
 pattern), the untrained starting point.
@@ -125,8 +127,9 @@ def loss_grad(theta, hidden, samples):
         u, y = apply_m(f["omega"], Q, src, dst, first, sl, s["b"])
         nx = np.sum(np.abs(s["x"]) ** 2, axis=0)
         r = y - s["x"]
-        loss += float(np.sum(np.sum(np.abs(r) ** 2, axis=0) / nx)) / S
-        G = 2.0 * r / nx[None, :] / S                       # dL/dRe y + i dL/dIm y
+        rs = np.sum(np.abs(r) ** 2, axis=0) / nx
+        loss += float(np.sum(np.log1p(rs))) / S
+        G = 2.0 * r / (nx * (1.0 + rs))[None, :] / S          # dL/dRe y + i dL/dIm y
         # dL/du = G + N1^H G
         Gu = G.copy()
         E = len(src)

@@ -24,6 +24,7 @@ STATUS_NAMES = {
 
 SOLVER_AUTO, SOLVER_MIXED, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGRM, SOLVER_GGGM, SOLVER_GGTF = 0, 1, 2, 3, 4, 5, 6
 SOLVER_BICGSTAB = 7
+PRECOND_NONE, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2
 
 
 class FmsLibraryMissing(RuntimeError):
@@ -130,6 +131,18 @@ def lib():
     L.refeff_fms_run_in_dir.argtypes = [C.c_char_p, i32, i32p, C.c_char_p, C.c_size_t]
     L.refeff_mkgtr_run_in_dir.argtypes = [C.c_char_p, i32p, C.c_char_p, C.c_size_t]
     L.refeff_fms_version.restype = C.c_char_p
+    L.refeff_fms_gnn_create.argtypes = [vp, C.c_float, C.c_uint64, C.POINTER(vp)]
+    L.refeff_fms_gnn_destroy.argtypes = [vp]
+    L.refeff_fms_gnn_destroy.restype = None
+    L.refeff_fms_gnn_info.argtypes = [vp, i32p, i32p, i32p]
+    L.refeff_fms_gnn_get_params.argtypes = [vp, f32p]
+    L.refeff_fms_gnn_set_params.argtypes = [vp, f32p]
+    L.refeff_fms_gnn_collect.argtypes = [vp, vp, i32, f64p, f64p, i32p]
+    L.refeff_fms_gnn_loss_grad.argtypes = [vp, f64p, f32p]
+    L.refeff_fms_gnn_train.argtypes = [vp, i32, C.c_float, f64p]
+    L.refeff_fms_gnn_weights.argtypes = [vp, i32, f32p]
+    L.refeff_fms_gnn_apply.argtypes = [vp, i32, f64p, f64p]
+    L.refeff_fms_plan_set_preconditioner.argtypes = [vp, i32, vp]
     L.refeff_fms_plan_profile.argtypes = [vp, i32]
     L.refeff_fms_plan_profile_read.argtypes = [vp, C.POINTER(Profile)]
     _lib = L

@@ -2338,7 +2338,8 @@ __global__ __launch_bounds__(256) void k_update(double2* X, const typename Cx<T>
 // Krylov vectors: KV[e][vec][npad][nrhs] float2.
 // ---------------------------------------------------------------------------
 constexpr int kBicgVecs = 6;
-enum BicgVec { kBvRh = 0, kBvR = 1, kBvP = 2, kBvV = 3, kBvS = 4, kBvT = 5 };
+constexpr int kBicgVecsPre = 9;  // + p^ = M p, s^ = M s and a temporary of the two-stage M
+enum BicgVec { kBvRh = 0, kBvR = 1, kBvP = 2, kBvV = 3, kBvS = 4, kBvT = 5, kBvPh = 6, kBvSh = 7, kBvTmp = 8 };
 
 struct BicgCol {
     double2 rho, alpha, omega, beta;
@@ -2346,7 +2347,8 @@ struct BicgCol {
     double bb;   // |b|^2
     double rt;   // FP64 residual |b - A64 x|^2 at the last replacement
     int active;  // still iterating
-    int flags;   // 1: breakdown (zero or non-finite scalar)
+    int flags;   // 1: breakdown (zero or non-finite scalar); 2: held (recursive residual within
+                 // the tolerance, waiting for the FP64 residual)
 };
 
 struct BicgArgs {
@@ -2361,6 +2363,7 @@ struct BicgArgs {
     size_t sa, sx, skv;
     int n, nrhs, nblk;
     double tol2;      // squared relative residual tolerance
+    int pre;          // right preconditioning: x is updated along p^ = M p and s^ = M s
 };
 
 __device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {  // conj(a) * b
@@ -2406,7 +2409,8 @@ enum BicgVecMode { kBicgInit = 0, kBicgS = 1, kBicgXR = 2, kBicgP = 3, kBicgRepl
 // of one energy, with the block's partial column sums where the step needs them:
 //   init: x = 0, r = rh = p = f32(b); pd = (rh, r), pn = |b|^2 (FP64)
 //   S:    s = r - alpha v
-//   XR:   x += alpha p + omega s (FP64), r = s - omega t; pd = (rh, r), pn = |r|^2
+//   XR:   x += alpha p + omega s (FP64; p^, s^ when preconditioned), r = s - omega t;
+//         pd = (rh, r), pn = |r|^2
 //   P:    p = r + beta (p - omega v)
 //   repl: r = f32(R64); pn = |R64|^2 (FP64)
 template <int MODE>
@@ -2439,9 +2443,10 @@ __global__ __launch_bounds__(256) void k_bicg_vec(BicgArgs a, const int* emap) {
             const float2 al = make_float2(static_cast<float>(cc.alpha.x), static_cast<float>(cc.alpha.y));
             kv[kBvS * a.sx + o] = csub(rv, cmul(al, v));
         } else if constexpr (MODE == kBicgXR) {
-            const float2 pv = kv[kBvP * a.sx + o], sv = kv[kBvS * a.sx + o], tv = kv[kBvT * a.sx + o];
+            const float2 pv = kv[(a.pre ? kBvPh : kBvP) * a.sx + o], sv = kv[kBvS * a.sx + o], tv = kv[kBvT * a.sx + o];
+            const float2 sh = a.pre ? kv[kBvSh * a.sx + o] : sv;
             double2 x = a.x64[e * a.sx + o];
-            x = cadd(x, cadd(cmul(cc.alpha, to_d(pv)), cmul(cc.omega, to_d(sv))));
+            x = cadd(x, cadd(cmul(cc.alpha, to_d(pv)), cmul(cc.omega, to_d(sh))));
             a.x64[e * a.sx + o] = x;
             const float2 om = make_float2(static_cast<float>(cc.omega.x), static_cast<float>(cc.omega.y));
             const float2 rn = csub(sv, cmul(om, tv));
@@ -2486,7 +2491,8 @@ enum BicgStage { kBicgStInit = 0, kBicgStAlpha = 1, kBicgStOmega = 2, kBicgStBet
 //   init:  rho = (rh, r), bb = |b|^2, rr = rt = bb, active = bb > 0
 //   alpha: alpha = rho / (rh, v)
 //   omega: omega = (t, s) / (t, t)           (pd holds (s, t) = conj((t, s)))
-//   beta:  rho' = (rh, r), beta = (rho'/rho)(alpha/omega), rho = rho', rr = |r|^2
+//   beta:  rho' = (rh, r), beta = (rho'/rho)(alpha/omega), rho = rho', rr = |r|^2;
+//          rr within the tolerance holds the column (flag 2) until the FP64 residual decides
 //   true:  rt = rr = |R64|^2; a column whose FP64 residual is within tol stops
 template <int STAGE>
 __global__ __launch_bounds__(64) void k_bicg_reduce(BicgArgs a, const int* emap) {
@@ -2512,9 +2518,10 @@ __global__ __launch_bounds__(64) void k_bicg_reduce(BicgArgs a, const int* emap)
         } else if constexpr (STAGE == kBicgStTrue) {
             cc.rt = q;
             cc.rr = q;
+            cc.flags &= ~2;
             if (cc.active && q <= a.tol2 * cc.bb) cc.active = 0;
         } else {
-            if (!cc.active) {
+            if (!cc.active || cc.flags) {  // done, or broken down: zero steps keep it where it is
                 cc.alpha = cc.omega = cc.beta = zero;
                 continue;
             }
@@ -2544,6 +2551,7 @@ __global__ __launch_bounds__(64) void k_bicg_reduce(BicgArgs a, const int* emap)
                     if (!finite2(cc.beta)) cc.flags |= 1;
                 }
                 cc.rho = d;
+                if (q <= a.tol2 * cc.bb) cc.flags |= 2;  // hold until the FP64 residual decides
             }
         }
     }
@@ -3179,6 +3187,8 @@ __global__ __launch_bounds__(64) void k_trace(TraceArgs a) {
     if (tid == 0) a.gtr[e] = acc;
 }
 
+#include "fms_precond_kernels.inl"
+
 }  // namespace
 }  // namespace refeff
 
@@ -3257,7 +3267,19 @@ struct refeff_fms_plan {
     int bicg_batch = 32;         // energies per BiCGSTAB device batch
     double bicg_tol = 1.0e-10;   // relative FP64 residual per column
     double bicg_delta = 1.0e-2;  // reliable update once |r| < delta * max |r| since the last one
-    float2* d_kv = nullptr;      // Krylov vectors [e][kBicgVecs][npad][nrhs]
+    float2* d_kv = nullptr;      // Krylov vectors [e][bicg_vecs][npad][nrhs]
+    int bicg_vecs = kBicgVecs;   // vector slots per energy (9 with a preconditioner: p^, s^ and a temporary)
+    int bicg_check = 4;          // iterations between host polls of the per-column scalars
+    // BiCGSTAB preconditioner (refeff_fms_plan_set_preconditioner)
+    int precond = 0;             // REFEFF_FMS_PRECOND_*
+    refeff_fms_gnn* gnn = nullptr;
+    int bj_groups = 0;
+    int *d_bj_state = nullptr, *d_bj_size = nullptr;  // groups of the block-Jacobi preconditioner
+    float2* d_bj_inv = nullptr;  // [e][group][64][64], sized for bicg_batch energies
+    int bj_cap = 0;
+    refeff_fms_gnn* capture = nullptr;  // refeff_fms_gnn_collect: keep training samples of this solve
+    std::vector<float> h_pos;           // host copies: positions, first state and lmax of each atom
+    std::vector<int> h_first, h_atom_lmax;
     double2* d_pd = nullptr;     // block partial dots [e][npad/32][nrhs]
     double* d_pn = nullptr;      // block partial norms [e][npad/32][nrhs]
     BicgCol* d_bcol = nullptr;   // per-column scalars [e][nrhs]
@@ -3365,7 +3387,7 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
         b += (static_cast<size_t>(3) * (p->gemm_bf16 == 3 ? 3 : 2) * 2 * p->npad * p->outer_block +
               4 * static_cast<size_t>(p->npad)) * sizeof(uint16_t);
     if (p->bicg)
-        b += static_cast<size_t>(kBicgVecs) * nr * sizeof(float2) +
+        b += static_cast<size_t>(p->bicg_vecs) * nr * sizeof(float2) +
              static_cast<size_t>(p->npad / kPanel) * p->nrhs * (sizeof(double2) + sizeof(double)) +
              static_cast<size_t>(p->nrhs) * sizeof(BicgCol);
     return b;
@@ -3399,7 +3421,7 @@ bool workspace_layout(refeff_fms_plan* p, int nb, Take&& take) {
         ok = ok && t(&p->d_iter, static_cast<size_t>(kIterVecs) * p->nrhs * p->npad * nb * sizeof(double2));
     if (p->bicg) {
         const size_t pb = static_cast<size_t>(p->npad / kPanel) * p->nrhs * nb;
-        ok = ok && t(&p->d_kv, static_cast<size_t>(kBicgVecs) * nr * nb * sizeof(float2)) &&
+        ok = ok && t(&p->d_kv, static_cast<size_t>(p->bicg_vecs) * nr * nb * sizeof(float2)) &&
              t(&p->d_pd, pb * sizeof(double2)) && t(&p->d_pn, pb * sizeof(double)) &&
              t(&p->d_bcol, static_cast<size_t>(p->nrhs) * nb * sizeof(BicgCol));
     }
@@ -3872,14 +3894,16 @@ bool sync_groups(const std::vector<Group>& gs) {
 // whose recursive residual fell by bicg_delta (or under the tolerance) gets a
 // reliable update R64 = B - A64 X, whose norm decides convergence.  ok[e] = 1:
 // X64 holds the solution; iters[e]: iterations run.
+#include "fms_precond_host.inl"
+
 int run_bicgstab(refeff_fms_plan* p, const Group& g, std::vector<int>& iters, std::vector<char>& ok) {
     const int n = p->npad, nr = p->nrhs, nblk = n / kPanel, chunks = (nr + 15) / 16, nb = g.nb;
     const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
     const size_t pb = static_cast<size_t>(nblk) * nr;
     BicgArgs a{p->d_a32 + sa * g.e0, p->d_b64 + sx * g.e0, p->d_x64 + sx * g.e0, p->d_r64 + sx * g.e0,
-               p->d_kv + kBicgVecs * sx * g.e0, p->d_pd + pb * g.e0, p->d_pn + pb * g.e0,
-               p->d_bcol + static_cast<size_t>(nr) * g.e0, sa, sx, kBicgVecs * sx, n, nr, nblk,
-               p->bicg_tol * p->bicg_tol};
+               p->d_kv + p->bicg_vecs * sx * g.e0, p->d_pd + pb * g.e0, p->d_pn + pb * g.e0,
+               p->d_bcol + static_cast<size_t>(nr) * g.e0, sa, sx, p->bicg_vecs * sx, n, nr, nblk,
+               p->bicg_tol * p->bicg_tol, p->precond != REFEFF_FMS_PRECOND_NONE ? 1 : 0};
     hipStream_t s = g.s;
     std::vector<BicgCol> cols(static_cast<size_t>(nb) * nr);
     auto pull = [&]() {
@@ -3890,6 +3914,10 @@ int run_bicgstab(refeff_fms_plan* p, const Group& g, std::vector<int>& iters, st
     const double nn = static_cast<double>(n) * n;
     iters.assign(nb, 0);
     ok.assign(nb, 0);
+    if (a.pre) {
+        const int r = precond_setup(p, g);
+        if (r) return r;
+    }
     hipLaunchKernelGGL(k_bicg_vec<kBicgInit>, dim3(nblk, nb, chunks), dim3(256), 0, s, a, nullptr);
     hipLaunchKernelGGL(k_bicg_reduce<kBicgStInit>, dim3(nb), dim3(64), 0, s, a, nullptr);
     if (!pull()) return REFEFF_FMS_DEVICE_ERROR;
@@ -3909,34 +3937,45 @@ int run_bicgstab(refeff_fms_plan* p, const Group& g, std::vector<int>& iters, st
     int* em_live = p->d_emap + 2 * g.e0;
     int* em_repl = em_live + nb;
     const double d2 = p->bicg_delta * p->bicg_delta, t2 = p->bicg_tol * p->bicg_tol;
-    for (int it = 1; it <= p->bicg_maxit && !live.empty(); ++it) {
+    // Iterations run in chunks of bicg_check between host polls: the per-column
+    // scalars stop a column on the device (converged or broken down: zero
+    // steps), the host reads them once per chunk to trigger reliable updates,
+    // retire energies and send broken-down ones to the LU.
+    for (int it = 1; it <= p->bicg_maxit && !live.empty();) {
         const int m = static_cast<int>(live.size());
+        const int run = std::max(1, std::min(p->bicg_check, p->bicg_maxit - it + 1));
         if (hipMemcpyAsync(em_live, live.data(), m * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
             return REFEFF_FMS_DEVICE_ERROR;
         const dim3 gmv(nblk, m, chunks), gv(nblk, m, chunks);
-        launch(p, s, REFEFF_FMS_K_SOLVE, 16.0 * nn * nr * m, 16.0 * nn * m, [&] {
-            hipLaunchKernelGGL(k_bicg_matvec, gmv, dim3(512), 0, s, a, static_cast<int>(kBvP), static_cast<int>(kBvV),
-                               static_cast<int>(kBvRh), em_live);
-            hipLaunchKernelGGL(k_bicg_reduce<kBicgStAlpha>, dim3(m), dim3(64), 0, s, a, em_live);
-            hipLaunchKernelGGL(k_bicg_vec<kBicgS>, gv, dim3(256), 0, s, a, em_live);
-            hipLaunchKernelGGL(k_bicg_matvec, gmv, dim3(512), 0, s, a, static_cast<int>(kBvS), static_cast<int>(kBvT),
-                               static_cast<int>(kBvS), em_live);
-            hipLaunchKernelGGL(k_bicg_reduce<kBicgStOmega>, dim3(m), dim3(64), 0, s, a, em_live);
-            hipLaunchKernelGGL(k_bicg_vec<kBicgXR>, gv, dim3(256), 0, s, a, em_live);
-            hipLaunchKernelGGL(k_bicg_reduce<kBicgStBeta>, dim3(m), dim3(64), 0, s, a, em_live);
-            hipLaunchKernelGGL(k_bicg_vec<kBicgP>, gv, dim3(256), 0, s, a, em_live);
-        });
+        for (int q = 0; q < run; ++q) {
+            launch(p, s, REFEFF_FMS_K_SOLVE, 16.0 * nn * nr * m, 16.0 * nn * m, [&] {
+                if (a.pre) precond_apply(p, g, a, kBvP, kBvPh, em_live, m);
+                hipLaunchKernelGGL(k_bicg_matvec, gmv, dim3(512), 0, s, a, static_cast<int>(a.pre ? kBvPh : kBvP),
+                                   static_cast<int>(kBvV), static_cast<int>(kBvRh), em_live);
+                hipLaunchKernelGGL(k_bicg_reduce<kBicgStAlpha>, dim3(m), dim3(64), 0, s, a, em_live);
+                hipLaunchKernelGGL(k_bicg_vec<kBicgS>, gv, dim3(256), 0, s, a, em_live);
+                if (a.pre) precond_apply(p, g, a, kBvS, kBvSh, em_live, m);
+                hipLaunchKernelGGL(k_bicg_matvec, gmv, dim3(512), 0, s, a, static_cast<int>(a.pre ? kBvSh : kBvS),
+                                   static_cast<int>(kBvT), static_cast<int>(kBvS), em_live);
+                hipLaunchKernelGGL(k_bicg_reduce<kBicgStOmega>, dim3(m), dim3(64), 0, s, a, em_live);
+                hipLaunchKernelGGL(k_bicg_vec<kBicgXR>, gv, dim3(256), 0, s, a, em_live);
+                hipLaunchKernelGGL(k_bicg_reduce<kBicgStBeta>, dim3(m), dim3(64), 0, s, a, em_live);
+                hipLaunchKernelGGL(k_bicg_vec<kBicgP>, gv, dim3(256), 0, s, a, em_live);
+            });
+        }
+        it += run;
+        const int done_it = it - 1;
         if (!pull()) return REFEFF_FMS_DEVICE_ERROR;
         next.clear();
         repl.clear();
         for (int e : live) {
-            iters[e] = it;
+            iters[e] = done_it;
             bool brk = false, trig = false;
             for (int c = 0; c < nr; ++c) {
                 const size_t i = static_cast<size_t>(e) * nr + c;
                 const BicgCol& cc = cols[i];
                 if (!cc.active) continue;
-                if (cc.flags || !std::isfinite(cc.rr)) brk = true;
+                if ((cc.flags & 1) || !std::isfinite(cc.rr)) brk = true;
                 if (cc.rr <= d2 * maxrr[i] || cc.rr <= t2 * cc.bb) trig = true;
                 maxrr[i] = std::max(maxrr[i], cc.rr);
             }
@@ -4304,6 +4343,13 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         }
         t_first = now_s();
     }
+    // training samples of the learned preconditioner (refeff_fms_gnn_collect)
+    const int cap_base = p->capture ? p->capture->ns : 0;
+    if (p->capture && mixed && !bo)
+        for (const auto& g : gs) {
+            const int r = gnn_capture_group(p, g);
+            if (r) return r;
+        }
     std::vector<int> info(nb);
     const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
     for (const auto& g : gs) {
@@ -4326,6 +4372,7 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
         if (s && !rc) rc = s;
         const size_t slot = static_cast<size_t>(eidx ? eidx[e] : ebase + e);
         if (slot < p->energy_err.size()) p->energy_err[slot] = energy_detail(p, e, info[e], s, sweeps, est_err[e]);
+        if (p->capture && mixed && !bo) p->capture->sample_ok[cap_base + e] = s == REFEFF_FMS_OK;
     }
     if (st) {
         float asm_ms = 0.f;
@@ -4444,6 +4491,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         for (int l = 0; l <= d->global_lmax; ++l)
             if (d->calculated_l[l]) p->lcalc_mask |= 1u << l;
     }
+    p->h_pos.assign(d->positions, d->positions + 3 * static_cast<size_t>(d->atom_count));
     p->h_sl = stt.l;
     p->h_ss = stt.spin;
     p->h_spot.resize(stt.atom.size());
@@ -4481,6 +4529,8 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
             s += (atom_lmax[a] + 1) * (atom_lmax[a] + 1) * p->nsp;
         }
     }
+    p->h_first = atom_first;
+    p->h_atom_lmax = atom_lmax;
     int nrhs = 0;
     for (int ip = p->pstart; ip <= p->pend; ++ip) {
         if (p->rep[ip] < 0) {
@@ -4585,6 +4635,7 @@ void refeff_fms_plan_destroy(refeff_fms_plan* p) {
     if (!p) return;
     free_workspace(p);
     dfree(p->d_pos); dfree(p->d_sig); dfree(p->d_atom_first); dfree(p->d_atom_lmax); dfree(p->d_atom_rhs);
+    dfree(p->d_bj_state); dfree(p->d_bj_size); dfree(p->d_bj_inv);
     dfree(p->d_sl); dfree(p->d_sm); dfree(p->d_ss); dfree(p->d_spot); dfree(p->d_rep); dfree(p->d_col0);
     dfree(p->d_ipart); dfree(p->d_rhs_src); dfree(p->d_rhs_pot); dfree(p->d_xnlm); dfree(p->d_so_plus); dfree(p->d_so_minus);
     for (hipEvent_t ev : p->pool) (void)hipEventDestroy(ev);
@@ -5101,6 +5152,217 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
 }
 
 #endif  // REFEFF_FMS_TEST_HOOKS
+
+// ---- learned preconditioner and block-Jacobi --------------------------------
+
+static bool same_cluster(const refeff_fms_gnn* q, const refeff_fms_plan* p) {
+    return q && p && q->nat == p->nat && q->pos == p->h_pos && q->n == p->n && q->nrhs == p->nrhs;
+}
+
+int refeff_fms_gnn_create(const refeff_fms_plan* p, float rcut, uint64_t seed, refeff_fms_gnn** out) {
+    if (!p || !out || !std::isfinite(rcut) || !(rcut > 0.f)) return REFEFF_FMS_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (p->nsp != 1 || p->L > 3) return REFEFF_FMS_UNSUPPORTED;  // blocks of at most 16 states
+    auto* q = new refeff_fms_gnn();
+    q->nat = p->nat;
+    q->n = p->n;
+    q->npad = p->npad;
+    q->nrhs = p->nrhs;
+    q->rcut = rcut;
+    q->pos = p->h_pos;
+    q->first = p->h_first;
+    q->nbs.resize(p->nat);
+    for (int a = 0; a < p->nat; ++a) q->nbs[a] = (p->h_atom_lmax[a] + 1) * (p->h_atom_lmax[a] + 1);
+    q->rowp.assign(p->nat + 1, 0);
+    std::vector<int> indeg(p->nat, 0);
+    for (int a = 0; a < p->nat; ++a) {
+        for (int b = 0; b < p->nat; ++b) {
+            if (b == a) continue;
+            double s2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double d = static_cast<double>(q->pos[3 * a + c]) - q->pos[3 * b + c];
+                s2 += d * d;
+            }
+            const double dd = std::sqrt(s2);
+            if (dd < rcut) {
+                q->src.push_back(a);
+                q->dst.push_back(b);
+                q->dist.push_back(static_cast<float>(dd));
+                ++indeg[b];
+            }
+        }
+        q->rowp[a + 1] = static_cast<int>(q->src.size());
+    }
+    q->E = static_cast<int>(q->src.size());
+    q->inp.assign(p->nat + 1, 0);
+    for (int b = 0; b < p->nat; ++b) q->inp[b + 1] = q->inp[b] + indeg[b];
+    q->ine.assign(q->E, 0);
+    {
+        std::vector<int> fill(q->inp.begin(), q->inp.end() - 1);
+        for (int ed = 0; ed < q->E; ++ed) q->ine[fill[q->dst[ed]]++] = ed;
+    }
+    const std::vector<float> init = gnn_initial_params(seed);
+    const std::vector<float> zeros(kGnnParams, 0.0f);
+    bool ok = q->E > 0 && dupload(&q->d_src, q->src.data(), q->src.size()) &&
+              dupload(&q->d_dst, q->dst.data(), q->dst.size()) && dupload(&q->d_rowp, q->rowp.data(), q->rowp.size()) &&
+              dupload(&q->d_inp, q->inp.data(), q->inp.size()) && dupload(&q->d_ine, q->ine.data(), q->ine.size()) &&
+              dupload(&q->d_first, q->first.data(), q->first.size()) &&
+              dupload(&q->d_nbs, q->nbs.data(), q->nbs.size()) && dupload(&q->d_dist, q->dist.data(), q->dist.size()) &&
+              dupload(&q->d_P, init.data(), init.size()) && dupload(&q->d_m1, zeros.data(), zeros.size()) &&
+              dupload(&q->d_m2, zeros.data(), zeros.size()) && dalloc(&q->d_grad, kGnnParams) &&
+              dalloc(&q->d_part, static_cast<size_t>(kGnnChunks) * kGnnParams) && dalloc(&q->d_loss, 1) &&
+              hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        refeff_fms_gnn_destroy(q);
+        (void)hipGetLastError();
+        return q->E > 0 ? REFEFF_FMS_NO_MEMORY : REFEFF_FMS_INVALID_ARGUMENT;
+    }
+    *out = q;
+    return REFEFF_FMS_OK;
+}
+
+void refeff_fms_gnn_destroy(refeff_fms_gnn* q) {
+    if (!q) return;
+    gnn_free_work(q);
+    gnn_free_samples(q);
+    gnn_free_inference(q);
+    dfree_any(q->d_src); dfree_any(q->d_dst); dfree_any(q->d_rowp); dfree_any(q->d_inp); dfree_any(q->d_ine);
+    dfree_any(q->d_first); dfree_any(q->d_nbs); dfree_any(q->d_dist); dfree_any(q->d_P); dfree_any(q->d_grad);
+    dfree_any(q->d_m1); dfree_any(q->d_m2); dfree_any(q->d_part); dfree_any(q->d_loss);
+    if (q->stream) (void)hipStreamDestroy(q->stream);
+    delete q;
+}
+
+int refeff_fms_gnn_info(const refeff_fms_gnn* q, int32_t* param_count, int32_t* edge_count, int32_t* samples) {
+    if (!q) return REFEFF_FMS_INVALID_ARGUMENT;
+    if (param_count) *param_count = kGnnParams;
+    if (edge_count) *edge_count = q->E;
+    if (samples) *samples = q->ns;
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_gnn_get_params(const refeff_fms_gnn* q, float* out) {
+    if (!q || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    return check_hip(hipMemcpy(out, q->d_P, kGnnParams * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+int refeff_fms_gnn_set_params(refeff_fms_gnn* q, const float* in) {
+    if (!q || !in) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(q->mu);
+    return check_hip(hipMemcpy(q->d_P, in, kGnnParams * sizeof(float), hipMemcpyHostToDevice));
+}
+
+int refeff_fms_gnn_collect(refeff_fms_gnn* q, refeff_fms_plan* p, int32_t ne, const double* k, const double* ph,
+                           int32_t* status) {
+    if (!q || !p || ne < 0 || (ne > 0 && (!k || !ph))) return REFEFF_FMS_INVALID_ARGUMENT;
+    if (!same_cluster(q, p)) return REFEFF_FMS_INVALID_ARGUMENT;
+    if (p->solver != REFEFF_FMS_SOLVER_MIXED || p->full) return REFEFF_FMS_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(q->mu);
+    const bool bicg = p->bicg;
+    p->bicg = false;  // the samples are the LU's solves
+    p->capture = q;
+    std::vector<double> sc(2 * static_cast<size_t>(ne) * p->ch * p->ch * p->npot);
+    std::vector<int32_t> st(std::max(1, ne));
+    const int rc = solve_common(p, ne, k, ph, sc.data(), st.data(), nullptr, false, nullptr);
+    p->capture = nullptr;
+    p->bicg = bicg;
+    if (status)
+        for (int e = 0; e < ne; ++e) status[e] = st[e];
+    if (rc == REFEFF_FMS_DEVICE_ERROR || rc == REFEFF_FMS_NO_MEMORY) return rc;
+    int nok = 0;
+    for (int s : q->sample_ok) nok += s ? 1 : 0;
+    std::vector<float> sw(q->ns, 0.0f);
+    for (int s = 0; s < q->ns; ++s)
+        if (q->sample_ok[s]) sw[s] = 1.0f / (static_cast<float>(nok) * q->nrhs);
+    if (q->ns && hipMemcpy(q->d_sw, sw.data(), sw.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return REFEFF_FMS_DEVICE_ERROR;
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_gnn_loss_grad(refeff_fms_gnn* q, double* loss, float* grad) {
+    if (!q || q->ns == 0) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(q->mu);
+    int rc = gnn_epoch(q, 0.0f, q->d_loss);
+    if (rc) return rc;
+    if (hipStreamSynchronize(q->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    if (loss && hipMemcpy(loss, q->d_loss, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    if (grad && hipMemcpy(grad, q->d_grad, kGnnParams * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return REFEFF_FMS_DEVICE_ERROR;
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_gnn_train(refeff_fms_gnn* q, int32_t epochs, float lr, double* history) {
+    if (!q || q->ns == 0 || epochs < 0 || !std::isfinite(lr) || !(lr > 0.f)) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(q->mu);
+    double* d_hist = nullptr;
+    if (hipMalloc(&d_hist, std::max(1, epochs) * sizeof(double)) != hipSuccess) return REFEFF_FMS_NO_MEMORY;
+    int rc = REFEFF_FMS_OK;
+    for (int i = 0; i < epochs && !rc; ++i) rc = gnn_epoch(q, lr, d_hist + i);
+    if (!rc && hipStreamSynchronize(q->stream) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
+    if (!rc && history && epochs > 0 &&
+        hipMemcpy(history, d_hist, epochs * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = REFEFF_FMS_DEVICE_ERROR;
+    (void)hipFree(d_hist);
+    return rc;
+}
+
+int refeff_fms_gnn_weights(refeff_fms_gnn* q, int32_t sample, float* out) {
+    if (!q || !out || sample < 0 || sample >= q->ns) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(q->mu);
+    if (!gnn_reserve_work(q, q->ns)) return REFEFF_FMS_NO_MEMORY;
+    GnnActs ac{q->d_xn, q->d_ze, q->d_m, q->d_g, q->d_om};
+    gnn_forward(q, q->stream, ac, q->ns, nullptr, 0, nullptr, 0, nullptr, false);
+    if (hipStreamSynchronize(q->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    return check_hip(hipMemcpy(out, q->d_om + static_cast<size_t>(sample) * q->E * kGnnOut,
+                               static_cast<size_t>(q->E) * kGnnOut * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+int refeff_fms_gnn_apply(refeff_fms_gnn* q, int32_t sample, const double* v, double* y) {
+    if (!q || !v || !y || sample < 0 || sample >= q->ns) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(q->mu);
+    if (!gnn_reserve_work(q, q->ns)) return REFEFF_FMS_NO_MEMORY;
+    GnnActs ac{q->d_xn, q->d_ze, q->d_m, q->d_g, q->d_om};
+    gnn_forward(q, q->stream, ac, q->ns, nullptr, 0, nullptr, 0, nullptr, false);
+    const size_t sv = static_cast<size_t>(q->npad) * q->nrhs;
+    std::vector<float2> hv(sv);
+    for (size_t i = 0; i < sv; ++i) hv[i] = make_float2(static_cast<float>(v[2 * i]), static_cast<float>(v[2 * i + 1]));
+    if (hipMemcpy(q->d_u, hv.data(), sv * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    const GnnGraph gr = gnn_graph(q);
+    const BlockSrc bs{q->d_q + static_cast<size_t>(sample) * q->E * kGnnB * kGnnB, nullptr, 0, 0};
+    const float* om = q->d_om + static_cast<size_t>(sample) * q->E * kGnnOut;
+    const VecRef vu{q->d_u, sv, q->nrhs}, vt{q->d_G, sv, q->nrhs}, vy{q->d_y, sv, q->nrhs};
+    const int chunks = (q->nrhs + 15) / 16;
+    hipLaunchKernelGGL(k_gnn_apply, dim3(q->nat, 1, chunks), dim3(256), 0, q->stream, gr, om, bs, vu, vt, 1,
+                       static_cast<const int*>(nullptr));
+    hipLaunchKernelGGL(k_gnn_apply, dim3(q->nat, 1, chunks), dim3(256), 0, q->stream, gr, om, bs, vt, vy, 0,
+                       static_cast<const int*>(nullptr));
+    if (hipStreamSynchronize(q->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    if (hipMemcpy(hv.data(), q->d_y, sv * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    for (size_t i = 0; i < sv; ++i) {
+        y[2 * i] = hv[i].x;
+        y[2 * i + 1] = hv[i].y;
+    }
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_set_preconditioner(refeff_fms_plan* p, int32_t kind, refeff_fms_gnn* q) {
+    if (!p || kind < REFEFF_FMS_PRECOND_NONE || kind > REFEFF_FMS_PRECOND_LEARNED) return REFEFF_FMS_INVALID_ARGUMENT;
+    if (!p->bicg && kind != REFEFF_FMS_PRECOND_NONE) return REFEFF_FMS_UNSUPPORTED;  // BiCGSTAB plans only
+    if (kind == REFEFF_FMS_PRECOND_LEARNED && !same_cluster(q, p)) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (kind == REFEFF_FMS_PRECOND_BLOCK_JACOBI && !p->d_bj_state) {
+        const int r = bj_build_groups(p);
+        if (r) return r;
+    }
+    p->precond = kind;
+    p->gnn = kind == REFEFF_FMS_PRECOND_LEARNED ? q : nullptr;
+    const int vecs = kind == REFEFF_FMS_PRECOND_NONE ? kBicgVecs : kBicgVecsPre;
+    if (vecs != p->bicg_vecs) {
+        p->bicg_vecs = vecs;
+        free_workspace(p);  // re-laid out on the next solve
+    }
+    return REFEFF_FMS_OK;
+}
 
 int refeff_fms_plan_profile(refeff_fms_plan* p, int32_t enable) {
     if (!p) return REFEFF_FMS_INVALID_ARGUMENT;

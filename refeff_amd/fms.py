@@ -12,7 +12,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (SOLVER_AUTO, SOLVER_BICGSTAB, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGGM, SOLVER_GGRM, SOLVER_GGTF,
+from ._lib import (PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_NONE, SOLVER_AUTO, SOLVER_BICGSTAB, SOLVER_FP64, SOLVER_GGBI, SOLVER_GGGM, SOLVER_GGRM, SOLVER_GGTF,
                    SOLVER_MIXED,
                    ERROR_VARIANTS, ErrorDetail, FmsError, PlanDesc, PlanInfo, Stats, check, last_error, lib)
 
@@ -188,6 +188,13 @@ class FmsRealSpacePlan:
         except Exception:
             pass
 
+    def set_preconditioner(self, kind: int, learned=None):
+        """BiCGSTAB preconditioner: PRECOND_NONE, PRECOND_BLOCK_JACOBI or PRECOND_LEARNED
+        (with a LearnedPreconditioner built on this plan's cluster)."""
+        check(lib().refeff_fms_plan_set_preconditioner(self._h, int(kind), learned.handle if learned else None),
+              "plan_set_preconditioner")
+        self._learned = learned  # the plan holds a reference
+
     def energy_error(self, energy: int):
         """FmsError detail of one energy of the last solve: (variant, values, real_value, name, message)."""
         e = ErrorDetail()
@@ -284,3 +291,85 @@ def mkgtr_green_trace(gg, rkk, lind, bmat_flat, lmax: int, spin_channels: int):
                                        _f64p(ggc.view(np.float64)), _f64p(rk.view(np.float64)),
                                        _f64p(out.view(np.float64))), "mkgtr_green_trace")
     return out
+
+
+class LearnedPreconditioner:
+    """Graph network over the atom blocks of one cluster that emits the BiCGSTAB
+    preconditioner M = (I + N_1)(I + N_2) (include/refeff_fms.h, refeff_fms_gnn_*),
+    trained on the GPU on exact solves of the plan's mixed LU."""
+
+    def __init__(self, plan: FmsRealSpacePlan, neighbor_cutoff: float, seed: int = 0):
+        self._h = C.c_void_p()
+        check(lib().refeff_fms_gnn_create(plan.handle, float(neighbor_cutoff), int(seed), C.byref(self._h)),
+              "gnn_create")
+        pc, ec, ns = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().refeff_fms_gnn_info(self._h, C.byref(pc), C.byref(ec), C.byref(ns)), "gnn_info")
+        self.param_count, self.edge_count = pc.value, ec.value
+        self.plan = plan
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def samples(self) -> int:
+        ns = C.c_int32()
+        check(lib().refeff_fms_gnn_info(self._h, None, None, C.byref(ns)), "gnn_info")
+        return ns.value
+
+    def close(self):
+        if self._h:
+            lib().refeff_fms_gnn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.param_count, np.float32)
+        check(lib().refeff_fms_gnn_get_params(self._h, _f32p(out)), "gnn_get_params")
+        return out
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(theta, np.float32)
+        check(lib().refeff_fms_gnn_set_params(self._h, _f32p(t)), "gnn_set_params")
+
+    def collect(self, wave_numbers, phase_shifts):
+        """Solve the energies with the plan's mixed LU and keep them as training samples."""
+        nsp, L = self.plan.spin_channels, self.plan.global_lmax
+        k = np.ascontiguousarray(np.asarray(wave_numbers, np.complex128).reshape(-1, nsp))
+        ne = k.shape[0]
+        ph = np.ascontiguousarray(np.asarray(phase_shifts, np.complex128).reshape(ne, nsp, 2 * L + 1,
+                                                                                   self.plan.potential_count))
+        status = np.zeros(ne, np.int32)
+        check(lib().refeff_fms_gnn_collect(self._h, self.plan.handle, ne, _f64p(k.view(np.float64)),
+                                           _f64p(ph.view(np.float64)), _i32p(status)), "gnn_collect")
+        return status
+
+    def loss_grad(self):
+        loss = np.zeros(1)
+        grad = np.zeros(self.param_count, np.float32)
+        check(lib().refeff_fms_gnn_loss_grad(self._h, _f64p(loss), _f32p(grad)), "gnn_loss_grad")
+        return float(loss[0]), grad
+
+    def train(self, epochs: int, learning_rate: float):
+        hist = np.zeros(max(1, epochs))
+        check(lib().refeff_fms_gnn_train(self._h, int(epochs), float(learning_rate), _f64p(hist)), "gnn_train")
+        return hist[:epochs]
+
+    def weights(self, sample: int) -> np.ndarray:
+        """[edges, 2, 16] complex network output w of one held sample."""
+        out = np.zeros(self.edge_count * 64, np.float32)
+        check(lib().refeff_fms_gnn_weights(self._h, int(sample), _f32p(out)), "gnn_weights")
+        return out.reshape(-1, 2, 16, 2)[..., 0] + 1j * out.reshape(-1, 2, 16, 2)[..., 1]
+
+    def apply(self, sample: int, v):
+        """M v with held sample `sample`'s blocks; v [npad, rhs] complex."""
+        vv = np.ascontiguousarray(v, np.complex128)
+        y = np.zeros_like(vv)
+        check(lib().refeff_fms_gnn_apply(self._h, int(sample), _f64p(vv.view(np.float64).reshape(-1)),
+                                         _f64p(y.view(np.float64).reshape(-1))), "gnn_apply")
+        return y
