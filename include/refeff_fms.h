@@ -296,6 +296,68 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, con
                                 int32_t* status, refeff_fms_stats* stats);
 
 /* ---------------------------------------------------------------------- */
+/* autotuner and plan sets (many clusters in one graph launch)              */
+/* ---------------------------------------------------------------------- */
+
+/* No counterpart in refeff: faer picks its own blocking
+ * (crates/refeff-linalg/src/lu.rs:385-397).  The LU variant of a plan:
+ * outer block width (K of the trailing MFMA update), the row count from which
+ * FP32 panels use tournament pivoting, the 3M complex product, and the fixed
+ * refinement sweeps a plan set's graph runs (mixed solver). */
+typedef struct refeff_fms_tuning {
+    int32_t lu_block;
+    int32_t calu_min_rows;
+    int32_t gemm_3m;
+    int32_t graph_sweeps;
+    double seconds_per_energy;  /* measured by refeff_fms_plan_tune (0 otherwise) */
+} refeff_fms_tuning;
+
+int refeff_fms_plan_get_tuning(const refeff_fms_plan* plan, refeff_fms_tuning* tuning);
+int refeff_fms_plan_set_tuning(refeff_fms_plan* plan, const refeff_fms_tuning* tuning);
+/* Time the LU variants on up to 8 of the given energies (host arrays, layout of
+ * refeff_fms_solve_batch), keep the fastest on the plan and return it; the
+ * graph sweeps are one more than the eager refinement needed (at most 8). */
+int refeff_fms_plan_tune(refeff_fms_plan* plan, int32_t energy_count, const double* wave_numbers,
+                         const double* phase_shifts, refeff_fms_tuning* best);
+
+/* A plan set: one plan per cluster (descriptors as for refeff_fms_plan_create;
+ * mixed or FP64 LU solvers), solved together.  Replaces the per-caller loop of
+ * fms_real_space_plan / fms_real_space_spectrum over clusters
+ * (crates/refeff-engine/src/fms.rs:1396-1458, 2154-2206, 2896-2931) with one
+ * graph launch.  tune_cache: path of the autotuner's text cache (read at
+ * creation, written by refeff_fms_plan_set_tune), or NULL. */
+typedef struct refeff_fms_plan_set refeff_fms_plan_set;
+
+typedef struct refeff_fms_set_stats {
+    double seconds_total;
+    double seconds_capture;     /* graph capture + instantiation (0 when the kept graph was replayed) */
+    double seconds_graph;       /* the graph launch, to completion */
+    int64_t graph_nodes;
+    int32_t captured;           /* 1: this call captured a new graph */
+    int32_t resolved_energies;  /* energies the fixed sweeps left unsettled, solved again eagerly */
+    int32_t energies;
+} refeff_fms_set_stats;
+
+int refeff_fms_plan_set_create(const refeff_fms_plan_desc* descs, int32_t cluster_count, const char* tune_cache,
+                               refeff_fms_plan_set** set);
+void refeff_fms_plan_set_destroy(refeff_fms_plan_set* set);
+/* The plan of one cluster (owned by the set; query, tune or solve it alone). */
+int refeff_fms_plan_set_plan(refeff_fms_plan_set* set, int32_t cluster, refeff_fms_plan** plan);
+/* Tune every cluster whose size the cache does not cover yet, on its own energies
+ * (host arrays per cluster), once per size bucket; writes the cache. */
+int refeff_fms_plan_set_tune(refeff_fms_plan_set* set, const int32_t* energy_counts, const double* const* wave_numbers,
+                             const double* const* phase_shifts, int32_t* tuned);
+/* Solve every cluster's energies: per cluster c, energy_counts[c] energies with
+ * device arrays d_wave_numbers[c], d_phase_shifts[c] (layouts of
+ * refeff_fms_solve_batch_device) and output d_scattering[c]; status: host
+ * [sum of energy_counts], clusters in order.  The first call (and any call
+ * with other pointers or counts) captures and instantiates the graph; later
+ * calls replay it. */
+int refeff_fms_plan_set_solve(refeff_fms_plan_set* set, const int32_t* energy_counts, const double* const* d_wave_numbers,
+                              const double* const* d_phase_shifts, double* const* d_scattering, int32_t* status,
+                              refeff_fms_set_stats* stats);
+
+/* ---------------------------------------------------------------------- */
 /* BiCGSTAB preconditioners (REFEFF_FMS_SOLVER_BICGSTAB plans)              */
 /* ---------------------------------------------------------------------- */
 

@@ -143,6 +143,15 @@ def lib():
     L.refeff_fms_gnn_weights.argtypes = [vp, i32, f32p]
     L.refeff_fms_gnn_apply.argtypes = [vp, i32, f64p, f64p]
     L.refeff_fms_plan_set_preconditioner.argtypes = [vp, i32, vp]
+    L.refeff_fms_plan_get_tuning.argtypes = [vp, vp]
+    L.refeff_fms_plan_set_tuning.argtypes = [vp, vp]
+    L.refeff_fms_plan_tune.argtypes = [vp, i32, f64p, f64p, vp]
+    L.refeff_fms_plan_set_create.argtypes = [vp, i32, C.c_char_p, C.POINTER(vp)]
+    L.refeff_fms_plan_set_destroy.argtypes = [vp]
+    L.refeff_fms_plan_set_destroy.restype = None
+    L.refeff_fms_plan_set_plan.argtypes = [vp, i32, C.POINTER(vp)]
+    L.refeff_fms_plan_set_tune.argtypes = [vp, vp, vp, vp, i32p]
+    L.refeff_fms_plan_set_solve.argtypes = [vp, vp, vp, vp, vp, i32p, vp]
     L.refeff_fms_plan_profile.argtypes = [vp, i32]
     L.refeff_fms_plan_profile_read.argtypes = [vp, C.POINTER(Profile)]
     _lib = L

@@ -3246,6 +3246,7 @@ struct refeff_fms_plan {
     std::mutex mu;
     unsigned lcalc_mask = ~0u;                   // FEFF lcalc: bit l set = source columns of l solved (iterative)
     int device = 0;                              // HIP device the plan was created on
+    int graph_sweeps = 4;                        // fixed refinement sweeps inside a plan set's graph (tuned)
     std::vector<int> h_sl, h_ss, h_spot;         // host copies of the state table (l, spin, potential)
     std::vector<refeff_fms_error> energy_err;    // per-energy detail of the last solve call
     // kernel-level instrumentation (refeff_fms_plan_profile)
@@ -4142,8 +4143,20 @@ refeff_fms_error energy_detail(refeff_fms_plan* p, int e, int info, int status, 
     return make_error(status, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "solve");
 }
 
+// Graph mode of solve_device_batch (plan sets): every launch goes to one
+// stream that a HIP graph capture has joined, nothing waits on the host, the
+// mixed solver refines a fixed number of sweeps over every energy, and the
+// per-energy results the host needs afterwards are copied to device arrays:
+// info [nb], and the correction norms [sweep][nb][4] (k_update's layout).
+struct GraphCtx {
+    hipStream_t s;
+    int sweeps;
+    int* info_out;
+    double* norms_out;
+};
+
 int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status, refeff_fms_stats* st,
-                       BicgOut* bo = nullptr, int ebase = 0, const int* eidx = nullptr) {
+                       BicgOut* bo = nullptr, int ebase = 0, const int* eidx = nullptr, const GraphCtx* gc = nullptr) {
     const int n = p->npad;
     const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * p->nrhs;
     const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
@@ -4155,12 +4168,12 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     const double t0 = now_s();
     std::vector<Group> gs;
     {
-        const int G = bo ? 1 : std::max(1, std::min(p->nstreams, nb));
+        const int G = (bo || gc) ? 1 : std::max(1, std::min(p->nstreams, nb));
         int e0 = 0;
         for (int i = 0; i < G; ++i) {
             const int cnt = nb / G + (i < nb % G ? 1 : 0);
-            Group g{p->streams[i] ? p->streams[i] : p->stream, e0, cnt};
-            if (p->lookahead && p->aux[i]) {
+            Group g{gc ? gc->s : (p->streams[i] ? p->streams[i] : p->stream), e0, cnt};
+            if (!gc && p->lookahead && p->aux[i]) {
                 g.s2 = p->aux[i];
                 g.ev_u12 = p->aux_ev[i][0];
                 g.ev_far = p->aux_ev[i][1];
@@ -4233,7 +4246,61 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     std::vector<double> norms(4 * static_cast<size_t>(nb));
     std::vector<double> est_err(nb, 0.0);  // estimated relative error left after refinement
     const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
-    if (bo) {
+    if (gc) {
+        // fixed sweeps over every energy, norms to the graph's device array
+        for (const auto& g : gs) {
+            if (mixed) {
+                float2* w = static_cast<float2*>(p->d_w);
+                hipLaunchKernelGGL(k_fill_double, dim3(std::max<size_t>(1, std::min<size_t>(65535, (2 * sx * g.nb + 255) / 256))),
+                                   dim3(256), 0, g.s, reinterpret_cast<double*>(p->d_x64 + sx * g.e0), 0.0, 2 * sx * g.nb);
+                for (int sweep = 0; sweep < gc->sweeps; ++sweep) {
+                    double* gn = p->d_norms + 4 * static_cast<size_t>(g.e0);
+                    hipLaunchKernelGGL(k_fill_double, dim3((4 * g.nb + 255) / 256), dim3(256), 0, g.s, gn, 0.0,
+                                       static_cast<size_t>(4 * g.nb));
+                    const double2* rhs = p->d_b64 + sx * g.e0;
+                    if (sweep > 0) {
+                        const int ntile = p->nrhs <= 16 ? 1 : 2;
+                        const int chunks = (p->nrhs + 16 * ntile - 1) / (16 * ntile);
+                        const dim3 grid(n / kPanel, g.nb, chunks), block(512);
+                        if (ntile == 1)
+                            hipLaunchKernelGGL(k_residual_wg<1>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
+                                               p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
+                                               p->nrhs, n, gn, static_cast<const int*>(nullptr));
+                        else
+                            hipLaunchKernelGGL(k_residual_wg<2>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
+                                               p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
+                                               p->nrhs, n, gn, static_cast<const int*>(nullptr));
+                        rhs = p->d_r64 + sx * g.e0;
+                    }
+                    hipLaunchKernelGGL(k_gather<float>, dim3(gx, g.nb), dim3(256), 0, g.s, rhs, sx,
+                                       p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0,
+                                       static_cast<const int*>(nullptr));
+                    run_solve<float>(p, g, p->d_a32 + sa * g.e0,
+                                     static_cast<const float2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
+                                     w + sx * g.e0);
+                    hipLaunchKernelGGL(k_update<float>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_x64 + sx * g.e0,
+                                       w + sx * g.e0, sx, n, p->nrhs, gn, static_cast<const int*>(nullptr));
+                    (void)hipMemcpyAsync(gc->norms_out + (static_cast<size_t>(sweep) * nb + g.e0) * 4, gn,
+                                         4 * g.nb * sizeof(double), hipMemcpyDeviceToDevice, g.s);
+                }
+            } else {
+                hipLaunchKernelGGL(k_gather<double>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, sx,
+                                   p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, p->d_x64 + sx * g.e0,
+                                   static_cast<const int*>(nullptr));
+                run_solve<double>(p, g, p->d_a64 + sa * g.e0,
+                                  static_cast<const double2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
+                                  p->d_x64 + sx * g.e0);
+            }
+            const size_t per = static_cast<size_t>(p->ch) * p->ch * p->npot;
+            ExtractArgs ea{p->d_x64 + sx * g.e0, sx, p->nrhs, p->ch, p->npot, p->pstart, p->pend, p->d_rep, p->d_col0,
+                           p->d_ipart, out + sst * g.e0, p->d_sl, ~0u};
+            hipLaunchKernelGGL(k_extract, dim3(std::max<size_t>(1, std::min<size_t>(64, (per + 255) / 256)), g.nb),
+                               dim3(256), 0, g.s, ea);
+            (void)hipMemcpyAsync(gc->info_out + g.e0, p->d_info + g.e0, g.nb * sizeof(int), hipMemcpyDeviceToDevice,
+                                 g.s);
+        }
+        return hipGetLastError() == hipSuccess ? REFEFF_FMS_OK : REFEFF_FMS_DEVICE_ERROR;
+    } else if (bo) {
         t_first = now_s();
     } else if (mixed) {
         float2* w = static_cast<float2*>(p->d_w);
@@ -5426,3 +5493,5 @@ int refeff_fms_mkgtr_trace(int32_t ne, int32_t nch, int32_t nsp, int32_t lmax, c
 }
 
 }  // extern "C"
+
+#include "fms_planset_host.inl"

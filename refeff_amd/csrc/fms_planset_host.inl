@@ -1,0 +1,468 @@
+// Plan sets and the autotuner (included at the end of fms_device.hip).
+//
+// A plan set solves many clusters of uneven size and lmax (each with its own
+// energy list) in one HIP graph launch: every cluster is a branch of the graph,
+// forked from one origin stream and joined back, its energies in batches on its
+// own stream; branches are enqueued largest first.  Inside the graph the mixed
+// solver refines a fixed number of sweeps over every energy and leaves its
+// correction norms in device memory; after the launch the host applies the
+// same convergence rule as the eager path and re-solves any energy the fixed
+// sweeps did not settle.  The instantiated graph is kept and replayed while
+// the call's device pointers and energy counts stay the same.
+//
+// The autotuner times the LU variants (outer block, tall-panel method) of a
+// plan on a few of its energies, keeps the fastest, records the refinement
+// sweeps it needed, and caches the choice per (padded order, right-hand sides,
+// solver) in a text file.
+
+namespace {
+
+bool tuning_valid(const refeff_fms_tuning* t, int npad) {
+    return t && t->lu_block >= kPanel && t->lu_block % kPanel == 0 && t->lu_block <= 1024 && t->lu_block <= npad &&
+           t->calu_min_rows >= 0 && t->graph_sweeps >= 1 && t->graph_sweeps <= 16;
+}
+
+void apply_tuning(refeff_fms_plan* p, const refeff_fms_tuning& t) {
+    p->outer_block = t.lu_block;
+    p->calu_min_rows = t.calu_min_rows;
+    p->gemm_3m = t.gemm_3m != 0;
+    p->graph_sweeps = t.graph_sweeps;
+}
+
+refeff_fms_tuning current_tuning(const refeff_fms_plan* p) {
+    refeff_fms_tuning t{};
+    t.lu_block = p->outer_block;
+    t.calu_min_rows = p->calu_min_rows;
+    t.gemm_3m = p->gemm_3m ? 1 : 0;
+    t.graph_sweeps = p->graph_sweeps;
+    return t;
+}
+
+struct TuneEntry {
+    int npad, nrhs, solver;
+    refeff_fms_tuning t;
+};
+
+std::vector<TuneEntry> read_tune_cache(const std::string& path) {
+    std::vector<TuneEntry> out;
+    if (path.empty()) return out;
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return out;
+    char arch[32];
+    TuneEntry e{};
+    while (std::fscanf(f, "%31s %d %d %d %d %d %d %d %lf", arch, &e.npad, &e.nrhs, &e.solver, &e.t.lu_block,
+                       &e.t.calu_min_rows, &e.t.gemm_3m, &e.t.graph_sweeps, &e.t.seconds_per_energy) == 9)
+        if (std::strcmp(arch, "gfx950") == 0) out.push_back(e);
+    std::fclose(f);
+    return out;
+}
+
+bool write_tune_cache(const std::string& path, const std::vector<TuneEntry>& es) {
+    if (path.empty()) return true;
+    const std::string tmp = path + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) return false;
+    for (const auto& e : es)
+        std::fprintf(f, "gfx950 %d %d %d %d %d %d %d %.6e\n", e.npad, e.nrhs, e.solver, e.t.lu_block, e.t.calu_min_rows,
+                     e.t.gemm_3m, e.t.graph_sweeps, e.t.seconds_per_energy);
+    const bool ok = std::fclose(f) == 0;
+    return ok && std::rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+// The cached choice for a plan: same right-hand sides and solver, padded order
+// within 1/8 (the nearest such entry), else none.
+const TuneEntry* lookup_tuning(const std::vector<TuneEntry>& es, const refeff_fms_plan* p) {
+    const TuneEntry* best = nullptr;
+    for (const auto& e : es) {
+        if (e.nrhs != p->nrhs || e.solver != p->solver) continue;
+        const int d = std::abs(e.npad - p->npad);
+        if (8 * d > p->npad) continue;
+        if (!best || d < std::abs(best->npad - p->npad)) best = &e;
+    }
+    return best;
+}
+
+}  // namespace
+
+struct refeff_fms_plan_set {
+    std::vector<refeff_fms_plan*> plans;
+    std::vector<int> order;        // clusters by decreasing work (npad^3 per energy)
+    std::string tune_cache;
+    std::vector<TuneEntry> tuned;  // cache contents (read at creation, grown by tuning)
+    std::vector<char> has_tuning;  // per cluster: a cached or tuned choice was applied
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void*> key;  // pointers and counts the graph was captured for
+    std::vector<int> gbatch;       // energies per batch of each cluster in the graph
+    std::vector<size_t> eoff;      // first energy of each cluster in the concatenated lists
+    std::vector<int> sweeps;       // fixed refinement sweeps of each cluster
+    int* d_info = nullptr;
+    double* d_norms = nullptr;     // per batch [sweep][nb][4]
+    std::vector<size_t> norm_off;  // per cluster: offset of its first batch's norms
+    size_t graph_nodes = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+void plan_set_drop_graph(refeff_fms_plan_set* s) {
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+    s->key.clear();
+    if (s->d_info) (void)hipFree(s->d_info);
+    if (s->d_norms) (void)hipFree(s->d_norms);
+    s->d_info = nullptr;
+    s->d_norms = nullptr;
+}
+
+// Per-energy status from the graph's info and fixed-sweep norms, by the rule of
+// the eager path: after a correction of relative size d_s (previous d_{s-1})
+// the error left is about d_s min(1, d_s / d_{s-1}); converged when that is
+// <= 1e-9, or when a correction <= 1e-9 stopped shrinking.
+int graph_energy_status(const refeff_fms_plan* p, int info, const double* nrm, int nb, int j, int sweeps,
+                        double* est_out) {
+    const int s = map_info(info);
+    if (s || p->solver != REFEFF_FMS_SOLVER_MIXED) return s;
+    double prev = 1.0e300, est = 1.0e300;
+    for (int w = 0; w < sweeps; ++w) {
+        const double* q = nrm + (static_cast<size_t>(w) * nb + j) * 4;
+        const double rel = q[2] > 0 ? q[1] / q[2] : 0.0;
+        const bool floor = rel <= 1.0e-9 && rel >= 0.5 * prev;
+        est = (w == 0) ? rel : (floor ? rel : rel * std::min(1.0, rel / prev));
+        prev = rel;
+    }
+    *est_out = est;
+    return (est <= 1.0e-9) ? REFEFF_FMS_OK : REFEFF_FMS_NO_CONVERGENCE;
+}
+
+}  // namespace
+
+extern "C" {
+
+int refeff_fms_plan_get_tuning(const refeff_fms_plan* p, refeff_fms_tuning* out) {
+    if (!p || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    *out = current_tuning(p);
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_set_tuning(refeff_fms_plan* p, const refeff_fms_tuning* t) {
+    if (!p || !tuning_valid(t, p->npad)) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(p->mu);
+    apply_tuning(p, *t);
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, refeff_fms_tuning* best) {
+    if (!p || ne <= 0 || !k || !ph) return REFEFF_FMS_INVALID_ARGUMENT;
+    if (p->solver != REFEFF_FMS_SOLVER_MIXED && p->solver != REFEFF_FMS_SOLVER_FP64) return REFEFF_FMS_UNSUPPORTED;
+    const int nt = std::min<int>(ne, std::max(1, std::min(p->batch, 8)));
+    std::vector<double> sc(2 * static_cast<size_t>(nt) * p->ch * p->ch * p->npot);
+    std::vector<int32_t> st(nt);
+    std::vector<refeff_fms_tuning> cands;
+    for (int lb : {64, 128, 192, 256, 320})
+        for (int calu : {1024, 3072, 1 << 30}) {
+            if (lb > p->npad || (calu != 3072 && p->solver != REFEFF_FMS_SOLVER_MIXED)) continue;
+            if (calu == 1024 && p->npad < 1024) continue;  // same as 3072 below 1024 rows
+            if (calu == 3072 && p->npad < 3072 && p->solver == REFEFF_FMS_SOLVER_MIXED) continue;  // same as never
+            refeff_fms_tuning t{};
+            t.lu_block = lb;
+            t.calu_min_rows = calu;
+            t.gemm_3m = 1;
+            t.graph_sweeps = 8;
+            cands.push_back(t);
+        }
+    const refeff_fms_tuning keep = current_tuning(p);
+    refeff_fms_tuning win = keep;
+    double best_s = 1.0e300;
+    int best_sweeps = 1;
+    for (const auto& c : cands) {
+        apply_tuning(p, c);
+        refeff_fms_stats s1{};
+        int r = solve_common(p, nt, k, ph, sc.data(), st.data(), &s1, false, nullptr);  // warm
+        if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) {
+            apply_tuning(p, keep);
+            return r;
+        }
+        const double t0 = now_s();
+        r = solve_common(p, nt, k, ph, sc.data(), st.data(), &s1, false, nullptr);
+        const double dt = now_s() - t0;
+        if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) {
+            apply_tuning(p, keep);
+            return r;
+        }
+        if (dt < best_s) {
+            best_s = dt;
+            win = c;
+            best_sweeps = s1.refinement_sweeps;
+        }
+    }
+    win.graph_sweeps = std::max(1, std::min(8, best_sweeps + 1));
+    win.seconds_per_energy = best_s / nt;
+    apply_tuning(p, win);
+    if (best) *best = win;
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_set_create(const refeff_fms_plan_desc* descs, int32_t count, const char* tune_cache,
+                               refeff_fms_plan_set** out) {
+    if (!descs || count <= 0 || !out) return REFEFF_FMS_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* s = new refeff_fms_plan_set();
+    if (tune_cache) s->tune_cache = tune_cache;
+    s->tuned = read_tune_cache(s->tune_cache);
+    for (int c = 0; c < count; ++c) {
+        refeff_fms_plan_desc d = descs[c];
+        refeff_fms_plan* p = nullptr;
+        const int rc = refeff_fms_plan_create(&d, &p);
+        if (rc) {
+            refeff_fms_plan_set_destroy(s);
+            return rc;
+        }
+        if (p->bicg || (p->solver != REFEFF_FMS_SOLVER_MIXED && p->solver != REFEFF_FMS_SOLVER_FP64)) {
+            refeff_fms_plan_destroy(p);
+            refeff_fms_plan_set_destroy(s);
+            return REFEFF_FMS_UNSUPPORTED;  // LU solvers only
+        }
+        s->plans.push_back(p);
+        const TuneEntry* e = lookup_tuning(s->tuned, p);
+        if (e && tuning_valid(&e->t, p->npad)) {
+            apply_tuning(p, e->t);
+                }
+        s->has_tuning.push_back(e ? 1 : 0);
+    }
+    s->order.resize(count);
+    for (int c = 0; c < count; ++c) s->order[c] = c;
+    std::stable_sort(s->order.begin(), s->order.end(), [&](int a, int b) { return s->plans[a]->npad > s->plans[b]->npad; });
+    bool ok = hipStreamCreateWithFlags(&s->origin, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) == hipSuccess;
+    s->join.assign(count, nullptr);
+    for (int c = 0; ok && c < count; ++c) ok = hipEventCreateWithFlags(&s->join[c], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        refeff_fms_plan_set_destroy(s);
+        return REFEFF_FMS_DEVICE_ERROR;
+    }
+    *out = s;
+    return REFEFF_FMS_OK;
+}
+
+void refeff_fms_plan_set_destroy(refeff_fms_plan_set* s) {
+    if (!s) return;
+    plan_set_drop_graph(s);
+    for (auto* p : s->plans) refeff_fms_plan_destroy(p);
+    for (hipEvent_t e : s->join)
+        if (e) (void)hipEventDestroy(e);
+    if (s->fork) (void)hipEventDestroy(s->fork);
+    if (s->origin) (void)hipStreamDestroy(s->origin);
+    delete s;
+}
+
+int refeff_fms_plan_set_plan(refeff_fms_plan_set* s, int32_t cluster, refeff_fms_plan** plan) {
+    if (!s || !plan || cluster < 0 || cluster >= static_cast<int>(s->plans.size())) return REFEFF_FMS_INVALID_ARGUMENT;
+    *plan = s->plans[cluster];
+    return REFEFF_FMS_OK;
+}
+
+int refeff_fms_plan_set_tune(refeff_fms_plan_set* s, const int32_t* energy_counts, const double* const* k,
+                             const double* const* ph, int32_t* tuned) {
+    if (!s || !energy_counts || !k || !ph) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(s->mu);
+    int made = 0;
+    for (int c : s->order) {
+        refeff_fms_plan* p = s->plans[c];
+        if (s->has_tuning[c]) continue;
+        const TuneEntry* e = lookup_tuning(s->tuned, p);  // tuned meanwhile for a close size
+        if (e) {
+            apply_tuning(p, e->t);
+                    s->has_tuning[c] = 1;
+            continue;
+        }
+        if (energy_counts[c] <= 0 || !k[c] || !ph[c]) continue;
+        refeff_fms_tuning t{};
+        const int r = refeff_fms_plan_tune(p, energy_counts[c], k[c], ph[c], &t);
+        if (r) return r;
+        s->tuned.push_back(TuneEntry{p->npad, p->nrhs, p->solver, t});
+        s->has_tuning[c] = 1;
+        ++made;
+    }
+    if (tuned) *tuned = made;
+    plan_set_drop_graph(s);  // the LU variants changed
+    return write_tune_cache(s->tune_cache, s->tuned) ? REFEFF_FMS_OK : REFEFF_FMS_IO_ERROR;
+}
+
+int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_counts, const double* const* d_k,
+                              const double* const* d_ph, double* const* d_scattering, int32_t* status,
+                              refeff_fms_set_stats* stats) {
+    if (!s || !energy_counts) return REFEFF_FMS_INVALID_ARGUMENT;
+    const int nc = static_cast<int>(s->plans.size());
+    for (int c = 0; c < nc; ++c)
+        if (energy_counts[c] < 0 || (energy_counts[c] > 0 && (!d_k || !d_ph || !d_scattering || !d_k[c] || !d_ph[c] ||
+                                                               !d_scattering[c])))
+            return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(s->mu);
+    const double t0 = now_s();
+    auto make_key = [&]() {
+    std::vector<const void*> key;
+    for (int c = 0; c < nc; ++c) {
+        key.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(energy_counts[c])));
+        key.push_back(energy_counts[c] ? d_k[c] : nullptr);
+        key.push_back(energy_counts[c] ? d_ph[c] : nullptr);
+        key.push_back(energy_counts[c] ? d_scattering[c] : nullptr);
+        // the graph holds the plans' workspace addresses: a reallocation (a larger
+        // eager batch on the same plan) forces a new capture
+        key.push_back(s->plans[c]->d_a64);
+        key.push_back(s->plans[c]->d_a32);
+        key.push_back(s->plans[c]->d_k);
+    }
+    return key;
+    };
+    bool captured = false;
+    const std::vector<const void*> key = make_key();
+    if (!s->exec || key != s->key) {
+        plan_set_drop_graph(s);
+        // batch sizes: every cluster's workspace is resident at once, within 70 %
+        // of free device memory; the largest clusters give up energies first
+        s->gbatch.assign(nc, 0);
+        s->eoff.assign(nc + 1, 0);
+        s->sweeps.assign(nc, 1);
+        for (int c = 0; c < nc; ++c) {
+            refeff_fms_plan* p = s->plans[c];
+            s->gbatch[c] = std::max(0, std::min(energy_counts[c], p->batch));
+            s->eoff[c + 1] = s->eoff[c] + energy_counts[c];
+            s->sweeps[c] = p->solver == REFEFF_FMS_SOLVER_MIXED ? p->graph_sweeps : 1;
+        }
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        for (int c = 0; c < nc; ++c) free_b += static_cast<size_t>(s->plans[c]->ws_batch) * bytes_per_energy(s->plans[c]);
+        const size_t budget = free_b / 10 * 7;
+        auto need = [&]() {
+            size_t b = 0;
+            for (int c = 0; c < nc; ++c) b += static_cast<size_t>(s->gbatch[c]) * bytes_per_energy(s->plans[c]);
+            return b;
+        };
+        while (need() > budget) {
+            int w = -1;
+            for (int c = 0; c < nc; ++c)
+                if (s->gbatch[c] > 1 && (w < 0 || s->gbatch[c] * bytes_per_energy(s->plans[c]) >
+                                                      s->gbatch[w] * bytes_per_energy(s->plans[w])))
+                    w = c;
+            if (w < 0) return REFEFF_FMS_NO_MEMORY;
+            s->gbatch[w] = (s->gbatch[w] + 1) / 2;
+        }
+        for (int c = 0; c < nc; ++c) {
+            refeff_fms_plan* p = s->plans[c];
+            if (s->gbatch[c] == 0) continue;
+            // a workspace of exactly this batch (a larger one would not fit beside the others)
+            if (p->ws_batch != s->gbatch[c]) free_workspace(p);
+            const int r = ensure_workspace(p, s->gbatch[c]);
+            if (r) return r;
+            p->profile = false;
+        }
+        s->norm_off.assign(nc + 1, 0);
+        for (int c = 0; c < nc; ++c)
+            s->norm_off[c + 1] = s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * (s->eoff[c + 1] - s->eoff[c]);
+        if (hipMalloc(&s->d_info, std::max<size_t>(1, s->eoff[nc]) * sizeof(int)) != hipSuccess ||
+            hipMalloc(&s->d_norms, std::max<size_t>(1, s->norm_off[nc]) * sizeof(double)) != hipSuccess) {
+            plan_set_drop_graph(s);
+            return REFEFF_FMS_NO_MEMORY;
+        }
+        // capture: fork from the origin stream into every cluster's stream
+        if (hipStreamBeginCapture(s->origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+        int rc = hipEventRecord(s->fork, s->origin) == hipSuccess ? REFEFF_FMS_OK : REFEFF_FMS_DEVICE_ERROR;
+        for (int c : s->order) {
+            refeff_fms_plan* p = s->plans[c];
+            if (rc || energy_counts[c] == 0) continue;
+            if (hipStreamWaitEvent(p->stream, s->fork, 0) != hipSuccess) {
+                rc = REFEFF_FMS_DEVICE_ERROR;
+                break;
+            }
+            const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+            const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
+            for (int e0 = 0; e0 < energy_counts[c] && !rc; e0 += s->gbatch[c]) {
+                const int nb = std::min(s->gbatch[c], energy_counts[c] - e0);
+                if (hipMemcpyAsync(p->d_k, d_k[c] + 2 * kst * e0, 2 * kst * nb * sizeof(double), hipMemcpyDeviceToDevice,
+                                   p->stream) != hipSuccess ||
+                    hipMemcpyAsync(p->d_ph, d_ph[c] + 2 * pst * e0, 2 * pst * nb * sizeof(double),
+                                   hipMemcpyDeviceToDevice, p->stream) != hipSuccess) {
+                    rc = REFEFF_FMS_DEVICE_ERROR;
+                    break;
+                }
+                const GraphCtx gc{p->stream, s->sweeps[c], s->d_info + s->eoff[c] + e0,
+                                  s->d_norms + s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * e0};
+                rc = solve_device_batch(p, nb, reinterpret_cast<double2*>(d_scattering[c]) + sst * e0, nullptr,
+                                        nullptr, nullptr, 0, nullptr, &gc);
+            }
+            if (!rc && (hipEventRecord(s->join[c], p->stream) != hipSuccess ||
+                        hipStreamWaitEvent(s->origin, s->join[c], 0) != hipSuccess))
+                rc = REFEFF_FMS_DEVICE_ERROR;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s->origin, &g);
+        if (rc || ec != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            plan_set_drop_graph(s);
+            (void)hipGetLastError();
+            return rc ? rc : REFEFF_FMS_DEVICE_ERROR;
+        }
+        s->graph = g;
+        if (hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0) != hipSuccess) {
+            plan_set_drop_graph(s);
+            (void)hipGetLastError();
+            return REFEFF_FMS_DEVICE_ERROR;
+        }
+        size_t nodes = 0;
+        (void)hipGraphGetNodes(s->graph, nullptr, &nodes);
+        s->graph_nodes = nodes;
+        s->key = make_key();  // with the workspace the capture allocated
+        captured = true;
+    }
+    const double t_capture = now_s() - t0;
+    const double t1 = now_s();
+    if (hipGraphLaunch(s->exec, s->origin) != hipSuccess || hipStreamSynchronize(s->origin) != hipSuccess)
+        return REFEFF_FMS_DEVICE_ERROR;
+    const double t_graph = now_s() - t1;
+    // statuses, and eager re-solves of the energies the fixed sweeps left unsettled
+    std::vector<int> info(std::max<size_t>(1, s->eoff[nc]));
+    std::vector<double> nrm(std::max<size_t>(1, s->norm_off[nc]));
+    if (hipMemcpy(info.data(), s->d_info, s->eoff[nc] * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(nrm.data(), s->d_norms, s->norm_off[nc] * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return REFEFF_FMS_DEVICE_ERROR;
+    int rc = REFEFF_FMS_OK, redo = 0;
+    for (int c = 0; c < nc; ++c) {
+        refeff_fms_plan* p = s->plans[c];
+        const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+        const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
+        for (int e = 0; e < energy_counts[c]; ++e) {
+            const int b0 = (e / s->gbatch[c]) * s->gbatch[c], nb = std::min(s->gbatch[c], energy_counts[c] - b0);
+            const double* nb_norms = nrm.data() + s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * b0;
+            double est = 0.0;
+            int stt = graph_energy_status(p, info[s->eoff[c] + e], nb_norms, nb, e - b0, s->sweeps[c], &est);
+            if (stt == REFEFF_FMS_NO_CONVERGENCE) {
+                ++redo;
+                int32_t one = 0;
+                const int r = refeff_fms_solve_batch_device(p, 1, d_k[c] + 2 * kst * e, d_ph[c] + 2 * pst * e,
+                                                            d_scattering[c] + 2 * sst * e, &one, nullptr, nullptr);
+                if (r == REFEFF_FMS_DEVICE_ERROR) return r;
+                stt = one;
+            }
+            if (status) status[s->eoff[c] + e] = stt;
+            if (stt && !rc) rc = stt;
+        }
+    }
+    if (stats) {
+        stats->seconds_total = now_s() - t0;
+        stats->seconds_capture = captured ? t_capture : 0.0;
+        stats->seconds_graph = t_graph;
+        stats->graph_nodes = static_cast<int64_t>(s->graph_nodes);
+        stats->captured = captured ? 1 : 0;
+        stats->resolved_energies = redo;
+        stats->energies = static_cast<int32_t>(s->eoff[nc]);
+    }
+    return rc;
+}
+
+}  // extern "C"

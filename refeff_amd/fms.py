@@ -129,7 +129,7 @@ class FmsRealSpacePlan:
                  max_potential=None, global_lmax=3, raw_potential_lmax=None, direct_cutoff=1.0e9,
                  mean_square_displacements=None, convergence_tolerance=1e-3, zero_tolerance=1e-3,
                  full_scattering_matrix_requested=False, solver=SOLVER_AUTO, max_batch=0, state_capacity=None,
-                 calculated_l=None):
+                 calculated_l=None, _defer=False):
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.potentials = np.ascontiguousarray(potentials, dtype=np.int32)
         nat = len(self.potentials)
@@ -157,13 +157,20 @@ class FmsRealSpacePlan:
             if len(self._lcalc) < global_lmax + 1:
                 raise ValueError("calculated_l needs global_lmax + 1 entries")
             desc.calculated_l = self._lcalc.ctypes.data
+        self.desc = desc
         self._h = C.c_void_p()
+        if _defer:  # a plan set creates the plan from self.desc
+            return
         rc = lib().refeff_fms_plan_create(C.byref(desc), C.byref(self._h))
         check(rc, "fms_real_space_plan", error=last_error() if rc else None)
+        self._query()
+
+    def _query(self):
         info = PlanInfo()
         check(lib().refeff_fms_plan_query(self._h, C.byref(info)), "plan_query")
         self.info = info
         self.last_stats = Stats()
+        self._learned = None
 
     @property
     def channel_count(self) -> int:
@@ -178,9 +185,9 @@ class FmsRealSpacePlan:
         return self._h
 
     def close(self):
-        if self._h:
+        if self._h and not getattr(self, "_owned_by_set", False):
             lib().refeff_fms_plan_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -373,3 +380,115 @@ class LearnedPreconditioner:
         check(lib().refeff_fms_gnn_apply(self._h, int(sample), _f64p(vv.view(np.float64).reshape(-1)),
                                          _f64p(y.view(np.float64).reshape(-1))), "gnn_apply")
         return y
+
+
+class Tuning(C.Structure):
+    _fields_ = [("lu_block", C.c_int32), ("calu_min_rows", C.c_int32), ("gemm_3m", C.c_int32),
+                ("graph_sweeps", C.c_int32), ("seconds_per_energy", C.c_double)]
+
+
+class SetStats(C.Structure):
+    _fields_ = [("seconds_total", C.c_double), ("seconds_capture", C.c_double), ("seconds_graph", C.c_double),
+                ("graph_nodes", C.c_int64), ("captured", C.c_int32), ("resolved_energies", C.c_int32),
+                ("energies", C.c_int32)]
+
+
+def plan_tune(plan: FmsRealSpacePlan, wave_numbers, phase_shifts) -> Tuning:
+    """Autotune one plan's LU variant on (up to 8 of) the given energies."""
+    nsp, L = plan.spin_channels, plan.global_lmax
+    k = np.ascontiguousarray(np.asarray(wave_numbers, np.complex128).reshape(-1, nsp))
+    ph = np.ascontiguousarray(np.asarray(phase_shifts, np.complex128).reshape(k.shape[0], nsp, 2 * L + 1,
+                                                                               plan.potential_count))
+    t = Tuning()
+    check(lib().refeff_fms_plan_tune(plan.handle, k.shape[0], _f64p(k.view(np.float64)), _f64p(ph.view(np.float64)),
+                                     C.byref(t)), "plan_tune")
+    return t
+
+
+class FmsPlanSet:
+    """Many clusters of uneven size and lmax solved in one graph launch
+    (include/refeff_fms.h, refeff_fms_plan_set_*).  plans: FmsRealSpacePlan
+    descriptions made with _defer=True (see plan_desc)."""
+
+    def __init__(self, plan_descs, tune_cache=None):
+        self._keep = list(plan_descs)
+        arr = (PlanDesc * len(self._keep))(*[p.desc for p in self._keep])
+        self._h = C.c_void_p()
+        check(lib().refeff_fms_plan_set_create(arr, len(self._keep), tune_cache.encode() if tune_cache else None,
+                                               C.byref(self._h)), "plan_set_create")
+        self.plans = []
+        for i, p in enumerate(self._keep):
+            h = C.c_void_p()
+            check(lib().refeff_fms_plan_set_plan(self._h, i, C.byref(h)), "plan_set_plan")
+            p._h = h
+            p._owned_by_set = True
+            p._query()
+            self.plans.append(p)
+        self.last_stats = SetStats()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            for p in self.plans:
+                p._h = C.c_void_p()
+            lib().refeff_fms_plan_set_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arrays(self, ks, phs):
+        ks = [np.ascontiguousarray(np.asarray(k, np.complex128).reshape(-1, p.spin_channels)) for k, p in zip(ks, self.plans)]
+        phs = [np.ascontiguousarray(np.asarray(ph, np.complex128).reshape(len(k), p.spin_channels, 2 * p.global_lmax + 1,
+                                                                            p.potential_count))
+               for ph, k, p in zip(phs, ks, self.plans)]
+        return ks, phs
+
+    def tune(self, ks, phs) -> int:
+        ks, phs = self._arrays(ks, phs)
+        n = len(self.plans)
+        counts = (C.c_int32 * n)(*[len(k) for k in ks])
+        kp = (C.POINTER(C.c_double) * n)(*[_f64p(k.view(np.float64)) for k in ks])
+        pp = (C.POINTER(C.c_double) * n)(*[_f64p(ph.view(np.float64)) for ph in phs])
+        made = C.c_int32()
+        check(lib().refeff_fms_plan_set_tune(self._h, counts, kp, pp, C.byref(made)), "plan_set_tune")
+        return made.value
+
+    def solve_device(self, dks, dphs, douts, counts):
+        """Device pointers per cluster (DeviceArray); returns the status array (clusters concatenated)."""
+        n = len(self.plans)
+        cc = (C.c_int32 * n)(*counts)
+        kp = (C.c_void_p * n)(*[d.ptr.value if d is not None else None for d in dks])
+        pp = (C.c_void_p * n)(*[d.ptr.value if d is not None else None for d in dphs])
+        op = (C.c_void_p * n)(*[d.ptr.value if d is not None else None for d in douts])
+        status = np.zeros(max(1, sum(counts)), np.int32)
+        st = SetStats()
+        rc = lib().refeff_fms_plan_set_solve(self._h, cc, kp, pp, op, _i32p(status), C.byref(st))
+        self.last_stats = st
+        if rc not in (0,) and not status.any():
+            check(rc, "plan_set_solve")
+        return status[:sum(counts)]
+
+    def solve(self, ks, phs):
+        """Host arrays in and out: list of scattering [ne, npot, ch, ch] per cluster, statuses concatenated."""
+        from .hip import DeviceArray
+        ks, phs = self._arrays(ks, phs)
+        dks = [DeviceArray(k) for k in ks]
+        dphs = [DeviceArray(ph) for ph in phs]
+        outs = [np.zeros((len(k), p.potential_count, p.channel_count, p.channel_count), np.complex128)
+                for k, p in zip(ks, self.plans)]
+        douts = [DeviceArray.empty(o.nbytes) for o in outs]
+        status = self.solve_device(dks, dphs, douts, [len(k) for k in ks])
+        res = []
+        for o, d in zip(outs, douts):
+            d.download(o)
+            res.append(o.transpose(0, 1, 3, 2).copy())
+        for d in dks + dphs + douts:
+            d.free()
+        return res, status

@@ -156,9 +156,15 @@ def rocksalt_mno(seed: int, ne: int = 80) -> Workload:
     return Workload(f"rocksalt MnO seed={seed} R={radius:.2f}", pos, pot, lm, L, e, rdirec=2.0 * radius)
 
 
+def rocksalt_set(count: int = 64, ne: int = 80) -> list:
+    """The batched uneven workload: MnO-like clusters of seeds 0 .. count-1."""
+    return [rocksalt_mno(seed, ne) for seed in range(count)]
+
+
 CONFIGS = {
     "cfg1": lambda: fcc_cu(5.2, 60, name="Cu fcc R=5.2 (55 atoms, N=880), 60 energies"),
     "cfg2": lambda: fcc_cu(8.0, 120, name="Cu fcc R=8.0 (177 atoms, N=2832), 120 energies"),
+    "cfg3": lambda: rocksalt_set(64, 80),
     "cfg4": lambda: bcc_fe(12.0, 200),
     "cfg5": lambda: alloy(10.0, 150),
 }
