@@ -192,6 +192,8 @@ typedef struct refeff_fms_stats {
     int32_t krylov_energies;     /* BICGSTAB: energies BiCGSTAB solved */
     int32_t krylov_iterations;   /* BICGSTAB: their iterations, summed */
     int32_t lu_energies;         /* BICGSTAB: energies the mixed LU solved */
+    int32_t fp64_fallback_energies; /* MIXED: energies the FP32 factors + FP64 refinement did not settle within
+                                       8 sweeps, solved again by the plain FP64 LU (their status is that solve's) */
 } refeff_fms_stats;
 
 /* Typed failure detail.  The status codes above are the FmsError families; this
@@ -316,7 +318,7 @@ int refeff_fms_plan_get_tuning(const refeff_fms_plan* plan, refeff_fms_tuning* t
 int refeff_fms_plan_set_tuning(refeff_fms_plan* plan, const refeff_fms_tuning* tuning);
 /* Time the LU variants on up to 8 of the given energies (host arrays, layout of
  * refeff_fms_solve_batch), keep the fastest on the plan and return it; the
- * graph sweeps are one more than the eager refinement needed (at most 8). */
+ * graph sweeps are three more than the eager refinement needed (at most 8). */
 int refeff_fms_plan_tune(refeff_fms_plan* plan, int32_t energy_count, const double* wave_numbers,
                          const double* phase_shifts, refeff_fms_tuning* best);
 

@@ -84,7 +84,7 @@ class Stats(C.Structure):
     _fields_ = [("seconds_total", C.c_double), ("seconds_assemble", C.c_double), ("seconds_factor", C.c_double),
                 ("seconds_solve", C.c_double), ("refinement_sweeps", C.c_int32), ("batches", C.c_int32),
                 ("max_correction", C.c_double), ("krylov_energies", C.c_int32), ("krylov_iterations", C.c_int32),
-                ("lu_energies", C.c_int32)]
+                ("lu_energies", C.c_int32), ("fp64_fallback_energies", C.c_int32)]
 
 
 class Profile(C.Structure):

@@ -3246,7 +3246,14 @@ struct refeff_fms_plan {
     std::mutex mu;
     unsigned lcalc_mask = ~0u;                   // FEFF lcalc: bit l set = source columns of l solved (iterative)
     int device = 0;                              // HIP device the plan was created on
-    int graph_sweeps = 4;                        // fixed refinement sweeps inside a plan set's graph (tuned)
+    int graph_sweeps = 8;                        // fixed refinement sweeps inside a plan set's graph
+    // the creation inputs, kept for the FP64 twin plan that takes the energies the
+    // mixed refinement cannot settle
+    refeff_fms_plan_desc desc{};
+    std::vector<int32_t> desc_pot, desc_raw_lmax;
+    std::vector<float> desc_msd;
+    std::vector<uint8_t> desc_lcalc;
+    refeff_fms_plan* twin64 = nullptr;
     std::vector<int> h_sl, h_ss, h_spot;         // host copies of the state table (l, spin, potential)
     std::vector<refeff_fms_error> energy_err;    // per-energy detail of the last solve call
     // kernel-level instrumentation (refeff_fms_plan_profile)
@@ -4559,6 +4566,13 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
             if (d->calculated_l[l]) p->lcalc_mask |= 1u << l;
     }
     p->h_pos.assign(d->positions, d->positions + 3 * static_cast<size_t>(d->atom_count));
+    p->desc = *d;
+    p->desc_pot.assign(d->potentials, d->potentials + d->atom_count);
+    p->desc_raw_lmax.assign(d->raw_potential_lmax, d->raw_potential_lmax + d->max_potential + 1);
+    if (d->mean_square_displacements)
+        p->desc_msd.assign(d->mean_square_displacements,
+                           d->mean_square_displacements + static_cast<size_t>(d->atom_count) * d->atom_count);
+    if (d->calculated_l) p->desc_lcalc.assign(d->calculated_l, d->calculated_l + d->global_lmax + 1);
     p->h_sl = stt.l;
     p->h_ss = stt.spin;
     p->h_spot.resize(stt.atom.size());
@@ -4700,6 +4714,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
 
 void refeff_fms_plan_destroy(refeff_fms_plan* p) {
     if (!p) return;
+    refeff_fms_plan_destroy(p->twin64);
     free_workspace(p);
     dfree(p->d_pos); dfree(p->d_sig); dfree(p->d_atom_first); dfree(p->d_atom_lmax); dfree(p->d_atom_rhs);
     dfree(p->d_bj_state); dfree(p->d_bj_size); dfree(p->d_bj_inv);
@@ -4869,7 +4884,75 @@ struct ScratchArena {
 
 static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc, int32_t* status,
                         refeff_fms_stats* st, bool device, hipStream_t ext, double* full_out = nullptr,
-                        const ScratchArena* arena = nullptr) {
+                        const ScratchArena* arena = nullptr);
+
+// The plain FP64 LU solve of the energies `idx` (host or device inputs and
+// outputs, as the caller's), by the plan's FP64 twin (created on first use).
+static int fp64_fallback(refeff_fms_plan* p, const std::vector<int>& idx, const double* k, const double* ph,
+                         double* sc, int32_t* status, bool device) {
+    if (!p->twin64) {
+        refeff_fms_plan_desc d = p->desc;
+        d.positions = p->h_pos.data();
+        d.potentials = p->desc_pot.data();
+        d.raw_potential_lmax = p->desc_raw_lmax.data();
+        d.mean_square_displacements = p->desc_msd.empty() ? nullptr : p->desc_msd.data();
+        d.calculated_l = p->desc_lcalc.empty() ? nullptr : p->desc_lcalc.data();
+        d.solver = REFEFF_FMS_SOLVER_FP64;
+        const refeff_fms_error keep = t_last_error;
+        const int r = refeff_fms_plan_create(&d, &p->twin64);
+        t_last_error = keep;
+        if (r) return r;
+    }
+    refeff_fms_plan* q = p->twin64;
+    const int m = static_cast<int>(idx.size());
+    const size_t kst = 2 * static_cast<size_t>(p->nsp), pst = 2 * static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+    const size_t sst = 2 * static_cast<size_t>(p->ch) * p->ch * p->npot;
+    std::vector<int32_t> st2(m);
+    int r = REFEFF_FMS_OK;
+    if (!device) {
+        std::vector<double> kk(kst * m), pp(pst * m), out(sst * m);
+        for (int i = 0; i < m; ++i) {
+            std::memcpy(kk.data() + kst * i, k + kst * idx[i], kst * sizeof(double));
+            std::memcpy(pp.data() + pst * i, ph + pst * idx[i], pst * sizeof(double));
+        }
+        r = solve_common(q, m, kk.data(), pp.data(), out.data(), st2.data(), nullptr, false, nullptr);
+        if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) return r;
+        for (int i = 0; i < m; ++i) std::memcpy(sc + sst * idx[i], out.data() + sst * i, sst * sizeof(double));
+    } else {
+        double *dk = nullptr, *dp = nullptr, *dout = nullptr;
+        if (hipMalloc(&dk, kst * m * sizeof(double)) != hipSuccess || hipMalloc(&dp, pst * m * sizeof(double)) != hipSuccess ||
+            hipMalloc(&dout, sst * m * sizeof(double)) != hipSuccess) {
+            if (dk) (void)hipFree(dk);
+            if (dp) (void)hipFree(dp);
+            return REFEFF_FMS_NO_MEMORY;
+        }
+        bool ok = true;
+        for (int i = 0; i < m && ok; ++i)
+            ok = hipMemcpy(dk + kst * i, k + kst * idx[i], kst * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess &&
+                 hipMemcpy(dp + pst * i, ph + pst * idx[i], pst * sizeof(double), hipMemcpyDeviceToDevice) == hipSuccess;
+        if (ok) {
+            r = solve_common(q, m, dk, dp, dout, st2.data(), nullptr, true, nullptr);
+            for (int i = 0; i < m && ok; ++i)
+                ok = hipMemcpy(sc + sst * idx[i], dout + sst * i, sst * sizeof(double), hipMemcpyDeviceToDevice) ==
+                     hipSuccess;
+        }
+        (void)hipFree(dk);
+        (void)hipFree(dp);
+        (void)hipFree(dout);
+        if (!ok) return REFEFF_FMS_DEVICE_ERROR;
+        if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) return r;
+    }
+    for (int i = 0; i < m; ++i) {
+        status[idx[i]] = st2[i];
+        if (static_cast<size_t>(idx[i]) < p->energy_err.size() && static_cast<size_t>(i) < q->energy_err.size())
+            p->energy_err[idx[i]] = q->energy_err[i];
+    }
+    return REFEFF_FMS_OK;
+}
+
+static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const double* ph, double* sc, int32_t* status,
+                        refeff_fms_stats* st, bool device, hipStream_t ext, double* full_out,
+                        const ScratchArena* arena) {
     if (!p || ne < 0 || (ne > 0 && (!k || !ph || !sc))) return REFEFF_FMS_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(p->mu);
     p->energy_err.assign(static_cast<size_t>(ne), refeff_fms_error{});
@@ -4914,6 +4997,11 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
     const size_t pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
     const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
     int rc = REFEFF_FMS_OK;
+    std::vector<int32_t> st_local;
+    if (!status) {
+        st_local.assign(std::max(1, ne), 0);
+        status = st_local.data();
+    }
     if (device && ext) (void)hipStreamSynchronize(ext);
     for (int e0 = 0; e0 < ne; e0 += batch) {
         const int nb = std::min(batch, ne - e0);
@@ -4926,9 +5014,8 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
         // the sub-batch streams read the inputs
         if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
         double2* out = device ? reinterpret_cast<double2*>(sc) + sst * e0 : p->d_out;
-        r = solve_device_batch(p, nb, out, status ? status + e0 : nullptr, st, nullptr, e0);
+        r = solve_device_batch(p, nb, out, status + e0, st, nullptr, e0);
         if (r == REFEFF_FMS_DEVICE_ERROR) return r;
-        if (r && !rc) rc = r;
         if (!device) {
             if (check_hip(hipMemcpy(sc + 2 * sst * e0, p->d_out, 2 * sst * nb * sizeof(double), hipMemcpyDeviceToHost)))
                 return REFEFF_FMS_DEVICE_ERROR;
@@ -4950,6 +5037,21 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
         }
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    // Energies the FP32 factors and FP64 refinement did not settle go to the plain
+    // FP64 LU of a twin plan (not in the no-allocation call, which must not
+    // allocate, nor for training captures or full scattering requests).
+    if (p->solver == REFEFF_FMS_SOLVER_MIXED && !arena && !full_out && !p->capture) {
+        std::vector<int> bad;
+        for (int e = 0; e < ne; ++e)
+            if (status[e] == REFEFF_FMS_NO_CONVERGENCE) bad.push_back(e);
+        if (!bad.empty()) {
+            const int r = fp64_fallback(p, bad, k, ph, sc, status, device);
+            if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) return r;
+            if (st) st->fp64_fallback_energies += static_cast<int32_t>(bad.size());
+        }
+    }
+    for (int e = 0; e < ne; ++e)
+        if (status[e] && !rc) rc = status[e];
     if (st) st->seconds_total = now_s() - t0;
     return rc;
 }

@@ -1,9 +1,12 @@
 // Plan sets and the autotuner (included at the end of fms_device.hip).
 //
 // A plan set solves many clusters of uneven size and lmax (each with its own
-// energy list) in one HIP graph launch: every cluster is a branch of the graph,
-// forked from one origin stream and joined back, its energies in batches on its
-// own stream; branches are enqueued largest first.  Inside the graph the mixed
+// energy list) in one HIP graph launch.  The graph has a few lanes, forked from
+// one origin stream and joined back; clusters are dealt to the lanes largest
+// first, each to the lane with the least work so far, and a lane solves its
+// clusters one after another in a device arena of its own (each cluster's
+// batch buffers carved from it, so every lane holds one cluster's workspace at
+// a time and batches stay large).  Inside the graph the mixed
 // solver refines a fixed number of sweeps over every energy and leaves its
 // correction norms in device memory; after the launch the host applies the
 // same convergence rule as the eager path and re-solves any energy the fixed
@@ -97,6 +100,10 @@ struct refeff_fms_plan_set {
     hipGraphExec_t exec = nullptr;
     std::vector<const void*> key;  // pointers and counts the graph was captured for
     std::vector<int> gbatch;       // energies per batch of each cluster in the graph
+    std::vector<std::vector<int>> lanes;  // clusters of each lane, in solve order
+    std::vector<unsigned char*> arena;    // one device arena per lane
+    std::vector<size_t> arena_bytes;
+    std::vector<hipStream_t> lane_stream;
     std::vector<size_t> eoff;      // first energy of each cluster in the concatenated lists
     std::vector<int> sweeps;       // fixed refinement sweeps of each cluster
     int* d_info = nullptr;
@@ -110,6 +117,10 @@ namespace {
 
 void plan_set_drop_graph(refeff_fms_plan_set* s) {
     if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    for (auto* a : s->arena)
+        if (a) (void)hipFree(a);
+    s->arena.clear();
+    s->arena_bytes.clear();
     if (s->graph) (void)hipGraphDestroy(s->graph);
     s->exec = nullptr;
     s->graph = nullptr;
@@ -201,7 +212,7 @@ int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const 
             best_sweeps = s1.refinement_sweeps;
         }
     }
-    win.graph_sweeps = std::max(1, std::min(8, best_sweeps + 1));
+    win.graph_sweeps = std::max(1, std::min(8, best_sweeps + 3));  // margin for the harder energies of the sweep
     win.seconds_per_energy = best_s / nt;
     apply_tuning(p, win);
     if (best) *best = win;
@@ -240,8 +251,9 @@ int refeff_fms_plan_set_create(const refeff_fms_plan_desc* descs, int32_t count,
     std::stable_sort(s->order.begin(), s->order.end(), [&](int a, int b) { return s->plans[a]->npad > s->plans[b]->npad; });
     bool ok = hipStreamCreateWithFlags(&s->origin, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) == hipSuccess;
-    s->join.assign(count, nullptr);
-    for (int c = 0; ok && c < count; ++c) ok = hipEventCreateWithFlags(&s->join[c], hipEventDisableTiming) == hipSuccess;
+    s->join.assign(std::max(count, 4), nullptr);
+    for (size_t c = 0; ok && c < s->join.size(); ++c)
+        ok = hipEventCreateWithFlags(&s->join[c], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         refeff_fms_plan_set_destroy(s);
         return REFEFF_FMS_DEVICE_ERROR;
@@ -256,6 +268,8 @@ void refeff_fms_plan_set_destroy(refeff_fms_plan_set* s) {
     for (auto* p : s->plans) refeff_fms_plan_destroy(p);
     for (hipEvent_t e : s->join)
         if (e) (void)hipEventDestroy(e);
+    for (hipStream_t q : s->lane_stream)
+        if (q) (void)hipStreamDestroy(q);
     if (s->fork) (void)hipEventDestroy(s->fork);
     if (s->origin) (void)hipStreamDestroy(s->origin);
     delete s;
@@ -312,55 +326,66 @@ int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_coun
         key.push_back(energy_counts[c] ? d_k[c] : nullptr);
         key.push_back(energy_counts[c] ? d_ph[c] : nullptr);
         key.push_back(energy_counts[c] ? d_scattering[c] : nullptr);
-        // the graph holds the plans' workspace addresses: a reallocation (a larger
-        // eager batch on the same plan) forces a new capture
-        key.push_back(s->plans[c]->d_a64);
-        key.push_back(s->plans[c]->d_a32);
-        key.push_back(s->plans[c]->d_k);
     }
     return key;
     };
     bool captured = false;
     const std::vector<const void*> key = make_key();
+    (void)make_key;
     if (!s->exec || key != s->key) {
         plan_set_drop_graph(s);
-        // batch sizes: every cluster's workspace is resident at once, within 70 %
-        // of free device memory; the largest clusters give up energies first
+        // lanes: as many as hold the largest cluster's full batch within 70 % of
+        // free device memory (at most kMaxLanes); clusters dealt largest first to
+        // the lane with the least work (npad^3 x energies)
+        constexpr int kMaxLanes = 4;
         s->gbatch.assign(nc, 0);
         s->eoff.assign(nc + 1, 0);
         s->sweeps.assign(nc, 1);
+        size_t biggest = 0;
         for (int c = 0; c < nc; ++c) {
             refeff_fms_plan* p = s->plans[c];
             s->gbatch[c] = std::max(0, std::min(energy_counts[c], p->batch));
             s->eoff[c + 1] = s->eoff[c] + energy_counts[c];
             s->sweeps[c] = p->solver == REFEFF_FMS_SOLVER_MIXED ? p->graph_sweeps : 1;
+            if (s->gbatch[c]) biggest = std::max(biggest, carved_bytes(p, s->gbatch[c]));
         }
         size_t free_b = 0, total_b = 0;
         (void)hipMemGetInfo(&free_b, &total_b);
-        for (int c = 0; c < nc; ++c) free_b += static_cast<size_t>(s->plans[c]->ws_batch) * bytes_per_energy(s->plans[c]);
         const size_t budget = free_b / 10 * 7;
-        auto need = [&]() {
-            size_t b = 0;
-            for (int c = 0; c < nc; ++c) b += static_cast<size_t>(s->gbatch[c]) * bytes_per_energy(s->plans[c]);
-            return b;
-        };
-        while (need() > budget) {
-            int w = -1;
-            for (int c = 0; c < nc; ++c)
-                if (s->gbatch[c] > 1 && (w < 0 || s->gbatch[c] * bytes_per_energy(s->plans[c]) >
-                                                      s->gbatch[w] * bytes_per_energy(s->plans[w])))
-                    w = c;
-            if (w < 0) return REFEFF_FMS_NO_MEMORY;
-            s->gbatch[w] = (s->gbatch[w] + 1) / 2;
-        }
-        for (int c = 0; c < nc; ++c) {
+        int max_lanes = kMaxLanes;
+        if (const char* e = std::getenv("REFEFF_FMS_SET_LANES")) max_lanes = std::max(1, std::min(16, std::atoi(e)));
+        int nl = std::max(1, std::min<int>(max_lanes, biggest ? static_cast<int>(budget / biggest) : 1));
+        size_t arena = biggest ? std::min(budget / nl, std::max(biggest, budget / nl)) : 256;
+        if (biggest > arena) arena = budget;  // one lane, smaller batches below
+        for (int c = 0; c < nc; ++c) {  // batches that fit the arena
             refeff_fms_plan* p = s->plans[c];
-            if (s->gbatch[c] == 0) continue;
-            // a workspace of exactly this batch (a larger one would not fit beside the others)
-            if (p->ws_batch != s->gbatch[c]) free_workspace(p);
-            const int r = ensure_workspace(p, s->gbatch[c]);
-            if (r) return r;
-            p->profile = false;
+            while (s->gbatch[c] > 1 && carved_bytes(p, s->gbatch[c]) > arena) s->gbatch[c] = (s->gbatch[c] + 1) / 2;
+            if (s->gbatch[c] && carved_bytes(p, s->gbatch[c]) > arena) return REFEFF_FMS_NO_MEMORY;
+        }
+        s->lanes.assign(nl, {});
+        std::vector<double> load(nl, 0.0);
+        for (int c : s->order) {
+            if (energy_counts[c] == 0) continue;
+            const int l = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            s->lanes[l].push_back(c);
+            load[l] += std::pow(static_cast<double>(s->plans[c]->npad), 3) * energy_counts[c];
+        }
+        for (int l = 0; l < nl; ++l) {
+            size_t need = 256;
+            for (int c : s->lanes[l]) need = std::max(need, carved_bytes(s->plans[c], s->gbatch[c]));
+            unsigned char* ptr = nullptr;
+            if (hipMalloc(&ptr, need) != hipSuccess) {
+                plan_set_drop_graph(s);
+                (void)hipGetLastError();
+                return REFEFF_FMS_NO_MEMORY;
+            }
+            s->arena.push_back(ptr);
+            s->arena_bytes.push_back(need);
+        }
+        while (static_cast<int>(s->lane_stream.size()) < nl) {
+            hipStream_t q = nullptr;
+            if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+            s->lane_stream.push_back(q);
         }
         s->norm_off.assign(nc + 1, 0);
         for (int c = 0; c < nc; ++c)
@@ -370,34 +395,45 @@ int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_coun
             plan_set_drop_graph(s);
             return REFEFF_FMS_NO_MEMORY;
         }
-        // capture: fork from the origin stream into every cluster's stream
+        // capture: fork from the origin stream into every lane
         if (hipStreamBeginCapture(s->origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
         int rc = hipEventRecord(s->fork, s->origin) == hipSuccess ? REFEFF_FMS_OK : REFEFF_FMS_DEVICE_ERROR;
-        for (int c : s->order) {
-            refeff_fms_plan* p = s->plans[c];
-            if (rc || energy_counts[c] == 0) continue;
-            if (hipStreamWaitEvent(p->stream, s->fork, 0) != hipSuccess) {
+        for (int l = 0; l < nl && !rc; ++l) {
+            hipStream_t q = s->lane_stream[l];
+            if (hipStreamWaitEvent(q, s->fork, 0) != hipSuccess) {
                 rc = REFEFF_FMS_DEVICE_ERROR;
                 break;
             }
-            const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
-            const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
-            for (int e0 = 0; e0 < energy_counts[c] && !rc; e0 += s->gbatch[c]) {
-                const int nb = std::min(s->gbatch[c], energy_counts[c] - e0);
-                if (hipMemcpyAsync(p->d_k, d_k[c] + 2 * kst * e0, 2 * kst * nb * sizeof(double), hipMemcpyDeviceToDevice,
-                                   p->stream) != hipSuccess ||
-                    hipMemcpyAsync(p->d_ph, d_ph[c] + 2 * pst * e0, 2 * pst * nb * sizeof(double),
-                                   hipMemcpyDeviceToDevice, p->stream) != hipSuccess) {
-                    rc = REFEFF_FMS_DEVICE_ERROR;
-                    break;
+            for (int c : s->lanes[l]) {
+                refeff_fms_plan* p = s->plans[c];
+                WorkspaceSnapshot keep(p);  // the plan's own workspace comes back at the end of the scope
+                size_t off = 0;
+                workspace_layout(p, s->gbatch[c], [&](void** slot, size_t bytes) {
+                    *slot = s->arena[l] + off;
+                    off += (bytes + 255) / 256 * 256;
+                    return true;
+                });
+                p->ws_batch = s->gbatch[c];
+                p->profile = false;
+                const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+                const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
+                for (int e0 = 0; e0 < energy_counts[c] && !rc; e0 += s->gbatch[c]) {
+                    const int nb = std::min(s->gbatch[c], energy_counts[c] - e0);
+                    if (hipMemcpyAsync(p->d_k, d_k[c] + 2 * kst * e0, 2 * kst * nb * sizeof(double),
+                                       hipMemcpyDeviceToDevice, q) != hipSuccess ||
+                        hipMemcpyAsync(p->d_ph, d_ph[c] + 2 * pst * e0, 2 * pst * nb * sizeof(double),
+                                       hipMemcpyDeviceToDevice, q) != hipSuccess) {
+                        rc = REFEFF_FMS_DEVICE_ERROR;
+                        break;
+                    }
+                    const GraphCtx gc{q, s->sweeps[c], s->d_info + s->eoff[c] + e0,
+                                      s->d_norms + s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * e0};
+                    rc = solve_device_batch(p, nb, reinterpret_cast<double2*>(d_scattering[c]) + sst * e0, nullptr,
+                                            nullptr, nullptr, 0, nullptr, &gc);
                 }
-                const GraphCtx gc{p->stream, s->sweeps[c], s->d_info + s->eoff[c] + e0,
-                                  s->d_norms + s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * e0};
-                rc = solve_device_batch(p, nb, reinterpret_cast<double2*>(d_scattering[c]) + sst * e0, nullptr,
-                                        nullptr, nullptr, 0, nullptr, &gc);
+                if (rc) break;
             }
-            if (!rc && (hipEventRecord(s->join[c], p->stream) != hipSuccess ||
-                        hipStreamWaitEvent(s->origin, s->join[c], 0) != hipSuccess))
+            if (!rc && (hipEventRecord(s->join[l], q) != hipSuccess || hipStreamWaitEvent(s->origin, s->join[l], 0) != hipSuccess))
                 rc = REFEFF_FMS_DEVICE_ERROR;
         }
         hipGraph_t g = nullptr;
@@ -417,7 +453,7 @@ int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_coun
         size_t nodes = 0;
         (void)hipGraphGetNodes(s->graph, nullptr, &nodes);
         s->graph_nodes = nodes;
-        s->key = make_key();  // with the workspace the capture allocated
+        s->key = key;
         captured = true;
     }
     const double t_capture = now_s() - t0;
@@ -431,27 +467,32 @@ int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_coun
     if (hipMemcpy(info.data(), s->d_info, s->eoff[nc] * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(nrm.data(), s->d_norms, s->norm_off[nc] * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         return REFEFF_FMS_DEVICE_ERROR;
+    // statuses; the energies the fixed sweeps left unsettled go, per cluster in
+    // one call, to the plain FP64 LU (the mixed path's own fallback)
     int rc = REFEFF_FMS_OK, redo = 0;
+    std::vector<int32_t> st_all(std::max<size_t>(1, s->eoff[nc]), 0);
     for (int c = 0; c < nc; ++c) {
         refeff_fms_plan* p = s->plans[c];
-        const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
-        const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
+        std::vector<int> bad;
         for (int e = 0; e < energy_counts[c]; ++e) {
             const int b0 = (e / s->gbatch[c]) * s->gbatch[c], nb = std::min(s->gbatch[c], energy_counts[c] - b0);
             const double* nb_norms = nrm.data() + s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * b0;
             double est = 0.0;
-            int stt = graph_energy_status(p, info[s->eoff[c] + e], nb_norms, nb, e - b0, s->sweeps[c], &est);
-            if (stt == REFEFF_FMS_NO_CONVERGENCE) {
-                ++redo;
-                int32_t one = 0;
-                const int r = refeff_fms_solve_batch_device(p, 1, d_k[c] + 2 * kst * e, d_ph[c] + 2 * pst * e,
-                                                            d_scattering[c] + 2 * sst * e, &one, nullptr, nullptr);
-                if (r == REFEFF_FMS_DEVICE_ERROR) return r;
-                stt = one;
-            }
-            if (status) status[s->eoff[c] + e] = stt;
-            if (stt && !rc) rc = stt;
+            const int stt = graph_energy_status(p, info[s->eoff[c] + e], nb_norms, nb, e - b0, s->sweeps[c], &est);
+            st_all[s->eoff[c] + e] = stt;
+            if (stt == REFEFF_FMS_NO_CONVERGENCE) bad.push_back(e);
         }
+        if (!bad.empty()) {
+            redo += static_cast<int>(bad.size());
+            std::lock_guard<std::mutex> plock(p->mu);
+            p->energy_err.assign(static_cast<size_t>(energy_counts[c]), refeff_fms_error{});
+            const int r = fp64_fallback(p, bad, d_k[c], d_ph[c], d_scattering[c], st_all.data() + s->eoff[c], true);
+            if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) return r;
+        }
+    }
+    for (size_t i = 0; i < s->eoff[nc]; ++i) {
+        if (status) status[i] = st_all[i];
+        if (st_all[i] && !rc) rc = st_all[i];
     }
     if (stats) {
         stats->seconds_total = now_s() - t0;

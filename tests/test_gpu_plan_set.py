@@ -126,3 +126,26 @@ def test_autotuner_caches_and_reuses(mno):
         for a, b in zip(got, got2):
             assert np.array_equal(a, b)
         ps2.close()
+
+
+def test_unsettled_energies_fall_back_to_the_fp64_lu():
+    """MnO cluster seed 33 (N=1416, lmax 4 absorber): at the lowest energies the
+    FP32 factors' refinement does not settle within 8 sweeps; those energies are
+    solved again by the plain FP64 LU, so every status is OK and gg matches the
+    FP64 LU's."""
+    w = W.rocksalt_mno(33, 80)
+    sel = list(range(0, 8))
+    k, ph = w.wave_numbers()[sel], w.phase_shifts()[sel]
+    res = {}
+    for solver in (fms.SOLVER_MIXED, fms.SOLVER_FP64):
+        plan = fms.FmsRealSpacePlan(w.positions, w.potentials, global_lmax=w.global_lmax, raw_potential_lmax=w.raw_lmax,
+                                    direct_cutoff=w.rdirec, solver=solver)
+        sc, st = plan.solve(k, ph)
+        banner(f"seed 33 (N={w.state_count}) solver {solver}: statuses {list(st)}, FP64 fallback energies "
+               f"{plan.last_stats.fp64_fallback_energies}, sweeps {plan.last_stats.refinement_sweeps}")
+        assert not st.any()
+        res[solver] = sc
+        plan.close()
+    ip0 = int(w.potentials[0])
+    for e in range(len(sel)):
+        assert rel(res[fms.SOLVER_MIXED][e, ip0], res[fms.SOLVER_FP64][e, ip0]) <= GG_TOL
