@@ -594,19 +594,33 @@ def run_plan_set(args):
         t = fms.Tuning()
         fms.lib().refeff_fms_plan_get_tuning(p.handle, fms.C.byref(t))
         tuning.append((p.info.padded_order, t.lu_block, t.calu_min_rows, t.graph_sweeps))
+    # the same set in eager mode: same lanes and arenas, direct launches
+    ps.set_mode(fms.FmsPlanSet.EAGER)
+    st_em = ps.solve_device(dks, dphs, douts, counts)
+    synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st_em = ps.solve_device(dks, dphs, douts, counts)
+    synchronize()
+    t_set_eager = (time.perf_counter() - t0) / args.steps
+    progress(f"plan set, eager mode: {t_set_eager:.3f} s per step")
     ps.close()
-    # the eager loop: one plan and one device solve per cluster
+    # the eager loop: one plan and one device solve per cluster, each plan's
+    # batch workspace given back after its solve (64 plans holding theirs at
+    # once would not fit in HBM)
     plans = [fms.FmsRealSpacePlan(w.positions, w.potentials, global_lmax=w.global_lmax, raw_potential_lmax=w.raw_lmax,
                                   direct_cutoff=w.rdirec, solver=fms.SOLVER_MIXED) for w in wls]
-    from refeff_amd._lib import Stats
     import ctypes as C
-    stv = [np.zeros(n, np.int32) for n in counts]
+    stv = [np.full(n, -1, np.int32) for n in counts]
     L = fms.lib()
 
     def eager():
-        for p, dk, dph, dout, s_ in zip(plans, dks, dphs, douts, stv):
-            L.refeff_fms_solve_batch_device(p.handle, len(s_), dk.ptr, dph.ptr, dout.ptr,
-                                            s_.ctypes.data_as(C.POINTER(C.c_int32)), None, None)
+        for c, (p, dk, dph, dout, s_) in enumerate(zip(plans, dks, dphs, douts, stv)):
+            rc = L.refeff_fms_solve_batch_device(p.handle, len(s_), dk.ptr, dph.ptr, dout.ptr,
+                                                 s_.ctypes.data_as(C.POINTER(C.c_int32)), None, None)
+            if rc not in (0, 13):  # OK, or NO_CONVERGENCE (per-energy statuses tell)
+                raise RuntimeError(f"eager solve of cluster {c} failed with status {rc}")
+            p.release_workspace()
     eager()
     synchronize()
     t0 = time.perf_counter()
@@ -638,7 +652,11 @@ def run_plan_set(args):
                        "states_min": min(ns), "states_max": max(ns), "tuned_buckets": tuned,
                        "tune_seconds": round(t_tune, 2)},
             "eager_per_cluster_loop": {"seconds": round(t_eager, 4), "energy_points_per_s": round(total / t_eager, 3),
-                                       "graph_speedup": round(t_eager / (dt / args.steps), 3)},
+                                       "graph_speedup": round(t_eager / (dt / args.steps), 3),
+                                       "note": "one plan per cluster, workspace given back after each solve"},
+            "plan_set_eager_mode": {"seconds": round(t_set_eager, 4), "energy_points_per_s": round(total / t_set_eager, 3),
+                                    "graph_speedup": round(t_set_eager / (dt / args.steps), 3),
+                                    "status_nonzero": int(np.count_nonzero(st_em))},
             "resolved_energies_per_step": resolved / max(1, args.steps),
             "status_nonzero": int(np.count_nonzero(st)),
             "status_nonzero_where": [(int(c), int(e), int(st[o + e])) for c, o in

@@ -249,6 +249,10 @@ int refeff_fms_plan_energy_error(const refeff_fms_plan* plan, int32_t energy, re
 int refeff_fms_plan_create(const refeff_fms_plan_desc* desc, refeff_fms_plan** plan);
 void refeff_fms_plan_destroy(refeff_fms_plan* plan);
 int refeff_fms_plan_query(const refeff_fms_plan* plan, refeff_fms_plan_info* info);
+/* No counterpart in refeff (its plans hold no device memory): give back the
+ * plan's batch workspace (the next solve allocates it again), for callers that
+ * keep many plans and solve them one after another. */
+int refeff_fms_plan_release_workspace(refeff_fms_plan* plan);
 
 /* Solve every energy point.  Replaces fms_real_space_spectrum,
  * crates/refeff-core/src/fms/driver.rs:231-236, and packs the scattering table
@@ -304,8 +308,9 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, con
 /* No counterpart in refeff: faer picks its own blocking
  * (crates/refeff-linalg/src/lu.rs:385-397).  The LU variant of a plan:
  * outer block width (K of the trailing MFMA update), the row count from which
- * FP32 panels use tournament pivoting, the 3M complex product, and the fixed
- * refinement sweeps a plan set's graph runs (mixed solver). */
+ * FP32 panels use tournament pivoting, the 3M complex product, and the most
+ * refinement sweeps a plan set's graph launches (mixed solver; energies that
+ * have settled skip the rest). */
 typedef struct refeff_fms_tuning {
     int32_t lu_block;
     int32_t calu_min_rows;
@@ -317,8 +322,8 @@ typedef struct refeff_fms_tuning {
 int refeff_fms_plan_get_tuning(const refeff_fms_plan* plan, refeff_fms_tuning* tuning);
 int refeff_fms_plan_set_tuning(refeff_fms_plan* plan, const refeff_fms_tuning* tuning);
 /* Time the LU variants on up to 8 of the given energies (host arrays, layout of
- * refeff_fms_solve_batch), keep the fastest on the plan and return it; the
- * graph sweeps are three more than the eager refinement needed (at most 8). */
+ * refeff_fms_solve_batch), keep the fastest on the plan and return it (graph
+ * sweeps 8, the eager refinement's limit). */
 int refeff_fms_plan_tune(refeff_fms_plan* plan, int32_t energy_count, const double* wave_numbers,
                          const double* phase_shifts, refeff_fms_tuning* best);
 
@@ -336,7 +341,7 @@ typedef struct refeff_fms_set_stats {
     double seconds_graph;       /* the graph launch, to completion */
     int64_t graph_nodes;
     int32_t captured;           /* 1: this call captured a new graph */
-    int32_t resolved_energies;  /* energies the fixed sweeps left unsettled, solved again eagerly */
+    int32_t resolved_energies;  /* energies the refinement left unsettled, solved again by the FP64 LU */
     int32_t energies;
 } refeff_fms_set_stats;
 
@@ -358,6 +363,13 @@ int refeff_fms_plan_set_tune(refeff_fms_plan_set* set, const int32_t* energy_cou
 int refeff_fms_plan_set_solve(refeff_fms_plan_set* set, const int32_t* energy_counts, const double* const* d_wave_numbers,
                               const double* const* d_phase_shifts, double* const* d_scattering, int32_t* status,
                               refeff_fms_set_stats* stats);
+/* How refeff_fms_plan_set_solve runs the lanes: GRAPH (default) replays the
+ * captured graph; EAGER launches directly, one host thread per lane, in the
+ * same arenas, with the refinement's stopping rule on the host.  Results are
+ * the same to the refinement tolerance; which is faster depends on the mix of
+ * cluster sizes (launch overhead against idle refinement sweeps). */
+enum { REFEFF_FMS_SET_GRAPH = 0, REFEFF_FMS_SET_EAGER = 1 };
+int refeff_fms_plan_set_mode(refeff_fms_plan_set* set, int32_t mode);
 
 /* ---------------------------------------------------------------------- */
 /* BiCGSTAB preconditioners (REFEFF_FMS_SOLVER_BICGSTAB plans)              */

@@ -121,6 +121,8 @@ def lib():
     L.refeff_fms_plan_energy_error.argtypes = [vp, i32, C.POINTER(ErrorDetail)]
     L.refeff_fms_plan_destroy.argtypes = [vp]
     L.refeff_fms_plan_destroy.restype = None
+    L.refeff_fms_plan_release_workspace.argtypes = [vp]
+    L.refeff_fms_plan_release_workspace.restype = C.c_int
     L.refeff_fms_plan_query.argtypes = [vp, C.POINTER(PlanInfo)]
     L.refeff_fms_solve_batch.argtypes = [vp, i32, f64p, f64p, f64p, i32p, C.POINTER(Stats)]
     L.refeff_fms_solve_batch_device.argtypes = [vp, i32, vp, vp, vp, i32p, C.POINTER(Stats), vp]
@@ -152,6 +154,7 @@ def lib():
     L.refeff_fms_plan_set_plan.argtypes = [vp, i32, C.POINTER(vp)]
     L.refeff_fms_plan_set_tune.argtypes = [vp, vp, vp, vp, i32p]
     L.refeff_fms_plan_set_solve.argtypes = [vp, vp, vp, vp, vp, i32p, vp]
+    L.refeff_fms_plan_set_mode.argtypes = [vp, i32]
     L.refeff_fms_plan_profile.argtypes = [vp, i32]
     L.refeff_fms_plan_profile_read.argtypes = [vp, C.POINTER(Profile)]
     _lib = L

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "fms_common.h"
@@ -2031,6 +2032,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, const int* permb, int n, int nrhs,
                                                 typename Cx<T>::type* X, const int* emap) {
     const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y);
+    if (e < 0) return;  // a settled energy (graph-mode refinement)
     const size_t total = static_cast<size_t>(n) * nrhs;
     for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
         const int i = static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
@@ -2204,6 +2206,7 @@ __global__ __launch_bounds__(NW * 64) void k_trsm_wg(const typename Cx<T>::type*
     __shared__ T2 red[NW / 2][BS][NC];
     __shared__ T2 dia[BS][BS + 1];
     const int e = emap ? emap[blockIdx.x] : static_cast<int>(blockIdx.x), tid = threadIdx.x;
+    if (e < 0) return;
     const int c0 = blockIdx.y * NC;
     const int ncol = min(NC, ldx - c0);
     const T2* LU = LUb + e * sa;
@@ -2256,6 +2259,7 @@ __global__ __launch_bounds__(NW * 64) void k_solve_gemv(const typename Cx<T>::ty
     constexpr int BS = kPanel, NC = NT * 16;
     __shared__ T2 red[NW / 2][BS][NC];
     const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y), tid = threadIdx.x;
+    if (e < 0) return;
     const int R0 = r_lo + blockIdx.x * BS, c0 = blockIdx.z * NC;
     const int ncol = min(NC, ldx - c0);
     T2* X = Xb + e * sx;
@@ -2280,6 +2284,7 @@ __global__ __launch_bounds__(512) void k_residual_wg(const double2* Ab, size_t s
     constexpr int BS = kPanel, NC = NT * 16;
     __shared__ double2 red[4][BS][NC];
     const int R0 = blockIdx.x * BS, e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y), tid = threadIdx.x;
+    if (e < 0) return;
     const int c0 = blockIdx.z * NC;
     const int ncol = min(NC, ldx - c0);
     typename Mfma<double>::acc cr[2][NT], ci[2][NT];
@@ -2304,6 +2309,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_update(double2* X, const typename Cx<T>::type* D, size_t sx, int n, int nrhs,
                                                 double* norms, const int* emap) {
     const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y);
+    if (e < 0) return;
     const size_t total = static_cast<size_t>(n) * nrhs;
     double md = 0.0, mxv = 0.0;
     for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
@@ -2323,6 +2329,58 @@ __global__ __launch_bounds__(256) void k_update(double2* X, const typename Cx<T>
         atomic_max_nonneg(&norms[e * 4 + 1], md);
         atomic_max_nonneg(&norms[e * 4 + 2], mxv);
     }
+}
+
+// Device-side stopping rule of the refinement, for solves that cannot wait on
+// the host between sweeps (graph mode).  After sweep `sweep` (norms from
+// k_update) each energy still refining updates its state
+//   state[4e + 0] = previous correction d_{s-1} / |x|,
+//   state[4e + 1] = estimated relative error left,
+//   state[4e + 2] = 1 while it refines, 0 once settled,
+//   state[4e + 3] = sweeps it has run,
+// by the host loop's rule: the error left after a correction d_s is about
+// d_s min(1, d_s / d_{s-1}); an energy settles from its second sweep on when
+// that is <= 1e-10 or when a correction <= 1e-9 stopped shrinking.  The
+// energies still refining are then listed in emap[0 .. m) in order and
+// emap[m .. nb) is -1, so the next sweep's blocks for settled energies exit.
+// One workgroup of 256 threads; thread t owns a contiguous run of energies.
+__global__ __launch_bounds__(256) void k_refine_live(const double* norms, int nb, int sweep, double* state, int* emap) {
+    __shared__ int cnt[256];
+    const int tid = threadIdx.x;
+    const int per = (nb + 255) / 256, lo = min(nb, tid * per), hi = min(nb, lo + per);
+    int m = 0;
+    for (int e = lo; e < hi; ++e) {
+        double* q = state + 4 * static_cast<size_t>(e);
+        if (sweep == 0) {
+            q[0] = 1.0e300;
+            q[2] = 1.0;
+        }
+        if (q[2] != 0.0) {
+            const double rel = norms[4 * e + 2] > 0.0 ? norms[4 * e + 1] / norms[4 * e + 2] : 0.0;
+            const double prev = q[0];
+            const double est = sweep == 0 ? rel : rel * fmin(1.0, rel / prev);
+            const bool small = est <= 1.0e-10;
+            const bool floor = rel <= 1.0e-9 && rel >= 0.5 * prev;
+            q[1] = floor ? rel : est;
+            q[0] = rel;
+            q[3] = sweep + 1;
+            if (sweep > 0 && (small || floor)) q[2] = 0.0;
+        }
+        m += q[2] != 0.0 ? 1 : 0;
+    }
+    cnt[tid] = m;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {  // inclusive scan of the counts
+        const int v = tid >= off ? cnt[tid - off] : 0;
+        __syncthreads();
+        cnt[tid] += v;
+        __syncthreads();
+    }
+    int pos = cnt[tid] - m;
+    const int total = cnt[255];
+    for (int e = lo; e < hi; ++e)
+        if (state[4 * static_cast<size_t>(e) + 2] != 0.0) emap[pos++] = e;
+    for (int i = total + tid; i < nb; i += 256) emap[i] = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -4151,15 +4209,17 @@ refeff_fms_error energy_detail(refeff_fms_plan* p, int e, int info, int status, 
 }
 
 // Graph mode of solve_device_batch (plan sets): every launch goes to one
-// stream that a HIP graph capture has joined, nothing waits on the host, the
-// mixed solver refines a fixed number of sweeps over every energy, and the
-// per-energy results the host needs afterwards are copied to device arrays:
-// info [nb], and the correction norms [sweep][nb][4] (k_update's layout).
+// stream that a HIP graph capture has joined and nothing waits on the host.
+// The mixed solver launches `sweeps` refinement sweeps; k_refine_live applies
+// the stopping rule on the device, so a sweep's launches skip the energies
+// already settled.  The per-energy results the host needs afterwards are left
+// in device arrays: info [nb], and the refinement state [nb][4]
+// (k_refine_live's layout; entry 1 is the estimated relative error left).
 struct GraphCtx {
     hipStream_t s;
     int sweeps;
     int* info_out;
-    double* norms_out;
+    double* state_out;
 };
 
 int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status, refeff_fms_stats* st,
@@ -4254,13 +4314,16 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
     std::vector<double> est_err(nb, 0.0);  // estimated relative error left after refinement
     const int gx = std::max(1, std::min(1024, static_cast<int>((sx + 255) / 256)));
     if (gc) {
-        // fixed sweeps over every energy, norms to the graph's device array
+        // sweeps with the stopping rule on the device: from the second sweep
+        // on, emap lists the energies still refining (-1 for the rest)
         for (const auto& g : gs) {
             if (mixed) {
                 float2* w = static_cast<float2*>(p->d_w);
                 hipLaunchKernelGGL(k_fill_double, dim3(std::max<size_t>(1, std::min<size_t>(65535, (2 * sx * g.nb + 255) / 256))),
                                    dim3(256), 0, g.s, reinterpret_cast<double*>(p->d_x64 + sx * g.e0), 0.0, 2 * sx * g.nb);
+                int* live_map = p->d_emap + g.e0;
                 for (int sweep = 0; sweep < gc->sweeps; ++sweep) {
+                    const int* emap = sweep > 0 ? live_map : nullptr;
                     double* gn = p->d_norms + 4 * static_cast<size_t>(g.e0);
                     hipLaunchKernelGGL(k_fill_double, dim3((4 * g.nb + 255) / 256), dim3(256), 0, g.s, gn, 0.0,
                                        static_cast<size_t>(4 * g.nb));
@@ -4272,23 +4335,22 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                         if (ntile == 1)
                             hipLaunchKernelGGL(k_residual_wg<1>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
                                                p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
-                                               p->nrhs, n, gn, static_cast<const int*>(nullptr));
+                                               p->nrhs, n, gn, emap);
                         else
                             hipLaunchKernelGGL(k_residual_wg<2>, grid, block, 0, g.s, p->d_a64 + sa * g.e0, sa, n,
                                                p->d_x64 + sx * g.e0, p->d_b64 + sx * g.e0, p->d_r64 + sx * g.e0, sx,
-                                               p->nrhs, n, gn, static_cast<const int*>(nullptr));
+                                               p->nrhs, n, gn, emap);
                         rhs = p->d_r64 + sx * g.e0;
                     }
                     hipLaunchKernelGGL(k_gather<float>, dim3(gx, g.nb), dim3(256), 0, g.s, rhs, sx,
-                                       p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0,
-                                       static_cast<const int*>(nullptr));
+                                       p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0, emap);
                     run_solve<float>(p, g, p->d_a32 + sa * g.e0,
                                      static_cast<const float2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
-                                     w + sx * g.e0);
+                                     w + sx * g.e0, emap);
                     hipLaunchKernelGGL(k_update<float>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_x64 + sx * g.e0,
-                                       w + sx * g.e0, sx, n, p->nrhs, gn, static_cast<const int*>(nullptr));
-                    (void)hipMemcpyAsync(gc->norms_out + (static_cast<size_t>(sweep) * nb + g.e0) * 4, gn,
-                                         4 * g.nb * sizeof(double), hipMemcpyDeviceToDevice, g.s);
+                                       w + sx * g.e0, sx, n, p->nrhs, gn, emap);
+                    hipLaunchKernelGGL(k_refine_live, dim3(1), dim3(256), 0, g.s, gn, g.nb, sweep,
+                                       gc->state_out + 4 * static_cast<size_t>(g.e0), live_map);
                 }
             } else {
                 hipLaunchKernelGGL(k_gather<double>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_b64 + sx * g.e0, sx,
@@ -4712,6 +4774,20 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     return REFEFF_FMS_OK;
 }
 
+int refeff_fms_plan_release_workspace(refeff_fms_plan* p) {
+    if (!p) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (p->stream && hipStreamSynchronize(p->stream) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    for (hipStream_t q : p->streams)
+        if (q && hipStreamSynchronize(q) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    free_workspace(p);
+    if (p->twin64) {
+        std::lock_guard<std::mutex> lock64(p->twin64->mu);
+        free_workspace(p->twin64);
+    }
+    return REFEFF_FMS_OK;
+}
+
 void refeff_fms_plan_destroy(refeff_fms_plan* p) {
     if (!p) return;
     refeff_fms_plan_destroy(p->twin64);
@@ -5004,8 +5080,15 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
     }
     if (device && ext) (void)hipStreamSynchronize(ext);
     for (int e0 = 0; e0 < ne; e0 += batch) {
-        const int nb = std::min(batch, ne - e0);
+        int nb = std::min(batch, ne - e0);
         int r = arena ? REFEFF_FMS_OK : ensure_workspace(p, nb);
+        // the device memory this plan's batch was sized for at creation may
+        // have gone to other plans since: go on with smaller batches
+        while (r == REFEFF_FMS_NO_MEMORY && batch > 1) {
+            batch = std::max(1, batch / 2);
+            nb = std::min(batch, ne - e0);
+            r = ensure_workspace(p, nb);
+        }
         if (r) return r;
         const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         if (hipMemcpyAsync(p->d_k, k + 2 * kst * e0, 2 * kst * nb * sizeof(double), kind, p->stream) != hipSuccess ||

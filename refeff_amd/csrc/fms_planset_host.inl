@@ -6,17 +6,19 @@
 // first, each to the lane with the least work so far, and a lane solves its
 // clusters one after another in a device arena of its own (each cluster's
 // batch buffers carved from it, so every lane holds one cluster's workspace at
-// a time and batches stay large).  Inside the graph the mixed
-// solver refines a fixed number of sweeps over every energy and leaves its
-// correction norms in device memory; after the launch the host applies the
-// same convergence rule as the eager path and re-solves any energy the fixed
-// sweeps did not settle.  The instantiated graph is kept and replayed while
-// the call's device pointers and energy counts stay the same.
+// a time and batches stay large).  Inside the graph the mixed solver's
+// refinement applies the eager path's stopping rule on the device (settled
+// energies skip the later sweeps' work) and leaves each energy's estimated
+// error in device memory; after the launch the host turns that into statuses
+// and solves any energy the sweeps did not settle with the FP64 LU.  The
+// instantiated graph is kept and replayed while the call's device pointers
+// and energy counts stay the same.  The eager mode runs the same lanes and
+// arenas without a graph: one host thread per lane launches its clusters'
+// solves directly, with the refinement's stopping rule on the host.
 //
 // The autotuner times the LU variants (outer block, tall-panel method) of a
-// plan on a few of its energies, keeps the fastest, records the refinement
-// sweeps it needed, and caches the choice per (padded order, right-hand sides,
-// solver) in a text file.
+// plan on a few of its energies, keeps the fastest, and caches the choice per
+// (padded order, right-hand sides, solver) in a text file.
 
 namespace {
 
@@ -105,11 +107,13 @@ struct refeff_fms_plan_set {
     std::vector<size_t> arena_bytes;
     std::vector<hipStream_t> lane_stream;
     std::vector<size_t> eoff;      // first energy of each cluster in the concatenated lists
-    std::vector<int> sweeps;       // fixed refinement sweeps of each cluster
+    std::vector<int> sweeps;       // refinement sweeps launched for each cluster (settled energies skip)
     int* d_info = nullptr;
-    double* d_norms = nullptr;     // per batch [sweep][nb][4]
-    std::vector<size_t> norm_off;  // per cluster: offset of its first batch's norms
+    double* d_norms = nullptr;     // refinement state per energy [4] (k_refine_live)
+    std::vector<size_t> norm_off;  // per cluster: offset of its first energy's state
     size_t graph_nodes = 0;
+    int mode = REFEFF_FMS_SET_GRAPH;
+    std::vector<int32_t> layout_counts;  // energy counts the lanes and arenas were laid out for
     std::mutex mu;
 };
 
@@ -129,28 +133,203 @@ void plan_set_drop_graph(refeff_fms_plan_set* s) {
     if (s->d_norms) (void)hipFree(s->d_norms);
     s->d_info = nullptr;
     s->d_norms = nullptr;
+    s->layout_counts.clear();
 }
 
-// Per-energy status from the graph's info and fixed-sweep norms, by the rule of
-// the eager path: after a correction of relative size d_s (previous d_{s-1})
-// the error left is about d_s min(1, d_s / d_{s-1}); converged when that is
-// <= 1e-9, or when a correction <= 1e-9 stopped shrinking.
-int graph_energy_status(const refeff_fms_plan* p, int info, const double* nrm, int nb, int j, int sweeps,
-                        double* est_out) {
+// Per-energy status from the graph's info and the refinement state that
+// k_refine_live left ([4]: previous correction, estimated error, live, sweeps):
+// converged, as on the eager path, when the estimated error is <= 1e-9.
+int graph_energy_status(const refeff_fms_plan* p, int info, const double* state, double* est_out) {
     const int s = map_info(info);
     if (s || p->solver != REFEFF_FMS_SOLVER_MIXED) return s;
-    double prev = 1.0e300, est = 1.0e300;
-    for (int w = 0; w < sweeps; ++w) {
-        const double* q = nrm + (static_cast<size_t>(w) * nb + j) * 4;
-        const double rel = q[2] > 0 ? q[1] / q[2] : 0.0;
-        const bool floor = rel <= 1.0e-9 && rel >= 0.5 * prev;
-        est = (w == 0) ? rel : (floor ? rel : rel * std::min(1.0, rel / prev));
-        prev = rel;
-    }
-    *est_out = est;
-    return (est <= 1.0e-9) ? REFEFF_FMS_OK : REFEFF_FMS_NO_CONVERGENCE;
+    *est_out = state[1];
+    return (state[1] <= 1.0e-9) ? REFEFF_FMS_OK : REFEFF_FMS_NO_CONVERGENCE;
 }
 
+// Lanes and arenas for the given energy counts (both modes) and the graph's
+// per-energy result arrays.
+int plan_set_layout(refeff_fms_plan_set* s, const int32_t* energy_counts) {
+    const int nc = static_cast<int>(s->plans.size());
+    plan_set_drop_graph(s);
+    // lanes: as many as hold the largest cluster's full batch within 70 % of
+    // free device memory (at most kMaxLanes); clusters dealt largest first to
+    // the lane with the least work (npad^3 x energies)
+    constexpr int kMaxLanes = 4;
+    s->gbatch.assign(nc, 0);
+    s->eoff.assign(nc + 1, 0);
+    s->sweeps.assign(nc, 1);
+    size_t biggest = 0;
+    for (int c = 0; c < nc; ++c) {
+        refeff_fms_plan* p = s->plans[c];
+        s->gbatch[c] = std::max(0, std::min(energy_counts[c], p->batch));
+        s->eoff[c + 1] = s->eoff[c] + energy_counts[c];
+        s->sweeps[c] = p->solver == REFEFF_FMS_SOLVER_MIXED ? p->graph_sweeps : 1;
+        if (s->gbatch[c]) biggest = std::max(biggest, carved_bytes(p, s->gbatch[c]));
+    }
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const size_t budget = free_b / 10 * 7;
+    int max_lanes = kMaxLanes;
+    if (const char* e = std::getenv("REFEFF_FMS_SET_LANES")) max_lanes = std::max(1, std::min(16, std::atoi(e)));
+    int nl = std::max(1, std::min<int>(max_lanes, biggest ? static_cast<int>(budget / biggest) : 1));
+    size_t arena = biggest ? std::min(budget / nl, std::max(biggest, budget / nl)) : 256;
+    if (biggest > arena) arena = budget;  // one lane, smaller batches below
+    for (int c = 0; c < nc; ++c) {  // batches that fit the arena
+        refeff_fms_plan* p = s->plans[c];
+        while (s->gbatch[c] > 1 && carved_bytes(p, s->gbatch[c]) > arena) s->gbatch[c] = (s->gbatch[c] + 1) / 2;
+        if (s->gbatch[c] && carved_bytes(p, s->gbatch[c]) > arena) return REFEFF_FMS_NO_MEMORY;
+    }
+    s->lanes.assign(nl, {});
+    std::vector<double> load(nl, 0.0);
+    for (int c : s->order) {
+        if (energy_counts[c] == 0) continue;
+        const int l = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        s->lanes[l].push_back(c);
+        load[l] += std::pow(static_cast<double>(s->plans[c]->npad), 3) * energy_counts[c];
+    }
+    for (int l = 0; l < nl; ++l) {
+        size_t need = 256;
+        for (int c : s->lanes[l]) need = std::max(need, carved_bytes(s->plans[c], s->gbatch[c]));
+        unsigned char* ptr = nullptr;
+        if (hipMalloc(&ptr, need) != hipSuccess) {
+            plan_set_drop_graph(s);
+            (void)hipGetLastError();
+            return REFEFF_FMS_NO_MEMORY;
+        }
+        s->arena.push_back(ptr);
+        s->arena_bytes.push_back(need);
+    }
+    while (static_cast<int>(s->lane_stream.size()) < nl) {
+        hipStream_t q = nullptr;
+        if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+        s->lane_stream.push_back(q);
+    }
+    s->norm_off.assign(nc + 1, 0);
+    for (int c = 0; c < nc; ++c)
+        s->norm_off[c + 1] = s->norm_off[c] + 4 * (s->eoff[c + 1] - s->eoff[c]);
+    if (hipMalloc(&s->d_info, std::max<size_t>(1, s->eoff[nc]) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&s->d_norms, std::max<size_t>(1, s->norm_off[nc]) * sizeof(double)) != hipSuccess) {
+        plan_set_drop_graph(s);
+        return REFEFF_FMS_NO_MEMORY;
+    }
+    s->layout_counts.assign(energy_counts, energy_counts + nc);
+    return REFEFF_FMS_OK;
+}
+
+// Capture of every lane's solves into one graph (fork from the origin stream,
+// join back) and its instantiation.
+int plan_set_capture(refeff_fms_plan_set* s, const int32_t* energy_counts, const double* const* d_k,
+                     const double* const* d_ph, double* const* d_scattering) {
+    const int nl = static_cast<int>(s->lanes.size());
+    // capture: fork from the origin stream into every lane
+    if (hipStreamBeginCapture(s->origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
+    int rc = hipEventRecord(s->fork, s->origin) == hipSuccess ? REFEFF_FMS_OK : REFEFF_FMS_DEVICE_ERROR;
+    for (int l = 0; l < nl && !rc; ++l) {
+        hipStream_t q = s->lane_stream[l];
+        if (hipStreamWaitEvent(q, s->fork, 0) != hipSuccess) {
+            rc = REFEFF_FMS_DEVICE_ERROR;
+            break;
+        }
+        for (int c : s->lanes[l]) {
+            refeff_fms_plan* p = s->plans[c];
+            WorkspaceSnapshot keep(p);  // the plan's own workspace comes back at the end of the scope
+            size_t off = 0;
+            workspace_layout(p, s->gbatch[c], [&](void** slot, size_t bytes) {
+                *slot = s->arena[l] + off;
+                off += (bytes + 255) / 256 * 256;
+                return true;
+            });
+            p->ws_batch = s->gbatch[c];
+            p->profile = false;
+            const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
+            const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
+            for (int e0 = 0; e0 < energy_counts[c] && !rc; e0 += s->gbatch[c]) {
+                const int nb = std::min(s->gbatch[c], energy_counts[c] - e0);
+                if (hipMemcpyAsync(p->d_k, d_k[c] + 2 * kst * e0, 2 * kst * nb * sizeof(double),
+                                   hipMemcpyDeviceToDevice, q) != hipSuccess ||
+                    hipMemcpyAsync(p->d_ph, d_ph[c] + 2 * pst * e0, 2 * pst * nb * sizeof(double),
+                                   hipMemcpyDeviceToDevice, q) != hipSuccess) {
+                    rc = REFEFF_FMS_DEVICE_ERROR;
+                    break;
+                }
+                const GraphCtx gc{q, s->sweeps[c], s->d_info + s->eoff[c] + e0,
+                                  s->d_norms + s->norm_off[c] + 4 * static_cast<size_t>(e0)};
+                rc = solve_device_batch(p, nb, reinterpret_cast<double2*>(d_scattering[c]) + sst * e0, nullptr,
+                                        nullptr, nullptr, 0, nullptr, &gc);
+            }
+            if (rc) break;
+        }
+        if (!rc && (hipEventRecord(s->join[l], q) != hipSuccess || hipStreamWaitEvent(s->origin, s->join[l], 0) != hipSuccess))
+            rc = REFEFF_FMS_DEVICE_ERROR;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s->origin, &g);
+    if (rc || ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        plan_set_drop_graph(s);
+        (void)hipGetLastError();
+        return rc ? rc : REFEFF_FMS_DEVICE_ERROR;
+    }
+    s->graph = g;
+    if (hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0) != hipSuccess) {
+        plan_set_drop_graph(s);
+        (void)hipGetLastError();
+        return REFEFF_FMS_DEVICE_ERROR;
+    }
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(s->graph, nullptr, &nodes);
+    s->graph_nodes = nodes;
+    return REFEFF_FMS_OK;
+}
+
+// Eager mode: one host thread per lane solves its clusters one after another
+// in the lane's arena, with the host-driven stopping rule of the refinement;
+// energies it leaves unsettled go to the FP64 LU as in graph mode.
+int plan_set_eager(refeff_fms_plan_set* s, const int32_t* energy_counts, const double* const* d_k,
+                   const double* const* d_ph, double* const* d_scattering, int32_t* st_all, int* resolved) {
+    const int nl = static_cast<int>(s->lanes.size());
+    std::vector<int> lane_rc(nl, REFEFF_FMS_OK), lane_redo(nl, 0);
+    int device = 0;
+    (void)hipGetDevice(&device);
+    auto work = [&](int l) {
+        if (hipSetDevice(device) != hipSuccess) {
+            lane_rc[l] = REFEFF_FMS_DEVICE_ERROR;
+            return;
+        }
+        const ScratchArena a{s->arena[l], s->arena_bytes[l]};
+        for (int c : s->lanes[l]) {
+            refeff_fms_plan* p = s->plans[c];
+            int32_t* st = st_all + s->eoff[c];
+            const int r = solve_common(p, energy_counts[c], d_k[c], d_ph[c], d_scattering[c], st, nullptr, true,
+                                       nullptr, nullptr, &a);
+            if (r && r != REFEFF_FMS_NO_CONVERGENCE && r != REFEFF_FMS_SINGULAR && r != REFEFF_FMS_NON_FINITE) {
+                lane_rc[l] = r;
+                return;
+            }
+            std::vector<int> bad;
+            for (int e = 0; e < energy_counts[c]; ++e)
+                if (st[e] == REFEFF_FMS_NO_CONVERGENCE) bad.push_back(e);
+            if (bad.empty()) continue;
+            lane_redo[l] += static_cast<int>(bad.size());
+            std::lock_guard<std::mutex> plock(p->mu);
+            const int r2 = fp64_fallback(p, bad, d_k[c], d_ph[c], d_scattering[c], st, true);
+            if (r2 == REFEFF_FMS_DEVICE_ERROR || r2 == REFEFF_FMS_NO_MEMORY) {
+                lane_rc[l] = r2;
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int l = 1; l < nl; ++l) th.emplace_back(work, l);
+    if (nl > 0) work(0);
+    for (auto& t : th) t.join();
+    *resolved = 0;
+    for (int l = 0; l < nl; ++l) {
+        *resolved += lane_redo[l];
+        if (lane_rc[l]) return lane_rc[l];
+    }
+    return REFEFF_FMS_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -190,7 +369,6 @@ int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const 
     const refeff_fms_tuning keep = current_tuning(p);
     refeff_fms_tuning win = keep;
     double best_s = 1.0e300;
-    int best_sweeps = 1;
     for (const auto& c : cands) {
         apply_tuning(p, c);
         refeff_fms_stats s1{};
@@ -209,10 +387,9 @@ int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const 
         if (dt < best_s) {
             best_s = dt;
             win = c;
-            best_sweeps = s1.refinement_sweeps;
         }
     }
-    win.graph_sweeps = std::max(1, std::min(8, best_sweeps + 3));  // margin for the harder energies of the sweep
+    win.graph_sweeps = 8;  // the eager path's limit; settled energies skip the later sweeps
     win.seconds_per_energy = best_s / nt;
     apply_tuning(p, win);
     if (best) *best = win;
@@ -308,6 +485,13 @@ int refeff_fms_plan_set_tune(refeff_fms_plan_set* s, const int32_t* energy_count
     return write_tune_cache(s->tune_cache, s->tuned) ? REFEFF_FMS_OK : REFEFF_FMS_IO_ERROR;
 }
 
+int refeff_fms_plan_set_mode(refeff_fms_plan_set* s, int32_t mode) {
+    if (!s || (mode != REFEFF_FMS_SET_GRAPH && mode != REFEFF_FMS_SET_EAGER)) return REFEFF_FMS_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->mode = mode;
+    return REFEFF_FMS_OK;
+}
+
 int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_counts, const double* const* d_k,
                               const double* const* d_ph, double* const* d_scattering, int32_t* status,
                               refeff_fms_set_stats* stats) {
@@ -319,175 +503,63 @@ int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_coun
             return REFEFF_FMS_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(s->mu);
     const double t0 = now_s();
-    auto make_key = [&]() {
-    std::vector<const void*> key;
+    std::vector<const void*> key;  // what the kept graph was captured for
     for (int c = 0; c < nc; ++c) {
         key.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(energy_counts[c])));
         key.push_back(energy_counts[c] ? d_k[c] : nullptr);
         key.push_back(energy_counts[c] ? d_ph[c] : nullptr);
         key.push_back(energy_counts[c] ? d_scattering[c] : nullptr);
     }
-    return key;
-    };
+    const bool same_counts = s->layout_counts.size() == static_cast<size_t>(nc) &&
+                             std::equal(s->layout_counts.begin(), s->layout_counts.end(), energy_counts);
+    int rc = REFEFF_FMS_OK, redo = 0;
     bool captured = false;
-    const std::vector<const void*> key = make_key();
-    (void)make_key;
-    if (!s->exec || key != s->key) {
-        plan_set_drop_graph(s);
-        // lanes: as many as hold the largest cluster's full batch within 70 % of
-        // free device memory (at most kMaxLanes); clusters dealt largest first to
-        // the lane with the least work (npad^3 x energies)
-        constexpr int kMaxLanes = 4;
-        s->gbatch.assign(nc, 0);
-        s->eoff.assign(nc + 1, 0);
-        s->sweeps.assign(nc, 1);
-        size_t biggest = 0;
+    double t_capture = 0.0, t_graph = 0.0;
+    std::vector<int32_t> st_all;
+    if (s->mode == REFEFF_FMS_SET_EAGER) {
+        if (!same_counts && (rc = plan_set_layout(s, energy_counts)) != REFEFF_FMS_OK) return rc;
+        st_all.assign(std::max<size_t>(1, s->eoff[nc]), 0);
+        const double t1 = now_s();
+        rc = plan_set_eager(s, energy_counts, d_k, d_ph, d_scattering, st_all.data(), &redo);
+        t_graph = now_s() - t1;
+        if (rc) return rc;
+    } else {
+        if (!s->exec || key != s->key) {
+            if ((rc = plan_set_layout(s, energy_counts)) != REFEFF_FMS_OK) return rc;
+            if ((rc = plan_set_capture(s, energy_counts, d_k, d_ph, d_scattering)) != REFEFF_FMS_OK) return rc;
+            s->key = key;
+            captured = true;
+        }
+        t_capture = now_s() - t0;
+        const double t1 = now_s();
+        if (hipGraphLaunch(s->exec, s->origin) != hipSuccess || hipStreamSynchronize(s->origin) != hipSuccess)
+            return REFEFF_FMS_DEVICE_ERROR;
+        t_graph = now_s() - t1;
+        std::vector<int> info(std::max<size_t>(1, s->eoff[nc]));
+        std::vector<double> nrm(std::max<size_t>(1, s->norm_off[nc]));
+        if (hipMemcpy(info.data(), s->d_info, s->eoff[nc] * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(nrm.data(), s->d_norms, s->norm_off[nc] * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return REFEFF_FMS_DEVICE_ERROR;
+        // statuses; the energies the sweeps left unsettled go, per cluster in
+        // one call, to the plain FP64 LU (the mixed path's own fallback)
+        st_all.assign(std::max<size_t>(1, s->eoff[nc]), 0);
         for (int c = 0; c < nc; ++c) {
             refeff_fms_plan* p = s->plans[c];
-            s->gbatch[c] = std::max(0, std::min(energy_counts[c], p->batch));
-            s->eoff[c + 1] = s->eoff[c] + energy_counts[c];
-            s->sweeps[c] = p->solver == REFEFF_FMS_SOLVER_MIXED ? p->graph_sweeps : 1;
-            if (s->gbatch[c]) biggest = std::max(biggest, carved_bytes(p, s->gbatch[c]));
-        }
-        size_t free_b = 0, total_b = 0;
-        (void)hipMemGetInfo(&free_b, &total_b);
-        const size_t budget = free_b / 10 * 7;
-        int max_lanes = kMaxLanes;
-        if (const char* e = std::getenv("REFEFF_FMS_SET_LANES")) max_lanes = std::max(1, std::min(16, std::atoi(e)));
-        int nl = std::max(1, std::min<int>(max_lanes, biggest ? static_cast<int>(budget / biggest) : 1));
-        size_t arena = biggest ? std::min(budget / nl, std::max(biggest, budget / nl)) : 256;
-        if (biggest > arena) arena = budget;  // one lane, smaller batches below
-        for (int c = 0; c < nc; ++c) {  // batches that fit the arena
-            refeff_fms_plan* p = s->plans[c];
-            while (s->gbatch[c] > 1 && carved_bytes(p, s->gbatch[c]) > arena) s->gbatch[c] = (s->gbatch[c] + 1) / 2;
-            if (s->gbatch[c] && carved_bytes(p, s->gbatch[c]) > arena) return REFEFF_FMS_NO_MEMORY;
-        }
-        s->lanes.assign(nl, {});
-        std::vector<double> load(nl, 0.0);
-        for (int c : s->order) {
-            if (energy_counts[c] == 0) continue;
-            const int l = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-            s->lanes[l].push_back(c);
-            load[l] += std::pow(static_cast<double>(s->plans[c]->npad), 3) * energy_counts[c];
-        }
-        for (int l = 0; l < nl; ++l) {
-            size_t need = 256;
-            for (int c : s->lanes[l]) need = std::max(need, carved_bytes(s->plans[c], s->gbatch[c]));
-            unsigned char* ptr = nullptr;
-            if (hipMalloc(&ptr, need) != hipSuccess) {
-                plan_set_drop_graph(s);
-                (void)hipGetLastError();
-                return REFEFF_FMS_NO_MEMORY;
+            std::vector<int> bad;
+            for (int e = 0; e < energy_counts[c]; ++e) {
+                double est = 0.0;
+                const int stt = graph_energy_status(p, info[s->eoff[c] + e],
+                                                    nrm.data() + s->norm_off[c] + 4 * static_cast<size_t>(e), &est);
+                st_all[s->eoff[c] + e] = stt;
+                if (stt == REFEFF_FMS_NO_CONVERGENCE) bad.push_back(e);
             }
-            s->arena.push_back(ptr);
-            s->arena_bytes.push_back(need);
-        }
-        while (static_cast<int>(s->lane_stream.size()) < nl) {
-            hipStream_t q = nullptr;
-            if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
-            s->lane_stream.push_back(q);
-        }
-        s->norm_off.assign(nc + 1, 0);
-        for (int c = 0; c < nc; ++c)
-            s->norm_off[c + 1] = s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * (s->eoff[c + 1] - s->eoff[c]);
-        if (hipMalloc(&s->d_info, std::max<size_t>(1, s->eoff[nc]) * sizeof(int)) != hipSuccess ||
-            hipMalloc(&s->d_norms, std::max<size_t>(1, s->norm_off[nc]) * sizeof(double)) != hipSuccess) {
-            plan_set_drop_graph(s);
-            return REFEFF_FMS_NO_MEMORY;
-        }
-        // capture: fork from the origin stream into every lane
-        if (hipStreamBeginCapture(s->origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
-        int rc = hipEventRecord(s->fork, s->origin) == hipSuccess ? REFEFF_FMS_OK : REFEFF_FMS_DEVICE_ERROR;
-        for (int l = 0; l < nl && !rc; ++l) {
-            hipStream_t q = s->lane_stream[l];
-            if (hipStreamWaitEvent(q, s->fork, 0) != hipSuccess) {
-                rc = REFEFF_FMS_DEVICE_ERROR;
-                break;
+            if (!bad.empty()) {
+                redo += static_cast<int>(bad.size());
+                std::lock_guard<std::mutex> plock(p->mu);
+                p->energy_err.assign(static_cast<size_t>(energy_counts[c]), refeff_fms_error{});
+                const int r = fp64_fallback(p, bad, d_k[c], d_ph[c], d_scattering[c], st_all.data() + s->eoff[c], true);
+                if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) return r;
             }
-            for (int c : s->lanes[l]) {
-                refeff_fms_plan* p = s->plans[c];
-                WorkspaceSnapshot keep(p);  // the plan's own workspace comes back at the end of the scope
-                size_t off = 0;
-                workspace_layout(p, s->gbatch[c], [&](void** slot, size_t bytes) {
-                    *slot = s->arena[l] + off;
-                    off += (bytes + 255) / 256 * 256;
-                    return true;
-                });
-                p->ws_batch = s->gbatch[c];
-                p->profile = false;
-                const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
-                const size_t sst = static_cast<size_t>(p->ch) * p->ch * p->npot;
-                for (int e0 = 0; e0 < energy_counts[c] && !rc; e0 += s->gbatch[c]) {
-                    const int nb = std::min(s->gbatch[c], energy_counts[c] - e0);
-                    if (hipMemcpyAsync(p->d_k, d_k[c] + 2 * kst * e0, 2 * kst * nb * sizeof(double),
-                                       hipMemcpyDeviceToDevice, q) != hipSuccess ||
-                        hipMemcpyAsync(p->d_ph, d_ph[c] + 2 * pst * e0, 2 * pst * nb * sizeof(double),
-                                       hipMemcpyDeviceToDevice, q) != hipSuccess) {
-                        rc = REFEFF_FMS_DEVICE_ERROR;
-                        break;
-                    }
-                    const GraphCtx gc{q, s->sweeps[c], s->d_info + s->eoff[c] + e0,
-                                      s->d_norms + s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * e0};
-                    rc = solve_device_batch(p, nb, reinterpret_cast<double2*>(d_scattering[c]) + sst * e0, nullptr,
-                                            nullptr, nullptr, 0, nullptr, &gc);
-                }
-                if (rc) break;
-            }
-            if (!rc && (hipEventRecord(s->join[l], q) != hipSuccess || hipStreamWaitEvent(s->origin, s->join[l], 0) != hipSuccess))
-                rc = REFEFF_FMS_DEVICE_ERROR;
-        }
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(s->origin, &g);
-        if (rc || ec != hipSuccess || !g) {
-            if (g) (void)hipGraphDestroy(g);
-            plan_set_drop_graph(s);
-            (void)hipGetLastError();
-            return rc ? rc : REFEFF_FMS_DEVICE_ERROR;
-        }
-        s->graph = g;
-        if (hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0) != hipSuccess) {
-            plan_set_drop_graph(s);
-            (void)hipGetLastError();
-            return REFEFF_FMS_DEVICE_ERROR;
-        }
-        size_t nodes = 0;
-        (void)hipGraphGetNodes(s->graph, nullptr, &nodes);
-        s->graph_nodes = nodes;
-        s->key = key;
-        captured = true;
-    }
-    const double t_capture = now_s() - t0;
-    const double t1 = now_s();
-    if (hipGraphLaunch(s->exec, s->origin) != hipSuccess || hipStreamSynchronize(s->origin) != hipSuccess)
-        return REFEFF_FMS_DEVICE_ERROR;
-    const double t_graph = now_s() - t1;
-    // statuses, and eager re-solves of the energies the fixed sweeps left unsettled
-    std::vector<int> info(std::max<size_t>(1, s->eoff[nc]));
-    std::vector<double> nrm(std::max<size_t>(1, s->norm_off[nc]));
-    if (hipMemcpy(info.data(), s->d_info, s->eoff[nc] * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(nrm.data(), s->d_norms, s->norm_off[nc] * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-        return REFEFF_FMS_DEVICE_ERROR;
-    // statuses; the energies the fixed sweeps left unsettled go, per cluster in
-    // one call, to the plain FP64 LU (the mixed path's own fallback)
-    int rc = REFEFF_FMS_OK, redo = 0;
-    std::vector<int32_t> st_all(std::max<size_t>(1, s->eoff[nc]), 0);
-    for (int c = 0; c < nc; ++c) {
-        refeff_fms_plan* p = s->plans[c];
-        std::vector<int> bad;
-        for (int e = 0; e < energy_counts[c]; ++e) {
-            const int b0 = (e / s->gbatch[c]) * s->gbatch[c], nb = std::min(s->gbatch[c], energy_counts[c] - b0);
-            const double* nb_norms = nrm.data() + s->norm_off[c] + 4 * static_cast<size_t>(s->sweeps[c]) * b0;
-            double est = 0.0;
-            const int stt = graph_energy_status(p, info[s->eoff[c] + e], nb_norms, nb, e - b0, s->sweeps[c], &est);
-            st_all[s->eoff[c] + e] = stt;
-            if (stt == REFEFF_FMS_NO_CONVERGENCE) bad.push_back(e);
-        }
-        if (!bad.empty()) {
-            redo += static_cast<int>(bad.size());
-            std::lock_guard<std::mutex> plock(p->mu);
-            p->energy_err.assign(static_cast<size_t>(energy_counts[c]), refeff_fms_error{});
-            const int r = fp64_fallback(p, bad, d_k[c], d_ph[c], d_scattering[c], st_all.data() + s->eoff[c], true);
-            if (r == REFEFF_FMS_DEVICE_ERROR || r == REFEFF_FMS_NO_MEMORY) return r;
         }
     }
     for (size_t i = 0; i < s->eoff[nc]; ++i) {
@@ -498,7 +570,7 @@ int refeff_fms_plan_set_solve(refeff_fms_plan_set* s, const int32_t* energy_coun
         stats->seconds_total = now_s() - t0;
         stats->seconds_capture = captured ? t_capture : 0.0;
         stats->seconds_graph = t_graph;
-        stats->graph_nodes = static_cast<int64_t>(s->graph_nodes);
+        stats->graph_nodes = s->mode == REFEFF_FMS_SET_GRAPH ? static_cast<int64_t>(s->graph_nodes) : 0;
         stats->captured = captured ? 1 : 0;
         stats->resolved_energies = redo;
         stats->energies = static_cast<int32_t>(s->eoff[nc]);

@@ -189,6 +189,10 @@ class FmsRealSpacePlan:
             lib().refeff_fms_plan_destroy(self._h)
         self._h = C.c_void_p()
 
+    def release_workspace(self):
+        """Give back the batch workspace (device memory); the next solve allocates it again."""
+        check(lib().refeff_fms_plan_release_workspace(self._h), "plan_release_workspace")
+
     def __del__(self):
         try:
             self.close()
@@ -442,6 +446,13 @@ class FmsPlanSet:
             self.close()
         except Exception:
             pass
+
+    GRAPH, EAGER = 0, 1
+
+    def set_mode(self, mode: int):
+        """FmsPlanSet.GRAPH (replay one captured graph) or FmsPlanSet.EAGER (direct launches, one host
+        thread per lane, same arenas)."""
+        check(lib().refeff_fms_plan_set_mode(self._h, int(mode)), "plan_set_mode")
 
     def _arrays(self, ks, phs):
         ks = [np.ascontiguousarray(np.asarray(k, np.complex128).reshape(-1, p.spin_channels)) for k, p in zip(ks, self.plans)]

@@ -25,6 +25,7 @@ def hip():
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
         _hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        _hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     return _hip
 
 
@@ -38,11 +39,20 @@ def device_count() -> int:
     return n.value
 
 
+def mem_info() -> tuple[int, int]:
+    """(free, total) bytes of the current device."""
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    if hip().hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return free.value, total.value
+
+
 class DeviceArray:
     def __init__(self, host: np.ndarray):
         self.nbytes = host.nbytes
         self.ptr = C.c_void_p()
         if hip().hipMalloc(C.byref(self.ptr), max(1, self.nbytes)) != 0:
+            hip().hipGetLastError()  # a failed allocation must not reach the next caller's error check
             raise MemoryError("hipMalloc failed")
         self.upload(host)
 
@@ -52,6 +62,7 @@ class DeviceArray:
         obj.nbytes = nbytes
         obj.ptr = C.c_void_p()
         if hip().hipMalloc(C.byref(obj.ptr), max(1, nbytes)) != 0:
+            hip().hipGetLastError()
             raise MemoryError("hipMalloc failed")
         return obj
 

@@ -15,7 +15,7 @@ import pytest
 from oracle import oracle as orc
 from refeff_amd import fms
 from refeff_amd import workloads as W
-from refeff_amd.hip import DeviceArray
+from refeff_amd.hip import DeviceArray, mem_info
 
 pytestmark = pytest.mark.gpu
 GG_TOL = 1.0e-8
@@ -149,3 +149,76 @@ def test_unsettled_energies_fall_back_to_the_fp64_lu():
     ip0 = int(w.potentials[0])
     for e in range(len(sel)):
         assert rel(res[fms.SOLVER_MIXED][e, ip0], res[fms.SOLVER_FP64][e, ip0]) <= GG_TOL
+
+
+def test_released_workspace_and_shrunk_batches_give_the_same_gg():
+    """A plan gives its workspace back and allocates it again on the next solve;
+    when the device memory its batch was sized for has gone to another holder,
+    the solve goes on with smaller batches.  Both give the first solve's gg."""
+    w = W.rocksalt_mno(3, 32)
+    k, ph = w.wave_numbers(), w.phase_shifts()
+    plan = fms.FmsRealSpacePlan(w.positions, w.potentials, global_lmax=w.global_lmax, raw_potential_lmax=w.raw_lmax,
+                                direct_cutoff=w.rdirec, solver=fms.SOLVER_MIXED, max_batch=16)
+    sc0, st0 = plan.solve(k, ph)
+    b0 = plan.last_stats.batches
+    assert not st0.any() and b0 == 2
+    plan.release_workspace()
+    sc1, st1 = plan.solve(k, ph)
+    assert not st1.any()
+    ip0 = int(w.potentials[0])
+    for e in range(len(k)):
+        assert rel(sc1[e, ip0], sc0[e, ip0]) <= 1.0e-12
+    plan.release_workspace()
+    npad = plan.info.padded_order
+    per = 24 * npad * npad  # the two copies of A per energy, the bulk of the workspace
+    # take the device memory in 1 GiB, then 16 MiB pieces until an allocation
+    # fails, and give back about six energies' worth
+    held = []
+    for piece in (1 << 30, 1 << 24):
+        while True:
+            try:
+                held.append(DeviceArray.empty(piece))
+            except MemoryError:
+                break
+    back = 0
+    while back < 6 * per and held:
+        back += held[-1].nbytes
+        held.pop().free()
+    try:
+        sc2, st2 = plan.solve(k, ph)
+        b2 = plan.last_stats.batches
+    finally:
+        for h in held:
+            h.free()
+    banner(f"seed 3 (N={w.state_count}, padded {npad}): batches {b0} -> {b2} with {back / 2**20:.0f} MiB given back")
+    assert not st2.any() and b2 > b0
+    for e in range(len(k)):
+        assert rel(sc2[e, ip0], sc0[e, ip0]) <= 1.0e-12
+    plan.close()
+
+
+def test_eager_mode_matches_the_graph(mno):
+    """The eager mode (direct launches, one host thread per lane, in the same
+    arenas, host-side stopping rule) gives the graph replay's gg and statuses."""
+    wls = mno
+    ks = [w.wave_numbers() for w in wls]
+    phs = [w.phase_shifts() for w in wls]
+    ps = fms.FmsPlanSet(_descs(wls))
+    res_g, st_g = ps.solve(ks, phs)
+    ps.set_mode(fms.FmsPlanSet.EAGER)
+    res_e, st_e = ps.solve(ks, phs)
+    s = ps.last_stats
+    banner(f"eager mode: {s.seconds_graph:.3f} s, graph nodes {s.graph_nodes}, re-solved {s.resolved_energies}")
+    assert not st_g.any() and not st_e.any() and s.graph_nodes == 0
+    worst = 0.0
+    for w, a, b in zip(wls, res_g, res_e):
+        ip0 = int(w.potentials[0])
+        for e in range(a.shape[0]):
+            worst = max(worst, rel(b[e, ip0], a[e, ip0]))
+    banner(f"eager vs graph: max gg relative difference {worst:.2e}")
+    assert worst <= 1.0e-10
+    ps.set_mode(fms.FmsPlanSet.GRAPH)
+    res_g2, _ = ps.solve(ks, phs)
+    for a, b in zip(res_g, res_g2):
+        assert np.array_equal(a, b)
+    ps.close()
