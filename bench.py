@@ -24,17 +24,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "FMS energy-pts/s at ≤1e-6 xmu rel err; mixed+learned vs plain FP64 LU"
-PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+PEAK_TFLOPS = {"fp16": 2500.0, "fp32": 157.3, "fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
 
 
-def traffic_bytes_per_launch(config, solver):
+def traffic_bytes_per_launch(config, solver, kernel="k_gemm_v2"):
     """HBM bytes per launch of the LU trailing-update kernel, averaged over one
     factorisation's launches: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this
     script, each scaled by the same counter read for a 1 GiB device copy.  Read from
     the committed summaries under profiles/ (a line `bench.traffic config=.. solver=..
-    bytes_per_launch=..`); None where no summary covers the workload."""
-    pat = re.compile(r"^bench\.traffic config=(\S+) solver=(\S+) bytes_per_launch=(\d+)\s*$")
+    [kernel=..] bytes_per_launch=..`, kernel k_gemm_v2 when absent); None where no
+    summary covers the workload and kernel."""
+    pat = re.compile(r"^bench\.traffic config=(\S+) solver=(\S+)(?: kernel=(\S+))? bytes_per_launch=(\d+)\s*$")
     pdir = os.path.join(ROOT, "profiles")
     for name in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
         if not name.endswith((".md", ".txt")):
@@ -42,8 +43,8 @@ def traffic_bytes_per_launch(config, solver):
         with open(os.path.join(pdir, name)) as f:
             for ln in f:
                 m = pat.match(ln)
-                if m and m.group(1) == config and m.group(2) == solver:
-                    return int(m.group(3))
+                if m and m.group(1) == config and m.group(2) == solver and (m.group(3) or "k_gemm_v2") == kernel:
+                    return int(m.group(4))
     return None
 
 
@@ -166,9 +167,11 @@ def run_solver(lib, wl, solver_id, steps, warmup, profile):
     out = np.zeros((ne, npot, ch, ch), np.complex128)
     dout.download(out)
     sc = out.transpose(0, 1, 3, 2).copy()
+    tun = fms.Tuning()
+    check(lib.refeff_fms_plan_get_tuning(plan.handle, C.byref(tun)), "plan_get_tuning")
     res = dict(seconds=t1 - t0, ne=ne, status=status.copy(), scattering=sc, gtr=gtr.copy(), stats=st, profile=prof,
                breakdown=breakdown, n=plan.info.state_count, npad=plan.info.padded_order, batch=plan.info.batch,
-               lu_block=plan.info.lu_block)
+               lu_block=plan.info.lu_block, update=int(tun.trailing_update))
     for d in (dk, dph, dout):
         d.free()
     plan.close()
@@ -717,13 +720,11 @@ def main():
     value = ne * args.steps / main_res["seconds"]
     prof = main_res["profile"]
     gemm = GEMM_CLASS
-    prec = "fp32" if solver_id == 1 else "fp64"
+    upd = main_res["update"]  # 0 FP32, 1 FP32 3M, 4 split fp16 (REFEFF_FMS_UPDATE_*)
+    prec = ("fp16" if upd == 4 else "fp32") if solver_id == 1 else "fp64"
     g_ms = prof.milliseconds[gemm]
     achieved = prof.flops[gemm] / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
-    # the library's default (REFEFF_FMS_GEMM_3M unset or non-zero): 3M complex products
-    flag = os.environ.get("REFEFF_FMS_GEMM_3M")
-    lead = re.match(r"\s*[+-]?\d+", flag or "")
-    gemm_3m = prec == "fp32" and (flag is None or (lead is not None and int(lead.group()) != 0))
+    gemm_3m = prec == "fp32" and upd == 1
     bd = main_res["breakdown"]
     kb = main_res["lu_block"]
     n = main_res["n"]
@@ -734,28 +735,38 @@ def main():
                       if bd.milliseconds[i] > 0 else 0.0}
                for i, name in enumerate(["tmatrix", "build", "panel", "swap_trsm", "gemm", "solve", "residual",
                                          "other", "gemm_small"])}
-    method = ("complex FP32 MFMA LU (3M trailing update) + FP64 iterative refinement, energies batched"
-              if solver_id == 1 else "plain complex FP64 MFMA LU")
+    method = ("complex mixed-precision LU: FP32 panels, " +
+              ("trailing update on the f16 MFMA from two-part fp16 splits with row / column scales (FP32-like)"
+               if upd == 4 else "FP32 MFMA trailing update" + (" (3M)" if gemm_3m else "")) +
+              ", FP64 iterative refinement, energies batched" if solver_id == 1 else "plain complex FP64 MFMA LU")
+    if prec == "fp16":
+        rl_kernel = (f"k_gemm_h4 (LU trailing update, K={kb}; complex product as 4 real products x 3 fp16 split "
+                     "terms = 12 f16 MFMA products, achieved counts the 24MNK flops executed)")
+        cplx = 8.0 / 24.0
+    elif prec == "fp32":
+        rl_kernel = (f"k_gemm_v2<16, 3M> (LU trailing update, K={kb}; 3 real MFMA products per complex product, "
+                     "achieved counts the 6MNK flops executed)" if gemm_3m else f"k_gemm_v2<16> (LU trailing update, K={kb})")
+        cplx = 8.0 / 6.0 if gemm_3m else 1.0
+    else:
+        rl_kernel, cplx = f"k_gemm_sub<double> (LU trailing update, K={kb})", 1.0
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "energy-points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(main_res["seconds"] / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c64 LU + f64 refinement" if solver_id == 1 else "c128",
+        "dtype": (("c64 LU (f16x2 split trailing update)" if upd == 4 else "c64 LU") + " + f64 refinement"
+                  if solver_id == 1 else "c128"),
         "data": "synthetic (seeded lattice cluster, smooth model phase shifts)",
         "config": {"workload": wl.name, "atoms": wl.nat, "states": n, "padded": main_res["npad"], "energies": ne,
                    "solver": args.solver, "method": method, "learned_preconditioner": False,
                    "batch": main_res["batch"], "lu_block": kb,
                    "refinement_sweeps": int(main_res["stats"].refinement_sweeps)},
-        "roofline": {"bound": "mfma",
-                     "kernel": ((f"k_gemm_v2<16, 3M> (LU trailing update, K={kb}; 3 real MFMA products per "
-                                 "complex product, achieved counts the 6MNK flops executed)" if gemm_3m else
-                                 f"k_gemm_v2<16> (LU trailing update, K={kb})") if prec == "fp32" else
-                                f"k_gemm_sub<double> (LU trailing update, K={kb})"),
+        "roofline": {"bound": "mfma", "kernel": rl_kernel,
                      "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_TFLOPS[prec], 4),
-                     "traffic": traffic_bytes_per_launch(args.config, args.solver),
+                     "traffic": traffic_bytes_per_launch(args.config, args.solver,
+                                                         "k_gemm_h4" if prec == "fp16" else "k_gemm_v2"),
                      "launches": int(prof.launches[gemm]), "avg_launch_ms": round(g_ms / max(1, prof.launches[gemm]), 4),
-                     "complex_equiv_tflops": round(achieved * (8.0 / 6.0 if gemm_3m else 1.0), 3)},
+                     "complex_equiv_tflops": round(achieved * cplx, 3)},
         "kernels_one_profiled_sweep": kernels,
     }
     if not args.no_fp64_compare and solver_id == 1:

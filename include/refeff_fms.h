@@ -308,13 +308,20 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, con
 /* No counterpart in refeff: faer picks its own blocking
  * (crates/refeff-linalg/src/lu.rs:385-397).  The LU variant of a plan:
  * outer block width (K of the trailing MFMA update), the row count from which
- * FP32 panels use tournament pivoting, the 3M complex product, and the most
- * refinement sweeps a plan set's graph launches (mixed solver; energies that
- * have settled skip the rest). */
+ * FP32 panels use tournament pivoting, the trailing-update kernel of the
+ * mixed solver (REFEFF_FMS_UPDATE_*), and the most refinement sweeps a plan
+ * set's graph launches (mixed solver; energies that have settled skip the
+ * rest).  Mixed plans of padded order >= 1536 start with the split-fp16
+ * update, smaller ones with the FP32 3M update. */
+enum {
+    REFEFF_FMS_UPDATE_FP32 = 0,      /* FP32 MFMA, four real products per complex product */
+    REFEFF_FMS_UPDATE_FP32_3M = 1,   /* FP32 MFMA, three real products (3M) */
+    REFEFF_FMS_UPDATE_FP16X2 = 4     /* f16 MFMA on two-part fp16 splits with row / column scales (FP32-like) */
+};
 typedef struct refeff_fms_tuning {
     int32_t lu_block;
     int32_t calu_min_rows;
-    int32_t gemm_3m;
+    int32_t trailing_update;
     int32_t graph_sweeps;
     double seconds_per_energy;  /* measured by refeff_fms_plan_tune (0 otherwise) */
 } refeff_fms_tuning;

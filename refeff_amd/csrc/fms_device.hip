@@ -2004,6 +2004,282 @@ __global__ __launch_bounds__(512, 1) void k_gemm_h2(float2* Cb, int ldc, size_t 
             }
 }
 
+// ---------------------------------------------------------------------------
+// fp16 two-part split, four-product (4M) form of the trailing update
+// C -= A B (A M x K, B K x N, complex).  Each operand is stored as the four
+// fp16 planes (re, im) x (h, m), in the order a K step of a tile reads them:
+//   PA[kb][plane][kc][row][8], PB[kb][plane][kc][col][8]
+//   (k = 32 kb + 8 kc + j; plane = 2 c + part, c: 0 re, 1 im; part: 0 h, 1 m),
+// then the row scales sA[M] and column scales sB[N] (the power of two just
+// above the row's / column's largest |re|, |im|; x = s (h + m) to about 2^-22
+// of that maximum).  Re C -= Ar Br - Ai Bi and Im C -= Ar Bi + Ai Br, every
+// real product by the three terms h h', h m', m h': twelve f16 MFMAs per
+// 16x16x32 tile against the 3M form's nine, but 8 bytes of operand per complex
+// value instead of 12, and operand traffic is what bounds these kernels.
+struct HSplit4 {
+    uint16_t* base;  // [e] sS
+    size_t sS;
+    int M, N, K;
+    __device__ uint16_t* pa(int e) const { return base + e * sS; }
+    __device__ uint16_t* pb(int e) const { return base + e * sS + static_cast<size_t>(4) * M * K; }
+    __device__ float* sa(int e) const {
+        return reinterpret_cast<float*>(base + e * sS + static_cast<size_t>(4) * (M + N) * K);
+    }
+    __device__ float* sb(int e) const { return sa(e) + M; }
+};
+
+__device__ __forceinline__ size_t h4_index(int k, int plane, int rows, int row) {
+    return ((static_cast<size_t>(k >> 5) * 4 + plane) * 4 + ((k >> 3) & 3)) * static_cast<size_t>(rows) * 8 +
+           static_cast<size_t>(row) * 8 + (k & 7);
+}
+
+// A panel (M x K row-major complex, lda) -> planes and row scales, one wave per row
+__global__ __launch_bounds__(256) void k_h4split_rows(const float2* Ab, int lda, size_t sA, HSplit4 h) {
+    const int e = blockIdx.y, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= h.M) return;
+    const float2* a = Ab + e * sA + static_cast<size_t>(row) * lda;
+    float mx = 0.0f;
+    for (int k = lane; k < h.K; k += 64) {
+        const float2 v = a[k];
+        mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    float inv;
+    const float s = pow2_above(mx, &inv);
+    uint16_t* P = h.pa(e);
+    for (int k = lane; k < h.K; k += 64) {
+        const float2 v = a[k];
+        const float x[2] = {v.x * inv, v.y * inv};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const uint16_t hi = f2h(x[c]);
+            P[h4_index(k, 2 * c, h.M, row)] = hi;
+            P[h4_index(k, 2 * c + 1, h.M, row)] = f2h(x[c] - h2f(hi));
+        }
+    }
+    if (lane == 0) h.sa(e)[row] = s;
+}
+
+// B panel (K x N row-major complex, ldb) -> planes and column scales, one
+// thread per column, 8 k values (16 bytes) per plane and store
+__global__ __launch_bounds__(256) void k_h4split_cols(const float2* Bb, int ldb, size_t sB, HSplit4 h) {
+    const int e = blockIdx.y;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= h.N) return;
+    const float2* b = Bb + e * sB + col;
+    float mx = 0.0f;
+    for (int k = 0; k < h.K; ++k) {
+        const float2 v = b[static_cast<size_t>(k) * ldb];
+        mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    float inv;
+    const float s = pow2_above(mx, &inv);
+    uint16_t* P = h.pb(e);
+    for (int k0 = 0; k0 < h.K; k0 += 8) {
+        uint32_t w[4][4];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            uint16_t part[2][4];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const float2 v = b[static_cast<size_t>(k0 + u + t) * ldb];
+                const float x[2] = {v.x * inv, v.y * inv};
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    part[t][2 * c] = f2h(x[c]);
+                    part[t][2 * c + 1] = f2h(x[c] - h2f(part[t][2 * c]));
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p) w[p][u / 2] = part[0][p] | (static_cast<uint32_t>(part[1][p]) << 16);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            *reinterpret_cast<uint4*>(P + h4_index(k0, p, h.N, col)) = uint4{w[p][0], w[p][1], w[p][2], w[p][3]};
+    }
+    h.sb(e)[col] = s;
+}
+
+__device__ __forceinline__ v8h neg_h8(v8h x) {
+    uint4 u = __builtin_bit_cast(uint4, x);
+    u.x ^= 0x80008000u;
+    u.y ^= 0x80008000u;
+    u.z ^= 0x80008000u;
+    u.w ^= 0x80008000u;
+    return __builtin_bit_cast(v8h, u);
+}
+
+// Tile of an energy's M x N grid in groups of 4 tile rows walked column by
+// column, so the workgroups an XCD runs at once share a few A row panels and
+// B column panels in its L2.
+__device__ __forceinline__ void grouped_tile(int tile, int tilesM, int tilesN, int& tm, int& tn) {
+    constexpr int GM = 4;
+    const int group = tile / (GM * tilesN), idx = tile % (GM * tilesN);
+    const int gm = min(GM, tilesM - group * GM);
+    tm = group * GM + idx % gm;
+    tn = idx / gm;
+}
+
+// C -= s_row s_col (Re, Im) for a wave's 64 x 32 block of 4 x 2 MFMA tiles
+// (row r0 + 16 i + 4 (lane / 16) + q, column c0 + 16 j + lane % 16).  Every
+// scale and C value is loaded first, from clamped addresses, so the loads
+// share one latency; the stores skip rows / columns past M / N.
+__device__ __forceinline__ void fp16_epilogue(float2* C, int ldc, int M, int N, int r0, int c0, int lane,
+                                              const float* sa, const float* sb, const v4f (&cr)[4][2],
+                                              const v4f (&ci)[4][2]) {
+    float srow[4][4], scol[2];
+    float2 cv[4][2][4];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) scol[j] = sb[min(c0 + j * 16 + (lane & 15), N - 1)];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) srow[i][q] = sa[min(r0 + i * 16 + 4 * (lane >> 4) + q, M - 1)];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = min(r0 + i * 16 + 4 * (lane >> 4) + q, M - 1);
+                const int col = min(c0 + j * 16 + (lane & 15), N - 1);
+                cv[i][j][q] = C[static_cast<size_t>(row) * ldc + col];
+            }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = r0 + i * 16 + 4 * (lane >> 4) + q;
+                const int col = c0 + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    const float sc = srow[i][q] * scol[j];
+                    C[static_cast<size_t>(row) * ldc + col] =
+                        float2{cv[i][j][q].x - sc * cr[i][j][q], cv[i][j][q].y - sc * ci[i][j][q]};
+                }
+            }
+}
+
+// C (M x N, row-major complex) -= A B from the 4M planes: 128 x 128 tile per
+// 8-wave workgroup (2 x 4 waves of 64 x 32, 4 x 2 MFMA tiles of 16x16x32).
+// K in steps of 32 through two LDS stages ([operand][plane][kc][128 rows] of
+// 16-byte chunks: a wave's 16-lane read of 16 rows and its 64-lane store of 64
+// rows are both contiguous, so neither meets a bank conflict) and two register
+// sets: the global loads of step s + 2 are issued while step s multiplies, one
+// barrier per step.  -Ai is made in registers by
+// flipping sign bits.  Rows / columns past M / N are not written; K a
+// multiple of 32.
+__global__ __launch_bounds__(512, 1) void k_gemm_h4(float2* Cb, int ldc, size_t sC, HSplit4 h) {
+    constexpr int SLAB = 128;                   // uint4 per (plane, kc) slab
+    constexpr int OPER = 4 * 4 * SLAB;          // uint4 per operand and stage
+    __shared__ uint4 lds[2 * 2 * OPER];         // [stage][operand] = 128 KiB
+    const int M = h.M, N = h.N, K = h.K;
+    const int tilesM = (M + 127) / 128, tilesN = (N + 127) / 128;
+    int tile, e, tm, tn;
+    xcd_tile(tilesM * tilesN, tile, e);
+    grouped_tile(tile, tilesM, tilesN, tm, tn);
+    const int m0 = tm * 128, n0 = tn * 128;
+    const uint16_t* PA = h.pa(e);
+    const uint16_t* PB = h.pb(e);
+    float2* C = Cb + e * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    // this thread's chunk of each plane: kc = tid / 128, row = tid % 128
+    const int lkc = tid >> 7, lr = tid & 127;
+    const bool va = m0 + lr < M, vb = n0 + lr < N;
+    const size_t offa = static_cast<size_t>(lkc) * M + (va ? m0 + lr : 0);
+    const size_t offb = static_cast<size_t>(lkc) * N + (vb ? n0 + lr : 0);
+    // rows / columns past M / N load row / column 0 instead: they only reach
+    // outputs that are not written (no select on the loaded value, which would
+    // make the load wait at once)
+    struct Pref {
+        uint4 a0, a1, a2, a3, b0, b1, b2, b3;
+    };
+    auto gload = [&](int kb) -> Pref {
+        const size_t pl = static_cast<size_t>(kb) * 16;
+        const uint4* ga = reinterpret_cast<const uint4*>(PA) + pl * M + offa;
+        const uint4* gb = reinterpret_cast<const uint4*>(PB) + pl * N + offb;
+        const size_t sa4 = static_cast<size_t>(4) * M, sb4 = static_cast<size_t>(4) * N;
+        return Pref{ga[0], ga[sa4], ga[2 * sa4], ga[3 * sa4], gb[0], gb[sb4], gb[2 * sb4], gb[3 * sb4]};
+    };
+    auto sstore = [&](int stage, const Pref& r) {
+        uint4* la = lds + stage * 2 * OPER + lkc * SLAB + lr;
+        la[0 * 4 * SLAB] = r.a0;
+        la[1 * 4 * SLAB] = r.a1;
+        la[2 * 4 * SLAB] = r.a2;
+        la[3 * 4 * SLAB] = r.a3;
+        la[OPER + 0 * 4 * SLAB] = r.b0;
+        la[OPER + 1 * 4 * SLAB] = r.b1;
+        la[OPER + 2 * 4 * SLAB] = r.b2;
+        la[OPER + 3 * 4 * SLAB] = r.b3;
+    };
+    v4f cr[4][2], ci[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            cr[i][j] = v4f{0, 0, 0, 0};
+            ci[i][j] = v4f{0, 0, 0, 0};
+        }
+    const int orow = lane & 15, okc = lane >> 4;
+    auto compute = [&](int stage) {
+        const uint4* la = lds + stage * 2 * OPER;
+        const uint4* lb = la + OPER;
+        v8h b[4][2];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[p][j] = __builtin_bit_cast(v8h, lb[(p * 4 + okc) * SLAB + wc * 32 + j * 16 + orow]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            v8h a[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) a[p] = __builtin_bit_cast(v8h, la[(p * 4 + okc) * SLAB + wr * 64 + i * 16 + orow]);
+            const v8h nah = neg_h8(a[2]), nam = neg_h8(a[3]);
+            // (A operand, B plane) of the six terms of Re and of Im
+            const v8h ar[6] = {a[0], a[0], a[1], nah, nah, nam};
+            constexpr int br[6] = {0, 1, 0, 2, 3, 2};
+            const v8h ai[6] = {a[0], a[0], a[1], a[2], a[2], a[3]};
+            constexpr int bi[6] = {2, 3, 2, 0, 1, 0};
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar[t], b[br[t]][j], cr[i][j], 0, 0, 0);
+                    ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[t], b[bi[t]][j], ci[i][j], 0, 0, 0);
+                }
+        }
+    };
+    const int nk = K / 32;
+    Pref r0 = gload(0), r1{};
+    if (nk > 1) r1 = gload(1);
+    sstore(0, r0);
+    if (nk > 2) r0 = gload(2);
+    __syncthreads();
+    int kb = 0;
+    for (; kb + 2 < nk; kb += 2) {
+        sstore(1, r1);  // step kb + 1
+        if (kb + 3 < nk) r1 = gload(kb + 3);
+        compute(0);  // step kb
+        __syncthreads();
+        sstore(0, r0);  // step kb + 2
+        if (kb + 4 < nk) r0 = gload(kb + 4);
+        compute(1);  // step kb + 1
+        __syncthreads();
+    }
+    if (kb + 1 < nk) {  // steps kb (stage 0) and kb + 1 (registers r1) left
+        sstore(1, r1);
+        compute(0);
+        __syncthreads();
+        compute(1);
+    } else {  // step kb (stage 0) left
+        compute(0);
+    }
+    fp16_epilogue(C, ldc, M, N, m0 + wr * 64, n0 + wc * 32, lane, h.sa(e), h.sb(e), cr, ci);
+}
 
 // final row permutation of the LU: perm[i] = original row now at position i
 __global__ __launch_bounds__(256) void k_perm(const int* ipivb, int* permb, int n) {
@@ -3293,7 +3569,7 @@ struct refeff_fms_plan {
     bool trace = false;                 // REFEFF_FMS_TRACE: print refinement progress to stderr
     bool gemm_3m = true;                // REFEFF_FMS_GEMM_3M=0: FP32 trailing updates with four real products
     bool gemm_cpf = true;               // REFEFF_FMS_GEMM_CPF=0: C tile read in the epilogue
-    int gemm_bf16 = 0;                  // REFEFF_FMS_GEMM_BF16=2|3: trailing update on split-bf16 MFMA
+    int gemm_bf16 = 0;                  // trailing update on split operands: 4 fp16x2 4M (REFEFF_FMS_GEMM_BF16=0|2|3|4|16)
     uint16_t* d_split = nullptr;        // its operand planes, [e] split_stride bf16
     size_t split_stride = 0;
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -3593,6 +3869,13 @@ void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, typename Cx<T>::typ
 // k_gemm_bf.  M, N multiples of 64, K of 32, 16-byte aligned A rows.
 void gemm_bf_launch(hipStream_t s, int ns, float2* C, int ldc, size_t sC, const float2* A, int lda, size_t sA,
                     const float2* B, int ldb, size_t sB, int M, int N, int K, int nb, uint16_t* split, size_t ssplit) {
+    if (ns == 4) {  // fp16 two-part split, four-product form
+        const HSplit4 hs{split, ssplit, M, N, K};
+        hipLaunchKernelGGL(k_h4split_rows, dim3((M + 3) / 4, nb), dim3(256), 0, s, A, lda, sA, hs);
+        hipLaunchKernelGGL(k_h4split_cols, dim3((N + 255) / 256, nb), dim3(256), 0, s, B, ldb, sB, hs);
+        hipLaunchKernelGGL(k_gemm_h4, dim3(((M + 127) / 128) * ((N + 127) / 128), nb), dim3(512), 0, s, C, ldc, sC, hs);
+        return;
+    }
     if (ns == 16) {  // fp16 two-part split with row / column scales
         const HSplit hs{split, ssplit, M, N, K};
         hipLaunchKernelGGL(k_hsplit_rows, dim3((M + 3) / 4, nb), dim3(256), 0, s, A, lda, sA, hs);
@@ -3633,8 +3916,9 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
                     K % 32 == 0 && K <= p->outer_block && M <= p->npad && N <= p->npad && lda % 2 == 0 &&
                     reinterpret_cast<uintptr_t>(A) % 16 == 0;
     if (bf) {
-        const int terms = p->gemm_bf16 == 3 ? 6 : 3;
-        launch(p, g.s, REFEFF_FMS_K_GEMM, 3.0 * terms * 2.0 * M * N * K * nb,
+        // real products x terms as executed: 3 x 3 (fp16x2, 3M), 4 x 3 (fp16x2, 4M), 3 x 3 / 3 x 6 (bf16x2 / x3)
+        const int terms = p->gemm_bf16 == 3 ? 18 : p->gemm_bf16 == 4 ? 12 : 9;
+        launch(p, g.s, REFEFF_FMS_K_GEMM, terms * 2.0 * M * N * K * nb,
                (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
                    gemm_bf_launch(g.s, p->gemm_bf16, reinterpret_cast<float2*>(C), ldc, sC,
                                   reinterpret_cast<const float2*>(A), lda, sA, reinterpret_cast<const float2*>(B), ldb,
@@ -4733,9 +5017,12 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     p->trace = std::getenv("REFEFF_FMS_TRACE") != nullptr;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_3M")) p->gemm_3m = std::atoi(e) != 0;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_CPF")) p->gemm_cpf = std::atoi(e) != 0;
+    // the split-fp16 trailing update once the trailing updates dominate (the
+    // FP32 3M update below); REFEFF_FMS_GEMM_BF16 overrides
+    if (p->solver == REFEFF_FMS_SOLVER_MIXED && p->npad >= 1536) p->gemm_bf16 = 4;
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_BF16")) {
         const int v = std::atoi(e);
-        p->gemm_bf16 = (v == 2 || v == 3 || v == 16) ? v : 0;  // bf16x2, bf16x3, fp16x2
+        p->gemm_bf16 = (v == 2 || v == 3 || v == 4 || v == 16) ? v : 0;  // bf16x2, bf16x3, fp16x2 4M, fp16x2 3M
     }
     if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p->panel_calu = std::strcmp(e, "gepp") != 0;
     if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
@@ -5254,7 +5541,7 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
     const int kmul = (use_double == 0 || use_double == 3 || use_double == 5 || use_double >= 12) ? 32 : 16;
-    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 14 ||
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 15 ||
         use_double == 2 || (use_double >= 9 && use_double <= 11))
         return REFEFF_FMS_INVALID_ARGUMENT;
     if (use_double >= 12 && (M % 64 || N % 64)) return REFEFF_FMS_INVALID_ARGUMENT;
@@ -5276,8 +5563,8 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
             rc = REFEFF_FMS_DEVICE_ERROR;
         } else {
             if (use_double >= 12 && sizeof(T) == 4) {
-                // kinds 12 / 13: split-bf16 trailing update with 2 / 3 parts
-                const int ns = use_double == 12 ? 2 : use_double == 13 ? 3 : 16;
+                // kinds 12 / 13: split-bf16 trailing update with 2 / 3 parts; 14 / 15: fp16x2, 3M / 4M
+                const int ns = use_double == 12 ? 2 : use_double == 13 ? 3 : use_double == 14 ? 16 : 4;
                 uint16_t* sp = nullptr;
                 const size_t ss = static_cast<size_t>(3) * (ns == 3 ? 3 : 2) * (M + N) * K + 2 * static_cast<size_t>(M + N);
                 if (!dalloc(&sp, ss)) {
@@ -5354,11 +5641,24 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
         hipLaunchKernelGGL(k_fill_hash, fg, dim3(256), 0, nullptr, static_cast<float*>(db), nfl, 2u);
         hipLaunchKernelGGL(k_fill_hash, fg, dim3(256), 0, nullptr, static_cast<float*>(dc), nfl, 3u);
     }
+    // kernels 16 / 17: the fp16x2 trailing update, 3M / 4M, split kernels included
+    uint16_t* split = nullptr;
+    const size_t ss = (static_cast<size_t>(6) * (M + N) * K + 2 * static_cast<size_t>(M + N) + 7) / 8 * 8;
+    if ((kernel == 16 || kernel == 17) && hipMalloc(&split, ss * batch * sizeof(uint16_t)) != hipSuccess) {
+        (void)hipFree(da);
+        (void)hipFree(db);
+        (void)hipFree(dc);
+        return REFEFF_FMS_NO_MEMORY;
+    }
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     auto one = [&]() {
-        if (kernel >= 3 && kernel <= 8) {
+        if (kernel == 16 || kernel == 17) {
+            gemm_bf_launch(nullptr, kernel == 16 ? 16 : 4, static_cast<float2*>(dc), ld, per,
+                           static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K,
+                           batch, split, ss);
+        } else if (kernel >= 3 && kernel <= 8) {
             const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
             auto* c2 = static_cast<float2*>(dc);
             auto* a2 = static_cast<const float2*>(da);
@@ -5400,6 +5700,7 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     (void)hipFree(da);
     (void)hipFree(db);
     (void)hipFree(dc);
+    if (split) (void)hipFree(split);
     return rc;
 }
 

@@ -24,13 +24,19 @@ namespace {
 
 bool tuning_valid(const refeff_fms_tuning* t, int npad) {
     return t && t->lu_block >= kPanel && t->lu_block % kPanel == 0 && t->lu_block <= 1024 && t->lu_block <= npad &&
-           t->calu_min_rows >= 0 && t->graph_sweeps >= 1 && t->graph_sweeps <= 16;
+           t->calu_min_rows >= 0 && t->graph_sweeps >= 1 && t->graph_sweeps <= 16 &&
+           (t->trailing_update == REFEFF_FMS_UPDATE_FP32 || t->trailing_update == REFEFF_FMS_UPDATE_FP32_3M ||
+            t->trailing_update == REFEFF_FMS_UPDATE_FP16X2);
 }
 
 void apply_tuning(refeff_fms_plan* p, const refeff_fms_tuning& t) {
+    const int bf = t.trailing_update == REFEFF_FMS_UPDATE_FP16X2 ? 4 : 0;
+    // the workspace layout depends on the outer block and the split planes
+    if (bf != p->gemm_bf16 || t.lu_block != p->outer_block) free_workspace(p);
     p->outer_block = t.lu_block;
     p->calu_min_rows = t.calu_min_rows;
-    p->gemm_3m = t.gemm_3m != 0;
+    p->gemm_3m = t.trailing_update != REFEFF_FMS_UPDATE_FP32;  // also the in-block (K = 32) updates
+    p->gemm_bf16 = bf;
     p->graph_sweeps = t.graph_sweeps;
 }
 
@@ -38,7 +44,9 @@ refeff_fms_tuning current_tuning(const refeff_fms_plan* p) {
     refeff_fms_tuning t{};
     t.lu_block = p->outer_block;
     t.calu_min_rows = p->calu_min_rows;
-    t.gemm_3m = p->gemm_3m ? 1 : 0;
+    t.trailing_update = p->gemm_bf16 == 4 ? REFEFF_FMS_UPDATE_FP16X2
+                        : p->gemm_3m      ? REFEFF_FMS_UPDATE_FP32_3M
+                                          : REFEFF_FMS_UPDATE_FP32;
     t.graph_sweeps = p->graph_sweeps;
     return t;
 }
@@ -56,7 +64,7 @@ std::vector<TuneEntry> read_tune_cache(const std::string& path) {
     char arch[32];
     TuneEntry e{};
     while (std::fscanf(f, "%31s %d %d %d %d %d %d %d %lf", arch, &e.npad, &e.nrhs, &e.solver, &e.t.lu_block,
-                       &e.t.calu_min_rows, &e.t.gemm_3m, &e.t.graph_sweeps, &e.t.seconds_per_energy) == 9)
+                       &e.t.calu_min_rows, &e.t.trailing_update, &e.t.graph_sweeps, &e.t.seconds_per_energy) == 9)
         if (std::strcmp(arch, "gfx950") == 0) out.push_back(e);
     std::fclose(f);
     return out;
@@ -69,7 +77,7 @@ bool write_tune_cache(const std::string& path, const std::vector<TuneEntry>& es)
     if (!f) return false;
     for (const auto& e : es)
         std::fprintf(f, "gfx950 %d %d %d %d %d %d %d %.6e\n", e.npad, e.nrhs, e.solver, e.t.lu_block, e.t.calu_min_rows,
-                     e.t.gemm_3m, e.t.graph_sweeps, e.t.seconds_per_energy);
+                     e.t.trailing_update, e.t.graph_sweeps, e.t.seconds_per_energy);
     const bool ok = std::fclose(f) == 0;
     return ok && std::rename(tmp.c_str(), path.c_str()) == 0;
 }
@@ -354,18 +362,22 @@ int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const 
     std::vector<double> sc(2 * static_cast<size_t>(nt) * p->ch * p->ch * p->npot);
     std::vector<int32_t> st(nt);
     std::vector<refeff_fms_tuning> cands;
-    for (int lb : {64, 128, 192, 256, 320})
-        for (int calu : {1024, 3072, 1 << 30}) {
-            if (lb > p->npad || (calu != 3072 && p->solver != REFEFF_FMS_SOLVER_MIXED)) continue;
-            if (calu == 1024 && p->npad < 1024) continue;  // same as 3072 below 1024 rows
-            if (calu == 3072 && p->npad < 3072 && p->solver == REFEFF_FMS_SOLVER_MIXED) continue;  // same as never
-            refeff_fms_tuning t{};
-            t.lu_block = lb;
-            t.calu_min_rows = calu;
-            t.gemm_3m = 1;
-            t.graph_sweeps = 8;
-            cands.push_back(t);
-        }
+    const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
+    for (int upd : {REFEFF_FMS_UPDATE_FP32_3M, REFEFF_FMS_UPDATE_FP16X2})
+        for (int lb : {64, 128, 192, 256, 320, 384, 512})
+            for (int calu : {1024, 3072, 1 << 30}) {
+                if (upd == REFEFF_FMS_UPDATE_FP16X2 && (!mixed || p->npad < 512)) continue;
+                if (lb > 320 && upd != REFEFF_FMS_UPDATE_FP16X2) continue;  // wide blocks pay off on f16 only
+                if (lb > p->npad || (calu != 3072 && !mixed)) continue;
+                if (calu == 1024 && p->npad < 1024) continue;         // same as 3072 below 1024 rows
+                if (calu == 3072 && p->npad < 3072 && mixed) continue;  // same as never
+                refeff_fms_tuning t{};
+                t.lu_block = lb;
+                t.calu_min_rows = calu;
+                t.trailing_update = upd;
+                t.graph_sweeps = 8;
+                cands.push_back(t);
+            }
     const refeff_fms_tuning keep = current_tuning(p);
     refeff_fms_tuning win = keep;
     double best_s = 1.0e300;

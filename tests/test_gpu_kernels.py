@@ -44,14 +44,36 @@ def test_gemm_v2(kind, m, n, k):
     _gemm_case(kind, m, n, k)
 
 
-@pytest.mark.parametrize("kind,tol", [(12, 6e-5), (13, 2e-5), (14, 2e-5)])
+@pytest.mark.parametrize("kind,tol", [(12, 6e-5), (13, 2e-5), (14, 2e-5), (15, 2e-5)])
 @pytest.mark.parametrize("m,n,k", [(64, 64, 32), (256, 256, 128), (320, 448, 64), (1024, 512, 256), (192, 640, 96)])
 def test_gemm_split_bf16(kind, tol, m, n, k):
     """Trailing update on the bf16 MFMA from split operands: two parts (kind 12,
     relative error of order 2^-17 per product) and three parts (kind 13, FP32-like);
     on the f16 MFMA from two parts with power-of-two row / column scales (kind 14,
-    FP32-like, 128 x 128 tiles: the 64-multiple shapes leave partial tiles)."""
+    3M form, and kind 15, four-product form: FP32-like, 128 x 128 tiles, so the
+    64-multiple shapes leave partial tiles)."""
     _gemm_case(kind, m, n, k, tol)
+
+
+@pytest.mark.parametrize("kind", [14, 15])
+def test_gemm_split_fp16_keeps_scaled_rows_and_columns(kind):
+    """Rows of A and columns of B scaled over six decades: with a power-of-two
+    scale per row and per column every element of C - A B keeps an FP32-like
+    error relative to its own row and column scale, not just to the largest."""
+    m, n, k = 256, 320, 256
+    print(f"[gpu-kernels] gemm kind={kind} scaled rows / columns {m}x{n}x{k}", flush=True)
+    rng = np.random.default_rng(7)
+    r = 10.0 ** rng.uniform(-3, 3, m)
+    cs = 10.0 ** rng.uniform(-3, 3, n)
+    a, b = crand(rng, m, k) * r[:, None], crand(rng, k, n) * cs[None, :]
+    c = crand(rng, m, n) * r[:, None] * cs[None, :]
+    ref = c - a @ b
+    out = np.ascontiguousarray(c.copy())
+    rc = _lib().refeff_fms_debug_gemm(kind, m, n, k, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)), _p(out))
+    assert rc == 0
+    err = np.abs(out - ref) / (r[:, None] * cs[None, :])
+    scale = np.abs(ref / (r[:, None] * cs[None, :])).max()
+    assert err.max() <= 2e-5 * scale, err.max() / scale
 
 
 def _gemm_case(use_double, m, n, k, tol=None):
