@@ -3678,6 +3678,7 @@ struct Group {
     // events ordering it (null: no lookahead)
     hipStream_t s2 = nullptr;
     hipEvent_t ev_u12 = nullptr, ev_far = nullptr;
+    int split_slot = 0;  // split-operand slot of its trailing updates (1: the far update's)
 };
 
 template <typename F>
@@ -3714,6 +3715,20 @@ bool krylov_solver(int s) {
     return s == REFEFF_FMS_SOLVER_GGRM || s == REFEFF_FMS_SOLVER_GGGM || s == REFEFF_FMS_SOLVER_GGTF;
 }
 
+// Split-operand planes of one trailing update per energy (halves, a multiple
+// of 8 so every slot starts 16-byte aligned): the four-product fp16 form
+// holds 4 (M + N) K planes plus M + N float scales, the bf16 / 3M forms their
+// own plane counts.  With lookahead the near and far updates of a block run
+// at once on two streams, so each energy gets two slots.
+size_t split_slot_halves(const refeff_fms_plan* p) {
+    const size_t nk = static_cast<size_t>(p->npad) * p->outer_block;
+    const size_t h = p->gemm_bf16 == 4 ? 8 * nk + 4 * static_cast<size_t>(p->npad)
+                                       : static_cast<size_t>(3) * (p->gemm_bf16 == 3 ? 3 : 2) * 2 * nk +
+                                             4 * static_cast<size_t>(p->npad);
+    return (h + 7) / 8 * 8;
+}
+int split_slots(const refeff_fms_plan* p) { return p->lookahead ? 2 : 1; }
+
 size_t bytes_per_energy(const refeff_fms_plan* p) {
     const size_t n2 = static_cast<size_t>(p->npad) * p->npad;
     const size_t nr = static_cast<size_t>(p->npad) * p->nrhs;
@@ -3725,9 +3740,7 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
     b += 2 * p->npad * sizeof(int) + 2 * p->n * sizeof(double2);
     b += static_cast<size_t>(p->ch) * p->ch * p->npot * sizeof(double2);
     if (krylov_solver(p->solver)) b += static_cast<size_t>(kIterVecs) * nr * sizeof(double2);
-    if (mixed && p->gemm_bf16)
-        b += (static_cast<size_t>(3) * (p->gemm_bf16 == 3 ? 3 : 2) * 2 * p->npad * p->outer_block +
-              4 * static_cast<size_t>(p->npad)) * sizeof(uint16_t);
+    if (mixed && p->gemm_bf16) b += split_slots(p) * split_slot_halves(p) * sizeof(uint16_t);
     if (p->bicg)
         b += static_cast<size_t>(p->bicg_vecs) * nr * sizeof(float2) +
              static_cast<size_t>(p->npad / kPanel) * p->nrhs * (sizeof(double2) + sizeof(double)) +
@@ -3755,8 +3768,7 @@ bool workspace_layout(refeff_fms_plan* p, int nb, Take&& take) {
         ok = ok && t(&p->d_cand, 2 * static_cast<size_t>(p->calu_cap) * nb * sizeof(int));
     }
     if (mixed && p->gemm_bf16) {
-        p->split_stride = static_cast<size_t>(3) * (p->gemm_bf16 == 3 ? 3 : 2) * 2 * p->npad * p->outer_block +
-                          4 * static_cast<size_t>(p->npad);
+        p->split_stride = split_slots(p) * split_slot_halves(p);
         ok = ok && t(&p->d_split, p->split_stride * nb * sizeof(uint16_t));
     }
     if (krylov_solver(p->solver))
@@ -3802,6 +3814,7 @@ size_t carved_bytes(refeff_fms_plan* p, int nb) {
     probe_fields.npad = p->npad; probe_fields.n = p->n; probe_fields.nrhs = p->nrhs; probe_fields.nsp = p->nsp;
     probe_fields.L = p->L; probe_fields.npot = p->npot; probe_fields.ch = p->ch; probe_fields.solver = p->solver;
     probe_fields.gemm_bf16 = p->gemm_bf16; probe_fields.outer_block = p->outer_block; probe_fields.bicg = p->bicg;
+    probe_fields.lookahead = p->lookahead;
     workspace_layout(&probe_fields, nb, [&](void** slot, size_t bytes) {
         *slot = nullptr;
         off += (bytes + 255) / 256 * 256;
@@ -3813,16 +3826,19 @@ size_t carved_bytes(refeff_fms_plan* p, int nb) {
 // Register-blocked panel when the panel's rows fit the register tiles without
 // spilling (sub-panel width shrinking as the row count grows); the streaming
 // kernel otherwise.
+// e0: the group's first energy in the batch; each energy has its own
+// tournament-pivoting candidate lists (groups on other streams use theirs).
 template <typename T>
-void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, typename Cx<T>::type* A, size_t sa, int lda, int* ipiv,
-                  int* info, int j0, int n) {
+void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, int e0, typename Cx<T>::type* A, size_t sa, int lda,
+                  int* ipiv, int* info, int j0, int n) {
     const int m = n - j0;
     const dim3 grid(nb), block(1024);
     if constexpr (sizeof(T) == 4) {
         if (p->panel_calu && p->d_cand && m >= p->calu_min_rows) {
             // selection levels until <= 512 candidates remain, then the final pivoting
             const int cap = p->calu_cap;
-            int* buf[2] = {p->d_cand, p->d_cand + static_cast<size_t>(cap) * nb};
+            const size_t cb = static_cast<size_t>(std::max(1, p->ws_batch));  // energies per list
+            int* buf[2] = {p->d_cand + static_cast<size_t>(cap) * e0, p->d_cand + static_cast<size_t>(cap) * (cb + e0)};
             const int* in = nullptr;
             int count = m, cur = 0;
             while (count > kSelRows) {
@@ -3922,7 +3938,9 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
                (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
                    gemm_bf_launch(g.s, p->gemm_bf16, reinterpret_cast<float2*>(C), ldc, sC,
                                   reinterpret_cast<const float2*>(A), lda, sA, reinterpret_cast<const float2*>(B), ldb,
-                                  sB, M, N, K, nb, p->d_split + p->split_stride * g.e0, p->split_stride);
+                                  sB, M, N, K, nb,
+                                  p->d_split + p->split_stride * g.e0 + split_slot_halves(p) * g.split_slot,
+                                  p->split_stride);
                });
         return;
     }
@@ -3983,13 +4001,14 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
     const bool la = g.s2 != nullptr;
     Group g2 = g;
     g2.s = g.s2;
+    g2.split_slot = 1;
     bool far_pending = false;
     for (int k0 = 0; k0 < n; k0 += kOuter) {
         const int kend = std::min(k0 + kOuter, n);
         auto panel = [&](int j0) {
             const double m = n - j0;
             launch(p, g.s, REFEFF_FMS_K_PANEL, 8.0 * m * kPanel * kPanel / 2 * nb, 2.0 * m * kPanel * cb * nb, [&] {
-                launch_panel<T>(p, g.s, nb, A, sa, n, ipiv, info, j0, n);
+                launch_panel<T>(p, g.s, nb, g.e0, A, sa, n, ipiv, info, j0, n);
             });
         };
         // In-block steps.  Split: the block's left half is factored first with

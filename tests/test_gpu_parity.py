@@ -188,15 +188,23 @@ def test_mkgtr_trace_on_gpu_matches_oracle():
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("bf16,split", [("0", "0"), ("2", "0"), ("3", "0"), ("16", "0"), ("0", "1")])
-def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16, split):
+@pytest.mark.parametrize("bf16,split,streams,lookahead", [
+    ("0", "0", "1", "0"), ("2", "0", "1", "0"), ("3", "0", "1", "0"), ("16", "0", "1", "0"), ("4", "0", "1", "0"),
+    ("0", "1", "1", "0"), ("0", "0", "2", "1"), ("4", "1", "2", "1")])
+def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16, split, streams, lookahead):
     """Cu fcc R=8.0 (177 atoms, N=2832) at full size: the two device solvers
-    (FP32 LU + FP64 refinement vs plain FP64 LU) agree to the gg gate, also with
-    the trailing updates on split-bf16 MFMA (two or three parts) and with the
-    outer blocks factored as two halves (both LUs)."""
+    (mixed LU + FP64 refinement vs plain FP64 LU) agree to the gg gate with the
+    trailing updates in FP32, on split-bf16 MFMA (two or three parts) and on
+    split-fp16 MFMA (3M and four-product forms), with the outer blocks factored
+    as two halves (both LUs), and with the energies in two stream groups and the
+    far trailing update on a second stream (every group and stream with its own
+    pivoting and split buffers)."""
     monkeypatch.setenv("REFEFF_FMS_GEMM_BF16", bf16)
     monkeypatch.setenv("REFEFF_FMS_LU_SPLIT", split)
-    banner(f"Cu fcc R=8.0 N=2832 mixed (bf16 split {bf16}, LU halves {split}) vs fp64 (6 energies)")
+    monkeypatch.setenv("REFEFF_FMS_STREAMS", streams)
+    monkeypatch.setenv("REFEFF_FMS_LOOKAHEAD", lookahead)
+    banner(f"Cu fcc R=8.0 N=2832 mixed (split GEMM {bf16}, LU halves {split}, streams {streams}, lookahead "
+           f"{lookahead}) vs fp64 (6 energies)")
     wl = W.fcc_cu(8.0, 120)
     sel = [0, 20, 47, 80, 101, 119]
     k, ph = wl.wave_numbers()[sel], wl.phase_shifts()[sel]
