@@ -39,6 +39,7 @@ constexpr int kPanel = 32;                 // LU block width
 constexpr int kRhsChunk = 64;              // right-hand sides per solve/residual workgroup
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Cx;
@@ -61,6 +62,10 @@ __device__ __forceinline__ float2 cfms(float2 a, float2 b, float2 c) {
 }
 __device__ __forceinline__ double2 cfms(double2 a, double2 b, double2 c) {
     return make_double2(a.x - (b.x * c.x - b.y * c.y), a.y - (b.x * c.y + b.y * c.x));
+}
+// acc + a*b
+__device__ __forceinline__ double2 cmac(double2 acc, double2 a, double2 b) {
+    return make_double2(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.x * b.y + a.y * b.x));
 }
 __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
     // Smith's algorithm
@@ -336,7 +341,8 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
             const int s = tid % nsp, ec = tid / nsp;
             const double2 k = g.kv[static_cast<size_t>(e0 + ec) * nsp + s];
             const double2 rho = make_double2(k.x * dist, k.y * dist);
-            const double2 z = cdiv(make_double2(0.0, -1.0), rho);  // -i / rho
+            const double2 irho = cdiv(make_double2(1.0, 0.0), rho);
+            const double2 z = make_double2(irho.y, -irho.x);  // -i / rho
             double2* C = clm[ec][s];
             auto at = [&](int l, int m) -> double2& { return C[l * L1 + m]; };
             for (int i = 0; i < (L + 2) * L1; ++i) C[i] = make_double2(0.0, 0.0);
@@ -365,9 +371,12 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
             }
             // prefactor e^{i rho}/rho e^{-sigma^2 k^2} (geometry.rs:14-28)
             const double2 eir = cexp_d(make_double2(-rho.y, rho.x));
-            const double2 k2 = cmul(k, k);
-            const double2 damp = cexp_d(make_double2(-sigma * k2.x, -sigma * k2.y));
-            pref[ec][s] = cmul(cdiv(eir, rho), damp);
+            double2 pf = cmul(eir, irho);
+            if (sigma != 0.0) {
+                const double2 k2 = cmul(k, k);
+                pf = cmul(pf, cexp_d(make_double2(-sigma * k2.x, -sigma * k2.y)));
+            }
+            pref[ec][s] = pf;
         }
         __syncthreads();
         // z-axis propagator g_mu(l1, l2) with the prefactor folded in (xgllm)
@@ -392,48 +401,77 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
             gl[ec][s][r] = acc;
         }
         __syncthreads();
-        for (int idx = tid; idx < ne * cells; idx += blockDim.x) {
-            const int ec = idx / cells, cell = idx - ec * cells;
-            const int e = e0 + ec;
-            const double2* tm0 = g.tmat + static_cast<size_t>(e) * 2 * g.n;
-            const double2* tm1 = tm0 + g.n;
+        // One cell per thread at a time: its energy-independent rotation
+        // weights W_mu = d_b(l1; mu, m1) d_f(l2; m2, mu) are formed once per
+        // pass, then every energy of the pass takes G0 = sum_mu W_mu gl(|mu|,
+        // l1, l2).  mu runs over -LM..LM with zero weights (and table index 0)
+        // outside min(l1, l2), so the weights stay in registers.  The spin-
+        // orbit partner column of two-spin clusters takes the direct sum.
+        constexpr int NMU = 2 * LM + 1;
+        for (int cell = tid; cell < cells; cell += blockDim.x) {
             const int r = cell / nbk, c = cell - r * nbk;
             const int sr = r % nsp, lr = r / nsp, l1 = isqrt_small(lr), m1 = lr - l1 * l1 - l1;
-            auto g0 = [&](int cc) -> double2 {
-                const int sc = cc % nsp, lc = cc / nsp, l2 = isqrt_small(lc), m2 = lc - l2 * l2 - l2;
-                if (sc != sr) return make_double2(0.0, 0.0);
-                double2 acc = make_double2(0.0, 0.0);
-                const int mlim = min(l1, l2);
-                for (int mu = -mlim; mu <= mlim; ++mu) {
-                    const int amu = mu < 0 ? -mu : mu;
+            const int sc = c % nsp, lc = c / nsp, l2 = isqrt_small(lc), m2 = lc - l2 * l2 - l2;
+            const int mlim = (sc == sr) ? min(l1, l2) : -1;
+            const int base = l1 * L1 + l2;
+            double2 w[NMU];
+            unsigned valid = 0;
+#pragma unroll
+            for (int t = 0; t < NMU; ++t) {
+                const int mu = t - LM, amu = mu < 0 ? -mu : mu;
+                if (amu <= mlim) {
                     const double2 db = rb[rot_base(l1) + (mu + l1) * (2 * l1 + 1) + (m1 + l1)];
                     const double2 df = rf[rot_base(l2) + (m2 + l2) * (2 * l2 + 1) + (mu + l2)];
-                    acc = cadd(acc, cmul(cmul(db, gl[ec][sr][(amu * L1 + l1) * L1 + l2]), df));
+                    w[t] = cmul(db, df);
+                    valid |= 1u << t;
+                } else {
+                    w[t] = make_double2(0.0, 0.0);
                 }
-                return acc;
-            };
-            const double2 gv = g0(c);
-            if (GG) {
-                const size_t at = static_cast<size_t>(e) * g.sa + static_cast<size_t>(fa + r) * g.lda + fb + c;
-                g.a64[at] = (sqrt(gv.x * gv.x + gv.y * gv.y) > g.toler2) ? gv : make_double2(0.0, 0.0);
-                if (rhs >= 0) g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fb + c) * g.nrhs + rhs + r] = gv;
-                continue;
+            }
+            // spin-orbit partner column (two spins): t1 couples G0(r, c') into A(r, c);
+            // G0(r, c') vanishes unless c' has the row's spin
+            int cp = -1;
+            if (!GG && nsp == 2) {
+                const int proj = m2 + sc + 1;
+                if (proj > -l2 + 1 && proj < l2 + 2) cp = (sc == 0) ? c - 1 : c + 1;
+                if (cp >= 0 && cp % nsp != sr) cp = -1;
             }
             const int cs = fb + c;
-            double2 v = cmul(gv, tm0[cs]);
-            v = make_double2(-v.x, -v.y);
-            if (nsp == 2) {
-                const int sc = c % nsp, lc = c / nsp, l2 = isqrt_small(lc), m2 = lc - l2 * l2 - l2;
-                const int proj = m2 + sc + 1;
-                if (proj > -l2 + 1 && proj < l2 + 2) {
-                    const int cp = (sc == 0) ? c - 1 : c + 1;
-                    v = csub(v, cmul(g0(cp), tm1[cs]));
+            const size_t row_at = static_cast<size_t>(fa + r) * g.lda + fb + c;
+            for (int ec = 0; ec < ne; ++ec) {
+                const int e = e0 + ec;
+                const double2* glr = gl[ec][sr];
+                double2 gv = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int t = 0; t < NMU; ++t) {
+                    const int amu = t < LM ? LM - t : t - LM;
+                    gv = cmac(gv, w[t], glr[((valid >> t) & 1u) ? amu * L1 * L1 + base : 0]);
                 }
+                const size_t at = static_cast<size_t>(e) * g.sa + row_at;
+                if (GG) {
+                    g.a64[at] = (sqrt(gv.x * gv.x + gv.y * gv.y) > g.toler2) ? gv : make_double2(0.0, 0.0);
+                    if (rhs >= 0) g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fb + c) * g.nrhs + rhs + r] = gv;
+                    continue;
+                }
+                const double2* tm0 = g.tmat + static_cast<size_t>(e) * 2 * g.n;
+                double2 v = cmul(gv, tm0[cs]);
+                v = make_double2(-v.x, -v.y);
+                if (cp >= 0) {
+                    const int lpc = cp / nsp, lp2 = isqrt_small(lpc), mp2 = lpc - lp2 * lp2 - lp2;
+                    const int plim = min(l1, lp2);
+                    double2 gp = make_double2(0.0, 0.0);
+                    for (int mu = -plim; mu <= plim; ++mu) {
+                        const int amu = mu < 0 ? -mu : mu;
+                        const double2 db = rb[rot_base(l1) + (mu + l1) * (2 * l1 + 1) + (m1 + l1)];
+                        const double2 df = rf[rot_base(lp2) + (mp2 + lp2) * (2 * lp2 + 1) + (mu + lp2)];
+                        gp = cmac(gp, cmul(db, glr[(amu * L1 + l1) * L1 + lp2]), df);
+                    }
+                    v = csub(v, cmul(gp, tm0[g.n + cs]));
+                }
+                if (W32) g.a32[at] = make_float2((float)v.x, (float)v.y);
+                if (W64) g.a64[at] = v;
+                if (rhs >= 0) g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fa + r) * g.nrhs + rhs + c] = gv;
             }
-            const size_t at = static_cast<size_t>(e) * g.sa + static_cast<size_t>(fa + r) * g.lda + fb + c;
-            if (W32) g.a32[at] = make_float2((float)v.x, (float)v.y);
-            if (W64) g.a64[at] = v;
-            if (rhs >= 0) g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fa + r) * g.nrhs + rhs + c] = gv;
         }
         __syncthreads();
     }
@@ -2125,21 +2163,22 @@ __device__ __forceinline__ void grouped_tile(int tile, int tilesM, int tilesN, i
 // (row r0 + 16 i + 4 (lane / 16) + q, column c0 + 16 j + lane % 16).  Every
 // scale and C value is loaded first, from clamped addresses, so the loads
 // share one latency; the stores skip rows / columns past M / N.
+template <int TI, int TJ>
 __device__ __forceinline__ void fp16_epilogue(float2* C, int ldc, int M, int N, int r0, int c0, int lane,
-                                              const float* sa, const float* sb, const v4f (&cr)[4][2],
-                                              const v4f (&ci)[4][2]) {
-    float srow[4][4], scol[2];
-    float2 cv[4][2][4];
+                                              const float* sa, const float* sb, const v4f (&cr)[TI][TJ],
+                                              const v4f (&ci)[TI][TJ]) {
+    float srow[TI][4], scol[TJ];
+    float2 cv[TI][TJ][4];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) scol[j] = sb[min(c0 + j * 16 + (lane & 15), N - 1)];
+    for (int j = 0; j < TJ; ++j) scol[j] = sb[min(c0 + j * 16 + (lane & 15), N - 1)];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
         for (int q = 0; q < 4; ++q) srow[i][q] = sa[min(r0 + i * 16 + 4 * (lane >> 4) + q, M - 1)];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = min(r0 + i * 16 + 4 * (lane >> 4) + q, M - 1);
@@ -2147,9 +2186,9 @@ __device__ __forceinline__ void fp16_epilogue(float2* C, int ldc, int M, int N, 
                 cv[i][j][q] = C[static_cast<size_t>(row) * ldc + col];
             }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = r0 + i * 16 + 4 * (lane >> 4) + q;
@@ -2278,7 +2317,137 @@ __global__ __launch_bounds__(512, 1) void k_gemm_h4(float2* Cb, int ldc, size_t 
     } else {  // step kb (stage 0) left
         compute(0);
     }
-    fp16_epilogue(C, ldc, M, N, m0 + wr * 64, n0 + wc * 32, lane, h.sa(e), h.sb(e), cr, ci);
+    fp16_epilogue<4, 2>(C, ldc, M, N, m0 + wr * 64, n0 + wc * 32, lane, h.sa(e), h.sb(e), cr, ci);
+}
+
+// k_gemm_h4 with 32x32x16 MFMA tiles: each wave's 64 x 32 block is two 32 x 32
+// tiles, K = 32 steps as two k16 halves; half the MFMA instructions for the
+// same products and operand reads.  Output layout of a 32 x 32 tile: lane l
+// holds column l % 32, rows 8 i + 4 (l / 32) + q at element 4 i + q.
+__global__ __launch_bounds__(512, 1) void k_gemm_h4m(float2* Cb, int ldc, size_t sC, HSplit4 h) {
+    constexpr int SLAB = 128;
+    constexpr int OPER = 4 * 4 * SLAB;
+    __shared__ uint4 lds[2 * 2 * OPER];
+    const int M = h.M, N = h.N, K = h.K;
+    const int tilesM = (M + 127) / 128, tilesN = (N + 127) / 128;
+    int tile, e, tm, tn;
+    xcd_tile(tilesM * tilesN, tile, e);
+    grouped_tile(tile, tilesM, tilesN, tm, tn);
+    const int m0 = tm * 128, n0 = tn * 128;
+    const uint16_t* PA = h.pa(e);
+    const uint16_t* PB = h.pb(e);
+    float2* C = Cb + e * sC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int lkc = tid >> 7, lr = tid & 127;
+    const size_t offa = static_cast<size_t>(lkc) * M + min(m0 + lr, M - 1);
+    const size_t offb = static_cast<size_t>(lkc) * N + min(n0 + lr, N - 1);
+    struct Pref {
+        uint4 a0, a1, a2, a3, b0, b1, b2, b3;
+    };
+    auto gload = [&](int kb) -> Pref {
+        const size_t pl = static_cast<size_t>(kb) * 16;
+        const uint4* ga = reinterpret_cast<const uint4*>(PA) + pl * M + offa;
+        const uint4* gb = reinterpret_cast<const uint4*>(PB) + pl * N + offb;
+        const size_t sa4 = static_cast<size_t>(4) * M, sb4 = static_cast<size_t>(4) * N;
+        return Pref{ga[0], ga[sa4], ga[2 * sa4], ga[3 * sa4], gb[0], gb[sb4], gb[2 * sb4], gb[3 * sb4]};
+    };
+    auto sstore = [&](int stage, const Pref& r) {
+        uint4* la = lds + stage * 2 * OPER + lkc * SLAB + lr;
+        la[0 * 4 * SLAB] = r.a0;
+        la[1 * 4 * SLAB] = r.a1;
+        la[2 * 4 * SLAB] = r.a2;
+        la[3 * 4 * SLAB] = r.a3;
+        la[OPER + 0 * 4 * SLAB] = r.b0;
+        la[OPER + 1 * 4 * SLAB] = r.b1;
+        la[OPER + 2 * 4 * SLAB] = r.b2;
+        la[OPER + 3 * 4 * SLAB] = r.b3;
+    };
+    v16f cr[2], ci[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        cr[i] = v16f{};
+        ci[i] = v16f{};
+    }
+    const int orow = lane & 31, ohalf = lane >> 5;
+    auto compute = [&](int stage) {
+        const uint4* la = lds + stage * 2 * OPER;
+        const uint4* lb = la + OPER;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int kc = 2 * ks + ohalf;
+            v8h b[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) b[p] = __builtin_bit_cast(v8h, lb[(p * 4 + kc) * SLAB + wc * 32 + orow]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                v8h a[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+                    a[p] = __builtin_bit_cast(v8h, la[(p * 4 + kc) * SLAB + wr * 64 + i * 32 + orow]);
+                const v8h nah = neg_h8(a[2]), nam = neg_h8(a[3]);
+                const v8h ar[6] = {a[0], a[0], a[1], nah, nah, nam};
+                constexpr int br[6] = {0, 1, 0, 2, 3, 2};
+                const v8h ai[6] = {a[0], a[0], a[1], a[2], a[2], a[3]};
+                constexpr int bi[6] = {2, 3, 2, 0, 1, 0};
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    cr[i] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ar[t], b[br[t]], cr[i], 0, 0, 0);
+                    ci[i] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ai[t], b[bi[t]], ci[i], 0, 0, 0);
+                }
+            }
+        }
+    };
+    const int nk = K / 32;
+    Pref r0 = gload(0), r1{};
+    if (nk > 1) r1 = gload(1);
+    sstore(0, r0);
+    if (nk > 2) r0 = gload(2);
+    __syncthreads();
+    int kb = 0;
+    for (; kb + 2 < nk; kb += 2) {
+        sstore(1, r1);
+        if (kb + 3 < nk) r1 = gload(kb + 3);
+        compute(0);
+        __syncthreads();
+        sstore(0, r0);
+        if (kb + 4 < nk) r0 = gload(kb + 4);
+        compute(1);
+        __syncthreads();
+    }
+    if (kb + 1 < nk) {
+        sstore(1, r1);
+        compute(0);
+        __syncthreads();
+        compute(1);
+    } else {
+        compute(0);
+    }
+    // epilogue: all loads first (clamped), then the guarded stores
+    const float* sa = h.sa(e);
+    const float* sb = h.sb(e);
+    const int c0 = n0 + wc * 32 + orow;
+    const float scol = sb[min(c0, N - 1)];
+    float2 cv[2][16];
+    float srow[2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int row = min(m0 + wr * 64 + i * 32 + 8 * (q >> 2) + 4 * ohalf + (q & 3), M - 1);
+            srow[i][q] = sa[row];
+            cv[i][q] = C[static_cast<size_t>(row) * ldc + min(c0, N - 1)];
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int row = m0 + wr * 64 + i * 32 + 8 * (q >> 2) + 4 * ohalf + (q & 3);
+            if (row < M && c0 < N) {
+                const float sc = srow[i][q] * scol;
+                C[static_cast<size_t>(row) * ldc + c0] = float2{cv[i][q].x - sc * cr[i][q], cv[i][q].y - sc * ci[i][q]};
+            }
+        }
 }
 
 // final row permutation of the LU: perm[i] = original row now at position i
@@ -3885,11 +4054,15 @@ void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, int e0, typename Cx
 // k_gemm_bf.  M, N multiples of 64, K of 32, 16-byte aligned A rows.
 void gemm_bf_launch(hipStream_t s, int ns, float2* C, int ldc, size_t sC, const float2* A, int lda, size_t sA,
                     const float2* B, int ldb, size_t sB, int M, int N, int K, int nb, uint16_t* split, size_t ssplit) {
-    if (ns == 4) {  // fp16 two-part split, four-product form
+    if (ns == 4 || ns == 5) {  // fp16 two-part split, four-product form (5: 32x32 MFMA tiles)
         const HSplit4 hs{split, ssplit, M, N, K};
         hipLaunchKernelGGL(k_h4split_rows, dim3((M + 3) / 4, nb), dim3(256), 0, s, A, lda, sA, hs);
         hipLaunchKernelGGL(k_h4split_cols, dim3((N + 255) / 256, nb), dim3(256), 0, s, B, ldb, sB, hs);
-        hipLaunchKernelGGL(k_gemm_h4, dim3(((M + 127) / 128) * ((N + 127) / 128), nb), dim3(512), 0, s, C, ldc, sC, hs);
+        const dim3 grid(((M + 127) / 128) * ((N + 127) / 128), nb);
+        if (ns == 4)
+            hipLaunchKernelGGL(k_gemm_h4, grid, dim3(512), 0, s, C, ldc, sC, hs);
+        else
+            hipLaunchKernelGGL(k_gemm_h4m, grid, dim3(512), 0, s, C, ldc, sC, hs);
         return;
     }
     if (ns == 16) {  // fp16 two-part split with row / column scales
@@ -4082,9 +4255,9 @@ template <bool W32, bool W64, bool GG>
 void launch_build(hipStream_t s, const BuildArgs& ba, int nat, int L) {
     const dim3 grid(nat, nat), block(256);
     if (L <= 3)
-        hipLaunchKernelGGL((k_build<W32, W64, GG, 3, 8>), grid, block, 0, s, ba);
+        hipLaunchKernelGGL((k_build<W32, W64, GG, 3, 16>), grid, block, 0, s, ba);
     else if (L <= 4)
-        hipLaunchKernelGGL((k_build<W32, W64, GG, 4, 4>), grid, block, 0, s, ba);
+        hipLaunchKernelGGL((k_build<W32, W64, GG, 4, 8>), grid, block, 0, s, ba);
     else
         hipLaunchKernelGGL((k_build<W32, W64, GG, kMaxL, 2>), grid, block, 0, s, ba);
 }
@@ -5038,7 +5211,11 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_CPF")) p->gemm_cpf = std::atoi(e) != 0;
     // the split-fp16 trailing update once the trailing updates dominate (the
     // FP32 3M update below); REFEFF_FMS_GEMM_BF16 overrides
-    if (p->solver == REFEFF_FMS_SOLVER_MIXED && p->npad >= 1536) p->gemm_bf16 = 4;
+    if (p->solver == REFEFF_FMS_SOLVER_MIXED && p->npad >= 1536) {
+        p->gemm_bf16 = 4;
+        // the f16 update runs faster with a deeper K; the in-block work grows with it
+        if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_BLOCK")) p->outer_block = 384;
+    }
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_BF16")) {
         const int v = std::atoi(e);
         p->gemm_bf16 = (v == 2 || v == 3 || v == 4 || v == 16) ? v : 0;  // bf16x2, bf16x3, fp16x2 4M, fp16x2 3M
@@ -5560,7 +5737,7 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
     const int kmul = (use_double == 0 || use_double == 3 || use_double == 5 || use_double >= 12) ? 32 : 16;
-    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 15 ||
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 16 ||
         use_double == 2 || (use_double >= 9 && use_double <= 11))
         return REFEFF_FMS_INVALID_ARGUMENT;
     if (use_double >= 12 && (M % 64 || N % 64)) return REFEFF_FMS_INVALID_ARGUMENT;
@@ -5583,7 +5760,7 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
         } else {
             if (use_double >= 12 && sizeof(T) == 4) {
                 // kinds 12 / 13: split-bf16 trailing update with 2 / 3 parts; 14 / 15: fp16x2, 3M / 4M
-                const int ns = use_double == 12 ? 2 : use_double == 13 ? 3 : use_double == 14 ? 16 : 4;
+                const int ns = use_double == 12 ? 2 : use_double == 13 ? 3 : use_double == 14 ? 16 : use_double == 15 ? 4 : 5;
                 uint16_t* sp = nullptr;
                 const size_t ss = static_cast<size_t>(3) * (ns == 3 ? 3 : 2) * (M + N) * K + 2 * static_cast<size_t>(M + N);
                 if (!dalloc(&sp, ss)) {
@@ -5663,7 +5840,7 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     // kernels 16 / 17: the fp16x2 trailing update, 3M / 4M, split kernels included
     uint16_t* split = nullptr;
     const size_t ss = (static_cast<size_t>(6) * (M + N) * K + 2 * static_cast<size_t>(M + N) + 7) / 8 * 8;
-    if ((kernel == 16 || kernel == 17) && hipMalloc(&split, ss * batch * sizeof(uint16_t)) != hipSuccess) {
+    if ((kernel == 16 || kernel == 17 || kernel == 18) && hipMalloc(&split, ss * batch * sizeof(uint16_t)) != hipSuccess) {
         (void)hipFree(da);
         (void)hipFree(db);
         (void)hipFree(dc);
@@ -5673,8 +5850,8 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     auto one = [&]() {
-        if (kernel == 16 || kernel == 17) {
-            gemm_bf_launch(nullptr, kernel == 16 ? 16 : 4, static_cast<float2*>(dc), ld, per,
+        if (kernel == 16 || kernel == 17 || kernel == 18) {
+            gemm_bf_launch(nullptr, kernel == 16 ? 16 : kernel == 17 ? 4 : 5, static_cast<float2*>(dc), ld, per,
                            static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K,
                            batch, split, ss);
         } else if (kernel >= 3 && kernel <= 8) {
