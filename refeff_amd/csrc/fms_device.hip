@@ -39,7 +39,6 @@ constexpr int kPanel = 32;                 // LU block width
 constexpr int kRhsChunk = 64;              // right-hand sides per solve/residual workgroup
 
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef float v16f __attribute__((ext_vector_type(16)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Cx;
@@ -2320,136 +2319,6 @@ __global__ __launch_bounds__(512, 1) void k_gemm_h4(float2* Cb, int ldc, size_t 
     fp16_epilogue<4, 2>(C, ldc, M, N, m0 + wr * 64, n0 + wc * 32, lane, h.sa(e), h.sb(e), cr, ci);
 }
 
-// k_gemm_h4 with 32x32x16 MFMA tiles: each wave's 64 x 32 block is two 32 x 32
-// tiles, K = 32 steps as two k16 halves; half the MFMA instructions for the
-// same products and operand reads.  Output layout of a 32 x 32 tile: lane l
-// holds column l % 32, rows 8 i + 4 (l / 32) + q at element 4 i + q.
-__global__ __launch_bounds__(512, 1) void k_gemm_h4m(float2* Cb, int ldc, size_t sC, HSplit4 h) {
-    constexpr int SLAB = 128;
-    constexpr int OPER = 4 * 4 * SLAB;
-    __shared__ uint4 lds[2 * 2 * OPER];
-    const int M = h.M, N = h.N, K = h.K;
-    const int tilesM = (M + 127) / 128, tilesN = (N + 127) / 128;
-    int tile, e, tm, tn;
-    xcd_tile(tilesM * tilesN, tile, e);
-    grouped_tile(tile, tilesM, tilesN, tm, tn);
-    const int m0 = tm * 128, n0 = tn * 128;
-    const uint16_t* PA = h.pa(e);
-    const uint16_t* PB = h.pb(e);
-    float2* C = Cb + e * sC;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 2, wc = wave & 3;
-    const int lkc = tid >> 7, lr = tid & 127;
-    const size_t offa = static_cast<size_t>(lkc) * M + min(m0 + lr, M - 1);
-    const size_t offb = static_cast<size_t>(lkc) * N + min(n0 + lr, N - 1);
-    struct Pref {
-        uint4 a0, a1, a2, a3, b0, b1, b2, b3;
-    };
-    auto gload = [&](int kb) -> Pref {
-        const size_t pl = static_cast<size_t>(kb) * 16;
-        const uint4* ga = reinterpret_cast<const uint4*>(PA) + pl * M + offa;
-        const uint4* gb = reinterpret_cast<const uint4*>(PB) + pl * N + offb;
-        const size_t sa4 = static_cast<size_t>(4) * M, sb4 = static_cast<size_t>(4) * N;
-        return Pref{ga[0], ga[sa4], ga[2 * sa4], ga[3 * sa4], gb[0], gb[sb4], gb[2 * sb4], gb[3 * sb4]};
-    };
-    auto sstore = [&](int stage, const Pref& r) {
-        uint4* la = lds + stage * 2 * OPER + lkc * SLAB + lr;
-        la[0 * 4 * SLAB] = r.a0;
-        la[1 * 4 * SLAB] = r.a1;
-        la[2 * 4 * SLAB] = r.a2;
-        la[3 * 4 * SLAB] = r.a3;
-        la[OPER + 0 * 4 * SLAB] = r.b0;
-        la[OPER + 1 * 4 * SLAB] = r.b1;
-        la[OPER + 2 * 4 * SLAB] = r.b2;
-        la[OPER + 3 * 4 * SLAB] = r.b3;
-    };
-    v16f cr[2], ci[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        cr[i] = v16f{};
-        ci[i] = v16f{};
-    }
-    const int orow = lane & 31, ohalf = lane >> 5;
-    auto compute = [&](int stage) {
-        const uint4* la = lds + stage * 2 * OPER;
-        const uint4* lb = la + OPER;
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            const int kc = 2 * ks + ohalf;
-            v8h b[4];
-#pragma unroll
-            for (int p = 0; p < 4; ++p) b[p] = __builtin_bit_cast(v8h, lb[(p * 4 + kc) * SLAB + wc * 32 + orow]);
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                v8h a[4];
-#pragma unroll
-                for (int p = 0; p < 4; ++p)
-                    a[p] = __builtin_bit_cast(v8h, la[(p * 4 + kc) * SLAB + wr * 64 + i * 32 + orow]);
-                const v8h nah = neg_h8(a[2]), nam = neg_h8(a[3]);
-                const v8h ar[6] = {a[0], a[0], a[1], nah, nah, nam};
-                constexpr int br[6] = {0, 1, 0, 2, 3, 2};
-                const v8h ai[6] = {a[0], a[0], a[1], a[2], a[2], a[3]};
-                constexpr int bi[6] = {2, 3, 2, 0, 1, 0};
-#pragma unroll
-                for (int t = 0; t < 6; ++t) {
-                    cr[i] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ar[t], b[br[t]], cr[i], 0, 0, 0);
-                    ci[i] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ai[t], b[bi[t]], ci[i], 0, 0, 0);
-                }
-            }
-        }
-    };
-    const int nk = K / 32;
-    Pref r0 = gload(0), r1{};
-    if (nk > 1) r1 = gload(1);
-    sstore(0, r0);
-    if (nk > 2) r0 = gload(2);
-    __syncthreads();
-    int kb = 0;
-    for (; kb + 2 < nk; kb += 2) {
-        sstore(1, r1);
-        if (kb + 3 < nk) r1 = gload(kb + 3);
-        compute(0);
-        __syncthreads();
-        sstore(0, r0);
-        if (kb + 4 < nk) r0 = gload(kb + 4);
-        compute(1);
-        __syncthreads();
-    }
-    if (kb + 1 < nk) {
-        sstore(1, r1);
-        compute(0);
-        __syncthreads();
-        compute(1);
-    } else {
-        compute(0);
-    }
-    // epilogue: all loads first (clamped), then the guarded stores
-    const float* sa = h.sa(e);
-    const float* sb = h.sb(e);
-    const int c0 = n0 + wc * 32 + orow;
-    const float scol = sb[min(c0, N - 1)];
-    float2 cv[2][16];
-    float srow[2][16];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int row = min(m0 + wr * 64 + i * 32 + 8 * (q >> 2) + 4 * ohalf + (q & 3), M - 1);
-            srow[i][q] = sa[row];
-            cv[i][q] = C[static_cast<size_t>(row) * ldc + min(c0, N - 1)];
-        }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int row = m0 + wr * 64 + i * 32 + 8 * (q >> 2) + 4 * ohalf + (q & 3);
-            if (row < M && c0 < N) {
-                const float sc = srow[i][q] * scol;
-                C[static_cast<size_t>(row) * ldc + c0] = float2{cv[i][q].x - sc * cr[i][q], cv[i][q].y - sc * ci[i][q]};
-            }
-        }
-}
-
 // final row permutation of the LU: perm[i] = original row now at position i
 __global__ __launch_bounds__(256) void k_perm(const int* ipivb, int* permb, int n) {
     extern __shared__ int map[];
@@ -4054,15 +3923,11 @@ void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, int e0, typename Cx
 // k_gemm_bf.  M, N multiples of 64, K of 32, 16-byte aligned A rows.
 void gemm_bf_launch(hipStream_t s, int ns, float2* C, int ldc, size_t sC, const float2* A, int lda, size_t sA,
                     const float2* B, int ldb, size_t sB, int M, int N, int K, int nb, uint16_t* split, size_t ssplit) {
-    if (ns == 4 || ns == 5) {  // fp16 two-part split, four-product form (5: 32x32 MFMA tiles)
+    if (ns == 4) {  // fp16 two-part split, four-product form
         const HSplit4 hs{split, ssplit, M, N, K};
         hipLaunchKernelGGL(k_h4split_rows, dim3((M + 3) / 4, nb), dim3(256), 0, s, A, lda, sA, hs);
         hipLaunchKernelGGL(k_h4split_cols, dim3((N + 255) / 256, nb), dim3(256), 0, s, B, ldb, sB, hs);
-        const dim3 grid(((M + 127) / 128) * ((N + 127) / 128), nb);
-        if (ns == 4)
-            hipLaunchKernelGGL(k_gemm_h4, grid, dim3(512), 0, s, C, ldc, sC, hs);
-        else
-            hipLaunchKernelGGL(k_gemm_h4m, grid, dim3(512), 0, s, C, ldc, sC, hs);
+        hipLaunchKernelGGL(k_gemm_h4, dim3(((M + 127) / 128) * ((N + 127) / 128), nb), dim3(512), 0, s, C, ldc, sC, hs);
         return;
     }
     if (ns == 16) {  // fp16 two-part split with row / column scales
@@ -5737,7 +5602,7 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
                           double* C) {
     // C (M x N) -= A (M x K) B (K x N), all row-major complex, through k_gemm_sub
     const int kmul = (use_double == 0 || use_double == 3 || use_double == 5 || use_double >= 12) ? 32 : 16;
-    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 16 ||
+    if (M <= 0 || N <= 0 || K <= 0 || K % kmul || !A || !B || !C || use_double < 0 || use_double > 15 ||
         use_double == 2 || (use_double >= 9 && use_double <= 11))
         return REFEFF_FMS_INVALID_ARGUMENT;
     if (use_double >= 12 && (M % 64 || N % 64)) return REFEFF_FMS_INVALID_ARGUMENT;
@@ -5760,7 +5625,7 @@ int refeff_fms_debug_gemm(int32_t use_double, int32_t M, int32_t N, int32_t K, c
         } else {
             if (use_double >= 12 && sizeof(T) == 4) {
                 // kinds 12 / 13: split-bf16 trailing update with 2 / 3 parts; 14 / 15: fp16x2, 3M / 4M
-                const int ns = use_double == 12 ? 2 : use_double == 13 ? 3 : use_double == 14 ? 16 : use_double == 15 ? 4 : 5;
+                const int ns = use_double == 12 ? 2 : use_double == 13 ? 3 : use_double == 14 ? 16 : 4;
                 uint16_t* sp = nullptr;
                 const size_t ss = static_cast<size_t>(3) * (ns == 3 ? 3 : 2) * (M + N) * K + 2 * static_cast<size_t>(M + N);
                 if (!dalloc(&sp, ss)) {
@@ -5840,7 +5705,7 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     // kernels 16 / 17: the fp16x2 trailing update, 3M / 4M, split kernels included
     uint16_t* split = nullptr;
     const size_t ss = (static_cast<size_t>(6) * (M + N) * K + 2 * static_cast<size_t>(M + N) + 7) / 8 * 8;
-    if ((kernel == 16 || kernel == 17 || kernel == 18) && hipMalloc(&split, ss * batch * sizeof(uint16_t)) != hipSuccess) {
+    if ((kernel == 16 || kernel == 17) && hipMalloc(&split, ss * batch * sizeof(uint16_t)) != hipSuccess) {
         (void)hipFree(da);
         (void)hipFree(db);
         (void)hipFree(dc);
@@ -5850,8 +5715,8 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     auto one = [&]() {
-        if (kernel == 16 || kernel == 17 || kernel == 18) {
-            gemm_bf_launch(nullptr, kernel == 16 ? 16 : kernel == 17 ? 4 : 5, static_cast<float2*>(dc), ld, per,
+        if (kernel == 16 || kernel == 17) {
+            gemm_bf_launch(nullptr, kernel == 16 ? 16 : 4, static_cast<float2*>(dc), ld, per,
                            static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K,
                            batch, split, ss);
         } else if (kernel >= 3 && kernel <= 8) {
