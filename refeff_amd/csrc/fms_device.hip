@@ -2070,72 +2070,124 @@ __device__ __forceinline__ size_t h4_index(int k, int plane, int rows, int row) 
            static_cast<size_t>(row) * 8 + (k & 7);
 }
 
-// A panel (M x K row-major complex, lda) -> planes and row scales, one wave per row
-__global__ __launch_bounds__(256) void k_h4split_rows(const float2* Ab, int lda, size_t sA, HSplit4 h) {
-    const int e = blockIdx.y, lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= h.M) return;
-    const float2* a = Ab + e * sA + static_cast<size_t>(row) * lda;
-    float mx = 0.0f;
-    for (int k = lane; k < h.K; k += 64) {
-        const float2 v = a[k];
-        mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-    }
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    float inv;
-    const float s = pow2_above(mx, &inv);
-    uint16_t* P = h.pa(e);
-    for (int k = lane; k < h.K; k += 64) {
-        const float2 v = a[k];
-        const float x[2] = {v.x * inv, v.y * inv};
+// Eight complex values scaled by inv -> the 16-byte (8 x fp16) chunks of the
+// four planes re h, re m, im h, im m.
+__device__ __forceinline__ void h4_chunks(const float2 (&v)[8], float inv, uint4 (&out)[4]) {
+    uint32_t w[4][4];
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const uint16_t hi = f2h(x[c]);
-            P[h4_index(k, 2 * c, h.M, row)] = hi;
-            P[h4_index(k, 2 * c + 1, h.M, row)] = f2h(x[c] - h2f(hi));
+    for (int u = 0; u < 8; u += 2) {
+        uint16_t part[2][4];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const float x[2] = {v[u + t].x * inv, v[u + t].y * inv};
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                part[t][2 * c] = f2h(x[c]);
+                part[t][2 * c + 1] = f2h(x[c] - h2f(part[t][2 * c]));
+            }
         }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) w[p][u / 2] = part[0][p] | (static_cast<uint32_t>(part[1][p]) << 16);
     }
-    if (lane == 0) h.sa(e)[row] = s;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) out[p] = uint4{w[p][0], w[p][1], w[p][2], w[p][3]};
 }
 
-// B panel (K x N row-major complex, ldb) -> planes and column scales, one
-// thread per column, 8 k values (16 bytes) per plane and store
-__global__ __launch_bounds__(256) void k_h4split_cols(const float2* Bb, int ldb, size_t sB, HSplit4 h) {
-    const int e = blockIdx.y;
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= h.N) return;
-    const float2* b = Bb + e * sB + col;
+// A panel (M x K row-major complex, lda) -> planes and row scales.  A block
+// takes 16 rows; thread t handles row t % 16 and the 8-value chunks t / 16,
+// t / 16 + 16, ...: the chunk reads are 64 bytes, the plane stores of 16 rows
+// for one chunk are one contiguous 256-byte run, and the row maxima meet in LDS.
+__global__ __launch_bounds__(256) void k_h4split_rows(const float2* Ab, int lda, size_t sA, HSplit4 h) {
+    __shared__ float part[16][17];
+    __shared__ float invs[16];
+    const int e = blockIdx.y, tid = threadIdx.x;
+    const int rr = tid & 15, cg = tid >> 4;
+    const int row = blockIdx.x * 16 + rr;
+    const bool live = row < h.M;
+    const float2* a = Ab + e * sA + static_cast<size_t>(live ? row : 0) * lda;
+    const int nch = h.K / 8;
     float mx = 0.0f;
-    for (int k = 0; k < h.K; ++k) {
-        const float2 v = b[static_cast<size_t>(k) * ldb];
-        mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-    }
-    float inv;
-    const float s = pow2_above(mx, &inv);
-    uint16_t* P = h.pb(e);
-    for (int k0 = 0; k0 < h.K; k0 += 8) {
-        uint32_t w[4][4];
+    for (int ch = cg; ch < nch; ch += 16) {
+        const float4* src = reinterpret_cast<const float4*>(a + ch * 8);
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-            uint16_t part[2][4];
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const float2 v = b[static_cast<size_t>(k0 + u + t) * ldb];
-                const float x[2] = {v.x * inv, v.y * inv};
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    part[t][2 * c] = f2h(x[c]);
-                    part[t][2 * c + 1] = f2h(x[c] - h2f(part[t][2 * c]));
-                }
-            }
-#pragma unroll
-            for (int p = 0; p < 4; ++p) w[p][u / 2] = part[0][p] | (static_cast<uint32_t>(part[1][p]) << 16);
+        for (int q = 0; q < 4; ++q) {
+            const float4 v = src[q];
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
         }
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-            *reinterpret_cast<uint4*>(P + h4_index(k0, p, h.N, col)) = uint4{w[p][0], w[p][1], w[p][2], w[p][3]};
     }
-    h.sb(e)[col] = s;
+    part[rr][cg] = mx;
+    __syncthreads();
+    if (tid < 16) {
+        float m = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) m = fmaxf(m, part[tid][q]);
+        float inv;
+        const float s = pow2_above(m, &inv);
+        invs[tid] = inv;
+        if (blockIdx.x * 16 + tid < h.M) h.sa(e)[blockIdx.x * 16 + tid] = s;
+    }
+    __syncthreads();
+    if (!live) return;
+    const float inv = invs[rr];
+    uint16_t* P = h.pa(e);
+    for (int ch = cg; ch < nch; ch += 16) {
+        float2 v[8];
+        const float4* src = reinterpret_cast<const float4*>(a + ch * 8);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 x = src[q];
+            v[2 * q] = make_float2(x.x, x.y);
+            v[2 * q + 1] = make_float2(x.z, x.w);
+        }
+        uint4 out[4];
+        h4_chunks(v, inv, out);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) *reinterpret_cast<uint4*>(P + h4_index(ch * 8, p, h.M, row)) = out[p];
+    }
+}
+
+// B panel (K x N row-major complex, ldb) -> planes and column scales.  A block
+// takes 64 columns; thread t handles column t % 64 and the chunks t / 64,
+// t / 64 + 4, ...: reads of a k row are 512-byte runs, plane stores of 64
+// columns one 1 KiB run, column maxima meet in LDS.
+__global__ __launch_bounds__(256) void k_h4split_cols(const float2* Bb, int ldb, size_t sB, HSplit4 h) {
+    __shared__ float part[64][5];
+    __shared__ float invs[64];
+    const int e = blockIdx.y, tid = threadIdx.x;
+    const int cc = tid & 63, cg = tid >> 6;
+    const int col = blockIdx.x * 64 + cc;
+    const bool live = col < h.N;
+    const float2* b = Bb + e * sB + (live ? col : 0);
+    const int nch = h.K / 8;
+    float mx = 0.0f;
+    for (int ch = cg; ch < nch; ch += 4)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float2 v = b[static_cast<size_t>(ch * 8 + u) * ldb];
+            mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+    part[cc][cg] = mx;
+    __syncthreads();
+    if (tid < 64) {
+        const float m = fmaxf(fmaxf(part[tid][0], part[tid][1]), fmaxf(part[tid][2], part[tid][3]));
+        float inv;
+        const float s = pow2_above(m, &inv);
+        invs[tid] = inv;
+        if (blockIdx.x * 64 + tid < h.N) h.sb(e)[blockIdx.x * 64 + tid] = s;
+    }
+    __syncthreads();
+    if (!live) return;
+    const float inv = invs[cc];
+    uint16_t* P = h.pb(e);
+    for (int ch = cg; ch < nch; ch += 4) {
+        float2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = b[static_cast<size_t>(ch * 8 + u) * ldb];
+        uint4 out[4];
+        h4_chunks(v, inv, out);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) *reinterpret_cast<uint4*>(P + h4_index(ch * 8, p, h.N, col)) = out[p];
+    }
 }
 
 __device__ __forceinline__ v8h neg_h8(v8h x) {
@@ -3925,8 +3977,8 @@ void gemm_bf_launch(hipStream_t s, int ns, float2* C, int ldc, size_t sC, const 
                     const float2* B, int ldb, size_t sB, int M, int N, int K, int nb, uint16_t* split, size_t ssplit) {
     if (ns == 4) {  // fp16 two-part split, four-product form
         const HSplit4 hs{split, ssplit, M, N, K};
-        hipLaunchKernelGGL(k_h4split_rows, dim3((M + 3) / 4, nb), dim3(256), 0, s, A, lda, sA, hs);
-        hipLaunchKernelGGL(k_h4split_cols, dim3((N + 255) / 256, nb), dim3(256), 0, s, B, ldb, sB, hs);
+        hipLaunchKernelGGL(k_h4split_rows, dim3((M + 15) / 16, nb), dim3(256), 0, s, A, lda, sA, hs);
+        hipLaunchKernelGGL(k_h4split_cols, dim3((N + 63) / 64, nb), dim3(256), 0, s, B, ldb, sB, hs);
         hipLaunchKernelGGL(k_gemm_h4, dim3(((M + 127) / 128) * ((N + 127) / 128), nb), dim3(512), 0, s, C, ldc, sC, hs);
         return;
     }
