@@ -312,7 +312,8 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, con
  * mixed solver (REFEFF_FMS_UPDATE_*), and the most refinement sweeps a plan
  * set's graph launches (mixed solver; energies that have settled skip the
  * rest).  Mixed plans of padded order >= 1536 start with the split-fp16
- * update, smaller ones with the FP32 3M update. */
+ * update, smaller ones with the FP32 3M update; from padded order 4096 with
+ * outer blocks of 512 factored as two halves. */
 enum {
     REFEFF_FMS_UPDATE_FP32 = 0,      /* FP32 MFMA, four real products per complex product */
     REFEFF_FMS_UPDATE_FP32_3M = 1,   /* FP32 MFMA, three real products (3M) */
@@ -324,6 +325,8 @@ typedef struct refeff_fms_tuning {
     int32_t trailing_update;
     int32_t graph_sweeps;
     double seconds_per_energy;  /* measured by refeff_fms_plan_tune (0 otherwise) */
+    int32_t lu_split;           /* 1: each outer block factored as two halves, the right half and the far
+                                   U12 rows of the right half updated by one K = half trailing update */
 } refeff_fms_tuning;
 
 int refeff_fms_plan_get_tuning(const refeff_fms_plan* plan, refeff_fms_tuning* tuning);

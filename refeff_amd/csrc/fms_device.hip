@@ -4140,10 +4140,22 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
             swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 1, n);
         }
         if (kend < n) {
-            for (int j0 = k0; j0 < kend; j0 += kPanel) {
+            // U12 of the far columns in 32-row steps; split: the left half's rows
+            // first, then one K = half update of the right half's rows, then
+            // the right half's rows
+            for (int j0 = k0; j0 < km; j0 += kPanel) {
                 swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 0, kend);
                 gemm_sub<T>(p, g, at(j0 + kPanel, kend), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, kend), n, sa,
-                            kend - j0 - kPanel, n - kend, kPanel);
+                            km - j0 - kPanel, n - kend, kPanel);
+            }
+            if (split) {
+                gemm_sub<T>(p, g, at(km, kend), n, sa, at(km, k0), n, sa, at(k0, kend), n, sa, kend - km, n - kend,
+                            half);
+                for (int j0 = km; j0 < kend; j0 += kPanel) {
+                    swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 0, kend);
+                    gemm_sub<T>(p, g, at(j0 + kPanel, kend), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, kend), n, sa,
+                                kend - j0 - kPanel, n - kend, kPanel);
+                }
             }
             const int knext = std::min(kend + kOuter, n);
             if (la && knext < n) {
@@ -5130,8 +5142,13 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     // FP32 3M update below); REFEFF_FMS_GEMM_BF16 overrides
     if (p->solver == REFEFF_FMS_SOLVER_MIXED && p->npad >= 1536) {
         p->gemm_bf16 = 4;
-        // the f16 update runs faster with a deeper K; the in-block work grows with it
-        if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_BLOCK")) p->outer_block = 384;
+        // the f16 update runs faster with a deeper K; the in-block work grows with
+        // it.  From 4096 the blocks are 512 wide and factored as two halves, so half
+        // the in-block and far-U12 K = 32 work moves to K = 256 f16 updates (as
+        // measured, energy-points/s against blocks of 384 without halves: N = 9744,
+        // 200 energies 78.4 vs 75.6; N = 5904, 150 energies 261 vs 249)
+        if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_BLOCK")) p->outer_block = 512;
+        if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_SPLIT")) p->lu_split = true;
     }
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_BF16")) {
         const int v = std::atoi(e);
@@ -5162,11 +5179,15 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         (void)hipGetLastError();
         return fail(REFEFF_FMS_DEVICE_ERROR, REFEFF_FMS_E_DEVICE, 0, 0, 0, 0.0, "plan upload");
     }
-    // batch: bounded by free device memory (70%) and the caller's limit
+    // batch: bounded by the caller's limit and by a share of free device memory,
+    // 82% unless REFEFF_FMS_MEM_SHARE gives another percentage (at 82% bcc Fe
+    // R = 12 runs its 200 energies in two batches instead of three)
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     const size_t per = bytes_per_energy(p);
-    int nb = static_cast<int>(std::max<size_t>(1, (free_b / 10 * 7) / per));
+    size_t share = 82;
+    if (const char* e = std::getenv("REFEFF_FMS_MEM_SHARE")) share = std::max(10, std::min(92, std::atoi(e)));
+    int nb = static_cast<int>(std::max<size_t>(1, (free_b / 100 * share) / per));
     nb = std::min(nb, 256);
     if (p->max_batch > 0) nb = std::min(nb, p->max_batch);
     p->batch = nb;
@@ -5479,6 +5500,13 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
         status = st_local.data();
     }
     if (device && ext) (void)hipStreamSynchronize(ext);
+    // as many batches as the batch size needs, of even size: each batch pays the
+    // latency-bound panel steps once, so a short tail batch would cost nearly
+    // what a full one does
+    if (ne > batch) {
+        const int nbatches = (ne + batch - 1) / batch;
+        batch = (ne + nbatches - 1) / nbatches;
+    }
     for (int e0 = 0; e0 < ne; e0 += batch) {
         int nb = std::min(batch, ne - e0);
         int r = arena ? REFEFF_FMS_OK : ensure_workspace(p, nb);

@@ -24,7 +24,7 @@ namespace {
 
 bool tuning_valid(const refeff_fms_tuning* t, int npad) {
     return t && t->lu_block >= kPanel && t->lu_block % kPanel == 0 && t->lu_block <= 1024 && t->lu_block <= npad &&
-           t->calu_min_rows >= 0 && t->graph_sweeps >= 1 && t->graph_sweeps <= 16 &&
+           t->calu_min_rows >= 0 && t->graph_sweeps >= 1 && t->graph_sweeps <= 16 && (t->lu_split == 0 || t->lu_split == 1) &&
            (t->trailing_update == REFEFF_FMS_UPDATE_FP32 || t->trailing_update == REFEFF_FMS_UPDATE_FP32_3M ||
             t->trailing_update == REFEFF_FMS_UPDATE_FP16X2);
 }
@@ -38,6 +38,7 @@ void apply_tuning(refeff_fms_plan* p, const refeff_fms_tuning& t) {
     p->gemm_3m = t.trailing_update != REFEFF_FMS_UPDATE_FP32;  // also the in-block (K = 32) updates
     p->gemm_bf16 = bf;
     p->graph_sweeps = t.graph_sweeps;
+    p->lu_split = t.lu_split != 0;
 }
 
 refeff_fms_tuning current_tuning(const refeff_fms_plan* p) {
@@ -48,6 +49,7 @@ refeff_fms_tuning current_tuning(const refeff_fms_plan* p) {
                         : p->gemm_3m      ? REFEFF_FMS_UPDATE_FP32_3M
                                           : REFEFF_FMS_UPDATE_FP32;
     t.graph_sweeps = p->graph_sweeps;
+    t.lu_split = p->lu_split ? 1 : 0;
     return t;
 }
 
@@ -63,8 +65,9 @@ std::vector<TuneEntry> read_tune_cache(const std::string& path) {
     if (!f) return out;
     char arch[32];
     TuneEntry e{};
-    while (std::fscanf(f, "%31s %d %d %d %d %d %d %d %lf", arch, &e.npad, &e.nrhs, &e.solver, &e.t.lu_block,
-                       &e.t.calu_min_rows, &e.t.trailing_update, &e.t.graph_sweeps, &e.t.seconds_per_energy) == 9)
+    while (std::fscanf(f, "%31s %d %d %d %d %d %d %d %d %lf", arch, &e.npad, &e.nrhs, &e.solver, &e.t.lu_block,
+                       &e.t.calu_min_rows, &e.t.trailing_update, &e.t.graph_sweeps, &e.t.lu_split,
+                       &e.t.seconds_per_energy) == 10)
         if (std::strcmp(arch, "gfx950") == 0) out.push_back(e);
     std::fclose(f);
     return out;
@@ -76,8 +79,8 @@ bool write_tune_cache(const std::string& path, const std::vector<TuneEntry>& es)
     FILE* f = std::fopen(tmp.c_str(), "w");
     if (!f) return false;
     for (const auto& e : es)
-        std::fprintf(f, "gfx950 %d %d %d %d %d %d %d %.6e\n", e.npad, e.nrhs, e.solver, e.t.lu_block, e.t.calu_min_rows,
-                     e.t.trailing_update, e.t.graph_sweeps, e.t.seconds_per_energy);
+        std::fprintf(f, "gfx950 %d %d %d %d %d %d %d %d %.6e\n", e.npad, e.nrhs, e.solver, e.t.lu_block,
+                     e.t.calu_min_rows, e.t.trailing_update, e.t.graph_sweeps, e.t.lu_split, e.t.seconds_per_energy);
     const bool ok = std::fclose(f) == 0;
     return ok && std::rename(tmp.c_str(), path.c_str()) == 0;
 }
@@ -365,19 +368,23 @@ int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const 
     const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
     for (int upd : {REFEFF_FMS_UPDATE_FP32_3M, REFEFF_FMS_UPDATE_FP16X2})
         for (int lb : {64, 128, 192, 256, 320, 384, 512})
-            for (int calu : {1024, 3072, 1 << 30}) {
-                if (upd == REFEFF_FMS_UPDATE_FP16X2 && (!mixed || p->npad < 512)) continue;
-                if (lb > 320 && upd != REFEFF_FMS_UPDATE_FP16X2) continue;  // wide blocks pay off on f16 only
-                if (lb > p->npad || (calu != 3072 && !mixed)) continue;
-                if (calu == 1024 && p->npad < 1024) continue;         // same as 3072 below 1024 rows
-                if (calu == 3072 && p->npad < 3072 && mixed) continue;  // same as never
-                refeff_fms_tuning t{};
-                t.lu_block = lb;
-                t.calu_min_rows = calu;
-                t.trailing_update = upd;
-                t.graph_sweeps = 8;
-                cands.push_back(t);
-            }
+            for (int calu : {1024, 3072, 1 << 30})
+                for (int split : {0, 1}) {
+                    if (upd == REFEFF_FMS_UPDATE_FP16X2 && (!mixed || p->npad < 512)) continue;
+                    if (lb > 320 && upd != REFEFF_FMS_UPDATE_FP16X2) continue;  // wide blocks pay off on f16 only
+                    if (lb > p->npad || (calu != 3072 && !mixed)) continue;
+                    if (calu == 1024 && p->npad < 1024) continue;         // same as 3072 below 1024 rows
+                    if (calu == 3072 && p->npad < 3072 && mixed) continue;  // same as never
+                    // halves: K = half updates on the f16 MFMA, for blocks of 384 and 512
+                    if (split && (upd != REFEFF_FMS_UPDATE_FP16X2 || lb < 384)) continue;
+                    refeff_fms_tuning t{};
+                    t.lu_block = lb;
+                    t.calu_min_rows = calu;
+                    t.trailing_update = upd;
+                    t.graph_sweeps = 8;
+                    t.lu_split = split;
+                    cands.push_back(t);
+                }
     const refeff_fms_tuning keep = current_tuning(p);
     refeff_fms_tuning win = keep;
     double best_s = 1.0e300;

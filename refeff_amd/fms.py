@@ -388,7 +388,7 @@ class LearnedPreconditioner:
 
 class Tuning(C.Structure):
     _fields_ = [("lu_block", C.c_int32), ("calu_min_rows", C.c_int32), ("trailing_update", C.c_int32),
-                ("graph_sweeps", C.c_int32), ("seconds_per_energy", C.c_double)]
+                ("graph_sweeps", C.c_int32), ("seconds_per_energy", C.c_double), ("lu_split", C.c_int32)]
 
 
 class SetStats(C.Structure):
