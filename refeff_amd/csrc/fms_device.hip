@@ -39,6 +39,7 @@ constexpr int kPanel = 32;                 // LU block width
 constexpr int kRhsChunk = 64;              // right-hand sides per solve/residual workgroup
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Cx;
@@ -2213,7 +2214,9 @@ __device__ __forceinline__ void grouped_tile(int tile, int tilesM, int tilesN, i
 // C -= s_row s_col (Re, Im) for a wave's 64 x 32 block of 4 x 2 MFMA tiles
 // (row r0 + 16 i + 4 (lane / 16) + q, column c0 + 16 j + lane % 16).  Every
 // scale and C value is loaded first, from clamped addresses, so the loads
-// share one latency; the stores skip rows / columns past M / N.
+// share one latency; the stores skip rows / columns past M / N.  C is read and
+// written with non-temporal accesses: an update touches each C value once, so
+// caching it would only evict the operand planes the other tiles reuse.
 template <int TI, int TJ>
 __device__ __forceinline__ void fp16_epilogue(float2* C, int ldc, int M, int N, int r0, int c0, int lane,
                                               const float* sa, const float* sb, const v4f (&cr)[TI][TJ],
@@ -2234,7 +2237,8 @@ __device__ __forceinline__ void fp16_epilogue(float2* C, int ldc, int M, int N, 
             for (int q = 0; q < 4; ++q) {
                 const int row = min(r0 + i * 16 + 4 * (lane >> 4) + q, M - 1);
                 const int col = min(c0 + j * 16 + (lane & 15), N - 1);
-                cv[i][j][q] = C[static_cast<size_t>(row) * ldc + col];
+                const v2f v = __builtin_nontemporal_load(reinterpret_cast<const v2f*>(C + static_cast<size_t>(row) * ldc + col));
+                cv[i][j][q] = float2{v.x, v.y};
             }
 #pragma unroll
     for (int i = 0; i < TI; ++i)
@@ -2246,8 +2250,8 @@ __device__ __forceinline__ void fp16_epilogue(float2* C, int ldc, int M, int N, 
                 const int col = c0 + j * 16 + (lane & 15);
                 if (row < M && col < N) {
                     const float sc = srow[i][q] * scol[j];
-                    C[static_cast<size_t>(row) * ldc + col] =
-                        float2{cv[i][j][q].x - sc * cr[i][j][q], cv[i][j][q].y - sc * ci[i][j][q]};
+                    const v2f v{cv[i][j][q].x - sc * cr[i][j][q], cv[i][j][q].y - sc * ci[i][j][q]};
+                    __builtin_nontemporal_store(v, reinterpret_cast<v2f*>(C + static_cast<size_t>(row) * ldc + col));
                 }
             }
 }
@@ -5785,7 +5789,7 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     // kernels 16 / 17: the fp16x2 trailing update, 3M / 4M, split kernels included
     uint16_t* split = nullptr;
     const size_t ss = (static_cast<size_t>(6) * (M + N) * K + 2 * static_cast<size_t>(M + N) + 7) / 8 * 8;
-    if ((kernel == 16 || kernel == 17) && hipMalloc(&split, ss * batch * sizeof(uint16_t)) != hipSuccess) {
+    if (kernel >= 16 && kernel <= 18 && hipMalloc(&split, ss * batch * sizeof(uint16_t)) != hipSuccess) {
         (void)hipFree(da);
         (void)hipFree(db);
         (void)hipFree(dc);
@@ -5794,8 +5798,19 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
+    // kernel 18: the fp16x2 4M product kernel alone, on planes split once before the timing
+    const HSplit4 hs4{split, ss, M, N, K};
+    if (kernel == 18) {
+        hipLaunchKernelGGL(k_h4split_rows, dim3((M + 15) / 16, batch), dim3(256), 0, nullptr,
+                           static_cast<const float2*>(da), ld, per, hs4);
+        hipLaunchKernelGGL(k_h4split_cols, dim3((N + 63) / 64, batch), dim3(256), 0, nullptr,
+                           static_cast<const float2*>(db), ld, per, hs4);
+    }
     auto one = [&]() {
-        if (kernel == 16 || kernel == 17) {
+        if (kernel == 18) {
+            hipLaunchKernelGGL(k_gemm_h4, dim3(((M + 127) / 128) * ((N + 127) / 128), batch), dim3(512), 0, nullptr,
+                               static_cast<float2*>(dc), ld, per, hs4);
+        } else if (kernel == 16 || kernel == 17) {
             gemm_bf_launch(nullptr, kernel == 16 ? 16 : 4, static_cast<float2*>(dc), ld, per,
                            static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, M, N, K,
                            batch, split, ss);
