@@ -792,8 +792,42 @@ def main():
                    if want is not None else None)
         line["accuracy"] = {"max_gg_rel_err_vs_fp64_oracle": max(errs),
                             "max_gtr_rel_err_vs_fp64_oracle": gtr_err, "energies_checked": len(sample),
-                            "gate": {"gg": 1e-8, "gtr (xmu proxy)": 1e-6}}
+                            "gate": {"gg": 1e-8, "xmu": 1e-6}, "xmu": xmu_accuracy(args.config)}
     print(json.dumps(line), flush=True)
+
+
+def xmu_accuracy(config):
+    """xmu.dat's XANES rows end to end, on the GPU (mixed LU + refinement, MKGTR
+    trace, ff2x assembly) against the FP64 oracle's solve, trace and ff2x
+    restatement: a small cluster of the workload's lattice on a FEFF-shaped
+    XANES mesh (48 horizontal + 10 contour energies: ff2x's contour correction
+    needs that shape, which the throughput workload's mesh does not have)."""
+    from oracle import ff2x_oracle as F
+    from oracle import oracle as orc
+    from refeff_amd import ff2x, fms, workloads as W
+    small = {"cfg4": lambda: W.bcc_fe(5.0, 1), "cfg5": lambda: W.alloy(5.5, 1)}.get(config, lambda: W.fcc_cu(5.2, 1))
+    wl, xs, rkk = W.xanes(small())
+    k, ph = wl.wave_numbers(), wl.phase_shifts()
+    ip0 = int(wl.potentials[0])
+    lind, bmat = fms.transition_b_matrix_averaged(1, 0, wl.global_lmax, 0, 1)
+    kw = dict(global_lmax=wl.global_lmax, raw_potential_lmax=wl.raw_lmax, direct_cutoff=wl.rdirec,
+              mean_square_displacements=wl.sigsqr(), lfms=wl.lfms)
+    plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, solver=fms.SOLVER_MIXED, **kw)
+    sc, st = plan.solve(k, ph)
+    plan.close()
+    gtr = fms.mkgtr_green_trace(sc[:, ip0], rkk, lind, bmat, wl.global_lmax, 1)
+    oplan = orc.Plan(wl.positions, wl.potentials, global_lmax=wl.global_lmax, raw_lmax=wl.raw_lmax, rdirec=wl.rdirec,
+                     sigsqr=wl.sigsqr(), lfms=wl.lfms)
+    ref, ost = oplan.spectrum(k, ph, threads=min(16, os.cpu_count() or 1))
+    gtr_ref = orc.mkgtr_trace(ref[:, ip0], rkk, lind, bmat, wl.global_lmax, 1)
+    got = ff2x.xanes_xmu(fms_trace=gtr, amplitude_reduction=0.9, **xs)
+    want = F.xanes_xmu(fms_trace=gtr_ref, amplitude_reduction=0.9, **xs)
+    progress(f"xmu leg: {wl.name}, {len(k)} energies")
+    return {"max_mu_rel_err_vs_fp64_oracle": float(np.abs(got.mu - want["mu"]).max() / np.abs(want["mu"]).max()),
+            "max_chi_rel_err_vs_fp64_oracle": float(np.abs(got.chi - want["chi"]).max() / np.abs(want["chi"]).max()),
+            "status_nonzero": int(np.count_nonzero(st)) + int(np.count_nonzero(ost)),
+            "workload": f"{wl.name} ({wl.nat} atoms, N={wl.state_count}), {len(k)} XANES energies "
+                        f"(FEFF-shaped mesh: 48 horizontal at Im 0.6 eV + 10 contour), synthetic xsect.dat arrays"}
 
 
 if __name__ == "__main__":

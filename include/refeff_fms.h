@@ -752,6 +752,54 @@ int refeff_io_ldos_inp_string(const refeff_io_ldos_inp* in, char* out, size_t ou
  * values: [pair_count]. */
 int refeff_io_sig2_dat_parse(const char* text, size_t pair_count, double* values);
 
+/* ---------------------------------------------------------------------- */
+/* ff2x: XANES rows of xmu.dat from the FMS trace                           */
+/* ---------------------------------------------------------------------- */
+
+/* The array core of ff2x's regular XANES output (ispec +-1 and +-2): the
+ * combined trace (FMS trace times S02 plus an optional path sum,
+ * ff2x.rs:2569-2603, 2677-2702), FEFF's xscorr contour correction at zero
+ * electronic temperature (ff2x_xscorr, ff2x.rs:4437-4641, with its vrcorr
+ * shift of the contour), the corrected total and atomic components
+ * (ff2x_xanes_corrected_components, :3186-3263), the output energy grid
+ * (ff2x_output_energy_grid, :4333-4400) and the xmu.dat columns with the
+ * xsedge (or absolute) normalization and the sig_gk damping
+ * (ff2x_xanes_xmu_dat_from_components, :3309-3419).  Inputs are xsect.dat's
+ * arrays in Hartree (XsectFf2xHandoff, crates/refeff-io/src/xsect_dat.rs:73-110),
+ * feff.bin's edge energy and the output momentum grid (bohr^-1), and ff2x.inp's
+ * controls.  Branches not provided (REFEFF_FMS_UNSUPPORTED): a positive
+ * electronic temperature (ff2x_thermal_xscorr), mbconv > 0, and the Lorentzian
+ * vicorr / gamma-channel convolution of a run with no listed paths
+ * (ff2x_xanes_apply_vicorr_convolution). */
+typedef struct refeff_ff2x_xanes_input {
+    int32_t energy_count;             /* ne: the horizontal grid, then the contour */
+    int32_t main_energy_count;        /* ne1 */
+    int32_t fermi_index;              /* ik0, 0-based, < ne1 */
+    const double* energy;             /* [ne] complex (re, im) energy grid, Hartree (emxs) */
+    const double* omega;              /* [ne] photon energy, Hartree (omega) */
+    const double* cross_section;      /* [ne] complex atomic cross section (xsec) */
+    const double* background;         /* [ne] normalized atomic background (xsnorm) */
+    double amplitude_reduction;       /* S02 */
+    const double* fms_trace;          /* [ne] complex FMS trace (an fms.bin spectrum), NULL = none */
+    const double* path_total;         /* [ne] complex path sum, NULL = no paths */
+    int32_t listed_paths;             /* list.dat's path count */
+    int32_t ispec, absolu, mbconv;
+    int32_t atomic_background;        /* 1: the polarization carries the atomic background */
+    int32_t gamma_channel;            /* ff2x.inp i_gamma_ch */
+    double core_hole_width;           /* Hartree (gamach) */
+    double vrcorr_ev, vicorr_ev;      /* ff2x.inp corrections, eV */
+    double sig_gk;                    /* ff2x.inp sig_gk, Angstrom */
+    double electronic_temperature_ev;
+    double edge_energy;               /* feff.bin's edge energy, Hartree */
+    int32_t output_count;             /* = ne1 */
+    const double* output_momentum;    /* [ne1] bohr^-1 */
+} refeff_ff2x_xanes_input;
+
+/* omega_ev, e_ev, k, mu, mu0, chi: [output_count] (the xmu.dat columns, k in
+ * Angstrom^-1); normalization: the divisor applied (1 with absolu). */
+int refeff_ff2x_xanes_xmu(const refeff_ff2x_xanes_input* in, double* omega_ev, double* e_ev, double* k, double* mu,
+                          double* mu0, double* chi, double* normalization, char* message, size_t message_len);
+
 /* Library version string (static storage). */
 const char* refeff_fms_version(void);
 

@@ -157,6 +157,7 @@ def lib():
     L.refeff_fms_plan_set_mode.argtypes = [vp, i32]
     L.refeff_fms_plan_profile.argtypes = [vp, i32]
     L.refeff_fms_plan_profile_read.argtypes = [vp, C.POINTER(Profile)]
+    L.refeff_ff2x_xanes_xmu.argtypes = [vp, f64p, f64p, f64p, f64p, f64p, f64p, f64p, C.c_char_p, C.c_size_t]
     _lib = L
     return L
 

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librefeff_fms.so")
 TEST_LIB = os.path.join(HERE, "librefeff_fms_test.so")
-SOURCES = ["fms_host.cpp", "fms_device.hip", "fms_debye.hip", "fms_files.cpp", "fms_io.cpp"]
+SOURCES = ["fms_host.cpp", "fms_device.hip", "fms_debye.hip", "fms_files.cpp", "fms_io.cpp", "fms_ff2x.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

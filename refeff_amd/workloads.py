@@ -161,6 +161,42 @@ def rocksalt_set(count: int = 64, ne: int = 80) -> list:
     return [rocksalt_mno(seed, ne) for seed in range(count)]
 
 
+def feff_xanes_mesh(ne1, nc, efermi, emin, emax, xloss, above=2):
+    """A XANES energy mesh of FEFF's shape (XSPH phmesh2, refeff crates/refeff-core/
+    src/xsph/mesh.rs:116-150, 745-846), in the caller's energy unit: ne1 points on
+    the horizontal line Im E = xloss from emin to emax, one of them exactly at
+    the Fermi level (index ik0), then a vertical contour at Re E = efermi rising
+    from the real axis: nc points geometrically spaced below xloss and `above`
+    points over it (ff2x's contour correction skips those; the last point's real
+    part is its Fermi energy).  Returns (energies, ik0)."""
+    horiz = np.linspace(emin, emax, ne1)
+    ik0 = int(np.argmin(np.abs(horiz - efermi)))
+    horiz[ik0] = efermi
+    heights = xloss * np.geomspace(0.02, 0.9, nc)
+    tops = xloss * (1.5 + np.arange(above))
+    vert = efermi + 1j * np.concatenate([heights, tops])
+    return np.concatenate([horiz + 1j * xloss, vert]), ik0
+
+
+def xanes(wl: Workload, ne1: int = 48, nc: int = 8, xloss_ev: float = 0.6, seed: int = 5):
+    """The workload's cluster on a FEFF-shaped XANES mesh (E_F - 6 eV .. E_F + 40 eV,
+    Im xloss), with smooth synthetic stand-ins for xsect.dat's arrays (Hartree;
+    keyword names of refeff_amd.ff2x.xanes_xmu) and seeded radial matrix elements
+    for the MKGTR trace.  Returns (workload, xsect, rkk)."""
+    ef = FERMI_ABOVE_REFERENCE_EV / HARTREE_EV
+    mesh, ik0 = feff_xanes_mesh(ne1, nc, ef, ef - 6.0 / HARTREE_EV, ef + 40.0 / HARTREE_EV, xloss_ev / HARTREE_EV)
+    w = Workload(wl.name + " XANES mesh", wl.positions, wl.potentials, wl.raw_lmax, wl.global_lmax,
+                 mesh * HARTREE_EV, wl.sigma2, wl.rdirec, wl.lfms, dict(wl.extra))
+    x = mesh.real - ef
+    xsect = dict(energy=mesh, omega=mesh.real + 330.0,
+                 cross_section=1j * (2.0 + 1.5 * np.exp(-3.0 * np.clip(x, 0, None))) + 0.1 * np.exp(-x * x),
+                 background=1.2 + 0.3 * np.tanh(4.0 * x), fermi_index=ik0, main_energy_count=ne1, edge_energy=ef,
+                 output_momentum=np.sign(x[:ne1]) * np.sqrt(2.0 * np.abs(x[:ne1])))
+    rng = np.random.default_rng(seed)
+    rkk = (rng.normal(size=(len(mesh), 8, 1)) + 1j * rng.normal(size=(len(mesh), 8, 1))) * 0.3
+    return w, xsect, rkk
+
+
 CONFIGS = {
     "cfg1": lambda: fcc_cu(5.2, 60, name="Cu fcc R=5.2 (55 atoms, N=880), 60 energies"),
     "cfg2": lambda: fcc_cu(8.0, 120, name="Cu fcc R=8.0 (177 atoms, N=2832), 120 energies"),
