@@ -2211,49 +2211,65 @@ __device__ __forceinline__ void grouped_tile(int tile, int tilesM, int tilesN, i
     tn = idx / gm;
 }
 
-// C -= s_row s_col (Re, Im) for a wave's 64 x 32 block of 4 x 2 MFMA tiles
-// (row r0 + 16 i + 4 (lane / 16) + q, column c0 + 16 j + lane % 16).  Every
-// scale and C value is loaded first, from clamped addresses, so the loads
-// share one latency; the stores skip rows / columns past M / N.  C is read and
-// written with non-temporal accesses: an update touches each C value once, so
-// caching it would only evict the operand planes the other tiles reuse.
-template <int TI, int TJ>
-__device__ __forceinline__ void fp16_epilogue(float2* C, int ldc, int M, int N, int r0, int c0, int lane,
-                                              const float* sa, const float* sb, const v4f (&cr)[TI][TJ],
-                                              const v4f (&ci)[TI][TJ]) {
-    float srow[TI][4], scol[TJ];
-    float2 cv[TI][TJ][4];
+// The C update of a wave's 64 x 32 block of 4 x 2 MFMA tiles (row
+// r0 + 16 i + 4 (lane / 16) + q, column c0 + 16 j + lane % 16): C -= s_row s_col
+// (Re, Im).  C goes through a buffer resource on the block's corner: one lane
+// offset register, the row steps in scalar offsets and the column step in the
+// instruction's immediate, so the loads can be issued early without a 64-bit
+// address each.  The resource ends just past the block's last valid element
+// (row min(M, r0 + 64) - 1, column min(N, c0 + 32) - 1): loads past it read 0,
+// every offset stays below 64 ldc 8 bytes, and the stores keep their row /
+// column mask.  Non-temporal accesses: an update touches each C value once,
+// so caching C would only evict the operand planes the other tiles reuse.
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t c_rsrc(float2* C, int ldc, int M, int N, int r0, int c0) {
+    const long long rows = min(M, r0 + 64) - r0, cols = min(N, c0 + 32) - c0;
+    const long long rec = (rows <= 0 || cols <= 0) ? 0 : ((rows - 1) * ldc + cols) * 8;
+    return __builtin_amdgcn_make_buffer_rsrc(C + static_cast<ptrdiff_t>(r0) * ldc + c0, 0, static_cast<int>(rec),
+                                             0x00020000);
+}
+
+// C values of MFMA tile rows I0 <= i < I1
+template <int I0, int I1, int TI, int TJ>
+__device__ __forceinline__ void fp16_c_load(__amdgpu_buffer_rsrc_t rs, int ldc, int lane, float2 (&cv)[TI][TJ][4]) {
+    const int vo = (4 * (lane >> 4) * ldc + (lane & 15)) * 8;
 #pragma unroll
-    for (int j = 0; j < TJ; ++j) scol[j] = sb[min(c0 + j * 16 + (lane & 15), N - 1)];
+    for (int i = I0; i < I1; ++i)
 #pragma unroll
-    for (int i = 0; i < TI; ++i)
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) srow[i][q] = sa[min(r0 + i * 16 + 4 * (lane >> 4) + q, M - 1)];
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = min(r0 + i * 16 + 4 * (lane >> 4) + q, M - 1);
-                const int col = min(c0 + j * 16 + (lane & 15), N - 1);
-                const v2f v = __builtin_nontemporal_load(reinterpret_cast<const v2f*>(C + static_cast<size_t>(row) * ldc + col));
+            for (int j = 0; j < TJ; ++j) {
+                const v2f v = __builtin_bit_cast(v2f, __builtin_amdgcn_raw_buffer_load_b64(
+                                                          rs, vo + j * 128, (i * 16 + q) * ldc * 8, 2));
                 cv[i][j][q] = float2{v.x, v.y};
             }
+}
+
+// lrow / lcol: the block's 64 row scales and 32 column scales
+template <int TI, int TJ>
+__device__ __forceinline__ void fp16_c_update(__amdgpu_buffer_rsrc_t rs, int ldc, int M, int N, int r0, int c0,
+                                              int lane, const float* lrow, const float* lcol,
+                                              const float2 (&cv)[TI][TJ][4], const v4f (&cr)[TI][TJ],
+                                              const v4f (&ci)[TI][TJ]) {
+    const int vo = (4 * (lane >> 4) * ldc + (lane & 15)) * 8;
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < TJ; ++j)
+        for (int q = 0; q < 4; ++q) {
+            const int rl = i * 16 + 4 * (lane >> 4) + q;
+            const float sr = lrow[rl];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = r0 + i * 16 + 4 * (lane >> 4) + q;
-                const int col = c0 + j * 16 + (lane & 15);
-                if (row < M && col < N) {
-                    const float sc = srow[i][q] * scol[j];
+            for (int j = 0; j < TJ; ++j) {
+                const int cl = j * 16 + (lane & 15);
+                if (r0 + rl < M && c0 + cl < N) {
+                    const float sc = sr * lcol[cl];
                     const v2f v{cv[i][j][q].x - sc * cr[i][j][q], cv[i][j][q].y - sc * ci[i][j][q]};
-                    __builtin_nontemporal_store(v, reinterpret_cast<v2f*>(C + static_cast<size_t>(row) * ldc + col));
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), rs, vo + j * 128,
+                                                          (i * 16 + q) * ldc * 8, 2);
                 }
             }
+        }
 }
 
 // C (M x N, row-major complex) -= A B from the 4M planes: 128 x 128 tile per
@@ -2348,6 +2364,23 @@ __global__ __launch_bounds__(512, 1) void k_gemm_h4(float2* Cb, int ldc, size_t 
         }
     };
     const int nk = K / 32;
+    // The tile's row and column scales wait in LDS.  Half of the wave's C
+    // block (MFMA tile rows 0 and 1) goes into the registers the operand loads
+    // have freed, at the last operand store, so its latency hides behind the
+    // last one or two steps' MFMAs (the fence keeps the scheduler from sinking
+    // the loads to their use); all of it would not fit beside the operands.
+    __shared__ float lsc[256];
+    const int wu = __builtin_amdgcn_readfirstlane(wave);  // uniform: the block's corner sits in scalar registers
+    const int er0 = m0 + (wu >> 2) * 64, ec0 = n0 + (wu & 3) * 32;
+    float2 cv[4][2][4];
+    __amdgpu_buffer_rsrc_t rs;
+    auto cload = [&]() {
+        rs = c_rsrc(C, ldc, M, N, er0, ec0);
+        fp16_c_load<0, 2>(rs, ldc, lane, cv);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    if (tid < 128) lsc[tid] = h.sa(e)[min(m0 + tid, M - 1)];
+    else if (tid < 256) lsc[tid] = h.sb(e)[min(n0 + tid - 128, N - 1)];
     Pref r0 = gload(0), r1{};
     if (nk > 1) r1 = gload(1);
     sstore(0, r0);
@@ -2366,13 +2399,16 @@ __global__ __launch_bounds__(512, 1) void k_gemm_h4(float2* Cb, int ldc, size_t 
     }
     if (kb + 1 < nk) {  // steps kb (stage 0) and kb + 1 (registers r1) left
         sstore(1, r1);
+        cload();
         compute(0);
         __syncthreads();
         compute(1);
     } else {  // step kb (stage 0) left
+        cload();
         compute(0);
     }
-    fp16_epilogue<4, 2>(C, ldc, M, N, m0 + wr * 64, n0 + wc * 32, lane, h.sa(e), h.sb(e), cr, ci);
+    fp16_c_load<2, 4>(rs, ldc, lane, cv);
+    fp16_c_update<4, 2>(rs, ldc, M, N, er0, ec0, lane, lsc + (wu >> 2) * 64, lsc + 128 + (wu & 3) * 32, cv, cr, ci);
 }
 
 // final row permutation of the LU: perm[i] = original row now at position i
@@ -5851,6 +5887,7 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     float t = 0.f;
     (void)hipEventElapsedTime(&t, a, b);
     *ms = t / iters;
+
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(da);
