@@ -313,7 +313,7 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, con
  * set's graph launches (mixed solver; energies that have settled skip the
  * rest).  Mixed plans of padded order >= 1536 start with the split-fp16
  * update, smaller ones with the FP32 3M update; from padded order 4096 with
- * outer blocks of 512 factored as two halves. */
+ * outer blocks of 512 factored by recursive halving. */
 enum {
     REFEFF_FMS_UPDATE_FP32 = 0,      /* FP32 MFMA, four real products per complex product */
     REFEFF_FMS_UPDATE_FP32_3M = 1,   /* FP32 MFMA, three real products (3M) */
@@ -325,8 +325,9 @@ typedef struct refeff_fms_tuning {
     int32_t trailing_update;
     int32_t graph_sweeps;
     double seconds_per_energy;  /* measured by refeff_fms_plan_tune (0 otherwise) */
-    int32_t lu_split;           /* 1: each outer block factored as two halves, the right half and the far
-                                   U12 rows of the right half updated by one K = half trailing update */
+    int32_t lu_split;           /* 1: each outer block factored by recursive halving (halves joined by
+                                   K = half updates, down to 32-column panels), its far U12 rows
+                                   solved the same way; 0: in 32-column steps */
 } refeff_fms_tuning;
 
 int refeff_fms_plan_get_tuning(const refeff_fms_plan* plan, refeff_fms_tuning* tuning);
@@ -441,11 +442,12 @@ enum refeff_fms_kernel_class {
     REFEFF_FMS_K_BUILD = 1,     /* G0 + (1 - G0 t) assembly */
     REFEFF_FMS_K_PANEL = 2,     /* LU panel factorisation */
     REFEFF_FMS_K_SWAP = 3,      /* row interchanges + U12 triangular solve */
-    REFEFF_FMS_K_GEMM = 4,      /* LU trailing update, K = outer block width (MFMA) */
+    REFEFF_FMS_K_GEMM = 4,      /* LU trailing update (K = outer block width) and every update on the
+                                   split-fp16 path (MFMA) */
     REFEFF_FMS_K_SOLVE = 5,     /* forward/backward substitution + diagonal-block inverses */
     REFEFF_FMS_K_RESIDUAL = 6,  /* FP64 residual */
     REFEFF_FMS_K_OTHER = 7,
-    REFEFF_FMS_K_GEMM_SMALL = 8, /* panel-width (K = 32) LU updates inside an outer block */
+    REFEFF_FMS_K_GEMM_SMALL = 8, /* narrower LU updates inside an outer block (FP32 MFMA) */
     REFEFF_FMS_K_COUNT = 9
 };
 

@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -3690,7 +3691,7 @@ struct refeff_fms_plan {
     int* d_cand = nullptr;              // tournament-pivoting candidates: two [e][calu_cap] lists
     int calu_cap = 0;                   // candidate list length per energy
     bool panel_calu = true;             // REFEFF_FMS_PANEL=gepp: FP32 panels by partial pivoting
-    bool lu_split = false;              // REFEFF_FMS_LU_SPLIT=1: outer blocks factored as two halves
+    bool lu_split = false;              // REFEFF_FMS_LU_SPLIT=1: outer blocks factored by recursive halving
     int calu_min_rows = 3072;           // tournament pivoting for panels of at least this many rows
     double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_norms = nullptr;
@@ -3700,6 +3701,10 @@ struct refeff_fms_plan {
     bool gemm_3m = true;                // REFEFF_FMS_GEMM_3M=0: FP32 trailing updates with four real products
     bool gemm_cpf = true;               // REFEFF_FMS_GEMM_CPF=0: C tile read in the epilogue
     int gemm_bf16 = 0;                  // trailing update on split operands: 4 fp16x2 4M (REFEFF_FMS_GEMM_BF16=0|2|3|4|16)
+    // REFEFF_FMS_BF_MIN_K: in-block updates of at least this K take the split path
+    // (N = 9744, 200 energies, energy-points/s: 82.2 from K = 64, 83.4 from
+    // 128, 83.7 from 256)
+    int bf_min_k = 256;
     uint16_t* d_split = nullptr;        // its operand planes, [e] split_stride bf16
     size_t split_stride = 0;
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -4057,8 +4062,10 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
     // aligned, N must be even and K a multiple of 16
     const bool v2 = sizeof(T) == 4 && (N % 2 == 0) && (K % 16 == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
                     ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) % 16 == 0);
-    // the split-bf16 path for the trailing update (K = outer block)
-    const bool bf = sizeof(T) == 4 && p->gemm_bf16 && p->d_split && K > kPanel && M % 64 == 0 && N % 64 == 0 &&
+    // the split path for the trailing update (K = outer block) and for in-block
+    // updates of at least bf_min_k
+    const bool bf = sizeof(T) == 4 && p->gemm_bf16 && p->d_split && K > kPanel &&
+                    (K >= p->bf_min_k || K >= p->outer_block) && M % 64 == 0 && N % 64 == 0 &&
                     K % 32 == 0 && K <= p->outer_block && M <= p->npad && N <= p->npad && lda % 2 == 0 &&
                     reinterpret_cast<uintptr_t>(A) % 16 == 0;
     if (bf) {
@@ -4075,7 +4082,7 @@ void gemm_sub(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* C, int l
         return;
     }
     // flops as executed: 6 M N K real for the 3M form, 8 M N K otherwise
-    launch(p, g.s, K > kPanel ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL,
+    launch(p, g.s, 2 * K >= p->outer_block ? REFEFF_FMS_K_GEMM : REFEFF_FMS_K_GEMM_SMALL,
            (v2 && p->gemm_3m ? 6.0 : 8.0) * M * N * K * nb,
            (1.0 * M * K + 1.0 * K * N + 2.0 * M * N) * cb * nb, [&] {
                const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
@@ -4141,33 +4148,49 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
                 launch_panel<T>(p, g.s, nb, g.e0, A, sa, n, ipiv, info, j0, n);
             });
         };
-        // In-block steps.  Split: the block's left half is factored first with
-        // its right half treated like the far columns (interchanges only, then
-        // its U12 rows in 32-row steps and one K = half update of the rows
-        // below), so the right half is read and written once per half instead
-        // of once per 32-column panel.
-        const int half = ((kend - k0) / 2 / kPanel) * kPanel;
-        const bool split = p->lu_split && half >= 2 * kPanel && kend - k0 - half >= kPanel;
-        const int km = split ? k0 + half : kend;
-        for (int j0 = k0; j0 < km; j0 += kPanel) {
-            panel(j0);
-            swap_trsm<T>(p, g, A, ipiv, j0, k0, km, 1, j0 + kPanel);
-            if (split) swap_trsm<T>(p, g, A, ipiv, j0, km, kend, 1, kend);
-            gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, j0 + kPanel), n,
-                        sa, n - j0 - kPanel, km - j0 - kPanel, kPanel);
-        }
-        if (split) {
-            for (int j0 = k0; j0 < km; j0 += kPanel) {
-                swap_trsm<T>(p, g, A, ipiv, j0, km, kend, 0, km);
-                gemm_sub<T>(p, g, at(j0 + kPanel, km), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, km), n, sa,
-                            km - j0 - kPanel, kend - km, kPanel);
+        // Recursive halving (lu_split): rtrsm(a, m, c0, c1) solves rows [a, m) of
+        // columns [c0, c1) with the unit-lower block L[a:m, a:m] (interchanges
+        // already applied), as two halves joined by one K = half update;
+        // rfactor(a, b, ...) factors columns [a, b) (rows a..n, every update from
+        // columns left of a applied) as its left half, the left half's U12 over
+        // the right half, one K = half update of the rows below, then the right
+        // half.  Each panel's interchanges go to every other column of the
+        // outer block at once: they commute with the updates still pending
+        // there, whose L rows move with them and whose U rows lie above.  So the
+        // trailing rows of an outer block are read and written once per level
+        // of halving instead of once per 32-column panel.
+        std::function<void(int, int, int, int)> rtrsm = [&](int a, int m, int c0, int c1) {
+            if (m - a <= kPanel) {
+                swap_trsm<T>(p, g, A, ipiv, a, c0, c1, 0, c0);
+                return;
             }
-            gemm_sub<T>(p, g, at(km, km), n, sa, at(km, k0), n, sa, at(k0, km), n, sa, n - km, kend - km, half);
-            for (int j0 = km; j0 < kend; j0 += kPanel) {
+            const int mid = a + ((m - a) / 2 / kPanel) * kPanel;
+            rtrsm(a, mid, c0, c1);
+            gemm_sub<T>(p, g, at(mid, c0), n, sa, at(mid, a), n, sa, at(a, c0), n, sa, m - mid, c1 - c0, mid - a);
+            rtrsm(mid, m, c0, c1);
+        };
+        std::function<void(int, int, int, int)> rfactor = [&](int a, int b, int blo, int bhi) {
+            if (b - a <= kPanel) {
+                panel(a);
+                swap_trsm<T>(p, g, A, ipiv, a, blo, bhi, 1, bhi);
+                return;
+            }
+            const int m = a + ((b - a) / 2 / kPanel) * kPanel;
+            rfactor(a, m, blo, bhi);
+            rtrsm(a, m, m, b);
+            gemm_sub<T>(p, g, at(m, m), n, sa, at(m, a), n, sa, at(a, m), n, sa, n - m, b - m, m - a);
+            rfactor(m, b, blo, bhi);
+        };
+        const bool split = p->lu_split && kend - k0 >= 2 * kPanel;
+        if (split) {
+            rfactor(k0, kend, k0, kend);
+        } else {
+            // right-looking in 32-column steps
+            for (int j0 = k0; j0 < kend; j0 += kPanel) {
                 panel(j0);
                 swap_trsm<T>(p, g, A, ipiv, j0, k0, kend, 1, j0 + kPanel);
-                gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, j0 + kPanel),
-                            n, sa, n - j0 - kPanel, kend - j0 - kPanel, kPanel);
+                gemm_sub<T>(p, g, at(j0 + kPanel, j0 + kPanel), n, sa, at(j0 + kPanel, j0), n, sa,
+                            at(j0, j0 + kPanel), n, sa, n - j0 - kPanel, kend - j0 - kPanel, kPanel);
             }
         }
         if (far_pending) {
@@ -4180,18 +4203,11 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
             swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 1, n);
         }
         if (kend < n) {
-            // U12 of the far columns in 32-row steps; split: the left half's rows
-            // first, then one K = half update of the right half's rows, then
-            // the right half's rows
-            for (int j0 = k0; j0 < km; j0 += kPanel) {
-                swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 0, kend);
-                gemm_sub<T>(p, g, at(j0 + kPanel, kend), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, kend), n, sa,
-                            km - j0 - kPanel, n - kend, kPanel);
-            }
+            // U12 of the far columns
             if (split) {
-                gemm_sub<T>(p, g, at(km, kend), n, sa, at(km, k0), n, sa, at(k0, kend), n, sa, kend - km, n - kend,
-                            half);
-                for (int j0 = km; j0 < kend; j0 += kPanel) {
+                rtrsm(k0, kend, kend, n);
+            } else {
+                for (int j0 = k0; j0 < kend; j0 += kPanel) {
                     swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 0, kend);
                     gemm_sub<T>(p, g, at(j0 + kPanel, kend), n, sa, at(j0 + kPanel, j0), n, sa, at(j0, kend), n, sa,
                                 kend - j0 - kPanel, n - kend, kPanel);
@@ -5183,10 +5199,10 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (p->solver == REFEFF_FMS_SOLVER_MIXED && p->npad >= 1536) {
         p->gemm_bf16 = 4;
         // the f16 update runs faster with a deeper K; the in-block work grows with
-        // it.  From 4096 the blocks are 512 wide and factored as two halves, so half
-        // the in-block and far-U12 K = 32 work moves to K = 256 f16 updates (as
-        // measured, energy-points/s against blocks of 384 without halves: N = 9744,
-        // 200 energies 78.4 vs 75.6; N = 5904, 150 energies 261 vs 249)
+        // it.  From 4096 the blocks are 512 wide and factored by recursive
+        // halving, so the in-block and far-U12 work runs as K = 256 f16 and
+        // K = 128, 64, 32 FP32 updates (energy-points/s, N = 9744, 200
+        // energies: 83.7 against 76.8 for the same blocks in 32-column steps)
         if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_BLOCK")) p->outer_block = 512;
         if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_SPLIT")) p->lu_split = true;
     }
@@ -5194,6 +5210,7 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         const int v = std::atoi(e);
         p->gemm_bf16 = (v == 2 || v == 3 || v == 4 || v == 16) ? v : 0;  // bf16x2, bf16x3, fp16x2 4M, fp16x2 3M
     }
+    if (const char* e = std::getenv("REFEFF_FMS_BF_MIN_K")) p->bf_min_k = std::max(kPanel + 1, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p->panel_calu = std::strcmp(e, "gepp") != 0;
     if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
@@ -5685,6 +5702,11 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
     refeff_fms_plan p;
     p.npad = n;
     p.n = n;
+    if (const char* e = std::getenv("REFEFF_FMS_LU_SPLIT")) p.lu_split = std::atoi(e) != 0;
+    if (const char* e = std::getenv("REFEFF_FMS_LU_BLOCK")) {
+        const int v = std::atoi(e);
+        if (v >= kPanel && v % kPanel == 0 && v <= 1024) p.outer_block = v;
+    }
     if (hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking) != hipSuccess) return REFEFF_FMS_DEVICE_ERROR;
     const size_t nn = static_cast<size_t>(n) * n;
     int rc = REFEFF_FMS_OK;

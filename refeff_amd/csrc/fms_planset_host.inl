@@ -375,8 +375,8 @@ int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const 
                     if (lb > p->npad || (calu != 3072 && !mixed)) continue;
                     if (calu == 1024 && p->npad < 1024) continue;         // same as 3072 below 1024 rows
                     if (calu == 3072 && p->npad < 3072 && mixed) continue;  // same as never
-                    // halves: K = half updates on the f16 MFMA, for blocks of 384 and 512
-                    if (split && (upd != REFEFF_FMS_UPDATE_FP16X2 || lb < 384)) continue;
+                    // recursive halving: fewer passes over the block's rows, from blocks of 192
+                    if (split && lb < 192) continue;
                     refeff_fms_tuning t{};
                     t.lu_block = lb;
                     t.calu_min_rows = calu;

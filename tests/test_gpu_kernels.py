@@ -146,6 +146,42 @@ def test_lu_float_reconstructs_matrix(n, panel, monkeypatch):
     assert np.abs(lo @ up - pa).max() <= 1e-4 * np.sqrt(n / 64) * np.abs(a).max()
 
 
+@pytest.mark.parametrize("use_double,block,panel", [(1, 160, "gepp"), (1, 512, "gepp"), (0, 160, "calu"),
+                                                    (0, 512, "calu"), (0, 224, "gepp")])
+def test_lu_recursive_halving(use_double, block, panel, monkeypatch):
+    """Outer blocks factored by recursive halving (160 = 64 + 96, then 32 + 64;
+    512 = 256 + 256 down to 32; 224 = 96 + 128), with each panel's interchanges
+    applied to the whole block at once and the solves and updates deferred: in
+    FP64 the same pivots as LAPACK zgetrf and the same factors to rounding, in
+    FP32 factors that reconstruct P A.  1088 rows leave a partial last block."""
+    import scipy.linalg.lapack as lapack
+    monkeypatch.setenv("REFEFF_FMS_LU_SPLIT", "1")
+    monkeypatch.setenv("REFEFF_FMS_LU_BLOCK", str(block))
+    monkeypatch.setenv("REFEFF_FMS_PANEL", panel)
+    n = 1088
+    print(f"[gpu-kernels] lu recursive halving n={n} block={block} double={use_double} panel={panel}", flush=True)
+    rng = np.random.default_rng(block + use_double)
+    a = crand(rng, n, n)
+    work = np.ascontiguousarray(a.copy())
+    piv = np.zeros(n, np.int32)
+    info = C.c_int32(-1)
+    assert _lib().refeff_fms_debug_lu(n, use_double, _p(work), piv.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      C.byref(info)) == 0
+    assert info.value == 0
+    if use_double:
+        ref_lu, ref_piv, ref_info = lapack.zgetrf(a)
+        assert ref_info == 0
+        assert np.array_equal(piv, ref_piv)
+        assert np.abs(work - ref_lu).max() <= 1e-10 * np.abs(ref_lu).max()
+    else:
+        lo = np.tril(work, -1) + np.eye(n)
+        up = np.triu(work)
+        pa = a.copy()
+        for i, p in enumerate(piv):
+            pa[[i, p]] = pa[[p, i]]
+        assert np.abs(lo @ up - pa).max() <= 1e-4 * np.sqrt(n / 64) * np.abs(a).max()
+
+
 @pytest.mark.parametrize("n,use_double,panel", [(9792, 0, "calu"), (9792, 0, "gepp"), (8256, 1, "gepp")])
 def test_lu_tall_panels_solve_with_small_backward_error(n, use_double, panel, monkeypatch):
     """Panels of 8192-10240 rows (FP32: two tournament selection levels, or register

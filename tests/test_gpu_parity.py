@@ -196,7 +196,7 @@ def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16, split, streams, lo
     (mixed LU + FP64 refinement vs plain FP64 LU) agree to the gg gate with the
     trailing updates in FP32, on split-bf16 MFMA (two or three parts) and on
     split-fp16 MFMA (3M and four-product forms), with the outer blocks factored
-    as two halves (both LUs), and with the energies in two stream groups and the
+    by recursive halving (both LUs), and with the energies in two stream groups and the
     far trailing update on a second stream (every group and stream with its own
     pivoting and split buffers)."""
     monkeypatch.setenv("REFEFF_FMS_GEMM_BF16", bf16)
