@@ -1041,70 +1041,108 @@ __global__ __launch_bounds__(1024) void k_panel_rb1(typename Cx<T>::type* Ab, si
 // factorisation is what sets the final accuracy.
 constexpr int kSelRows = 512;  // rows per selection workgroup, one per thread
 
+// Wave-wide maximum: DPP within rows of 16 lanes, then the four row maxima
+// read lane by lane.
+__device__ __forceinline__ float wave_max(float x) {
+    auto dpp = [](float y, int ctrl) -> float {
+        switch (ctrl) {  // the control word must be a constant
+            case 0: return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, y), __builtin_bit_cast(int, y), 0xB1, 0xf, 0xf, false));
+            case 1: return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, y), __builtin_bit_cast(int, y), 0x4E, 0xf, 0xf, false));
+            case 2: return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, y), __builtin_bit_cast(int, y), 0x141, 0xf, 0xf, false));
+            default: return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, y), __builtin_bit_cast(int, y), 0x140, 0xf, 0xf, false));
+        }
+    };
+    x = fmaxf(x, dpp(x, 0));  // quad_perm [1, 0, 3, 2]
+    x = fmaxf(x, dpp(x, 1));  // quad_perm [2, 3, 0, 1]
+    x = fmaxf(x, dpp(x, 2));  // row_half_mirror
+    x = fmaxf(x, dpp(x, 3));  // row_mirror: every lane holds its row's maximum
+    auto lane_of = [&](int l) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l)); };
+    return fmaxf(fmaxf(lane_of(0), lane_of(16)), fmaxf(lane_of(32), lane_of(48)));
+}
+
+// Reciprocal of a nonzero complex pivot through a power-of-two scale and the
+// hardware reciprocal of |d|^2 (about 1 ulp): selection workgroups only
+// choose rows, so their multipliers need not be correctly rounded.
+__device__ __forceinline__ float2 crcp_fast(float2 d) {
+    const int ex = __builtin_amdgcn_frexp_expf(fmaxf(fabsf(d.x), fabsf(d.y)));
+    const float ax = __builtin_amdgcn_ldexpf(d.x, -ex), ay = __builtin_amdgcn_ldexpf(d.y, -ex);
+    const float r = __builtin_amdgcn_rcpf(ax * ax + ay * ay);
+    return make_float2(__builtin_amdgcn_ldexpf(ax * r, -ex), __builtin_amdgcn_ldexpf(-ay * r, -ex));
+}
+
+// LDS of one partial-pivoting pass: per column (double-buffered by column
+// parity) each wave's candidate row and its |re|+|im| and thread.
+struct CaluLds {
+    float2 cand[2][kSelRows / 64][kPanel];
+    float4 best[2][2];
+    int4 idx[2][2];
+};
+
 // Partial pivoting over the rows held one per thread (v, valid), without
 // moving them: each column's pivot is the first largest |re|+|im| among the
 // rows not yet chosen.  win[c] = thread chosen for column c (-1 when no row is
 // left); fact[c] = that row's factored values (multipliers left of c, U right
 // of it) when fact is given; zero_col = first column whose pivot is zero.
-template <typename T>
+// Per column, each wave finds its first largest row (DPP maximum, lowest lane
+// by ballot) and publishes it to LDS; after one barrier every thread takes
+// the first wave holding the largest and reads that row.  FAST: multipliers
+// through crcp_fast instead of a division per row.  FP32 only.
+template <typename T, bool FAST>
 __device__ __forceinline__ void calu_gepp(typename Cx<T>::type (&v)[kPanel], bool valid, int* win,
-                                          typename Cx<T>::type (*fact)[kPanel + 1], typename Cx<T>::type* prow,
-                                          T* wbest, int* widx, int* zero_col) {
-    using T2 = typename Cx<T>::type;
+                                          typename Cx<T>::type (*fact)[kPanel + 1], CaluLds& s, int* zero_col) {
+    static_assert(sizeof(T) == 4, "tournament pivoting runs on FP32 panels");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int NW = kSelRows / 64;
+    static_assert(kSelRows == 512, "eight waves");
     bool used = false;
     if (tid == 0) *zero_col = -1;
 #pragma unroll
     for (int c = 0; c < kPanel; ++c) {
-        // first largest |re|+|im| (lowest thread on ties): per wave by
-        // shuffles, then every thread reduces the per-wave results itself
-        T best = (valid && !used) ? fabs(v[c].x) + fabs(v[c].y) : T(-1);
-        int bi = tid;
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ob = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ob > best || (ob == best && oi < bi)) {
-                best = ob;
-                bi = oi;
-            }
+        const int buf = c & 1;
+        const float best = (valid && !used) ? fabsf(v[c].x) + fabsf(v[c].y) : -1.f;
+        const float m = wave_max(best);
+        const unsigned long long hit = __ballot(best == m);  // empty for a NaN maximum
+        const int wl = hit ? __builtin_ctzll(hit) : 0;
+        if (lane == wl) {
+#pragma unroll
+            for (int k = c; k < kPanel; ++k) s.cand[buf][wave][k] = v[k];
         }
         if (lane == 0) {
-            wbest[wave] = best;
-            widx[wave] = bi;
+            reinterpret_cast<float*>(s.best[buf])[wave] = hit ? m : -1.f;
+            reinterpret_cast<int*>(s.idx[buf])[wave] = wave * 64 + wl;
         }
         __syncthreads();
-        T pb = wbest[0];
-        int pt = widx[0];
+        const float4 b0 = s.best[buf][0], b1 = s.best[buf][1];
+        const int4 i0 = s.idx[buf][0], i1 = s.idx[buf][1];
+        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+        const int ii[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+        float pb = bb[0];
+        int pt = ii[0];
 #pragma unroll
-        for (int w = 1; w < NW; ++w)
-            if (wbest[w] > pb) {
-                pb = wbest[w];
-                pt = widx[w];
+        for (int w = 1; w < 8; ++w)
+            if (bb[w] > pb) {
+                pb = bb[w];
+                pt = ii[w];
             }
         if (!(pb >= 0)) {  // no row left (or a NaN pivot: treated as no pivot)
             if (tid == 0) {
                 win[c] = -1;
                 if (*zero_col < 0) *zero_col = c;
             }
-            __syncthreads();  // every thread has read wbest before it is rewritten
-            continue;
+            continue;  // the next column writes the other buffer
         }
         if (tid == pt) {
             used = true;
-#pragma unroll
-            for (int k = 0; k < kPanel; ++k) prow[k] = v[k];
             if (fact)
 #pragma unroll
                 for (int k = 0; k < kPanel; ++k) fact[c][k] = v[k];
             win[c] = pt;
             if (!(pb > 0) && *zero_col < 0) *zero_col = c;
         }
-        __syncthreads();
         if (valid && !used) {
-            const T2 d = prow[c];
+            const float2* prow = s.cand[buf][pt >> 6];
             const bool nz = pb > 0;
-            const T2 l = nz ? cdiv(v[c], d) : v[c];
+            float2 l = v[c];
+            if (nz) l = FAST ? cmul(v[c], crcp_fast(prow[c])) : cdiv(v[c], prow[c]);
             v[c] = l;
 #pragma unroll
             for (int k = c + 1; k < kPanel; ++k) v[k] = cfms(v[k], l, prow[k]);
@@ -1137,9 +1175,7 @@ __global__ __launch_bounds__(kSelRows) void k_calu_select(const typename Cx<T>::
                                                           int n, const int* in_rows, int in_stride, int in_count,
                                                           int* out_rows, int out_stride) {
     using T2 = typename Cx<T>::type;
-    __shared__ T2 prow[kPanel];
-    __shared__ T wbest[kSelRows / 64];
-    __shared__ int widx[kSelRows / 64];
+    __shared__ CaluLds cl;
     __shared__ int win[kPanel];
     __shared__ int zero_col;
     const int g = blockIdx.x, e = blockIdx.y;
@@ -1147,7 +1183,7 @@ __global__ __launch_bounds__(kSelRows) void k_calu_select(const typename Cx<T>::
     T2 v[kPanel];
     const int r = calu_load<T>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
                                in_count, g, v);
-    calu_gepp<T>(v, r >= 0, win, nullptr, prow, wbest, widx, &zero_col);
+    calu_gepp<T, true>(v, r >= 0, win, nullptr, cl, &zero_col);
     __syncthreads();
     // winners by column, as global rows (the thread that won column c holds r)
     __shared__ int rowof[kSelRows];
@@ -1164,9 +1200,7 @@ __global__ __launch_bounds__(kSelRows) void k_calu_final(typename Cx<T>::type* A
                                                          const int* in_rows, int in_stride, int in_count, int* ipivb,
                                                          int* info) {
     using T2 = typename Cx<T>::type;
-    __shared__ T2 prow[kPanel];
-    __shared__ T wbest[kSelRows / 64];
-    __shared__ int widx[kSelRows / 64];
+    __shared__ CaluLds cl;
     __shared__ int win[kPanel];
     __shared__ int zero_col;
     __shared__ T2 fact[kPanel][kPanel + 1];
@@ -1180,7 +1214,7 @@ __global__ __launch_bounds__(kSelRows) void k_calu_final(typename Cx<T>::type* A
     T2 v[kPanel];
     const int r = calu_load<T>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
                                in_count, 0, v);
-    calu_gepp<T>(v, r >= 0, win, fact, prow, wbest, widx, &zero_col);
+    calu_gepp<T, false>(v, r >= 0, win, fact, cl, &zero_col);
     rowof[tid] = r;
     for (int i = tid; i < kPanel * kPanel; i += kSelRows)
         orig[i / kPanel][i % kPanel] = A[static_cast<size_t>(j0 + i / kPanel) * lda + j0 + i % kPanel];
