@@ -197,46 +197,64 @@ struct BuildArgs {
     double toler2;   // G0 magnitude cutoff of the iterative system (GG mode)
 };
 
-// FEFF rotxan small-d recursion into d0[l][n][m] (1-based n, m), restated from
-// refeff fms/internals/rotation.rs:3-71 in double precision.
+// FEFF rotxan small-d recursion into d0[t][l][n][m] (1-based n, m) for the
+// two tables t = 0 (+beta) and t = 1 (-beta), restated from refeff
+// fms/internals/rotation.rs:3-71 in double precision.  The entries of one
+// level l depend only on level l - 1, so each level's (n, m) entries are
+// formed by the whole workgroup at once (the same expressions as the
+// sequential loop, so the same values), then the four corner entries that
+// copy level-l entries, with a barrier after each step.  Every thread of the
+// workgroup must call it; d0 must be zeroed.
 template <int S>
-__device__ void small_d(int lmax, double beta, double* d0) {
-    auto D = [&](int l, int n, int m) -> double& { return d0[(l * S + n) * S + m]; };
+__device__ void small_d(int lmax, double beta, double* d0, int tsize, double (&cs)[2][3]) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto D = [&](int t, int l, int n, int m) -> double& { return d0[t * tsize + (l * S + n) * S + m]; };
     const int ndm = 2 * lmax + 1;
-    const double xc = cos(0.5 * beta), xs = sin(0.5 * beta), s = sin(beta);
-    D(1, 1, 1) = 1.0;
-    if (lmax < 1) return;
-    D(2, 1, 1) = xc * xc;
-    D(2, 1, 2) = s / sqrt(2.0);
-    D(2, 1, 3) = xs * xs;
-    D(2, 2, 1) = -D(2, 1, 2);
-    D(2, 2, 2) = cos(beta);
-    D(2, 2, 3) = D(2, 1, 2);
-    D(2, 3, 1) = D(2, 1, 3);
-    D(2, 3, 2) = -D(2, 2, 3);
-    D(2, 3, 3) = D(2, 1, 1);
+    if (tid < 2) {
+        const double bt = tid == 0 ? beta : -beta;
+        const double xc = cos(0.5 * bt), xs = sin(0.5 * bt), sb = sin(bt);
+        cs[tid][0] = xc;
+        cs[tid][1] = xs;
+        cs[tid][2] = sb;
+        D(tid, 1, 1, 1) = 1.0;
+        if (lmax >= 1) {
+            D(tid, 2, 1, 1) = xc * xc;
+            D(tid, 2, 1, 2) = sb / sqrt(2.0);
+            D(tid, 2, 1, 3) = xs * xs;
+            D(tid, 2, 2, 1) = -D(tid, 2, 1, 2);
+            D(tid, 2, 2, 2) = cos(bt);
+            D(tid, 2, 2, 3) = D(tid, 2, 1, 2);
+            D(tid, 2, 3, 1) = D(tid, 2, 1, 3);
+            D(tid, 2, 3, 2) = -D(tid, 2, 2, 3);
+            D(tid, 2, 3, 3) = D(tid, 2, 1, 1);
+        }
+    }
+    __syncthreads();
     for (int l = 3; l <= lmax + 1; ++l) {
         const int ln = min(2 * l - 1, ndm), lm = min(2 * l - 3, ndm);
-        for (int nn = 1; nn <= ln; ++nn) {
-            for (int mm = 1; mm <= lm; ++mm) {
-                const double t1 = (double)((2 * l - 1 - nn) * (2 * l - 2 - nn));
-                const double t = (double)((2 * l - 1 - mm) * (2 * l - 2 - mm));
-                const double f1 = sqrt(t1 / t);
-                const double f2 = sqrt((double)((2 * l - 1 - nn) * (nn - 1)) / t);
-                const double f3 = sqrt((double)((nn - 2) * (nn - 1)) / t);
-                double v = f1 * xc * xc * D(l - 1, nn, mm);
-                if (nn > 1) v -= f2 * s * D(l - 1, nn - 1, mm);
-                if (nn > 2) v += f3 * xs * xs * D(l - 1, nn - 2, mm);
-                D(l, nn, mm) = v;
-                if (nn > 2 * l - 3) D(l, mm, nn) = (((nn - mm) & 1) ? -1.0 : 1.0) * v;
-            }
-            if (nn > 2 * l - 3) {
-                D(l, 2 * l - 2, 2 * l - 2) = D(l, 2, 2);
-                D(l, 2 * l - 1, 2 * l - 2) = -D(l, 1, 2);
-                D(l, 2 * l - 2, 2 * l - 1) = -D(l, 2, 1);
-                D(l, 2 * l - 1, 2 * l - 1) = D(l, 1, 1);
-            }
+        for (int i = tid; i < 2 * ln * lm; i += nt) {
+            const int t = i / (ln * lm), q = i - t * ln * lm;
+            const int nn = q / lm + 1, mm = q % lm + 1;
+            const double xc = cs[t][0], xs = cs[t][1], sb = cs[t][2];
+            const double t1 = (double)((2 * l - 1 - nn) * (2 * l - 2 - nn));
+            const double tt = (double)((2 * l - 1 - mm) * (2 * l - 2 - mm));
+            const double f1 = sqrt(t1 / tt);
+            const double f2 = sqrt((double)((2 * l - 1 - nn) * (nn - 1)) / tt);
+            const double f3 = sqrt((double)((nn - 2) * (nn - 1)) / tt);
+            double v = f1 * xc * xc * D(t, l - 1, nn, mm);
+            if (nn > 1) v -= f2 * sb * D(t, l - 1, nn - 1, mm);
+            if (nn > 2) v += f3 * xs * xs * D(t, l - 1, nn - 2, mm);
+            D(t, l, nn, mm) = v;
+            if (nn > 2 * l - 3) D(t, l, mm, nn) = (((nn - mm) & 1) ? -1.0 : 1.0) * v;
         }
+        __syncthreads();
+        if (tid < 2 && ln > 2 * l - 3) {
+            D(tid, l, 2 * l - 2, 2 * l - 2) = D(tid, l, 2, 2);
+            D(tid, l, 2 * l - 1, 2 * l - 2) = -D(tid, l, 1, 2);
+            D(tid, l, 2 * l - 2, 2 * l - 1) = -D(tid, l, 2, 1);
+            D(tid, l, 2 * l - 1, 2 * l - 1) = D(tid, l, 1, 1);
+        }
+        __syncthreads();
     }
 }
 
@@ -245,21 +263,24 @@ __device__ void small_d(int lmax, double beta, double* d0) {
 // B64 the rows of the representative blocks, B[j][rhs + r] = G0[rep + r, j].
 // LM: largest l this instance serves (LDS tables are sized by it); E: energies
 // whose Rehr-Albers polynomials, z-axis propagators and G0 cells are formed
-// together in one pass (one thread per energy and spin for the recursion).
-template <bool W32, bool W64, bool GG, int LM, int E>
+// together in one pass (one thread per energy and spin for the recursion);
+// NS: spin channels (g.nsp).
+template <bool W32, bool W64, bool GG, int LM, int E, int NS>
 __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
     constexpr int S = 2 * LM + 2;
     constexpr int LP = LM + 1;
     constexpr int NROT = LP * (4 * LP * LP - 1) / 3;  // rot_base(LM + 1)
     __shared__ double d0[2][(LM + 2) * S * S];
     __shared__ double2 rf[NROT], rb[NROT];
-    __shared__ double2 clm[E][2][(LM + 2) * LP];
-    __shared__ double2 gl[E][2][LP * LP * LP];
-    __shared__ double2 pref[E][2];
+    __shared__ double2 clm[E][NS][(LM + 2) * LP];
+    __shared__ double2 gl[E][NS][LP * LP * LP];
+    __shared__ double2 pref[E][NS];
     __shared__ double xn[LP * LP];
+    __shared__ double2 tms[GG ? 1 : E][NS][NS * LP * LP];  // t of the pass's energies at this block's columns
 
     const int b = blockIdx.x, a = blockIdx.y, tid = threadIdx.x;
-    const int L = g.L, nsp = g.nsp;
+    const int L = g.L;
+    constexpr int nsp = NS;
     const int la = g.atom_lmax[a], lb = g.atom_lmax[b];
     const int fa = g.atom_first[a], fb = g.atom_first[b];
     const int na = (la + 1) * (la + 1) * nsp, nbk = (lb + 1) * (lb + 1) * nsp;
@@ -313,9 +334,9 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
         xn[i] = g.xnlm[mu * (L + 1) + l];
     }
     __syncthreads();
-    if (tid == 0) small_d<S>(L, beta, d0[0]);          // forward table, +beta
-    if (tid == 64) small_d<S>(L, -beta, d0[1]);        // backward table, -beta (rotation.rs:60-77)
-    __syncthreads();
+    // forward table +beta, backward table -beta (rotation.rs:60-77)
+    __shared__ double dcs[2][3];
+    small_d<S>(L, beta, d0[0], (LM + 2) * S * S, dcs);
     const int nrot = rot_base(L + 1);
     for (int i = tid; i < nrot; i += blockDim.x) {
         int ll = 0;
@@ -379,6 +400,14 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
             }
             pref[ec][s] = pf;
         }
+        if (!GG) {
+            // t at the block's columns (and the spin partner's), staged so the
+            // cell loop below does not wait on a global load per energy
+            for (int i = tid; i < ne * NS * nbk; i += blockDim.x) {
+                const int ec = i / (NS * nbk), q = i - ec * NS * nbk, tt = q / nbk, c = q - tt * nbk;
+                tms[ec][tt][c] = g.tmat[static_cast<size_t>(e0 + ec) * 2 * g.n + static_cast<size_t>(tt) * g.n + fb + c];
+            }
+        }
         __syncthreads();
         // z-axis propagator g_mu(l1, l2) with the prefactor folded in (xgllm)
         const int per = nsp * L1 * L1 * L1;
@@ -437,7 +466,6 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
                 if (proj > -l2 + 1 && proj < l2 + 2) cp = (sc == 0) ? c - 1 : c + 1;
                 if (cp >= 0 && cp % nsp != sr) cp = -1;
             }
-            const int cs = fb + c;
             const size_t row_at = static_cast<size_t>(fa + r) * g.lda + fb + c;
             for (int ec = 0; ec < ne; ++ec) {
                 const int e = e0 + ec;
@@ -454,8 +482,7 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
                     if (rhs >= 0) g.b64[static_cast<size_t>(e) * g.sb + static_cast<size_t>(fb + c) * g.nrhs + rhs + r] = gv;
                     continue;
                 }
-                const double2* tm0 = g.tmat + static_cast<size_t>(e) * 2 * g.n;
-                double2 v = cmul(gv, tm0[cs]);
+                double2 v = cmul(gv, tms[ec][0][c]);
                 v = make_double2(-v.x, -v.y);
                 if (cp >= 0) {
                     const int lpc = cp / nsp, lp2 = isqrt_small(lpc), mp2 = lpc - lp2 * lp2 - lp2;
@@ -467,7 +494,7 @@ __global__ __launch_bounds__(256) void k_build(BuildArgs g) {
                         const double2 df = rf[rot_base(lp2) + (mp2 + lp2) * (2 * lp2 + 1) + (mu + lp2)];
                         gp = cmac(gp, cmul(db, glr[(amu * L1 + l1) * L1 + lp2]), df);
                     }
-                    v = csub(v, cmul(gp, tm0[g.n + cs]));
+                    v = csub(v, cmul(gp, tms[ec][1][c]));
                 }
                 if (W32) g.a32[at] = make_float2((float)v.x, (float)v.y);
                 if (W64) g.a64[at] = v;
@@ -4270,15 +4297,20 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
 }
 
 // G0 build launch: instance by the largest l (LDS tables) and energies per pass.
-template <bool W32, bool W64, bool GG>
-void launch_build(hipStream_t s, const BuildArgs& ba, int nat, int L) {
+template <bool W32, bool W64, bool GG, int NS>
+void launch_build_ns(hipStream_t s, const BuildArgs& ba, int nat, int L) {
     const dim3 grid(nat, nat), block(256);
     if (L <= 3)
-        hipLaunchKernelGGL((k_build<W32, W64, GG, 3, 16>), grid, block, 0, s, ba);
+        hipLaunchKernelGGL((k_build<W32, W64, GG, 3, 16, NS>), grid, block, 0, s, ba);
     else if (L <= 4)
-        hipLaunchKernelGGL((k_build<W32, W64, GG, 4, 8>), grid, block, 0, s, ba);
+        hipLaunchKernelGGL((k_build<W32, W64, GG, 4, 8, NS>), grid, block, 0, s, ba);
     else
-        hipLaunchKernelGGL((k_build<W32, W64, GG, kMaxL, 2>), grid, block, 0, s, ba);
+        hipLaunchKernelGGL((k_build<W32, W64, GG, kMaxL, 2, NS>), grid, block, 0, s, ba);
+}
+template <bool W32, bool W64, bool GG>
+void launch_build(hipStream_t s, const BuildArgs& ba, int nat, int L) {
+    if (ba.nsp == 2) launch_build_ns<W32, W64, GG, 2>(s, ba, nat, L);
+    else launch_build_ns<W32, W64, GG, 1>(s, ba, nat, L);
 }
 
 // Inverse diagonal blocks of a fresh LU factorisation (used by every solve).
