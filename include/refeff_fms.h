@@ -313,7 +313,7 @@ int refeff_fms_solve_batch_full(refeff_fms_plan* plan, int32_t energy_count, con
  * set's graph launches (mixed solver; energies that have settled skip the
  * rest).  Mixed plans of padded order >= 1536 start with the split-fp16
  * update, smaller ones with the FP32 3M update; from padded order 4096 with
- * outer blocks of 512 factored by recursive halving. */
+ * outer blocks of 512 (1024 from 8192) factored by recursive halving. */
 enum {
     REFEFF_FMS_UPDATE_FP32 = 0,      /* FP32 MFMA, four real products per complex product */
     REFEFF_FMS_UPDATE_FP32_3M = 1,   /* FP32 MFMA, three real products (3M) */

@@ -5265,11 +5265,13 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
     if (p->solver == REFEFF_FMS_SOLVER_MIXED && p->npad >= 1536) {
         p->gemm_bf16 = 4;
         // the f16 update runs faster with a deeper K; the in-block work grows with
-        // it.  From 4096 the blocks are 512 wide and factored by recursive
-        // halving, so the in-block and far-U12 work runs as K = 256 f16 and
-        // K = 128, 64, 32 FP32 updates (energy-points/s, N = 9744, 200
-        // energies: 83.7 against 76.8 for the same blocks in 32-column steps)
-        if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_BLOCK")) p->outer_block = 512;
+        // it.  From 4096 the blocks are 512 wide (1024 from 8192) and factored by
+        // recursive halving, so the in-block and far-U12 work runs as K = 512,
+        // 256 f16 and K = 128, 64, 32 FP32 updates (energy-points/s: N = 9744,
+        // 200 energies, blocks of 512 83.7 against 76.8 in 32-column steps,
+        // blocks of 1024 88.9 against 87.0 for 512; N = 5904, 150 energies,
+        // 296.6 for 512 against 295 for 1024)
+        if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_BLOCK")) p->outer_block = p->npad >= 8192 ? 1024 : 512;
         if (p->npad >= 4096 && !std::getenv("REFEFF_FMS_LU_SPLIT")) p->lu_split = true;
     }
     if (const char* e = std::getenv("REFEFF_FMS_GEMM_BF16")) {

@@ -367,7 +367,7 @@ int refeff_fms_plan_tune(refeff_fms_plan* p, int32_t ne, const double* k, const 
     std::vector<refeff_fms_tuning> cands;
     const bool mixed = p->solver == REFEFF_FMS_SOLVER_MIXED;
     for (int upd : {REFEFF_FMS_UPDATE_FP32_3M, REFEFF_FMS_UPDATE_FP16X2})
-        for (int lb : {64, 128, 192, 256, 320, 384, 512})
+        for (int lb : {64, 128, 192, 256, 320, 384, 512, 1024})
             for (int calu : {1024, 3072, 1 << 30})
                 for (int split : {0, 1}) {
                     if (upd == REFEFF_FMS_UPDATE_FP16X2 && (!mixed || p->npad < 512)) continue;
