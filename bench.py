@@ -90,7 +90,91 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 1 GiB device-to-device copy (the reference kernel for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling under rocprofv3 --pmc)")
+    ap.add_argument("--lu-traffic", action="store_true",
+                    help="for counter collection under rocprofv3 --pmc: the 1 GiB calibration copy, then one LU "
+                         "of a random matrix of the workload's padded order through the test library, with the "
+                         "plan's LU settings (one energy of the benchmark's update sequence); prints the "
+                         "sequence of split-path updates")
     return ap.parse_args()
+
+
+def split_path_updates(n, outer_block, lu_split, bf_min_k=256, panel=32):
+    """(M, N, K) of the LU's updates on the split-fp16 path, in launch order: a
+    replay of run_lu's update sequence (recursive halving of each outer block,
+    the far U12 rows solved the same way, then the trailing update; without
+    lookahead) and of gemm_sub's choice of the split path (K above the panel
+    width and at least bf_min_k or the outer block, M and N multiples of 64)."""
+    out = []
+
+    def gemm(M, N, K):
+        if M > 0 and N > 0 and K > panel and (K >= bf_min_k or K >= outer_block) and M % 64 == 0 and \
+                N % 64 == 0 and K % 32 == 0 and K <= outer_block:
+            out.append((M, N, K))
+
+    def rtrsm(a, m, c0, c1):
+        if m - a <= panel:
+            return
+        mid = a + ((m - a) // 2 // panel) * panel
+        rtrsm(a, mid, c0, c1)
+        gemm(m - mid, c1 - c0, mid - a)
+        rtrsm(mid, m, c0, c1)
+
+    def rfactor(a, b):
+        if b - a <= panel:
+            return
+        m = a + ((b - a) // 2 // panel) * panel
+        rfactor(a, m)
+        rtrsm(a, m, m, b)
+        gemm(n - m, b - m, m - a)
+        rfactor(m, b)
+
+    for k0 in range(0, n, outer_block):
+        kend = min(k0 + outer_block, n)
+        split = lu_split and kend - k0 >= 2 * panel
+        if split:
+            rfactor(k0, kend)
+        if kend < n:
+            if split:
+                rtrsm(k0, kend, kend, n)
+            gemm(n - kend, n - kend, kend - k0)
+    return out
+
+
+def run_lu_traffic(args):
+    """One LU of the workload's padded order with the plan's settings, through
+    the test library's single-matrix hook, after the calibration copy: the
+    split-path dispatches under rocprofv3 --pmc are one energy of the
+    benchmark's trailing-update launches."""
+    from refeff_amd import _lib as rl, fms, lib, workloads as W
+    wl = W.CONFIGS[args.config]()
+    plan = fms.FmsRealSpacePlan(wl.positions, wl.potentials, global_lmax=wl.global_lmax,
+                                raw_potential_lmax=wl.raw_lmax, direct_cutoff=wl.rdirec, lfms=wl.lfms,
+                                solver=1 if args.solver == "mixed" else 2)
+    tun = fms.Tuning()
+    rl.check(lib().refeff_fms_plan_get_tuning(plan.handle, C.byref(tun)), "plan_get_tuning")
+    n, batch = plan.info.padded_order, plan.info.batch
+    plan.close()
+    os.environ["REFEFF_FMS_LU_BLOCK"] = str(tun.lu_block)
+    os.environ["REFEFF_FMS_LU_SPLIT"] = str(int(tun.lu_split))
+    os.environ["REFEFF_FMS_GEMM_BF16"] = "4" if tun.trailing_update == 4 else "0"
+    ups = split_path_updates(n, tun.lu_block, bool(tun.lu_split)) if tun.trailing_update == 4 else []
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = np.ascontiguousarray(a / np.sqrt(n) + 2.0 * np.eye(n))
+    T = rl.test_lib()
+    T.refeff_fms_debug_lu.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32)]
+    copy_calibration()
+    piv = np.zeros(n, np.int32)
+    info = C.c_int32(-1)
+    rc = T.refeff_fms_debug_lu(n, 0, a.ctypes.data_as(C.POINTER(C.c_double)), piv.ctypes.data_as(C.POINTER(C.c_int32)),
+                               C.byref(info))
+    rl.check(rc, "debug_lu")
+    alg = [(M * K + K * N + 2 * M * N) * 8 for (M, N, K) in ups]
+    print(json.dumps({"lu_traffic": wl.name, "padded": n, "lu_block": tun.lu_block, "lu_split": int(tun.lu_split),
+                      "trailing_update": tun.trailing_update, "info": info.value, "plan_batch": batch,
+                      "split_path_updates": len(ups), "updates": ups,
+                      "algorithmic_bytes_per_update_one_energy": sum(alg) / max(1, len(alg))}))
 
 
 GEMM_CLASS = 4  # REFEFF_FMS_K_GEMM: the LU trailing update
@@ -705,6 +789,9 @@ def main():
         return
     if args.learned:
         run_learned(args)
+        return
+    if args.lu_traffic:
+        run_lu_traffic(args)
         return
     if args.copy_calibration:
         copy_calibration()

@@ -5793,6 +5793,15 @@ int refeff_fms_debug_lu(int32_t n, int32_t use_double, double* a, int32_t* ipiv,
         if (!dupload(&p.d_a32, af.data(), nn)) rc = REFEFF_FMS_DEVICE_ERROR;
         if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p.panel_calu = std::strcmp(e, "gepp") != 0;
         if (!rc && p.panel_calu && !alloc_calu(&p, 1)) rc = REFEFF_FMS_DEVICE_ERROR;
+        // REFEFF_FMS_GEMM_BF16=4: trailing and wide in-block updates on the split-fp16 path
+        if (const char* e = std::getenv("REFEFF_FMS_GEMM_BF16")) {
+            const int v = std::atoi(e);
+            p.gemm_bf16 = (v == 2 || v == 3 || v == 4 || v == 16) ? v : 0;
+        }
+        if (!rc && p.gemm_bf16) {
+            p.split_stride = split_slots(&p) * split_slot_halves(&p);
+            if (!dalloc(&p.d_split, p.split_stride)) rc = REFEFF_FMS_DEVICE_ERROR;
+        }
         if (!rc) run_lu<float>(&p, Group{p.stream, 0, 1}, p.d_a32, p.d_ipiv, p.d_perm, p.d_info);
         if (!rc && hipStreamSynchronize(p.stream) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
         if (!rc && hipMemcpy(af.data(), p.d_a32, nn * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess) rc = REFEFF_FMS_DEVICE_ERROR;
