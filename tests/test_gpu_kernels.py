@@ -55,6 +55,14 @@ def test_gemm_split_bf16(kind, tol, m, n, k):
     _gemm_case(kind, m, n, k, tol)
 
 
+@pytest.mark.parametrize("m,n,k", [(3072, 3072, 256), (3008, 3136, 128), (2944, 3072, 192)])
+def test_gemm_split_fp16_large(m, n, k):
+    """The four-product split-fp16 update over more tiles than the GPU has CUs
+    (576 and more), with partial tiles at the right and bottom edges and the
+    half of each wave's C block that is loaded during the last two K steps."""
+    _gemm_case(15, m, n, k, 2e-5)
+
+
 @pytest.mark.parametrize("kind", [14, 15])
 def test_gemm_split_fp16_keeps_scaled_rows_and_columns(kind):
     """Rows of A and columns of B scaled over six decades: with a power-of-two
