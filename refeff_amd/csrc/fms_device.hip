@@ -2495,7 +2495,59 @@ __global__ __launch_bounds__(256) void k_perm(const int* ipivb, int* permb, int 
     for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = map[i];
 }
 
-// X[e][i][c] = (T) B[e][perm[i]][c]
+// The composed interchanges of each outer block of kouter columns, for an LU
+// that leaves them unapplied left of the block: bperm[e][b][i], i in
+// [b kouter, n), is the row that lands at row i when block b's pivots are
+// applied in order (rows above the block do not move).
+__global__ __launch_bounds__(256) void k_perm_blocks(const int* ipivb, int* bpermb, int n, int kouter, int nblk) {
+    extern __shared__ int map[];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const int* ipiv = ipivb + static_cast<size_t>(e) * n;
+    int* bp = bpermb + static_cast<size_t>(e) * nblk * n;
+    for (int b = 0; b < nblk; ++b) {
+        const int k0 = b * kouter, kend = min(n, k0 + kouter);
+        for (int i = k0 + tid; i < n; i += blockDim.x) map[i] = i;
+        __syncthreads();
+        if (tid == 0)
+            for (int k = k0; k < kend; ++k) {
+                const int q = ipiv[k];
+                if (q != k) {
+                    const int t = map[k];
+                    map[k] = map[q];
+                    map[q] = t;
+                }
+            }
+        __syncthreads();
+        for (int i = k0 + tid; i < n; i += blockDim.x) bp[static_cast<size_t>(b) * n + i] = map[i];
+        __syncthreads();
+    }
+}
+
+// Rows [k0, n) of X through one block's composed interchanges (bperm as
+// above, per energy stride sp): phase 0 gathers them into Tm, phase 1 copies
+// them back.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rowperm(typename Cx<T>::type* Xb, size_t sx, typename Cx<T>::type* Tb, size_t st,
+                                                 const int* bpermb, size_t sp, int k0, int n, int nrhs, int phase,
+                                                 const int* emap) {
+    const int e = emap ? emap[blockIdx.y] : static_cast<int>(blockIdx.y);
+    if (e < 0) return;
+    typename Cx<T>::type* X = Xb + e * sx;
+    typename Cx<T>::type* Tm = Tb + e * st;
+    const int* bp = bpermb + e * sp;
+    const size_t total = static_cast<size_t>(n - k0) * nrhs;
+    for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
+        const size_t at = static_cast<size_t>(k0) * nrhs + idx;
+        if (phase == 0) {
+            const int i = k0 + static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
+            Tm[at] = X[static_cast<size_t>(bp[i]) * nrhs + c];
+        } else {
+            X[at] = Tm[at];
+        }
+    }
+}
+
+// X[e][i][c] = (T) B[e][perm[i]][c] (perm null: the identity)
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, const int* permb, int n, int nrhs,
                                                 typename Cx<T>::type* X, const int* emap) {
@@ -2504,7 +2556,7 @@ __global__ __launch_bounds__(256) void k_gather(const double2* B, size_t sb, con
     const size_t total = static_cast<size_t>(n) * nrhs;
     for (size_t idx = blockIdx.x * 256ull + threadIdx.x; idx < total; idx += 256ull * gridDim.x) {
         const int i = static_cast<int>(idx / nrhs), c = static_cast<int>(idx % nrhs);
-        const int src = permb[static_cast<size_t>(e) * n + i];
+        const int src = permb ? permb[static_cast<size_t>(e) * n + i] : i;
         X[e * sb + idx] = to_cx<T>(B[e * sb + static_cast<size_t>(src) * nrhs + c]);
     }
 }
@@ -3753,6 +3805,12 @@ struct refeff_fms_plan {
     int calu_cap = 0;                   // candidate list length per energy
     bool panel_calu = true;             // REFEFF_FMS_PANEL=gepp: FP32 panels by partial pivoting
     bool lu_split = false;              // REFEFF_FMS_LU_SPLIT=1: outer blocks factored by recursive halving
+    // mixed plans: an outer block's interchanges are not applied to the
+    // columns left of it; the forward solve applies each block's composed
+    // pivots (d_bperm) as it reaches the block (REFEFF_FMS_LU_LEFT_SWAPS=1:
+    // applied, LAPACK layout)
+    bool lu_lazy_left = false;
+    int* d_bperm = nullptr;             // [e][outer blocks][npad], k_perm_blocks
     int calu_min_rows = 3072;           // tournament pivoting for panels of at least this many rows
     double2 *d_tmat = nullptr, *d_k = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_norms = nullptr;
@@ -3838,7 +3896,7 @@ void free_workspace(refeff_fms_plan* p) {
     p->d_dinv = nullptr;
     dfree(p->d_ipiv); dfree(p->d_perm); dfree(p->d_info); dfree(p->d_emap); dfree(p->d_cand); dfree(p->d_iter); dfree(p->d_tmat); dfree(p->d_k); dfree(p->d_ph);
     dfree(p->d_out); dfree(p->d_norms);
-    dfree(p->d_kv); dfree(p->d_pd); dfree(p->d_pn); dfree(p->d_bcol); dfree(p->d_split);
+    dfree(p->d_kv); dfree(p->d_pd); dfree(p->d_pn); dfree(p->d_bcol); dfree(p->d_split); dfree(p->d_bperm);
     p->ws_batch = 0;
 }
 
@@ -3924,6 +3982,7 @@ size_t split_slot_halves(const refeff_fms_plan* p) {
     return (h + 7) / 8 * 8;
 }
 int split_slots(const refeff_fms_plan* p) { return p->lookahead ? 2 : 1; }
+int lu_blocks(const refeff_fms_plan* p) { return (p->npad + p->outer_block - 1) / p->outer_block; }
 
 size_t bytes_per_energy(const refeff_fms_plan* p) {
     const size_t n2 = static_cast<size_t>(p->npad) * p->npad;
@@ -3937,6 +3996,7 @@ size_t bytes_per_energy(const refeff_fms_plan* p) {
     b += static_cast<size_t>(p->ch) * p->ch * p->npot * sizeof(double2);
     if (krylov_solver(p->solver)) b += static_cast<size_t>(kIterVecs) * nr * sizeof(double2);
     if (mixed && p->gemm_bf16) b += split_slots(p) * split_slot_halves(p) * sizeof(uint16_t);
+    if (p->lu_lazy_left) b += static_cast<size_t>(lu_blocks(p)) * p->npad * sizeof(int);
     if (p->bicg)
         b += static_cast<size_t>(p->bicg_vecs) * nr * sizeof(float2) +
              static_cast<size_t>(p->npad / kPanel) * p->nrhs * (sizeof(double2) + sizeof(double)) +
@@ -3978,6 +4038,7 @@ bool workspace_layout(refeff_fms_plan* p, int nb, Take&& take) {
     ok = ok && t(&p->d_emap, 2 * static_cast<size_t>(nb) * sizeof(int)) &&
          t(&p->d_ipiv, static_cast<size_t>(p->npad) * nb * sizeof(int)) &&
          t(&p->d_perm, static_cast<size_t>(p->npad) * nb * sizeof(int)) && t(&p->d_info, nb * sizeof(int)) &&
+         (!p->lu_lazy_left || t(&p->d_bperm, static_cast<size_t>(lu_blocks(p)) * p->npad * nb * sizeof(int))) &&
          t(&p->d_tmat, 2 * static_cast<size_t>(p->n) * nb * sizeof(double2)) &&
          t(&p->d_k, static_cast<size_t>(p->nsp) * nb * sizeof(double2)) &&
          t(&p->d_ph, static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot * nb * sizeof(double2)) &&
@@ -4010,7 +4071,8 @@ size_t carved_bytes(refeff_fms_plan* p, int nb) {
     probe_fields.npad = p->npad; probe_fields.n = p->n; probe_fields.nrhs = p->nrhs; probe_fields.nsp = p->nsp;
     probe_fields.L = p->L; probe_fields.npot = p->npot; probe_fields.ch = p->ch; probe_fields.solver = p->solver;
     probe_fields.gemm_bf16 = p->gemm_bf16; probe_fields.outer_block = p->outer_block; probe_fields.bicg = p->bicg;
-    probe_fields.lookahead = p->lookahead;
+    probe_fields.lookahead = p->lookahead; probe_fields.bicg_vecs = p->bicg_vecs;
+    probe_fields.lu_lazy_left = p->lu_lazy_left;
     workspace_layout(&probe_fields, nb, [&](void** slot, size_t bytes) {
         *slot = nullptr;
         off += (bytes + 255) / 256 * 256;
@@ -4201,6 +4263,9 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
     g2.s = g.s2;
     g2.split_slot = 1;
     bool far_pending = false;
+    // FP32 factors of mixed plans leave each block's interchanges unapplied
+    // left of it (the forward solve applies them block by block)
+    const bool lazy = sizeof(T) == 4 && p->lu_lazy_left && p->d_bperm != nullptr;
     for (int k0 = 0; k0 < n; k0 += kOuter) {
         const int kend = std::min(k0 + kOuter, n);
         auto panel = [&](int j0) {
@@ -4258,9 +4323,10 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
             (void)hipStreamWaitEvent(g.s, g.ev_far, 0);
             far_pending = false;
         }
-        // all interchanges of the outer block, left and right of it, before any far update
+        // all interchanges of the outer block, right of it (and left of it
+        // unless the forward solve applies them), before any far update
         for (int j0 = k0; j0 < kend; j0 += kPanel) {
-            swap_trsm<T>(p, g, A, ipiv, j0, 0, k0, 1, n);
+            if (!lazy) swap_trsm<T>(p, g, A, ipiv, j0, 0, k0, 1, n);
             swap_trsm<T>(p, g, A, ipiv, j0, kend, n, 1, n);
         }
         if (kend < n) {
@@ -4291,6 +4357,14 @@ void run_lu(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, int* ip
         }
     }
     if (far_pending) (void)hipStreamWaitEvent(g.s, g.ev_far, 0);
+    if (lazy) {
+        const int nblk = lu_blocks(p);
+        launch(p, g.s, REFEFF_FMS_K_OTHER, 0.0, 4.0 * n * nblk * nb, [&] {
+            hipLaunchKernelGGL(k_perm_blocks, dim3(nb), dim3(256), n * sizeof(int), g.s, ipiv,
+                               p->d_bperm + static_cast<size_t>(nblk) * n * g.e0, n, kOuter, nblk);
+        });
+        return;
+    }
     launch(p, g.s, REFEFF_FMS_K_OTHER, 0.0, 8.0 * n * nb, [&] {
         hipLaunchKernelGGL(k_perm, dim3(nb), dim3(256), n * sizeof(int), g.s, ipiv, perm, n);
     });
@@ -4327,79 +4401,134 @@ void trtri_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type
 // RHS chunk width per solve workgroup: 64 columns in FP32, 32 in FP64.
 template <typename T> constexpr int solve_chunk() { return sizeof(T) == 4 ? 64 : 32; }
 
-// X <- U^-1 L^-1 X (X already row-permuted): one workgroup per energy and
-// right-hand-side chunk walks each triangle once.
-template <typename T, bool LOWER>
-void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
-                 typename Cx<T>::type* X, const int* emap) {
-    const int n = p->npad, nr = p->nrhs, nb = g.nb;
-    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
+// Right-hand-side tiling of the solve kernels: 16 ntile columns per workgroup
+// in `chunks` chunks.
+struct SolveTiling {
+    int ntile, chunks;
+};
+template <typename T>
+SolveTiling solve_tiling(int nr) {
     constexpr int CW = solve_chunk<T>();
     const int nt = (std::min(nr, CW) + 15) / 16;
     const int ntile = nt == 1 ? 1 : nt == 2 ? 2 : 4;
-    const int chunks = (nr + 16 * ntile - 1) / (16 * ntile);
-    const double cb = sizeof(typename Cx<T>::type);
-    const double nn = static_cast<double>(n) * n;
-    (void)nn;
-    // Blocked by row blocks of bs rows (about 16 per triangle): the coupling
-    // of each block row to the rows already solved is one launch over every
-    // (32-row block, energy) pair; only the diagonal blocks are walked by one
-    // workgroup per energy.
+    return SolveTiling{ntile, (nr + 16 * ntile - 1) / (16 * ntile)};
+}
+
+// X[r_lo, r_hi) -= LU[r_lo:r_hi, k_lo:k_hi) X[k_lo:k_hi): the coupling of a
+// block row to rows already solved, one workgroup per (32-row block, energy,
+// chunk).
+template <typename T>
+void solve_couple(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, typename Cx<T>::type* X,
+                  int r_lo, int r_hi, int k_lo, int k_hi, const int* emap) {
+    if (k_hi <= k_lo || r_hi <= r_lo) return;
+    const int n = p->npad, nr = p->nrhs, nb = g.nb;
+    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
+    const SolveTiling tl = solve_tiling<T>(nr);
+    const double cb = sizeof(typename Cx<T>::type), rows = r_hi - r_lo, kl = k_hi - k_lo;
+    launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * rows * kl * nr * nb, (rows * kl * tl.chunks + 2.0 * kl * nr) * cb * nb, [&] {
+        const dim3 grid((r_hi - r_lo) / kPanel, nb, tl.chunks);
+        if (tl.ntile == 1)
+            hipLaunchKernelGGL((k_solve_gemv<T, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo, k_lo, k_hi,
+                               emap);
+        else if (tl.ntile == 2 || sizeof(T) == 8)  // FP64 chunks hold at most 32 columns
+            hipLaunchKernelGGL((k_solve_gemv<T, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo, k_lo, k_hi,
+                               emap);
+        else if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((k_solve_gemv<T, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo, k_lo, k_hi,
+                               emap);
+    });
+}
+
+// The diagonal blocks of rows [r_lo, r_hi), walked by one workgroup per
+// (energy, chunk), the coupling inside the range included.
+template <typename T, bool LOWER>
+void solve_diag(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
+                typename Cx<T>::type* X, int r_lo, int r_hi, const int* emap) {
+    const int n = p->npad, nr = p->nrhs, nb = g.nb;
+    const size_t sa = static_cast<size_t>(n) * n, sx = static_cast<size_t>(n) * nr;
+    const SolveTiling tl = solve_tiling<T>(nr);
+    const double cb = sizeof(typename Cx<T>::type), rows = r_hi - r_lo;
+    launch(p, g.s, REFEFF_FMS_K_SOLVE, 4.0 * rows * rows * nr * nb,
+           (rows * rows / 2 * tl.chunks + 2.0 * rows * nr) * cb * nb, [&] {
+               const dim3 grid(nb, tl.chunks);
+               // 16 waves for the common <= 16 right-hand sides in FP32 (more
+               // loads in flight), 8 otherwise
+               if constexpr (sizeof(T) == 4) {
+                   if (tl.ntile == 1)
+                       hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 16>), grid, dim3(1024), 0, g.s, LU, sa, n, dinv, X, sx,
+                                          nr, n, r_lo, r_hi, emap);
+                   else if (tl.ntile == 2)
+                       hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx,
+                                          nr, n, r_lo, r_hi, emap);
+                   else
+                       hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx,
+                                          nr, n, r_lo, r_hi, emap);
+               } else {
+                   if (tl.ntile == 1)
+                       hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx,
+                                          nr, n, r_lo, r_hi, emap);
+                   else
+                       hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx,
+                                          nr, n, r_lo, r_hi, emap);
+               }
+           });
+}
+
+// X <- U^-1 L^-1 X (X already row-permuted), blocked by row blocks of bs
+// rows (about 16 per triangle): the coupling of each block row to the rows
+// already solved is one launch over every (32-row block, energy) pair; only
+// the diagonal blocks are walked by one workgroup per energy.
+template <typename T, bool LOWER>
+void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
+                 typename Cx<T>::type* X, const int* emap) {
+    const int n = p->npad;
     const int bs = std::max(256, ((n / 16 + kPanel - 1) / kPanel) * kPanel);
     const int nbig = (n + bs - 1) / bs;
     for (int step = 0; step < nbig; ++step) {
         const int I = LOWER ? step : nbig - 1 - step;
         const int r_lo = I * bs, r_hi = std::min(n, r_lo + bs);
-        const int k_lo = LOWER ? 0 : r_hi, k_hi = LOWER ? r_lo : n;
-        const double rows = r_hi - r_lo;
-        if (k_hi > k_lo) {
-            const double kl = k_hi - k_lo;
-            launch(p, g.s, REFEFF_FMS_K_SOLVE, 8.0 * rows * kl * nr * nb, (rows * kl * chunks + 2.0 * kl * nr) * cb * nb,
-                   [&] {
-                       const dim3 grid((r_hi - r_lo) / kPanel, nb, chunks);
-                       if (ntile == 1)
-                           hipLaunchKernelGGL((k_solve_gemv<T, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo,
-                                              k_lo, k_hi, emap);
-                       else if (ntile == 2 || sizeof(T) == 8)  // FP64 chunks hold at most 32 columns
-                           hipLaunchKernelGGL((k_solve_gemv<T, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo,
-                                              k_lo, k_hi, emap);
-                       else if constexpr (sizeof(T) == 4)
-                           hipLaunchKernelGGL((k_solve_gemv<T, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, X, sx, nr, r_lo,
-                                              k_lo, k_hi, emap);
-                   });
-        }
-        launch(p, g.s, REFEFF_FMS_K_SOLVE, 4.0 * rows * rows * nr * nb, (rows * rows / 2 * chunks + 2.0 * rows * nr) * cb * nb, [&] {
-            const dim3 grid(nb, chunks);
-            // 16 waves for the common <= 16 right-hand sides in FP32 (more
-            // loads in flight), 8 otherwise
-            if constexpr (sizeof(T) == 4) {
-                if (ntile == 1)
-                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 16>), grid, dim3(1024), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
-                                       r_lo, r_hi, emap);
-                else if (ntile == 2)
-                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
-                                       r_lo, r_hi, emap);
-                else
-                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 4, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
-                                       r_lo, r_hi, emap);
-            } else {
-                if (ntile == 1)
-                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 1, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
-                                       r_lo, r_hi, emap);
-                else
-                    hipLaunchKernelGGL((k_trsm_wg<T, LOWER, 2, 8>), grid, dim3(512), 0, g.s, LU, sa, n, dinv, X, sx, nr, n,
-                                       r_lo, r_hi, emap);
-            }
+        solve_couple<T>(p, g, LU, X, r_lo, r_hi, LOWER ? 0 : r_hi, LOWER ? r_lo : n, emap);
+        solve_diag<T, LOWER>(p, g, LU, dinv, X, r_lo, r_hi, emap);
+    }
+}
+
+// Forward solve with factors whose interchanges were left unapplied left of
+// each outer block (lu_lazy_left), X in the right-hand sides' own row order:
+// per outer block, its composed pivots move rows [k0, n) of X (through Tm, a
+// buffer of X's shape), its diagonal blocks are solved, and the rows below
+// take its coupling, right-looking, so every L column meets X in the row
+// order it was factored in.  Leaves X = L^-1 P X.
+template <typename T>
+void forward_lazy(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
+                  typename Cx<T>::type* X, typename Cx<T>::type* Tm, size_t st, const int* emap) {
+    const int n = p->npad, nr = p->nrhs, nb = g.nb, kOuter = p->outer_block, nblk = lu_blocks(p);
+    const size_t sx = static_cast<size_t>(n) * nr, sp = static_cast<size_t>(nblk) * n;
+    const int* bp0 = p->d_bperm + sp * g.e0;
+    for (int b = 0; b < nblk; ++b) {
+        const int k0 = b * kOuter, kend = std::min(n, k0 + kOuter);
+        const size_t cells = static_cast<size_t>(n - k0) * nr;
+        const unsigned gx = static_cast<unsigned>(std::min<size_t>(1024, (cells + 255) / 256));
+        launch(p, g.s, REFEFF_FMS_K_SOLVE, 0.0, 4.0 * cells * sizeof(typename Cx<T>::type) * nb, [&] {
+            for (int phase = 0; phase < 2; ++phase)
+                hipLaunchKernelGGL(k_rowperm<T>, dim3(gx, nb), dim3(256), 0, g.s, X, sx, Tm, st, bp0 + static_cast<size_t>(b) * n,
+                                   sp, k0, n, nr, phase, emap);
         });
+        solve_diag<T, true>(p, g, LU, dinv, X, k0, kend, emap);
+        solve_couple<T>(p, g, LU, X, kend, n, k0, kend, emap);
     }
 }
 
 template <typename T>
 // g.nb energies of the group: all of them when emap is null, else the
-// group-relative energies emap[0 .. g.nb)
+// group-relative energies emap[0 .. g.nb).  tmp: a buffer of X's shape for
+// factors of a lazy-left LU (X then in the right-hand sides' own order).
 void run_solve(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
-               typename Cx<T>::type* X, const int* emap = nullptr) {
-    trsm_launch<T, true>(p, g, LU, dinv, X, emap);
+               typename Cx<T>::type* X, const int* emap = nullptr, typename Cx<T>::type* tmp = nullptr,
+               size_t tmp_stride = 0) {
+    if (sizeof(T) == 4 && p->lu_lazy_left && tmp)
+        forward_lazy<T>(p, g, LU, dinv, X, tmp, tmp_stride, emap);
+    else
+        trsm_launch<T, true>(p, g, LU, dinv, X, emap);
     trsm_launch<T, false>(p, g, LU, dinv, X, emap);
 }
 
@@ -4870,10 +4999,12 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                         rhs = p->d_r64 + sx * g.e0;
                     }
                     hipLaunchKernelGGL(k_gather<float>, dim3(gx, g.nb), dim3(256), 0, g.s, rhs, sx,
-                                       p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0, emap);
+                                       p->lu_lazy_left ? nullptr : p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs,
+                                       w + sx * g.e0, emap);
+                    // R64 is free from here to the next residual: the lazy forward solve's buffer
                     run_solve<float>(p, g, p->d_a32 + sa * g.e0,
                                      static_cast<const float2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
-                                     w + sx * g.e0, emap);
+                                     w + sx * g.e0, emap, reinterpret_cast<float2*>(p->d_r64 + sx * g.e0), 2 * sx);
                     hipLaunchKernelGGL(k_update<float>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_x64 + sx * g.e0,
                                        w + sx * g.e0, sx, n, p->nrhs, gn, emap);
                     hipLaunchKernelGGL(k_refine_live, dim3(1), dim3(256), 0, g.s, gn, g.nb, sweep,
@@ -4953,10 +5084,12 @@ int solve_device_batch(refeff_fms_plan* p, int nb, double2* out, int32_t* status
                     rhs = p->d_r64 + sx * g.e0;
                 }
                 hipLaunchKernelGGL(k_gather<float>, dim3(gx, g.nb), dim3(256), 0, g.s, rhs, sx,
-                                   p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs, w + sx * g.e0, emap);
+                                   p->lu_lazy_left ? nullptr : p->d_perm + static_cast<size_t>(n) * g.e0, n, p->nrhs,
+                                   w + sx * g.e0, emap);
+                // R64 is free from here to the next residual: the lazy forward solve's buffer
                 run_solve<float>(p, g, p->d_a32 + sa * g.e0,
                                  static_cast<const float2*>(p->d_dinv) + 2 * static_cast<size_t>(n) * kPanel * g.e0,
-                                 w + sx * g.e0, emap);
+                                 w + sx * g.e0, emap, reinterpret_cast<float2*>(p->d_r64 + sx * g.e0), 2 * sx);
                 hipLaunchKernelGGL(k_update<float>, dim3(gx, g.nb), dim3(256), 0, g.s, p->d_x64 + sx * g.e0,
                                    w + sx * g.e0, sx, n, p->nrhs, gn, emap);
                 (void)hipMemcpyAsync(norms.data() + 4 * static_cast<size_t>(g.e0), gn, 4 * gall.nb * sizeof(double),
@@ -5279,6 +5412,8 @@ int refeff_fms_plan_create(const refeff_fms_plan_desc* d, refeff_fms_plan** out)
         p->gemm_bf16 = (v == 2 || v == 3 || v == 4 || v == 16) ? v : 0;  // bf16x2, bf16x3, fp16x2 4M, fp16x2 3M
     }
     if (const char* e = std::getenv("REFEFF_FMS_BF_MIN_K")) p->bf_min_k = std::max(kPanel + 1, std::atoi(e));
+    p->lu_lazy_left = p->solver == REFEFF_FMS_SOLVER_MIXED;
+    if (const char* e = std::getenv("REFEFF_FMS_LU_LEFT_SWAPS")) p->lu_lazy_left = p->lu_lazy_left && std::atoi(e) == 0;
     if (const char* e = std::getenv("REFEFF_FMS_PANEL")) p->panel_calu = std::strcmp(e, "gepp") != 0;
     if (const char* e = std::getenv("REFEFF_FMS_CALU_MIN_ROWS")) p->calu_min_rows = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("REFEFF_FMS_STREAMS")) p->nstreams = std::max(1, std::min(4, std::atoi(e)));
@@ -5472,7 +5607,7 @@ struct WorkspaceSnapshot {
 #define REFEFF_WS_FIELDS(X)                                                                                  \
     X(d_a32) X(d_a64) X(d_b64) X(d_x64) X(d_r64) X(d_w) X(d_dinv) X(d_cand) X(d_split) X(d_iter) X(d_kv)     \
     X(d_pd) X(d_pn) X(d_bcol) X(d_emap) X(d_ipiv) X(d_perm) X(d_info) X(d_tmat) X(d_k) X(d_ph) X(d_out)      \
-    X(d_norms)
+    X(d_norms) X(d_bperm)
 #define REFEFF_WS_DECL(f) decltype(refeff_fms_plan::f) f;
     REFEFF_WS_FIELDS(REFEFF_WS_DECL)
 #undef REFEFF_WS_DECL
@@ -5601,6 +5736,10 @@ static int solve_common(refeff_fms_plan* p, int32_t ne, const double* k, const d
             off += (bytes + 255) / 256 * 256;
             return true;
         });
+        if (off > arena->bytes) {  // carved_bytes chose the batch; the layout must agree with it
+            keep.reset();
+            return REFEFF_FMS_NO_MEMORY;
+        }
         p->ws_batch = batch;
     }
     if (p->bicg && !full_out) {

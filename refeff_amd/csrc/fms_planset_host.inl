@@ -250,6 +250,10 @@ int plan_set_capture(refeff_fms_plan_set* s, const int32_t* energy_counts, const
                 off += (bytes + 255) / 256 * 256;
                 return true;
             });
+            if (off != carved_bytes(p, s->gbatch[c])) {  // the arena was sized from carved_bytes
+                rc = REFEFF_FMS_NO_MEMORY;
+                break;
+            }
             p->ws_batch = s->gbatch[c];
             p->profile = false;
             const size_t kst = static_cast<size_t>(p->nsp), pst = static_cast<size_t>(p->nsp) * (2 * p->L + 1) * p->npot;
