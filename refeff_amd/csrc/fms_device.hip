@@ -4498,6 +4498,7 @@ void trsm_launch(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type*
 // buffer of X's shape), its diagonal blocks are solved, and the rows below
 // take its coupling, right-looking, so every L column meets X in the row
 // order it was factored in.  Leaves X = L^-1 P X.
+constexpr int kLazySub = 256;  // rows per diagonal walk, a multiple of kPanel
 template <typename T>
 void forward_lazy(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type* LU, const typename Cx<T>::type* dinv,
                   typename Cx<T>::type* X, typename Cx<T>::type* Tm, size_t st, const int* emap) {
@@ -4513,7 +4514,13 @@ void forward_lazy(refeff_fms_plan* p, const Group& g, const typename Cx<T>::type
                 hipLaunchKernelGGL(k_rowperm<T>, dim3(gx, nb), dim3(256), 0, g.s, X, sx, Tm, st, bp0 + static_cast<size_t>(b) * n,
                                    sp, k0, n, nr, phase, emap);
         });
-        solve_diag<T, true>(p, g, LU, dinv, X, k0, kend, emap);
+        // the block's diagonal walked in sub-blocks of 256 rows (one
+        // workgroup per energy walks each), left-looking inside the block
+        for (int s0 = k0; s0 < kend; s0 += kLazySub) {
+            const int s1 = std::min(kend, s0 + kLazySub);
+            if (s0 > k0) solve_couple<T>(p, g, LU, X, s0, s1, k0, s0, emap);
+            solve_diag<T, true>(p, g, LU, dinv, X, s0, s1, emap);
+        }
         solve_couple<T>(p, g, LU, X, kend, n, k0, kend, emap);
     }
 }
