@@ -188,23 +188,28 @@ def test_mkgtr_trace_on_gpu_matches_oracle():
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("bf16,split,streams,lookahead", [
-    ("0", "0", "1", "0"), ("2", "0", "1", "0"), ("3", "0", "1", "0"), ("16", "0", "1", "0"), ("4", "0", "1", "0"),
-    ("0", "1", "1", "0"), ("0", "0", "2", "1"), ("4", "1", "2", "1")])
-def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16, split, streams, lookahead):
+@pytest.mark.parametrize("bf16,split,streams,lookahead,left", [
+    ("0", "0", "1", "0", "0"), ("2", "0", "1", "0", "0"), ("3", "0", "1", "0", "0"), ("16", "0", "1", "0", "0"),
+    ("4", "0", "1", "0", "0"), ("0", "1", "1", "0", "0"), ("0", "0", "2", "1", "0"), ("4", "1", "2", "1", "0"),
+    ("4", "0", "1", "0", "1")])
+def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16, split, streams, lookahead, left):
     """Cu fcc R=8.0 (177 atoms, N=2832) at full size: the two device solvers
     (mixed LU + FP64 refinement vs plain FP64 LU) agree to the gg gate with the
     trailing updates in FP32, on split-bf16 MFMA (two or three parts) and on
     split-fp16 MFMA (3M and four-product forms), with the outer blocks factored
     by recursive halving (both LUs), and with the energies in two stream groups and the
     far trailing update on a second stream (every group and stream with its own
-    pivoting and split buffers)."""
+    pivoting and split buffers).  The mixed LU leaves the interchanges left of
+    each of its 15 outer blocks to the forward solve (left "0") or applies them
+    in the factors (left "1"); with the FP32 and the default split-fp16 updates
+    every energy settles in the refinement, none through the FP64 fallback."""
     monkeypatch.setenv("REFEFF_FMS_GEMM_BF16", bf16)
+    monkeypatch.setenv("REFEFF_FMS_LU_LEFT_SWAPS", left)
     monkeypatch.setenv("REFEFF_FMS_LU_SPLIT", split)
     monkeypatch.setenv("REFEFF_FMS_STREAMS", streams)
     monkeypatch.setenv("REFEFF_FMS_LOOKAHEAD", lookahead)
     banner(f"Cu fcc R=8.0 N=2832 mixed (split GEMM {bf16}, LU halves {split}, streams {streams}, lookahead "
-           f"{lookahead}) vs fp64 (6 energies)")
+           f"{lookahead}, left swaps in the factors {left}) vs fp64 (6 energies)")
     wl = W.fcc_cu(8.0, 120)
     sel = [0, 20, 47, 80, 101, 119]
     k, ph = wl.wave_numbers()[sel], wl.phase_shifts()[sel]
@@ -214,6 +219,10 @@ def test_cu_fcc_r80_mixed_agrees_with_fp64(monkeypatch, bf16, split, streams, lo
                                     direct_cutoff=wl.rdirec, solver=solver)
         sc, st = plan.solve(k, ph)
         assert not st.any()
+        if solver == fms.SOLVER_MIXED and bf16 in ("0", "4"):
+            stats = plan.last_stats
+            assert stats.fp64_fallback_energies == 0 and stats.refinement_sweeps <= 6, \
+                (stats.fp64_fallback_energies, stats.refinement_sweeps)
         res[solver] = sc
         plan.close()
     for e in range(len(sel)):
