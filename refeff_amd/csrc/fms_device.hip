@@ -1066,7 +1066,10 @@ __global__ __launch_bounds__(1024) void k_panel_rb1(typename Cx<T>::type* Ab, si
 // Appl. 32 (2011) 1317): its pivots differ from row-by-row partial pivoting
 // but it is stable in practice; the FP64 refinement around this
 // factorisation is what sets the final accuracy.
-constexpr int kSelRows = 512;  // rows per selection workgroup, one per thread
+constexpr int kSelRows = 512;  // rows per workgroup of the upper levels and the final step, one per thread
+// rows per first-level selection workgroup: four waves, so three workgroups
+// share a CU and one's per-column barrier waits are filled by the others
+constexpr int kSelLeaf = 256;
 
 // Wave-wide maximum: DPP within rows of 16 lanes, then the four row maxima
 // read lane by lane.
@@ -1099,10 +1102,12 @@ __device__ __forceinline__ float2 crcp_fast(float2 d) {
 
 // LDS of one partial-pivoting pass: per column (double-buffered by column
 // parity) each wave's candidate row and its |re|+|im| and thread.
+template <int NW>
 struct CaluLds {
-    float2 cand[2][kSelRows / 64][kPanel];
-    float4 best[2][2];
-    int4 idx[2][2];
+    static_assert(NW % 4 == 0, "whole float4 groups of wave results");
+    float2 cand[2][NW][kPanel];
+    float4 best[2][NW / 4];
+    int4 idx[2][NW / 4];
 };
 
 // Partial pivoting over the rows held one per thread (v, valid), without
@@ -1114,12 +1119,11 @@ struct CaluLds {
 // by ballot) and publishes it to LDS; after one barrier every thread takes
 // the first wave holding the largest and reads that row.  FAST: multipliers
 // through crcp_fast instead of a division per row.  FP32 only.
-template <typename T, bool FAST>
+template <typename T, bool FAST, int NW>
 __device__ __forceinline__ void calu_gepp(typename Cx<T>::type (&v)[kPanel], bool valid, int* win,
-                                          typename Cx<T>::type (*fact)[kPanel + 1], CaluLds& s, int* zero_col) {
+                                          typename Cx<T>::type (*fact)[kPanel + 1], CaluLds<NW>& s, int* zero_col) {
     static_assert(sizeof(T) == 4, "tournament pivoting runs on FP32 panels");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    static_assert(kSelRows == 512, "eight waves");
     bool used = false;
     if (tid == 0) *zero_col = -1;
 #pragma unroll
@@ -1138,14 +1142,19 @@ __device__ __forceinline__ void calu_gepp(typename Cx<T>::type (&v)[kPanel], boo
             reinterpret_cast<int*>(s.idx[buf])[wave] = wave * 64 + wl;
         }
         __syncthreads();
-        const float4 b0 = s.best[buf][0], b1 = s.best[buf][1];
-        const int4 i0 = s.idx[buf][0], i1 = s.idx[buf][1];
-        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-        const int ii[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+        float bb[NW];
+        int ii[NW];
+#pragma unroll
+        for (int q = 0; q < NW / 4; ++q) {
+            const float4 b = s.best[buf][q];
+            const int4 i = s.idx[buf][q];
+            bb[4 * q] = b.x, bb[4 * q + 1] = b.y, bb[4 * q + 2] = b.z, bb[4 * q + 3] = b.w;
+            ii[4 * q] = i.x, ii[4 * q + 1] = i.y, ii[4 * q + 2] = i.z, ii[4 * q + 3] = i.w;
+        }
         float pb = bb[0];
         int pt = ii[0];
 #pragma unroll
-        for (int w = 1; w < 8; ++w)
+        for (int w = 1; w < NW; ++w)
             if (bb[w] > pb) {
                 pb = bb[w];
                 pt = ii[w];
@@ -1177,12 +1186,13 @@ __device__ __forceinline__ void calu_gepp(typename Cx<T>::type (&v)[kPanel], boo
     }
 }
 
-// Row source of a selection workgroup: with rows == nullptr the contiguous
-// rows j0 + g*512 + t (< n), else rows[g*512 + t] (< count, -1 = none).
-template <typename T>
+// Row source of a selection workgroup of ROWS threads: with rows == nullptr
+// the contiguous rows j0 + g*ROWS + t (< n), else rows[g*ROWS + t] (< count,
+// -1 = none).
+template <typename T, int ROWS>
 __device__ __forceinline__ int calu_load(const typename Cx<T>::type* A, int lda, int j0, int n, const int* rows,
                                          int count, int g, typename Cx<T>::type (&v)[kPanel]) {
-    const int t = g * kSelRows + static_cast<int>(threadIdx.x);
+    const int t = g * ROWS + static_cast<int>(threadIdx.x);
     int r = -1;
     if (!rows) {
         if (j0 + t < n) r = j0 + t;
@@ -1197,23 +1207,23 @@ __device__ __forceinline__ int calu_load(const typename Cx<T>::type* A, int lda,
     return r;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kSelRows) void k_calu_select(const typename Cx<T>::type* Ab, size_t sa, int lda, int j0,
-                                                          int n, const int* in_rows, int in_stride, int in_count,
-                                                          int* out_rows, int out_stride) {
+template <typename T, int ROWS>
+__global__ __launch_bounds__(ROWS) void k_calu_select(const typename Cx<T>::type* Ab, size_t sa, int lda, int j0,
+                                                      int n, const int* in_rows, int in_stride, int in_count,
+                                                      int* out_rows, int out_stride) {
     using T2 = typename Cx<T>::type;
-    __shared__ CaluLds cl;
+    __shared__ CaluLds<ROWS / 64> cl;
     __shared__ int win[kPanel];
     __shared__ int zero_col;
     const int g = blockIdx.x, e = blockIdx.y;
     const T2* A = Ab + e * sa;
     T2 v[kPanel];
-    const int r = calu_load<T>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
-                               in_count, g, v);
-    calu_gepp<T, true>(v, r >= 0, win, nullptr, cl, &zero_col);
+    const int r = calu_load<T, ROWS>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
+                                     in_count, g, v);
+    calu_gepp<T, true, ROWS / 64>(v, r >= 0, win, nullptr, cl, &zero_col);
     __syncthreads();
     // winners by column, as global rows (the thread that won column c holds r)
-    __shared__ int rowof[kSelRows];
+    __shared__ int rowof[ROWS];
     rowof[threadIdx.x] = r;
     __syncthreads();
     if (threadIdx.x < kPanel) {
@@ -1227,7 +1237,7 @@ __global__ __launch_bounds__(kSelRows) void k_calu_final(typename Cx<T>::type* A
                                                          const int* in_rows, int in_stride, int in_count, int* ipivb,
                                                          int* info) {
     using T2 = typename Cx<T>::type;
-    __shared__ CaluLds cl;
+    __shared__ CaluLds<kSelRows / 64> cl;
     __shared__ int win[kPanel];
     __shared__ int zero_col;
     __shared__ T2 fact[kPanel][kPanel + 1];
@@ -1239,9 +1249,9 @@ __global__ __launch_bounds__(kSelRows) void k_calu_final(typename Cx<T>::type* A
     T2* A = Ab + e * sa;
     int* ipiv = ipivb + static_cast<size_t>(e) * n;
     T2 v[kPanel];
-    const int r = calu_load<T>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
-                               in_count, 0, v);
-    calu_gepp<T, false>(v, r >= 0, win, fact, cl, &zero_col);
+    const int r = calu_load<T, kSelRows>(A, lda, j0, n, in_rows ? in_rows + static_cast<size_t>(e) * in_stride : nullptr,
+                                         in_count, 0, v);
+    calu_gepp<T, false, kSelRows / 64>(v, r >= 0, win, fact, cl, &zero_col);
     rowof[tid] = r;
     for (int i = tid; i < kPanel * kPanel; i += kSelRows)
         orig[i / kPanel][i % kPanel] = A[static_cast<size_t>(j0 + i / kPanel) * lda + j0 + i % kPanel];
@@ -3955,7 +3965,7 @@ template <typename P> bool dalloc(P** ptr, size_t count) {
 
 // Tournament-pivoting buffers of the FP32 LU for nb energies.
 bool alloc_calu(refeff_fms_plan* p, int nb) {
-    p->calu_cap = ((p->npad + kSelRows - 1) / kSelRows) * kPanel;
+    p->calu_cap = ((p->npad + kSelLeaf - 1) / kSelLeaf) * kPanel;  // the first level's proposals
     return dalloc(&p->d_cand, 2 * static_cast<size_t>(p->calu_cap) * nb);
 }
 
@@ -4020,7 +4030,7 @@ bool workspace_layout(refeff_fms_plan* p, int nb, Take&& take) {
          t(&p->d_x64, nr * nb * sizeof(double2)) && t(&p->d_r64, nr * nb * sizeof(double2));
     ok = ok && t(&p->d_w, nr * nb * cs) && t(&p->d_dinv, 2 * static_cast<size_t>(p->npad) * kPanel * nb * cs);
     if (mixed) {
-        p->calu_cap = ((p->npad + kSelRows - 1) / kSelRows) * kPanel;
+        p->calu_cap = ((p->npad + kSelLeaf - 1) / kSelLeaf) * kPanel;  // the first level's proposals
         ok = ok && t(&p->d_cand, 2 * static_cast<size_t>(p->calu_cap) * nb * sizeof(int));
     }
     if (mixed && p->gemm_bf16) {
@@ -4093,16 +4103,23 @@ void launch_panel(refeff_fms_plan* p, hipStream_t s, int nb, int e0, typename Cx
     const dim3 grid(nb), block(1024);
     if constexpr (sizeof(T) == 4) {
         if (p->panel_calu && p->d_cand && m >= p->calu_min_rows) {
-            // selection levels until <= 512 candidates remain, then the final pivoting
+            // selection levels until <= 512 candidates remain (the panel's rows
+            // in 256-row workgroups, later levels in 512-row ones), then the
+            // final pivoting
             const int cap = p->calu_cap;
             const size_t cb = static_cast<size_t>(std::max(1, p->ws_batch));  // energies per list
             int* buf[2] = {p->d_cand + static_cast<size_t>(cap) * e0, p->d_cand + static_cast<size_t>(cap) * (cb + e0)};
             const int* in = nullptr;
             int count = m, cur = 0;
             while (count > kSelRows) {
-                const int G = (count + kSelRows - 1) / kSelRows;
-                hipLaunchKernelGGL(k_calu_select<float>, dim3(G, nb), dim3(kSelRows), 0, s, A, sa, lda, j0, n, in, cap,
-                                   count, buf[cur], cap);
+                const int rows = in ? kSelRows : kSelLeaf;
+                const int G = (count + rows - 1) / rows;
+                if (in)
+                    hipLaunchKernelGGL((k_calu_select<float, kSelRows>), dim3(G, nb), dim3(kSelRows), 0, s, A, sa, lda,
+                                       j0, n, in, cap, count, buf[cur], cap);
+                else
+                    hipLaunchKernelGGL((k_calu_select<float, kSelLeaf>), dim3(G, nb), dim3(kSelLeaf), 0, s, A, sa, lda,
+                                       j0, n, in, cap, count, buf[cur], cap);
                 in = buf[cur];
                 cur ^= 1;
                 count = G * kPanel;

@@ -135,8 +135,9 @@ def test_lu_double_large_matches_lapack(n):
 @pytest.mark.parametrize("panel", ["calu", "gepp"])
 @pytest.mark.parametrize("n", [64, 256, 1152, 4160])
 def test_lu_float_reconstructs_matrix(n, panel, monkeypatch):
-    """FP32 LU with tournament-pivoting panels (default; 1152 and 4160 rows take a
-    selection level) and with the row-by-row partial-pivoting panels."""
+    """FP32 LU with tournament-pivoting panels (default; 1152 rows take one
+    selection level of 256-row workgroups, 4160 rows a second of 512-row ones)
+    and with the row-by-row partial-pivoting panels."""
     monkeypatch.setenv("REFEFF_FMS_PANEL", panel)
     print(f"[gpu-kernels] lu float n={n} panel={panel}", flush=True)
     rng = np.random.default_rng(100 + n)
@@ -237,7 +238,7 @@ def test_lu_float_tournament_reports_singularity_and_top_row_pivots(monkeypatch)
     a = crand(rng, n, n)
     a[:16, :32] *= 50.0       # half the first panel's pivots come from its own top rows
     a[900:916, :32] *= 1e-3    # (never pivots of the first panel, so they stay in place)
-    a[900:916, 32:64] *= 50.0  # the second panel's from the third selection chunk
+    a[900:916, 32:64] *= 50.0  # the second panel's from a later selection chunk
     work = np.ascontiguousarray(a.copy())
     piv = np.zeros(n, np.int32)
     info = C.c_int32(-1)
