@@ -1069,6 +1069,7 @@ __global__ __launch_bounds__(1024) void k_panel_rb1(typename Cx<T>::type* Ab, si
 constexpr int kSelRows = 512;  // rows per workgroup of the upper levels and the final step, one per thread
 // rows per first-level selection workgroup: four waves, so three workgroups
 // share a CU and one's per-column barrier waits are filled by the others
+// (two waves and six workgroups: a further selection level costs more)
 constexpr int kSelLeaf = 256;
 
 // Wave-wide maximum: DPP within rows of 16 lanes, then the four row maxima
@@ -1104,10 +1105,9 @@ __device__ __forceinline__ float2 crcp_fast(float2 d) {
 // parity) each wave's candidate row and its |re|+|im| and thread.
 template <int NW>
 struct CaluLds {
-    static_assert(NW % 4 == 0, "whole float4 groups of wave results");
     float2 cand[2][NW][kPanel];
-    float4 best[2][NW / 4];
-    int4 idx[2][NW / 4];
+    float4 best[2][(NW + 3) / 4];
+    int4 idx[2][(NW + 3) / 4];
 };
 
 // Partial pivoting over the rows held one per thread (v, valid), without
@@ -1142,10 +1142,10 @@ __device__ __forceinline__ void calu_gepp(typename Cx<T>::type (&v)[kPanel], boo
             reinterpret_cast<int*>(s.idx[buf])[wave] = wave * 64 + wl;
         }
         __syncthreads();
-        float bb[NW];
-        int ii[NW];
+        float bb[4 * ((NW + 3) / 4)];  // entries past NW are never read
+        int ii[4 * ((NW + 3) / 4)];
 #pragma unroll
-        for (int q = 0; q < NW / 4; ++q) {
+        for (int q = 0; q < (NW + 3) / 4; ++q) {
             const float4 b = s.best[buf][q];
             const int4 i = s.idx[buf][q];
             bb[4 * q] = b.x, bb[4 * q + 1] = b.y, bb[4 * q + 2] = b.z, bb[4 * q + 3] = b.w;
