@@ -6,7 +6,8 @@ solver (complex FP32 MFMA LU + FP64 refinement) against the plain FP64 LU on
 4 energies spread over the mesh, and both against the FP64 oracle on the
 lowest and highest energy.  The oracle at this size is its own G0 / t-matrix
 assembly with LAPACK zgetrf / zgetrs for the solve (the oracle's dense solve
-is pinned against LAPACK on small systems).  Bar: ||gg - ref|| / ||ref|| <= 1e-8.
+is pinned against LAPACK on small systems).  Bar: ||gg - ref|| / ||ref|| <= 1e-8,
+with no mixed energy handed to the FP64 fallback.
 """
 import sys
 
@@ -60,6 +61,9 @@ def large_case(wl, sel, oracle_sel):
                                     mean_square_displacements=wl.sigsqr(), lfms=wl.lfms, solver=solver)
         sc, st = plan.solve(k[sel], ph[sel])
         assert not st.any(), st
+        if solver == fms.SOLVER_MIXED:  # every energy settled by the refinement itself
+            assert plan.last_stats.fp64_fallback_energies == 0, plan.last_stats.fp64_fallback_energies
+            banner(f"{wl.name}: mixed refinement sweeps {plan.last_stats.refinement_sweeps}")
         res[solver] = sc[:, ip0, :nb, :nb]
         plan.close()
     for i, e in enumerate(sel):
