@@ -28,6 +28,7 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <thread>
 #include <vector>
 
@@ -2263,6 +2264,129 @@ __global__ __launch_bounds__(256) void k_h4split_cols(const float2* Bb, int ldb,
     }
 }
 
+// Register-resident splits for K = 128 KC: every thread keeps its KC chunks of
+// 8 values from the maximum pass to the conversion, so a panel is read from
+// HBM once.  (The two-pass kernels above read it again for the conversion; at
+// K = 1024 the rows a CU's workgroups hold no longer stay in the XCD's L2
+// between the passes.)  Rows: 16 rows per 256-thread workgroup, thread t
+// handling row t % 16 and chunks t / 16 + 16 j; columns: 32 columns per
+// 512-thread workgroup, thread t handling column t % 32 and chunks t / 32 + 16 j.
+// The planes, scales and their layout are the two-pass kernels'.
+template <int KC>
+__global__ __launch_bounds__(256) void k_h4split_rows_reg(const float2* Ab, int lda, size_t sA, HSplit4 h) {
+    __shared__ float part[16][17];
+    __shared__ float invs[16];
+    const int e = blockIdx.y, tid = threadIdx.x;
+    const int rr = tid & 15, cg = tid >> 4;
+    const int row = blockIdx.x * 16 + rr;
+    const bool live = row < h.M;
+    const float2* a = Ab + e * sA + static_cast<size_t>(live ? row : 0) * lda;
+    float4 v[KC][4];
+    float mx = 0.0f;
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+        const float4* src = reinterpret_cast<const float4*>(a + (cg + 16 * j) * 8);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[j][q] = src[q];
+    }
+#pragma unroll
+    for (int j = 0; j < KC; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[j][q].x), fabsf(v[j][q].y)), fmaxf(fabsf(v[j][q].z), fabsf(v[j][q].w))));
+    part[rr][cg] = mx;
+    __syncthreads();
+    if (tid < 16) {
+        float m = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) m = fmaxf(m, part[tid][q]);
+        float inv;
+        const float sc = pow2_above(m, &inv);
+        invs[tid] = inv;
+        if (blockIdx.x * 16 + tid < h.M) h.sa(e)[blockIdx.x * 16 + tid] = sc;
+    }
+    __syncthreads();
+    if (!live) return;
+    const float inv = invs[rr];
+    uint16_t* P = h.pa(e);
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+        float2 w[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            w[2 * q] = make_float2(v[j][q].x, v[j][q].y);
+            w[2 * q + 1] = make_float2(v[j][q].z, v[j][q].w);
+        }
+        uint4 out[4];
+        h4_chunks(w, inv, out);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) *reinterpret_cast<uint4*>(P + h4_index((cg + 16 * j) * 8, p, h.M, row)) = out[p];
+    }
+}
+
+template <int KC>
+__global__ __launch_bounds__(512) void k_h4split_cols_reg(const float2* Bb, int ldb, size_t sB, HSplit4 h) {
+    __shared__ float part[32][17];
+    __shared__ float invs[32];
+    const int e = blockIdx.y, tid = threadIdx.x;
+    const int cc = tid & 31, cg = tid >> 5;
+    const int col = blockIdx.x * 32 + cc;
+    const bool live = col < h.N;
+    const float2* b = Bb + e * sB + (live ? col : 0);
+    float2 v[KC][8];
+#pragma unroll
+    for (int j = 0; j < KC; ++j)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[j][u] = b[static_cast<size_t>((cg + 16 * j) * 8 + u) * ldb];
+    float mx = 0.0f;
+#pragma unroll
+    for (int j = 0; j < KC; ++j)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fmaxf(fabsf(v[j][u].x), fabsf(v[j][u].y)));
+    part[cc][cg] = mx;
+    __syncthreads();
+    if (tid < 32) {
+        float m = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) m = fmaxf(m, part[tid][q]);
+        float inv;
+        const float sc = pow2_above(m, &inv);
+        invs[tid] = inv;
+        if (blockIdx.x * 32 + tid < h.N) h.sb(e)[blockIdx.x * 32 + tid] = sc;
+    }
+    __syncthreads();
+    if (!live) return;
+    const float inv = invs[cc];
+    uint16_t* P = h.pb(e);
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+        uint4 out[4];
+        h4_chunks(v[j], inv, out);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) *reinterpret_cast<uint4*>(P + h4_index((cg + 16 * j) * 8, p, h.N, col)) = out[p];
+    }
+}
+
+// Both splits of one update: the register-resident kernels for K = 128, 256,
+// 512 and 1024, the two-pass ones for other K (a multiple of 8).
+void h4_split_launch(hipStream_t s, const float2* A, int lda, size_t sA, const float2* B, int ldb, size_t sB,
+                     const HSplit4& hs, int nb) {
+    auto reg = [&](auto kc) {
+        constexpr int KC = decltype(kc)::value;
+        hipLaunchKernelGGL(k_h4split_rows_reg<KC>, dim3((hs.M + 15) / 16, nb), dim3(256), 0, s, A, lda, sA, hs);
+        hipLaunchKernelGGL(k_h4split_cols_reg<KC>, dim3((hs.N + 31) / 32, nb), dim3(512), 0, s, B, ldb, sB, hs);
+    };
+    switch (hs.K) {
+        case 128: reg(std::integral_constant<int, 1>{}); return;
+        case 256: reg(std::integral_constant<int, 2>{}); return;
+        case 512: reg(std::integral_constant<int, 4>{}); return;
+        case 1024: reg(std::integral_constant<int, 8>{}); return;
+        default:
+            hipLaunchKernelGGL(k_h4split_rows, dim3((hs.M + 15) / 16, nb), dim3(256), 0, s, A, lda, sA, hs);
+            hipLaunchKernelGGL(k_h4split_cols, dim3((hs.N + 63) / 64, nb), dim3(256), 0, s, B, ldb, sB, hs);
+    }
+}
+
 __device__ __forceinline__ v8h neg_h8(v8h x) {
     uint4 u = __builtin_bit_cast(uint4, x);
     u.x ^= 0x80008000u;
@@ -4162,8 +4286,7 @@ void gemm_bf_launch(hipStream_t s, int ns, float2* C, int ldc, size_t sC, const 
                     const float2* B, int ldb, size_t sB, int M, int N, int K, int nb, uint16_t* split, size_t ssplit) {
     if (ns == 4) {  // fp16 two-part split, four-product form
         const HSplit4 hs{split, ssplit, M, N, K};
-        hipLaunchKernelGGL(k_h4split_rows, dim3((M + 15) / 16, nb), dim3(256), 0, s, A, lda, sA, hs);
-        hipLaunchKernelGGL(k_h4split_cols, dim3((N + 63) / 64, nb), dim3(256), 0, s, B, ldb, sB, hs);
+        h4_split_launch(s, A, lda, sA, B, ldb, sB, hs, nb);
         hipLaunchKernelGGL(k_gemm_h4, dim3(((M + 127) / 128) * ((N + 127) / 128), nb), dim3(512), 0, s, C, ldc, sC, hs);
         return;
     }
@@ -6099,10 +6222,8 @@ int refeff_fms_debug_gemm_bench(int32_t kernel, int32_t M, int32_t N, int32_t K,
     // kernel 18: the fp16x2 4M product kernel alone, on planes split once before the timing
     const HSplit4 hs4{split, ss, M, N, K};
     if (kernel == 18) {
-        hipLaunchKernelGGL(k_h4split_rows, dim3((M + 15) / 16, batch), dim3(256), 0, nullptr,
-                           static_cast<const float2*>(da), ld, per, hs4);
-        hipLaunchKernelGGL(k_h4split_cols, dim3((N + 63) / 64, batch), dim3(256), 0, nullptr,
-                           static_cast<const float2*>(db), ld, per, hs4);
+        h4_split_launch(nullptr, static_cast<const float2*>(da), ld, per, static_cast<const float2*>(db), ld, per, hs4,
+                        batch);
     }
     auto one = [&]() {
         if (kernel == 18) {
