@@ -2264,8 +2264,9 @@ __global__ __launch_bounds__(256) void k_h4split_cols(const float2* Bb, int ldb,
     }
 }
 
-// Register-resident splits for K = 128 KC: every thread keeps its KC chunks of
-// 8 values from the maximum pass to the conversion, so a panel is read from
+// Register-resident splits for K <= 128 KC: every thread keeps its KC chunks of
+// 8 values (chunks past K / 8 skipped) from the maximum pass to the
+// conversion, so a panel is read from
 // HBM once.  (The two-pass kernels above read it again for the conversion; at
 // K = 1024 the rows a CU's workgroups hold no longer stay in the XCD's L2
 // between the passes.)  Rows: 16 rows per 256-thread workgroup, thread t
@@ -2281,13 +2282,14 @@ __global__ __launch_bounds__(256) void k_h4split_rows_reg(const float2* Ab, int 
     const int row = blockIdx.x * 16 + rr;
     const bool live = row < h.M;
     const float2* a = Ab + e * sA + static_cast<size_t>(live ? row : 0) * lda;
+    const int nch = h.K / 8;
     float4 v[KC][4];
     float mx = 0.0f;
 #pragma unroll
     for (int j = 0; j < KC; ++j) {
         const float4* src = reinterpret_cast<const float4*>(a + (cg + 16 * j) * 8);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[j][q] = src[q];
+        for (int q = 0; q < 4; ++q) v[j][q] = cg + 16 * j < nch ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
     for (int j = 0; j < KC; ++j)
@@ -2311,6 +2313,7 @@ __global__ __launch_bounds__(256) void k_h4split_rows_reg(const float2* Ab, int 
     uint16_t* P = h.pa(e);
 #pragma unroll
     for (int j = 0; j < KC; ++j) {
+        if (cg + 16 * j >= nch) break;
         float2 w[8];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -2333,11 +2336,13 @@ __global__ __launch_bounds__(512) void k_h4split_cols_reg(const float2* Bb, int 
     const int col = blockIdx.x * 32 + cc;
     const bool live = col < h.N;
     const float2* b = Bb + e * sB + (live ? col : 0);
+    const int nch = h.K / 8;
     float2 v[KC][8];
 #pragma unroll
     for (int j = 0; j < KC; ++j)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[j][u] = b[static_cast<size_t>((cg + 16 * j) * 8 + u) * ldb];
+        for (int u = 0; u < 8; ++u)
+            v[j][u] = cg + 16 * j < nch ? b[static_cast<size_t>((cg + 16 * j) * 8 + u) * ldb] : make_float2(0.f, 0.f);
     float mx = 0.0f;
 #pragma unroll
     for (int j = 0; j < KC; ++j)
@@ -2360,6 +2365,7 @@ __global__ __launch_bounds__(512) void k_h4split_cols_reg(const float2* Bb, int 
     uint16_t* P = h.pb(e);
 #pragma unroll
     for (int j = 0; j < KC; ++j) {
+        if (cg + 16 * j >= nch) break;
         uint4 out[4];
         h4_chunks(v[j], inv, out);
 #pragma unroll
@@ -2367,8 +2373,8 @@ __global__ __launch_bounds__(512) void k_h4split_cols_reg(const float2* Bb, int 
     }
 }
 
-// Both splits of one update: the register-resident kernels for K = 128, 256,
-// 512 and 1024, the two-pass ones for other K (a multiple of 8).
+// Both splits of one update (K a multiple of 8): the register-resident kernels
+// up to K = 1024, the two-pass ones beyond.
 void h4_split_launch(hipStream_t s, const float2* A, int lda, size_t sA, const float2* B, int ldb, size_t sB,
                      const HSplit4& hs, int nb) {
     auto reg = [&](auto kc) {
@@ -2376,11 +2382,15 @@ void h4_split_launch(hipStream_t s, const float2* A, int lda, size_t sA, const f
         hipLaunchKernelGGL(k_h4split_rows_reg<KC>, dim3((hs.M + 15) / 16, nb), dim3(256), 0, s, A, lda, sA, hs);
         hipLaunchKernelGGL(k_h4split_cols_reg<KC>, dim3((hs.N + 31) / 32, nb), dim3(512), 0, s, B, ldb, sB, hs);
     };
-    switch (hs.K) {
-        case 128: reg(std::integral_constant<int, 1>{}); return;
-        case 256: reg(std::integral_constant<int, 2>{}); return;
-        case 512: reg(std::integral_constant<int, 4>{}); return;
-        case 1024: reg(std::integral_constant<int, 8>{}); return;
+    switch ((hs.K + 127) / 128) {
+        case 1: reg(std::integral_constant<int, 1>{}); return;
+        case 2: reg(std::integral_constant<int, 2>{}); return;
+        case 3: reg(std::integral_constant<int, 3>{}); return;
+        case 4: reg(std::integral_constant<int, 4>{}); return;
+        case 5: reg(std::integral_constant<int, 5>{}); return;
+        case 6: reg(std::integral_constant<int, 6>{}); return;
+        case 7: reg(std::integral_constant<int, 7>{}); return;
+        case 8: reg(std::integral_constant<int, 8>{}); return;
         default:
             hipLaunchKernelGGL(k_h4split_rows, dim3((hs.M + 15) / 16, nb), dim3(256), 0, s, A, lda, sA, hs);
             hipLaunchKernelGGL(k_h4split_cols, dim3((hs.N + 63) / 64, nb), dim3(256), 0, s, B, ldb, sB, hs);
