@@ -4382,7 +4382,9 @@ void swap_trsm(refeff_fms_plan* p, const Group& g, typename Cx<T>::type* A, cons
                int c_hi, int do_swap, int trsm_from) {
     const int n = p->npad;
     if (c_hi <= c_lo) return;
-    constexpr int CW = (sizeof(T) == 4) ? 128 : 64;
+    // FP32: 256 columns per workgroup, whose 73 KB of LDS let a CU hold eight
+    // waves (six with 128 columns)
+    constexpr int CW = (sizeof(T) == 4) ? 256 : 64;
     const double cb = sizeof(typename Cx<T>::type);
     const double cols = c_hi - c_lo, tcols = std::max(0, c_hi - std::max(c_lo, trsm_from));
     launch(p, g.s, REFEFF_FMS_K_SWAP, 8.0 * kPanel * kPanel / 2 * tcols * g.nb, 2.0 * kPanel * cols * cb * g.nb, [&] {
