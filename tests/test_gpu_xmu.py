@@ -1,6 +1,6 @@
 """xmu.dat end to end: the GPU solve (mixed LU + FP64 refinement), the GPU
 MKGTR trace and ff2x's XANES assembly, against the FP64 oracle's solve, trace
-and ff2x restatement on the same inputs.  Bar: the north star's xmu gate,
+and ff2x restatement on the same inputs.  Bar: the xmu accuracy gate,
 max |mu - mu_ref| / max |mu_ref| <= 1e-6 (measured well below it).
 
 The XANES energies follow FEFF's mesh shape (XSPH phmesh2: a horizontal line
